@@ -1,0 +1,116 @@
+/*
+ * Extension C ABI of libspai_hip.so: the forms and entry points that the
+ * reference has no counterpart for in its C++ library, plus the PCG evaluator
+ * that the reference implements in Cython.
+ *
+ * All structs are plain data passed by value; pointers are host pointers unless
+ * the name says "dev".  Functions that return int64_t return a negative value
+ * on failure, and spai_last_error() then holds the message for the calling
+ * thread.
+ */
+#pragma once
+
+#include <stdint.h>
+
+#include "methods_c_api.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Method selector for the extension entry points.  The five values map one to
+ * one onto the reference's entry points (include/methods_c_api.h:48-56). */
+enum SpaiMethod {
+  SPAI_GLOBAL_CG = 0,
+  SPAI_GLOBAL_MR = 1,
+  SPAI_GLOBAL_LOMR = 2,
+  SPAI_INNER_OUTER_MR = 3,
+  SPAI_INNER_OUTER_LOMR = 4
+};
+
+/* Forms beside the reference semantics.  All-zero selects exactly the
+ * reference behaviour of spai_global_* / inner_outer_*. */
+struct SpaiExtParams {
+  /* 0: the pattern of M grows as in the reference.
+   * 1: M stays on S = pattern(A) | pattern(M0) | I; the search directions are
+   *    sampled on S and every step size is the exact minimiser of the method's
+   *    objective over S (projected line search).  Dropping must be off. */
+  int64_t fixed_pattern;
+  /* 0: FP64 throughout.  1: FP32 bulk: M, the directions, the residual and the
+   *    sparse products stream FP32 values; step sizes, trace inner products
+   *    and ||I - A M||_F are accumulated in FP64 (fixed_pattern must be 1). */
+  int64_t precision;
+  /* 0: the reference's global nnz budget.  k > 0 (dropping on): each column of
+   *    M and of the direction keeps its diagonal and its k largest |values|
+   *    off the diagonal (ties kept by the smaller row) instead. */
+  int64_t column_top_k;
+  int64_t reserved[5];
+};
+
+/* CSR matrix with int64 offsets (rows + 1) and column indices. */
+struct CsrData {
+  int64_t rows;
+  int64_t cols;
+  int64_t nnz;
+  const int64_t* row_offsets;
+  const int64_t* col_indices;
+  const double* values;
+};
+
+#ifndef __cplusplus
+typedef struct SpaiExtParams SpaiExtParams;
+typedef struct CsrData CsrData;
+#endif
+
+/* Any of the five methods in any form; the result is released by free_mem().
+ * seconds_out (nullable) receives the wall time of the device work, measured
+ * with HIP events from the first upload to the last download. */
+struct Result spai_run_ex(int64_t method, struct MatrixData A, struct MatrixData M0, struct MatrixData Pr,
+                          struct MethodParams params, struct SpaiExtParams ext, double* seconds_out);
+
+/* Preconditioned CG on the device; replaces methods_cython/backward_error.pyx:69-144
+ * (pcg).  history receives up to max_iterations relative residuals (entry 0 is
+ * 1.0, the last is the true residual), *count their number; x_out (nullable)
+ * receives the iterate.  m_fp32 != 0 streams M's values in FP32 (vectors,
+ * A and every reduction stay FP64).  Returns the reference's status code: 0 =
+ * iteration limit, 1 = converged, 2 = zero or non-finite recurrence scalar,
+ * 3 = non-finite residual; negative on error. */
+int64_t spai_pcg(struct CsrData A, struct CsrData M, const double* b, int64_t max_iterations, double tolerance,
+                 double* history, int64_t* count, double* x_out, int64_t m_fp32);
+
+/* y = A x on the device; replaces methods_cython/backward_error.pyx:56-66 (matvec). */
+int64_t spai_matvec(struct CsrData A, const double* x, double* y);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* spai_last_error(void);
+
+/* Name of the HIP device the library runs on, for reports. */
+int64_t spai_device_name(char* buffer, int64_t size);
+
+/* ------------------------------------------------------------------------
+ * Device-resident session: A, M0, Pr and the right-hand sides are uploaded once,
+ * then builds and solves run with every operand already in HBM.  Used by the
+ * benchmark and by callers that build several preconditioners for one A.
+ * ------------------------------------------------------------------------ */
+typedef struct SpaiSession SpaiSession;
+
+SpaiSession* spai_session_create(int64_t method, struct MatrixData A, struct MatrixData M0, struct MatrixData Pr,
+                                 struct MethodParams params, struct SpaiExtParams ext);
+void spai_session_destroy(SpaiSession* session);
+/* Builds M on the device; returns the number of iterations. */
+int64_t spai_session_build(SpaiSession* session);
+/* Uploads a right-hand side of length n into slot 0. */
+int64_t spai_session_set_rhs(SpaiSession* session, const double* b);
+/* PCG with the built M from x0 = 0; returns the status as spai_pcg. */
+int64_t spai_session_pcg(SpaiSession* session, int64_t max_iterations, double tolerance, int64_t* count);
+/* Copies the last PCG history (count entries) and iterate to the host (nullable). */
+int64_t spai_session_pcg_output(SpaiSession* session, double* history, double* x);
+/* Copies the built M and its history to a host Result (release with free_mem). */
+struct Result spai_session_result(SpaiSession* session);
+/* Per-kernel timing of the last PCG iteration loop: average microseconds of
+ * one SpMV with A (HIP events around a replay of the iteration graph). */
+double spai_session_pcg_iteration_us(SpaiSession* session);
+
+#ifdef __cplusplus
+}
+#endif
