@@ -1,0 +1,316 @@
+// C ABI of libspai_hip.so (include/methods_c_api.h, include/spai_ext.h).
+//
+// The five builders keep the reference's by-value structs and its error
+// contract (src/common/methods_c_api.cpp:59-126): nothing throws across the ABI;
+// a failure returns a zeroed Result whose error_message holds the reason.
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "engine.hpp"
+#include "fixed.hpp"
+#include "methods_c_api.h"
+#include "pcg.hpp"
+#include "spai_ext.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+Result ErrorResult(const std::string& msg) {
+  Result out{};
+  out.error_message = new char[msg.size() + 1];
+  std::memcpy(out.error_message, msg.c_str(), msg.size() + 1);
+  return out;
+}
+
+Result SuccessResult(const spai::EngineOutput& o) {
+  Result r{};
+  r.rows = o.M.rows;
+  r.cols = o.M.cols;
+  r.nnz = (int64_t)o.M.idx.size();
+  r.iterations = o.iterations;
+  r.converged = o.converged ? 1 : 0;
+  r.history_size = (int64_t)o.history.size();
+  try {
+    r.col_offsets = new int64_t[r.cols + 1];
+    r.row_indices = new int64_t[r.nnz > 0 ? r.nnz : 1];
+    r.values = new double[r.nnz > 0 ? r.nnz : 1];
+    r.history = new SingleIterationData[r.history_size > 0 ? r.history_size : 1];
+  } catch (...) {
+    delete[] r.col_offsets;
+    delete[] r.row_indices;
+    delete[] r.values;
+    delete[] r.history;
+    throw;
+  }
+  std::memcpy(r.col_offsets, o.M.ptr.data(), sizeof(int64_t) * (r.cols + 1));
+  if (r.nnz) {
+    std::memcpy(r.row_indices, o.M.idx.data(), sizeof(int64_t) * r.nnz);
+    std::memcpy(r.values, o.M.val.data(), sizeof(double) * r.nnz);
+  }
+  for (int64_t i = 0; i < r.history_size; ++i) r.history[i] = o.history[i];
+  return r;
+}
+
+Result RunMethod(int64_t method, const MatrixData& A, const MatrixData& M0, const MatrixData& Pr,
+                 const MethodParams& p, const SpaiExtParams* ext, double* seconds) {
+  try {
+    const spai::EngineParams ep = spai::ToEngineParams(p, ext);
+    const auto out = spai::RunEngine(method, A, p.use_initial_m ? &M0 : nullptr,
+                                     p.use_preconditioner ? &Pr : nullptr, ep);
+    if (seconds) *seconds = out.device_seconds;
+    return SuccessResult(out);
+  } catch (const std::exception& ex) {
+    return ErrorResult(ex.what());
+  } catch (...) {
+    return ErrorResult("Unknown failure.");
+  }
+}
+
+int64_t Fail(const std::string& msg) {
+  g_last_error = msg;
+  return -1;
+}
+
+void ValidateCsr(const CsrData& m) {
+  if (m.rows <= 0 || m.cols != m.rows) throw std::invalid_argument("A must be a nonempty square matrix");
+  for (int64_t i = 0; i < m.rows; ++i)
+    if (m.row_offsets[i] > m.row_offsets[i + 1]) throw std::invalid_argument("row offsets must be non-decreasing");
+  const int64_t nnz = m.row_offsets[m.rows] - m.row_offsets[0];
+  for (int64_t k = 0; k < nnz; ++k) {
+    const int64_t c = m.col_indices[m.row_offsets[0] + k];
+    if (c < 0 || c >= m.cols) throw std::invalid_argument("column index out of range");
+  }
+}
+
+}  // namespace
+
+struct SpaiSession {
+  std::unique_ptr<spai::FixedProblem> prob;
+  std::unique_ptr<spai::DCsr> a_csr, m_csr;
+  std::unique_ptr<spai::PcgSolver> pcg;
+  spai::DBuf<double> rhs;
+  bool fp32 = false;
+  bool m_stale = true;
+  int64_t last_count = 0;
+};
+
+extern "C" {
+
+Result spai_global_cg(MatrixData A, MatrixData M0, MatrixData Pr, MethodParams params) {
+  return RunMethod(SPAI_GLOBAL_CG, A, M0, Pr, params, nullptr, nullptr);
+}
+
+Result spai_global_mr(MatrixData A, MatrixData M0, MatrixData Pr, MethodParams params) {
+  return RunMethod(SPAI_GLOBAL_MR, A, M0, Pr, params, nullptr, nullptr);
+}
+
+Result spai_global_lomr(MatrixData A, MatrixData M0, MatrixData Pr, MethodParams params) {
+  return RunMethod(SPAI_GLOBAL_LOMR, A, M0, Pr, params, nullptr, nullptr);
+}
+
+Result inner_outer_mr(MatrixData A, MatrixData M0, MatrixData Pr, MethodParams params) {
+  return RunMethod(SPAI_INNER_OUTER_MR, A, M0, Pr, params, nullptr, nullptr);
+}
+
+Result inner_outer_lomr(MatrixData A, MatrixData M0, MatrixData Pr, MethodParams params) {
+  return RunMethod(SPAI_INNER_OUTER_LOMR, A, M0, Pr, params, nullptr, nullptr);
+}
+
+void free_mem(Result* result) {
+  if (result == nullptr) return;
+  delete[] result->col_offsets;
+  delete[] result->row_indices;
+  delete[] result->values;
+  delete[] result->history;
+  delete[] result->error_message;
+  result->col_offsets = nullptr;
+  result->row_indices = nullptr;
+  result->values = nullptr;
+  result->history = nullptr;
+  result->error_message = nullptr;
+}
+
+Result spai_run_ex(int64_t method, MatrixData A, MatrixData M0, MatrixData Pr, MethodParams params,
+                   SpaiExtParams ext, double* seconds_out) {
+  return RunMethod(method, A, M0, Pr, params, &ext, seconds_out);
+}
+
+int64_t spai_pcg(CsrData A, CsrData M, const double* b, int64_t max_iterations, double tolerance, double* history,
+                 int64_t* count, double* x_out, int64_t m_fp32) {
+  try {
+    if (A.rows <= 0 || max_iterations <= 0) throw std::invalid_argument("n, max_iterations and threads must be positive");
+    ValidateCsr(A);
+    ValidateCsr(M);
+    if (M.rows != A.rows) throw std::invalid_argument("A and M must have the same shape");
+    spai::Ctx ctx;
+    const int64_t n = A.rows;
+    spai::DCsr a = spai::CsrFromHost(ctx, n, A.nnz, A.row_offsets, A.col_indices, A.values);
+    spai::DCsr m = spai::CsrFromHost(ctx, n, M.nnz, M.row_offsets, M.col_indices, M.values);
+    if (m_fp32) spai::MakeFp32Copy(ctx, m);
+    auto bd = ctx.Alloc<double>(n);
+    SPAI_HIP_CHECK(hipMemcpyAsync(bd.get(), b, sizeof(double) * n, hipMemcpyHostToDevice, ctx.stream));
+    spai::PcgSolver solver(ctx, a, m, m_fp32 != 0);
+    int64_t cnt = 0;
+    const int status = solver.Solve(bd.get(), max_iterations, tolerance, &cnt);
+    solver.History(history, cnt);
+    if (x_out) {
+      SPAI_HIP_CHECK(hipMemcpyAsync(x_out, solver.X(), sizeof(double) * n, hipMemcpyDeviceToHost, ctx.stream));
+      ctx.Sync();
+    }
+    *count = cnt;
+    return status;
+  } catch (const std::exception& ex) {
+    return Fail(ex.what());
+  } catch (...) {
+    return Fail("Unknown failure.");
+  }
+}
+
+int64_t spai_matvec(CsrData A, const double* x, double* y) {
+  try {
+    ValidateCsr(A);
+    spai::Ctx ctx;
+    spai::DCsr a = spai::CsrFromHost(ctx, A.rows, A.nnz, A.row_offsets, A.col_indices, A.values);
+    spai::DMat dummy;
+    (void)dummy;
+    auto xd = ctx.Alloc<double>(A.rows);
+    auto yd = ctx.Alloc<double>(A.rows);
+    SPAI_HIP_CHECK(hipMemcpyAsync(xd.get(), x, sizeof(double) * A.rows, hipMemcpyHostToDevice, ctx.stream));
+    spai::SpMV(ctx, a, xd.get(), yd.get());
+    SPAI_HIP_CHECK(hipMemcpyAsync(y, yd.get(), sizeof(double) * A.rows, hipMemcpyDeviceToHost, ctx.stream));
+    ctx.Sync();
+    return 0;
+  } catch (const std::exception& ex) {
+    return Fail(ex.what());
+  } catch (...) {
+    return Fail("Unknown failure.");
+  }
+}
+
+const char* spai_last_error(void) { return g_last_error.c_str(); }
+
+int64_t spai_device_name(char* buffer, int64_t size) {
+  try {
+    int dev = 0;
+    SPAI_HIP_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    SPAI_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+    const std::string s = std::string(prop.name) + " (" + prop.gcnArchName + ", " +
+                          std::to_string(prop.multiProcessorCount) + " CUs)";
+    if (buffer && size > 0) {
+      std::strncpy(buffer, s.c_str(), (size_t)size - 1);
+      buffer[size - 1] = '\0';
+    }
+    return (int64_t)s.size();
+  } catch (const std::exception& ex) {
+    return Fail(ex.what());
+  }
+}
+
+SpaiSession* spai_session_create(int64_t method, MatrixData A, MatrixData M0, MatrixData Pr, MethodParams params,
+                                 SpaiExtParams ext) {
+  try {
+    const spai::EngineParams ep = spai::ToEngineParams(params, &ext);
+    if (!ep.fixed_pattern) throw std::invalid_argument("device-resident sessions need fixed_pattern = 1");
+    spai::ValidateMatrixData(A);
+    if (params.use_initial_m) spai::ValidateMatrixData(M0);
+    if (params.use_preconditioner) spai::ValidateMatrixData(Pr);
+    auto s = std::make_unique<SpaiSession>();
+    s->prob = spai::MakeFixedProblem(method, A, params.use_initial_m ? &M0 : nullptr,
+                                     params.use_preconditioner ? &Pr : nullptr, ep);
+    s->fp32 = ep.fp32;
+    spai::Ctx& c = s->prob->context();
+    s->a_csr = std::make_unique<spai::DCsr>(spai::CsrFromCsc(c, s->prob->MatrixA()));
+    s->rhs = c.Alloc<double>(A.rows);
+    c.Sync();
+    return s.release();
+  } catch (const std::exception& ex) {
+    g_last_error = ex.what();
+    return nullptr;
+  }
+}
+
+void spai_session_destroy(SpaiSession* s) { delete s; }
+
+int64_t spai_session_build(SpaiSession* s) {
+  try {
+    if (!s) throw std::invalid_argument("null session");
+    const int64_t it = s->prob->Build();
+    spai::Ctx& c = s->prob->context();
+    // PCG operand: CSR of M, rebuilt from the new values
+    s->pcg.reset();
+    s->m_csr = std::make_unique<spai::DCsr>(spai::CsrFromCsc(c, s->prob->MatrixM()));
+    if (s->fp32) spai::MakeFp32Copy(c, *s->m_csr);
+    s->pcg = std::make_unique<spai::PcgSolver>(c, *s->a_csr, *s->m_csr, s->fp32);
+    return it;
+  } catch (const std::exception& ex) {
+    return Fail(ex.what());
+  }
+}
+
+int64_t spai_session_set_rhs(SpaiSession* s, const double* b) {
+  try {
+    spai::Ctx& c = s->prob->context();
+    SPAI_HIP_CHECK(hipMemcpyAsync(s->rhs.get(), b, sizeof(double) * s->a_csr->n, hipMemcpyHostToDevice, c.stream));
+    c.Sync();
+    return 0;
+  } catch (const std::exception& ex) {
+    return Fail(ex.what());
+  }
+}
+
+int64_t spai_session_pcg(SpaiSession* s, int64_t max_iterations, double tolerance, int64_t* count) {
+  try {
+    if (!s || !s->pcg) throw std::invalid_argument("session has no built preconditioner");
+    int64_t cnt = 0;
+    const int st = s->pcg->Solve(s->rhs.get(), max_iterations, tolerance, &cnt);
+    s->last_count = cnt;
+    if (count) *count = cnt;
+    return st;
+  } catch (const std::exception& ex) {
+    return Fail(ex.what());
+  }
+}
+
+int64_t spai_session_pcg_output(SpaiSession* s, double* history, double* x) {
+  try {
+    spai::Ctx& c = s->prob->context();
+    if (history) s->pcg->History(history, s->last_count);
+    if (x) {
+      SPAI_HIP_CHECK(
+          hipMemcpyAsync(x, s->pcg->X(), sizeof(double) * s->a_csr->n, hipMemcpyDeviceToHost, c.stream));
+      c.Sync();
+    }
+    return 0;
+  } catch (const std::exception& ex) {
+    return Fail(ex.what());
+  }
+}
+
+Result spai_session_result(SpaiSession* s) {
+  try {
+    spai::EngineOutput out;
+    s->prob->Output(out);
+    return SuccessResult(out);
+  } catch (const std::exception& ex) {
+    return ErrorResult(ex.what());
+  }
+}
+
+double spai_session_pcg_iteration_us(SpaiSession* s) {
+  try {
+    return s->pcg->TimeSpmvA(20);
+  } catch (const std::exception& ex) {
+    g_last_error = ex.what();
+    return -1.0;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,50 @@
+// Host orchestration of the SPAI builders on the device.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "methods_c_api.h"
+#include "spai_ext.h"
+
+namespace spai {
+
+struct EngineParams {
+  int64_t max_iterations = 100;
+  double tolerance = 1e-9;
+  double max_density = 0.03;
+  bool enable_dropping = true;
+  int64_t num_threads = 1;  // number of contiguous column batches
+  int64_t inner_iterations = 2;
+  bool fixed_pattern = false;
+  bool fp32 = false;
+  int64_t top_k = 0;
+};
+
+struct HostMatrix {
+  int64_t rows = 0, cols = 0;
+  std::vector<int64_t> ptr, idx;
+  std::vector<double> val;
+};
+
+struct EngineOutput {
+  HostMatrix M;
+  int64_t iterations = 0;
+  bool converged = false;
+  std::vector<SingleIterationData> history;
+  double device_seconds = 0.0;
+};
+
+// Validates like src/common/methods_c_api.cpp:16-29 (CSC input) and
+// src/common/methods_common.cpp:12-36 (problem); throws std::invalid_argument
+// with the reference's messages.
+void ValidateMatrixData(const MatrixData& d);
+
+// Runs one of the five methods (SpaiMethod) on the current HIP device.
+EngineOutput RunEngine(int64_t method, const MatrixData& A, const MatrixData* M0, const MatrixData* Pr,
+                       const EngineParams& params);
+
+EngineParams ToEngineParams(const MethodParams& p, const SpaiExtParams* ext);
+
+}  // namespace spai

@@ -1,0 +1,709 @@
+// Fixed-pattern Global SPAI builders (CG, MR, LOMR) on the device, FP64 or
+// FP32-bulk.
+//
+// M, Z and P live on S = pattern(A) | pattern(M0) | I; the residual R and the
+// products A Z, A P live on S2 = pattern(A S) | S | I.  Both patterns are fixed
+// for the whole build, so every kernel is a flat pass over S2 (or S) and the
+// iteration runs without host synchronisation: step sizes, the stop flag and
+// the residual history stay in device memory.
+//
+// Products are inner products over k ascending -- (A Y)(i, j) =
+// sum_k A^T(k, i) Y(k, j) merged from column i of A^T and column j of Y -- which
+// is the same summation order as the reference's column-wise Gustavson
+// product (state_manager.cpp:60).  Reductions (trace inner products, ||R||_F)
+// accumulate in FP64 in both forms.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <vector>
+
+#include "dev_common.hpp"
+#include "fixed.hpp"
+#include "ls2x2.hpp"
+#include "ops.hpp"
+
+namespace spai {
+
+namespace {
+
+constexpr int kBlk = 256;
+constexpr int kMaxGrid = 1024;
+
+// Scalar slots in device memory.
+enum Slot : int {
+  kDelta = 0,
+  kGamma,
+  kBeta,
+  kAlpha,
+  kRz,
+  kPrevRz,
+  kNumSlots
+};
+
+struct Flags {
+  int stop;        // 0 running, 1 breakdown, 2 converged
+  int iterations;  // history rows appended - 1
+};
+
+template <class V>
+__device__ __forceinline__ V InnerProduct(const int* __restrict__ ai, const V* __restrict__ av, int ab, int ae,
+                                          const int* __restrict__ si, const V* __restrict__ y, int sb, int se) {
+  V s = V(0);
+  while (ab < ae && sb < se) {
+    const int ka = ai[ab], ks = si[sb];
+    if (ka == ks) {
+      s += av[ab] * y[sb];
+      ++ab;
+      ++sb;
+    } else if (ka < ks) {
+      ++ab;
+    } else {
+      ++sb;
+    }
+  }
+  return s;
+}
+
+template <class V>
+__device__ __forceinline__ void InnerProduct2(const int* __restrict__ ai, const V* __restrict__ av, int ab, int ae,
+                                              const int* __restrict__ si, const V* __restrict__ y0,
+                                              const V* __restrict__ y1, int sb, int se, V& s0, V& s1) {
+  s0 = V(0);
+  s1 = V(0);
+  while (ab < ae && sb < se) {
+    const int ka = ai[ab], ks = si[sb];
+    if (ka == ks) {
+      const V a = av[ab];
+      s0 += a * y0[sb];
+      s1 += a * y1[sb];
+      ++ab;
+      ++sb;
+    } else if (ka < ks) {
+      ++ab;
+    } else {
+      ++sb;
+    }
+  }
+}
+
+template <int K>
+__device__ __forceinline__ void StorePartials(double (&acc)[K], double* __restrict__ part) {
+  __shared__ double red[16];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double t = BlockSum<double>(acc[k], red);
+    if (threadIdx.x == 0) part[k * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// Deterministic final sum of K quantities over G block partials (1 block).
+template <int K>
+__device__ __forceinline__ void SumPartialsBlock(const double* __restrict__ part, int G, double (&out)[K]) {
+  __shared__ double red[16];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < G; b += blockDim.x) s += part[k * G + b];
+    out[k] = BlockSum<double>(s, red);
+  }
+}
+
+// --------------------------------------------------------------- residual
+// R = I - A M on S2 (value -(sum) + [i == j]); Z = Pr R sampled on S.
+// mode 1 (CG setup) also sets P = Z and accumulates <R, Z>.
+template <class V>
+__global__ __launch_bounds__(kBlk) void ResidualKernel(
+    int64_t nnz2, const int* __restrict__ s2row, const int* __restrict__ s2col, const int* __restrict__ s2_to_s,
+    const int* __restrict__ atp, const int* __restrict__ ati, const V* __restrict__ atv, const int* __restrict__ sp,
+    const int* __restrict__ si, const V* __restrict__ M, const V* __restrict__ pr, V* __restrict__ R,
+    V* __restrict__ Z, V* __restrict__ P, int mode, double* __restrict__ part, const Flags* __restrict__ flags) {
+  if (flags->stop) return;
+  double acc[2] = {0.0, 0.0};
+  for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < nnz2; q += int64_t(gridDim.x) * blockDim.x) {
+    const int i = s2row[q], j = s2col[q];
+    V r = -InnerProduct<V>(ati, atv, atp[i], atp[i + 1], si, M, sp[j], sp[j + 1]);
+    if (i == j) r += V(1);
+    R[q] = r;
+    acc[0] += double(r) * double(r);
+    const int p = s2_to_s[q];
+    if (p >= 0) {
+      const V z = pr[i] * r;
+      Z[p] = z;
+      if (mode == 1) {
+        P[p] = z;
+        acc[1] += double(r) * double(z);
+      }
+    }
+  }
+  StorePartials<2>(acc, part);
+}
+
+// --------------------------------------------------------------- LOMR
+// AZ, AP on S2 and the five Pr-weighted trace products of
+// locally_minimal_residual.cpp:66-87 (numerators projected: <Pr R, A Z>, <Pr R, A P>).
+template <class V>
+__global__ __launch_bounds__(kBlk) void LomrProject(
+    int64_t nnz2, const int* __restrict__ s2row, const int* __restrict__ s2col, const int* __restrict__ atp,
+    const int* __restrict__ ati, const V* __restrict__ atv, const int* __restrict__ sp, const int* __restrict__ si,
+    const V* __restrict__ Z, const V* __restrict__ P, const V* __restrict__ R, const V* __restrict__ pr,
+    V* __restrict__ AZ, V* __restrict__ AP, double* __restrict__ part, const Flags* __restrict__ flags) {
+  if (flags->stop) return;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < nnz2; q += int64_t(gridDim.x) * blockDim.x) {
+    const int i = s2row[q], j = s2col[q];
+    V az, ap;
+    InnerProduct2<V>(ati, atv, atp[i], atp[i + 1], si, Z, P, sp[j], sp[j + 1], az, ap);
+    AZ[q] = az;
+    AP[q] = ap;
+    const double daz = az, dap = ap, dpr = pr[i], dprr = double(pr[i] * R[q]);
+    acc[0] += daz * (dpr * daz);   // <AZ, Pr AZ>
+    acc[1] += dap * (dpr * dap);   // <AP, Pr AP>
+    acc[2] += daz * (dpr * dap);   // <AZ, Pr AP>
+    acc[3] += dprr * daz;          // <Pr R, AZ>
+    acc[4] += dprr * dap;          // <Pr R, AP>
+  }
+  StorePartials<5>(acc, part);
+}
+
+// GetDeltaGamma (locally_minimal_residual.cpp:36-50) on the device.
+__global__ __launch_bounds__(kBlk) void LomrSteps(const double* __restrict__ part, int G, int first,
+                                                  double* __restrict__ sc, Flags* __restrict__ flags) {
+  if (flags->stop) return;
+  double s[5];
+  SumPartialsBlock<5>(part, G, s);
+  if (threadIdx.x != 0) return;
+  const double az_praz = s[0], ap_prap = first ? 0.0 : s[1], az_prap = first ? 0.0 : s[2], z_az = s[3],
+               z_ap = first ? 0.0 : s[4];
+  double delta, gamma;
+  if (first) {
+    if (!(isfinite(az_praz) && fabs(az_praz) > 0.0)) {
+      flags->stop = 1;
+      return;
+    }
+    delta = z_az / az_praz;
+    gamma = 1.0;
+  } else {
+    SolveLeastSquares2x2(az_praz, az_prap, az_prap, ap_prap, z_az, z_ap, delta, gamma);
+  }
+  if (!isfinite(delta) || !isfinite(gamma)) {
+    flags->stop = 1;
+    return;
+  }
+  const double ratio = delta != 0.0 ? gamma / delta : 0.0;
+  sc[kDelta] = delta;
+  sc[kGamma] = gamma;
+  sc[kBeta] = isfinite(ratio) ? ratio : 0.0;
+}
+
+// R -= delta AZ + gamma AP; M += delta Z + gamma P; P = beta P + Z; Z = Pr R.
+template <class V>
+__global__ __launch_bounds__(kBlk) void LomrUpdate(int64_t nnz2, const int* __restrict__ s2row,
+                                                   const int* __restrict__ s2_to_s, const V* __restrict__ AZ,
+                                                   const V* __restrict__ AP, const V* __restrict__ pr,
+                                                   V* __restrict__ R, V* __restrict__ M, V* __restrict__ Z,
+                                                   V* __restrict__ P, const double* __restrict__ sc,
+                                                   double* __restrict__ part, const Flags* __restrict__ flags) {
+  if (flags->stop) return;
+  const V d = V(sc[kDelta]), g = V(sc[kGamma]), b = V(sc[kBeta]);
+  const V nd = V(-sc[kDelta]), ng = V(-sc[kGamma]);
+  double acc[1] = {0.0};
+  for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < nnz2; q += int64_t(gridDim.x) * blockDim.x) {
+    V r = R[q];
+    r = r + nd * AZ[q];
+    r = r + ng * AP[q];
+    R[q] = r;
+    acc[0] += double(r) * double(r);
+    const int p = s2_to_s[q];
+    if (p >= 0) {
+      const V z = Z[p], pp = P[p];
+      V m = M[p];
+      m = m + d * z;
+      m = m + g * pp;
+      M[p] = m;
+      P[p] = pp * b + z;
+      Z[p] = pr[s2row[q]] * r;
+    }
+  }
+  StorePartials<1>(acc, part);
+}
+
+// AppendIteration on the device: history[it] = ||R||_F, converged if < tol.
+__global__ __launch_bounds__(kBlk) void AppendResidual(const double* __restrict__ part, int G, double tol,
+                                                       double* __restrict__ hist, Flags* __restrict__ flags) {
+  if (flags->stop) return;
+  double s[1];
+  SumPartialsBlock<1>(part, G, s);
+  if (threadIdx.x != 0) return;
+  const double rn = sqrt(s[0]);
+  const int it = flags->iterations + 1;
+  hist[it] = rn;
+  flags->iterations = it;
+  if (rn < tol) flags->stop = 2;
+}
+
+__global__ __launch_bounds__(kBlk) void AppendInitial(const double* __restrict__ part, int G, double tol,
+                                                      double* __restrict__ hist, double* __restrict__ sc,
+                                                      Flags* __restrict__ flags) {
+  double s[2];
+  SumPartialsBlock<2>(part, G, s);
+  if (threadIdx.x != 0) return;
+  const double rn = sqrt(s[0]);
+  hist[0] = rn;
+  flags->iterations = 0;
+  sc[kRz] = s[1];
+  flags->stop = rn < tol ? 2 : 0;
+}
+
+// --------------------------------------------------------------- MR
+// num = <Pr R, Pr A Z>, den = ||Pr A Z||^2 (minimal_residual.cpp:44-54, projected numerator).
+template <class V>
+__global__ __launch_bounds__(kBlk) void MrProject(int64_t nnz2, const int* __restrict__ s2row,
+                                                  const int* __restrict__ s2col, const int* __restrict__ atp,
+                                                  const int* __restrict__ ati, const V* __restrict__ atv,
+                                                  const int* __restrict__ sp, const int* __restrict__ si,
+                                                  const V* __restrict__ Z, const V* __restrict__ R,
+                                                  const V* __restrict__ pr, double* __restrict__ part,
+                                                  const Flags* __restrict__ flags) {
+  if (flags->stop) return;
+  double acc[2] = {0.0, 0.0};
+  for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < nnz2; q += int64_t(gridDim.x) * blockDim.x) {
+    const int i = s2row[q], j = s2col[q];
+    const V az = InnerProduct<V>(ati, atv, atp[i], atp[i + 1], si, Z, sp[j], sp[j + 1]);
+    const double tmp = double(pr[i] * az);
+    acc[0] += double(pr[i] * R[q]) * tmp;
+    acc[1] += tmp * tmp;
+  }
+  StorePartials<2>(acc, part);
+}
+
+__global__ __launch_bounds__(kBlk) void MrStep(const double* __restrict__ part, int G, double* __restrict__ sc,
+                                               Flags* __restrict__ flags) {
+  if (flags->stop) return;
+  double s[2];
+  SumPartialsBlock<2>(part, G, s);
+  if (threadIdx.x != 0) return;
+  const double num = s[0], den = s[1];
+  if (!(isfinite(den) && fabs(den) > 0.0)) {
+    flags->stop = 1;
+    return;
+  }
+  const double alpha = num / den;
+  if (!isfinite(alpha)) {
+    flags->stop = 1;
+    return;
+  }
+  sc[kAlpha] = alpha;
+}
+
+// M += alpha * D over S.
+template <class V>
+__global__ __launch_bounds__(kBlk) void AxpyS(int64_t nnz, V* __restrict__ M, const V* __restrict__ D,
+                                              const double* __restrict__ sc, const Flags* __restrict__ flags) {
+  if (flags->stop) return;
+  const V a = V(sc[kAlpha]);
+  for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < nnz; p += int64_t(gridDim.x) * blockDim.x)
+    M[p] = M[p] + a * D[p];
+}
+
+// --------------------------------------------------------------- CG
+// AP on S2 and <P, AP> (conjugate_gradient.cpp:36-39).
+template <class V>
+__global__ __launch_bounds__(kBlk) void CgProject(int64_t nnz2, const int* __restrict__ s2row,
+                                                  const int* __restrict__ s2col, const int* __restrict__ s2_to_s,
+                                                  const int* __restrict__ atp, const int* __restrict__ ati,
+                                                  const V* __restrict__ atv, const int* __restrict__ sp,
+                                                  const int* __restrict__ si, const V* __restrict__ P,
+                                                  V* __restrict__ AP, double* __restrict__ part,
+                                                  const Flags* __restrict__ flags) {
+  if (flags->stop) return;
+  double acc[1] = {0.0};
+  for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < nnz2; q += int64_t(gridDim.x) * blockDim.x) {
+    const int i = s2row[q], j = s2col[q];
+    const V ap = InnerProduct<V>(ati, atv, atp[i], atp[i + 1], si, P, sp[j], sp[j + 1]);
+    AP[q] = ap;
+    const int p = s2_to_s[q];
+    if (p >= 0) acc[0] += double(P[p]) * double(ap);
+  }
+  StorePartials<1>(acc, part);
+}
+
+__global__ __launch_bounds__(kBlk) void CgAlpha(const double* __restrict__ part, int G, double* __restrict__ sc,
+                                                Flags* __restrict__ flags) {
+  if (flags->stop) return;
+  double s[1];
+  SumPartialsBlock<1>(part, G, s);
+  if (threadIdx.x != 0) return;
+  const double r_z = sc[kRz], den = s[0];
+  if (!(isfinite(r_z) && fabs(r_z) > 0.0) || !(isfinite(den) && fabs(den) > 0.0)) {
+    flags->stop = 1;
+    return;
+  }
+  const double alpha = r_z / den;
+  if (!isfinite(alpha)) {
+    flags->stop = 1;
+    return;
+  }
+  sc[kAlpha] = alpha;
+  sc[kPrevRz] = r_z;
+}
+
+// R -= alpha AP; M += alpha P; Z = Pr R on S; partials ||R||^2 and <R, Z>.
+template <class V>
+__global__ __launch_bounds__(kBlk) void CgUpdate(int64_t nnz2, const int* __restrict__ s2row,
+                                                 const int* __restrict__ s2_to_s, const V* __restrict__ AP,
+                                                 const V* __restrict__ pr, V* __restrict__ R, V* __restrict__ M,
+                                                 const V* __restrict__ P, V* __restrict__ Z,
+                                                 const double* __restrict__ sc, double* __restrict__ part,
+                                                 const Flags* __restrict__ flags) {
+  if (flags->stop) return;
+  const V a = V(sc[kAlpha]), na = V(-sc[kAlpha]);
+  double acc[2] = {0.0, 0.0};
+  for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < nnz2; q += int64_t(gridDim.x) * blockDim.x) {
+    const V r = R[q] + na * AP[q];
+    R[q] = r;
+    acc[0] += double(r) * double(r);
+    const int p = s2_to_s[q];
+    if (p >= 0) {
+      M[p] = M[p] + a * P[p];
+      const V z = pr[s2row[q]] * r;
+      Z[p] = z;
+      acc[1] += double(r) * double(z);
+    }
+  }
+  StorePartials<2>(acc, part);
+}
+
+// History row, then beta = <R, Z>_new / <R, Z>_old (conjugate_gradient.cpp:57-71).
+__global__ __launch_bounds__(kBlk) void CgAppendBeta(const double* __restrict__ part, int G, double tol,
+                                                     double* __restrict__ hist, double* __restrict__ sc,
+                                                     Flags* __restrict__ flags) {
+  if (flags->stop) return;
+  double s[2];
+  SumPartialsBlock<2>(part, G, s);
+  if (threadIdx.x != 0) return;
+  const double rn = sqrt(s[0]);
+  const int it = flags->iterations + 1;
+  hist[it] = rn;
+  flags->iterations = it;
+  if (rn < tol) {
+    flags->stop = 2;
+    return;
+  }
+  const double r_z = s[1];
+  sc[kRz] = r_z;
+  const double beta = r_z / sc[kPrevRz];
+  if (!isfinite(beta)) {
+    flags->stop = 1;
+    return;
+  }
+  sc[kBeta] = beta;
+}
+
+// P = P * beta + Z over S (LinearCombination(Z, 1, P, beta)).
+template <class V>
+__global__ __launch_bounds__(kBlk) void CgDirection(int64_t nnz, V* __restrict__ P, const V* __restrict__ Z,
+                                                    const double* __restrict__ sc, const Flags* __restrict__ flags) {
+  if (flags->stop) return;
+  const V b = V(sc[kBeta]);
+  for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < nnz; p += int64_t(gridDim.x) * blockDim.x)
+    P[p] = P[p] * b + Z[p];
+}
+
+// --------------------------------------------------------------- setup
+__global__ void ColumnOfEntry(int ncols, const int* __restrict__ ptr, int* __restrict__ col) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncols) return;
+  for (int p = ptr[j]; p < ptr[j + 1]; ++p) col[p] = j;
+}
+
+// s_to_s2 / s2_to_s maps (S is a subset of S2 column by column).
+__global__ void MapSubPattern(int ncols, const int* __restrict__ sp, const int* __restrict__ si,
+                              const int* __restrict__ s2p, const int* __restrict__ s2i, int* __restrict__ s2_to_s,
+                              int* __restrict__ error_flag) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncols) return;
+  for (int p = sp[j]; p < sp[j + 1]; ++p) {
+    const int q = FindSorted(s2i, s2p[j], s2p[j + 1], si[p]);
+    if (q < 0) {
+      atomicOr(error_flag, 1);
+      continue;
+    }
+    s2_to_s[q] = p;
+  }
+}
+
+// Values of X sampled on the pattern S (0 where X has no entry).
+template <class V>
+__global__ void SampleOnPattern(int ncols, const int* __restrict__ sp, const int* __restrict__ si,
+                                const int* __restrict__ xp, const int* __restrict__ xi, const double* __restrict__ xv,
+                                V* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncols) return;
+  for (int p = sp[j]; p < sp[j + 1]; ++p) {
+    const int q = FindSorted(xi, xp[j], xp[j + 1], si[p]);
+    out[p] = q >= 0 ? V(xv[q]) : V(0);
+  }
+}
+
+template <class V>
+__global__ void ConvertValues(int64_t n, const double* __restrict__ in, V* __restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = V(in[i]);
+}
+
+template <class V>
+__global__ void ToDouble(int64_t n, const V* __restrict__ in, double* __restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = double(in[i]);
+}
+
+template <class V>
+__global__ void FillZero(int64_t n, V* __restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = V(0);
+}
+
+unsigned GridFor(int64_t n) {
+  int64_t g = (n + kBlk - 1) / kBlk;
+  if (g < 1) g = 1;
+  if (g > kMaxGrid) g = kMaxGrid;
+  return (unsigned)g;
+}
+
+unsigned Blocks1(int64_t n) {
+  int64_t g = (n + kBlk - 1) / kBlk;
+  return (unsigned)std::max<int64_t>(g, 1);
+}
+
+template <class V>
+class FixedImpl final : public FixedProblem {
+ public:
+  FixedImpl(int64_t method, const MatrixData& Ah, const MatrixData* M0h, const MatrixData* Prh,
+            const EngineParams& p)
+      : method_(method), prm_(p) {
+    if (method != SPAI_GLOBAL_CG && method != SPAI_GLOBAL_MR && method != SPAI_GLOBAL_LOMR)
+      throw std::invalid_argument("fixed_pattern supports the global CG, MR and LOMR methods");
+    Ctx& c = ctx_;
+    if (Ah.rows != Ah.cols || Ah.rows == 0) throw std::invalid_argument("A must be a non-empty square matrix");
+    n_ = Ah.cols;
+    if (M0h && (M0h->rows != n_ || M0h->cols != n_))
+      throw std::invalid_argument("Initial preconditioner M0 must have the same shape as A");
+    if (Prh && (Prh->rows != n_ || Prh->cols != n_)) throw std::invalid_argument("Pr must have the same shape as A");
+    if (p.max_iterations <= 0) throw std::invalid_argument("max_iterations must be positive");
+    if (!std::isfinite(p.tolerance) || p.tolerance < 0.0)
+      throw std::invalid_argument("tolerance must be finite and non-negative");
+    if (!std::isfinite(p.max_density) || p.max_density <= 1e-9 || p.max_density > 1.0)
+      throw std::invalid_argument("max_density must be finite and in the interval (0, 1]");
+    if (p.enable_dropping) throw std::invalid_argument("fixed_pattern requires enable_dropping = 0");
+    if (p.num_threads <= 0) throw std::invalid_argument("Invalid threads count value");
+
+    A_ = Upload(c, Ah.rows, Ah.cols, Ah.col_offsets, Ah.row_indices, Ah.values);
+    DMat AT = Transpose(c, A_);
+    at_pat_ = AT.pat;
+    at_d_ = std::move(AT.val);
+    // Pr: diagonal (given, or diag(A)^-1 as methods_common.cpp:50-63).
+    pr_d_ = c.Alloc<double>(n_);
+    if (Prh) {
+      DMat Pr = Upload(c, Prh->rows, Prh->cols, Prh->col_offsets, Prh->row_indices, Prh->values);
+      if (!IsFullDiagonal(c, Pr)) throw std::invalid_argument("fixed_pattern requires a diagonal Pr");
+      ExtractDiagonal(c, Pr, pr_d_.get());
+    } else {
+      std::vector<double> d(n_);
+      ExtractDiagonal(c, A_, pr_d_.get());
+      c.ReadBack(d.data(), pr_d_.get(), sizeof(double) * n_);
+      for (double& v : d) {
+        if (!std::isfinite(v) || v <= 0.0)
+          throw std::invalid_argument("A has an invalid diagonal entry; provide Pr explicitly");
+        v = 1.0 / v;
+        if (!std::isfinite(v))
+          throw std::invalid_argument("A diagonal cannot be inverted safely; provide Pr explicitly");
+      }
+      SPAI_HIP_CHECK(hipMemcpyAsync(pr_d_.get(), d.data(), sizeof(double) * n_, hipMemcpyHostToDevice, c.stream));
+      c.Sync();
+    }
+    // S = pattern(A) | pattern(M0) | I
+    DMat I = Identity(c, n_);
+    DMat S = Merge(c, A_, I, MergeOp::kCopyLeft, 0, 0, false);
+    if (M0h) {
+      m0_ = Upload(c, M0h->rows, M0h->cols, M0h->col_offsets, M0h->row_indices, M0h->values);
+      S = Merge(c, S, m0_, MergeOp::kCopyLeft, 0, 0, false);
+    } else {
+      m0_ = std::move(I);
+    }
+    s_pat_ = S.pat;
+    // S2 = pattern(A S) | S | I
+    DMat AS = SpGemm(c, A_, S, false, true);
+    DMat S2 = Merge(c, AS, S, MergeOp::kCopyLeft, 0, 0, false);
+    s2_pat_ = S2.pat;
+    nnz_s_ = s_pat_->nnz;
+    nnz_s2_ = s2_pat_->nnz;
+    s2col_ = c.Alloc<int>(nnz_s2_);
+    ColumnOfEntry<<<Blocks1(n_), kBlk, 0, c.stream>>>((int)n_, s2_pat_->ptr.get(), s2col_.get());
+    s2_to_s_ = c.Alloc<int>(nnz_s2_);
+    SPAI_HIP_CHECK(hipMemsetAsync(s2_to_s_.get(), 0xFF, sizeof(int) * nnz_s2_, c.stream));
+    auto err = c.Alloc<int>(1);
+    SPAI_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), c.stream));
+    MapSubPattern<<<Blocks1(n_), kBlk, 0, c.stream>>>((int)n_, s_pat_->ptr.get(), s_pat_->idx.get(),
+                                                      s2_pat_->ptr.get(), s2_pat_->idx.get(), s2_to_s_.get(),
+                                                      err.get());
+    int e = 0;
+    c.ReadBack(&e, err.get(), sizeof(int));
+    if (e) throw std::runtime_error("fixed pattern S is not contained in pattern(A S)");
+    // value arrays
+    atv_ = c.Alloc<V>(at_pat_->nnz);
+    ConvertValues<V><<<Blocks1(at_pat_->nnz), kBlk, 0, c.stream>>>(at_pat_->nnz, at_d_.get(), atv_.get());
+    prv_ = c.Alloc<V>(n_);
+    ConvertValues<V><<<Blocks1(n_), kBlk, 0, c.stream>>>(n_, pr_d_.get(), prv_.get());
+    M_ = c.Alloc<V>(nnz_s_);
+    Z_ = c.Alloc<V>(nnz_s_);
+    P_ = c.Alloc<V>(nnz_s_);
+    R_ = c.Alloc<V>(nnz_s2_);
+    AZ_ = c.Alloc<V>(nnz_s2_);
+    AP_ = c.Alloc<V>(nnz_s2_);
+    grid2_ = GridFor(nnz_s2_);
+    grid1_ = GridFor(nnz_s_);
+    part_ = c.Alloc<double>(8 * kMaxGrid);
+    sc_ = c.Alloc<double>(kNumSlots);
+    flags_ = c.Alloc<Flags>(1);
+    hist_ = c.Alloc<double>(p.max_iterations + 1);
+    c.Sync();
+  }
+
+  Ctx& context() override { return ctx_; }
+  const DMat& MatrixA() const override { return A_; }
+
+  int64_t Build() override {
+    Ctx& c = ctx_;
+    hipStream_t st = c.stream;
+    // M = M0 on S, P = 0
+    SampleOnPattern<V><<<Blocks1(n_), kBlk, 0, st>>>((int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), m0_.ptr(),
+                                                     m0_.idx(), m0_.v(), M_.get());
+    FillZero<V><<<Blocks1(nnz_s_), kBlk, 0, st>>>(nnz_s_, P_.get());
+    SPAI_HIP_CHECK(hipMemsetAsync(flags_.get(), 0, sizeof(Flags), st));
+    const int* s2r = s2_pat_->idx.get();
+    const int* s2c = s2col_.get();
+    const int* map = s2_to_s_.get();
+    const int* atp = at_pat_->ptr.get();
+    const int* ati = at_pat_->idx.get();
+    const int* sp = s_pat_->ptr.get();
+    const int* si = s_pat_->idx.get();
+    const double tol = prm_.tolerance;
+    const int G2 = (int)grid2_, G1 = (int)grid1_;
+    ResidualKernel<V><<<grid2_, kBlk, 0, st>>>(nnz_s2_, s2r, s2c, map, atp, ati, atv_.get(), sp, si, M_.get(),
+                                               prv_.get(), R_.get(), Z_.get(), P_.get(),
+                                               method_ == SPAI_GLOBAL_CG ? 1 : 0, part_.get(), flags_.get());
+    AppendInitial<<<1, kBlk, 0, st>>>(part_.get(), G2, tol, hist_.get(), sc_.get(), flags_.get());
+    for (int64_t it = 1; it <= prm_.max_iterations; ++it) {
+      if (method_ == SPAI_GLOBAL_LOMR) {
+        LomrProject<V><<<grid2_, kBlk, 0, st>>>(nnz_s2_, s2r, s2c, atp, ati, atv_.get(), sp, si, Z_.get(), P_.get(),
+                                                R_.get(), prv_.get(), AZ_.get(), AP_.get(), part_.get(),
+                                                flags_.get());
+        LomrSteps<<<1, kBlk, 0, st>>>(part_.get(), G2, it == 1 ? 1 : 0, sc_.get(), flags_.get());
+        LomrUpdate<V><<<grid2_, kBlk, 0, st>>>(nnz_s2_, s2r, map, AZ_.get(), AP_.get(), prv_.get(), R_.get(),
+                                               M_.get(), Z_.get(), P_.get(), sc_.get(), part_.get(), flags_.get());
+        AppendResidual<<<1, kBlk, 0, st>>>(part_.get(), G2, tol, hist_.get(), flags_.get());
+      } else if (method_ == SPAI_GLOBAL_MR) {
+        MrProject<V><<<grid2_, kBlk, 0, st>>>(nnz_s2_, s2r, s2c, atp, ati, atv_.get(), sp, si, Z_.get(), R_.get(),
+                                              prv_.get(), part_.get(), flags_.get());
+        MrStep<<<1, kBlk, 0, st>>>(part_.get(), G2, sc_.get(), flags_.get());
+        AxpyS<V><<<grid1_, kBlk, 0, st>>>(nnz_s_, M_.get(), Z_.get(), sc_.get(), flags_.get());
+        ResidualKernel<V><<<grid2_, kBlk, 0, st>>>(nnz_s2_, s2r, s2c, map, atp, ati, atv_.get(), sp, si, M_.get(),
+                                                   prv_.get(), R_.get(), Z_.get(), P_.get(), 0, part_.get(),
+                                                   flags_.get());
+        AppendResidual<<<1, kBlk, 0, st>>>(part_.get(), G2, tol, hist_.get(), flags_.get());
+      } else {
+        CgProject<V><<<grid2_, kBlk, 0, st>>>(nnz_s2_, s2r, s2c, map, atp, ati, atv_.get(), sp, si, P_.get(),
+                                              AP_.get(), part_.get(), flags_.get());
+        CgAlpha<<<1, kBlk, 0, st>>>(part_.get(), G2, sc_.get(), flags_.get());
+        CgUpdate<V><<<grid2_, kBlk, 0, st>>>(nnz_s2_, s2r, map, AP_.get(), prv_.get(), R_.get(), M_.get(), P_.get(),
+                                             Z_.get(), sc_.get(), part_.get(), flags_.get());
+        CgAppendBeta<<<1, kBlk, 0, st>>>(part_.get(), G2, tol, hist_.get(), sc_.get(), flags_.get());
+        CgDirection<V><<<grid1_, kBlk, 0, st>>>(nnz_s_, P_.get(), Z_.get(), sc_.get(), flags_.get());
+      }
+      (void)G1;
+    }
+    SPAI_HIP_CHECK(hipGetLastError());
+    Flags f;
+    c.ReadBack(&f, flags_.get(), sizeof(Flags));
+    last_flags_ = f;
+    return f.iterations;
+  }
+
+  void Output(EngineOutput& out) override {
+    Ctx& c = ctx_;
+    const Flags f = last_flags_;
+    out.iterations = f.iterations;
+    out.converged = f.stop == 2;
+    std::vector<double> h(f.iterations + 1);
+    c.ReadBack(h.data(), hist_.get(), sizeof(double) * h.size());
+    const double entries = (double)n_ * (double)n_;
+    const double dens = (double)nnz_s_ / entries;
+    out.history.clear();
+    for (int64_t i = 0; i <= f.iterations; ++i) out.history.push_back(SingleIterationData{i, h[i], dens, dens});
+    DMat Md = MatrixM();
+    out.M.rows = n_;
+    out.M.cols = n_;
+    out.M.ptr.resize(n_ + 1);
+    out.M.idx.resize(nnz_s_);
+    out.M.val.resize(nnz_s_);
+    Download(c, Md, out.M.ptr.data(), out.M.idx.data(), out.M.val.data());
+  }
+
+  DMat MatrixM() override {
+    Ctx& c = ctx_;
+    DMat Md;
+    Md.pat = s_pat_;
+    Md.val = c.Alloc<double>(nnz_s_);
+    ToDouble<V><<<Blocks1(nnz_s_), kBlk, 0, c.stream>>>(nnz_s_, M_.get(), Md.val.get());
+    return Md;
+  }
+
+ private:
+  Ctx ctx_;
+  int64_t method_;
+  EngineParams prm_;
+  int64_t n_ = 0, nnz_s_ = 0, nnz_s2_ = 0;
+  DMat A_, m0_;
+  PatternPtr at_pat_, s_pat_, s2_pat_;
+  DBuf<double> at_d_, pr_d_;
+  DBuf<V> atv_, prv_, M_, Z_, P_, R_, AZ_, AP_;
+  DBuf<int> s2col_, s2_to_s_;
+  DBuf<double> part_, sc_, hist_;
+  DBuf<Flags> flags_;
+  unsigned grid1_ = 1, grid2_ = 1;
+  Flags last_flags_{0, 0};
+};
+
+}  // namespace
+
+std::unique_ptr<FixedProblem> MakeFixedProblem(int64_t method, const MatrixData& A, const MatrixData* M0,
+                                               const MatrixData* Pr, const EngineParams& params) {
+  if (params.fp32) return std::make_unique<FixedImpl<float>>(method, A, M0, Pr, params);
+  return std::make_unique<FixedImpl<double>>(method, A, M0, Pr, params);
+}
+
+EngineOutput RunFixedEngine(int64_t method, const MatrixData& A, const MatrixData* M0, const MatrixData* Pr,
+                            const EngineParams& params) {
+  auto prob = MakeFixedProblem(method, A, M0, Pr, params);
+  Ctx& c = prob->context();
+  hipEvent_t e0, e1;
+  SPAI_HIP_CHECK(hipEventCreate(&e0));
+  SPAI_HIP_CHECK(hipEventCreate(&e1));
+  SPAI_HIP_CHECK(hipEventRecord(e0, c.stream));
+  prob->Build();
+  EngineOutput out;
+  prob->Output(out);
+  SPAI_HIP_CHECK(hipEventRecord(e1, c.stream));
+  SPAI_HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  SPAI_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  out.device_seconds = ms * 1e-3;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return out;
+}
+
+}  // namespace spai

@@ -1,0 +1,33 @@
+// Fixed-pattern builders (M restricted to S = pattern(A) | pattern(M0) | I)
+// in FP64 or FP32-bulk form.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+
+#include "engine.hpp"
+#include "ops.hpp"
+
+namespace spai {
+
+EngineOutput RunFixedEngine(int64_t method, const MatrixData& A, const MatrixData* M0, const MatrixData* Pr,
+                            const EngineParams& params);
+
+// Device-resident fixed-pattern problem: A, S and S2 = pattern(A S) | S | I
+// set up once, builds replayable with every operand in HBM.
+class FixedProblem {
+ public:
+  virtual ~FixedProblem() = default;
+  // Runs the build from M0; returns the iterations completed.
+  virtual int64_t Build() = 0;
+  virtual void Output(EngineOutput& out) = 0;
+  // Device copy of the built M as FP64 CSC on S (for the PCG).
+  virtual DMat MatrixM() = 0;
+  virtual const DMat& MatrixA() const = 0;
+  virtual Ctx& context() = 0;
+};
+
+std::unique_ptr<FixedProblem> MakeFixedProblem(int64_t method, const MatrixData& A, const MatrixData* M0,
+                                               const MatrixData* Pr, const EngineParams& params);
+
+}  // namespace spai

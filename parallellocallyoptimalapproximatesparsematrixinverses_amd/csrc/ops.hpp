@@ -1,0 +1,111 @@
+// Host-side entry points of the device kernels (one call = one or a few
+// launches on ctx.stream).  Every matrix is CSC with sorted rows.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "device.hpp"
+
+namespace spai {
+
+struct Ctx {
+  hipStream_t stream = nullptr;
+  Pool pool;
+  int device = 0;
+  int num_cus = 256;
+  // pinned host staging for small device-to-host reads
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0;
+
+  Ctx();
+  ~Ctx();
+  Ctx(const Ctx&) = delete;
+  Ctx& operator=(const Ctx&) = delete;
+
+  template <class T>
+  DBuf<T> Alloc(size_t n) {
+    return DBuf<T>(&pool, n);
+  }
+  // Synchronous small device-to-host copy through pinned memory.
+  void ReadBack(void* host, const void* dev, size_t bytes);
+  void Sync();
+};
+
+// ---------------------------------------------------------------- patterns
+PatternPtr MakePattern(Ctx& ctx, int64_t rows, int64_t cols, int64_t nnz);
+// Upload a CSC matrix with int64 host indices (validated by the caller).
+DMat Upload(Ctx& ctx, int64_t rows, int64_t cols, const int64_t* ptr, const int64_t* idx, const double* val);
+// Download into host arrays (ptr: cols + 1, idx/val: nnz).
+void Download(Ctx& ctx, const DMat& m, int64_t* ptr, int64_t* idx, double* val);
+DMat EmptyMatrix(Ctx& ctx, int64_t rows, int64_t cols);
+DMat Identity(Ctx& ctx, int64_t n);
+DMat Clone(Ctx& ctx, const DMat& m);
+// Values-only copy sharing the pattern.
+DMat ShareClone(Ctx& ctx, const DMat& m);
+
+// Exclusive scan of int32 counts into ptr[0..n]; returns the total (blocking).
+int64_t ScanCounts(Ctx& ctx, const int* counts, int64_t n, int* ptr);
+
+// ---------------------------------------------------------------- products
+// Conservative product C = X * Y (Eigen's column-wise Gustavson order:
+// sum over Y(:, j)'s rows k ascending, first product then +=).
+// residual: C = I - X * Y with the diagonal always present, then exact zeros
+// pruned.  prune_zero: drop exact zeros from C.
+DMat SpGemm(Ctx& ctx, const DMat& X, const DMat& Y, bool prune_zero, bool residual);
+// C = diag(d) * Y, d on the device (length Y.rows).
+DMat DiagScale(Ctx& ctx, const double* d, const DMat& Y, bool prune_zero);
+
+// ---------------------------------------------------------------- unions
+enum class MergeOp : int {
+  kAddScaled = 0,  // l + a * r        (AddScaled; l + 0.0 / 0.0 + a * r)
+  kLinComb = 1,    // l * b + a * r    (LinearCombination(y=l, x=r))
+  kSymmetric = 2,  // 0.5 * (l + r)    (EnsureSymmetry)
+  kCopyLeft = 3,   // l               (prune only)
+};
+DMat Merge(Ctx& ctx, const DMat& L, const DMat& R, MergeOp op, double a, double b, bool prune_zero);
+// Drop exact zeros (returns the input when it has none).
+DMat PruneZeros(Ctx& ctx, DMat&& X);
+// X(:, j) *= f[j] (f on the device), then drop exact zeros.
+DMat ColScale(Ctx& ctx, const DMat& X, const double* f, bool prune_zero);
+DMat Transpose(Ctx& ctx, const DMat& X);
+
+// ---------------------------------------------------------------- reductions
+double FrobDot(Ctx& ctx, const DMat& X, const DMat& Y);
+double SqNorm(Ctx& ctx, const DMat& X);
+// Per-column intersection dots / squared norms into out[0..cols) (device).
+void ColumnDots(Ctx& ctx, const DMat& X, const DMat& Y, double* out);
+void ColumnSqNorms(Ctx& ctx, const DMat& X, double* out);
+// Per-batch sequential squared norms for the batch column ranges (device out).
+void BatchSqNorms(Ctx& ctx, const DMat& X, const int* batch_first, int nbatches, double* out);
+
+// ---------------------------------------------------------------- dropping
+// ApplyDroppingStrategy stage 1: keep v != 0 and (diagonal or |v| >= eps*||col||).
+DMat ThresholdColumns(Ctx& ctx, const DMat& M, double eps);
+// DropForMatrixType stage 1: keep |v| > sqrt(batch_sq[b]) * eps for the batch
+// holding the column (batch_of_col on device).
+DMat ThresholdBatches(Ctx& ctx, const DMat& X, const double* batch_sq, const int* batch_of_col, double eps);
+// Per-column top-k: keep the diagonal and the k largest |v| off the diagonal.
+DMat ColumnTopK(Ctx& ctx, const DMat& X, int64_t k);
+// Global select: among candidates (drop keys in CSC order), drops the
+// `drop` smallest under (key, position); candidates are the off-diagonal
+// entries (score mode, scores m^2 c_k + 2 m (A R)_kl) or every entry (|v| mode).
+DMat DropSmallestByScore(Ctx& ctx, const DMat& M, const DMat& AR, const double* a_col_sq, int64_t drop);
+DMat DropSmallestByAbs(Ctx& ctx, const DMat& X, int64_t drop);
+
+// ---------------------------------------------------------------- misc
+void FillValue(Ctx& ctx, double* p, int64_t n, double v);
+// Inner-outer per-column step sizes (inner_outer_common.hpp:50-69): delta,
+// gamma and the normalisations 1/delta from the column dots.  two_term selects
+// the LOMR 2x2 solve.
+void InnerOuterSteps(Ctx& ctx, int64_t n, bool lomr, bool two_term, const double* r_az, const double* az_sq,
+                     const double* az_aq, const double* aq_sq, const double* r_aq, double* delta, double* gamma,
+                     double* normalization);
+// out[j] = GetSafeQuotient(num[j], den[j]).
+void SafeQuotients(Ctx& ctx, int64_t n, const double* num, const double* den, double* out);
+// Device vector of the diagonal entries of A (0 where absent).
+void ExtractDiagonal(Ctx& ctx, const DMat& A, double* d);
+// True when X has exactly one entry per column, on the diagonal.
+bool IsFullDiagonal(Ctx& ctx, const DMat& X);
+
+}  // namespace spai

@@ -1,0 +1,62 @@
+// Preconditioned CG (methods_cython/backward_error.pyx:69-144) on the device.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "ops.hpp"
+
+namespace spai {
+
+// CSR matrix on the device (int32 offsets / columns); values FP64, plus an
+// optional FP32 copy for the FP32-bulk preconditioner apply.
+struct DCsr {
+  int64_t n = 0, nnz = 0;
+  DBuf<int> ptr, col;
+  DBuf<double> val;
+  DBuf<float> val32;
+};
+
+DCsr CsrFromHost(Ctx& ctx, int64_t n, int64_t nnz, const int64_t* ptr, const int64_t* col, const double* val);
+// CSR of the matrix stored as CSC in m (i.e. the transpose of its CSC arrays).
+DCsr CsrFromCsc(Ctx& ctx, const DMat& m);
+void MakeFp32Copy(Ctx& ctx, DCsr& m);
+// y = A x (methods_cython/backward_error.pyx:27-34 _multiply).
+void SpMV(Ctx& ctx, const DCsr& A, const double* x, double* y);
+
+class PcgSolver {
+ public:
+  PcgSolver(Ctx& ctx, const DCsr& A, const DCsr& M, bool m_fp32);
+  ~PcgSolver();
+  // Runs from x0 = 0 for b (device pointer, length n).  Returns the status.
+  int Solve(const double* b_dev, int64_t max_iterations, double tolerance, int64_t* count);
+  void History(double* host, int64_t count);
+  const double* X() const { return x_.get(); }
+  // Average device time of one iteration's A-SpMV kernel over the last solve
+  // (microseconds, from HIP events around a standalone replay).
+  double TimeSpmvA(int reps);
+  double TimeIterationUs(int reps);
+
+ private:
+  void Record(bool fused, int64_t iterations);
+  Ctx& ctx_;
+  const DCsr& A_;
+  const DCsr& M_;
+  bool m_fp32_;
+  int64_t n_;
+  unsigned grid_ = 1;
+  DBuf<double> x_, r_, z_, p_, ap_, part_, hist_;
+  DBuf<double> scal_;
+  DBuf<int> ist_;
+  DBuf<unsigned> counters_;
+  const double* b_ = nullptr;
+  double tol_ = 0.0;
+  int64_t max_it_ = 0;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t exec_ = nullptr;
+  int64_t graph_key_max_ = -1;
+  double graph_key_tol_ = -1.0;
+  const double* graph_key_b_ = nullptr;
+};
+
+}  // namespace spai

@@ -1,0 +1,1276 @@
+// Sparse kernels of the reference-semantics (dynamic pattern) path: scan,
+// two-phase SpGEMM, union merges, compaction, transposes and reductions.
+//
+// Values follow Eigen's operation order exactly where the reference's result
+// depends on it (products, unions, scalings: no contraction, -ffp-contract=off),
+// so the patterns these kernels produce are bit-identical to the reference's.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <stdexcept>
+#include <string>
+
+#include "dev_common.hpp"
+#include "ls2x2.hpp"
+#include "ops.hpp"
+
+namespace spai {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned Blocks(int64_t n, int per_block = kBlock) {
+  int64_t b = (n + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  if (b > (int64_t(1) << 30)) throw std::runtime_error("grid too large");
+  return static_cast<unsigned>(b);
+}
+
+// ------------------------------------------------------------------ scan
+constexpr int kScanItems = 4;  // per thread
+constexpr int kScanTile = kBlock * kScanItems;
+
+__global__ __launch_bounds__(kBlock) void ScanTileSums(const int* __restrict__ in, int64_t n,
+                                                       int64_t* __restrict__ tile_sums) {
+  __shared__ int64_t red[16];
+  const int64_t base = int64_t(blockIdx.x) * kScanTile;
+  int64_t s = 0;
+  for (int t = 0; t < kScanItems; ++t) {
+    const int64_t i = base + int64_t(t) * kBlock + threadIdx.x;
+    if (i < n) s += in[i];
+  }
+  s = BlockSum<int64_t>(s, red);
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = s;
+}
+
+// Single block: exclusive scan of the tile sums in place; total to *total.
+__global__ __launch_bounds__(1024) void ScanTiles(int64_t* __restrict__ sums, int64_t ntiles,
+                                                  int64_t* __restrict__ total) {
+  __shared__ int64_t wsum[16];
+  __shared__ int64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t b0 = 0; b0 < ntiles; b0 += blockDim.x) {
+    const int64_t i = b0 + threadIdx.x;
+    const int64_t v = i < ntiles ? sums[i] : 0;
+    int64_t incl = WaveInclusiveScan64(v);
+    if (LaneId() == kWave - 1) wsum[WaveInBlock()] = incl;
+    __syncthreads();
+    int64_t wbase = 0;
+    for (int w = 0; w < WaveInBlock(); ++w) wbase += wsum[w];
+    const int64_t excl = carry + wbase + incl - v;
+    __syncthreads();
+    if (i < ntiles) sums[i] = excl;
+    if (threadIdx.x == blockDim.x - 1) carry = excl + v;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void ScanApply(const int* __restrict__ in, int64_t n,
+                                                    const int64_t* __restrict__ tile_offsets,
+                                                    int* __restrict__ out) {
+  __shared__ int64_t wsum[16];
+  __shared__ int64_t carry;
+  const int64_t base = int64_t(blockIdx.x) * kScanTile;
+  if (threadIdx.x == 0) carry = tile_offsets[blockIdx.x];
+  __syncthreads();
+  for (int t = 0; t < kScanItems; ++t) {
+    const int64_t i = base + int64_t(t) * kBlock + threadIdx.x;
+    const int64_t v = i < n ? in[i] : 0;
+    const int64_t incl = WaveInclusiveScan64(v);
+    if (LaneId() == kWave - 1) wsum[WaveInBlock()] = incl;
+    __syncthreads();
+    int64_t wbase = 0;
+    for (int w = 0; w < WaveInBlock(); ++w) wbase += wsum[w];
+    const int64_t excl = carry + wbase + incl - v;
+    if (i < n) out[i] = static_cast<int>(excl);
+    __syncthreads();
+    if (threadIdx.x == kBlock - 1) carry = excl + v;
+    __syncthreads();
+  }
+}
+
+__global__ void WriteLast(int* __restrict__ out, int64_t n, const int64_t* __restrict__ total) {
+  out[n] = static_cast<int>(*total);
+}
+
+// ------------------------------------------------------------------ SpGEMM
+constexpr int kEscCap = 2048;     // expansion keys per wavefront (LDS sort tier)
+constexpr int kSymWaves = 2;      // wavefronts per block in the symbolic kernel
+constexpr int kNumCap = 1024;     // output rows per column held in LDS (numeric)
+constexpr int kNumWaves = 4;
+
+// u_j = sum over Y(:, j) of nnz(X(:, k)) (+1 for the identity); columns whose
+// expansion exceeds the LDS tier go to `big_list`.
+__global__ __launch_bounds__(kBlock) void ExpansionSizes(int ncols, const int* __restrict__ xptr,
+                                                         const int* __restrict__ yptr, const int* __restrict__ yidx,
+                                                         int add_identity, int64_t* __restrict__ upper,
+                                                         int* __restrict__ big_list, int* __restrict__ big_count) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncols) return;
+  int64_t u = add_identity ? 1 : 0;
+  for (int p = yptr[j]; p < yptr[j + 1]; ++p) {
+    const int k = yidx[p];
+    u += xptr[k + 1] - xptr[k];
+  }
+  upper[j] = u;
+  if (u > kEscCap) {
+    const int slot = atomicAdd(big_count, 1);
+    big_list[slot] = j;
+  }
+}
+
+// Expand-sort-compress symbolic phase, one wavefront per output column:
+// gathers the rows of X(:, k) for every k in Y(:, j) into LDS, bitonic-sorts
+// them and counts (fill == 0) or writes (fill == 1) the distinct rows.
+__global__ __launch_bounds__(kSymWaves* kWave) void SymbolicEsc(
+    int ncols, const int* __restrict__ xptr, const int* __restrict__ xidx, const int* __restrict__ yptr,
+    const int* __restrict__ yidx, const int64_t* __restrict__ upper, int add_identity, int fill,
+    int* __restrict__ counts, const int* __restrict__ cptr, int* __restrict__ cidx) {
+  __shared__ int keys[kSymWaves][kEscCap];
+  __shared__ int pre[kSymWaves][kWave];
+  __shared__ int beg[kSymWaves][kWave];
+  const int w = WaveInBlock(), lane = LaneId();
+  const int j = blockIdx.x * kSymWaves + w;
+  if (j >= ncols) return;
+  if (upper[j] > kEscCap) return;
+  int* K = keys[w];
+  int off = 0;
+  const int ye = yptr[j + 1];
+  for (int p0 = yptr[j]; p0 < ye; p0 += kWave) {
+    const int p = p0 + lane;
+    int b = 0, c = 0;
+    if (p < ye) {
+      const int k = yidx[p];
+      b = xptr[k];
+      c = xptr[k + 1] - b;
+    }
+    const int incl = WaveInclusiveScan(c);
+    const int tot = __shfl(incl, kWave - 1, kWave);
+    pre[w][lane] = incl - c;
+    beg[w][lane] = b;
+    WaveFence();
+    for (int e = lane; e < tot; e += kWave) {
+      int lo = 0, hi = kWave;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[w][mid] <= e)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      K[off + e] = xidx[beg[w][lo] + (e - pre[w][lo])];
+    }
+    off += tot;
+    WaveFence();
+  }
+  if (add_identity) {
+    if (lane == 0) K[off] = j;
+    off += 1;
+  }
+  int n2 = 1;
+  while (n2 < off) n2 <<= 1;
+  for (int e = off + lane; e < n2; e += kWave) K[e] = INT_MAX;
+  WaveFence();
+  for (int k = 2; k <= n2; k <<= 1) {
+    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+      for (int i = lane; i < n2; i += kWave) {
+        const int ixj = i ^ jj;
+        if (ixj > i) {
+          const int a = K[i], b = K[ixj];
+          const bool asc = (i & k) == 0;
+          if (asc ? (a > b) : (a < b)) {
+            K[i] = b;
+            K[ixj] = a;
+          }
+        }
+      }
+      WaveFence();
+    }
+  }
+  if (!fill) {
+    int cnt = 0;
+    for (int i = lane; i < off; i += kWave) cnt += (i == 0 || K[i] != K[i - 1]) ? 1 : 0;
+    cnt = WaveSumInt(cnt);
+    if (lane == 0) counts[j] = cnt;
+  } else {
+    int base = cptr[j];
+    for (int i0 = 0; i0 < off; i0 += kWave) {
+      const int i = i0 + lane;
+      const bool flag = i < off && (i == 0 || K[i] != K[i - 1]);
+      const uint64_t mask = __ballot(flag);
+      if (flag) cidx[base + LanePrefixCount(mask)] = K[i];
+      base += __popcll(mask);
+    }
+  }
+}
+
+// Block-wide ordered compaction helper: returns this thread's exclusive rank
+// among flagged threads and writes the block total to *total.
+__device__ int BlockRank(bool flag, int* wcount /* LDS >= 16 */, int* total) {
+  const uint64_t mask = __ballot(flag);
+  const int w = WaveInBlock();
+  const int nw = blockDim.x / kWave;
+  if (LaneId() == 0) wcount[w] = __popcll(mask);
+  __syncthreads();
+  int base = 0, t = 0;
+  for (int i = 0; i < nw; ++i) {
+    if (i < w) base += wcount[i];
+    t += wcount[i];
+  }
+  *total = t;
+  __syncthreads();
+  return base + LanePrefixCount(mask);
+}
+
+// Symbolic phase for columns whose expansion exceeds the LDS tier: a block per
+// column marks rows in a per-block stamp array (length nrows, initialised to
+// -1) and compacts them in row order.
+__global__ __launch_bounds__(kBlock) void SymbolicStamp(int nlist, const int* __restrict__ list, int nrows,
+                                                        const int* __restrict__ xptr, const int* __restrict__ xidx,
+                                                        const int* __restrict__ yptr, const int* __restrict__ yidx,
+                                                        int add_identity, int fill, int* __restrict__ stamp_ws,
+                                                        int* __restrict__ counts, const int* __restrict__ cptr,
+                                                        int* __restrict__ cidx) {
+  __shared__ int wcount[16];
+  int* stamp = stamp_ws + size_t(blockIdx.x) * size_t(nrows);
+  for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
+    const int j = list[li];
+    for (int p = yptr[j]; p < yptr[j + 1]; ++p) {
+      const int k = yidx[p];
+      for (int q = xptr[k] + threadIdx.x; q < xptr[k + 1]; q += blockDim.x) stamp[xidx[q]] = j;
+    }
+    if (add_identity && threadIdx.x == 0) stamp[j] = j;
+    __syncthreads();
+    int base = fill ? cptr[j] : 0;
+    for (int r0 = 0; r0 < nrows; r0 += blockDim.x) {
+      const int r = r0 + threadIdx.x;
+      const bool flag = r < nrows && stamp[r] == j;
+      int tot = 0;
+      const int rank = BlockRank(flag, wcount, &tot);
+      if (fill && flag) cidx[base + rank] = r;
+      base += tot;
+    }
+    if (!fill && threadIdx.x == 0) counts[j] = base;
+    __syncthreads();
+  }
+}
+
+// Numeric phase, one wavefront per output column.  Contributions are added in
+// Eigen's order: Y(:, j)'s entries k ascending, each scattering X(:, k) into
+// the column's accumulator (distinct rows inside one k, so the lanes never
+// collide).  residual: value = -(sum) (+1 on the diagonal).
+__global__ __launch_bounds__(kNumWaves* kWave) void NumericGustavson(
+    int ncols, const int* __restrict__ xptr, const int* __restrict__ xidx, const double* __restrict__ xval,
+    const int* __restrict__ yptr, const int* __restrict__ yidx, const double* __restrict__ yval,
+    const int* __restrict__ cptr, const int* __restrict__ cidx, double* __restrict__ cval, int residual,
+    int* __restrict__ error_flag) {
+  __shared__ int rows_s[kNumWaves][kNumCap];
+  __shared__ double acc_s[kNumWaves][kNumCap];
+  const int w = WaveInBlock(), lane = LaneId();
+  const int j = blockIdx.x * kNumWaves + w;
+  if (j >= ncols) return;
+  const int cb = cptr[j];
+  const int L = cptr[j + 1] - cb;
+  const bool in_lds = L <= kNumCap;
+  if (in_lds) {
+    for (int e = lane; e < L; e += kWave) {
+      rows_s[w][e] = cidx[cb + e];
+      acc_s[w][e] = 0.0;
+    }
+  } else {
+    for (int e = lane; e < L; e += kWave) cval[cb + e] = 0.0;
+  }
+  WaveFence();
+  const int* R = in_lds ? rows_s[w] : (cidx + cb);
+  for (int p = yptr[j]; p < yptr[j + 1]; ++p) {
+    const int k = yidx[p];
+    const double y = yval[p];
+    for (int q = xptr[k] + lane; q < xptr[k + 1]; q += kWave) {
+      const int i = xidx[q];
+      const double x = xval[q];
+      const int pos = LowerBound(R, 0, L, i);
+      if (pos >= L || R[pos] != i) {
+        atomicOr(error_flag, 1);
+        continue;
+      }
+      if (in_lds) {
+        acc_s[w][pos] += x * y;
+      } else {
+        atomicAdd(&cval[cb + pos], x * y);
+      }
+    }
+    WaveFence();
+  }
+  for (int e = lane; e < L; e += kWave) {
+    double v = in_lds ? acc_s[w][e] : __hip_atomic_load(&cval[cb + e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (residual) {
+      v = -v;
+      if (R[e] == j) v += 1.0;
+    }
+    cval[cb + e] = v;
+  }
+}
+
+// ------------------------------------------------------------------ filters
+// Keep flags -> per-column counts.
+__global__ void CountKept(int ncols, const int* __restrict__ ptr, const unsigned char* __restrict__ keep,
+                          int* __restrict__ counts) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncols) return;
+  int c = 0;
+  for (int p = ptr[j]; p < ptr[j + 1]; ++p) c += keep[p];
+  counts[j] = c;
+}
+
+__global__ void FillKept(int ncols, const int* __restrict__ ptr, const int* __restrict__ idx,
+                         const double* __restrict__ val, const unsigned char* __restrict__ keep,
+                         const int* __restrict__ optr, int* __restrict__ oidx, double* __restrict__ oval) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncols) return;
+  int o = optr[j];
+  for (int p = ptr[j]; p < ptr[j + 1]; ++p) {
+    if (keep[p]) {
+      oidx[o] = idx[p];
+      oval[o] = val[p];
+      ++o;
+    }
+  }
+}
+
+__global__ void FlagNonZero(int64_t nnz, const double* __restrict__ val, unsigned char* __restrict__ keep,
+                            int* __restrict__ dropped) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nnz) return;
+  const double v = val[i];
+  const bool k = !(fabs(v) <= 0.0);  // prune(0.0): keep |v| > 0 and NaN
+  keep[i] = k ? 1 : 0;
+  if (!k) atomicAdd(dropped, 1);
+}
+
+// ------------------------------------------------------------------ elementwise
+__global__ void DiagScaleKernel(int ncols, const int* __restrict__ ptr, const int* __restrict__ idx,
+                                const double* __restrict__ d, const double* __restrict__ yv,
+                                double* __restrict__ out) {
+  const int j = blockIdx.x;
+  for (int p = ptr[j] + threadIdx.x; p < ptr[j + 1]; p += blockDim.x) out[p] = d[idx[p]] * yv[p];
+}
+
+__global__ void ColScaleKernel(int ncols, const int* __restrict__ ptr, const double* __restrict__ f,
+                               const double* __restrict__ xv, double* __restrict__ out) {
+  const int j = blockIdx.x;
+  const double s = f[j];
+  for (int p = ptr[j] + threadIdx.x; p < ptr[j + 1]; p += blockDim.x) out[p] = xv[p] * s;
+}
+
+// ------------------------------------------------------------------ merges
+__device__ __forceinline__ double MergeBoth(int op, double l, double r, double a, double b) {
+  switch (op) {
+    case 0: return l + a * r;
+    case 1: return l * b + a * r;
+    case 2: return 0.5 * (l + r);
+    default: return l;
+  }
+}
+__device__ __forceinline__ double MergeLeft(int op, double l, double a, double b) {
+  switch (op) {
+    case 0: return l + 0.0;
+    case 1: return l * b + 0.0;
+    case 2: return 0.5 * (l + 0.0);
+    default: return l;
+  }
+}
+__device__ __forceinline__ double MergeRight(int op, double r, double a, double b) {
+  switch (op) {
+    case 0: return 0.0 + a * r;
+    case 1: return 0.0 + a * r;
+    case 2: return 0.5 * (0.0 + r);
+    default: return r;
+  }
+}
+
+// Union of two sorted columns, one thread per column (Eigen's binary
+// iterator order).  fill == 0 counts kept entries; fill == 1 writes them.
+__global__ void MergeKernel(int ncols, const int* __restrict__ lp, const int* __restrict__ li,
+                            const double* __restrict__ lv, const int* __restrict__ rp, const int* __restrict__ ri,
+                            const double* __restrict__ rv, int op, double a, double b, int prune, int fill,
+                            int* __restrict__ counts, const int* __restrict__ optr, int* __restrict__ oidx,
+                            double* __restrict__ oval) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncols) return;
+  int x = lp[j], xe = lp[j + 1], y = rp[j], ye = rp[j + 1];
+  int o = fill ? optr[j] : 0;
+  int c = 0;
+  while (x < xe || y < ye) {
+    int row;
+    double v;
+    if (x < xe && y < ye && li[x] == ri[y]) {
+      row = li[x];
+      v = MergeBoth(op, lv[x], rv[y], a, b);
+      ++x;
+      ++y;
+    } else if (x < xe && (y >= ye || li[x] < ri[y])) {
+      row = li[x];
+      v = MergeLeft(op, lv[x], a, b);
+      ++x;
+    } else {
+      row = ri[y];
+      v = MergeRight(op, rv[y], a, b);
+      ++y;
+    }
+    if (prune && !(fabs(v) > 0.0) && !isnan(v)) continue;
+    if (fill) {
+      oidx[o] = row;
+      oval[o] = v;
+      ++o;
+    }
+    ++c;
+  }
+  if (!fill) counts[j] = c;
+}
+
+// ------------------------------------------------------------------ reductions
+// Per-column intersection dot (or squared norm when Y == X) into out[j].
+__global__ void ColumnDotKernel(int ncols, const int* __restrict__ xp, const int* __restrict__ xi,
+                                const double* __restrict__ xv, const int* __restrict__ yp,
+                                const int* __restrict__ yi, const double* __restrict__ yv, int same,
+                                double* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncols) return;
+  double s = 0.0;
+  if (same) {
+    for (int p = xp[j]; p < xp[j + 1]; ++p) s += xv[p] * xv[p];
+  } else {
+    s = ColumnDot(xi, xv, xp[j], xp[j + 1], yi, yv, yp[j], yp[j + 1]);
+  }
+  out[j] = s;
+}
+
+// Deterministic sum of v[0..n): per-block partials, then one block.
+__global__ __launch_bounds__(kBlock) void SumPartials(const double* __restrict__ v, int64_t n,
+                                                      double* __restrict__ partial) {
+  __shared__ double red[16];
+  double s = 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    s += v[i];
+  s = BlockSum<double>(s, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// Sequential squared norm of each batch's column range (entries in CSC order).
+__global__ void BatchSqNormKernel(int nb, const int* __restrict__ first, const int* __restrict__ ptr,
+                                  const double* __restrict__ v, double* __restrict__ out) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double s = 0.0;
+  for (int p = ptr[first[b]]; p < ptr[first[b + 1]]; ++p) s += v[p] * v[p];
+  out[b] = s;
+}
+
+// ------------------------------------------------------------------ dropping
+constexpr double kDoubleEps = 2.220446049250313080847e-16;
+
+// ApplyDroppingStrategy threshold (dropping.cpp:124-133): sequential column
+// norm, keep v != 0 && (diag || |v| >= eps * norm).
+__global__ void ThresholdColumnsKernel(int ncols, const int* __restrict__ ptr, const int* __restrict__ idx,
+                                       const double* __restrict__ v, double eps, unsigned char* __restrict__ keep) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncols) return;
+  double s = 0.0;
+  for (int p = ptr[j]; p < ptr[j + 1]; ++p) s += v[p] * v[p];
+  const double thr = eps * sqrt(s);
+  for (int p = ptr[j]; p < ptr[j + 1]; ++p) {
+    const double x = v[p];
+    keep[p] = (x != 0.0 && (idx[p] == j || fabs(x) >= thr)) ? 1 : 0;
+  }
+}
+
+// DropForMatrixType per-batch prune (dropping.cpp:182-185): keep |v| > norm*eps.
+__global__ void ThresholdBatchesKernel(int64_t nnz_cols, int ncols, const int* __restrict__ ptr,
+                                       const double* __restrict__ v, const double* __restrict__ batch_sq,
+                                       const int* __restrict__ batch_of_col, double eps,
+                                       unsigned char* __restrict__ keep) {
+  const int j = blockIdx.x;
+  if (j >= ncols) return;
+  const double thr = sqrt(batch_sq[batch_of_col[j]]) * eps;
+  for (int p = ptr[j] + threadIdx.x; p < ptr[j + 1]; p += blockDim.x) keep[p] = !(fabs(v[p]) <= thr) ? 1 : 0;
+}
+
+// Per-column top-k by |v| (diagonal kept, ties keep the smaller row): rank of
+// each off-diagonal entry among the column's off-diagonal entries.
+__global__ void TopKKernel(int ncols, const int* __restrict__ ptr, const int* __restrict__ idx,
+                           const double* __restrict__ v, int64_t k, unsigned char* __restrict__ keep) {
+  const int j = blockIdx.x;
+  const int b = ptr[j], e = ptr[j + 1];
+  for (int p = b + threadIdx.x; p < e; p += blockDim.x) {
+    if (idx[p] == j) {
+      keep[p] = 1;
+      continue;
+    }
+    const double a = fabs(v[p]);
+    int64_t rank = 0;
+    for (int q = b; q < e; ++q) {
+      if (q == p || idx[q] == j) continue;
+      const double c = fabs(v[q]);
+      if (c > a || (c == a && idx[q] < idx[p])) ++rank;
+    }
+    keep[p] = rank < k ? 1 : 0;
+  }
+}
+
+// Candidate keys in CSC order.  mode 0: scores of off-diagonal entries of M
+// (m^2 * ||A e_k||^2 + 2 m (A R)_kl, -inf when non-finite), diagonal entries
+// are not candidates (key = UINT64_MAX).  mode 1: |v| of every entry.
+__global__ void CandidateKeys(int ncols, const int* __restrict__ ptr, const int* __restrict__ idx,
+                              const double* __restrict__ v, int mode, const int* __restrict__ arp,
+                              const int* __restrict__ ari, const double* __restrict__ arv,
+                              const double* __restrict__ col_sq, uint64_t* __restrict__ keys) {
+  const int j = blockIdx.x;
+  for (int p = ptr[j] + threadIdx.x; p < ptr[j + 1]; p += blockDim.x) {
+    const int row = idx[p];
+    if (mode == 1) {
+      keys[p] = OrderedKey(fabs(v[p]));
+      continue;
+    }
+    if (row == j) {
+      keys[p] = ~0ull;
+      continue;
+    }
+    const int q = FindSorted(ari, arp[j], arp[j + 1], row);
+    const double t = q >= 0 ? arv[q] : 0.0;
+    const double m = v[p];
+    double s = m * m * col_sq[row] + 2.0 * m * t;
+    if (!isfinite(s)) s = -INFINITY;
+    keys[p] = OrderedKey(s);
+  }
+}
+
+// Radix select state: [0] prefix, [1] mask, [2] remaining rank (1-based).
+__global__ __launch_bounds__(kBlock) void RadixHistogram(int64_t n, const uint64_t* __restrict__ keys,
+                                                         const uint64_t* __restrict__ state, int shift,
+                                                         unsigned* __restrict__ hist) {
+  __shared__ unsigned h[256];
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  const uint64_t prefix = state[0], mask = state[1];
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const uint64_t k = keys[i];
+    if (k == ~0ull || (k & mask) != prefix) continue;
+    atomicAdd(&h[(k >> shift) & 0xFF], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 256; i += blockDim.x)
+    if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+__global__ void RadixPick(uint64_t* __restrict__ state, int shift, unsigned* __restrict__ hist) {
+  if (threadIdx.x != 0) return;
+  uint64_t rem = state[2];
+  uint64_t cum = 0;
+  int bin = 255;
+  for (int b = 0; b < 256; ++b) {
+    if (cum + hist[b] >= rem) {
+      bin = b;
+      break;
+    }
+    cum += hist[b];
+  }
+  state[2] = rem - cum;
+  state[0] |= uint64_t(bin) << shift;
+  state[1] |= uint64_t(0xFF) << shift;
+  for (int b = 0; b < 256; ++b) hist[b] = 0;
+}
+
+__global__ void EqualFlags(int64_t n, const uint64_t* __restrict__ keys, const uint64_t* __restrict__ state,
+                           int* __restrict__ flags) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  flags[i] = keys[i] == state[0] ? 1 : 0;
+}
+
+// Finds the position of the r-th (1-based) key equal to the cutoff.
+__global__ void FindTiePosition(int64_t n, const int* __restrict__ flags, const int* __restrict__ scan,
+                                const uint64_t* __restrict__ state, int64_t* __restrict__ pos) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (flags[i] && uint64_t(scan[i]) + 1 == state[2]) *pos = i;
+}
+
+// Keep the entry unless it is a candidate at or below the cutoff (key, pos).
+__global__ void KeepAboveCutoff(int64_t n, const uint64_t* __restrict__ keys, const uint64_t* __restrict__ state,
+                                const int64_t* __restrict__ cut_pos, unsigned char* __restrict__ keep) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t k = keys[i];
+  const uint64_t c = state[0];
+  bool drop = false;
+  if (k != ~0ull) drop = k < c || (k == c && i <= *cut_pos);
+  keep[i] = drop ? 0 : 1;
+}
+
+// ------------------------------------------------------------------ transpose
+__global__ void RowCounts(int64_t nnz, const int* __restrict__ idx, int* __restrict__ counts) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nnz) return;
+  atomicAdd(&counts[idx[i]], 1);
+}
+
+__global__ void ScatterTranspose(int ncols, const int* __restrict__ ptr, const int* __restrict__ idx,
+                                 const double* __restrict__ v, const int* __restrict__ tptr,
+                                 int* __restrict__ cursor, int* __restrict__ tidx, double* __restrict__ tval) {
+  const int j = blockIdx.x;
+  for (int p = ptr[j] + threadIdx.x; p < ptr[j + 1]; p += blockDim.x) {
+    const int r = idx[p];
+    const int d = tptr[r] + atomicAdd(&cursor[r], 1);
+    tidx[d] = j;
+    tval[d] = v[p];
+  }
+}
+
+constexpr int kSortCap = 2048;
+
+// Sorts each segment [tptr[c], tptr[c+1]) by index (values follow).  Short
+// segments: insertion sort by one thread; medium: wavefront bitonic sort of
+// (index << 11 | slot) keys in LDS; long segments are listed for the stamp pass.
+__global__ void SortSegmentsSmall(int nseg, const int* __restrict__ tptr, int* __restrict__ tidx,
+                                  double* __restrict__ tval, int* __restrict__ long_list,
+                                  int* __restrict__ long_count, int* __restrict__ mid_list,
+                                  int* __restrict__ mid_count) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nseg) return;
+  const int b = tptr[c], e = tptr[c + 1];
+  const int len = e - b;
+  if (len > kSortCap) {
+    long_list[atomicAdd(long_count, 1)] = c;
+    return;
+  }
+  if (len > 32) {
+    mid_list[atomicAdd(mid_count, 1)] = c;
+    return;
+  }
+  for (int i = b + 1; i < e; ++i) {
+    const int key = tidx[i];
+    const double val = tval[i];
+    int k = i - 1;
+    while (k >= b && tidx[k] > key) {
+      tidx[k + 1] = tidx[k];
+      tval[k + 1] = tval[k];
+      --k;
+    }
+    tidx[k + 1] = key;
+    tval[k + 1] = val;
+  }
+}
+
+__global__ __launch_bounds__(kWave) void SortSegmentsWave(int nlist, const int* __restrict__ list,
+                                                          const int* __restrict__ tptr, int* __restrict__ tidx,
+                                                          double* __restrict__ tval, double* __restrict__ vtmp) {
+  __shared__ uint64_t K[kSortCap];
+  const int lane = LaneId();
+  for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
+    const int c = list[li];
+    const int b = tptr[c], len = tptr[c + 1] - b;
+    int n2 = 1;
+    while (n2 < len) n2 <<= 1;
+    for (int i = lane; i < n2; i += kWave)
+      K[i] = i < len ? ((uint64_t(uint32_t(tidx[b + i])) << 11) | uint64_t(i)) : ~0ull;
+    for (int i = lane; i < len; i += kWave) vtmp[b + i] = tval[b + i];
+    WaveFence();
+    for (int k = 2; k <= n2; k <<= 1) {
+      for (int jj = k >> 1; jj > 0; jj >>= 1) {
+        for (int i = lane; i < n2; i += kWave) {
+          const int ixj = i ^ jj;
+          if (ixj > i) {
+            const uint64_t x = K[i], y = K[ixj];
+            const bool asc = (i & k) == 0;
+            if (asc ? (x > y) : (x < y)) {
+              K[i] = y;
+              K[ixj] = x;
+            }
+          }
+        }
+        WaveFence();
+      }
+    }
+    for (int i = lane; i < len; i += kWave) {
+      const uint64_t key = K[i];
+      tidx[b + i] = int(key >> 11);
+      tval[b + i] = vtmp[b + int(key & 2047)];
+    }
+    WaveFence();
+  }
+}
+
+// Long segments: a block per segment, stamp[index] = slot, ordered compaction
+// over all indices (ws length = ncols per block, initialised to -1).
+__global__ __launch_bounds__(kBlock) void SortSegmentsStamp(int nlist, const int* __restrict__ list,
+                                                            const int* __restrict__ tptr, int* __restrict__ tidx,
+                                                            double* __restrict__ tval, int nidx,
+                                                            int* __restrict__ ws, int* __restrict__ itmp,
+                                                            double* __restrict__ vtmp) {
+  __shared__ int wcount[16];
+  int* stamp = ws + size_t(blockIdx.x) * size_t(nidx);
+  for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
+    const int c = list[li];
+    const int b = tptr[c], len = tptr[c + 1] - b;
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+      stamp[tidx[b + i]] = i;
+      vtmp[b + i] = tval[b + i];
+      itmp[b + i] = tidx[b + i];
+    }
+    __syncthreads();
+    int base = 0;
+    for (int r0 = 0; r0 < nidx; r0 += blockDim.x) {
+      const int r = r0 + threadIdx.x;
+      int slot = -1;
+      if (r < nidx) {
+        slot = stamp[r];
+        // a stamp left by another segment is valid only if it points back here
+        if (slot >= 0 && (slot >= len || itmp[b + slot] != r)) slot = -1;
+      }
+      int tot = 0;
+      const int rank = BlockRank(slot >= 0, wcount, &tot);
+      if (slot >= 0) {
+        tidx[b + base + rank] = r;
+        tval[b + base + rank] = vtmp[b + slot];
+      }
+      base += tot;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < len; i += blockDim.x) stamp[itmp[b + i]] = -1;
+    __syncthreads();
+  }
+}
+
+__global__ void ExtractDiagonalKernel(int ncols, const int* __restrict__ ptr, const int* __restrict__ idx,
+                                      const double* __restrict__ v, double* __restrict__ d) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncols) return;
+  const int q = FindSorted(idx, ptr[j], ptr[j + 1], j);
+  d[j] = q >= 0 ? v[q] : 0.0;
+}
+
+__global__ void FullDiagonalCheck(int ncols, const int* __restrict__ ptr, const int* __restrict__ idx,
+                                  int* __restrict__ bad) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncols) return;
+  if (ptr[j + 1] - ptr[j] != 1 || idx[ptr[j]] != j) atomicAdd(bad, 1);
+}
+
+__global__ void IdentityKernel(int n, int* __restrict__ ptr, int* __restrict__ idx, double* __restrict__ v) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j > n) return;
+  ptr[j] = j;
+  if (j < n) {
+    idx[j] = j;
+    v[j] = 1.0;
+  }
+}
+
+__global__ void Int64ToInt32(int64_t n, const int64_t* __restrict__ in, int64_t base, int* __restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = static_cast<int>(in[i] - base);
+}
+
+__global__ void Int32ToInt64(int64_t n, const int* __restrict__ in, int64_t* __restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = in[i];
+}
+
+__global__ void FillKernel(int64_t n, double* __restrict__ p, double v) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__device__ __forceinline__ double SafeQuotientDev(double num, double den) {
+  if (!(isfinite(den) && fabs(den) > 0.0)) return 0.0;
+  const double q = num / den;
+  return isfinite(q) ? q : 0.0;
+}
+
+__global__ void InnerOuterStepsKernel(int64_t n, int lomr, int two_term, const double* __restrict__ r_az,
+                                      const double* __restrict__ az_sq, const double* __restrict__ az_aq,
+                                      const double* __restrict__ aq_sq, const double* __restrict__ r_aq,
+                                      double* __restrict__ delta, double* __restrict__ gamma,
+                                      double* __restrict__ normalization) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double d, g;
+  if (!two_term) {
+    d = SafeQuotientDev(r_az[j], az_sq[j]);
+    g = lomr ? 1.0 : 0.0;
+  } else {
+    SolveLeastSquares2x2(az_sq[j], az_aq[j], az_aq[j], aq_sq[j], r_az[j], r_aq[j], d, g);
+  }
+  delta[j] = d;
+  gamma[j] = g;
+  const double inv = d != 0.0 ? 1.0 / d : 1.0;
+  normalization[j] = isfinite(inv) ? inv : 1.0;
+}
+
+__global__ void SafeQuotientKernel(int64_t n, const double* __restrict__ num, const double* __restrict__ den,
+                                   double* __restrict__ out) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j < n) out[j] = SafeQuotientDev(num[j], den[j]);
+}
+
+}  // namespace
+
+// ====================================================================== host
+
+void FillValue(Ctx& ctx, double* p, int64_t n, double v) {
+  if (n > 0) FillKernel<<<Blocks(n), kBlock, 0, ctx.stream>>>(n, p, v);
+}
+
+void InnerOuterSteps(Ctx& ctx, int64_t n, bool lomr, bool two_term, const double* r_az, const double* az_sq,
+                     const double* az_aq, const double* aq_sq, const double* r_aq, double* delta, double* gamma,
+                     double* normalization) {
+  if (n > 0)
+    InnerOuterStepsKernel<<<Blocks(n), kBlock, 0, ctx.stream>>>(n, lomr ? 1 : 0, two_term ? 1 : 0, r_az, az_sq,
+                                                                az_aq, aq_sq, r_aq, delta, gamma, normalization);
+}
+
+void SafeQuotients(Ctx& ctx, int64_t n, const double* num, const double* den, double* out) {
+  if (n > 0) SafeQuotientKernel<<<Blocks(n), kBlock, 0, ctx.stream>>>(n, num, den, out);
+}
+
+Ctx::Ctx() {
+  SPAI_HIP_CHECK(hipGetDevice(&device));
+  hipDeviceProp_t prop;
+  SPAI_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+  num_cus = prop.multiProcessorCount;
+  SPAI_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  pinned_bytes = 1 << 16;
+  SPAI_HIP_CHECK(hipHostMalloc(&pinned, pinned_bytes, hipHostMallocDefault));
+}
+
+Ctx::~Ctx() {
+  if (stream) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipStreamDestroy(stream);
+  }
+  if (pinned) (void)hipHostFree(pinned);
+}
+
+void Ctx::ReadBack(void* host, const void* dev, size_t bytes) {
+  if (bytes <= pinned_bytes) {
+    SPAI_HIP_CHECK(hipMemcpyAsync(pinned, dev, bytes, hipMemcpyDeviceToHost, stream));
+    SPAI_HIP_CHECK(hipStreamSynchronize(stream));
+    std::memcpy(host, pinned, bytes);
+  } else {
+    SPAI_HIP_CHECK(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, stream));
+    SPAI_HIP_CHECK(hipStreamSynchronize(stream));
+  }
+}
+
+void Ctx::Sync() { SPAI_HIP_CHECK(hipStreamSynchronize(stream)); }
+
+PatternPtr MakePattern(Ctx& ctx, int64_t rows, int64_t cols, int64_t nnz) {
+  if (nnz > kMaxDeviceNnz) throw std::runtime_error("matrix exceeds 2^31-1 stored entries on the device");
+  if (rows > kMaxDeviceNnz || cols > kMaxDeviceNnz) throw std::runtime_error("matrix dimension exceeds int32");
+  auto p = std::make_shared<Pattern>();
+  p->rows = rows;
+  p->cols = cols;
+  p->nnz = nnz;
+  p->ptr = ctx.Alloc<int>(cols + 1);
+  p->idx = ctx.Alloc<int>(nnz > 0 ? nnz : 1);
+  return p;
+}
+
+int64_t ScanCounts(Ctx& ctx, const int* counts, int64_t n, int* ptr) {
+  const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+  auto sums = ctx.Alloc<int64_t>(tiles > 0 ? tiles : 1);
+  auto total = ctx.Alloc<int64_t>(1);
+  if (n > 0) {
+    ScanTileSums<<<Blocks(n, kScanTile), kBlock, 0, ctx.stream>>>(counts, n, sums.get());
+    ScanTiles<<<1, 1024, 0, ctx.stream>>>(sums.get(), tiles, total.get());
+    ScanApply<<<Blocks(n, kScanTile), kBlock, 0, ctx.stream>>>(counts, n, sums.get(), ptr);
+  } else {
+    SPAI_HIP_CHECK(hipMemsetAsync(total.get(), 0, sizeof(int64_t), ctx.stream));
+  }
+  WriteLast<<<1, 1, 0, ctx.stream>>>(ptr, n, total.get());
+  int64_t t = 0;
+  ctx.ReadBack(&t, total.get(), sizeof(t));
+  if (t > kMaxDeviceNnz) throw std::runtime_error("result exceeds 2^31-1 stored entries on the device");
+  return t;
+}
+
+DMat Upload(Ctx& ctx, int64_t rows, int64_t cols, const int64_t* ptr, const int64_t* idx, const double* val) {
+  const int64_t base = ptr[0];
+  const int64_t nnz = ptr[cols] - base;
+  DMat m;
+  m.pat = MakePattern(ctx, rows, cols, nnz);
+  m.val = ctx.Alloc<double>(nnz > 0 ? nnz : 1);
+  auto tmp = ctx.Alloc<int64_t>(std::max<int64_t>(cols + 1, nnz) + 1);
+  SPAI_HIP_CHECK(hipMemcpyAsync(tmp.get(), ptr, sizeof(int64_t) * (cols + 1), hipMemcpyHostToDevice, ctx.stream));
+  Int64ToInt32<<<Blocks(cols + 1), kBlock, 0, ctx.stream>>>(cols + 1, tmp.get(), base, m.pat->ptr.get());
+  if (nnz > 0) {
+    auto tmp2 = ctx.Alloc<int64_t>(nnz);
+    SPAI_HIP_CHECK(hipMemcpyAsync(tmp2.get(), idx + base, sizeof(int64_t) * nnz, hipMemcpyHostToDevice, ctx.stream));
+    Int64ToInt32<<<Blocks(nnz), kBlock, 0, ctx.stream>>>(nnz, tmp2.get(), 0, m.pat->idx.get());
+    SPAI_HIP_CHECK(hipMemcpyAsync(m.val.get(), val + base, sizeof(double) * nnz, hipMemcpyHostToDevice, ctx.stream));
+    ctx.Sync();  // host buffers may be released by the caller after return
+  }
+  ctx.Sync();
+  return m;
+}
+
+void Download(Ctx& ctx, const DMat& m, int64_t* ptr, int64_t* idx, double* val) {
+  const int64_t cols = m.cols(), nnz = m.nnz();
+  auto t = ctx.Alloc<int64_t>(std::max<int64_t>(cols + 1, nnz));
+  Int32ToInt64<<<Blocks(cols + 1), kBlock, 0, ctx.stream>>>(cols + 1, m.ptr(), t.get());
+  SPAI_HIP_CHECK(hipMemcpyAsync(ptr, t.get(), sizeof(int64_t) * (cols + 1), hipMemcpyDeviceToHost, ctx.stream));
+  ctx.Sync();
+  if (nnz > 0) {
+    Int32ToInt64<<<Blocks(nnz), kBlock, 0, ctx.stream>>>(nnz, m.idx(), t.get());
+    SPAI_HIP_CHECK(hipMemcpyAsync(idx, t.get(), sizeof(int64_t) * nnz, hipMemcpyDeviceToHost, ctx.stream));
+    SPAI_HIP_CHECK(hipMemcpyAsync(val, m.v(), sizeof(double) * nnz, hipMemcpyDeviceToHost, ctx.stream));
+  }
+  ctx.Sync();
+}
+
+DMat EmptyMatrix(Ctx& ctx, int64_t rows, int64_t cols) {
+  DMat m;
+  m.pat = MakePattern(ctx, rows, cols, 0);
+  SPAI_HIP_CHECK(hipMemsetAsync(m.pat->ptr.get(), 0, sizeof(int) * (cols + 1), ctx.stream));
+  m.val = ctx.Alloc<double>(1);
+  return m;
+}
+
+DMat Identity(Ctx& ctx, int64_t n) {
+  DMat m;
+  m.pat = MakePattern(ctx, n, n, n);
+  m.val = ctx.Alloc<double>(n > 0 ? n : 1);
+  IdentityKernel<<<Blocks(n + 1), kBlock, 0, ctx.stream>>>((int)n, m.pat->ptr.get(), m.pat->idx.get(), m.val.get());
+  return m;
+}
+
+DMat Clone(Ctx& ctx, const DMat& m) {
+  DMat c;
+  c.pat = MakePattern(ctx, m.rows(), m.cols(), m.nnz());
+  SPAI_HIP_CHECK(hipMemcpyAsync(c.pat->ptr.get(), m.ptr(), sizeof(int) * (m.cols() + 1), hipMemcpyDeviceToDevice,
+                                ctx.stream));
+  if (m.nnz() > 0)
+    SPAI_HIP_CHECK(
+        hipMemcpyAsync(c.pat->idx.get(), m.idx(), sizeof(int) * m.nnz(), hipMemcpyDeviceToDevice, ctx.stream));
+  c.val = ctx.Alloc<double>(m.nnz() > 0 ? m.nnz() : 1);
+  if (m.nnz() > 0)
+    SPAI_HIP_CHECK(hipMemcpyAsync(c.val.get(), m.v(), sizeof(double) * m.nnz(), hipMemcpyDeviceToDevice, ctx.stream));
+  return c;
+}
+
+DMat ShareClone(Ctx& ctx, const DMat& m) {
+  DMat c;
+  c.pat = m.pat;
+  c.val = ctx.Alloc<double>(m.nnz() > 0 ? m.nnz() : 1);
+  if (m.nnz() > 0)
+    SPAI_HIP_CHECK(hipMemcpyAsync(c.val.get(), m.v(), sizeof(double) * m.nnz(), hipMemcpyDeviceToDevice, ctx.stream));
+  return c;
+}
+
+namespace {
+
+// New matrix made of the entries of X whose keep flag is set.
+DMat Filter(Ctx& ctx, const DMat& X, const unsigned char* keep) {
+  const int64_t n = X.cols();
+  auto counts = ctx.Alloc<int>(n > 0 ? n : 1);
+  CountKept<<<Blocks(n), kBlock, 0, ctx.stream>>>((int)n, X.ptr(), keep, counts.get());
+  DMat out;
+  out.pat = MakePattern(ctx, X.rows(), n, 0);
+  const int64_t total = ScanCounts(ctx, counts.get(), n, out.pat->ptr.get());
+  out.pat->nnz = total;
+  out.pat->idx = ctx.Alloc<int>(total > 0 ? total : 1);
+  out.val = ctx.Alloc<double>(total > 0 ? total : 1);
+  FillKept<<<Blocks(n), kBlock, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), X.v(), keep, out.pat->ptr.get(),
+                                                 out.pat->idx.get(), out.val.get());
+  return out;
+}
+
+}  // namespace
+
+DMat PruneZeros(Ctx& ctx, DMat&& X) {
+  const int64_t nnz = X.nnz();
+  if (nnz == 0) return std::move(X);
+  auto keep = ctx.Alloc<unsigned char>(nnz);
+  auto dropped = ctx.Alloc<int>(1);
+  SPAI_HIP_CHECK(hipMemsetAsync(dropped.get(), 0, sizeof(int), ctx.stream));
+  FlagNonZero<<<Blocks(nnz), kBlock, 0, ctx.stream>>>(nnz, X.v(), keep.get(), dropped.get());
+  int d = 0;
+  ctx.ReadBack(&d, dropped.get(), sizeof(int));
+  if (d == 0) return std::move(X);
+  return Filter(ctx, X, keep.get());
+}
+
+DMat SpGemm(Ctx& ctx, const DMat& X, const DMat& Y, bool prune_zero, bool residual) {
+  if (X.cols() != Y.rows()) throw std::invalid_argument("Sparse matrix dimensions do not match");
+  const int64_t n = Y.cols();
+  const int64_t nrows = X.rows();
+  if (residual && nrows != n) throw std::invalid_argument("residual product needs a square result");
+  auto upper = ctx.Alloc<int64_t>(n > 0 ? n : 1);
+  auto big_list = ctx.Alloc<int>(n > 0 ? n : 1);
+  auto big_count = ctx.Alloc<int>(1);
+  SPAI_HIP_CHECK(hipMemsetAsync(big_count.get(), 0, sizeof(int), ctx.stream));
+  ExpansionSizes<<<Blocks(n), kBlock, 0, ctx.stream>>>((int)n, X.ptr(), Y.ptr(), Y.idx(), residual ? 1 : 0,
+                                                       upper.get(), big_list.get(), big_count.get());
+  int nbig = 0;
+  ctx.ReadBack(&nbig, big_count.get(), sizeof(int));
+
+  auto counts = ctx.Alloc<int>(n > 0 ? n : 1);
+  DMat C;
+  C.pat = MakePattern(ctx, nrows, n, 0);
+  const unsigned sym_blocks = Blocks(n, kSymWaves);
+  SymbolicEsc<<<sym_blocks, kSymWaves * kWave, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), Y.ptr(), Y.idx(),
+                                                                upper.get(), residual ? 1 : 0, 0, counts.get(),
+                                                                nullptr, nullptr);
+  DBuf<int> stamp;
+  unsigned stamp_blocks = 0;
+  if (nbig > 0) {
+    const int64_t max_ws = (int64_t(1) << 28) / std::max<int64_t>(nrows, 1);  // <= 1 GiB of stamps
+    stamp_blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>({(int64_t)nbig, max_ws, 1024}));
+    stamp = ctx.Alloc<int>(size_t(stamp_blocks) * size_t(nrows));
+    SPAI_HIP_CHECK(hipMemsetAsync(stamp.get(), 0xFF, sizeof(int) * size_t(stamp_blocks) * size_t(nrows), ctx.stream));
+    SymbolicStamp<<<stamp_blocks, kBlock, 0, ctx.stream>>>(nbig, big_list.get(), (int)nrows, X.ptr(), X.idx(),
+                                                           Y.ptr(), Y.idx(), residual ? 1 : 0, 0, stamp.get(),
+                                                           counts.get(), nullptr, nullptr);
+  }
+  const int64_t total = ScanCounts(ctx, counts.get(), n, C.pat->ptr.get());
+  C.pat->nnz = total;
+  C.pat->idx = ctx.Alloc<int>(total > 0 ? total : 1);
+  C.val = ctx.Alloc<double>(total > 0 ? total : 1);
+  SymbolicEsc<<<sym_blocks, kSymWaves * kWave, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), Y.ptr(), Y.idx(),
+                                                                upper.get(), residual ? 1 : 0, 1, nullptr,
+                                                                C.pat->ptr.get(), C.pat->idx.get());
+  if (nbig > 0) {
+    SPAI_HIP_CHECK(hipMemsetAsync(stamp.get(), 0xFF, sizeof(int) * size_t(stamp_blocks) * size_t(nrows), ctx.stream));
+    SymbolicStamp<<<stamp_blocks, kBlock, 0, ctx.stream>>>(nbig, big_list.get(), (int)nrows, X.ptr(), X.idx(),
+                                                           Y.ptr(), Y.idx(), residual ? 1 : 0, 1, stamp.get(),
+                                                           nullptr, C.pat->ptr.get(), C.pat->idx.get());
+  }
+  auto err = ctx.Alloc<int>(1);
+  SPAI_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), ctx.stream));
+  NumericGustavson<<<Blocks(n, kNumWaves), kNumWaves * kWave, 0, ctx.stream>>>(
+      (int)n, X.ptr(), X.idx(), X.v(), Y.ptr(), Y.idx(), Y.v(), C.pat->ptr.get(), C.pat->idx.get(), C.val.get(),
+      residual ? 1 : 0, err.get());
+  int e = 0;
+  ctx.ReadBack(&e, err.get(), sizeof(int));
+  if (e) throw std::runtime_error("SpGEMM numeric phase found a row outside the symbolic pattern");
+  if (prune_zero) return PruneZeros(ctx, std::move(C));
+  return C;
+}
+
+DMat DiagScale(Ctx& ctx, const double* d, const DMat& Y, bool prune_zero) {
+  DMat C;
+  C.pat = Y.pat;
+  C.val = ctx.Alloc<double>(Y.nnz() > 0 ? Y.nnz() : 1);
+  if (Y.cols() > 0 && Y.nnz() > 0)
+    DiagScaleKernel<<<(unsigned)Y.cols(), 64, 0, ctx.stream>>>((int)Y.cols(), Y.ptr(), Y.idx(), d, Y.v(), C.val.get());
+  if (prune_zero) return PruneZeros(ctx, std::move(C));
+  return C;
+}
+
+DMat ColScale(Ctx& ctx, const DMat& X, const double* f, bool prune_zero) {
+  DMat C;
+  C.pat = X.pat;
+  C.val = ctx.Alloc<double>(X.nnz() > 0 ? X.nnz() : 1);
+  if (X.cols() > 0 && X.nnz() > 0)
+    ColScaleKernel<<<(unsigned)X.cols(), 64, 0, ctx.stream>>>((int)X.cols(), X.ptr(), f, X.v(), C.val.get());
+  if (prune_zero) return PruneZeros(ctx, std::move(C));
+  return C;
+}
+
+DMat Merge(Ctx& ctx, const DMat& L, const DMat& R, MergeOp op, double a, double b, bool prune_zero) {
+  if (L.rows() != R.rows() || L.cols() != R.cols()) throw std::invalid_argument("Sparse matrices must have the same shape");
+  const int64_t n = L.cols();
+  auto counts = ctx.Alloc<int>(n > 0 ? n : 1);
+  MergeKernel<<<Blocks(n), kBlock, 0, ctx.stream>>>((int)n, L.ptr(), L.idx(), L.v(), R.ptr(), R.idx(), R.v(),
+                                                    (int)op, a, b, prune_zero ? 1 : 0, 0, counts.get(), nullptr,
+                                                    nullptr, nullptr);
+  DMat C;
+  C.pat = MakePattern(ctx, L.rows(), n, 0);
+  const int64_t total = ScanCounts(ctx, counts.get(), n, C.pat->ptr.get());
+  C.pat->nnz = total;
+  C.pat->idx = ctx.Alloc<int>(total > 0 ? total : 1);
+  C.val = ctx.Alloc<double>(total > 0 ? total : 1);
+  MergeKernel<<<Blocks(n), kBlock, 0, ctx.stream>>>((int)n, L.ptr(), L.idx(), L.v(), R.ptr(), R.idx(), R.v(),
+                                                    (int)op, a, b, prune_zero ? 1 : 0, 1, nullptr, C.pat->ptr.get(),
+                                                    C.pat->idx.get(), C.val.get());
+  return C;
+}
+
+DMat Transpose(Ctx& ctx, const DMat& X) {
+  const int64_t rows = X.rows(), cols = X.cols(), nnz = X.nnz();
+  auto counts = ctx.Alloc<int>(rows > 0 ? rows : 1);
+  SPAI_HIP_CHECK(hipMemsetAsync(counts.get(), 0, sizeof(int) * rows, ctx.stream));
+  if (nnz > 0) RowCounts<<<Blocks(nnz), kBlock, 0, ctx.stream>>>(nnz, X.idx(), counts.get());
+  DMat T;
+  T.pat = MakePattern(ctx, cols, rows, nnz);
+  T.val = ctx.Alloc<double>(nnz > 0 ? nnz : 1);
+  ScanCounts(ctx, counts.get(), rows, T.pat->ptr.get());
+  if (nnz == 0) return T;
+  SPAI_HIP_CHECK(hipMemsetAsync(counts.get(), 0, sizeof(int) * rows, ctx.stream));
+  ScatterTranspose<<<(unsigned)cols, 64, 0, ctx.stream>>>((int)cols, X.ptr(), X.idx(), X.v(), T.pat->ptr.get(),
+                                                          counts.get(), T.pat->idx.get(), T.val.get());
+  auto lists = ctx.Alloc<int>(2 * rows + 2);
+  int* long_list = lists.get();
+  int* mid_list = lists.get() + rows;
+  int* cnts = lists.get() + 2 * rows;
+  SPAI_HIP_CHECK(hipMemsetAsync(cnts, 0, 2 * sizeof(int), ctx.stream));
+  SortSegmentsSmall<<<Blocks(rows), kBlock, 0, ctx.stream>>>((int)rows, T.pat->ptr.get(), T.pat->idx.get(),
+                                                             T.val.get(), long_list, cnts, mid_list, cnts + 1);
+  int nl[2] = {0, 0};
+  ctx.ReadBack(nl, cnts, sizeof(nl));
+  auto vtmp = ctx.Alloc<double>(nnz);
+  if (nl[1] > 0) {
+    const unsigned g = (unsigned)std::min(nl[1], 65535);
+    SortSegmentsWave<<<g, kWave, 0, ctx.stream>>>(nl[1], mid_list, T.pat->ptr.get(), T.pat->idx.get(), T.val.get(),
+                                                  vtmp.get());
+  }
+  if (nl[0] > 0) {
+    const int64_t max_ws = (int64_t(1) << 28) / std::max<int64_t>(cols, 1);
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>({(int64_t)nl[0], max_ws, 1024}));
+    auto ws = ctx.Alloc<int>(size_t(g) * size_t(cols));
+    auto itmp = ctx.Alloc<int>(nnz);
+    SPAI_HIP_CHECK(hipMemsetAsync(ws.get(), 0xFF, sizeof(int) * size_t(g) * size_t(cols), ctx.stream));
+    SortSegmentsStamp<<<g, kBlock, 0, ctx.stream>>>(nl[0], long_list, T.pat->ptr.get(), T.pat->idx.get(),
+                                                    T.val.get(), (int)cols, ws.get(), itmp.get(), vtmp.get());
+  }
+  return T;
+}
+
+namespace {
+double SumDevice(Ctx& ctx, const double* v, int64_t n) {
+  const unsigned g = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (n + kBlock - 1) / kBlock), 1024);
+  auto part = ctx.Alloc<double>(g);
+  SumPartials<<<g, kBlock, 0, ctx.stream>>>(v, n, part.get());
+  auto fin = ctx.Alloc<double>(1);
+  SumPartials<<<1, kBlock, 0, ctx.stream>>>(part.get(), g, fin.get());
+  double s = 0.0;
+  ctx.ReadBack(&s, fin.get(), sizeof(double));
+  return s;
+}
+}  // namespace
+
+void ColumnDots(Ctx& ctx, const DMat& X, const DMat& Y, double* out) {
+  const int64_t n = X.cols();
+  ColumnDotKernel<<<Blocks(n), kBlock, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), X.v(), Y.ptr(), Y.idx(), Y.v(), 0,
+                                                        out);
+}
+
+void ColumnSqNorms(Ctx& ctx, const DMat& X, double* out) {
+  const int64_t n = X.cols();
+  ColumnDotKernel<<<Blocks(n), kBlock, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), X.v(), X.ptr(), X.idx(), X.v(), 1,
+                                                        out);
+}
+
+double FrobDot(Ctx& ctx, const DMat& X, const DMat& Y) {
+  if (X.rows() != Y.rows() || X.cols() != Y.cols()) throw std::invalid_argument("Sparse matrices must have the same shape");
+  const int64_t n = X.cols();
+  auto cd = ctx.Alloc<double>(n > 0 ? n : 1);
+  ColumnDots(ctx, X, Y, cd.get());
+  return SumDevice(ctx, cd.get(), n);
+}
+
+double SqNorm(Ctx& ctx, const DMat& X) {
+  const int64_t n = X.cols();
+  auto cd = ctx.Alloc<double>(n > 0 ? n : 1);
+  ColumnSqNorms(ctx, X, cd.get());
+  return SumDevice(ctx, cd.get(), n);
+}
+
+void BatchSqNorms(Ctx& ctx, const DMat& X, const int* batch_first, int nbatches, double* out) {
+  BatchSqNormKernel<<<Blocks(nbatches), kBlock, 0, ctx.stream>>>(nbatches, batch_first, X.ptr(), X.v(), out);
+}
+
+DMat ThresholdColumns(Ctx& ctx, const DMat& M, double eps) {
+  const int64_t n = M.cols(), nnz = M.nnz();
+  auto keep = ctx.Alloc<unsigned char>(nnz > 0 ? nnz : 1);
+  ThresholdColumnsKernel<<<Blocks(n), kBlock, 0, ctx.stream>>>((int)n, M.ptr(), M.idx(), M.v(), eps, keep.get());
+  return Filter(ctx, M, keep.get());
+}
+
+DMat ThresholdBatches(Ctx& ctx, const DMat& X, const double* batch_sq, const int* batch_of_col, double eps) {
+  const int64_t n = X.cols(), nnz = X.nnz();
+  auto keep = ctx.Alloc<unsigned char>(nnz > 0 ? nnz : 1);
+  if (n > 0)
+    ThresholdBatchesKernel<<<(unsigned)n, 64, 0, ctx.stream>>>(nnz, (int)n, X.ptr(), X.v(), batch_sq, batch_of_col,
+                                                              eps, keep.get());
+  return Filter(ctx, X, keep.get());
+}
+
+DMat ColumnTopK(Ctx& ctx, const DMat& X, int64_t k) {
+  const int64_t n = X.cols(), nnz = X.nnz();
+  auto keep = ctx.Alloc<unsigned char>(nnz > 0 ? nnz : 1);
+  if (n > 0) TopKKernel<<<(unsigned)n, 64, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), X.v(), k, keep.get());
+  return Filter(ctx, X, keep.get());
+}
+
+namespace {
+
+// Drops the `drop` smallest candidates under (key, position).
+DMat DropByKeys(Ctx& ctx, const DMat& X, const uint64_t* keys, int64_t drop) {
+  const int64_t nnz = X.nnz();
+  auto state = ctx.Alloc<uint64_t>(3);
+  const uint64_t init[3] = {0, 0, (uint64_t)drop};
+  SPAI_HIP_CHECK(hipMemcpyAsync(state.get(), init, sizeof(init), hipMemcpyHostToDevice, ctx.stream));
+  auto hist = ctx.Alloc<unsigned>(256);
+  SPAI_HIP_CHECK(hipMemsetAsync(hist.get(), 0, 256 * sizeof(unsigned), ctx.stream));
+  const unsigned g = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (nnz + kBlock - 1) / kBlock), 2048);
+  for (int shift = 56; shift >= 0; shift -= 8) {
+    RadixHistogram<<<g, kBlock, 0, ctx.stream>>>(nnz, keys, state.get(), shift, hist.get());
+    RadixPick<<<1, 64, 0, ctx.stream>>>(state.get(), shift, hist.get());
+  }
+  auto flags = ctx.Alloc<int>(nnz);
+  auto scan = ctx.Alloc<int>(nnz + 1);
+  EqualFlags<<<Blocks(nnz), kBlock, 0, ctx.stream>>>(nnz, keys, state.get(), flags.get());
+  ScanCounts(ctx, flags.get(), nnz, scan.get());
+  auto pos = ctx.Alloc<int64_t>(1);
+  SPAI_HIP_CHECK(hipMemsetAsync(pos.get(), 0xFF, sizeof(int64_t), ctx.stream));
+  FindTiePosition<<<Blocks(nnz), kBlock, 0, ctx.stream>>>(nnz, flags.get(), scan.get(), state.get(), pos.get());
+  auto keep = ctx.Alloc<unsigned char>(nnz);
+  KeepAboveCutoff<<<Blocks(nnz), kBlock, 0, ctx.stream>>>(nnz, keys, state.get(), pos.get(), keep.get());
+  return Filter(ctx, X, keep.get());
+}
+
+}  // namespace
+
+DMat DropSmallestByScore(Ctx& ctx, const DMat& M, const DMat& AR, const double* a_col_sq, int64_t drop) {
+  const int64_t n = M.cols(), nnz = M.nnz();
+  auto keys = ctx.Alloc<uint64_t>(nnz > 0 ? nnz : 1);
+  if (n > 0)
+    CandidateKeys<<<(unsigned)n, 64, 0, ctx.stream>>>((int)n, M.ptr(), M.idx(), M.v(), 0, AR.ptr(), AR.idx(), AR.v(),
+                                                      a_col_sq, keys.get());
+  return DropByKeys(ctx, M, keys.get(), drop);
+}
+
+DMat DropSmallestByAbs(Ctx& ctx, const DMat& X, int64_t drop) {
+  const int64_t n = X.cols(), nnz = X.nnz();
+  auto keys = ctx.Alloc<uint64_t>(nnz > 0 ? nnz : 1);
+  if (n > 0)
+    CandidateKeys<<<(unsigned)n, 64, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), X.v(), 1, nullptr, nullptr, nullptr,
+                                                      nullptr, keys.get());
+  return DropByKeys(ctx, X, keys.get(), drop);
+}
+
+void ExtractDiagonal(Ctx& ctx, const DMat& A, double* d) {
+  const int64_t n = A.cols();
+  ExtractDiagonalKernel<<<Blocks(n), kBlock, 0, ctx.stream>>>((int)n, A.ptr(), A.idx(), A.v(), d);
+}
+
+bool IsFullDiagonal(Ctx& ctx, const DMat& X) {
+  if (X.rows() != X.cols() || X.nnz() != X.cols()) return false;
+  auto bad = ctx.Alloc<int>(1);
+  SPAI_HIP_CHECK(hipMemsetAsync(bad.get(), 0, sizeof(int), ctx.stream));
+  FullDiagonalCheck<<<Blocks(X.cols()), kBlock, 0, ctx.stream>>>((int)X.cols(), X.ptr(), X.idx(), bad.get());
+  int b = 0;
+  ctx.ReadBack(&b, bad.get(), sizeof(int));
+  return b == 0;
+}
+
+}  // namespace spai

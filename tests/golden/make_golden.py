@@ -1,0 +1,49 @@
+"""Writes the golden fixtures of tests/golden/ from the reference's committed
+data files (run in a container where the reference tree is mounted).
+
+  * <name>.npz: the sample matrix (scripts/sample_datasets/<name>-dataset.mtx),
+    canonicalised as scripts/common.py:396-400 does (CSC, duplicates summed,
+    zeros eliminated, indices sorted).
+  * <name>-<mode>-d<density>-convergence.csv: the first rows of the reference's
+    per-iteration ||R_i||_F table for the seven method variants
+    (scripts/results/results_sample/<mode>/max_iter_*/max_density_*/<name>/*-convergence.csv).
+"""
+import csv
+import glob
+import sys
+from pathlib import Path
+
+import numpy as np
+from scipy import sparse
+from scipy.io import mmread
+
+REF = Path("/root/reference/scripts")
+OUT = Path(__file__).resolve().parent
+ROWS = 41
+
+DATASETS = ["tri100eigs4k", "bcsstk21"]
+
+
+def main():
+    for name in DATASETS:
+        A = mmread(REF / "sample_datasets" / f"{name}-dataset.mtx").tocsc().astype(np.float64)
+        A.sum_duplicates()
+        A.eliminate_zeros()
+        A.sort_indices()
+        np.savez_compressed(OUT / f"{name}.npz", indptr=A.indptr.astype(np.int64),
+                            indices=A.indices.astype(np.int64), data=A.data, shape=np.array(A.shape))
+        for f in sorted(glob.glob(str(REF / f"results/results_sample/*/max_iter_*/max_density_*/{name}/*-convergence.csv"))):
+            if "no_dropping" in f and "max_density_0.030" in f:
+                continue  # written by an earlier revision (see tests/test_oracle_golden.py)
+            p = Path(f)
+            mode = p.parts[-5]
+            dens = p.parts[-3].split("_")[-1]
+            with open(f) as fh:
+                rows = list(csv.reader(fh))
+            with open(OUT / f"{name}-{mode}-d{dens}-convergence.csv", "w", newline="") as fh:
+                csv.writer(fh).writerows(rows[:ROWS + 1])
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

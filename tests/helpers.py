@@ -1,0 +1,31 @@
+"""Shared comparisons for the parity tests."""
+import numpy as np
+from scipy import sparse
+
+
+def tri(n):
+    """tests/test.cpp:13-28 fixture: tridiag(-1, 2, -1)."""
+    return sparse.diags([-np.ones(n - 1), 2 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1], format="csc")
+
+
+def rel_fro(a, b):
+    d = sparse.linalg.norm(sparse.csc_matrix(a) - sparse.csc_matrix(b)) if sparse.issparse(a) else np.linalg.norm(a - b)
+    nb = sparse.linalg.norm(b) if sparse.issparse(b) else np.linalg.norm(b)
+    return d / nb if nb > 0 else d
+
+
+def assert_same_result(g, o, hist_rtol=1e-10, m_rtol=1e-12, same_pattern=True):
+    """Device result g against oracle result o (dicts from _lib.run / oracle.run)."""
+    assert g["iterations"] == o["iterations"], (g["iterations"], o["iterations"])
+    assert g["converged"] == o["converged"]
+    hg, ho = g["history"], o["history"]
+    assert len(hg) == len(ho)
+    np.testing.assert_array_equal(hg["iteration"], ho["iteration"])
+    np.testing.assert_allclose(hg["residual_norm"], ho["residual_norm"], rtol=hist_rtol, atol=1e-300)
+    if same_pattern:
+        np.testing.assert_array_equal(hg["density_m"], ho["density_m"])
+        np.testing.assert_array_equal(hg["density_direction"], ho["density_direction"])
+        Mg, Mo = g["M"], o["M"]
+        np.testing.assert_array_equal(Mg.indptr, Mo.indptr)
+        np.testing.assert_array_equal(Mg.indices, Mo.indices)
+    assert rel_fro(g["M"], o["M"]) <= m_rtol, rel_fro(g["M"], o["M"])
