@@ -1,0 +1,158 @@
+"""Benchmark: Global SPAI LOMR + PCG time-to-1e-8 on the 64^3 heterogeneous-diffusion problem.
+
+Workload (one "step" = one full pass, operands already resident in HBM):
+  A: 3D 7-point variable-coefficient diffusion on 64^3 (n=262,144, nnz=1,810,432),
+     face coefficients = harmonic mean of exp(g), g ~ N(0,1), seed 1 (synthetic).
+  build: Global SPAI LOMR, M on pattern(A), M0 = diag(A)^-1, 30 iterations.
+  solve: PCG with b = A x_true, x_true ~ N(0,1) seed 2, x0 = 0, tolerance 1e-8.
+The timed form is FP32-bulk (M, directions, residual and products in FP32;
+step sizes, trace products, ||R||_F and the whole PCG recurrence in FP64); the
+plain FP64 form is timed the same way and reported beside it.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "SPAI+PCG time-to-1e-8 (s), FP32-bulk vs FP64 on GPU; SpGEMM HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+ITERATIONS = 30
+TOL = 1e-8
+PCG_LIMIT = 20000
+
+
+def problem():
+    from parallellocallyoptimalapproximatesparsematrixinverses_amd import problems
+
+    A = problems.diffusion3d(64, seed=1)
+    Pr = problems.diag_inverse(A)
+    x_true = np.random.default_rng(2).standard_normal(A.shape[0])
+    return A, Pr, A @ x_true
+
+
+def time_form(lib, A, M0, b, precision, steps, warmup):
+    s = lib.Session("lomr", A, M0=M0, max_iterations=ITERATIONS, precision=precision)
+    s.set_rhs(b)
+    out = {}
+    for _ in range(warmup):
+        s.build()
+        s.pcg(PCG_LIMIT, TOL)
+    t_build = t_pcg = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a = time.perf_counter()
+        it = s.build()
+        c = time.perf_counter()
+        st, cnt = s.pcg(PCG_LIMIT, TOL)
+        d = time.perf_counter()
+        t_build += c - a
+        t_pcg += d - c
+    total = time.perf_counter() - t0
+    hist, x = s.pcg_output(cnt)
+    out.update(seconds=total / steps, build_s=t_build / steps, pcg_s=t_pcg / steps, spai_iterations=it,
+               pcg_iterations=cnt - 1, status=st,
+               true_rel_residual=float(np.linalg.norm(b - A @ x) / np.linalg.norm(b)))
+    out["M"] = s.result()["M"]
+    out["spmv_a_us"] = s.spmv_a_us()
+    s.close()
+    return out
+
+
+def cpu_baseline(A, M0, b, threads):
+    """Oracle (C++/OpenMP restatement of the reference) on the host cores: FP64
+    fixed-pattern LOMR at 2 and 6 iterations (slope -> 30 iterations) plus the
+    full PCG with the 6-iteration M."""
+    from oracle import oracle
+
+    t = time.perf_counter()
+    oracle.run("lomr", A, M0, None, 2, 0.0, 1.0, False, threads, fixed_pattern=True, omp_threads=threads)
+    t2 = time.perf_counter() - t
+    t = time.perf_counter()
+    r6 = oracle.run("lomr", A, M0, None, 6, 0.0, 1.0, False, threads, fixed_pattern=True, omp_threads=threads)
+    t6 = time.perf_counter() - t
+    t = time.perf_counter()
+    oracle.pcg(A.tocsr(), r6["M"].tocsr(), b, PCG_LIMIT, TOL, threads=threads)
+    tp = time.perf_counter() - t
+    per_it = (t6 - t2) / 4.0
+    est = t2 + per_it * (ITERATIONS - 2) + tp
+    return {"value": round(est, 4), "unit": "s", "cores": threads, "kind": "port",
+            "sample": f"oracle LOMR FP64 at 2 and 6 iterations ({t2:.2f} s, {t6:.2f} s; slope extrapolated to "
+                      f"{ITERATIONS}) + full PCG ({tp:.2f} s), {threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    from parallellocallyoptimalapproximatesparsematrixinverses_amd import _lib
+
+    A, M0, b = problem()
+    n, nnz = A.shape[0], A.nnz
+    fp64 = time_form(_lib, A, M0, b, "fp64", args.steps, args.warmup)
+    fp32 = time_form(_lib, A, M0, b, "fp32", args.steps, args.warmup)
+    dM = float(np.linalg.norm((fp32["M"] - fp64["M"]).data) / np.linalg.norm(fp64["M"].data))
+
+    # roofline: the PCG SpMV with A (CSR, int32 indices, FP64 values), per launch:
+    # (n+1)*4 + nnz*(4+8) + 2*8*n bytes (read x, write y)
+    spmv_bytes = (n + 1) * 4 + nnz * 12 + 16 * n
+    us = fp32["spmv_a_us"]
+    achieved = spmv_bytes / (us * 1e-6) / 1e9
+    traffic = None
+    prof = ROOT / "profiles" / "pcg_spmv_a_diffusion64_traffic.json"
+    if prof.exists():
+        traffic = json.loads(prof.read_text()).get("bytes_per_launch")
+
+    threads = min(16, os.cpu_count() or 1)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(A, M0, b, threads)
+
+    line = {
+        "metric": METRIC,
+        "value": round(fp32["seconds"], 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(fp32["seconds"] * 1e3, 4),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32 bulk + f64 reductions",
+        "data": "synthetic",
+        "config": {"workload": "diffusion3d 64^3 (n=262144, nnz=1810432), Global SPAI LOMR on pattern(A), "
+                               "30 iterations, M0=diag(A)^-1, PCG to 1e-8, x_true~N(0,1) seed 2",
+                   "n": n, "nnz": nnz},
+        "fp32_bulk": {k: v for k, v in fp32.items() if k not in ("M",)},
+        "fp64": {k: v for k, v in fp64.items() if k not in ("M",)},
+        "speedup_fp32_vs_fp64": round(fp64["seconds"] / fp32["seconds"], 4),
+        "extra_pcg_iterations_fp32": fp32["pcg_iterations"] - fp64["pcg_iterations"],
+        "rel_fro_M32_vs_M64": dM,
+        "roofline": {"kernel": "PcgSpmvA (CSR SpMV with A + <p, Ap>)", "bound": "hbm",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": spmv_bytes, "avg_launch_us": round(us, 3)},
+        "cpu_baseline": cpu,
+        "device": _lib.device_name(),
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -44,7 +44,14 @@ struct SpaiExtParams {
    *    M and of the direction keeps its diagonal and its k largest |values|
    *    off the diagonal (ties kept by the smaller row) instead. */
   int64_t column_top_k;
-  int64_t reserved[5];
+  /* 0: trace inner products and norms are tree reductions (fast; results
+   *    differ from the reference in the last bits).
+   * 1: they are summed in the reference's order -- sequentially within each of
+   *    the num_threads column batches, then over the batches in order -- so
+   *    the reference-semantics builders reproduce its arithmetic bit for bit
+   *    (slower for few batches: one device thread per batch). */
+  int64_t batch_ordered_reductions;
+  int64_t reserved[4];
 };
 
 /* CSR matrix with int64 offsets (rows + 1) and column indices. */

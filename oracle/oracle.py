@@ -93,7 +93,8 @@ class SpaiExtParams(ctypes.Structure):
         ("fixed_pattern", ctypes.c_int64),
         ("precision", ctypes.c_int64),
         ("column_top_k", ctypes.c_int64),
-        ("reserved", ctypes.c_int64 * 5),
+        ("batch_ordered_reductions", ctypes.c_int64),
+        ("reserved", ctypes.c_int64 * 4),
     ]
 
 

@@ -52,7 +52,7 @@ class Result(ctypes.Structure):
 
 class SpaiExtParams(ctypes.Structure):
     _fields_ = [("fixed_pattern", ctypes.c_int64), ("precision", ctypes.c_int64), ("column_top_k", ctypes.c_int64),
-                ("reserved", ctypes.c_int64 * 5)]
+                ("batch_ordered_reductions", ctypes.c_int64), ("reserved", ctypes.c_int64 * 4)]
 
 
 class CsrData(ctypes.Structure):
@@ -185,7 +185,7 @@ def _params(max_iterations, tolerance, max_density, enable_dropping, num_threads
 
 def run(method: str, A, M0=None, Pr=None, max_iterations=100, tolerance=1e-9, max_density=0.03,
         enable_dropping=True, num_threads=1, inner_iterations=2, fixed_pattern=False, precision="fp64",
-        column_top_k=0, plain_entry=False):
+        column_top_k=0, batch_ordered=False, plain_entry=False):
     """Runs a builder on the device.  plain_entry=True calls the reference-named
     entry point (spai_global_* / inner_outer_*) instead of spai_run_ex."""
     L = load()
@@ -201,7 +201,8 @@ def run(method: str, A, M0=None, Pr=None, max_iterations=100, tolerance=1e-9, ma
                 "io_mr": "inner_outer_mr", "io_lomr": "inner_outer_lomr"}[method]
         res = getattr(L, name)(*args)
     else:
-        ext = SpaiExtParams(1 if fixed_pattern else 0, 1 if precision == "fp32" else 0, int(column_top_k))
+        ext = SpaiExtParams(1 if fixed_pattern else 0, 1 if precision == "fp32" else 0, int(column_top_k),
+                            1 if batch_ordered else 0)
         res = L.spai_run_ex(METHOD_IDS[method], *args, ext, ctypes.byref(seconds))
     out = result_to_python(res)
     out["device_seconds"] = seconds.value

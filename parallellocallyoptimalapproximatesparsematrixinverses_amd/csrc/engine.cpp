@@ -69,7 +69,17 @@ class State {
     return std::max<int64_t>(n, (int64_t)std::floor(m));
   }
 
-  double ResidualNorm() { return std::sqrt(SqNorm(ctx_, R)); }
+  // Trace inner products: tree reductions, or the reference's batch order.
+  double Dot(const DMat& X, const DMat& Y) {
+    if (prm_.batch_ordered) return BatchOrderedDot(ctx_, X, Y, batch_first.get(), (int)nb, false);
+    return FrobDot(ctx_, X, Y);
+  }
+  double Sq(const DMat& X) {
+    if (prm_.batch_ordered) return BatchOrderedDot(ctx_, X, X, batch_first.get(), (int)nb, true);
+    return SqNorm(ctx_, X);
+  }
+
+  double ResidualNorm() { return std::sqrt(Sq(R)); }
 
   // StabilizeResiduals (state_manager.cpp:81-95).
   void StabilizeResiduals() {
@@ -218,6 +228,9 @@ void Prepare(State& s, const MatrixData& Ah, const MatrixData* M0, const MatrixD
   s.initial_residual = s.ResidualNorm();
 }
 
+double Dot_(State& s, const DMat& X, const DMat& Y) { return s.Dot(X, Y); }
+double Sq_(State& s, const DMat& X) { return s.Sq(X); }
+
 // OutputResult::AppendIteration (src/common/result_operations.cpp:7-67).
 bool Append(EngineOutput& out, State& s, const DMat& direction) {
   const int64_t it = (int64_t)out.history.size();
@@ -233,12 +246,12 @@ bool Append(EngineOutput& out, State& s, const DMat& direction) {
 void RunCG(State& s, EngineOutput& out) {
   s.Z = s.MulPr(s.R);
   s.P = ShareClone(s.ctx_, s.Z);
-  double r_z = FrobDot(s.ctx_, s.R, s.Z);
+  double r_z = Dot_(s, s.R, s.Z);
   s.DropForMatrixType(s.P);
   if (Append(out, s, s.P)) return;
   s.AP = s.MulA(s.P);
   for (int64_t i = 1; i <= s.prm_.max_iterations; ++i) {
-    const double den = FrobDot(s.ctx_, s.P, s.AP);
+    const double den = Dot_(s, s.P, s.AP);
     if (!IsFiniteNonZero(r_z, 0.0) || !IsFiniteNonZero(den, 0.0)) break;
     const double alpha = r_z / den;
     if (!std::isfinite(alpha)) break;
@@ -250,7 +263,7 @@ void RunCG(State& s, EngineOutput& out) {
       s.R = s.AddScaled(s.R, s.AP, -alpha);
     if (Append(out, s, s.P)) break;
     s.Z = s.MulPr(s.R);
-    r_z = FrobDot(s.ctx_, s.R, s.Z);
+    r_z = Dot_(s, s.R, s.Z);
     const double beta = r_z / prev;
     if (!std::isfinite(beta)) break;
     s.P = s.LinearCombination(s.Z, 1.0, s.P, beta);
@@ -267,8 +280,8 @@ void RunMR(State& s, EngineOutput& out) {
   auto project = [&]() {
     s.AZ = s.MulA(s.Z);
     s.Tmp = s.MulPr(s.AZ);
-    num = FrobDot(s.ctx_, s.Z, s.Tmp);
-    den = SqNorm(s.ctx_, s.Tmp);
+    num = Dot_(s, s.Z, s.Tmp);
+    den = Sq_(s, s.Tmp);
   };
   project();
   for (int64_t i = 1; i <= s.prm_.max_iterations; ++i) {
@@ -294,14 +307,14 @@ void RunLOMR(State& s, EngineOutput& out) {
     s.AP = s.MulA(s.P);
     s.AZ = s.MulA(s.Z);
     s.Tmp = s.MulPr(s.AZ);
-    double az_praz = FrobDot(s.ctx_, s.AZ, s.Tmp);
-    double z_az = FrobDot(s.ctx_, s.Z, s.AZ);
+    double az_praz = Dot_(s, s.AZ, s.Tmp);
+    double z_az = Dot_(s, s.Z, s.AZ);
     double ap_prap = 0.0, az_prap = 0.0, z_ap = 0.0;
     if (i > 1) {
       s.Tmp = s.MulPr(s.AP);
-      ap_prap = FrobDot(s.ctx_, s.AP, s.Tmp);
-      az_prap = FrobDot(s.ctx_, s.AZ, s.Tmp);
-      z_ap = FrobDot(s.ctx_, s.Z, s.AP);
+      ap_prap = Dot_(s, s.AP, s.Tmp);
+      az_prap = Dot_(s, s.AZ, s.Tmp);
+      z_ap = Dot_(s, s.Z, s.AP);
     }
     // GetDeltaGamma (locally_minimal_residual.cpp:36-50)
     double delta = 0.0, gamma = 0.0;
@@ -418,6 +431,7 @@ EngineParams ToEngineParams(const MethodParams& p, const SpaiExtParams* ext) {
     e.fixed_pattern = ext->fixed_pattern != 0;
     e.fp32 = ext->precision != 0;
     e.top_k = ext->column_top_k;
+    e.batch_ordered = ext->batch_ordered_reductions != 0;
   }
   return e;
 }

@@ -20,6 +20,7 @@ struct EngineParams {
   bool fixed_pattern = false;
   bool fp32 = false;
   int64_t top_k = 0;
+  bool batch_ordered = false;
 };
 
 struct HostMatrix {

@@ -30,7 +30,7 @@ namespace spai {
 namespace {
 
 constexpr int kBlk = 256;
-constexpr int kMaxGrid = 1024;
+constexpr int kMaxGrid = 2048;
 
 // Scalar slots in device memory.
 enum Slot : int {
@@ -48,45 +48,63 @@ struct Flags {
   int iterations;  // history rows appended - 1
 };
 
+// (A Y)(i, j) = sum_k A^T(k, i) Y(k, j) over k ascending (the reference
+// product's order).  Row i of A (column i of A^T) is read in groups of 8;
+// each of its k is located in the sorted column Y(:, j) by an independent
+// binary search, so the 8 searches of a group overlap in flight; the products
+// are then added in k order.
+constexpr int kIlp = 8;
+
+template <class V, int NY>
+__device__ __forceinline__ void Gather(const int* __restrict__ ati, const V* __restrict__ atv, int ab, int ae,
+                                       const int* __restrict__ si, const V* __restrict__ y0,
+                                       const V* __restrict__ y1, int sb, int se, V& s0, V& s1) {
+  s0 = V(0);
+  s1 = V(0);
+  for (int t0 = ab; t0 < ae; t0 += kIlp) {
+    int pos[kIlp];
+    V a[kIlp];
+#pragma unroll
+    for (int u = 0; u < kIlp; ++u) {
+      const int t = t0 + u;
+      pos[u] = -1;
+      if (t < ae) {
+        const int k = ati[t];
+        a[u] = atv[t];
+        int lo = sb, hi = se;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (si[mid] < k)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        if (lo < se && si[lo] == k) pos[u] = lo;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kIlp; ++u) {
+      if (pos[u] >= 0) {
+        s0 += a[u] * y0[pos[u]];
+        if (NY > 1) s1 += a[u] * y1[pos[u]];
+      }
+    }
+  }
+}
+
 template <class V>
 __device__ __forceinline__ V InnerProduct(const int* __restrict__ ai, const V* __restrict__ av, int ab, int ae,
                                           const int* __restrict__ si, const V* __restrict__ y, int sb, int se) {
-  V s = V(0);
-  while (ab < ae && sb < se) {
-    const int ka = ai[ab], ks = si[sb];
-    if (ka == ks) {
-      s += av[ab] * y[sb];
-      ++ab;
-      ++sb;
-    } else if (ka < ks) {
-      ++ab;
-    } else {
-      ++sb;
-    }
-  }
-  return s;
+  V s0, s1;
+  Gather<V, 1>(ai, av, ab, ae, si, y, y, sb, se, s0, s1);
+  return s0;
 }
 
 template <class V>
 __device__ __forceinline__ void InnerProduct2(const int* __restrict__ ai, const V* __restrict__ av, int ab, int ae,
                                               const int* __restrict__ si, const V* __restrict__ y0,
                                               const V* __restrict__ y1, int sb, int se, V& s0, V& s1) {
-  s0 = V(0);
-  s1 = V(0);
-  while (ab < ae && sb < se) {
-    const int ka = ai[ab], ks = si[sb];
-    if (ka == ks) {
-      const V a = av[ab];
-      s0 += a * y0[sb];
-      s1 += a * y1[sb];
-      ++ab;
-      ++sb;
-    } else if (ka < ks) {
-      ++ab;
-    } else {
-      ++sb;
-    }
-  }
+  Gather<V, 2>(ai, av, ab, ae, si, y0, y1, sb, se, s0, s1);
 }
 
 template <int K>
@@ -591,7 +609,7 @@ class FixedImpl final : public FixedProblem {
     const int* sp = s_pat_->ptr.get();
     const int* si = s_pat_->idx.get();
     const double tol = prm_.tolerance;
-    const int G2 = (int)grid2_, G1 = (int)grid1_;
+    const int G2 = (int)grid2_;
     ResidualKernel<V><<<grid2_, kBlk, 0, st>>>(nnz_s2_, s2r, s2c, map, atp, ati, atv_.get(), sp, si, M_.get(),
                                                prv_.get(), R_.get(), Z_.get(), P_.get(),
                                                method_ == SPAI_GLOBAL_CG ? 1 : 0, part_.get(), flags_.get());
@@ -623,7 +641,6 @@ class FixedImpl final : public FixedProblem {
         CgAppendBeta<<<1, kBlk, 0, st>>>(part_.get(), G2, tol, hist_.get(), sc_.get(), flags_.get());
         CgDirection<V><<<grid1_, kBlk, 0, st>>>(nnz_s_, P_.get(), Z_.get(), sc_.get(), flags_.get());
       }
-      (void)G1;
     }
     SPAI_HIP_CHECK(hipGetLastError());
     Flags f;

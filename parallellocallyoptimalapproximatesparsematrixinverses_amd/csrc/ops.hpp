@@ -76,6 +76,10 @@ double SqNorm(Ctx& ctx, const DMat& X);
 // Per-column intersection dots / squared norms into out[0..cols) (device).
 void ColumnDots(Ctx& ctx, const DMat& X, const DMat& Y, double* out);
 void ColumnSqNorms(Ctx& ctx, const DMat& X, double* out);
+// Frobenius dot (or squared norm when same) summed as the reference does:
+// sequentially within each batch [first[b], first[b+1]) in CSC order, then over
+// the batches in order (methods_common.cpp:148-154).
+double BatchOrderedDot(Ctx& ctx, const DMat& X, const DMat& Y, const int* batch_first, int nbatches, bool same);
 // Per-batch sequential squared norms for the batch column ranges (device out).
 void BatchSqNorms(Ctx& ctx, const DMat& X, const int* batch_first, int nbatches, double* out);
 

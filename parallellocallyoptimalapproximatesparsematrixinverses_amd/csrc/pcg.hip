@@ -4,11 +4,14 @@
 // One iteration is five kernels: SpMV(A, p) + <p, Ap>; the x / r update +
 // <r, r>; the true-residual refresh (a no-op unless the recursive residual
 // crossed the tolerance, backward_error.pyx:116-122); SpMV(M, r) + <r, z>; and
-// the direction update.  Each reduction is finished by the last block of its
-// kernel (fixed-order sum of the block partials, so results are deterministic),
-// which also applies the reference's breakdown and stop rules to a device-side
-// state.  Iterations are replayed in chunks from a captured hipGraph; once the
-// state says done, the remaining kernels of a chunk return at once.
+// the direction update.  Each kernel writes one partial sum per block; the
+// blocks of the next kernel each finish that reduction themselves (the same
+// fixed-order sum in every block, so every block takes the same decision and
+// the results are deterministic), and block 0 records the outcome in a
+// device-side state read only by later kernels.  r.z and the iteration count
+// are double-buffered by iteration parity so no kernel reads a slot that the
+// same kernel writes.  Iterations are replayed in chunks from a captured
+// hipGraph; once the state says done, the remaining kernels return at once.
 
 #include "pcg.hpp"
 
@@ -25,11 +28,12 @@ namespace spai {
 namespace {
 
 constexpr int kBlk = 256;
-constexpr int kChunk = 32;  // iterations per captured graph
+constexpr int kChunk = 32;  // iterations per captured graph (even)
 
-enum DSlot : int { kRz = 0, kBnorm, kAlpha, kBeta, kErr, kTol, kNumD };
-enum ISlot : int { kCount = 0, kStatus, kDone, kRefreshed, kMax, kNumI };
-enum CSlot : int { kCntSpmvA = 0, kCntUpdate, kCntRefresh, kCntSpmvM, kCntInit, kCntFinal, kNumC };
+enum DSlot : int { kBnorm = 0, kTol, kRz0, kRz1, kErrPre, kNumD };
+enum ISlot : int { kDone = 0, kStatus, kRefreshed, kMax, kCount0, kCount1, kCountLatest, kNumI };
+// partial-sum arrays, G doubles each
+enum PSlot : int { kP1 = 0, kP2, kP3, kP4, kPA, kPB, kPF, kNumP };
 
 template <int SG, class MV>
 __device__ __forceinline__ double RowDot(const int* __restrict__ ptr, const int* __restrict__ col,
@@ -42,45 +46,30 @@ __device__ __forceinline__ double RowDot(const int* __restrict__ ptr, const int*
   return s;
 }
 
-// Block sum of v, partial to part[block]; returns true in the grid's last block
-// (after which part[] holds every block's partial).
-__device__ __forceinline__ bool LastBlock(double v, double* __restrict__ part, unsigned* __restrict__ counter) {
+// Block sum of v written to part[blockIdx.x].
+__device__ __forceinline__ void StorePartial(double v, double* __restrict__ part) {
   __shared__ double red[16];
-  __shared__ int last;
   v = BlockSum<double>(v, red);
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = v;
-    __threadfence();
-    const unsigned prev = atomicAdd(counter, 1u);
-    last = (prev == gridDim.x - 1) ? 1 : 0;
-  }
-  __syncthreads();
-  if (last) __threadfence();
-  return last != 0;
+  if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
 
-// Fixed-order sum of the partials (last block only); resets the counter.
-__device__ __forceinline__ double FinishSum(const double* __restrict__ part, unsigned* __restrict__ counter) {
+// Every block: fixed-order sum of the previous kernel's G partials.
+__device__ __forceinline__ double SumPartials(const double* __restrict__ part, int G) {
   __shared__ double red[16];
   double s = 0.0;
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-    s += __hip_atomic_load(&part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  s = BlockSum<double>(s, red);
-  if (threadIdx.x == 0) *counter = 0u;
-  return s;
+  for (int b = threadIdx.x; b < G; b += blockDim.x) s += part[b];
+  return BlockSum<double>(s, red);
 }
 
 __device__ __forceinline__ bool Finite(double v) { return isfinite(v); }
 
-// r = b, x = 0, z = M b, p = z; <r, r>, <r, z>; history[0] = 1.
+// r = b, x = 0, z = M b, p = z; partials of <r, r> and <r, z>.
 template <int SG, class MV>
 __global__ __launch_bounds__(kBlk) void PcgInit(int n, const int* __restrict__ mp, const int* __restrict__ mc,
                                                 const MV* __restrict__ mv, const double* __restrict__ b,
                                                 double* __restrict__ x, double* __restrict__ r,
                                                 double* __restrict__ z, double* __restrict__ p,
-                                                double* __restrict__ part, unsigned* __restrict__ cnt,
-                                                double* __restrict__ ds, int* __restrict__ is,
-                                                double* __restrict__ hist) {
+                                                double* __restrict__ part) {
   const int groups = kBlk / SG, sl = threadIdx.x % SG;
   double rr = 0.0, rz = 0.0;
   for (int row = blockIdx.x * groups + threadIdx.x / SG; row < n; row += gridDim.x * groups) {
@@ -95,40 +84,32 @@ __global__ __launch_bounds__(kBlk) void PcgInit(int n, const int* __restrict__ m
       rz += br * zr;
     }
   }
-  // two reductions: rr into part[0..G), rz into part[G..2G)
-  const bool last1 = LastBlock(rr, part, cnt + kCntInit);
-  __shared__ double rrs;
-  if (last1) {
-    const double s = FinishSum(part, cnt + kCntInit);
-    if (threadIdx.x == 0) rrs = s;
-  }
-  const bool last2 = LastBlock(rz, part + gridDim.x, cnt + kCntFinal);
-  if (last2) {
-    const double s = FinishSum(part + gridDim.x, cnt + kCntFinal);
-    if (threadIdx.x == 0) {
-      // both sums are final in the block that finishes the second one
-      double rr_total = 0.0;
-      for (unsigned bb = 0; bb < gridDim.x; ++bb)
-        rr_total += __hip_atomic_load(&part[bb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      (void)rrs;
-      ds[kBnorm] = sqrt(rr_total);
-      ds[kRz] = s;
-      hist[0] = 1.0;
-      is[kCount] = 1;
-      is[kStatus] = 0;
-      is[kRefreshed] = 0;
-      is[kDone] = (1 < is[kMax] && 1.0 > ds[kTol]) ? 0 : 1;
-    }
-  }
+  StorePartial(rr, part + kPA * gridDim.x);
+  StorePartial(rz, part + kPB * gridDim.x);
 }
 
-// Ap = A p and <p, Ap>; last block: alpha (backward_error.pyx:101-109).
+// ||b||, r.z, history[0] = 1 (backward_error.pyx:94-100).
+__global__ __launch_bounds__(kBlk) void PcgInitFinish(const double* __restrict__ part, int G, double* __restrict__ ds,
+                                                      int* __restrict__ is, double* __restrict__ hist) {
+  const double rr = SumPartials(part + kPA * G, G);
+  const double rz = SumPartials(part + kPB * G, G);
+  if (threadIdx.x != 0) return;
+  ds[kBnorm] = sqrt(rr);
+  ds[kRz0] = rz;
+  hist[0] = 1.0;
+  is[kCount0] = 1;
+  is[kCountLatest] = 1;
+  is[kStatus] = 0;
+  is[kRefreshed] = 0;
+  is[kDone] = (1 < is[kMax] && 1.0 > ds[kTol]) ? 0 : 1;
+}
+
+// Ap = A p, partials of <p, Ap>.
 template <int SG>
 __global__ __launch_bounds__(kBlk) void PcgSpmvA(int n, const int* __restrict__ ap_, const int* __restrict__ ac,
                                                  const double* __restrict__ av, const double* __restrict__ p,
                                                  double* __restrict__ Ap, double* __restrict__ part,
-                                                 unsigned* __restrict__ cnt, double* __restrict__ ds,
-                                                 int* __restrict__ is, int force) {
+                                                 const int* __restrict__ is, int force) {
   if (!force && is[kDone]) return;
   const int groups = kBlk / SG, sl = threadIdx.x % SG;
   double acc = 0.0;
@@ -139,31 +120,28 @@ __global__ __launch_bounds__(kBlk) void PcgSpmvA(int n, const int* __restrict__ 
       acc += p[row] * s;
     }
   }
-  if (!LastBlock(acc, part, cnt + kCntSpmvA)) return;
-  const double den = FinishSum(part, cnt + kCntSpmvA);
-  if (threadIdx.x != 0 || force) return;
-  const double r_z = ds[kRz];
-  if (!Finite(r_z) || !Finite(den) || r_z == 0.0 || den == 0.0) {
-    is[kStatus] = 2;
-    is[kDone] = 1;
-    return;
-  }
-  const double alpha = r_z / den;
-  if (!Finite(alpha)) {
-    is[kStatus] = 2;
-    is[kDone] = 1;
-    return;
-  }
-  ds[kAlpha] = alpha;
+  StorePartial(acc, part + kP1 * gridDim.x);
 }
 
-// x += alpha p; r -= alpha Ap; error = ||r|| / ||b|| (backward_error.pyx:111-116).
-__global__ __launch_bounds__(kBlk) void PcgUpdate(int n, double* __restrict__ x, double* __restrict__ r,
+// alpha (backward_error.pyx:102-109), then x += alpha p; r -= alpha Ap.
+__global__ __launch_bounds__(kBlk) void PcgUpdate(int n, int par, double* __restrict__ x, double* __restrict__ r,
                                                   const double* __restrict__ p, const double* __restrict__ Ap,
-                                                  double* __restrict__ part, unsigned* __restrict__ cnt,
-                                                  double* __restrict__ ds, int* __restrict__ is) {
+                                                  double* __restrict__ part, double* __restrict__ ds,
+                                                  int* __restrict__ is) {
   if (is[kDone]) return;
-  const double alpha = ds[kAlpha];
+  const int G = gridDim.x;
+  const double den = SumPartials(part + kP1 * G, G);
+  const double r_z = ds[par ? kRz1 : kRz0];
+  bool brk = !Finite(r_z) || !Finite(den) || r_z == 0.0 || den == 0.0;
+  const double alpha = brk ? 0.0 : r_z / den;
+  brk = brk || !Finite(alpha);
+  if (brk) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      is[kStatus] = 2;
+      is[kDone] = 1;
+    }
+    return;
+  }
   double acc = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     x[i] += alpha * p[i];
@@ -171,23 +149,26 @@ __global__ __launch_bounds__(kBlk) void PcgUpdate(int n, double* __restrict__ x,
     r[i] = ri;
     acc += ri * ri;
   }
-  if (!LastBlock(acc, part, cnt + kCntUpdate)) return;
-  const double rr = FinishSum(part, cnt + kCntUpdate);
-  if (threadIdx.x != 0) return;
-  const double err = sqrt(rr) / ds[kBnorm];
-  ds[kErr] = err;
-  is[kRefreshed] = err <= ds[kTol] ? 1 : 0;
+  StorePartial(acc, part + kP2 * G);
 }
 
-// Refresh at the tolerance crossing: r = b - A x (backward_error.pyx:117-122).
+// error = ||r|| / ||b||; at the tolerance crossing r = b - A x (backward_error.pyx:114-122).
 template <int SG>
 __global__ __launch_bounds__(kBlk) void PcgRefresh(int n, const int* __restrict__ ap_, const int* __restrict__ ac,
                                                    const double* __restrict__ av, const double* __restrict__ x,
                                                    const double* __restrict__ b, double* __restrict__ r,
                                                    double* __restrict__ Ap, double* __restrict__ part,
-                                                   unsigned* __restrict__ cnt, double* __restrict__ ds,
-                                                   int* __restrict__ is) {
-  if (is[kDone] || !is[kRefreshed]) return;
+                                                   double* __restrict__ ds, int* __restrict__ is) {
+  if (is[kDone]) return;
+  const int G = gridDim.x;
+  const double rr = SumPartials(part + kP2 * G, G);
+  const double err = sqrt(rr) / ds[kBnorm];
+  const bool refreshed = err <= ds[kTol];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ds[kErrPre] = err;
+    is[kRefreshed] = refreshed ? 1 : 0;
+  }
+  if (!refreshed) return;
   const int groups = kBlk / SG, sl = threadIdx.x % SG;
   double acc = 0.0;
   for (int row = blockIdx.x * groups + threadIdx.x / SG; row < n; row += gridDim.x * groups) {
@@ -199,19 +180,15 @@ __global__ __launch_bounds__(kBlk) void PcgRefresh(int n, const int* __restrict_
       acc += ri * ri;
     }
   }
-  if (!LastBlock(acc, part, cnt + kCntRefresh)) return;
-  const double rr = FinishSum(part, cnt + kCntRefresh);
-  if (threadIdx.x != 0) return;
-  ds[kErr] = sqrt(rr) / ds[kBnorm];
+  StorePartial(acc, part + kP3 * G);
 }
 
-// z = M r, <r, z>; last block: beta, history, stop rules (backward_error.pyx:123-133,100).
+// z = M r, partials of <r, z>.
 template <int SG, class MV>
 __global__ __launch_bounds__(kBlk) void PcgSpmvM(int n, const int* __restrict__ mp, const int* __restrict__ mc,
                                                  const MV* __restrict__ mv, const double* __restrict__ r,
                                                  double* __restrict__ z, double* __restrict__ part,
-                                                 unsigned* __restrict__ cnt, double* __restrict__ ds,
-                                                 int* __restrict__ is, double* __restrict__ hist) {
+                                                 const int* __restrict__ is) {
   if (is[kDone]) return;
   const int groups = kBlk / SG, sl = threadIdx.x % SG;
   double acc = 0.0;
@@ -222,46 +199,45 @@ __global__ __launch_bounds__(kBlk) void PcgSpmvM(int n, const int* __restrict__ 
       acc += r[row] * s;
     }
   }
-  if (!LastBlock(acc, part, cnt + kCntSpmvM)) return;
-  const double r_z = FinishSum(part, cnt + kCntSpmvM);
-  if (threadIdx.x != 0) return;
-  const bool refreshed = is[kRefreshed] != 0;
-  const double beta = refreshed ? 0.0 : r_z / ds[kRz];
-  ds[kRz] = r_z;
-  const double err = ds[kErr];
-  const int count = is[kCount];
-  hist[count] = err;
-  is[kCount] = count + 1;
-  ds[kBeta] = beta;
-  if (!Finite(err)) {
-    is[kStatus] = 3;
-    is[kDone] = 1;
-    return;
-  }
-  if (!Finite(beta)) {
-    is[kStatus] = 2;
-    is[kDone] = 1;
-    return;
-  }
-  if (!(count + 1 < is[kMax] && err > ds[kTol])) is[kDone] = 1;
+  StorePartial(acc, part + kP4 * gridDim.x);
 }
 
-// p = z + beta p (backward_error.pyx:134-135).
-__global__ __launch_bounds__(kBlk) void PcgDirection(int n, double* __restrict__ p, const double* __restrict__ z,
-                                                     const double* __restrict__ ds, const int* __restrict__ is) {
+// beta, history, stop rules (backward_error.pyx:124-133,100), then p = z + beta p.
+__global__ __launch_bounds__(kBlk) void PcgDirection(int n, int par, double* __restrict__ p,
+                                                     const double* __restrict__ z, const double* __restrict__ part,
+                                                     double* __restrict__ ds, int* __restrict__ is,
+                                                     double* __restrict__ hist) {
   if (is[kDone]) return;
-  const double beta = ds[kBeta];
+  const int G = gridDim.x;
+  const double r_z = SumPartials(part + kP4 * G, G);
+  const bool refreshed = is[kRefreshed] != 0;
+  const double err = refreshed ? sqrt(SumPartials(part + kP3 * G, G)) / ds[kBnorm] : ds[kErrPre];
+  const double beta = refreshed ? 0.0 : r_z / ds[par ? kRz1 : kRz0];
+  const int count = is[par ? kCount1 : kCount0];
+  int status = 0;
+  if (!Finite(err))
+    status = 3;
+  else if (!Finite(beta))
+    status = 2;
+  const bool stop = status != 0 || !(count + 1 < is[kMax] && err > ds[kTol]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    hist[count] = err;
+    is[par ? kCount0 : kCount1] = count + 1;
+    is[kCountLatest] = count + 1;
+    ds[par ? kRz0 : kRz1] = r_z;
+    if (status) is[kStatus] = status;
+    if (stop) is[kDone] = 1;
+  }
+  if (stop) return;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + beta * p[i];
 }
 
-// Final true residual overwrites the last history entry (backward_error.pyx:136-143).
+// Final true residual (backward_error.pyx:136-139): partials of ||b - A x||^2.
 template <int SG>
 __global__ __launch_bounds__(kBlk) void PcgFinal(int n, const int* __restrict__ ap_, const int* __restrict__ ac,
                                                  const double* __restrict__ av, const double* __restrict__ x,
                                                  const double* __restrict__ b, double* __restrict__ r,
-                                                 double* __restrict__ Ap, double* __restrict__ part,
-                                                 unsigned* __restrict__ cnt, double* __restrict__ ds,
-                                                 int* __restrict__ is, double* __restrict__ hist) {
+                                                 double* __restrict__ Ap, double* __restrict__ part) {
   const int groups = kBlk / SG, sl = threadIdx.x % SG;
   double acc = 0.0;
   for (int row = blockIdx.x * groups + threadIdx.x / SG; row < n; row += gridDim.x * groups) {
@@ -273,10 +249,17 @@ __global__ __launch_bounds__(kBlk) void PcgFinal(int n, const int* __restrict__ 
       acc += ri * ri;
     }
   }
-  if (!LastBlock(acc, part, cnt + kCntFinal)) return;
-  const double rr = FinishSum(part, cnt + kCntFinal);
+  StorePartial(acc, part + kPF * gridDim.x);
+}
+
+// Overwrites the last history entry with the true residual; status 1 / 3
+// (backward_error.pyx:139-143).
+__global__ __launch_bounds__(kBlk) void PcgFinalFinish(const double* __restrict__ part, int G,
+                                                       const double* __restrict__ ds, int* __restrict__ is,
+                                                       double* __restrict__ hist) {
+  const double rr = SumPartials(part + kPF * G, G);
   if (threadIdx.x != 0) return;
-  const int count = is[kCount];
+  const int count = is[kCountLatest];
   const double h = sqrt(rr) / ds[kBnorm];
   hist[count - 1] = h;
   if (!Finite(h))
@@ -374,11 +357,9 @@ PcgSolver::PcgSolver(Ctx& ctx, const DCsr& A, const DCsr& M, bool m_fp32)
   z_ = ctx.Alloc<double>(n_);
   p_ = ctx.Alloc<double>(n_);
   ap_ = ctx.Alloc<double>(n_);
-  part_ = ctx.Alloc<double>(2 * grid_ + 2);
+  part_ = ctx.Alloc<double>(size_t(kNumP) * grid_);
   scal_ = ctx.Alloc<double>(kNumD);
   ist_ = ctx.Alloc<int>(kNumI);
-  counters_ = ctx.Alloc<unsigned>(kNumC);
-  SPAI_HIP_CHECK(hipMemsetAsync(counters_.get(), 0, sizeof(unsigned) * kNumC, ctx.stream));
 }
 
 PcgSolver::~PcgSolver() {
@@ -412,24 +393,25 @@ void PcgSolver::Record(bool fused, int64_t iterations) {
   hipStream_t st = ctx_.stream;
   const int n = (int)n_;
   for (int64_t it = 0; it < iterations; ++it) {
+    const int par = int(it & 1);
     SPAI_SG_DISPATCH(sga, (PcgSpmvA<SG><<<grid_, kBlk, 0, st>>>(n, A_.ptr.get(), A_.col.get(), A_.val.get(),
-                                                                p_.get(), ap_.get(), part_.get(), counters_.get(),
-                                                                scal_.get(), ist_.get(), 0)));
-    PcgUpdate<<<grid_, kBlk, 0, st>>>(n, x_.get(), r_.get(), p_.get(), ap_.get(), part_.get(), counters_.get(),
-                                      scal_.get(), ist_.get());
+                                                                p_.get(), ap_.get(), part_.get(), ist_.get(), 0)));
+    PcgUpdate<<<grid_, kBlk, 0, st>>>(n, par, x_.get(), r_.get(), p_.get(), ap_.get(), part_.get(), scal_.get(),
+                                      ist_.get());
     SPAI_SG_DISPATCH(sga, (PcgRefresh<SG><<<grid_, kBlk, 0, st>>>(n, A_.ptr.get(), A_.col.get(), A_.val.get(),
                                                                   x_.get(), b_, r_.get(), ap_.get(), part_.get(),
-                                                                  counters_.get(), scal_.get(), ist_.get())));
+                                                                  scal_.get(), ist_.get())));
     if (m_fp32_) {
-      SPAI_SG_DISPATCH(sgm, (PcgSpmvM<SG, float><<<grid_, kBlk, 0, st>>>(
-                                n, M_.ptr.get(), M_.col.get(), M_.val32.get(), r_.get(), z_.get(), part_.get(),
-                                counters_.get(), scal_.get(), ist_.get(), hist_.get())));
+      SPAI_SG_DISPATCH(sgm, (PcgSpmvM<SG, float><<<grid_, kBlk, 0, st>>>(n, M_.ptr.get(), M_.col.get(),
+                                                                         M_.val32.get(), r_.get(), z_.get(),
+                                                                         part_.get(), ist_.get())));
     } else {
-      SPAI_SG_DISPATCH(sgm, (PcgSpmvM<SG, double><<<grid_, kBlk, 0, st>>>(
-                                n, M_.ptr.get(), M_.col.get(), M_.val.get(), r_.get(), z_.get(), part_.get(),
-                                counters_.get(), scal_.get(), ist_.get(), hist_.get())));
+      SPAI_SG_DISPATCH(sgm, (PcgSpmvM<SG, double><<<grid_, kBlk, 0, st>>>(n, M_.ptr.get(), M_.col.get(),
+                                                                          M_.val.get(), r_.get(), z_.get(),
+                                                                          part_.get(), ist_.get())));
     }
-    PcgDirection<<<grid_, kBlk, 0, st>>>(n, p_.get(), z_.get(), scal_.get(), ist_.get());
+    PcgDirection<<<grid_, kBlk, 0, st>>>(n, par, p_.get(), z_.get(), part_.get(), scal_.get(), ist_.get(),
+                                         hist_.get());
   }
 }
 
@@ -438,38 +420,29 @@ int PcgSolver::Solve(const double* b_dev, int64_t max_iterations, double toleran
   if (max_iterations > (int64_t(1) << 30)) throw std::invalid_argument("max_iterations too large");
   hipStream_t st = ctx_.stream;
   const int n = (int)n_;
-  if ((int64_t)hist_.size() < max_iterations) {
-    hist_ = ctx_.Alloc<double>(max_iterations);
-    if (exec_) {
-      (void)hipGraphExecDestroy(exec_);
-      exec_ = nullptr;
-    }
-    if (graph_) {
-      (void)hipGraphDestroy(graph_);
-      graph_ = nullptr;
-    }
+  if ((int64_t)hist_.size() < max_iterations || b_dev != graph_key_b_) {
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    exec_ = nullptr;
+    graph_ = nullptr;
+    if ((int64_t)hist_.size() < max_iterations) hist_ = ctx_.Alloc<double>(max_iterations);
   }
   b_ = b_dev;
-  if (graph_key_b_ != b_dev && exec_) {
-    (void)hipGraphExecDestroy(exec_);
-    exec_ = nullptr;
-    (void)hipGraphDestroy(graph_);
-    graph_ = nullptr;
-  }
-  const double hd[kNumD] = {0, 0, 0, 0, 0, tolerance};
-  const int hi[kNumI] = {0, 0, 0, 0, (int)max_iterations};
+  const double hd[kNumD] = {0.0, tolerance, 0.0, 0.0, 0.0};
+  int hi[kNumI] = {0, 0, 0, (int)max_iterations, 0, 0, 0};
   SPAI_HIP_CHECK(hipMemcpyAsync(scal_.get(), hd, sizeof(hd), hipMemcpyHostToDevice, st));
   SPAI_HIP_CHECK(hipMemcpyAsync(ist_.get(), hi, sizeof(hi), hipMemcpyHostToDevice, st));
   const int sgm = SubgroupFor(M_.n, M_.nnz), sga = SubgroupFor(A_.n, A_.nnz);
   if (m_fp32_) {
-    SPAI_SG_DISPATCH(sgm, (PcgInit<SG, float><<<grid_, kBlk, 0, st>>>(
-                              n, M_.ptr.get(), M_.col.get(), M_.val32.get(), b_dev, x_.get(), r_.get(), z_.get(),
-                              p_.get(), part_.get(), counters_.get(), scal_.get(), ist_.get(), hist_.get())));
+    SPAI_SG_DISPATCH(sgm, (PcgInit<SG, float><<<grid_, kBlk, 0, st>>>(n, M_.ptr.get(), M_.col.get(), M_.val32.get(),
+                                                                      b_dev, x_.get(), r_.get(), z_.get(), p_.get(),
+                                                                      part_.get())));
   } else {
-    SPAI_SG_DISPATCH(sgm, (PcgInit<SG, double><<<grid_, kBlk, 0, st>>>(
-                              n, M_.ptr.get(), M_.col.get(), M_.val.get(), b_dev, x_.get(), r_.get(), z_.get(),
-                              p_.get(), part_.get(), counters_.get(), scal_.get(), ist_.get(), hist_.get())));
+    SPAI_SG_DISPATCH(sgm, (PcgInit<SG, double><<<grid_, kBlk, 0, st>>>(n, M_.ptr.get(), M_.col.get(), M_.val.get(),
+                                                                       b_dev, x_.get(), r_.get(), z_.get(),
+                                                                       p_.get(), part_.get())));
   }
+  PcgInitFinish<<<1, kBlk, 0, st>>>(part_.get(), (int)grid_, scal_.get(), ist_.get(), hist_.get());
   if (!exec_) {
     SPAI_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     Record(true, kChunk);
@@ -486,12 +459,11 @@ int PcgSolver::Solve(const double* b_dev, int64_t max_iterations, double toleran
     if (hst[kDone] || launched >= max_iterations) break;
   }
   SPAI_SG_DISPATCH(sga, (PcgFinal<SG><<<grid_, kBlk, 0, st>>>(n, A_.ptr.get(), A_.col.get(), A_.val.get(), x_.get(),
-                                                              b_dev, r_.get(), ap_.get(), part_.get(),
-                                                              counters_.get(), scal_.get(), ist_.get(),
-                                                              hist_.get())));
+                                                              b_dev, r_.get(), ap_.get(), part_.get())));
+  PcgFinalFinish<<<1, kBlk, 0, st>>>(part_.get(), (int)grid_, scal_.get(), ist_.get(), hist_.get());
   SPAI_HIP_CHECK(hipGetLastError());
   ctx_.ReadBack(hst, ist_.get(), sizeof(hst));
-  *count = hst[kCount];
+  *count = hst[kCountLatest];
   return hst[kStatus];
 }
 
@@ -508,8 +480,7 @@ double PcgSolver::TimeSpmvA(int reps) {
   for (int i = 0; i < reps; ++i) {
     SPAI_HIP_CHECK(hipEventRecord(ev[2 * i], st));
     SPAI_SG_DISPATCH(sga, (PcgSpmvA<SG><<<grid_, kBlk, 0, st>>>(n, A_.ptr.get(), A_.col.get(), A_.val.get(),
-                                                                p_.get(), ap_.get(), part_.get(), counters_.get(),
-                                                                scal_.get(), ist_.get(), 1)));
+                                                                p_.get(), ap_.get(), part_.get(), ist_.get(), 1)));
     SPAI_HIP_CHECK(hipEventRecord(ev[2 * i + 1], st));
   }
   SPAI_HIP_CHECK(hipStreamSynchronize(st));
@@ -530,9 +501,5 @@ void SpMV(Ctx& ctx, const DCsr& A, const double* x, double* y) {
                                                                     x, y)));
 }
 
-double PcgSolver::TimeIterationUs(int reps) {
-  (void)reps;
-  return 0.0;
-}
 
 }  // namespace spai

@@ -35,7 +35,6 @@ class PcgSolver {
   // Average device time of one iteration's A-SpMV kernel over the last solve
   // (microseconds, from HIP events around a standalone replay).
   double TimeSpmvA(int reps);
-  double TimeIterationUs(int reps);
 
  private:
   void Record(bool fused, int64_t iterations);
@@ -48,14 +47,9 @@ class PcgSolver {
   DBuf<double> x_, r_, z_, p_, ap_, part_, hist_;
   DBuf<double> scal_;
   DBuf<int> ist_;
-  DBuf<unsigned> counters_;
   const double* b_ = nullptr;
-  double tol_ = 0.0;
-  int64_t max_it_ = 0;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t exec_ = nullptr;
-  int64_t graph_key_max_ = -1;
-  double graph_key_tol_ = -1.0;
   const double* graph_key_b_ = nullptr;
 };
 

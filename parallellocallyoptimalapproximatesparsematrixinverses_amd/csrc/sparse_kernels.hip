@@ -472,8 +472,43 @@ __global__ void BatchSqNormKernel(int nb, const int* __restrict__ first, const i
   out[b] = s;
 }
 
+// Per-batch sequential intersection dot over the batch's columns (CSC order).
+__global__ void BatchDotKernel(int nb, const int* __restrict__ first, const int* __restrict__ xp,
+                               const int* __restrict__ xi, const double* __restrict__ xv, const int* __restrict__ yp,
+                               const int* __restrict__ yi, const double* __restrict__ yv, int same,
+                               double* __restrict__ out) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double s = 0.0;
+  for (int j = first[b]; j < first[b + 1]; ++j) {
+    if (same) {
+      for (int p = xp[j]; p < xp[j + 1]; ++p) s += xv[p] * xv[p];
+    } else {
+      int x = xp[j], xe = xp[j + 1], y = yp[j], ye = yp[j + 1];
+      while (x < xe && y < ye) {
+        if (xi[x] == yi[y]) {
+          s += xv[x] * yv[y];
+          ++x;
+          ++y;
+        } else if (xi[x] < yi[y]) {
+          ++x;
+        } else {
+          ++y;
+        }
+      }
+    }
+  }
+  out[b] = s;
+}
+
+__global__ void SequentialSum(int n, const double* __restrict__ v, double* __restrict__ out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += v[i];
+  *out = s;
+}
+
 // ------------------------------------------------------------------ dropping
-constexpr double kDoubleEps = 2.220446049250313080847e-16;
 
 // ApplyDroppingStrategy threshold (dropping.cpp:124-133): sequential column
 // norm, keep v != 0 && (diag || |v| >= eps * norm).
@@ -1182,6 +1217,17 @@ double SqNorm(Ctx& ctx, const DMat& X) {
   auto cd = ctx.Alloc<double>(n > 0 ? n : 1);
   ColumnSqNorms(ctx, X, cd.get());
   return SumDevice(ctx, cd.get(), n);
+}
+
+double BatchOrderedDot(Ctx& ctx, const DMat& X, const DMat& Y, const int* batch_first, int nbatches, bool same) {
+  if (X.rows() != Y.rows() || X.cols() != Y.cols()) throw std::invalid_argument("Sparse matrices must have the same shape");
+  auto part = ctx.Alloc<double>(nbatches + 1);
+  BatchDotKernel<<<Blocks(nbatches, 64), 64, 0, ctx.stream>>>(nbatches, batch_first, X.ptr(), X.idx(), X.v(), Y.ptr(),
+                                                              Y.idx(), Y.v(), same ? 1 : 0, part.get());
+  SequentialSum<<<1, 64, 0, ctx.stream>>>(nbatches, part.get(), part.get() + nbatches);
+  double s = 0.0;
+  ctx.ReadBack(&s, part.get() + nbatches, sizeof(double));
+  return s;
 }
 
 void BatchSqNorms(Ctx& ctx, const DMat& X, const int* batch_first, int nbatches, double* out) {

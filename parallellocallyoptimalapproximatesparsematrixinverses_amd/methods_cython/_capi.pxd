@@ -51,7 +51,8 @@ cdef extern from "spai_ext.h":
         int64_t fixed_pattern
         int64_t precision
         int64_t column_top_k
-        int64_t reserved[5]
+        int64_t batch_ordered_reductions
+        int64_t reserved[4]
 
     cdef struct CsrData:
         int64_t rows
