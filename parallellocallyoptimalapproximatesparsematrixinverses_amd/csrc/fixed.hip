@@ -129,6 +129,246 @@ __device__ __forceinline__ void SumPartialsBlock(const double* __restrict__ part
   }
 }
 
+// ------------------------------------------------------------ slot product
+// (A Y)(:, j) on S2(:, j) by column-wise Gustavson: for each S(:, j) entry
+// (k, y) in order -- k ascending, the reference product's order -- the group's
+// lanes stream column A(:, k) and add a * y into an LDS accumulator.  The
+// accumulator slot of every contribution (its row's position in S2(:, j)) is
+// precomputed once for the fixed pattern ("plan", one uint8/uint16 per product
+// term, laid out in the order the terms are visited), so the iteration does no
+// searching: A's columns, the plan and Y are read contiguously.  Rows inside
+// one A(:, k) are distinct, so lanes never collide within a step.
+template <class V, class SlotT, int GL, int NY, class Epi>
+__global__ __launch_bounds__(kBlk) void SlotProduct(int n, const int* __restrict__ sp, const int* __restrict__ si,
+                                                    const V* __restrict__ y0, const V* __restrict__ y1,
+                                                    const int* __restrict__ ap, const V* __restrict__ av,
+                                                    const int64_t* __restrict__ plan_ptr,
+                                                    const SlotT* __restrict__ plan, const int* __restrict__ s2p,
+                                                    const int* __restrict__ s2i, int lmax, Epi epi,
+                                                    const double* __restrict__ sc, double* __restrict__ part,
+                                                    const Flags* __restrict__ flags) {
+  if (flags->stop) return;
+  epi.Load(sc);  // epilogues that use the step sizes read them here
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int kGroups = kBlk / GL;
+  const int g = threadIdx.x / GL, gl = threadIdx.x % GL;
+  V* acc = reinterpret_cast<V*>(smem_raw) + size_t(g) * NY * lmax;
+  double red[Epi::K];
+#pragma unroll
+  for (int k = 0; k < Epi::K; ++k) red[k] = 0.0;
+  for (int j = blockIdx.x * kGroups + g; j < n; j += gridDim.x * kGroups) {
+    const int q0 = s2p[j], L = s2p[j + 1] - q0;
+    for (int t = gl; t < L; t += GL) {
+      acc[t] = V(0);
+      if (NY > 1) acc[lmax + t] = V(0);
+    }
+    int64_t pp = plan_ptr[j];
+    const int e1 = sp[j + 1];
+    for (int eb = sp[j]; eb < e1; eb += GL) {
+      // each lane fetches one S entry of the column; the steps broadcast them
+      int my_a0 = 0, my_c = 0;
+      V my_ya = V(0), my_yb = V(0);
+      if (eb + gl < e1) {
+        const int k = si[eb + gl];
+        my_a0 = ap[k];
+        my_c = ap[k + 1] - my_a0;
+        my_ya = y0[eb + gl];
+        if (NY > 1) my_yb = y1[eb + gl];
+      }
+      const int ns = min(GL, e1 - eb);
+      for (int st = 0; st < ns; ++st) {
+        const int a0 = __shfl(my_a0, st, GL);
+        const int c = __shfl(my_c, st, GL);
+        const V ya = __shfl(my_ya, st, GL);
+        const V yb = NY > 1 ? __shfl(my_yb, st, GL) : V(0);
+        // LDS operations of one wavefront complete in issue order, so step st+1
+        // observes step st's sums without a fence.
+        for (int u = gl; u < c; u += GL) {
+          const int slot = plan[pp + u];
+          const V a = av[a0 + u];
+          acc[slot] += a * ya;
+          if (NY > 1) acc[lmax + slot] += a * yb;
+        }
+        pp += c;
+      }
+    }
+    for (int t = gl; t < L; t += GL) {
+      const int64_t q = q0 + t;
+      epi(q, s2i[q], j, acc[t], NY > 1 ? acc[lmax + t] : V(0), red);
+    }
+  }
+  StorePartials<Epi::K>(red, part);
+}
+
+// Plan of the slot product: slot of every (j, s, e) term, visited order.
+template <class SlotT>
+__global__ void BuildPlan(int n, const int* __restrict__ sp, const int* __restrict__ si, const int* __restrict__ ap,
+                          const int* __restrict__ ai, const int* __restrict__ s2p, const int* __restrict__ s2i,
+                          const int64_t* __restrict__ plan_ptr, SlotT* __restrict__ plan,
+                          int* __restrict__ error_flag) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int64_t pp = plan_ptr[j];
+  const int qb = s2p[j], qe = s2p[j + 1];
+  for (int e = sp[j]; e < sp[j + 1]; ++e) {
+    const int k = si[e];
+    for (int t = ap[k]; t < ap[k + 1]; ++t) {
+      const int q = FindSorted(s2i, qb, qe, ai[t]);
+      if (q < 0) atomicOr(error_flag, 1);
+      plan[pp++] = SlotT(q < 0 ? 0 : q - qb);
+    }
+  }
+}
+
+// Number of product terms of each column (u_j) and the longest A column.
+__global__ void PlanSizes(int n, const int* __restrict__ sp, const int* __restrict__ si, const int* __restrict__ ap,
+                          const int* __restrict__ s2p, int64_t* __restrict__ terms, int* __restrict__ ext) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int64_t u = 0;
+  for (int e = sp[j]; e < sp[j + 1]; ++e) u += ap[si[e] + 1] - ap[si[e]];
+  terms[j] = u;
+  atomicMax(&ext[0], ap[j + 1] - ap[j]);    // longest column of A
+  atomicMax(&ext[1], s2p[j + 1] - s2p[j]);  // longest column of S2
+}
+
+__global__ void ExclusiveScan64Serial(int n, const int64_t* __restrict__ in, int64_t* __restrict__ out) {
+  // one block: chunked scan, carry kept by thread 0
+  __shared__ int64_t wsum[16];
+  __shared__ int64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int b0 = 0; b0 < n; b0 += blockDim.x) {
+    const int i = b0 + threadIdx.x;
+    const int64_t v = i < n ? in[i] : 0;
+    const int64_t incl = WaveInclusiveScan64(v);
+    if (LaneId() == kWave - 1) wsum[WaveInBlock()] = incl;
+    __syncthreads();
+    int64_t wb = 0;
+    for (int w = 0; w < WaveInBlock(); ++w) wb += wsum[w];
+    const int64_t ex = carry + wb + incl - v;
+    __syncthreads();
+    if (i < n) out[i] = ex;
+    if (threadIdx.x == blockDim.x - 1) carry = ex + v;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[n] = carry;
+}
+
+// R = I - A M; Z = Pr R on S (mode 1: P = Z and <R, Z>).
+template <class V>
+struct ResidualEpi {
+  static constexpr int K = 2;
+  const int* s2_to_s;
+  const V* pr;
+  V* R;
+  V* Z;
+  V* P;
+  int mode;
+  __device__ void Load(const double*) {}
+  __device__ void operator()(int64_t q, int i, int j, V a0, V, double* acc) const {
+    V r = -a0;
+    if (i == j) r += V(1);
+    R[q] = r;
+    acc[0] += double(r) * double(r);
+    const int p = s2_to_s[q];
+    if (p >= 0) {
+      const V z = pr[i] * r;
+      Z[p] = z;
+      if (mode == 1) {
+        P[p] = z;
+        acc[1] += double(r) * double(z);
+      }
+    }
+  }
+};
+
+// AZ, AP and the five trace products of the LOMR step.
+template <class V>
+struct LomrProjectEpi {
+  static constexpr int K = 5;
+  const V* R;
+  const V* pr;
+  __device__ void Load(const double*) {}
+  __device__ void operator()(int64_t q, int i, int, V az, V ap, double* acc) const {
+    const double daz = az, dap = ap, dpr = pr[i], dprr = double(pr[i] * R[q]);
+    acc[0] += daz * (dpr * daz);
+    acc[1] += dap * (dpr * dap);
+    acc[2] += daz * (dpr * dap);
+    acc[3] += dprr * daz;
+    acc[4] += dprr * dap;
+  }
+};
+
+// LOMR update from recomputed AZ, AP (locally_minimal_residual.cpp:92-107):
+// R -= delta AZ + gamma AP; on S: M += delta Z + gamma P, P = beta P + Z, and
+// Z = Pr R for the next step.  Every entry touched belongs to the column whose
+// product was just formed, so no other group reads it in this kernel.
+template <class V>
+struct LomrUpdateEpi {
+  static constexpr int K = 1;
+  const int* s2_to_s;
+  const V* pr;
+  V* R;
+  V* M;
+  V* Z;
+  V* P;
+  V d, g, b, nd, ng;
+  __device__ void Load(const double* sc) {
+    d = V(sc[kDelta]);
+    g = V(sc[kGamma]);
+    b = V(sc[kBeta]);
+    nd = V(-sc[kDelta]);
+    ng = V(-sc[kGamma]);
+  }
+  __device__ void operator()(int64_t q, int i, int, V az, V ap, double* acc) const {
+    V r = R[q];
+    r = r + nd * az;
+    r = r + ng * ap;
+    R[q] = r;
+    acc[0] += double(r) * double(r);
+    const int p = s2_to_s[q];
+    if (p >= 0) {
+      const V z = Z[p], pp = P[p];
+      V m = M[p];
+      m = m + d * z;
+      m = m + g * pp;
+      M[p] = m;
+      P[p] = pp * b + z;
+      Z[p] = pr[i] * r;
+    }
+  }
+};
+
+// num = <Pr R, Pr A Z>, den = ||Pr A Z||^2.
+template <class V>
+struct MrProjectEpi {
+  static constexpr int K = 2;
+  const V* R;
+  const V* pr;
+  __device__ void Load(const double*) {}
+  __device__ void operator()(int64_t q, int i, int, V az, V, double* acc) const {
+    const double tmp = double(pr[i] * az);
+    acc[0] += double(pr[i] * R[q]) * tmp;
+    acc[1] += tmp * tmp;
+  }
+};
+
+// AP and <P, AP>.
+template <class V>
+struct CgProjectEpi {
+  static constexpr int K = 1;
+  const int* s2_to_s;
+  const V* P;
+  V* AP;
+  __device__ void Load(const double*) {}
+  __device__ void operator()(int64_t q, int, int, V ap, V, double* acc) const {
+    AP[q] = ap;
+    const int p = s2_to_s[q];
+    if (p >= 0) acc[0] += double(P[p]) * double(ap);
+  }
+};
+
 // --------------------------------------------------------------- residual
 // R = I - A M on S2 (value -(sum) + [i == j]); Z = Pr R sampled on S.
 // mode 1 (CG setup) also sets P = Z and accumulates <R, Z>.
@@ -570,6 +810,48 @@ class FixedImpl final : public FixedProblem {
     int e = 0;
     c.ReadBack(&e, err.get(), sizeof(int));
     if (e) throw std::runtime_error("fixed pattern S is not contained in pattern(A S)");
+    // slot plan of the products (fixed pattern)
+    {
+      auto terms = c.Alloc<int64_t>(n_);
+      auto ext = c.Alloc<int>(2);
+      SPAI_HIP_CHECK(hipMemsetAsync(ext.get(), 0, 2 * sizeof(int), c.stream));
+      PlanSizes<<<Blocks1(n_), kBlk, 0, c.stream>>>((int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), A_.ptr(),
+                                                    s2_pat_->ptr.get(), terms.get(), ext.get());
+      plan_ptr_ = c.Alloc<int64_t>(n_ + 1);
+      ExclusiveScan64Serial<<<1, 1024, 0, c.stream>>>((int)n_, terms.get(), plan_ptr_.get());
+      int h[2];
+      c.ReadBack(h, ext.get(), sizeof(h));
+      int64_t total = 0;
+      c.ReadBack(&total, plan_ptr_.get() + n_, sizeof(int64_t));
+      maxcol_ = std::max(h[0], 1);
+      lmax_ = std::max(h[1], 1);
+      gl_ = maxcol_ <= 8 ? 8 : maxcol_ <= 16 ? 16 : maxcol_ <= 32 ? 32 : 64;
+      smem_ = size_t(kBlk / gl_) * 2 * size_t(lmax_) * sizeof(V);
+      slot16_ = lmax_ > 256;
+      use_plan_ = lmax_ <= 65535 && smem_ <= 64 * 1024;
+      if (use_plan_) {
+        auto err2 = c.Alloc<int>(1);
+        SPAI_HIP_CHECK(hipMemsetAsync(err2.get(), 0, sizeof(int), c.stream));
+        if (slot16_) {
+          plan16_ = c.Alloc<uint16_t>(std::max<int64_t>(total, 1));
+          BuildPlan<uint16_t><<<Blocks1(n_), kBlk, 0, c.stream>>>(
+              (int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), A_.ptr(), A_.idx(), s2_pat_->ptr.get(),
+              s2_pat_->idx.get(), plan_ptr_.get(), plan16_.get(), err2.get());
+        } else {
+          plan8_ = c.Alloc<uint8_t>(std::max<int64_t>(total, 1));
+          BuildPlan<uint8_t><<<Blocks1(n_), kBlk, 0, c.stream>>>(
+              (int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), A_.ptr(), A_.idx(), s2_pat_->ptr.get(),
+              s2_pat_->idx.get(), plan_ptr_.get(), plan8_.get(), err2.get());
+        }
+        int e2 = 0;
+        c.ReadBack(&e2, err2.get(), sizeof(int));
+        if (e2) throw std::runtime_error("product term outside pattern(A S)");
+        const int64_t groups = kBlk / gl_;
+        gridp_ = (unsigned)std::min<int64_t>(std::max<int64_t>((n_ + groups - 1) / groups, 1), kMaxGrid);
+      }
+      av_ = c.Alloc<V>(A_.nnz());
+      ConvertValues<V><<<Blocks1(A_.nnz()), kBlk, 0, c.stream>>>(A_.nnz(), A_.v(), av_.get());
+    }
     // value arrays
     atv_ = c.Alloc<V>(at_pat_->nnz);
     ConvertValues<V><<<Blocks1(at_pat_->nnz), kBlk, 0, c.stream>>>(at_pat_->nnz, at_d_.get(), atv_.get());
@@ -610,32 +892,61 @@ class FixedImpl final : public FixedProblem {
     const int* si = s_pat_->idx.get();
     const double tol = prm_.tolerance;
     const int G2 = (int)grid2_;
-    ResidualKernel<V><<<grid2_, kBlk, 0, st>>>(nnz_s2_, s2r, s2c, map, atp, ati, atv_.get(), sp, si, M_.get(),
-                                               prv_.get(), R_.get(), Z_.get(), P_.get(),
-                                               method_ == SPAI_GLOBAL_CG ? 1 : 0, part_.get(), flags_.get());
-    AppendInitial<<<1, kBlk, 0, st>>>(part_.get(), G2, tol, hist_.get(), sc_.get(), flags_.get());
+    const int Gp = use_plan_ ? (int)gridp_ : G2;  // partials of the product kernels
+    auto residual = [&](int mode) {
+      if (use_plan_) {
+        Slot<1>(ResidualEpi<V>{map, prv_.get(), R_.get(), Z_.get(), P_.get(), mode}, M_.get(), nullptr);
+      } else {
+        ResidualKernel<V><<<grid2_, kBlk, 0, st>>>(nnz_s2_, s2r, s2c, map, atp, ati, atv_.get(), sp, si, M_.get(),
+                                                   prv_.get(), R_.get(), Z_.get(), P_.get(), mode, part_.get(),
+                                                   flags_.get());
+      }
+    };
+    residual(method_ == SPAI_GLOBAL_CG ? 1 : 0);
+    AppendInitial<<<1, kBlk, 0, st>>>(part_.get(), Gp, tol, hist_.get(), sc_.get(), flags_.get());
     for (int64_t it = 1; it <= prm_.max_iterations; ++it) {
       if (method_ == SPAI_GLOBAL_LOMR) {
-        LomrProject<V><<<grid2_, kBlk, 0, st>>>(nnz_s2_, s2r, s2c, atp, ati, atv_.get(), sp, si, Z_.get(), P_.get(),
-                                                R_.get(), prv_.get(), AZ_.get(), AP_.get(), part_.get(),
-                                                flags_.get());
-        LomrSteps<<<1, kBlk, 0, st>>>(part_.get(), G2, it == 1 ? 1 : 0, sc_.get(), flags_.get());
-        LomrUpdate<V><<<grid2_, kBlk, 0, st>>>(nnz_s2_, s2r, map, AZ_.get(), AP_.get(), prv_.get(), R_.get(),
-                                               M_.get(), Z_.get(), P_.get(), sc_.get(), part_.get(), flags_.get());
-        AppendResidual<<<1, kBlk, 0, st>>>(part_.get(), G2, tol, hist_.get(), flags_.get());
+        if (use_plan_) {
+          // products are formed twice (project, then update) instead of storing A Z and A P
+          Slot<2>(LomrProjectEpi<V>{R_.get(), prv_.get()}, Z_.get(), P_.get());
+          LomrSteps<<<1, kBlk, 0, st>>>(part_.get(), Gp, it == 1 ? 1 : 0, sc_.get(), flags_.get());
+          LomrUpdateEpi<V> ue{};
+          ue.s2_to_s = map;
+          ue.pr = prv_.get();
+          ue.R = R_.get();
+          ue.M = M_.get();
+          ue.Z = Z_.get();
+          ue.P = P_.get();
+          Slot<2>(ue, Z_.get(), P_.get());
+          AppendResidual<<<1, kBlk, 0, st>>>(part_.get(), Gp, tol, hist_.get(), flags_.get());
+        } else {
+          LomrProject<V><<<grid2_, kBlk, 0, st>>>(nnz_s2_, s2r, s2c, atp, ati, atv_.get(), sp, si, Z_.get(),
+                                                  P_.get(), R_.get(), prv_.get(), AZ_.get(), AP_.get(), part_.get(),
+                                                  flags_.get());
+          LomrSteps<<<1, kBlk, 0, st>>>(part_.get(), G2, it == 1 ? 1 : 0, sc_.get(), flags_.get());
+          LomrUpdate<V><<<grid2_, kBlk, 0, st>>>(nnz_s2_, s2r, map, AZ_.get(), AP_.get(), prv_.get(), R_.get(),
+                                                 M_.get(), Z_.get(), P_.get(), sc_.get(), part_.get(), flags_.get());
+          AppendResidual<<<1, kBlk, 0, st>>>(part_.get(), G2, tol, hist_.get(), flags_.get());
+        }
       } else if (method_ == SPAI_GLOBAL_MR) {
-        MrProject<V><<<grid2_, kBlk, 0, st>>>(nnz_s2_, s2r, s2c, atp, ati, atv_.get(), sp, si, Z_.get(), R_.get(),
-                                              prv_.get(), part_.get(), flags_.get());
-        MrStep<<<1, kBlk, 0, st>>>(part_.get(), G2, sc_.get(), flags_.get());
+        if (use_plan_) {
+          Slot<1>(MrProjectEpi<V>{R_.get(), prv_.get()}, Z_.get(), nullptr);
+        } else {
+          MrProject<V><<<grid2_, kBlk, 0, st>>>(nnz_s2_, s2r, s2c, atp, ati, atv_.get(), sp, si, Z_.get(), R_.get(),
+                                                prv_.get(), part_.get(), flags_.get());
+        }
+        MrStep<<<1, kBlk, 0, st>>>(part_.get(), Gp, sc_.get(), flags_.get());
         AxpyS<V><<<grid1_, kBlk, 0, st>>>(nnz_s_, M_.get(), Z_.get(), sc_.get(), flags_.get());
-        ResidualKernel<V><<<grid2_, kBlk, 0, st>>>(nnz_s2_, s2r, s2c, map, atp, ati, atv_.get(), sp, si, M_.get(),
-                                                   prv_.get(), R_.get(), Z_.get(), P_.get(), 0, part_.get(),
-                                                   flags_.get());
-        AppendResidual<<<1, kBlk, 0, st>>>(part_.get(), G2, tol, hist_.get(), flags_.get());
+        residual(0);
+        AppendResidual<<<1, kBlk, 0, st>>>(part_.get(), Gp, tol, hist_.get(), flags_.get());
       } else {
-        CgProject<V><<<grid2_, kBlk, 0, st>>>(nnz_s2_, s2r, s2c, map, atp, ati, atv_.get(), sp, si, P_.get(),
-                                              AP_.get(), part_.get(), flags_.get());
-        CgAlpha<<<1, kBlk, 0, st>>>(part_.get(), G2, sc_.get(), flags_.get());
+        if (use_plan_) {
+          Slot<1>(CgProjectEpi<V>{map, P_.get(), AP_.get()}, P_.get(), nullptr);
+        } else {
+          CgProject<V><<<grid2_, kBlk, 0, st>>>(nnz_s2_, s2r, s2c, map, atp, ati, atv_.get(), sp, si, P_.get(),
+                                                AP_.get(), part_.get(), flags_.get());
+        }
+        CgAlpha<<<1, kBlk, 0, st>>>(part_.get(), Gp, sc_.get(), flags_.get());
         CgUpdate<V><<<grid2_, kBlk, 0, st>>>(nnz_s2_, s2r, map, AP_.get(), prv_.get(), R_.get(), M_.get(), P_.get(),
                                              Z_.get(), sc_.get(), part_.get(), flags_.get());
         CgAppendBeta<<<1, kBlk, 0, st>>>(part_.get(), G2, tol, hist_.get(), sc_.get(), flags_.get());
@@ -686,7 +997,43 @@ class FixedImpl final : public FixedProblem {
   DMat A_, m0_;
   PatternPtr at_pat_, s_pat_, s2_pat_;
   DBuf<double> at_d_, pr_d_;
-  DBuf<V> atv_, prv_, M_, Z_, P_, R_, AZ_, AP_;
+  DBuf<V> atv_, av_, prv_, M_, Z_, P_, R_, AZ_, AP_;
+  // slot plan
+  DBuf<int64_t> plan_ptr_;
+  DBuf<uint8_t> plan8_;
+  DBuf<uint16_t> plan16_;
+  int maxcol_ = 1, lmax_ = 1, gl_ = 8;
+  size_t smem_ = 0;
+  bool slot16_ = false, use_plan_ = false;
+  unsigned gridp_ = 1;
+
+  // Launches the slot product with epilogue e over Y = (y0, y1).
+  template <int NY, class Epi>
+  void Slot(const Epi& e, const V* y0, const V* y1) {
+    hipStream_t st = ctx_.stream;
+    const int* sp = s_pat_->ptr.get();
+    const int* si = s_pat_->idx.get();
+    const int* s2p = s2_pat_->ptr.get();
+    const int* s2i = s2_pat_->idx.get();
+#define SPAI_SLOT_LAUNCH(SlotT, plan, GLV)                                                                     \
+  SlotProduct<V, SlotT, GLV, NY, Epi><<<gridp_, kBlk, smem_, st>>>((int)n_, sp, si, y0, y1, A_.ptr(), av_.get(), \
+                                                                   plan_ptr_.get(), plan, s2p, s2i, lmax_, e,  \
+                                                                   sc_.get(), part_.get(), flags_.get())
+#define SPAI_SLOT_GL(SlotT, plan)             \
+  switch (gl_) {                              \
+    case 8: SPAI_SLOT_LAUNCH(SlotT, plan, 8); break;   \
+    case 16: SPAI_SLOT_LAUNCH(SlotT, plan, 16); break; \
+    case 32: SPAI_SLOT_LAUNCH(SlotT, plan, 32); break; \
+    default: SPAI_SLOT_LAUNCH(SlotT, plan, 64); break; \
+  }
+    if (slot16_) {
+      SPAI_SLOT_GL(uint16_t, plan16_.get());
+    } else {
+      SPAI_SLOT_GL(uint8_t, plan8_.get());
+    }
+#undef SPAI_SLOT_GL
+#undef SPAI_SLOT_LAUNCH
+  }
   DBuf<int> s2col_, s2_to_s_;
   DBuf<double> part_, sc_, hist_;
   DBuf<Flags> flags_;

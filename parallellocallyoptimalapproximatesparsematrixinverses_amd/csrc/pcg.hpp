@@ -37,7 +37,8 @@ class PcgSolver {
   double TimeSpmvA(int reps);
 
  private:
-  void Record(bool fused, int64_t iterations);
+  void Record(int first_parity, int64_t iterations);
+  void DropGraphs();
   Ctx& ctx_;
   const DCsr& A_;
   const DCsr& M_;
@@ -48,8 +49,8 @@ class PcgSolver {
   DBuf<double> scal_;
   DBuf<int> ist_;
   const double* b_ = nullptr;
-  hipGraph_t graph_ = nullptr;
-  hipGraphExec_t exec_ = nullptr;
+  hipGraph_t graph_[2] = {nullptr, nullptr};
+  hipGraphExec_t exec_[2] = {nullptr, nullptr};
   const double* graph_key_b_ = nullptr;
 };
 
