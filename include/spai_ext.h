@@ -118,6 +118,29 @@ struct Result spai_session_result(SpaiSession* session);
  * one SpMV with A (HIP events around a replay of the iteration graph). */
 double spai_session_pcg_iteration_us(SpaiSession* session);
 
+/* ------------------------------------------------------------------------
+ * Differentiation form: f(M, A) = ||I - A M||_F^2 on fixed patterns.
+ * df/dM = -2 (A^T R) sampled on pattern(M), df/dA = -2 (R M^T) sampled on
+ * pattern(A), R = I - A M.  Values are device pointers in the CSC order of the
+ * patterns given at creation (FP64, or FP32 when precision = 1, with FP64
+ * reductions).  R is formed per block of block_cols columns (0 = all at once)
+ * and never stored whole.  Work runs on `stream` (a hipStream_t, 0 = null
+ * stream).  No counterpart in the reference, whose closest quantity is the
+ * sampled A R of its drop score (src/common/dropping.cpp:19-29).
+ * ------------------------------------------------------------------------ */
+typedef struct SpaiLossPlan SpaiLossPlan;
+
+SpaiLossPlan* spai_loss_plan_create(struct MatrixData A, struct MatrixData M, int64_t precision, int64_t block_cols,
+                                    uint64_t stream);
+void spai_loss_plan_destroy(SpaiLossPlan* plan);
+/* out4: nnz(pattern(A M) | I), largest block of R, number of blocks, nnz(M). */
+int64_t spai_loss_plan_info(SpaiLossPlan* plan, int64_t* out4);
+int64_t spai_loss_forward(SpaiLossPlan* plan, const void* m_values_dev, const void* a_values_dev, double* loss_out);
+/* grad_m = grad_scale * df/dM, grad_a = grad_scale * df/dA (device outputs). */
+int64_t spai_loss_backward(SpaiLossPlan* plan, const void* m_values_dev, const void* a_values_dev, double grad_scale,
+                           void* grad_m_dev, void* grad_a_dev);
+const char* spai_loss_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,7 +19,10 @@ struct Ctx {
   size_t pinned_bytes = 0;
 
   Ctx();
+  // Runs on a caller-owned stream (not destroyed with the context).
+  explicit Ctx(hipStream_t external);
   ~Ctx();
+  bool owns_stream = true;
   Ctx(const Ctx&) = delete;
   Ctx& operator=(const Ctx&) = delete;
 

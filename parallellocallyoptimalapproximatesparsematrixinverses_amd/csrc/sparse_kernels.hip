@@ -886,11 +886,20 @@ Ctx::Ctx() {
   SPAI_HIP_CHECK(hipHostMalloc(&pinned, pinned_bytes, hipHostMallocDefault));
 }
 
+Ctx::Ctx(hipStream_t external) {
+  SPAI_HIP_CHECK(hipGetDevice(&device));
+  hipDeviceProp_t prop;
+  SPAI_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+  num_cus = prop.multiProcessorCount;
+  stream = external;
+  owns_stream = false;
+  pinned_bytes = 1 << 16;
+  SPAI_HIP_CHECK(hipHostMalloc(&pinned, pinned_bytes, hipHostMallocDefault));
+}
+
 Ctx::~Ctx() {
-  if (stream) {
-    (void)hipStreamSynchronize(stream);
-    (void)hipStreamDestroy(stream);
-  }
+  (void)hipStreamSynchronize(stream);
+  if (owns_stream && stream) (void)hipStreamDestroy(stream);
   if (pinned) (void)hipHostFree(pinned);
 }
 

@@ -39,3 +39,19 @@ def assert_same_result(g, o, hist_rtol=1e-10, m_rtol=1e-12, same_pattern=True, d
         np.testing.assert_array_equal(Mg.indptr, Mo.indptr)
         np.testing.assert_array_equal(Mg.indices, Mo.indices)
     assert rel_fro(g["M"], o["M"]) <= m_rtol, rel_fro(g["M"], o["M"])
+
+
+def dense_loss_reference(A, M):
+    """numpy FP64 f = ||I - A M||_F^2 and its gradients sampled on the patterns
+    (df/dM = -2 A^T R on pattern(M), df/dA = -2 R M^T on pattern(A)), CSC order."""
+    A = sparse.csc_matrix(A)
+    M = sparse.csc_matrix(M)
+    A.sort_indices()
+    M.sort_indices()
+    Ad, Md = A.toarray(), M.toarray()
+    R = np.eye(A.shape[0]) - Ad @ Md
+    GM = -2.0 * (Ad.T @ R)
+    GA = -2.0 * (R @ Md.T)
+    gm = np.array([GM[M.indices[p], j] for j in range(M.shape[1]) for p in range(M.indptr[j], M.indptr[j + 1])])
+    ga = np.array([GA[A.indices[p], j] for j in range(A.shape[1]) for p in range(A.indptr[j], A.indptr[j + 1])])
+    return float((R * R).sum()), gm, ga

@@ -1,0 +1,137 @@
+"""Differentiation form f(M, A) = ||I - A M||_F^2: device forward/backward
+against dense numpy, central finite differences, the reference's own sampled
+product A R (its drop-score term, src/common/dropping.cpp:19-29), blocked vs
+unblocked R, and FP32 against FP64."""
+import numpy as np
+import pytest
+import torch
+from scipy import sparse
+
+from helpers import dense_loss_reference
+from parallellocallyoptimalapproximatesparsematrixinverses_amd import problems
+
+pytestmark = pytest.mark.gpu
+
+
+def spd(n, per_col, seed):
+    rng = np.random.default_rng(seed)
+    rows = rng.integers(0, n, size=n * per_col)
+    cols = np.repeat(np.arange(n), per_col)
+    B = sparse.csc_matrix((rng.standard_normal(n * per_col), (rows, cols)), shape=(n, n))
+    A = (B.T @ B + sparse.identity(n)).tocsc()
+    A.sort_indices()
+    return A
+
+
+def with_values(P, seed):
+    P = sparse.csc_matrix(P, dtype=np.float64, copy=True)
+    P.sort_indices()
+    P.data = np.random.default_rng(seed).standard_normal(P.nnz) * 0.1
+    return P
+
+
+def dev(x, dtype=torch.float64):
+    return torch.tensor(np.asarray(x), dtype=dtype, device="cuda")
+
+
+@pytest.fixture(scope="module")
+def small():
+    A = spd(60, 3, 11)
+    M = with_values(A + sparse.identity(60), 5)
+    return A, M
+
+
+def test_forward_backward_dense(small):
+    from parallellocallyoptimalapproximatesparsematrixinverses_amd.autograd import ResidualLoss
+    A, M = small
+    f_ref, gm_ref, ga_ref = dense_loss_reference(A, M)
+    plan = ResidualLoss(A, M, torch.float64)
+    f = plan.forward_value(dev(M.data), dev(A.data))
+    gm, ga = plan.gradients(dev(M.data), dev(A.data))
+    assert f == pytest.approx(f_ref, rel=1e-13)
+    np.testing.assert_allclose(gm.cpu().numpy(), gm_ref, rtol=1e-11, atol=1e-12)
+    np.testing.assert_allclose(ga.cpu().numpy(), ga_ref, rtol=1e-11, atol=1e-12)
+
+
+def test_autograd_function(small):
+    from parallellocallyoptimalapproximatesparsematrixinverses_amd.autograd import ResidualLoss
+    A, M = small
+    plan = ResidualLoss(A, M, torch.float64)
+    m = dev(M.data).requires_grad_(True)
+    a = dev(A.data).requires_grad_(True)
+    loss = plan(m, a)
+    (3.0 * loss).backward()
+    gm, ga = plan.gradients(dev(M.data), dev(A.data), 3.0)
+    torch.testing.assert_close(m.grad, gm, rtol=0, atol=0)
+    torch.testing.assert_close(a.grad, ga, rtol=0, atol=0)
+
+
+def test_blocked_matches_unblocked(small):
+    from parallellocallyoptimalapproximatesparsematrixinverses_amd.autograd import ResidualLoss
+    A, M = small
+    full = ResidualLoss(A, M, torch.float64, block_cols=0)
+    blk = ResidualLoss(A, M, torch.float64, block_cols=7)
+    assert blk.info()["blocks"] == 9 and full.info()["blocks"] == 1
+    f0, f1 = full.forward_value(dev(M.data), dev(A.data)), blk.forward_value(dev(M.data), dev(A.data))
+    assert f1 == pytest.approx(f0, rel=1e-14)
+    g0 = full.gradients(dev(M.data), dev(A.data))
+    g1 = blk.gradients(dev(M.data), dev(A.data))
+    torch.testing.assert_close(g1[0], g0[0], rtol=0, atol=0)
+    torch.testing.assert_close(g1[1], g0[1], rtol=1e-14, atol=1e-14)
+
+
+def test_central_differences():
+    """64 random entries of M and of A, h = 1e-6 (BASELINE config 3 check, small size)."""
+    from parallellocallyoptimalapproximatesparsematrixinverses_amd.autograd import ResidualLoss
+    A = spd(400, 6, 3)
+    M = with_values(A, 9)
+    plan = ResidualLoss(A, M, torch.float64)
+    m0, a0 = dev(M.data), dev(A.data)
+    gm, ga = (g.cpu().numpy() for g in plan.gradients(m0, a0))
+    rng = np.random.default_rng(0)
+    h = 1e-6
+    f0 = plan.forward_value(m0, a0)
+    # f is quadratic in each entry, so central differences are exact up to the
+    # rounding of f itself (about eps * f per evaluation).
+    tol_abs = 64 * np.finfo(float).eps * f0 / h
+    for vec, grad, which in ((M.data, gm, "m"), (A.data, ga, "a")):
+        for idx in rng.choice(vec.size, 64, replace=False):
+            plus, minus = vec.copy(), vec.copy()
+            plus[idx] += h
+            minus[idx] -= h
+            if which == "m":
+                fp, fm = plan.forward_value(dev(plus), a0), plan.forward_value(dev(minus), a0)
+            else:
+                fp, fm = plan.forward_value(m0, dev(plus)), plan.forward_value(m0, dev(minus))
+            fd = (fp - fm) / (2 * h)
+            assert abs(fd - grad[idx]) <= tol_abs + 1e-7 * abs(grad[idx]), (which, idx, fd, grad[idx], tol_abs)
+
+
+def test_grad_m_is_reference_sampled_product(oracle_lib):
+    """For symmetric A, df/dM = -2 (A R) on pattern(M), with A R the reference's
+    conservative product (its drop score's G term)."""
+    from parallellocallyoptimalapproximatesparsematrixinverses_amd.autograd import ResidualLoss
+    A = problems.diffusion3d(6, seed=1)
+    M = with_values(A, 4)
+    plan = ResidualLoss(A, M, torch.float64)
+    gm = plan.gradients(dev(M.data), dev(A.data))[0].cpu().numpy()
+    R = oracle_lib.product(A, M, residual=True)
+    AR = oracle_lib.product(A, R).tocsc()
+    Mc = M.tocsc()
+    G = np.array([AR[Mc.indices[p], j] for j in range(Mc.shape[1]) for p in range(Mc.indptr[j], Mc.indptr[j + 1])])
+    np.testing.assert_allclose(gm, -2.0 * G, rtol=1e-12, atol=1e-13)
+
+
+def test_fp32_against_fp64():
+    from parallellocallyoptimalapproximatesparsematrixinverses_amd.autograd import ResidualLoss
+    A = problems.diffusion3d(12, seed=4)
+    M = with_values(A, 2)
+    p64 = ResidualLoss(A, M, torch.float64, block_cols=500)
+    p32 = ResidualLoss(A, M, torch.float32, block_cols=500)
+    f64 = p64.forward_value(dev(M.data), dev(A.data))
+    f32 = p32.forward_value(dev(M.data, torch.float32), dev(A.data, torch.float32))
+    assert abs(f32 - f64) / f64 < 1e-5
+    g64 = p64.gradients(dev(M.data), dev(A.data))
+    g32 = p32.gradients(dev(M.data, torch.float32), dev(A.data, torch.float32))
+    for a, b in zip(g32, g64):
+        assert torch.linalg.norm(a.double() - b) / torch.linalg.norm(b) < 1e-5
