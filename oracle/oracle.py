@@ -140,6 +140,9 @@ def lib():
                                      ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                      ctypes.POINTER(ctypes.c_double)]
         L.oracle_solve_ls2x2.argtypes = [ctypes.POINTER(ctypes.c_double)] * 3
+        L.oracle_column_topk.restype = ctypes.c_int64
+        L.oracle_column_topk.argtypes = [MatrixData, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64),
+                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]
         L.oracle_pcg.restype = ctypes.c_int64
         L.oracle_pcg.argtypes = [CsrData, CsrData, ctypes.POINTER(ctypes.c_double), ctypes.c_int64, ctypes.c_double,
                                  ctypes.c_int64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
@@ -225,6 +228,21 @@ def product(X, Y, residual=False, prune_zero=False) -> sparse.csc_matrix:
                      idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                      val.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     return sparse.csc_matrix((val[:nnz], idx[:nnz], ptr), shape=(X.shape[0], Y.shape[1]))
+
+
+def column_topk(X, k) -> sparse.csc_matrix:
+    """Per-column top-k (diagonal kept) as the oracle's extension applies it."""
+    keep = []
+    x = _md(as_csc(X), keep)
+    L = lib()
+    nnz = L.oracle_column_topk(x, int(k), None, None, None)
+    ptr = np.zeros(X.shape[1] + 1, np.int64)
+    idx = np.zeros(max(nnz, 1), np.int64)
+    val = np.zeros(max(nnz, 1), np.float64)
+    L.oracle_column_topk(x, int(k), ptr.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                         idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                         val.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return sparse.csc_matrix((val[:nnz], idx[:nnz], ptr), shape=X.shape)
 
 
 def solve_ls2x2(m, x):

@@ -1301,6 +1301,22 @@ int64_t oracle_product(MatrixData X, MatrixData Y, int64_t residual, int64_t pru
   }
 }
 
+// Per-column top-k of X (diagonal kept), exposed for pinning.
+int64_t oracle_column_topk(MatrixData X, int64_t k, int64_t* out_ptr, int64_t* out_idx, double* out_val) {
+  try {
+    Csc x = FromData(X);
+    State::KeepColumnTopK(x, k, true);
+    if (out_ptr) {
+      std::copy(x.ptr.begin(), x.ptr.end(), out_ptr);
+      std::copy(x.idx.begin(), x.idx.end(), out_idx);
+      std::copy(x.val.begin(), x.val.end(), out_val);
+    }
+    return x.nnz();
+  } catch (...) {
+    return -1;
+  }
+}
+
 // SolveLeastSquares2x2 restatement, exposed for pinning.
 void oracle_solve_ls2x2(const double* m4, const double* x2, double* out2) {
   const double m[2][2] = {{m4[0], m4[1]}, {m4[2], m4[3]}};

@@ -240,11 +240,12 @@ class Session:
     """Device-resident build + PCG for one A (fixed-pattern forms)."""
 
     def __init__(self, method, A, M0=None, Pr=None, max_iterations=30, tolerance=0.0, max_density=1.0,
-                 num_threads=1, precision="fp64"):
+                 num_threads=1, precision="fp64", inner_iterations=2):
         L = load()
         self._keep = [as_csc(A), as_csc(M0) if M0 is not None else None, as_csc(Pr) if Pr is not None else None]
         self.n = A.shape[0]
-        params = _params(max_iterations, tolerance, max_density, False, num_threads, M0 is not None, Pr is not None, 2)
+        params = _params(max_iterations, tolerance, max_density, False, num_threads, M0 is not None, Pr is not None,
+                         inner_iterations)
         ext = SpaiExtParams(1, 1 if precision == "fp32" else 0, 0)
         self._h = L.spai_session_create(METHOD_IDS[method], matrix_data(self._keep[0]), matrix_data(self._keep[1]),
                                         matrix_data(self._keep[2]), params, ext)

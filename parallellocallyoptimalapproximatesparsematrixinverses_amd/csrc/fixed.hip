@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
+#include <type_traits>
 #include <vector>
 
 #include "dev_common.hpp"
@@ -369,6 +370,127 @@ struct CgProjectEpi {
   }
 };
 
+// ------------------------------------------------------------ inner-outer
+// Z = M_outer R sampled on S: (i, j) -> sum_k M_outer(i, k) R(k, j) over k
+// ascending (row i of M_outer = column i of S^T, gathered through stperm,
+// against column j of R on S2).
+template <class V>
+__global__ __launch_bounds__(kBlk) void SampledOuterProduct(int64_t nnz_s, const int* __restrict__ si,
+                                                            const int* __restrict__ scol, const int* __restrict__ stp,
+                                                            const int* __restrict__ sti,
+                                                            const int* __restrict__ stperm,
+                                                            const V* __restrict__ mout, const int* __restrict__ s2p,
+                                                            const int* __restrict__ s2i, const V* __restrict__ R,
+                                                            V* __restrict__ Z) {
+  for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < nnz_s; p += int64_t(gridDim.x) * blockDim.x) {
+    const int i = si[p], j = scol[p];
+    int a = stp[i], ae = stp[i + 1], b = s2p[j], be = s2p[j + 1];
+    V acc = V(0);
+    while (a < ae && b < be) {
+      const int ka = sti[a], kb = s2i[b];
+      if (ka == kb) {
+        acc += mout[stperm[a]] * R[b];
+        ++a;
+        ++b;
+      } else if (ka < kb) {
+        ++a;
+      } else {
+        ++b;
+      }
+    }
+    Z[p] = acc;
+  }
+}
+
+// Stores A Z and A q on S2.
+template <class V>
+struct StoreTwoEpi {
+  static constexpr int K = 1;
+  V* AZ;
+  V* AQ;
+  __device__ void Load(const double*) {}
+  __device__ void operator()(int64_t q, int, int, V az, V aq, double*) const {
+    AZ[q] = az;
+    if (AQ) AQ[q] = aq;
+  }
+};
+
+// Per-column step sizes of inner_outer_common.hpp:50-69: sequential column dots
+// over S2 (row order), delta = <r, az> / ||az||^2 or the 2x2 solve, and the
+// normalisation 1 / delta.  FP64 throughout.
+template <class V>
+__global__ void IoColumnSteps(int n, const int* __restrict__ s2p, const V* __restrict__ R, const V* __restrict__ AZ,
+                              const V* __restrict__ AQ, int lomr, int two_term, double rank_eps,
+                              double* __restrict__ delta, double* __restrict__ gamma, double* __restrict__ norm) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double r_az = 0.0, az_sq = 0.0, az_aq = 0.0, aq_sq = 0.0, r_aq = 0.0;
+  for (int q = s2p[j]; q < s2p[j + 1]; ++q) {
+    const double r = R[q], az = AZ[q];
+    r_az += r * az;
+    az_sq += az * az;
+    if (two_term) {
+      const double aq = AQ[q];
+      az_aq += az * aq;
+      aq_sq += aq * aq;
+      r_aq += r * aq;
+    }
+  }
+  double d, g;
+  if (!two_term) {
+    d = (isfinite(az_sq) && fabs(az_sq) > 0.0) ? r_az / az_sq : 0.0;
+    if (!isfinite(d)) d = 0.0;
+    g = lomr ? 1.0 : 0.0;
+  } else {
+    SolveLeastSquares2x2(az_sq, az_aq, az_aq, aq_sq, r_az, r_aq, d, g, rank_eps);
+  }
+  delta[j] = d;
+  gamma[j] = g;
+  const double inv = d != 0.0 ? 1.0 / d : 1.0;
+  norm[j] = isfinite(inv) ? inv : 1.0;
+}
+
+// P = Z diag(delta) + q diag(gamma) (inner_outer_common.hpp:70-75), formed in
+// FP64 and rounded once: near-collinear columns give large delta and gamma of
+// opposite sign whose products cancel.
+template <class V>
+__global__ void IoDirection(int64_t nnz_s, const int* __restrict__ scol, const V* __restrict__ Z,
+                            const V* __restrict__ Q, const double* __restrict__ delta,
+                            const double* __restrict__ gamma, int lomr, V* __restrict__ P) {
+  for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < nnz_s; p += int64_t(gridDim.x) * blockDim.x) {
+    const int j = scol[p];
+    double v = double(Z[p]) * delta[j];
+    if (lomr) v = v + 1.0 * (double(Q[p]) * gamma[j]);
+    P[p] = V(v);
+  }
+}
+
+// R -= A P; M += P; q = P diag(1/delta) (inner_outer_common.hpp:90-98).
+template <class V>
+struct IoUpdateEpi {
+  static constexpr int K = 1;
+  const int* s2_to_s;
+  const int* ncol_unused;
+  const V* P;
+  const double* norm;
+  V* R;
+  V* M;
+  V* Q;
+  int lomr;
+  __device__ void Load(const double*) {}
+  __device__ void operator()(int64_t q, int, int j, V ap, V, double* acc) const {
+    const V r = R[q] + V(-1) * ap;
+    R[q] = r;
+    acc[0] += double(r) * double(r);
+    const int p = s2_to_s[q];
+    if (p >= 0) {
+      const V pv = P[p];
+      M[p] = M[p] + V(1) * pv;
+      if (lomr) Q[p] = pv * V(norm[j]);
+    }
+  }
+};
+
 // --------------------------------------------------------------- residual
 // R = I - A M on S2 (value -(sum) + [i == j]); Z = Pr R sampled on S.
 // mode 1 (CG setup) also sets P = Z and accumulates <R, Z>.
@@ -706,6 +828,16 @@ __global__ void SampleOnPattern(int ncols, const int* __restrict__ sp, const int
   }
 }
 
+__global__ void IotaD(int64_t n, double* __restrict__ v) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = double(i);
+}
+
+__global__ void ToIntD(int64_t n, const double* __restrict__ v, int* __restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = int(v[i]);
+}
+
 template <class V>
 __global__ void ConvertValues(int64_t n, const double* __restrict__ in, V* __restrict__ out) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -742,8 +874,9 @@ class FixedImpl final : public FixedProblem {
   FixedImpl(int64_t method, const MatrixData& Ah, const MatrixData* M0h, const MatrixData* Prh,
             const EngineParams& p)
       : method_(method), prm_(p) {
-    if (method != SPAI_GLOBAL_CG && method != SPAI_GLOBAL_MR && method != SPAI_GLOBAL_LOMR)
-      throw std::invalid_argument("fixed_pattern supports the global CG, MR and LOMR methods");
+    if (method < SPAI_GLOBAL_CG || method > SPAI_INNER_OUTER_LOMR) throw std::invalid_argument("unknown method");
+    if (method >= SPAI_INNER_OUTER_MR && p.inner_iterations <= 0)
+      throw std::invalid_argument("inner_iterations must be positive");
     Ctx& c = ctx_;
     if (Ah.rows != Ah.cols || Ah.rows == 0) throw std::invalid_argument("A must be a non-empty square matrix");
     n_ = Ah.cols;
@@ -852,6 +985,23 @@ class FixedImpl final : public FixedProblem {
       av_ = c.Alloc<V>(A_.nnz());
       ConvertValues<V><<<Blocks1(A_.nnz()), kBlk, 0, c.stream>>>(A_.nnz(), A_.v(), av_.get());
     }
+    if (method_ >= SPAI_INNER_OUTER_MR) {
+      if (!use_plan_) throw std::runtime_error("inner-outer fixed_pattern needs S2 columns of at most 65535 rows");
+      // S^T with positions (rows of M for the sampled product M_outer R)
+      DMat P;
+      P.pat = s_pat_;
+      P.val = c.Alloc<double>(nnz_s_);
+      IotaD<<<Blocks1(nnz_s_), kBlk, 0, c.stream>>>(nnz_s_, P.val.get());
+      DMat T = Transpose(c, P);
+      st_pat_ = T.pat;
+      stperm_ = c.Alloc<int>(nnz_s_);
+      ToIntD<<<Blocks1(nnz_s_), kBlk, 0, c.stream>>>(nnz_s_, T.val.get(), stperm_.get());
+      scol_ = c.Alloc<int>(nnz_s_);
+      ColumnOfEntry<<<Blocks1(n_), kBlk, 0, c.stream>>>((int)n_, s_pat_->ptr.get(), scol_.get());
+      Mout_ = c.Alloc<V>(nnz_s_);
+      Q_ = c.Alloc<V>(nnz_s_);
+      io_cols_ = c.Alloc<double>(3 * n_);
+    }
     // value arrays
     atv_ = c.Alloc<V>(at_pat_->nnz);
     ConvertValues<V><<<Blocks1(at_pat_->nnz), kBlk, 0, c.stream>>>(at_pat_->nnz, at_d_.get(), atv_.get());
@@ -875,7 +1025,81 @@ class FixedImpl final : public FixedProblem {
   Ctx& context() override { return ctx_; }
   const DMat& MatrixA() const override { return A_; }
 
+  // Fixed-order host sum of the G block partials of slot k.
+  double HostPartialSum(int G, int k) {
+    std::vector<double> h(G);
+    ctx_.ReadBack(h.data(), part_.get() + size_t(k) * G, sizeof(double) * G);
+    double t = 0.0;
+    for (double v : h) t += v;
+    return t;
+  }
+
+  // inner_outer::Run (src/inner_outer/inner_outer_common.hpp:14-107) on the
+  // fixed pattern: Z = M_outer R sampled on S, per-column step sizes,
+  // StabilizeResiduals (state_manager.cpp:81-95).  Host-driven (one norm
+  // read-back per inner iteration).
+  int64_t BuildInnerOuter() {
+    Ctx& c = ctx_;
+    hipStream_t st = c.stream;
+    const bool lomr = method_ == SPAI_INNER_OUTER_LOMR;
+    const int* map = s2_to_s_.get();
+    SampleOnPattern<V><<<Blocks1(n_), kBlk, 0, st>>>((int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), m0_.ptr(),
+                                                     m0_.idx(), m0_.v(), M_.get());
+    SPAI_HIP_CHECK(hipMemsetAsync(flags_.get(), 0, sizeof(Flags), st));
+    const double tol = prm_.tolerance;
+    const double thr = std::max(10.0 * tol, 64.0 * 2.220446049250313e-16 * std::sqrt((double)n_));
+    // FP32 data: pivots of the scaled per-column Gram below sqrt(FLT_EPSILON)
+    // of the largest are noise (cos(az, aq) within ~1.5 degrees of +-1).
+    const double rank_eps = std::is_same<V, float>::value ? 3.4526698300124393e-04 : 2.220446049250313e-16;
+    auto residual = [&]() {
+      Slot<1>(ResidualEpi<V>{map, prv_.get(), R_.get(), Z_.get(), P_.get(), 0}, M_.get(), nullptr);
+      return std::sqrt(HostPartialSum((int)gridp_, 0));
+    };
+    io_hist_.clear();
+    io_converged_ = false;
+    double rn = residual();
+    io_hist_.push_back(rn);
+    if (rn < tol) {
+      io_converged_ = true;
+      return 0;
+    }
+    double* delta = io_cols_.get();
+    double* gamma = delta + n_;
+    double* norm = gamma + n_;
+    const int* s2p = s2_pat_->ptr.get();
+    for (int64_t it = 0; it < prm_.max_iterations;) {
+      SPAI_HIP_CHECK(hipMemcpyAsync(Mout_.get(), M_.get(), sizeof(V) * nnz_s_, hipMemcpyDeviceToDevice, st));
+      residual();
+      FillZero<V><<<Blocks1(nnz_s_), kBlk, 0, st>>>(nnz_s_, Q_.get());
+      const int64_t inner_count = std::min(prm_.inner_iterations, prm_.max_iterations - it);
+      for (int64_t inner = 0; inner < inner_count; ++inner, ++it) {
+        SampledOuterProduct<V><<<GridFor(nnz_s_), kBlk, 0, st>>>(
+            nnz_s_, s_pat_->idx.get(), scol_.get(), st_pat_->ptr.get(), st_pat_->idx.get(), stperm_.get(),
+            Mout_.get(), s2p, s2_pat_->idx.get(), R_.get(), Z_.get());
+        if (lomr)
+          Slot<2>(StoreTwoEpi<V>{AZ_.get(), AP_.get()}, Z_.get(), Q_.get());
+        else
+          Slot<1>(StoreTwoEpi<V>{AZ_.get(), nullptr}, Z_.get(), nullptr);
+        IoColumnSteps<V><<<Blocks1(n_), kBlk, 0, st>>>((int)n_, s2p, R_.get(), AZ_.get(), AP_.get(), lomr ? 1 : 0,
+                                                       (lomr && inner > 0) ? 1 : 0, rank_eps, delta, gamma, norm);
+        IoDirection<V><<<GridFor(nnz_s_), kBlk, 0, st>>>(nnz_s_, scol_.get(), Z_.get(), Q_.get(), delta, gamma,
+                                                         lomr ? 1 : 0, P_.get());
+        Slot<1>(IoUpdateEpi<V>{map, nullptr, P_.get(), norm, R_.get(), M_.get(), Q_.get(), lomr ? 1 : 0}, P_.get(),
+                nullptr);
+        rn = std::sqrt(HostPartialSum((int)gridp_, 0));
+        if (!(std::isfinite(rn) && rn > thr)) rn = residual();
+        io_hist_.push_back(rn);
+        if (rn < tol) {
+          io_converged_ = true;
+          return (int64_t)io_hist_.size() - 1;
+        }
+      }
+    }
+    return (int64_t)io_hist_.size() - 1;
+  }
+
   int64_t Build() override {
+    if (method_ >= SPAI_INNER_OUTER_MR) return BuildInnerOuter();
     Ctx& c = ctx_;
     hipStream_t st = c.stream;
     // M = M0 on S, P = 0
@@ -962,15 +1186,22 @@ class FixedImpl final : public FixedProblem {
 
   void Output(EngineOutput& out) override {
     Ctx& c = ctx_;
-    const Flags f = last_flags_;
-    out.iterations = f.iterations;
-    out.converged = f.stop == 2;
-    std::vector<double> h(f.iterations + 1);
-    c.ReadBack(h.data(), hist_.get(), sizeof(double) * h.size());
+    std::vector<double> h;
+    if (method_ >= SPAI_INNER_OUTER_MR) {
+      h = io_hist_;
+      out.iterations = (int64_t)h.size() - 1;
+      out.converged = io_converged_;
+    } else {
+      const Flags f = last_flags_;
+      out.iterations = f.iterations;
+      out.converged = f.stop == 2;
+      h.resize(f.iterations + 1);
+      c.ReadBack(h.data(), hist_.get(), sizeof(double) * h.size());
+    }
     const double entries = (double)n_ * (double)n_;
     const double dens = (double)nnz_s_ / entries;
     out.history.clear();
-    for (int64_t i = 0; i <= f.iterations; ++i) out.history.push_back(SingleIterationData{i, h[i], dens, dens});
+    for (int64_t i = 0; i <= out.iterations; ++i) out.history.push_back(SingleIterationData{i, h[i], dens, dens});
     DMat Md = MatrixM();
     out.M.rows = n_;
     out.M.cols = n_;
@@ -999,6 +1230,13 @@ class FixedImpl final : public FixedProblem {
   DBuf<double> at_d_, pr_d_;
   DBuf<V> atv_, av_, prv_, M_, Z_, P_, R_, AZ_, AP_;
   // slot plan
+  // inner-outer
+  PatternPtr st_pat_;
+  DBuf<int> stperm_, scol_;
+  DBuf<V> Mout_, Q_;
+  DBuf<double> io_cols_;
+  std::vector<double> io_hist_;
+  bool io_converged_ = false;
   DBuf<int64_t> plan_ptr_;
   DBuf<uint8_t> plan8_;
   DBuf<uint16_t> plan16_;

@@ -8,6 +8,11 @@
 // v = [1, e], beta = -sign(c0) * ||.||, tau = (beta - c0) / beta, rank =
 // #{|r_ii| > max pivot * 2 eps}), and for rank 1 one Householder reflection
 // from the right.  Any non-finite input or output yields (0, 0).
+//
+// rank_eps replaces eps in the rank decision only.  The default is the
+// reference's (DBL_EPSILON); FP32-bulk callers whose Gram entries carry FP32
+// rounding pass a larger one so that pivots below the data's precision count
+// as zero instead of producing large cancelling coefficients.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -48,7 +53,8 @@ __host__ __device__ inline void ApplyHouseholder2(double e, double tau, double& 
 
 // m = [[m00, m01], [m10, m11]], x = (x0, x1); result in (d, g).
 __host__ __device__ inline void SolveLeastSquares2x2(double m00, double m01, double m10, double m11, double x0,
-                                                     double x1, double& d, double& g) {
+                                                     double x1, double& d, double& g,
+                                                     double rank_eps = DBL_EPSILON) {
   d = 0.0;
   g = 0.0;
   if (!__builtin_isfinite(m00) || !__builtin_isfinite(m01) || !__builtin_isfinite(m10) || !__builtin_isfinite(m11) || !__builtin_isfinite(x0) || !__builtin_isfinite(x1)) return;
@@ -95,7 +101,7 @@ __host__ __device__ inline void SolveLeastSquares2x2(double m00, double m01, dou
   }
   if (nonzero_pivots == 2 && nu1 * nu1 < helper * 1.0) nonzero_pivots = 1;
   if (fabs(a11) > maxpivot) maxpivot = fabs(a11);
-  const double thr = maxpivot * (eps * 2.0);
+  const double thr = maxpivot * (rank_eps * 2.0);
   int rank = 0;
   if (nonzero_pivots > 0 && fabs(a00) > thr) ++rank;
   if (nonzero_pivots > 1 && fabs(a11) > thr) ++rank;
