@@ -202,8 +202,10 @@ int64_t spai_device_name(char* buffer, int64_t size) {
     SPAI_HIP_CHECK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     SPAI_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
-    const std::string s = std::string(prop.name) + " (" + prop.gcnArchName + ", " +
-                          std::to_string(prop.multiProcessorCount) + " CUs)";
+    // the marketing name can be empty under restricted device access
+    const std::string name = prop.name[0] != '\0' ? std::string(prop.name) : std::string("AMD GPU");
+    const std::string s = name + " (" + prop.gcnArchName + ", " + std::to_string(prop.multiProcessorCount) +
+                          " CUs, " + std::to_string(prop.totalGlobalMem >> 30) + " GiB)";
     if (buffer && size > 0) {
       std::strncpy(buffer, s.c_str(), (size_t)size - 1);
       buffer[size - 1] = '\0';

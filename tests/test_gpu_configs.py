@@ -128,3 +128,65 @@ def test_config5_stencil27_full_size(gpu_lib):
     assert g32["iterations"] == 10 and h64[-1] < 0.5 * h64[0]
     np.testing.assert_allclose(h32, h64, rtol=1e-6)
     assert rel_fro(g32["M"], g64["M"]) < 1e-5
+
+
+def _dev(x, dtype):
+    import torch
+    return torch.tensor(np.asarray(x), dtype=dtype, device="cuda")
+
+
+def test_config3_autograd_full_size(gpu_lib):
+    """Config 3's differentiation check at n=100,000: M from CG with top-16
+    dropping (25 iterations); df/dM and df/dA at 64 random entries each against
+    central differences with h = 1e-6."""
+    import torch
+    from parallellocallyoptimalapproximatesparsematrixinverses_amd.autograd import ResidualLoss
+    A, sigma, lam = problems.random_gram(100_000, seed=3, power_iterations=50)
+    Pr = problems.diag_inverse(A)
+    M = gpu_lib.run("cg", A, None, Pr, 25, 0.0, 0.5, True, 256, column_top_k=16)["M"].tocsc()
+    M.sort_indices()
+    plan = ResidualLoss(A, M, torch.float64)
+    m0, a0 = _dev(M.data, torch.float64), _dev(A.data, torch.float64)
+    f0 = plan.forward_value(m0, a0)
+    gm, ga = (g.cpu().numpy() for g in plan.gradients(m0, a0))
+    rng = np.random.default_rng(0)
+    h = 1e-6
+    tol_abs = 64 * np.finfo(float).eps * f0 / h
+    for vec, grad, which in ((M.data, gm, "m"), (A.data, ga, "a")):
+        for idx in rng.choice(vec.size, 64, replace=False):
+            plus, minus = vec.copy(), vec.copy()
+            plus[idx] += h
+            minus[idx] -= h
+            if which == "m":
+                fp = plan.forward_value(_dev(plus, torch.float64), a0)
+                fm = plan.forward_value(_dev(minus, torch.float64), a0)
+            else:
+                fp = plan.forward_value(m0, _dev(plus, torch.float64))
+                fm = plan.forward_value(m0, _dev(minus, torch.float64))
+            fd = (fp - fm) / (2 * h)
+            assert abs(fd - grad[idx]) <= tol_abs + 1e-7 * abs(grad[idx]), (which, idx, fd, grad[idx], tol_abs)
+    plan.close()
+
+
+def test_config5_autograd_full_size(gpu_lib):
+    """Config 5's differentiation form: 27-point 128^3, M from FP32-bulk MR (10
+    iterations); forward and backward of ||I - AM||_F^2 in FP32 with FP64
+    reductions, R recomputed per 65,536-column block, against the FP64 form."""
+    import torch
+    from parallellocallyoptimalapproximatesparsematrixinverses_amd.autograd import ResidualLoss
+    A = problems.stencil27(128, seed=4)
+    M0 = problems.diag_inverse(A)
+    M = gpu_lib.run("mr", A, M0, None, 10, 0.0, 1.0, False, 1, fixed_pattern=True, precision="fp32")["M"].tocsc()
+    M.sort_indices()
+    p32 = ResidualLoss(A, M, torch.float32, block_cols=65536)
+    p64 = ResidualLoss(A, M, torch.float64, block_cols=65536)
+    assert p32.info()["blocks"] == 32
+    f32 = p32.forward_value(_dev(M.data, torch.float32), _dev(A.data, torch.float32))
+    f64 = p64.forward_value(_dev(M.data, torch.float64), _dev(A.data, torch.float64))
+    assert abs(f32 - f64) / f64 < 1e-5
+    g32 = p32.gradients(_dev(M.data, torch.float32), _dev(A.data, torch.float32))
+    g64 = p64.gradients(_dev(M.data, torch.float64), _dev(A.data, torch.float64))
+    for a, b in zip(g32, g64):
+        assert float(torch.linalg.norm(a.double() - b) / torch.linalg.norm(b)) < 1e-4
+    p32.close()
+    p64.close()
