@@ -91,13 +91,33 @@ def cpu_baseline(A, M0, b, threads):
                       f"{ITERATIONS}) + full PCG ({tp:.2f} s), {threads} OpenMP threads"}
 
 
+def calibrate_copy(mib):
+    """Plain streaming kernel of known size: reads and writes mib MiB once per
+    launch.  Its FETCH_SIZE / WRITE_SIZE counters give the factor between the
+    counters and HBM bytes."""
+    import torch
+
+    x = torch.ones(mib * 2 ** 17, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "torch.mul fp64", "bytes_read": x.numel() * 8, "bytes_written": y.numel() * 8,
+                      "launches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 5 launches of an FP64 y = 1.0 * x kernel over MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)

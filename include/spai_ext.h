@@ -95,6 +95,23 @@ const char* spai_last_error(void);
 int64_t spai_device_name(char* buffer, int64_t size);
 
 /* ------------------------------------------------------------------------
+ * Prepared PCG system: A and M uploaded once, then any number of right-hand
+ * sides solved with every operand in HBM.  Replaces the harness classes
+ * PreparedPCGSystem / PreparedPCGSolver (scripts/utils/backward_error.py:63-96),
+ * which call backward_error.pcg once per right-hand side on host copies.
+ * ------------------------------------------------------------------------ */
+typedef struct SpaiPcgSystem SpaiPcgSystem;
+
+/* Validates and uploads A and M (same checks as spai_pcg); NULL on error. */
+SpaiPcgSystem* spai_pcg_prepare(struct CsrData A, struct CsrData M, int64_t m_fp32);
+void spai_pcg_release(SpaiPcgSystem* system);
+/* As spai_pcg for one right-hand side b (host, length n). */
+int64_t spai_pcg_solve(SpaiPcgSystem* system, const double* b, int64_t max_iterations, double tolerance,
+                       double* history, int64_t* count, double* x_out);
+/* y = A x with the prepared A (row dots in column order, as spai_matvec). */
+int64_t spai_pcg_matvec(SpaiPcgSystem* system, const double* x, double* y);
+
+/* ------------------------------------------------------------------------
  * Device-resident session: A, M0, Pr and the right-hand sides are uploaded once,
  * then builds and solves run with every operand already in HBM.  Used by the
  * benchmark and by callers that build several preconditioners for one A.

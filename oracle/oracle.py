@@ -6,21 +6,26 @@ and the benchmark's CPU-baseline leg import this module; the product never does.
 
 Parity pins (what checks each oracle function; tests named in brackets):
   * conservative product / residual (src/common/state_manager.cpp:60,
-    src/common/dropping.cpp:97-104): scipy products with all-ones and random
-    values plus brute-force Boolean patterns [test_oracle_ops.py].
-  * SolveLeastSquares2x2 (src/common/sparse_operations.cpp:58-73): closed-form
-    inverse for nonsingular systems, numpy min-norm lstsq for rank 1, zero on
-    non-finite input [test_oracle_ops.py].  The exact rank threshold near
-    singularity is Eigen's and is "parity unpinned" below 1e-14 relative.
+    src/common/dropping.cpp:97-104): pattern equal to the Boolean product of
+    the stored patterns (explicit zeros kept), values against exact rational
+    products, no transposition, residual diagonal insertion and zero pruning
+    [test_oracle_ops.py].
+  * SolveLeastSquares2x2 (src/common/sparse_operations.cpp:58-73): numpy solve
+    for nonsingular Grams, the minimum-norm solution in the diagonally scaled
+    coordinates for rank 1, zero on non-finite input [test_oracle_ops.py].
+    The exact rank threshold of Eigen's decomposition near singularity is
+    "parity unpinned" (pivots within a few ulps of 2 eps * max pivot).
   * Global CG/MR/LOMR and inner-outer MR/LOMR end to end: the reference's own
     committed per-iteration ||R_i||_F histories (tests/golden/) at 256 column
-    batches, the tests/test.cpp invariants, and a dense pure-Python
-    restatement on small matrices [test_oracle_methods.py, test_oracle_golden.py].
-  * PCG (methods_cython/backward_error.pyx:69-144): textbook CG identities
-    (x_k minimises the A-norm error over the Krylov space; exact solve in <= n
-    steps) and a pure-Python restatement [test_oracle_pcg.py].
-  * Extensions (fixed pattern, per-column top-k): dense brute force on small
-    matrices (projected line search optimality, column-wise sort)
+    batches [test_oracle_golden.py], the tests/test.cpp invariants
+    [test_oracle_invariants.py], and a dense pure-Python restatement on small
+    matrices [test_oracle_dense.py].
+  * PCG (methods_cython/backward_error.pyx:69-144): exact termination in
+    <= n steps, the A-norm optimality of x_k over the preconditioned Krylov
+    space, history[0] = 1 with a final true-residual entry, status codes
+    [test_oracle_pcg.py].
+  * Extensions (fixed pattern, per-column top-k): projected line-search
+    optimality, two-term normal equations, projected CG, brute-force sort
     [test_oracle_extensions.py].
 """
 from __future__ import annotations

@@ -101,6 +101,14 @@ struct SpaiSession {
   int64_t last_count = 0;
 };
 
+struct SpaiPcgSystem {
+  spai::Ctx ctx;
+  std::unique_ptr<spai::DCsr> a, m;
+  std::unique_ptr<spai::PcgSolver> solver;
+  spai::DBuf<double> b, x, y;
+  int64_t n = 0;
+};
+
 extern "C" {
 
 Result spai_global_cg(MatrixData A, MatrixData M0, MatrixData Pr, MethodParams params) {
@@ -178,14 +186,76 @@ int64_t spai_matvec(CsrData A, const double* x, double* y) {
     ValidateCsr(A);
     spai::Ctx ctx;
     spai::DCsr a = spai::CsrFromHost(ctx, A.rows, A.nnz, A.row_offsets, A.col_indices, A.values);
-    spai::DMat dummy;
-    (void)dummy;
     auto xd = ctx.Alloc<double>(A.rows);
     auto yd = ctx.Alloc<double>(A.rows);
     SPAI_HIP_CHECK(hipMemcpyAsync(xd.get(), x, sizeof(double) * A.rows, hipMemcpyHostToDevice, ctx.stream));
     spai::SpMV(ctx, a, xd.get(), yd.get());
     SPAI_HIP_CHECK(hipMemcpyAsync(y, yd.get(), sizeof(double) * A.rows, hipMemcpyDeviceToHost, ctx.stream));
     ctx.Sync();
+    return 0;
+  } catch (const std::exception& ex) {
+    return Fail(ex.what());
+  } catch (...) {
+    return Fail("Unknown failure.");
+  }
+}
+
+SpaiPcgSystem* spai_pcg_prepare(CsrData A, CsrData M, int64_t m_fp32) {
+  try {
+    ValidateCsr(A);
+    ValidateCsr(M);
+    if (M.rows != A.rows) throw std::invalid_argument("A and M must have the same shape");
+    auto s = std::make_unique<SpaiPcgSystem>();
+    s->n = A.rows;
+    s->a = std::make_unique<spai::DCsr>(
+        spai::CsrFromHost(s->ctx, A.rows, A.nnz, A.row_offsets, A.col_indices, A.values));
+    s->m = std::make_unique<spai::DCsr>(
+        spai::CsrFromHost(s->ctx, M.rows, M.nnz, M.row_offsets, M.col_indices, M.values));
+    if (m_fp32) spai::MakeFp32Copy(s->ctx, *s->m);
+    s->b = s->ctx.Alloc<double>(s->n);
+    s->x = s->ctx.Alloc<double>(s->n);
+    s->y = s->ctx.Alloc<double>(s->n);
+    s->solver = std::make_unique<spai::PcgSolver>(s->ctx, *s->a, *s->m, m_fp32 != 0);
+    s->ctx.Sync();
+    return s.release();
+  } catch (const std::exception& ex) {
+    g_last_error = ex.what();
+    return nullptr;
+  }
+}
+
+void spai_pcg_release(SpaiPcgSystem* system) { delete system; }
+
+int64_t spai_pcg_solve(SpaiPcgSystem* s, const double* b, int64_t max_iterations, double tolerance, double* history,
+                       int64_t* count, double* x_out) {
+  try {
+    if (!s) throw std::invalid_argument("null PCG system");
+    if (max_iterations <= 0) throw std::invalid_argument("n, max_iterations and threads must be positive");
+    SPAI_HIP_CHECK(hipMemcpyAsync(s->b.get(), b, sizeof(double) * s->n, hipMemcpyHostToDevice, s->ctx.stream));
+    int64_t cnt = 0;
+    const int status = s->solver->Solve(s->b.get(), max_iterations, tolerance, &cnt);
+    if (history) s->solver->History(history, cnt);
+    if (x_out) {
+      SPAI_HIP_CHECK(
+          hipMemcpyAsync(x_out, s->solver->X(), sizeof(double) * s->n, hipMemcpyDeviceToHost, s->ctx.stream));
+    }
+    s->ctx.Sync();
+    if (count) *count = cnt;
+    return status;
+  } catch (const std::exception& ex) {
+    return Fail(ex.what());
+  } catch (...) {
+    return Fail("Unknown failure.");
+  }
+}
+
+int64_t spai_pcg_matvec(SpaiPcgSystem* s, const double* x, double* y) {
+  try {
+    if (!s) throw std::invalid_argument("null PCG system");
+    SPAI_HIP_CHECK(hipMemcpyAsync(s->x.get(), x, sizeof(double) * s->n, hipMemcpyHostToDevice, s->ctx.stream));
+    spai::SpMV(s->ctx, *s->a, s->x.get(), s->y.get());
+    SPAI_HIP_CHECK(hipMemcpyAsync(y, s->y.get(), sizeof(double) * s->n, hipMemcpyDeviceToHost, s->ctx.stream));
+    s->ctx.Sync();
     return 0;
   } catch (const std::exception& ex) {
     return Fail(ex.what());

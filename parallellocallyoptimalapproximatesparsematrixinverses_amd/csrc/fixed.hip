@@ -32,6 +32,7 @@ namespace {
 
 constexpr int kBlk = 256;
 constexpr int kMaxGrid = 2048;
+constexpr int kMaxFusedGrid = 1024;
 
 // Scalar slots in device memory.
 enum Slot : int {
@@ -231,6 +232,7 @@ __global__ void PlanSizes(int n, const int* __restrict__ sp, const int* __restri
   terms[j] = u;
   atomicMax(&ext[0], ap[j + 1] - ap[j]);    // longest column of A
   atomicMax(&ext[1], s2p[j + 1] - s2p[j]);  // longest column of S2
+  atomicMax(&ext[2], sp[j + 1] - sp[j]);    // longest column of S
 }
 
 __global__ void ExclusiveScan64Serial(int n, const int64_t* __restrict__ in, int64_t* __restrict__ out) {
@@ -254,6 +256,274 @@ __global__ void ExclusiveScan64Serial(int n, const int64_t* __restrict__ in, int
     __syncthreads();
   }
   if (threadIdx.x == 0) out[n] = carry;
+}
+
+// ------------------------------------------------------- fused LOMR pass
+// One kernel per LOMR iteration (locally_minimal_residual.cpp:57-114 on the
+// fixed pattern).  What an iteration does to column j -- A [Z P](:, j), the
+// updates of R(:, j), M(:, j), P(:, j), Z(:, j), and the products
+// A [Z' P'](:, j) whose traces give the next step sizes -- reads column j
+// alone, so a wavefront takes W = 64 / GL consecutive columns (GL lanes each),
+// keeps their accumulators and new directions in LDS and touches R, M, P and
+// Z once per iteration.  Only the five trace products and ||R||^2 cross
+// columns: each block stores its partial sums, and every block of the next
+// pass sums all of them in the same fixed order, so all blocks take the same
+// step sizes and stop decisions without a device-scope fence; block 0 records
+// the history.  The R/S-sized reads and writes of a tile run over the tile's
+// contiguous range with all 64 lanes, so stores cover whole lines.
+
+// Per-group LDS region of the fused pass: accumulators (2 x lmax), Pr(i)
+// and Pr(i) r on S2 (2 x lmax), Z' and P' on S (2 x smax).
+__host__ __device__ constexpr int FusedGroupWords(int lmax, int smax) { return 4 * lmax + 2 * smax; }
+
+// Column-wise Gustavson accumulation of A [y0 y1](:, j) into acc0/acc1 (see
+// SlotProduct).  YGet(e) returns the S entry e's values.
+template <class V, class SlotT, int GL, int NY, class YGet>
+__device__ __forceinline__ void SlotAccumulate(int j, int gl, const int* __restrict__ sp, const int* __restrict__ si,
+                                               const int* __restrict__ ap, const V* __restrict__ av,
+                                               const int64_t* __restrict__ plan_ptr, const SlotT* __restrict__ plan,
+                                               V* acc0, V* acc1, YGet yget) {
+  int64_t pp = plan_ptr[j];
+  const int e0 = sp[j], e1 = sp[j + 1];
+  for (int eb = e0; eb < e1; eb += GL) {
+    int my_a0 = 0, my_c = 0;
+    V my_ya = V(0), my_yb = V(0);
+    if (eb + gl < e1) {
+      const int k = si[eb + gl];
+      my_a0 = ap[k];
+      my_c = ap[k + 1] - my_a0;
+      yget(eb + gl, my_ya, my_yb);
+    }
+    const int ns = min(GL, e1 - eb);
+    for (int st = 0; st < ns; ++st) {
+      const int a0 = __shfl(my_a0, st, GL);
+      const int c = __shfl(my_c, st, GL);
+      const V ya = __shfl(my_ya, st, GL);
+      const V yb = NY > 1 ? __shfl(my_yb, st, GL) : V(0);
+      for (int u = gl; u < c; u += GL) {
+        const int slot = plan[pp + u];
+        const V a = av[a0 + u];
+        acc0[slot] += a * ya;
+        if (NY > 1) acc1[slot] += a * yb;
+      }
+      pp += c;
+    }
+  }
+}
+
+// Tile bounds: bnd[c] = s2p[min(j0 + c, n)], sb[c] = sp[min(j0 + c, n)].
+template <int W>
+struct TileBounds {
+  int bnd[W + 1];
+  int sb[W + 1];
+  __device__ __forceinline__ void Load(int j0, int n, int lane, const int* __restrict__ s2p,
+                                       const int* __restrict__ sp) {
+    const int jb = min(j0 + (lane <= W ? lane : 0), n);
+    const int mb = s2p[jb], ms = sp[jb];
+#pragma unroll
+    for (int c = 0; c <= W; ++c) {
+      bnd[c] = __shfl(mb, c);
+      sb[c] = __shfl(ms, c);
+    }
+  }
+  // Column (within the tile) of S2 position q, its S2 and S starts.
+  __device__ __forceinline__ int Find(int q, int& q0, int& s0) const {
+    int g = 0;
+    q0 = bnd[0];
+    s0 = sb[0];
+#pragma unroll
+    for (int c = 1; c < W; ++c) {
+      if (q >= bnd[c]) {
+        g = c;
+        q0 = bnd[c];
+        s0 = sb[c];
+      }
+    }
+    return g;
+  }
+};
+
+// kInit: M = M0 already on S; R = I - A M, Z = Pr R on S, P = 0, and the
+// traces of A Z (the pass before iteration 1).  Otherwise iteration `it`:
+// step sizes from the previous pass, update, next traces.
+template <class V, class SlotT, int GL, bool kInit>
+__global__ __launch_bounds__(kBlk) void LomrFused(
+    int n, int it, double tol, const int* __restrict__ sp, const int* __restrict__ si, const int* __restrict__ s2p,
+    const int* __restrict__ s2i, const int* __restrict__ s2_to_s, const int* __restrict__ ap,
+    const V* __restrict__ av, const int64_t* __restrict__ plan_ptr, const SlotT* __restrict__ plan, int lmax,
+    int smax, const V* __restrict__ pr, V* __restrict__ R, V* __restrict__ M, V* __restrict__ Z,
+    V* __restrict__ P, const double* __restrict__ part_in, double* __restrict__ part_out,
+    double* __restrict__ hist, Flags* __restrict__ flags) {
+  constexpr int W = kWave / GL;
+  // block 0 of this same pass may set flags->stop: read it once per block
+  __shared__ int stop_seen;
+  if (threadIdx.x == 0) stop_seen = flags->stop;
+  __syncthreads();
+  if (stop_seen) return;
+  V d = V(0), gm = V(0), be = V(0), nd = V(0), ng = V(0);
+  if constexpr (!kInit) {
+    double s[6];
+    SumPartialsBlock<6>(part_in, gridDim.x, s);
+    const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
+    // history row it - 1 (result_operations.cpp:7-67: converged iff ||R|| < tol)
+    const double rn = sqrt(s[5]);
+    if (writer) {
+      hist[it - 1] = rn;
+      flags->iterations = it - 1;
+    }
+    if (rn < tol) {
+      if (writer) flags->stop = 2;
+      return;
+    }
+    // GetDeltaGamma (locally_minimal_residual.cpp:36-50)
+    const bool first = it == 1;
+    const double az_praz = s[0], ap_prap = first ? 0.0 : s[1], az_prap = first ? 0.0 : s[2], z_az = s[3],
+                 z_ap = first ? 0.0 : s[4];
+    double delta = 0.0, gamma = 0.0;
+    bool brk = false;
+    if (first) {
+      if (!(isfinite(az_praz) && fabs(az_praz) > 0.0)) {
+        brk = true;
+      } else {
+        delta = z_az / az_praz;
+        gamma = 1.0;
+      }
+    } else {
+      SolveLeastSquares2x2(az_praz, az_prap, az_prap, ap_prap, z_az, z_ap, delta, gamma);
+    }
+    if (brk || !isfinite(delta) || !isfinite(gamma)) {
+      if (writer) flags->stop = 1;
+      return;
+    }
+    const double ratio = delta != 0.0 ? gamma / delta : 0.0;
+    d = V(delta);
+    gm = V(gamma);
+    be = V(isfinite(ratio) ? ratio : 0.0);
+    nd = V(-delta);
+    ng = V(-gamma);
+  }
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int lane = LaneId(), wib = threadIdx.x / kWave;
+  const int g = lane / GL, gl = lane % GL;
+  const int words = FusedGroupWords(lmax, smax);
+  V* const wave_lds = reinterpret_cast<V*>(smem_raw) + size_t(wib) * W * words;
+  V* const mine = wave_lds + size_t(g) * words;
+  double red[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int ntiles = (n + W - 1) / W;
+  const int wstride = gridDim.x * (kBlk / kWave);
+  for (int tile = blockIdx.x * (kBlk / kWave) + wib; tile < ntiles; tile += wstride) {
+    const int j0 = tile * W;
+    const int j = j0 + g;
+    const bool valid = j < n;
+    TileBounds<W> tb;
+    tb.Load(j0, n, lane, s2p, sp);
+    const int L = valid ? s2p[j + 1] - s2p[j] : 0;
+    // product 1: A [Z P](:, j)  (kInit: A M(:, j))
+    for (int t = gl; t < L; t += GL) {
+      mine[t] = V(0);
+      if (!kInit) mine[lmax + t] = V(0);
+    }
+    if (valid) {
+      if constexpr (kInit) {
+        SlotAccumulate<V, SlotT, GL, 1>(j, gl, sp, si, ap, av, plan_ptr, plan, mine, mine + lmax,
+                                        [&](int e, V& ya, V& yb) { ya = M[e]; });
+      } else {
+        SlotAccumulate<V, SlotT, GL, 2>(j, gl, sp, si, ap, av, plan_ptr, plan, mine, mine + lmax,
+                                        [&](int e, V& ya, V& yb) {
+                                          ya = Z[e];
+                                          yb = P[e];
+                                        });
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // update over the tile's contiguous S2 range
+    const int qend = tb.bnd[W];
+    for (int q = tb.bnd[0] + lane; q < qend; q += kWave) {
+      int q0, s0;
+      const int gg = tb.Find(q, q0, s0);
+      const int t = q - q0;
+      V* const grp = wave_lds + size_t(gg) * words;
+      const int i = s2i[q];
+      V r;
+      if constexpr (kInit) {
+        r = -grp[t];
+        if (i == j0 + gg) r += V(1);
+      } else {
+        r = R[q];
+        r = r + nd * grp[t];
+        r = r + ng * grp[lmax + t];
+      }
+      R[q] = r;
+      red[5] += double(r) * double(r);
+      const V pri = pr[i];
+      grp[2 * lmax + t] = pri;
+      grp[3 * lmax + t] = pri * r;
+      const int p = s2_to_s[q];
+      if (p >= 0) {
+        const V zn = pri * r;
+        V pn;
+        if constexpr (kInit) {
+          pn = V(0);
+        } else {
+          const V z = Z[p], pv = P[p];
+          V m = M[p];
+          m = m + d * z;
+          m = m + gm * pv;
+          M[p] = m;
+          pn = pv * be + z;
+        }
+        Z[p] = zn;
+        P[p] = pn;
+        grp[4 * lmax + (p - s0)] = zn;
+        grp[4 * lmax + smax + (p - s0)] = pn;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // product 2: A [Z' P'](:, j) from LDS
+    for (int t = gl; t < L; t += GL) {
+      mine[t] = V(0);
+      mine[lmax + t] = V(0);
+    }
+    if (valid) {
+      const int sj = sp[j];
+      const V* yz = mine + 4 * lmax;
+      const V* yp = yz + smax;
+      SlotAccumulate<V, SlotT, GL, 2>(j, gl, sp, si, ap, av, plan_ptr, plan, mine, mine + lmax,
+                                      [&](int e, V& ya, V& yb) {
+                                        ya = yz[e - sj];
+                                        yb = yp[e - sj];
+                                      });
+    }
+    __builtin_amdgcn_wave_barrier();
+    // traces of the next step (LomrProjectEpi order)
+    for (int q = tb.bnd[0] + lane; q < qend; q += kWave) {
+      int q0, s0;
+      const int gg = tb.Find(q, q0, s0);
+      const int t = q - q0;
+      const V* grp = wave_lds + size_t(gg) * words;
+      const double daz = grp[t], dap = grp[lmax + t], dpr = grp[2 * lmax + t], dprr = grp[3 * lmax + t];
+      red[0] += daz * (dpr * daz);
+      red[1] += dap * (dpr * dap);
+      red[2] += daz * (dpr * dap);
+      red[3] += dprr * daz;
+      red[4] += dprr * dap;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  StorePartials<6>(red, part_out);
+}
+
+// History row of the last pass (AppendResidual).
+__global__ __launch_bounds__(kBlk) void LomrFusedFinish(const double* __restrict__ part, int G, int it, double tol,
+                                                        double* __restrict__ hist, Flags* __restrict__ flags) {
+  if (flags->stop) return;
+  double s[6];
+  SumPartialsBlock<6>(part, G, s);
+  if (threadIdx.x != 0) return;
+  const double rn = sqrt(s[5]);
+  hist[it] = rn;
+  flags->iterations = it;
+  if (rn < tol) flags->stop = 2;
 }
 
 // R = I - A M; Z = Pr R on S (mode 1: P = Z and <R, Z>).
@@ -946,13 +1216,13 @@ class FixedImpl final : public FixedProblem {
     // slot plan of the products (fixed pattern)
     {
       auto terms = c.Alloc<int64_t>(n_);
-      auto ext = c.Alloc<int>(2);
-      SPAI_HIP_CHECK(hipMemsetAsync(ext.get(), 0, 2 * sizeof(int), c.stream));
+      auto ext = c.Alloc<int>(3);
+      SPAI_HIP_CHECK(hipMemsetAsync(ext.get(), 0, 3 * sizeof(int), c.stream));
       PlanSizes<<<Blocks1(n_), kBlk, 0, c.stream>>>((int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), A_.ptr(),
                                                     s2_pat_->ptr.get(), terms.get(), ext.get());
       plan_ptr_ = c.Alloc<int64_t>(n_ + 1);
       ExclusiveScan64Serial<<<1, 1024, 0, c.stream>>>((int)n_, terms.get(), plan_ptr_.get());
-      int h[2];
+      int h[3];
       c.ReadBack(h, ext.get(), sizeof(h));
       int64_t total = 0;
       c.ReadBack(&total, plan_ptr_.get() + n_, sizeof(int64_t));
@@ -981,6 +1251,13 @@ class FixedImpl final : public FixedProblem {
         if (e2) throw std::runtime_error("product term outside pattern(A S)");
         const int64_t groups = kBlk / gl_;
         gridp_ = (unsigned)std::min<int64_t>(std::max<int64_t>((n_ + groups - 1) / groups, 1), kMaxGrid);
+        // fused LOMR pass: W = 64 / GL columns per wavefront
+        smax_ = std::max(h[2], 1);
+        smemf_ = size_t(kBlk / gl_) * size_t(FusedGroupWords(lmax_, smax_)) * sizeof(V);
+        fused_ok_ = smemf_ <= 60 * 1024;  // plus the kernel's static LDS, under 64 KiB
+        const int64_t ntiles = (n_ + (kWave / gl_) - 1) / (kWave / gl_);
+        gridf_ = (unsigned)std::min<int64_t>(std::max<int64_t>((ntiles + kBlk / kWave - 1) / (kBlk / kWave), 1),
+                                             kMaxFusedGrid);
       }
       av_ = c.Alloc<V>(A_.nnz());
       ConvertValues<V><<<Blocks1(A_.nnz()), kBlk, 0, c.stream>>>(A_.nnz(), A_.v(), av_.get());
@@ -1016,6 +1293,7 @@ class FixedImpl final : public FixedProblem {
     grid2_ = GridFor(nnz_s2_);
     grid1_ = GridFor(nnz_s_);
     part_ = c.Alloc<double>(8 * kMaxGrid);
+    partf_ = c.Alloc<double>(2 * 6 * size_t(kMaxFusedGrid));
     sc_ = c.Alloc<double>(kNumSlots);
     flags_ = c.Alloc<Flags>(1);
     hist_ = c.Alloc<double>(p.max_iterations + 1);
@@ -1098,8 +1376,59 @@ class FixedImpl final : public FixedProblem {
     return (int64_t)io_hist_.size() - 1;
   }
 
+  // Global LOMR with one fused pass per iteration (LomrFused).
+  int64_t BuildLomrFused() {
+    Ctx& c = ctx_;
+    hipStream_t st = c.stream;
+    SampleOnPattern<V><<<Blocks1(n_), kBlk, 0, st>>>((int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), m0_.ptr(),
+                                                     m0_.idx(), m0_.v(), M_.get());
+    SPAI_HIP_CHECK(hipMemsetAsync(flags_.get(), 0, sizeof(Flags), st));
+    const size_t half = 6 * size_t(gridf_);
+    double* part[2] = {partf_.get(), partf_.get() + half};
+    const double tol = prm_.tolerance;
+    for (int64_t it = 0; it <= prm_.max_iterations; ++it) {
+      double* pin = part[(it + 1) & 1];
+      double* pout = part[it & 1];
+#define SPAI_FUSED_LAUNCH(SlotT, plan, GLV)                                                                       \
+  do {                                                                                                            \
+    if (it == 0)                                                                                                  \
+      LomrFused<V, SlotT, GLV, true><<<gridf_, kBlk, smemf_, st>>>(                                               \
+          (int)n_, 0, tol, s_pat_->ptr.get(), s_pat_->idx.get(), s2_pat_->ptr.get(), s2_pat_->idx.get(),          \
+          s2_to_s_.get(), A_.ptr(), av_.get(), plan_ptr_.get(), plan, lmax_, smax_, prv_.get(), R_.get(),        \
+          M_.get(), Z_.get(), P_.get(), pin, pout, hist_.get(), flags_.get());                                   \
+    else                                                                                                          \
+      LomrFused<V, SlotT, GLV, false><<<gridf_, kBlk, smemf_, st>>>(                                              \
+          (int)n_, (int)it, tol, s_pat_->ptr.get(), s_pat_->idx.get(), s2_pat_->ptr.get(), s2_pat_->idx.get(),    \
+          s2_to_s_.get(), A_.ptr(), av_.get(), plan_ptr_.get(), plan, lmax_, smax_, prv_.get(), R_.get(),        \
+          M_.get(), Z_.get(), P_.get(), pin, pout, hist_.get(), flags_.get());                                   \
+  } while (0)
+#define SPAI_FUSED_GL(SlotT, plan)                          \
+  switch (gl_) {                                            \
+    case 8: SPAI_FUSED_LAUNCH(SlotT, plan, 8); break;       \
+    case 16: SPAI_FUSED_LAUNCH(SlotT, plan, 16); break;     \
+    case 32: SPAI_FUSED_LAUNCH(SlotT, plan, 32); break;     \
+    default: SPAI_FUSED_LAUNCH(SlotT, plan, 64); break;     \
+  }
+      if (slot16_) {
+        SPAI_FUSED_GL(uint16_t, plan16_.get());
+      } else {
+        SPAI_FUSED_GL(uint8_t, plan8_.get());
+      }
+#undef SPAI_FUSED_GL
+#undef SPAI_FUSED_LAUNCH
+    }
+    LomrFusedFinish<<<1, kBlk, 0, st>>>(part[prm_.max_iterations & 1], (int)gridf_, (int)prm_.max_iterations, tol,
+                                        hist_.get(), flags_.get());
+    SPAI_HIP_CHECK(hipGetLastError());
+    Flags f;
+    c.ReadBack(&f, flags_.get(), sizeof(Flags));
+    last_flags_ = f;
+    return f.iterations;
+  }
+
   int64_t Build() override {
     if (method_ >= SPAI_INNER_OUTER_MR) return BuildInnerOuter();
+    if (method_ == SPAI_GLOBAL_LOMR && use_plan_ && fused_ok_) return BuildLomrFused();
     Ctx& c = ctx_;
     hipStream_t st = c.stream;
     // M = M0 on S, P = 0
@@ -1240,7 +1569,11 @@ class FixedImpl final : public FixedProblem {
   DBuf<int64_t> plan_ptr_;
   DBuf<uint8_t> plan8_;
   DBuf<uint16_t> plan16_;
-  int maxcol_ = 1, lmax_ = 1, gl_ = 8;
+  int maxcol_ = 1, lmax_ = 1, gl_ = 8, smax_ = 1;
+  size_t smemf_ = 0;
+  bool fused_ok_ = false;
+  unsigned gridf_ = 1;
+  DBuf<double> partf_;
   size_t smem_ = 0;
   bool slot16_ = false, use_plan_ = false;
   unsigned gridp_ = 1;

@@ -152,9 +152,17 @@ __global__ void ToInt(int64_t n, const double* __restrict__ v, int* __restrict__
   if (i < n) out[i] = int(v[i]);
 }
 
+// Grid of the grid-stride kernels (capped; partial sums fit `part`).
 unsigned GridOf(int64_t n) {
   int64_t g = (n + kBlk - 1) / kBlk;
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 4096));
+}
+
+// Grid of the one-thread-per-element kernels: every element needs a thread.
+unsigned GridAll(int64_t n) {
+  const int64_t g = (n + kBlk - 1) / kBlk;
+  if (g > (int64_t(1) << 31) - 1) throw std::runtime_error("grid too large");
+  return (unsigned)std::max<int64_t>(1, g);
 }
 
 }  // namespace
@@ -187,10 +195,10 @@ spai::PatternPtr TransposeWithPerm(spai::Ctx& c, const spai::DMat& X, spai::DBuf
   spai::DMat P;
   P.pat = X.pat;
   P.val = c.Alloc<double>(std::max<int64_t>(X.nnz(), 1));
-  spai::Iota<<<spai::GridOf(X.nnz()), spai::kBlk, 0, c.stream>>>(X.nnz(), P.val.get());
+  spai::Iota<<<spai::GridAll(X.nnz()), spai::kBlk, 0, c.stream>>>(X.nnz(), P.val.get());
   spai::DMat T = spai::Transpose(c, P);
   perm = c.Alloc<int>(std::max<int64_t>(X.nnz(), 1));
-  spai::ToInt<<<spai::GridOf(X.nnz()), spai::kBlk, 0, c.stream>>>(X.nnz(), T.val.get(), perm.get());
+  spai::ToInt<<<spai::GridAll(X.nnz()), spai::kBlk, 0, c.stream>>>(X.nnz(), T.val.get(), perm.get());
   return T.pat;
 }
 
@@ -233,7 +241,7 @@ int64_t Forward(SpaiLossPlan* p, const V* m, const V* a, double* loss, bool back
     }
   }
   if (backward)
-    ScaleOut<V><<<GridOf(p->A.nnz()), kBlk, 0, st>>>(p->A.nnz(), p->acc.get(), scale, ga);
+    ScaleOut<V><<<GridAll(p->A.nnz()), kBlk, 0, st>>>(p->A.nnz(), p->acc.get(), scale, ga);
   SPAI_HIP_CHECK(hipGetLastError());
   SPAI_HIP_CHECK(hipStreamSynchronize(st));
   if (loss) *loss = total;
@@ -266,9 +274,9 @@ SpaiLossPlan* spai_loss_plan_create(MatrixData A, MatrixData M, int64_t precisio
     p->s2col = c.Alloc<int>(std::max<int64_t>(p->S2.nnz(), 1));
     p->scol = c.Alloc<int>(std::max<int64_t>(p->S.nnz(), 1));
     p->acol = c.Alloc<int>(std::max<int64_t>(p->A.nnz(), 1));
-    ColOf<<<GridOf(p->n), kBlk, 0, c.stream>>>((int)p->n, p->S2.ptr(), p->s2col.get());
-    ColOf<<<GridOf(p->n), kBlk, 0, c.stream>>>((int)p->n, p->S.ptr(), p->scol.get());
-    ColOf<<<GridOf(p->n), kBlk, 0, c.stream>>>((int)p->n, p->A.ptr(), p->acol.get());
+    ColOf<<<GridAll(p->n), kBlk, 0, c.stream>>>((int)p->n, p->S2.ptr(), p->s2col.get());
+    ColOf<<<GridAll(p->n), kBlk, 0, c.stream>>>((int)p->n, p->S.ptr(), p->scol.get());
+    ColOf<<<GridAll(p->n), kBlk, 0, c.stream>>>((int)p->n, p->A.ptr(), p->acol.get());
     std::vector<int> s2p(p->n + 1);
     SPAI_HIP_CHECK(hipMemcpyAsync(s2p.data(), p->S2.ptr(), sizeof(int) * (p->n + 1), hipMemcpyDeviceToHost, c.stream));
     c.Sync();
