@@ -32,7 +32,7 @@ namespace {
 
 constexpr int kBlk = 256;
 constexpr int kMaxGrid = 2048;
-constexpr int kMaxFusedGrid = 1024;
+constexpr int kMaxFusedGrid = 2048;
 
 // Scalar slots in device memory.
 enum Slot : int {
@@ -128,6 +128,34 @@ __device__ __forceinline__ void SumPartialsBlock(const double* __restrict__ part
     double s = 0.0;
     for (int b = threadIdx.x; b < G; b += blockDim.x) s += part[k * G + b];
     out[k] = BlockSum<double>(s, red);
+  }
+}
+
+// As SumPartialsBlock, with all K sums read in one pass and reduced together
+// (the same per-thread and per-wave order for each k).
+template <int K>
+__device__ __forceinline__ void SumPartialsBlockFused(const double* __restrict__ part, int G, double (&out)[K]) {
+  __shared__ double red[K][16];
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = 0.0;
+  for (int b = threadIdx.x; b < G; b += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] += part[k * G + b];
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = WaveSum(v[k]);
+  const int w = WaveInBlock(), nw = (blockDim.x + kWave - 1) / kWave;
+  if (LaneId() == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) red[k][w] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double t = 0.0;
+    for (int i = 0; i < nw; ++i) t += red[k][i];
+    out[k] = t;
   }
 }
 
@@ -266,47 +294,94 @@ __global__ void ExclusiveScan64Serial(int n, const int64_t* __restrict__ in, int
 // alone, so a wavefront takes W = 64 / GL consecutive columns (GL lanes each),
 // keeps their accumulators and new directions in LDS and touches R, M, P and
 // Z once per iteration.  Only the five trace products and ||R||^2 cross
-// columns: each block stores its partial sums, and every block of the next
-// pass sums all of them in the same fixed order, so all blocks take the same
-// step sizes and stop decisions without a device-scope fence; block 0 records
-// the history.  The R/S-sized reads and writes of a tile run over the tile's
-// contiguous range with all 64 lanes, so stores cover whole lines.
+// columns: each block stores its partial sums and a one-block kernel
+// (LomrFusedStep) sums them in a fixed order, records the history row and
+// the stop decision and turns the traces into the next step sizes.  The
+// R/S-sized reads and writes of a tile run over the tile's contiguous range
+// with all 64 lanes, so stores cover whole lines.
 
-// Per-group LDS region of the fused pass: accumulators (2 x lmax), Pr(i)
-// and Pr(i) r on S2 (2 x lmax), Z' and P' on S (2 x smax).
-__host__ __device__ constexpr int FusedGroupWords(int lmax, int smax) { return 4 * lmax + 2 * smax; }
+// Per-group LDS region of the fused pass: accumulators (2 x lmax), R' of the
+// column (lmax), Z' and P' of the column (2 x smax).
+__host__ __device__ constexpr int FusedGroupWords(int lmax, int smax) { return 3 * lmax + 2 * smax; }
 
-// Column-wise Gustavson accumulation of A [y0 y1](:, j) into acc0/acc1 (see
-// SlotProduct).  YGet(e) returns the S entry e's values.
+
+// Column-wise Gustavson accumulation of A [y0 y1](:, j) into acc0/acc1, in
+// the order of SlotProduct (S entries of the column in order, k ascending).
+// YGet(e, ya, yb) returns S entry e's values.  A step (one S entry, column k
+// of A) gives lane gl the term u = gl when |A(:, k)| <= GL: the plan slots and
+// A values of up to kPre steps are loaded before any of them is added, so a
+// column waits for one load latency per kPre steps, not one per step.
 template <class V, class SlotT, int GL, int NY, class YGet>
-__device__ __forceinline__ void SlotAccumulate(int j, int gl, const int* __restrict__ sp, const int* __restrict__ si,
-                                               const int* __restrict__ ap, const V* __restrict__ av,
-                                               const int64_t* __restrict__ plan_ptr, const SlotT* __restrict__ plan,
-                                               V* acc0, V* acc1, YGet yget) {
+__device__ __forceinline__ void SlotAccumulate(int j, int gl, const int* __restrict__ sp, const int2* __restrict__ a0c,
+                                               const V* __restrict__ av, const int64_t* __restrict__ plan_ptr,
+                                               const SlotT* __restrict__ plan, V* acc0, V* acc1, YGet yget) {
+  constexpr int kPre = 4;
   int64_t pp = plan_ptr[j];
   const int e0 = sp[j], e1 = sp[j + 1];
   for (int eb = e0; eb < e1; eb += GL) {
     int my_a0 = 0, my_c = 0;
     V my_ya = V(0), my_yb = V(0);
     if (eb + gl < e1) {
-      const int k = si[eb + gl];
-      my_a0 = ap[k];
-      my_c = ap[k + 1] - my_a0;
+      const int2 ac = a0c[eb + gl];
+      my_a0 = ac.x;
+      my_c = ac.y;
       yget(eb + gl, my_ya, my_yb);
     }
     const int ns = min(GL, e1 - eb);
-    for (int st = 0; st < ns; ++st) {
-      const int a0 = __shfl(my_a0, st, GL);
-      const int c = __shfl(my_c, st, GL);
-      const V ya = __shfl(my_ya, st, GL);
-      const V yb = NY > 1 ? __shfl(my_yb, st, GL) : V(0);
-      for (int u = gl; u < c; u += GL) {
-        const int slot = plan[pp + u];
-        const V a = av[a0 + u];
-        acc0[slot] += a * ya;
-        if (NY > 1) acc1[slot] += a * yb;
+    for (int s0 = 0; s0 < ns; s0 += kPre) {
+      const int cnt = min(kPre, ns - s0);
+      SlotT sl[kPre];
+      V a[kPre], ya[kPre], yb[kPre];
+      int cc[kPre];
+      int64_t off = pp;
+      bool wide = false;
+#pragma unroll
+      for (int s = 0; s < kPre; ++s) {
+        cc[s] = 0;
+        sl[s] = SlotT(0);
+        a[s] = ya[s] = yb[s] = V(0);
+        if (s < cnt) {
+          const int st = s0 + s;
+          const int a0 = __shfl(my_a0, st, GL);
+          const int c = __shfl(my_c, st, GL);
+          ya[s] = __shfl(my_ya, st, GL);
+          if (NY > 1) yb[s] = __shfl(my_yb, st, GL);
+          cc[s] = c;
+          wide = wide || c > GL;
+          if (gl < c) {
+            sl[s] = plan[off + gl];
+            a[s] = av[a0 + gl];
+          }
+          off += c;
+        }
       }
-      pp += c;
+      if (!wide) {
+#pragma unroll
+        for (int s = 0; s < kPre; ++s) {
+          if (s < cnt && gl < cc[s]) {
+            acc0[sl[s]] += a[s] * ya[s];
+            if (NY > 1) acc1[sl[s]] += a[s] * yb[s];
+          }
+        }
+      } else {
+        // a column of A longer than GL: lanes stride through each step
+        int64_t o2 = pp;
+        for (int s = 0; s < cnt; ++s) {
+          const int st = s0 + s;
+          const int a0 = __shfl(my_a0, st, GL);
+          const int c = __shfl(my_c, st, GL);
+          const V y0 = __shfl(my_ya, st, GL);
+          const V y1 = NY > 1 ? __shfl(my_yb, st, GL) : V(0);
+          for (int u = gl; u < c; u += GL) {
+            const int slot = plan[o2 + u];
+            const V av_u = av[a0 + u];
+            acc0[slot] += av_u * y0;
+            if (NY > 1) acc1[slot] += av_u * y1;
+          }
+          o2 += c;
+        }
+      }
+      pp = off;
     }
   }
 }
@@ -320,10 +395,11 @@ struct TileBounds {
                                        const int* __restrict__ sp) {
     const int jb = min(j0 + (lane <= W ? lane : 0), n);
     const int mb = s2p[jb], ms = sp[jb];
+    // wave-uniform: read lanes into scalar registers
 #pragma unroll
     for (int c = 0; c <= W; ++c) {
-      bnd[c] = __shfl(mb, c);
-      sb[c] = __shfl(ms, c);
+      bnd[c] = __builtin_amdgcn_readlane(mb, c);
+      sb[c] = __builtin_amdgcn_readlane(ms, c);
     }
   }
   // Column (within the tile) of S2 position q, its S2 and S starts.
@@ -344,63 +420,27 @@ struct TileBounds {
 };
 
 // kInit: M = M0 already on S; R = I - A M, Z = Pr R on S, P = 0, and the
-// traces of A Z (the pass before iteration 1).  Otherwise iteration `it`:
-// step sizes from the previous pass, update, next traces.
+// traces of A Z (the pass before iteration 1).  Otherwise one iteration with
+// the step sizes in sc: update, then the traces of the next iteration.
 template <class V, class SlotT, int GL, bool kInit>
 __global__ __launch_bounds__(kBlk) void LomrFused(
-    int n, int it, double tol, const int* __restrict__ sp, const int* __restrict__ si, const int* __restrict__ s2p,
-    const int* __restrict__ s2i, const int* __restrict__ s2_to_s, const int* __restrict__ ap,
-    const V* __restrict__ av, const int64_t* __restrict__ plan_ptr, const SlotT* __restrict__ plan, int lmax,
-    int smax, const V* __restrict__ pr, V* __restrict__ R, V* __restrict__ M, V* __restrict__ Z,
-    V* __restrict__ P, const double* __restrict__ part_in, double* __restrict__ part_out,
-    double* __restrict__ hist, Flags* __restrict__ flags) {
+    int n, const int* __restrict__ sp, const int2* __restrict__ a0c, const uint16_t* __restrict__ s_loc,
+    const int* __restrict__ s2p, const int* __restrict__ s2i, const V* __restrict__ av,
+    const int64_t* __restrict__ plan_ptr, const SlotT* __restrict__ plan, int lmax, int smax,
+    const V* __restrict__ prS, const V* __restrict__ prS2, V* __restrict__ R, V* __restrict__ M, V* __restrict__ Z,
+    V* __restrict__ P, const double* __restrict__ sc, double* __restrict__ part_out,
+    const Flags* __restrict__ flags) {
   constexpr int W = kWave / GL;
-  // block 0 of this same pass may set flags->stop: read it once per block
-  __shared__ int stop_seen;
-  if (threadIdx.x == 0) stop_seen = flags->stop;
-  __syncthreads();
-  if (stop_seen) return;
+  constexpr int RQ = 4;  // S2 entries of a tile held in registers per lane
+  constexpr int RS = 2;  // S entries of a tile held in registers per lane
+  if (flags->stop) return;
   V d = V(0), gm = V(0), be = V(0), nd = V(0), ng = V(0);
   if constexpr (!kInit) {
-    double s[6];
-    SumPartialsBlock<6>(part_in, gridDim.x, s);
-    const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
-    // history row it - 1 (result_operations.cpp:7-67: converged iff ||R|| < tol)
-    const double rn = sqrt(s[5]);
-    if (writer) {
-      hist[it - 1] = rn;
-      flags->iterations = it - 1;
-    }
-    if (rn < tol) {
-      if (writer) flags->stop = 2;
-      return;
-    }
-    // GetDeltaGamma (locally_minimal_residual.cpp:36-50)
-    const bool first = it == 1;
-    const double az_praz = s[0], ap_prap = first ? 0.0 : s[1], az_prap = first ? 0.0 : s[2], z_az = s[3],
-                 z_ap = first ? 0.0 : s[4];
-    double delta = 0.0, gamma = 0.0;
-    bool brk = false;
-    if (first) {
-      if (!(isfinite(az_praz) && fabs(az_praz) > 0.0)) {
-        brk = true;
-      } else {
-        delta = z_az / az_praz;
-        gamma = 1.0;
-      }
-    } else {
-      SolveLeastSquares2x2(az_praz, az_prap, az_prap, ap_prap, z_az, z_ap, delta, gamma);
-    }
-    if (brk || !isfinite(delta) || !isfinite(gamma)) {
-      if (writer) flags->stop = 1;
-      return;
-    }
-    const double ratio = delta != 0.0 ? gamma / delta : 0.0;
-    d = V(delta);
-    gm = V(gamma);
-    be = V(isfinite(ratio) ? ratio : 0.0);
-    nd = V(-delta);
-    ng = V(-gamma);
+    d = V(sc[kDelta]);
+    gm = V(sc[kGamma]);
+    be = V(sc[kBeta]);
+    nd = V(-sc[kDelta]);
+    ng = V(-sc[kGamma]);
   }
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = LaneId(), wib = threadIdx.x / kWave;
@@ -417,18 +457,48 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
     const bool valid = j < n;
     TileBounds<W> tb;
     tb.Load(j0, n, lane, s2p, sp);
+    const int qa = tb.bnd[0], qe = tb.bnd[W], pa = tb.sb[0], pe = tb.sb[W];
+    // operands of the updates, issued before the product so their latency
+    // overlaps it: R and Pr(i) on the lane's S2 entries, M, Z, P, Pr(i) and
+    // the S2 position on its S entries
+    V rq[RQ], pq[RQ];
+    int iq[RQ];
+#pragma unroll
+    for (int r = 0; r < RQ; ++r) {
+      const int q = qa + lane + r * kWave;
+      const bool in = q < qe;
+      rq[r] = in && !kInit ? R[q] : V(0);
+      pq[r] = in ? prS2[q] : V(0);
+      iq[r] = in && kInit ? s2i[q] : -1;
+    }
+    V mp[RS], zp[RS], ppv[RS], ps[RS];
+    int lp[RS];
+#pragma unroll
+    for (int r = 0; r < RS; ++r) {
+      const int p = pa + lane + r * kWave;
+      const bool in = p < pe;
+      mp[r] = in && !kInit ? M[p] : V(0);
+      zp[r] = in && !kInit ? Z[p] : V(0);
+      ppv[r] = in && !kInit ? P[p] : V(0);
+      ps[r] = in ? prS[p] : V(0);
+      lp[r] = in ? int(s_loc[p]) : 0;
+    }
     const int L = valid ? s2p[j + 1] - s2p[j] : 0;
+    V* const acc0 = mine;
+    V* const acc1 = mine + lmax;
+    V* const yz = mine + 3 * lmax;
+    V* const yp = yz + smax;
     // product 1: A [Z P](:, j)  (kInit: A M(:, j))
     for (int t = gl; t < L; t += GL) {
-      mine[t] = V(0);
-      if (!kInit) mine[lmax + t] = V(0);
+      acc0[t] = V(0);
+      if (!kInit) acc1[t] = V(0);
     }
     if (valid) {
       if constexpr (kInit) {
-        SlotAccumulate<V, SlotT, GL, 1>(j, gl, sp, si, ap, av, plan_ptr, plan, mine, mine + lmax,
+        SlotAccumulate<V, SlotT, GL, 1>(j, gl, sp, a0c, av, plan_ptr, plan, acc0, acc1,
                                         [&](int e, V& ya, V& yb) { ya = M[e]; });
       } else {
-        SlotAccumulate<V, SlotT, GL, 2>(j, gl, sp, si, ap, av, plan_ptr, plan, mine, mine + lmax,
+        SlotAccumulate<V, SlotT, GL, 2>(j, gl, sp, a0c, av, plan_ptr, plan, acc0, acc1,
                                         [&](int e, V& ya, V& yb) {
                                           ya = Z[e];
                                           yb = P[e];
@@ -436,59 +506,85 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
       }
     }
     __builtin_amdgcn_wave_barrier();
-    // update over the tile's contiguous S2 range
-    const int qend = tb.bnd[W];
-    for (int q = tb.bnd[0] + lane; q < qend; q += kWave) {
+    // R' = R - delta A Z - gamma A P over the tile's S2 range (kInit: I - A M)
+    auto update_r = [&](int q, V rold, int i) {
       int q0, s0;
       const int gg = tb.Find(q, q0, s0);
       const int t = q - q0;
       V* const grp = wave_lds + size_t(gg) * words;
-      const int i = s2i[q];
-      V r;
+      V rv;
       if constexpr (kInit) {
-        r = -grp[t];
-        if (i == j0 + gg) r += V(1);
+        rv = -grp[t];
+        if (i == j0 + gg) rv += V(1);
       } else {
-        r = R[q];
-        r = r + nd * grp[t];
-        r = r + ng * grp[lmax + t];
+        rv = rold;
+        rv = rv + nd * grp[t];
+        rv = rv + ng * grp[lmax + t];
       }
-      R[q] = r;
-      red[5] += double(r) * double(r);
-      const V pri = pr[i];
-      grp[2 * lmax + t] = pri;
-      grp[3 * lmax + t] = pri * r;
-      const int p = s2_to_s[q];
-      if (p >= 0) {
-        const V zn = pri * r;
-        V pn;
-        if constexpr (kInit) {
-          pn = V(0);
-        } else {
-          const V z = Z[p], pv = P[p];
-          V m = M[p];
-          m = m + d * z;
-          m = m + gm * pv;
-          M[p] = m;
-          pn = pv * be + z;
+      R[q] = rv;
+      red[5] += double(rv) * double(rv);
+      grp[2 * lmax + t] = rv;
+    };
+#pragma unroll
+    for (int r = 0; r < RQ; ++r) {
+      const int q = qa + lane + r * kWave;
+      if (q < qe) update_r(q, rq[r], iq[r]);
+    }
+    for (int r = RQ; qa + r * kWave < qe; ++r) {
+      const int q = qa + lane + r * kWave;
+      if (q < qe) update_r(q, kInit ? V(0) : R[q], kInit ? s2i[q] : -1);
+    }
+    __builtin_amdgcn_wave_barrier();
+    // M' = M + delta Z + gamma P, P' = beta P + Z, Z' = Pr R' over the S range
+    auto update_s = [&](int p, V mo, V zo, V po, V pri, int t) {
+      int gg = 0, sbase = tb.sb[0];
+#pragma unroll
+      for (int c = 1; c < W; ++c) {
+        if (p >= tb.sb[c]) {
+          gg = c;
+          sbase = tb.sb[c];
         }
-        Z[p] = zn;
-        P[p] = pn;
-        grp[4 * lmax + (p - s0)] = zn;
-        grp[4 * lmax + smax + (p - s0)] = pn;
+      }
+      V* const grp = wave_lds + size_t(gg) * words;
+      const V zn = pri * grp[2 * lmax + t];
+      V pn;
+      if constexpr (kInit) {
+        pn = V(0);
+      } else {
+        V m = mo;
+        m = m + d * zo;
+        m = m + gm * po;
+        M[p] = m;
+        pn = po * be + zo;
+      }
+      Z[p] = zn;
+      P[p] = pn;
+      grp[3 * lmax + (p - sbase)] = zn;
+      grp[3 * lmax + smax + (p - sbase)] = pn;
+    };
+#pragma unroll
+    for (int r = 0; r < RS; ++r) {
+      const int p = pa + lane + r * kWave;
+      if (p < pe) update_s(p, mp[r], zp[r], ppv[r], ps[r], lp[r]);
+    }
+    for (int r = RS; pa + r * kWave < pe; ++r) {
+      const int p = pa + lane + r * kWave;
+      if (p < pe) {
+        if constexpr (kInit)
+          update_s(p, V(0), V(0), V(0), prS[p], int(s_loc[p]));
+        else
+          update_s(p, M[p], Z[p], P[p], prS[p], int(s_loc[p]));
       }
     }
     __builtin_amdgcn_wave_barrier();
     // product 2: A [Z' P'](:, j) from LDS
     for (int t = gl; t < L; t += GL) {
-      mine[t] = V(0);
-      mine[lmax + t] = V(0);
+      acc0[t] = V(0);
+      acc1[t] = V(0);
     }
     if (valid) {
       const int sj = sp[j];
-      const V* yz = mine + 4 * lmax;
-      const V* yp = yz + smax;
-      SlotAccumulate<V, SlotT, GL, 2>(j, gl, sp, si, ap, av, plan_ptr, plan, mine, mine + lmax,
+      SlotAccumulate<V, SlotT, GL, 2>(j, gl, sp, a0c, av, plan_ptr, plan, acc0, acc1,
                                       [&](int e, V& ya, V& yb) {
                                         ya = yz[e - sj];
                                         yb = yp[e - sj];
@@ -496,21 +592,72 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
     }
     __builtin_amdgcn_wave_barrier();
     // traces of the next step (LomrProjectEpi order)
-    for (int q = tb.bnd[0] + lane; q < qend; q += kWave) {
+    auto traces = [&](int q, V pri) {
       int q0, s0;
       const int gg = tb.Find(q, q0, s0);
       const int t = q - q0;
       const V* grp = wave_lds + size_t(gg) * words;
-      const double daz = grp[t], dap = grp[lmax + t], dpr = grp[2 * lmax + t], dprr = grp[3 * lmax + t];
+      const V rv = grp[2 * lmax + t];
+      const double daz = grp[t], dap = grp[lmax + t], dpr = pri, dprr = double(pri * rv);
       red[0] += daz * (dpr * daz);
       red[1] += dap * (dpr * dap);
       red[2] += daz * (dpr * dap);
       red[3] += dprr * daz;
       red[4] += dprr * dap;
+    };
+#pragma unroll
+    for (int r = 0; r < RQ; ++r) {
+      const int q = qa + lane + r * kWave;
+      if (q < qe) traces(q, pq[r]);
+    }
+    for (int r = RQ; qa + r * kWave < qe; ++r) {
+      const int q = qa + lane + r * kWave;
+      if (q < qe) traces(q, prS2[q]);
     }
     __builtin_amdgcn_wave_barrier();
   }
   StorePartials<6>(red, part_out);
+}
+
+// Between passes it - 1 and it: history row it - 1 (result_operations.cpp:7-67,
+// converged iff ||R|| < tol), then GetDeltaGamma (locally_minimal_residual.cpp:36-50)
+// and the direction ratio gamma / delta for iteration it.
+__global__ __launch_bounds__(1024) void LomrFusedStep(const double* __restrict__ part, int G, int it, double tol,
+                                                      double* __restrict__ hist, double* __restrict__ sc,
+                                                      Flags* __restrict__ flags) {
+  if (flags->stop) return;
+  double s[6];
+  SumPartialsBlockFused<6>(part, G, s);
+  if (threadIdx.x != 0) return;
+  const double rn = sqrt(s[5]);
+  hist[it - 1] = rn;
+  flags->iterations = it - 1;
+  if (rn < tol) {
+    flags->stop = 2;
+    return;
+  }
+  const bool first = it == 1;
+  const double az_praz = s[0], ap_prap = first ? 0.0 : s[1], az_prap = first ? 0.0 : s[2], z_az = s[3],
+               z_ap = first ? 0.0 : s[4];
+  double delta = 0.0, gamma = 0.0;
+  if (first) {
+    if (!(isfinite(az_praz) && fabs(az_praz) > 0.0)) {
+      flags->stop = 1;
+      return;
+    }
+    delta = z_az / az_praz;
+    gamma = 1.0;
+  } else {
+    SolveLeastSquares2x2(az_praz, az_prap, az_prap, ap_prap, z_az, z_ap, delta, gamma);
+  }
+  if (!isfinite(delta) || !isfinite(gamma)) {
+    flags->stop = 1;
+    return;
+  }
+  const double ratio = delta != 0.0 ? gamma / delta : 0.0;
+  sc[kDelta] = delta;
+  sc[kGamma] = gamma;
+  sc[kBeta] = isfinite(ratio) ? ratio : 0.0;
 }
 
 // History row of the last pass (AppendResidual).
@@ -1126,6 +1273,30 @@ __global__ void FillZero(int64_t n, V* __restrict__ out) {
   if (i < n) out[i] = V(0);
 }
 
+// Fused-pass operand tables: (A(:, k) start, length) of each S entry,
+// Pr(i) of each S and S2 entry, and each S entry's position in its S2 column.
+__global__ void SEntryA(int64_t nnz_s, const int* __restrict__ si, const int* __restrict__ ap,
+                        int2* __restrict__ a0c) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= nnz_s) return;
+  const int k = si[e];
+  a0c[e] = make_int2(ap[k], ap[k + 1] - ap[k]);
+}
+
+template <class V>
+__global__ void GatherRows(int64_t nnz, const int* __restrict__ rows, const V* __restrict__ pr, V* __restrict__ out) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < nnz) out[e] = pr[rows[e]];
+}
+
+__global__ void SLocal(int64_t nnz2, const int* __restrict__ s2_to_s, const int* __restrict__ s2col,
+                       const int* __restrict__ s2p, uint16_t* __restrict__ s_loc) {
+  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= nnz2) return;
+  const int p = s2_to_s[q];
+  if (p >= 0) s_loc[p] = uint16_t(q - s2p[s2col[q]]);
+}
+
 unsigned GridFor(int64_t n) {
   int64_t g = (n + kBlk - 1) / kBlk;
   if (g < 1) g = 1;
@@ -1284,6 +1455,17 @@ class FixedImpl final : public FixedProblem {
     ConvertValues<V><<<Blocks1(at_pat_->nnz), kBlk, 0, c.stream>>>(at_pat_->nnz, at_d_.get(), atv_.get());
     prv_ = c.Alloc<V>(n_);
     ConvertValues<V><<<Blocks1(n_), kBlk, 0, c.stream>>>(n_, pr_d_.get(), prv_.get());
+    if (fused_ok_ && method_ == SPAI_GLOBAL_LOMR) {
+      a0c_ = c.Alloc<int2>(nnz_s_);
+      SEntryA<<<Blocks1(nnz_s_), kBlk, 0, c.stream>>>(nnz_s_, s_pat_->idx.get(), A_.ptr(), a0c_.get());
+      s_loc_ = c.Alloc<uint16_t>(nnz_s_);
+      SLocal<<<Blocks1(nnz_s2_), kBlk, 0, c.stream>>>(nnz_s2_, s2_to_s_.get(), s2col_.get(), s2_pat_->ptr.get(),
+                                                     s_loc_.get());
+      prS_ = c.Alloc<V>(nnz_s_);
+      GatherRows<V><<<Blocks1(nnz_s_), kBlk, 0, c.stream>>>(nnz_s_, s_pat_->idx.get(), prv_.get(), prS_.get());
+      prS2_ = c.Alloc<V>(nnz_s2_);
+      GatherRows<V><<<Blocks1(nnz_s2_), kBlk, 0, c.stream>>>(nnz_s2_, s2_pat_->idx.get(), prv_.get(), prS2_.get());
+    }
     M_ = c.Alloc<V>(nnz_s_);
     Z_ = c.Alloc<V>(nnz_s_);
     P_ = c.Alloc<V>(nnz_s_);
@@ -1293,7 +1475,7 @@ class FixedImpl final : public FixedProblem {
     grid2_ = GridFor(nnz_s2_);
     grid1_ = GridFor(nnz_s_);
     part_ = c.Alloc<double>(8 * kMaxGrid);
-    partf_ = c.Alloc<double>(2 * 6 * size_t(kMaxFusedGrid));
+    partf_ = c.Alloc<double>(6 * size_t(kMaxFusedGrid));
     sc_ = c.Alloc<double>(kNumSlots);
     flags_ = c.Alloc<Flags>(1);
     hist_ = c.Alloc<double>(p.max_iterations + 1);
@@ -1383,24 +1565,23 @@ class FixedImpl final : public FixedProblem {
     SampleOnPattern<V><<<Blocks1(n_), kBlk, 0, st>>>((int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), m0_.ptr(),
                                                      m0_.idx(), m0_.v(), M_.get());
     SPAI_HIP_CHECK(hipMemsetAsync(flags_.get(), 0, sizeof(Flags), st));
-    const size_t half = 6 * size_t(gridf_);
-    double* part[2] = {partf_.get(), partf_.get() + half};
+    double* part = partf_.get();
     const double tol = prm_.tolerance;
     for (int64_t it = 0; it <= prm_.max_iterations; ++it) {
-      double* pin = part[(it + 1) & 1];
-      double* pout = part[it & 1];
+      if (it > 0)
+        LomrFusedStep<<<1, 1024, 0, st>>>(part, (int)gridf_, (int)it, tol, hist_.get(), sc_.get(), flags_.get());
 #define SPAI_FUSED_LAUNCH(SlotT, plan, GLV)                                                                       \
   do {                                                                                                            \
     if (it == 0)                                                                                                  \
       LomrFused<V, SlotT, GLV, true><<<gridf_, kBlk, smemf_, st>>>(                                               \
-          (int)n_, 0, tol, s_pat_->ptr.get(), s_pat_->idx.get(), s2_pat_->ptr.get(), s2_pat_->idx.get(),          \
-          s2_to_s_.get(), A_.ptr(), av_.get(), plan_ptr_.get(), plan, lmax_, smax_, prv_.get(), R_.get(),        \
-          M_.get(), Z_.get(), P_.get(), pin, pout, hist_.get(), flags_.get());                                   \
+          (int)n_, s_pat_->ptr.get(), a0c_.get(), s_loc_.get(), s2_pat_->ptr.get(), s2_pat_->idx.get(),          \
+          av_.get(), plan_ptr_.get(), plan, lmax_, smax_, prS_.get(), prS2_.get(), R_.get(), M_.get(), Z_.get(), \
+          P_.get(), sc_.get(), part, flags_.get());                                                               \
     else                                                                                                          \
       LomrFused<V, SlotT, GLV, false><<<gridf_, kBlk, smemf_, st>>>(                                              \
-          (int)n_, (int)it, tol, s_pat_->ptr.get(), s_pat_->idx.get(), s2_pat_->ptr.get(), s2_pat_->idx.get(),    \
-          s2_to_s_.get(), A_.ptr(), av_.get(), plan_ptr_.get(), plan, lmax_, smax_, prv_.get(), R_.get(),        \
-          M_.get(), Z_.get(), P_.get(), pin, pout, hist_.get(), flags_.get());                                   \
+          (int)n_, s_pat_->ptr.get(), a0c_.get(), s_loc_.get(), s2_pat_->ptr.get(), s2_pat_->idx.get(),          \
+          av_.get(), plan_ptr_.get(), plan, lmax_, smax_, prS_.get(), prS2_.get(), R_.get(), M_.get(), Z_.get(), \
+          P_.get(), sc_.get(), part, flags_.get());                                                               \
   } while (0)
 #define SPAI_FUSED_GL(SlotT, plan)                          \
   switch (gl_) {                                            \
@@ -1417,8 +1598,8 @@ class FixedImpl final : public FixedProblem {
 #undef SPAI_FUSED_GL
 #undef SPAI_FUSED_LAUNCH
     }
-    LomrFusedFinish<<<1, kBlk, 0, st>>>(part[prm_.max_iterations & 1], (int)gridf_, (int)prm_.max_iterations, tol,
-                                        hist_.get(), flags_.get());
+    LomrFusedFinish<<<1, kBlk, 0, st>>>(part, (int)gridf_, (int)prm_.max_iterations, tol, hist_.get(),
+                                        flags_.get());
     SPAI_HIP_CHECK(hipGetLastError());
     Flags f;
     c.ReadBack(&f, flags_.get(), sizeof(Flags));
@@ -1574,6 +1755,9 @@ class FixedImpl final : public FixedProblem {
   bool fused_ok_ = false;
   unsigned gridf_ = 1;
   DBuf<double> partf_;
+  DBuf<int2> a0c_;
+  DBuf<uint16_t> s_loc_;
+  DBuf<V> prS_, prS2_;
   size_t smem_ = 0;
   bool slot16_ = false, use_plan_ = false;
   unsigned gridp_ = 1;

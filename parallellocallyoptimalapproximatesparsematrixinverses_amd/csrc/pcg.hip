@@ -1,21 +1,25 @@
 // Preconditioned conjugate gradient of methods_cython/backward_error.pyx:69-144
 // on the device.
 //
-// One iteration is three kernels:
-//   K1  finish the previous iteration (beta, history, stop rules), then
-//       p = z + beta p fused into Ap = A p, and <p, Ap>;
-//   K2  alpha, x += alpha p, r -= alpha Ap, <r, r>;
-//   K3  error = ||r|| / ||b||, z = M r, <r, z>.
-// Each kernel writes one partial sum per block; every block of the next kernel
-// finishes that reduction itself (the same fixed-order sum in every block, so
-// all blocks take the same decision and results are deterministic), and block
-// 0 records outcomes in a device-side state that only later kernels read;
-// values that one kernel reads and the next rewrites are double-buffered by
-// iteration parity.  When the recursive residual crosses the tolerance, K3
-// pauses the device loop and the host runs the true-residual refresh of
-// backward_error.pyx:116-122 (three kernels, once per solve) before resuming.
-// Iterations are replayed in chunks from captured hipGraphs (one per starting
-// parity); once the state says done, the remaining kernels return at once.
+// One iteration is two kernels:
+//   K1  finish the previous iteration (error, refresh decision, beta, history,
+//       stop rules), then p = z + beta p fused into Ap = A p, and <p, Ap>;
+//   K2  alpha, x += alpha p, r' = r - alpha Ap, and z = M r' with the entries
+//       of r' that a row of M needs formed on the fly from r and Ap (the same
+//       expression, so the same bits), then <r', r'> and <r', z>.
+// r is double-buffered by iteration parity so that K2 reads the old r while
+// it writes the new one.  Each kernel writes one partial sum per block; every
+// block of the next kernel finishes that reduction itself (the same
+// fixed-order sum in every block, so all blocks take the same decision and
+// results are deterministic), and block 0 records outcomes in a device-side
+// state that only later kernels read.  When the recursive residual crosses
+// the tolerance, K1 pauses the device loop and the host runs the
+// true-residual refresh of backward_error.pyx:116-122 (three kernels, once per
+// solve) before resuming.  Iterations are replayed in chunks from captured
+// hipGraphs (one per starting parity); once the state says done, the
+// remaining kernels return at once.  Rows are mapped to blocks so that each
+// XCD (blocks b = x mod 8) works on one contiguous eighth of the rows, which
+// keeps a stencil's plane neighbours in the same L2.
 
 #include "pcg.hpp"
 
@@ -135,6 +139,13 @@ __device__ __forceinline__ double SumPartials(const double* __restrict__ part, i
 
 __device__ __forceinline__ bool Finite(double v) { return isfinite(v); }
 
+// Logical block of this block: consecutive logical blocks share an XCD
+// (workgroups are dealt to the 8 XCDs round robin).
+__device__ __forceinline__ int XcdBlock() {
+  const int G = gridDim.x, b = blockIdx.x;
+  return (G & 7) ? b : (b & 7) * (G >> 3) + (b >> 3);
+}
+
 // End of an iteration (backward_error.pyx:123-133,100): beta, history entry,
 // counters, status.  Returns beta; sets stop when the loop must end.  Writes
 // happen in block 0 only; slots written are never read by the same kernel.
@@ -204,8 +215,35 @@ __global__ __launch_bounds__(kBlk) void PcgInitFinish(const double* __restrict__
   is[kDone] = (1 < is[kMax] && 1.0 > ds[kTol]) ? 0 : 1;
 }
 
-// K1 of iteration with parity par: end of the previous iteration, then
-// p_out = z + beta p_in and Ap = A p_out; partials of <p_out, Ap>.
+// Operands of a thread's first row (one thread per row): loaded before the
+// kernel's prologue reduction so that their latency overlaps it.  x0/x1 are
+// the two gathered vectors at the row's first 8 columns.
+template <class MV>
+struct RowPrefetch {
+  int b, e;
+  MV v[8];
+  double x0[8], x1[8];
+  // row < 0: nothing to load (empty row)
+  __device__ __forceinline__ void Load(const int* __restrict__ ptr, const int* __restrict__ col,
+                                       const MV* __restrict__ val, const double* __restrict__ y0,
+                                       const double* __restrict__ y1, int row) {
+    b = row >= 0 ? ptr[row] : 0;
+    e = row >= 0 ? ptr[row + 1] : 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int k = b + u;
+      const bool in = k < e;
+      const int c = in ? col[k] : 0;
+      v[u] = in ? val[k] : MV(0);
+      x0[u] = in ? y0[c] : 0.0;
+      x1[u] = in ? y1[c] : 0.0;
+    }
+  }
+};
+
+// K1 of iteration with parity par: end of the previous iteration (error from
+// the previous K2's <r', r'>; at the tolerance crossing, pause for the host
+// refresh), then p_out = z + beta p_in and Ap = A p_out; partials of <p_out, Ap>.
 template <int SG>
 __global__ __launch_bounds__(kBlk) void PcgDirSpmvA(int n, int par, const int* __restrict__ ap_,
                                                     const int* __restrict__ ac, const double* __restrict__ av,
@@ -216,19 +254,66 @@ __global__ __launch_bounds__(kBlk) void PcgDirSpmvA(int n, int par, const int* _
   if (is[kDone]) return;
   const int G = gridDim.x;
   const int prev = par ^ 1;
+  const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
+  const int row0 = XcdBlock() * (kBlk / SG) + threadIdx.x / SG;
+  RowPrefetch<double> pf;
+  double z0 = 0.0, p0 = 0.0;
+  if constexpr (SG == 1) {
+    const bool have = row0 < n;
+    pf.Load(ap_, ac, av, z, p_in, have ? row0 : -1);
+    z0 = have ? z[row0] : 0.0;
+    p0 = have ? p_in[row0] : 0.0;
+  }
   double beta = 0.0;
   if (is[prev ? kPend1 : kPend0]) {
+    const double err = sqrt(SumPartials(part + kP2 * G, G)) / ds[kBnorm];
+    if (err <= ds[kTol]) {
+      if (writer) {
+        ds[kErrPre] = err;
+        is[kNeedRefresh] = 1;
+        is[kDone] = 1;
+      }
+      return;
+    }
     const double r_z = SumPartials(part + kP4 * G, G);
     bool stop = false;
-    beta = EndIteration(prev, r_z, ds[kErrPre], false, ds, is, hist, stop, blockIdx.x == 0 && threadIdx.x == 0);
+    beta = EndIteration(prev, r_z, err, false, ds, is, hist, stop, writer);
     if (stop) return;
   }
   const int groups = kBlk / SG, sl = threadIdx.x % SG;
   double acc = 0.0;
-  for (int row = blockIdx.x * groups + threadIdx.x / SG; row < n; row += gridDim.x * groups) {
-    const double s = RowDotDir<SG>(ap_, ac, av, z, p_in, beta, row, sl);
+  for (int row = row0; row < n; row += G * groups) {
+    double s, pr;
+    if (SG == 1 && row == row0) {
+      // first 8 entries from the prefetched operands, the rest as RowDotDir
+      double t[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) t[u] = pf.b + u < pf.e ? pf.v[u] * (pf.x0[u] + beta * pf.x1[u]) : 0.0;
+      s = 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (pf.b + u < pf.e) s += t[u];
+      for (int k0 = pf.b + 8; k0 < pf.e; k0 += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int k = k0 + u;
+          if (k < pf.e) {
+            const int c = ac[k];
+            t[u] = av[k] * (z[c] + beta * p_in[c]);
+          } else {
+            t[u] = 0.0;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (k0 + u < pf.e) s += t[u];
+      }
+      pr = z0 + beta * p0;
+    } else {
+      s = RowDotDir<SG>(ap_, ac, av, z, p_in, beta, row, sl);
+      pr = z[row] + beta * p_in[row];
+    }
     if (sl == 0) {
-      const double pr = z[row] + beta * p_in[row];
       p_out[row] = pr;
       Ap[row] = s;
       acc += pr * s;
@@ -237,13 +322,65 @@ __global__ __launch_bounds__(kBlk) void PcgDirSpmvA(int n, int par, const int* _
   StorePartial(acc, part + kP1 * G);
 }
 
-// K2: alpha (backward_error.pyx:102-110), x += alpha p, r -= alpha Ap; <r, r>.
-__global__ __launch_bounds__(kBlk) void PcgUpdate(int n, int par, double* __restrict__ x, double* __restrict__ r,
-                                                  const double* __restrict__ p, const double* __restrict__ Ap,
-                                                  double* __restrict__ part, double* __restrict__ ds,
-                                                  int* __restrict__ is) {
+// Row of M times r' = r - alpha Ap, r' formed per entry (same order as RowDot).
+template <int SG, class MV>
+__device__ __forceinline__ double RowDotUpdated(const int* __restrict__ ptr, const int* __restrict__ col,
+                                                const MV* __restrict__ val, const double* __restrict__ r,
+                                                const double* __restrict__ Ap, double alpha, int row, int sl) {
+  double s = 0.0;
+  if constexpr (SG == 1) {
+    const int b = ptr[row], e = ptr[row + 1];
+    for (int k0 = b; k0 < e; k0 += 8) {
+      double t[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = k0 + u;
+        if (k < e) {
+          const int c = col[k];
+          t[u] = double(val[k]) * (r[c] - alpha * Ap[c]);
+        } else {
+          t[u] = 0.0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (k0 + u < e) s += t[u];
+    }
+  } else {
+    const int e = ptr[row + 1];
+    for (int k = ptr[row] + sl; k < e; k += SG) {
+      const int c = col[k];
+      s += double(val[k]) * (r[c] - alpha * Ap[c]);
+    }
+#pragma unroll
+    for (int off = SG / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, SG);
+  }
+  return s;
+}
+
+// K2: alpha (backward_error.pyx:102-110), x += alpha p, r' = r - alpha Ap,
+// z = M r'; partials of <r', r'> and <r', z> (backward_error.pyx:111-124).
+template <int SG, class MV>
+__global__ __launch_bounds__(kBlk) void PcgUpdateSpmvM(int n, int par, const int* __restrict__ mp,
+                                                       const int* __restrict__ mc, const MV* __restrict__ mv,
+                                                       double* __restrict__ x, const double* __restrict__ r_in,
+                                                       double* __restrict__ r_out, const double* __restrict__ p,
+                                                       const double* __restrict__ Ap, double* __restrict__ z,
+                                                       double* __restrict__ part, double* __restrict__ ds,
+                                                       int* __restrict__ is) {
   if (is[kDone]) return;
   const int G = gridDim.x;
+  const int row0 = XcdBlock() * (kBlk / SG) + threadIdx.x / SG;
+  RowPrefetch<MV> pf;
+  double x0 = 0.0, p0 = 0.0, r0 = 0.0, a0 = 0.0;
+  if constexpr (SG == 1) {
+    const bool have = row0 < n;
+    pf.Load(mp, mc, mv, r_in, Ap, have ? row0 : -1);
+    x0 = have ? x[row0] : 0.0;
+    p0 = have ? p[row0] : 0.0;
+    r0 = have ? r_in[row0] : 0.0;
+    a0 = have ? Ap[row0] : 0.0;
+  }
   const double den = SumPartials(part + kP1 * G, G);
   const double r_z = ds[par ? kRz1 : kRz0];
   bool brk = !Finite(r_z) || !Finite(den) || r_z == 0.0 || den == 0.0;
@@ -256,51 +393,55 @@ __global__ __launch_bounds__(kBlk) void PcgUpdate(int n, int par, double* __rest
       is[kDone] = 1;
     } else {
       is[kIter] = is[kIter] + 1;
-    }
-  }
-  if (brk) return;
-  double acc = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    x[i] += alpha * p[i];
-    const double ri = r[i] - alpha * Ap[i];
-    r[i] = ri;
-    acc += ri * ri;
-  }
-  StorePartial(acc, part + kP2 * G);
-}
-
-// K3: error; at the tolerance crossing pause for the host-side refresh;
-// otherwise z = M r and <r, z>.
-template <int SG, class MV>
-__global__ __launch_bounds__(kBlk) void PcgSpmvM(int n, int par, const int* __restrict__ mp,
-                                                 const int* __restrict__ mc, const MV* __restrict__ mv,
-                                                 const double* __restrict__ r, double* __restrict__ z,
-                                                 double* __restrict__ part, double* __restrict__ ds,
-                                                 int* __restrict__ is) {
-  if (is[kDone]) return;
-  const int G = gridDim.x;
-  const double err = sqrt(SumPartials(part + kP2 * G, G)) / ds[kBnorm];
-  const bool refresh = err <= ds[kTol];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    ds[kErrPre] = err;
-    if (refresh) {
-      is[kNeedRefresh] = 1;
-      is[kDone] = 1;
-    } else {
       is[par ? kPend1 : kPend0] = 1;
     }
   }
-  if (refresh) return;
+  if (brk) return;
   const int groups = kBlk / SG, sl = threadIdx.x % SG;
-  double acc = 0.0;
-  for (int row = blockIdx.x * groups + threadIdx.x / SG; row < n; row += gridDim.x * groups) {
-    const double s = RowDot<SG, MV>(mp, mc, mv, r, row, sl);
+  double rr = 0.0, rz = 0.0;
+  for (int row = row0; row < n; row += G * groups) {
+    double zs, xv, ri;
+    if (SG == 1 && row == row0) {
+      double t[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        t[u] = pf.b + u < pf.e ? double(pf.v[u]) * (pf.x0[u] - alpha * pf.x1[u]) : 0.0;
+      zs = 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (pf.b + u < pf.e) zs += t[u];
+      for (int k0 = pf.b + 8; k0 < pf.e; k0 += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int k = k0 + u;
+          if (k < pf.e) {
+            const int c = mc[k];
+            t[u] = double(mv[k]) * (r_in[c] - alpha * Ap[c]);
+          } else {
+            t[u] = 0.0;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (k0 + u < pf.e) zs += t[u];
+      }
+      xv = x0 + alpha * p0;
+      ri = r0 - alpha * a0;
+    } else {
+      zs = RowDotUpdated<SG, MV>(mp, mc, mv, r_in, Ap, alpha, row, sl);
+      xv = x[row] + alpha * p[row];
+      ri = r_in[row] - alpha * Ap[row];
+    }
     if (sl == 0) {
-      z[row] = s;
-      acc += r[row] * s;
+      x[row] = xv;
+      r_out[row] = ri;
+      z[row] = zs;
+      rr += ri * ri;
+      rz += ri * zs;
     }
   }
-  StorePartial(acc, part + kP4 * G);
+  StorePartial(rr, part + kP2 * G);
+  StorePartial(rz, part + kP4 * G);
 }
 
 // Host-launched refresh (backward_error.pyx:117-122): r = b - A x; <r, r>.
@@ -351,6 +492,7 @@ __global__ __launch_bounds__(kBlk) void PcgRefreshFinish(int par, const double* 
   if (threadIdx.x != 0) return;
   is[kNeedRefresh] = 0;
   is[kDone] = 0;
+  is[par ? kPend1 : kPend0] = 0;  // finished here; K1 of the next iteration starts with beta = 0
   bool stop = false;
   EndIteration(par, r_z, ds[kErrPre], true, ds, is, hist, stop, true);
 }
@@ -395,7 +537,7 @@ __global__ __launch_bounds__(kBlk) void PcgSpmvA(int n, const int* __restrict__ 
                                                  double* __restrict__ Ap, double* __restrict__ part) {
   const int groups = kBlk / SG, sl = threadIdx.x % SG;
   double acc = 0.0;
-  for (int row = blockIdx.x * groups + threadIdx.x / SG; row < n; row += gridDim.x * groups) {
+  for (int row = XcdBlock() * groups + threadIdx.x / SG; row < n; row += gridDim.x * groups) {
     const double s = RowDot<SG, double>(ap_, ac, av, p, row, sl);
     if (sl == 0) {
       Ap[row] = s;
@@ -489,7 +631,7 @@ PcgSolver::PcgSolver(Ctx& ctx, const DCsr& A, const DCsr& M, bool m_fp32)
   if (m_fp32 && !M.val32.get()) throw std::invalid_argument("FP32 copy of M missing");
   grid_ = Grid(n_, kBlk, ctx.num_cus);
   x_ = ctx.Alloc<double>(n_);
-  r_ = ctx.Alloc<double>(n_);
+  r_ = ctx.Alloc<double>(2 * n_);
   z_ = ctx.Alloc<double>(n_);
   p_ = ctx.Alloc<double>(2 * n_);
   ap_ = ctx.Alloc<double>(n_);
@@ -533,20 +675,20 @@ void PcgSolver::Record(int first_parity, int64_t iterations) {
     const int par = int((first_parity + it) & 1);
     double* p_in = p_.get() + size_t(par) * n_;
     double* p_out = p_.get() + size_t(par ^ 1) * n_;
+    const double* r_in = r_.get() + size_t(par) * n_;
+    double* r_out = r_.get() + size_t(par ^ 1) * n_;
     SPAI_SG_DISPATCH(sga, (PcgDirSpmvA<SG><<<grid_, kBlk, 0, st>>>(n, par, A_.ptr.get(), A_.col.get(),
                                                                    A_.val.get(), z_.get(), p_in, p_out, ap_.get(),
                                                                    part_.get(), scal_.get(), ist_.get(),
                                                                    hist_.get())));
-    PcgUpdate<<<grid_, kBlk, 0, st>>>(n, par, x_.get(), r_.get(), p_out, ap_.get(), part_.get(), scal_.get(),
-                                      ist_.get());
     if (m_fp32_) {
-      SPAI_SG_DISPATCH(sgm, (PcgSpmvM<SG, float><<<grid_, kBlk, 0, st>>>(n, par, M_.ptr.get(), M_.col.get(),
-                                                                         M_.val32.get(), r_.get(), z_.get(),
-                                                                         part_.get(), scal_.get(), ist_.get())));
+      SPAI_SG_DISPATCH(sgm, (PcgUpdateSpmvM<SG, float><<<grid_, kBlk, 0, st>>>(
+                                n, par, M_.ptr.get(), M_.col.get(), M_.val32.get(), x_.get(), r_in, r_out, p_out,
+                                ap_.get(), z_.get(), part_.get(), scal_.get(), ist_.get())));
     } else {
-      SPAI_SG_DISPATCH(sgm, (PcgSpmvM<SG, double><<<grid_, kBlk, 0, st>>>(n, par, M_.ptr.get(), M_.col.get(),
-                                                                          M_.val.get(), r_.get(), z_.get(),
-                                                                          part_.get(), scal_.get(), ist_.get())));
+      SPAI_SG_DISPATCH(sgm, (PcgUpdateSpmvM<SG, double><<<grid_, kBlk, 0, st>>>(
+                                n, par, M_.ptr.get(), M_.col.get(), M_.val.get(), x_.get(), r_in, r_out, p_out,
+                                ap_.get(), z_.get(), part_.get(), scal_.get(), ist_.get())));
     }
   }
 }
@@ -595,16 +737,18 @@ int PcgSolver::Solve(const double* b_dev, int64_t max_iterations, double toleran
     ++chunks;
     ctx_.ReadBack(hst, ist_.get(), sizeof(hst));
     if (hst[kNeedRefresh]) {
+      // the last K2 had parity par and left r' in buffer par ^ 1
       const int par = (hst[kIter] - 1) & 1;
+      double* r_cur = r_.get() + size_t(par ^ 1) * n_;
       SPAI_SG_DISPATCH(sga, (PcgRefresh<SG><<<grid_, kBlk, 0, st>>>(n, A_.ptr.get(), A_.col.get(), A_.val.get(),
-                                                                    x_.get(), b_dev, r_.get(), part_.get())));
+                                                                    x_.get(), b_dev, r_cur, part_.get())));
       if (m_fp32_) {
         SPAI_SG_DISPATCH(sgm, (PcgRefreshSpmvM<SG, float><<<grid_, kBlk, 0, st>>>(
-                                  n, M_.ptr.get(), M_.col.get(), M_.val32.get(), r_.get(), z_.get(), part_.get(),
+                                  n, M_.ptr.get(), M_.col.get(), M_.val32.get(), r_cur, z_.get(), part_.get(),
                                   scal_.get())));
       } else {
         SPAI_SG_DISPATCH(sgm, (PcgRefreshSpmvM<SG, double><<<grid_, kBlk, 0, st>>>(
-                                  n, M_.ptr.get(), M_.col.get(), M_.val.get(), r_.get(), z_.get(), part_.get(),
+                                  n, M_.ptr.get(), M_.col.get(), M_.val.get(), r_cur, z_.get(), part_.get(),
                                   scal_.get())));
       }
       PcgRefreshFinish<<<1, kBlk, 0, st>>>(par, part_.get(), (int)grid_, scal_.get(), ist_.get(), hist_.get());
