@@ -386,46 +386,21 @@ __device__ __forceinline__ void SlotAccumulate(int j, int gl, const int* __restr
   }
 }
 
-// Tile bounds: bnd[c] = s2p[min(j0 + c, n)], sb[c] = sp[min(j0 + c, n)].
-template <int W>
-struct TileBounds {
-  int bnd[W + 1];
-  int sb[W + 1];
-  __device__ __forceinline__ void Load(int j0, int n, int lane, const int* __restrict__ s2p,
-                                       const int* __restrict__ sp) {
-    const int jb = min(j0 + (lane <= W ? lane : 0), n);
-    const int mb = s2p[jb], ms = sp[jb];
-    // wave-uniform: read lanes into scalar registers
-#pragma unroll
-    for (int c = 0; c <= W; ++c) {
-      bnd[c] = __builtin_amdgcn_readlane(mb, c);
-      sb[c] = __builtin_amdgcn_readlane(ms, c);
-    }
-  }
-  // Column (within the tile) of S2 position q, its S2 and S starts.
-  __device__ __forceinline__ int Find(int q, int& q0, int& s0) const {
-    int g = 0;
-    q0 = bnd[0];
-    s0 = sb[0];
-#pragma unroll
-    for (int c = 1; c < W; ++c) {
-      if (q >= bnd[c]) {
-        g = c;
-        q0 = bnd[c];
-        s0 = sb[c];
-      }
-    }
-    return g;
-  }
-};
+// Tile-local position tables of the fused pass.  qinfo[q] (S2 entry q of
+// column j): (j mod W) << kQInfoShift | slot of q in S2(:, j).  sinfo[p] (S
+// entry p of column j): (j mod W) << 26 | slot of p in S2(:, j) << 16 |
+// position of p in S(:, j).  W = 64 / GL columns per wavefront tile.
+constexpr int kQInfoShift = 10;
+constexpr int kQInfoMask = (1 << kQInfoShift) - 1;
 
 // kInit: M = M0 already on S; R = I - A M, Z = Pr R on S, P = 0, and the
 // traces of A Z (the pass before iteration 1).  Otherwise one iteration with
 // the step sizes in sc: update, then the traces of the next iteration.
 template <class V, class SlotT, int GL, bool kInit>
 __global__ __launch_bounds__(kBlk) void LomrFused(
-    int n, const int* __restrict__ sp, const int2* __restrict__ a0c, const uint16_t* __restrict__ s_loc,
-    const int* __restrict__ s2p, const int* __restrict__ s2i, const V* __restrict__ av,
+    int n, const int* __restrict__ sp, const int2* __restrict__ a0c, const uint16_t* __restrict__ qinfo,
+    const uint32_t* __restrict__ sinfo, const int* __restrict__ s2p, const int* __restrict__ s2i,
+    const V* __restrict__ av,
     const int64_t* __restrict__ plan_ptr, const SlotT* __restrict__ plan, int lmax, int smax,
     const V* __restrict__ prS, const V* __restrict__ prS2, V* __restrict__ R, V* __restrict__ M, V* __restrict__ Z,
     V* __restrict__ P, const double* __restrict__ sc, double* __restrict__ part_out,
@@ -455,14 +430,14 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
     const int j0 = tile * W;
     const int j = j0 + g;
     const bool valid = j < n;
-    TileBounds<W> tb;
-    tb.Load(j0, n, lane, s2p, sp);
-    const int qa = tb.bnd[0], qe = tb.bnd[W], pa = tb.sb[0], pe = tb.sb[W];
+    const int je = min(j0 + W, n);
+    const int qa = __builtin_amdgcn_readfirstlane(s2p[j0]), qe = __builtin_amdgcn_readfirstlane(s2p[je]);
+    const int pa = __builtin_amdgcn_readfirstlane(sp[j0]), pe = __builtin_amdgcn_readfirstlane(sp[je]);
     // operands of the updates, issued before the product so their latency
     // overlaps it: R and Pr(i) on the lane's S2 entries, M, Z, P, Pr(i) and
     // the S2 position on its S entries
     V rq[RQ], pq[RQ];
-    int iq[RQ];
+    int iq[RQ], nq[RQ];
 #pragma unroll
     for (int r = 0; r < RQ; ++r) {
       const int q = qa + lane + r * kWave;
@@ -470,9 +445,10 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
       rq[r] = in && !kInit ? R[q] : V(0);
       pq[r] = in ? prS2[q] : V(0);
       iq[r] = in && kInit ? s2i[q] : -1;
+      nq[r] = in ? int(qinfo[q]) : 0;
     }
     V mp[RS], zp[RS], ppv[RS], ps[RS];
-    int lp[RS];
+    uint32_t np[RS];
 #pragma unroll
     for (int r = 0; r < RS; ++r) {
       const int p = pa + lane + r * kWave;
@@ -481,7 +457,7 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
       zp[r] = in && !kInit ? Z[p] : V(0);
       ppv[r] = in && !kInit ? P[p] : V(0);
       ps[r] = in ? prS[p] : V(0);
-      lp[r] = in ? int(s_loc[p]) : 0;
+      np[r] = in ? sinfo[p] : 0u;
     }
     const int L = valid ? s2p[j + 1] - s2p[j] : 0;
     V* const acc0 = mine;
@@ -507,10 +483,8 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
     }
     __builtin_amdgcn_wave_barrier();
     // R' = R - delta A Z - gamma A P over the tile's S2 range (kInit: I - A M)
-    auto update_r = [&](int q, V rold, int i) {
-      int q0, s0;
-      const int gg = tb.Find(q, q0, s0);
-      const int t = q - q0;
+    auto update_r = [&](int q, V rold, int i, int info) {
+      const int gg = info >> kQInfoShift, t = info & kQInfoMask;
       V* const grp = wave_lds + size_t(gg) * words;
       V rv;
       if constexpr (kInit) {
@@ -528,23 +502,16 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
 #pragma unroll
     for (int r = 0; r < RQ; ++r) {
       const int q = qa + lane + r * kWave;
-      if (q < qe) update_r(q, rq[r], iq[r]);
+      if (q < qe) update_r(q, rq[r], iq[r], nq[r]);
     }
     for (int r = RQ; qa + r * kWave < qe; ++r) {
       const int q = qa + lane + r * kWave;
-      if (q < qe) update_r(q, kInit ? V(0) : R[q], kInit ? s2i[q] : -1);
+      if (q < qe) update_r(q, kInit ? V(0) : R[q], kInit ? s2i[q] : -1, int(qinfo[q]));
     }
     __builtin_amdgcn_wave_barrier();
     // M' = M + delta Z + gamma P, P' = beta P + Z, Z' = Pr R' over the S range
-    auto update_s = [&](int p, V mo, V zo, V po, V pri, int t) {
-      int gg = 0, sbase = tb.sb[0];
-#pragma unroll
-      for (int c = 1; c < W; ++c) {
-        if (p >= tb.sb[c]) {
-          gg = c;
-          sbase = tb.sb[c];
-        }
-      }
+    auto update_s = [&](int p, V mo, V zo, V po, V pri, uint32_t info) {
+      const int gg = int(info >> 26), t = int((info >> 16) & 1023u), pl = int(info & 0xFFFFu);
       V* const grp = wave_lds + size_t(gg) * words;
       const V zn = pri * grp[2 * lmax + t];
       V pn;
@@ -559,21 +526,21 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
       }
       Z[p] = zn;
       P[p] = pn;
-      grp[3 * lmax + (p - sbase)] = zn;
-      grp[3 * lmax + smax + (p - sbase)] = pn;
+      grp[3 * lmax + pl] = zn;
+      grp[3 * lmax + smax + pl] = pn;
     };
 #pragma unroll
     for (int r = 0; r < RS; ++r) {
       const int p = pa + lane + r * kWave;
-      if (p < pe) update_s(p, mp[r], zp[r], ppv[r], ps[r], lp[r]);
+      if (p < pe) update_s(p, mp[r], zp[r], ppv[r], ps[r], np[r]);
     }
     for (int r = RS; pa + r * kWave < pe; ++r) {
       const int p = pa + lane + r * kWave;
       if (p < pe) {
         if constexpr (kInit)
-          update_s(p, V(0), V(0), V(0), prS[p], int(s_loc[p]));
+          update_s(p, V(0), V(0), V(0), prS[p], sinfo[p]);
         else
-          update_s(p, M[p], Z[p], P[p], prS[p], int(s_loc[p]));
+          update_s(p, M[p], Z[p], P[p], prS[p], sinfo[p]);
       }
     }
     __builtin_amdgcn_wave_barrier();
@@ -592,10 +559,8 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
     }
     __builtin_amdgcn_wave_barrier();
     // traces of the next step (LomrProjectEpi order)
-    auto traces = [&](int q, V pri) {
-      int q0, s0;
-      const int gg = tb.Find(q, q0, s0);
-      const int t = q - q0;
+    auto traces = [&](int q, V pri, int info) {
+      const int gg = info >> kQInfoShift, t = info & kQInfoMask;
       const V* grp = wave_lds + size_t(gg) * words;
       const V rv = grp[2 * lmax + t];
       const double daz = grp[t], dap = grp[lmax + t], dpr = pri, dprr = double(pri * rv);
@@ -608,11 +573,11 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
 #pragma unroll
     for (int r = 0; r < RQ; ++r) {
       const int q = qa + lane + r * kWave;
-      if (q < qe) traces(q, pq[r]);
+      if (q < qe) traces(q, pq[r], nq[r]);
     }
     for (int r = RQ; qa + r * kWave < qe; ++r) {
       const int q = qa + lane + r * kWave;
-      if (q < qe) traces(q, prS2[q]);
+      if (q < qe) traces(q, prS2[q], int(qinfo[q]));
     }
     __builtin_amdgcn_wave_barrier();
   }
@@ -1289,12 +1254,17 @@ __global__ void GatherRows(int64_t nnz, const int* __restrict__ rows, const V* _
   if (e < nnz) out[e] = pr[rows[e]];
 }
 
-__global__ void SLocal(int64_t nnz2, const int* __restrict__ s2_to_s, const int* __restrict__ s2col,
-                       const int* __restrict__ s2p, uint16_t* __restrict__ s_loc) {
+__global__ void TileInfo(int64_t nnz2, int W, const int* __restrict__ s2_to_s, const int* __restrict__ s2col,
+                         const int* __restrict__ s2p, const int* __restrict__ sp, uint16_t* __restrict__ qinfo,
+                         uint32_t* __restrict__ sinfo) {
   const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (q >= nnz2) return;
+  const int j = s2col[q];
+  const int t = int(q - s2p[j]);
+  const int g = j % W;
+  qinfo[q] = uint16_t((g << kQInfoShift) | t);
   const int p = s2_to_s[q];
-  if (p >= 0) s_loc[p] = uint16_t(q - s2p[s2col[q]]);
+  if (p >= 0) sinfo[p] = (uint32_t(g) << 26) | (uint32_t(t) << 16) | uint32_t(p - sp[j]);
 }
 
 unsigned GridFor(int64_t n) {
@@ -1425,7 +1395,9 @@ class FixedImpl final : public FixedProblem {
         // fused LOMR pass: W = 64 / GL columns per wavefront
         smax_ = std::max(h[2], 1);
         smemf_ = size_t(kBlk / gl_) * size_t(FusedGroupWords(lmax_, smax_)) * sizeof(V);
-        fused_ok_ = smemf_ <= 60 * 1024;  // plus the kernel's static LDS, under 64 KiB
+        // LDS under 64 KiB with the kernel's static part; tile tables need
+        // S2 columns of at most 1024 entries and S columns of at most 65536
+        fused_ok_ = smemf_ <= 60 * 1024 && lmax_ <= 1024 && smax_ <= 65536;
         const int64_t ntiles = (n_ + (kWave / gl_) - 1) / (kWave / gl_);
         gridf_ = (unsigned)std::min<int64_t>(std::max<int64_t>((ntiles + kBlk / kWave - 1) / (kBlk / kWave), 1),
                                              kMaxFusedGrid);
@@ -1458,9 +1430,11 @@ class FixedImpl final : public FixedProblem {
     if (fused_ok_ && method_ == SPAI_GLOBAL_LOMR) {
       a0c_ = c.Alloc<int2>(nnz_s_);
       SEntryA<<<Blocks1(nnz_s_), kBlk, 0, c.stream>>>(nnz_s_, s_pat_->idx.get(), A_.ptr(), a0c_.get());
-      s_loc_ = c.Alloc<uint16_t>(nnz_s_);
-      SLocal<<<Blocks1(nnz_s2_), kBlk, 0, c.stream>>>(nnz_s2_, s2_to_s_.get(), s2col_.get(), s2_pat_->ptr.get(),
-                                                     s_loc_.get());
+      qinfo_ = c.Alloc<uint16_t>(nnz_s2_);
+      sinfo_ = c.Alloc<uint32_t>(nnz_s_);
+      TileInfo<<<Blocks1(nnz_s2_), kBlk, 0, c.stream>>>(nnz_s2_, kWave / gl_, s2_to_s_.get(), s2col_.get(),
+                                                       s2_pat_->ptr.get(), s_pat_->ptr.get(), qinfo_.get(),
+                                                       sinfo_.get());
       prS_ = c.Alloc<V>(nnz_s_);
       GatherRows<V><<<Blocks1(nnz_s_), kBlk, 0, c.stream>>>(nnz_s_, s_pat_->idx.get(), prv_.get(), prS_.get());
       prS2_ = c.Alloc<V>(nnz_s2_);
@@ -1574,14 +1548,14 @@ class FixedImpl final : public FixedProblem {
   do {                                                                                                            \
     if (it == 0)                                                                                                  \
       LomrFused<V, SlotT, GLV, true><<<gridf_, kBlk, smemf_, st>>>(                                               \
-          (int)n_, s_pat_->ptr.get(), a0c_.get(), s_loc_.get(), s2_pat_->ptr.get(), s2_pat_->idx.get(),          \
-          av_.get(), plan_ptr_.get(), plan, lmax_, smax_, prS_.get(), prS2_.get(), R_.get(), M_.get(), Z_.get(), \
-          P_.get(), sc_.get(), part, flags_.get());                                                               \
+          (int)n_, s_pat_->ptr.get(), a0c_.get(), qinfo_.get(), sinfo_.get(), s2_pat_->ptr.get(),              \
+          s2_pat_->idx.get(), av_.get(), plan_ptr_.get(), plan, lmax_, smax_, prS_.get(), prS2_.get(), R_.get(), \
+          M_.get(), Z_.get(), P_.get(), sc_.get(), part, flags_.get());                                           \
     else                                                                                                          \
       LomrFused<V, SlotT, GLV, false><<<gridf_, kBlk, smemf_, st>>>(                                              \
-          (int)n_, s_pat_->ptr.get(), a0c_.get(), s_loc_.get(), s2_pat_->ptr.get(), s2_pat_->idx.get(),          \
-          av_.get(), plan_ptr_.get(), plan, lmax_, smax_, prS_.get(), prS2_.get(), R_.get(), M_.get(), Z_.get(), \
-          P_.get(), sc_.get(), part, flags_.get());                                                               \
+          (int)n_, s_pat_->ptr.get(), a0c_.get(), qinfo_.get(), sinfo_.get(), s2_pat_->ptr.get(),              \
+          s2_pat_->idx.get(), av_.get(), plan_ptr_.get(), plan, lmax_, smax_, prS_.get(), prS2_.get(), R_.get(), \
+          M_.get(), Z_.get(), P_.get(), sc_.get(), part, flags_.get());                                           \
   } while (0)
 #define SPAI_FUSED_GL(SlotT, plan)                          \
   switch (gl_) {                                            \
@@ -1756,7 +1730,8 @@ class FixedImpl final : public FixedProblem {
   unsigned gridf_ = 1;
   DBuf<double> partf_;
   DBuf<int2> a0c_;
-  DBuf<uint16_t> s_loc_;
+  DBuf<uint16_t> qinfo_;
+  DBuf<uint32_t> sinfo_;
   DBuf<V> prS_, prS2_;
   size_t smem_ = 0;
   bool slot16_ = false, use_plan_ = false;
