@@ -300,40 +300,54 @@ __global__ void ExclusiveScan64Serial(int n, const int64_t* __restrict__ in, int
 // R/S-sized reads and writes of a tile run over the tile's contiguous range
 // with all 64 lanes, so stores cover whole lines.
 
-// Per-group LDS region of the fused pass: accumulators (2 x lmax), R' of the
-// column (lmax), Z' and P' of the column (2 x smax).
-__host__ __device__ constexpr int FusedGroupWords(int lmax, int smax) { return 3 * lmax + 2 * smax; }
+// Per-group LDS region of the fused pass, in values of vsize bytes:
+// accumulators (2 x lmax), R' of the column (lmax), Z' and P' of the column
+// (2 x smax), then the staging of SlotAccumulate (GL x (int2 + 2 values)),
+// 16-byte aligned.
+__host__ __device__ constexpr int FusedGroupWords(int lmax, int smax, int gl, int vsize) {
+  return (((3 * lmax + 2 * smax) * vsize + 15) / 16 * 16 + gl * (8 + 2 * vsize) + 15) / 16 * 16 / vsize;
+}
+__host__ __device__ constexpr int FusedStageOffset(int lmax, int smax, int vsize) {
+  return ((3 * lmax + 2 * smax) * vsize + 15) / 16 * 16 / vsize;
+}
 
 
 // Column-wise Gustavson accumulation of A [y0 y1](:, j) into acc0/acc1, in
 // the order of SlotProduct (S entries of the column in order, k ascending).
-// YGet(e, ya, yb) returns S entry e's values.  A step (one S entry, column k
-// of A) gives lane gl the term u = gl when |A(:, k)| <= GL: the plan slots and
-// A values of up to kPre steps are loaded before any of them is added, so a
-// column waits for one load latency per kPre steps, not one per step.
+// YGet(e, ya, yb) returns S entry e's values.  The group stages up to GL S
+// entries -- (A(:, k) start, length) and (ya, yb) -- in LDS, then every step
+// (one S entry, column k of A) reads them with one broadcast LDS load per
+// array; when |A(:, k)| <= GL lane gl takes the term u = gl, and the plan
+// slots and A values of up to kPre steps are loaded before any of them is
+// added, so a column waits for one load latency per kPre steps.
+template <class V>
+struct alignas(2 * sizeof(V)) V2 {
+  V a, b;
+};
+
 template <class V, class SlotT, int GL, int NY, class YGet>
 __device__ __forceinline__ void SlotAccumulate(int j, int gl, const int* __restrict__ sp, const int2* __restrict__ a0c,
                                                const V* __restrict__ av, const int64_t* __restrict__ plan_ptr,
-                                               const SlotT* __restrict__ plan, V* acc0, V* acc1, YGet yget) {
+                                               const SlotT* __restrict__ plan, V* acc0, V* acc1, int2* st_ac,
+                                               V2<V>* st_y, YGet yget) {
   constexpr int kPre = 4;
-  int64_t pp = plan_ptr[j];
+  uint32_t pp = uint32_t(plan_ptr[j]);
   const int e0 = sp[j], e1 = sp[j + 1];
   for (int eb = e0; eb < e1; eb += GL) {
-    int my_a0 = 0, my_c = 0;
-    V my_ya = V(0), my_yb = V(0);
     if (eb + gl < e1) {
-      const int2 ac = a0c[eb + gl];
-      my_a0 = ac.x;
-      my_c = ac.y;
-      yget(eb + gl, my_ya, my_yb);
+      V ya = V(0), yb = V(0);
+      yget(eb + gl, ya, yb);
+      st_ac[gl] = a0c[eb + gl];
+      st_y[gl] = V2<V>{ya, yb};
     }
+    __builtin_amdgcn_wave_barrier();
     const int ns = min(GL, e1 - eb);
     for (int s0 = 0; s0 < ns; s0 += kPre) {
       const int cnt = min(kPre, ns - s0);
       SlotT sl[kPre];
       V a[kPre], ya[kPre], yb[kPre];
       int cc[kPre];
-      int64_t off = pp;
+      uint32_t off = pp;
       bool wide = false;
 #pragma unroll
       for (int s = 0; s < kPre; ++s) {
@@ -341,18 +355,17 @@ __device__ __forceinline__ void SlotAccumulate(int j, int gl, const int* __restr
         sl[s] = SlotT(0);
         a[s] = ya[s] = yb[s] = V(0);
         if (s < cnt) {
-          const int st = s0 + s;
-          const int a0 = __shfl(my_a0, st, GL);
-          const int c = __shfl(my_c, st, GL);
-          ya[s] = __shfl(my_ya, st, GL);
-          if (NY > 1) yb[s] = __shfl(my_yb, st, GL);
-          cc[s] = c;
-          wide = wide || c > GL;
-          if (gl < c) {
-            sl[s] = plan[off + gl];
-            a[s] = av[a0 + gl];
+          const int2 ac = st_ac[s0 + s];
+          const V2<V> y = st_y[s0 + s];
+          ya[s] = y.a;
+          yb[s] = y.b;
+          cc[s] = ac.y;
+          wide = wide || ac.y > GL;
+          if (gl < ac.y) {
+            sl[s] = plan[off + uint32_t(gl)];
+            a[s] = av[uint32_t(ac.x + gl)];
           }
-          off += c;
+          off += uint32_t(ac.y);
         }
       }
       if (!wide) {
@@ -365,24 +378,22 @@ __device__ __forceinline__ void SlotAccumulate(int j, int gl, const int* __restr
         }
       } else {
         // a column of A longer than GL: lanes stride through each step
-        int64_t o2 = pp;
+        uint32_t o2 = pp;
         for (int s = 0; s < cnt; ++s) {
-          const int st = s0 + s;
-          const int a0 = __shfl(my_a0, st, GL);
-          const int c = __shfl(my_c, st, GL);
-          const V y0 = __shfl(my_ya, st, GL);
-          const V y1 = NY > 1 ? __shfl(my_yb, st, GL) : V(0);
-          for (int u = gl; u < c; u += GL) {
-            const int slot = plan[o2 + u];
-            const V av_u = av[a0 + u];
-            acc0[slot] += av_u * y0;
-            if (NY > 1) acc1[slot] += av_u * y1;
+          const int2 ac = st_ac[s0 + s];
+          const V2<V> y = st_y[s0 + s];
+          for (int u = gl; u < ac.y; u += GL) {
+            const int slot = plan[o2 + uint32_t(u)];
+            const V av_u = av[uint32_t(ac.x + u)];
+            acc0[slot] += av_u * y.a;
+            if (NY > 1) acc1[slot] += av_u * y.b;
           }
-          o2 += c;
+          o2 += uint32_t(ac.y);
         }
       }
       pp = off;
     }
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -420,9 +431,11 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = LaneId(), wib = threadIdx.x / kWave;
   const int g = lane / GL, gl = lane % GL;
-  const int words = FusedGroupWords(lmax, smax);
+  const int words = FusedGroupWords(lmax, smax, GL, int(sizeof(V)));
   V* const wave_lds = reinterpret_cast<V*>(smem_raw) + size_t(wib) * W * words;
   V* const mine = wave_lds + size_t(g) * words;
+  int2* const st_ac = reinterpret_cast<int2*>(mine + FusedStageOffset(lmax, smax, int(sizeof(V))));
+  V2<V>* const st_y = reinterpret_cast<V2<V>*>(st_ac + GL);
   double red[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   const int ntiles = (n + W - 1) / W;
   const int wstride = gridDim.x * (kBlk / kWave);
@@ -471,10 +484,10 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
     }
     if (valid) {
       if constexpr (kInit) {
-        SlotAccumulate<V, SlotT, GL, 1>(j, gl, sp, a0c, av, plan_ptr, plan, acc0, acc1,
+        SlotAccumulate<V, SlotT, GL, 1>(j, gl, sp, a0c, av, plan_ptr, plan, acc0, acc1, st_ac, st_y,
                                         [&](int e, V& ya, V& yb) { ya = M[e]; });
       } else {
-        SlotAccumulate<V, SlotT, GL, 2>(j, gl, sp, a0c, av, plan_ptr, plan, acc0, acc1,
+        SlotAccumulate<V, SlotT, GL, 2>(j, gl, sp, a0c, av, plan_ptr, plan, acc0, acc1, st_ac, st_y,
                                         [&](int e, V& ya, V& yb) {
                                           ya = Z[e];
                                           yb = P[e];
@@ -551,7 +564,7 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
     }
     if (valid) {
       const int sj = sp[j];
-      SlotAccumulate<V, SlotT, GL, 2>(j, gl, sp, a0c, av, plan_ptr, plan, acc0, acc1,
+      SlotAccumulate<V, SlotT, GL, 2>(j, gl, sp, a0c, av, plan_ptr, plan, acc0, acc1, st_ac, st_y,
                                       [&](int e, V& ya, V& yb) {
                                         ya = yz[e - sj];
                                         yb = yp[e - sj];
@@ -1394,10 +1407,10 @@ class FixedImpl final : public FixedProblem {
         gridp_ = (unsigned)std::min<int64_t>(std::max<int64_t>((n_ + groups - 1) / groups, 1), kMaxGrid);
         // fused LOMR pass: W = 64 / GL columns per wavefront
         smax_ = std::max(h[2], 1);
-        smemf_ = size_t(kBlk / gl_) * size_t(FusedGroupWords(lmax_, smax_)) * sizeof(V);
+        smemf_ = size_t(kBlk / gl_) * size_t(FusedGroupWords(lmax_, smax_, gl_, int(sizeof(V)))) * sizeof(V);
         // LDS under 64 KiB with the kernel's static part; tile tables need
         // S2 columns of at most 1024 entries and S columns of at most 65536
-        fused_ok_ = smemf_ <= 60 * 1024 && lmax_ <= 1024 && smax_ <= 65536;
+        fused_ok_ = smemf_ <= 60 * 1024 && lmax_ <= 1024 && smax_ <= 65536 && total < (int64_t(1) << 32);
         const int64_t ntiles = (n_ + (kWave / gl_) - 1) / (kWave / gl_);
         gridf_ = (unsigned)std::min<int64_t>(std::max<int64_t>((ntiles + kBlk / kWave - 1) / (kBlk / kWave), 1),
                                              kMaxFusedGrid);
