@@ -9,6 +9,11 @@ The timed form is FP32-bulk (M, directions, residual and products in FP32;
 step sizes, trace products, ||R||_F and the whole PCG recurrence in FP64); the
 plain FP64 form is timed the same way and reported beside it.
 
+roofline: the dominant kernel of the FP32-bulk step, the fused LOMR pass (one
+iteration), timed with HIP events on the session's stream over one extra build
+after the timed steps; algorithmic bytes per launch are stated in main().
+roofline_pcg_spmv: the PCG's A-SpMV, timed by a standalone replay.
+
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
 from __future__ import annotations
@@ -65,8 +70,25 @@ def time_form(lib, A, M0, b, precision, steps, warmup):
                true_rel_residual=float(np.linalg.norm(b - A @ x) / np.linalg.norm(b)))
     out["M"] = s.result()["M"]
     out["spmv_a_us"] = s.spmv_a_us()
+    # kernel timing: the same build once more with HIP events recorded on the
+    # session's stream around every per-iteration pass (outside the timed
+    # steps, whose pacing the events would change)
+    s.set_kernel_timing(True)
+    s.build()
+    us, launches = s.build_kernel_us()
+    out["lomr_pass_us"] = us if us > 0 else None
+    out["lomr_passes_timed"] = launches
     s.close()
     return out
+
+
+def s2_nnz(A):
+    """nnz of pattern(A S) | S | I with S = pattern(A) (Boolean product)."""
+    from scipy import sparse
+
+    B = sparse.csc_matrix((np.ones(A.nnz), A.indices, A.indptr), shape=A.shape)
+    C = (B @ B + B + sparse.identity(A.shape[0], format="csc")).tocsc()
+    return int(C.nnz)
 
 
 def cpu_baseline(A, M0, b, threads):
@@ -130,15 +152,29 @@ def main():
     fp32 = time_form(_lib, A, M0, b, "fp32", args.steps, args.warmup)
     dM = float(np.linalg.norm((fp32["M"] - fp64["M"]).data) / np.linalg.norm(fp64["M"].data))
 
-    # roofline: the PCG SpMV with A (CSR, int32 indices, FP64 values), per launch:
-    # (n+1)*4 + nnz*(4+8) + 2*8*n bytes (read x, write y)
-    spmv_bytes = (n + 1) * 4 + nnz * 12 + 16 * n
-    us = fp32["spmv_a_us"]
-    achieved = spmv_bytes / (us * 1e-6) / 1e9
+    # roofline of the dominant kernel of the FP32-bulk step: the fused LOMR
+    # pass (one iteration).  Algorithmic bytes per launch, int32 indices and
+    # pointers, s_v = 4: A once ((n+1)*4 + nnz(A)*(4+4)), the patterns of S
+    # and S2 once ((n+1)*4 + nnz*4 each), R read and written (2*nnz(S2)*4),
+    # M, Z, P read and written (6*nnz(S)*4).
+    nnz_s = nnz  # S = pattern(A) | pattern(M0) | I = pattern(A) here
+    nnz_s2 = s2_nnz(A)
+    sv = 4
+    pass_bytes = ((n + 1) * 4 + nnz * (4 + sv) + (n + 1) * 4 + nnz_s * 4 + (n + 1) * 4 + nnz_s2 * 4
+                  + 2 * nnz_s2 * sv + 6 * nnz_s * sv)
+    us = fp32["lomr_pass_us"]
+    achieved = pass_bytes / (us * 1e-6) / 1e9
     traffic = None
-    prof = ROOT / "profiles" / "pcg_spmv_a_diffusion64_traffic.json"
+    prof = ROOT / "profiles" / "lomr_fused_diffusion64_traffic.json"
     if prof.exists():
-        traffic = json.loads(prof.read_text()).get("bytes_per_launch")
+        traffic = json.loads(prof.read_text()).get("fp32_bytes_per_launch")
+    # the PCG's A-SpMV, (n+1)*4 + nnz*(4+8) + 2*8*n bytes per launch
+    spmv_bytes = (n + 1) * 4 + nnz * 12 + 16 * n
+    spmv_us = fp32["spmv_a_us"]
+    spmv_traffic = None
+    prof2 = ROOT / "profiles" / "pcg_spmv_a_diffusion64_traffic.json"
+    if prof2.exists():
+        spmv_traffic = json.loads(prof2.read_text()).get("bytes_per_launch")
 
     threads = min(16, os.cpu_count() or 1)
     cpu = None if args.no_cpu_baseline else cpu_baseline(A, M0, b, threads)
@@ -164,10 +200,16 @@ def main():
         "speedup_fp32_vs_fp64": round(fp64["seconds"] / fp32["seconds"], 4),
         "extra_pcg_iterations_fp32": fp32["pcg_iterations"] - fp64["pcg_iterations"],
         "rel_fro_M32_vs_M64": dM,
-        "roofline": {"kernel": "PcgSpmvA (CSR SpMV with A + <p, Ap>)", "bound": "hbm",
+        "roofline": {"kernel": "LomrFused<float> (one fused LOMR iteration)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": spmv_bytes, "avg_launch_us": round(us, 3)},
+                     "algorithmic_bytes_per_launch": pass_bytes, "avg_launch_us": round(us, 3),
+                     "launches_timed": fp32["lomr_passes_timed"]},
+        "roofline_pcg_spmv": {"kernel": "PcgSpmvA (CSR SpMV with A + <p, Ap>), standalone replay", "bound": "hbm",
+                              "achieved": round(spmv_bytes / (spmv_us * 1e-6) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": round(spmv_bytes / (spmv_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                              "traffic": spmv_traffic, "algorithmic_bytes_per_launch": spmv_bytes,
+                              "avg_launch_us": round(spmv_us, 3)},
         "cpu_baseline": cpu,
         "device": _lib.device_name(),
     }

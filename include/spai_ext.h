@@ -134,6 +134,12 @@ struct Result spai_session_result(SpaiSession* session);
 /* Per-kernel timing of the last PCG iteration loop: average microseconds of
  * one SpMV with A (HIP events around a replay of the iteration graph). */
 double spai_session_pcg_iteration_us(SpaiSession* session);
+/* on != 0: later builds record HIP events, on the session's stream, around
+ * each per-iteration pass of the fused LOMR build. */
+int64_t spai_session_set_kernel_timing(SpaiSession* session, int64_t on);
+/* Average microseconds of that pass in the last build and (launches) how many
+ * passes were averaged; negative when the last build was not timed. */
+double spai_session_build_kernel_us(SpaiSession* session, int64_t* launches);
 
 /* ------------------------------------------------------------------------
  * Differentiation form: f(M, A) = ||I - A M||_F^2 on fixed patterns.

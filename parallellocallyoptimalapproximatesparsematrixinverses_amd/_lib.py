@@ -107,6 +107,10 @@ def load():
     L.spai_session_result.argtypes = [ctypes.c_void_p]
     L.spai_session_pcg_iteration_us.restype = ctypes.c_double
     L.spai_session_pcg_iteration_us.argtypes = [ctypes.c_void_p]
+    L.spai_session_set_kernel_timing.restype = ctypes.c_int64
+    L.spai_session_set_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int64]
+    L.spai_session_build_kernel_us.restype = ctypes.c_double
+    L.spai_session_build_kernel_us.argtypes = [ctypes.c_void_p, I64P]
     _lib = L
     return L
 
@@ -285,6 +289,16 @@ class Session:
         if v < 0:
             raise RuntimeError(last_error())
         return float(v)
+
+    def set_kernel_timing(self, on=True):
+        if load().spai_session_set_kernel_timing(self._h, 1 if on else 0) < 0:
+            raise RuntimeError(last_error())
+
+    def build_kernel_us(self):
+        """(average microseconds, passes) of the timed per-iteration pass of the last build."""
+        n = ctypes.c_int64(0)
+        us = load().spai_session_build_kernel_us(self._h, ctypes.byref(n))
+        return float(us), int(n.value)
 
     def close(self):
         if self._h:

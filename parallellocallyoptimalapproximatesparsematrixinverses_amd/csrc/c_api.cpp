@@ -264,6 +264,24 @@ int64_t spai_pcg_matvec(SpaiPcgSystem* s, const double* x, double* y) {
   }
 }
 
+int64_t spai_session_set_kernel_timing(SpaiSession* s, int64_t on) {
+  try {
+    if (!s) throw std::invalid_argument("null session");
+    s->prob->SetKernelTiming(on != 0);
+    return 0;
+  } catch (const std::exception& ex) {
+    return Fail(ex.what());
+  }
+}
+
+double spai_session_build_kernel_us(SpaiSession* s, int64_t* launches) {
+  if (!s) {
+    g_last_error = "null session";
+    return -1.0;
+  }
+  return s->prob->KernelMicros(launches);
+}
+
 const char* spai_last_error(void) { return g_last_error.c_str(); }
 
 int64_t spai_device_name(char* buffer, int64_t size) {

@@ -1545,10 +1545,28 @@ class FixedImpl final : public FixedProblem {
     return (int64_t)io_hist_.size() - 1;
   }
 
+  ~FixedImpl() override {
+    for (hipEvent_t e : events_)
+      if (e) (void)hipEventDestroy(e);
+  }
+
+  void SetKernelTiming(bool on) override { timing_ = on; }
+
+  double KernelMicros(int64_t* launches) override {
+    if (launches) *launches = timed_launches_;
+    return timed_launches_ > 0 ? timed_us_ : -1.0;
+  }
+
   // Global LOMR with one fused pass per iteration (LomrFused).
   int64_t BuildLomrFused() {
     Ctx& c = ctx_;
     hipStream_t st = c.stream;
+    timed_launches_ = 0;
+    if (timing_ && (int64_t)events_.size() < 2 * prm_.max_iterations) {
+      for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+      events_.assign(2 * prm_.max_iterations, nullptr);
+      for (auto& e : events_) SPAI_HIP_CHECK(hipEventCreate(&e));
+    }
     SampleOnPattern<V><<<Blocks1(n_), kBlk, 0, st>>>((int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), m0_.ptr(),
                                                      m0_.idx(), m0_.v(), M_.get());
     SPAI_HIP_CHECK(hipMemsetAsync(flags_.get(), 0, sizeof(Flags), st));
@@ -1577,11 +1595,14 @@ class FixedImpl final : public FixedProblem {
     case 32: SPAI_FUSED_LAUNCH(SlotT, plan, 32); break;     \
     default: SPAI_FUSED_LAUNCH(SlotT, plan, 64); break;     \
   }
+      const bool timed = timing_ && it > 0;
+      if (timed) SPAI_HIP_CHECK(hipEventRecord(events_[2 * (it - 1)], st));
       if (slot16_) {
         SPAI_FUSED_GL(uint16_t, plan16_.get());
       } else {
         SPAI_FUSED_GL(uint8_t, plan8_.get());
       }
+      if (timed) SPAI_HIP_CHECK(hipEventRecord(events_[2 * (it - 1) + 1], st));
 #undef SPAI_FUSED_GL
 #undef SPAI_FUSED_LAUNCH
     }
@@ -1591,10 +1612,22 @@ class FixedImpl final : public FixedProblem {
     Flags f;
     c.ReadBack(&f, flags_.get(), sizeof(Flags));
     last_flags_ = f;
+    if (timing_) {
+      // passes 1..iterations ran in full; later ones returned at once
+      double total = 0.0;
+      for (int64_t it = 1; it <= f.iterations; ++it) {
+        float ms = 0.f;
+        SPAI_HIP_CHECK(hipEventElapsedTime(&ms, events_[2 * (it - 1)], events_[2 * (it - 1) + 1]));
+        total += ms;
+      }
+      timed_launches_ = f.iterations;
+      timed_us_ = f.iterations > 0 ? total * 1e3 / double(f.iterations) : 0.0;
+    }
     return f.iterations;
   }
 
   int64_t Build() override {
+    timed_launches_ = 0;
     if (method_ >= SPAI_INNER_OUTER_MR) return BuildInnerOuter();
     if (method_ == SPAI_GLOBAL_LOMR && use_plan_ && fused_ok_) return BuildLomrFused();
     Ctx& c = ctx_;
@@ -1740,6 +1773,10 @@ class FixedImpl final : public FixedProblem {
   int maxcol_ = 1, lmax_ = 1, gl_ = 8, smax_ = 1;
   size_t smemf_ = 0;
   bool fused_ok_ = false;
+  bool timing_ = false;
+  std::vector<hipEvent_t> events_;
+  double timed_us_ = 0.0;
+  int64_t timed_launches_ = 0;
   unsigned gridf_ = 1;
   DBuf<double> partf_;
   DBuf<int2> a0c_;

@@ -153,3 +153,26 @@ def test_session_build_and_solve(gpu_lib, oracle_lib):
     # rebuild reuses the session
     assert s.build() == 12
     s.close()
+
+
+def test_session_kernel_timing(gpu_lib):
+    """HIP-event timing of the fused LOMR pass: one average per build over the
+    iterations that ran; builds of other methods report none."""
+    A = problems.diffusion3d(12, seed=1)
+    Pr = problems.diag_inverse(A)
+    s = gpu_lib.Session("lomr", A, M0=Pr, max_iterations=7)
+    s.build()
+    assert s.build_kernel_us() == (-1.0, 0)
+    s.set_kernel_timing(True)
+    assert s.build() == 7
+    us, launches = s.build_kernel_us()
+    assert launches == 7 and 0.0 < us < 1e5
+    s.set_kernel_timing(False)
+    s.build()
+    assert s.build_kernel_us()[1] == 0
+    s.close()
+    m = gpu_lib.Session("mr", A, M0=Pr, max_iterations=3)
+    m.set_kernel_timing(True)
+    m.build()
+    assert m.build_kernel_us() == (-1.0, 0)
+    m.close()
