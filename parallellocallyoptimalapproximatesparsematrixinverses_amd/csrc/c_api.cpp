@@ -334,11 +334,18 @@ int64_t spai_session_build(SpaiSession* s) {
     if (!s) throw std::invalid_argument("null session");
     const int64_t it = s->prob->Build();
     spai::Ctx& c = s->prob->context();
-    // PCG operand: CSR of M, rebuilt from the new values
-    s->pcg.reset();
-    s->m_csr = std::make_unique<spai::DCsr>(spai::CsrFromCsc(c, s->prob->MatrixM()));
-    if (s->fp32) spai::MakeFp32Copy(c, *s->m_csr);
-    s->pcg = std::make_unique<spai::PcgSolver>(c, *s->a_csr, *s->m_csr, s->fp32);
+    // PCG operand: CSR of M.  The pattern is fixed, so after the first build
+    // only the values are refreshed and the solver (and its recorded graphs)
+    // is kept.
+    spai::DCsr fresh = spai::CsrFromCsc(c, s->prob->MatrixM());
+    if (!s->m_csr) {
+      s->m_csr = std::make_unique<spai::DCsr>(std::move(fresh));
+      if (s->fp32) spai::MakeFp32Copy(c, *s->m_csr);
+      s->pcg = std::make_unique<spai::PcgSolver>(c, *s->a_csr, *s->m_csr, s->fp32);
+    } else {
+      spai::CopyValues(c, *s->m_csr, fresh);
+      c.Sync();
+    }
     return it;
   } catch (const std::exception& ex) {
     return Fail(ex.what());

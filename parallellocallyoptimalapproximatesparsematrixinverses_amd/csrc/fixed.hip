@@ -638,6 +638,12 @@ __global__ __launch_bounds__(1024) void LomrFusedStep(const double* __restrict__
   sc[kBeta] = isfinite(ratio) ? ratio : 0.0;
 }
 
+// Clears the stop flags (a kernel, so that recorded graphs hold no memset).
+__global__ void ResetFlags(Flags* __restrict__ flags) {
+  flags->stop = 0;
+  flags->iterations = 0;
+}
+
 // History row of the last pass (AppendResidual).
 __global__ __launch_bounds__(kBlk) void LomrFusedFinish(const double* __restrict__ part, int G, int it, double tol,
                                                         double* __restrict__ hist, Flags* __restrict__ flags) {
@@ -1548,9 +1554,19 @@ class FixedImpl final : public FixedProblem {
   ~FixedImpl() override {
     for (hipEvent_t e : events_)
       if (e) (void)hipEventDestroy(e);
+    if (build_exec_) (void)hipGraphExecDestroy(build_exec_);
+    if (build_graph_) (void)hipGraphDestroy(build_graph_);
   }
 
   void SetKernelTiming(bool on) override { timing_ = on; }
+
+  int64_t FinishFusedBuild() {
+    SPAI_HIP_CHECK(hipGetLastError());
+    Flags f;
+    ctx_.ReadBack(&f, flags_.get(), sizeof(Flags));
+    last_flags_ = f;
+    return f.iterations;
+  }
 
   double KernelMicros(int64_t* launches) override {
     if (launches) *launches = timed_launches_;
@@ -1567,9 +1583,16 @@ class FixedImpl final : public FixedProblem {
       events_.assign(2 * prm_.max_iterations, nullptr);
       for (auto& e : events_) SPAI_HIP_CHECK(hipEventCreate(&e));
     }
+    // untimed builds replay one recorded graph of the whole launch sequence
+    const bool record = !timing_ && !build_exec_;
+    if (!timing_ && build_exec_) {
+      SPAI_HIP_CHECK(hipGraphLaunch(build_exec_, st));
+      return FinishFusedBuild();
+    }
+    if (record) SPAI_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     SampleOnPattern<V><<<Blocks1(n_), kBlk, 0, st>>>((int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), m0_.ptr(),
                                                      m0_.idx(), m0_.v(), M_.get());
-    SPAI_HIP_CHECK(hipMemsetAsync(flags_.get(), 0, sizeof(Flags), st));
+    ResetFlags<<<1, 1, 0, st>>>(flags_.get());
     double* part = partf_.get();
     const double tol = prm_.tolerance;
     for (int64_t it = 0; it <= prm_.max_iterations; ++it) {
@@ -1608,6 +1631,12 @@ class FixedImpl final : public FixedProblem {
     }
     LomrFusedFinish<<<1, kBlk, 0, st>>>(part, (int)gridf_, (int)prm_.max_iterations, tol, hist_.get(),
                                         flags_.get());
+    if (record) {
+      SPAI_HIP_CHECK(hipStreamEndCapture(st, &build_graph_));
+      SPAI_HIP_CHECK(hipGraphInstantiate(&build_exec_, build_graph_, nullptr, nullptr, 0));
+      SPAI_HIP_CHECK(hipGraphLaunch(build_exec_, st));
+      return FinishFusedBuild();
+    }
     SPAI_HIP_CHECK(hipGetLastError());
     Flags f;
     c.ReadBack(&f, flags_.get(), sizeof(Flags));
@@ -1774,6 +1803,8 @@ class FixedImpl final : public FixedProblem {
   size_t smemf_ = 0;
   bool fused_ok_ = false;
   bool timing_ = false;
+  hipGraph_t build_graph_ = nullptr;
+  hipGraphExec_t build_exec_ = nullptr;
   std::vector<hipEvent_t> events_;
   double timed_us_ = 0.0;
   int64_t timed_launches_ = 0;

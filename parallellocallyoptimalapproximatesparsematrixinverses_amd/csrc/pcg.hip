@@ -625,6 +625,15 @@ void MakeFp32Copy(Ctx& ctx, DCsr& m) {
     ToFloat<<<(unsigned)((m.nnz + 255) / 256), 256, 0, ctx.stream>>>(m.nnz, m.val.get(), m.val32.get());
 }
 
+void CopyValues(Ctx& ctx, DCsr& dst, const DCsr& src) {
+  if (dst.n != src.n || dst.nnz != src.nnz) throw std::invalid_argument("CopyValues: patterns differ");
+  if (dst.nnz == 0) return;
+  SPAI_HIP_CHECK(hipMemcpyAsync(dst.val.get(), src.val.get(), sizeof(double) * dst.nnz, hipMemcpyDeviceToDevice,
+                                ctx.stream));
+  if (dst.val32.get())
+    ToFloat<<<(unsigned)((dst.nnz + 255) / 256), 256, 0, ctx.stream>>>(dst.nnz, dst.val.get(), dst.val32.get());
+}
+
 PcgSolver::PcgSolver(Ctx& ctx, const DCsr& A, const DCsr& M, bool m_fp32)
     : ctx_(ctx), A_(A), M_(M), m_fp32_(m_fp32), n_(A.n) {
   if (M.n != A.n) throw std::invalid_argument("A and M must have the same shape");

@@ -21,6 +21,9 @@ DCsr CsrFromHost(Ctx& ctx, int64_t n, int64_t nnz, const int64_t* ptr, const int
 // CSR of the matrix stored as CSC in m (i.e. the transpose of its CSC arrays).
 DCsr CsrFromCsc(Ctx& ctx, const DMat& m);
 void MakeFp32Copy(Ctx& ctx, DCsr& m);
+// dst's values (and FP32 copy, if it has one) from src of the same pattern;
+// dst's buffers stay where they are, so graphs recorded on them stay valid.
+void CopyValues(Ctx& ctx, DCsr& dst, const DCsr& src);
 // y = A x (methods_cython/backward_error.pyx:27-34 _multiply).
 void SpMV(Ctx& ctx, const DCsr& A, const double* x, double* y);
 

@@ -176,3 +176,33 @@ def test_session_kernel_timing(gpu_lib):
     m.build()
     assert m.build_kernel_us() == (-1.0, 0)
     m.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_session_build_graph_replays(gpu_lib, oracle_lib, precision):
+    """Untimed builds replay one recorded graph: every replay, with eager
+    (timed) builds and solves in between, gives the same M and history as the
+    eager build, and the first matches the oracle."""
+    A = problems.diffusion3d(10, seed=2)
+    Pr = problems.diag_inverse(A)
+    b = A @ np.random.default_rng(3).standard_normal(A.shape[0])
+    s = gpu_lib.Session("lomr", A, M0=Pr, max_iterations=9, precision=precision)
+    s.set_rhs(b)
+    s.set_kernel_timing(True)
+    assert s.build() == 9
+    eager = s.result()
+    s.set_kernel_timing(False)
+    o = oracle_lib.run("lomr", A, Pr, None, 9, 0.0, 1.0, False, 1, fixed_pattern=True)
+    assert rel_fro(eager["M"], o["M"]) < (1e-12 if precision == "fp64" else 1e-4)
+    for k in range(4):
+        assert s.build() == 9
+        r = s.result()
+        np.testing.assert_array_equal(r["M"].data, eager["M"].data)
+        np.testing.assert_array_equal(r["history"]["residual_norm"], eager["history"]["residual_norm"])
+        st, cnt = s.pcg(2000, 1e-8)
+        assert st == 1
+        if k == 1:
+            s.set_kernel_timing(True)
+            s.build()
+            s.set_kernel_timing(False)
+    s.close()
