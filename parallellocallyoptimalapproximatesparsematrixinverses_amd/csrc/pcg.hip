@@ -741,9 +741,15 @@ int PcgSolver::Solve(const double* b_dev, int64_t max_iterations, double toleran
   ctx_.ReadBack(hst, ist_.get(), sizeof(hst));
   int64_t chunks = 0;
   const int64_t max_chunks = max_iterations / kChunk + 4;
+  // launch as many chunks as the previous solve needed before looking at the
+  // state (chunks after the stop return at once); then one at a time
+  int64_t ahead = std::max<int64_t>(1, (last_iterations_ + kChunk - 1) / kChunk);
   while (!hst[kDone] && chunks < max_chunks) {
-    SPAI_HIP_CHECK(hipGraphLaunch(exec_[hst[kIter] & 1], st));
-    ++chunks;
+    const int64_t k = std::min<int64_t>(ahead, max_chunks - chunks);
+    // chunks hold an even number of iterations: each starts at the same parity
+    for (int64_t c = 0; c < k; ++c) SPAI_HIP_CHECK(hipGraphLaunch(exec_[hst[kIter] & 1], st));
+    chunks += k;
+    ahead = 1;
     ctx_.ReadBack(hst, ist_.get(), sizeof(hst));
     if (hst[kNeedRefresh]) {
       // the last K2 had parity par and left r' in buffer par ^ 1
@@ -770,6 +776,7 @@ int PcgSolver::Solve(const double* b_dev, int64_t max_iterations, double toleran
   SPAI_HIP_CHECK(hipGetLastError());
   ctx_.ReadBack(hst, ist_.get(), sizeof(hst));
   *count = hst[kCountLatest];
+  last_iterations_ = hst[kIter];
   return hst[kStatus];
 }
 

@@ -55,6 +55,7 @@ class PcgSolver {
   hipGraph_t graph_[2] = {nullptr, nullptr};
   hipGraphExec_t exec_[2] = {nullptr, nullptr};
   const double* graph_key_b_ = nullptr;
+  int64_t last_iterations_ = 0;  // iterations of the previous solve, for the launch-ahead
 };
 
 }  // namespace spai
