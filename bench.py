@@ -128,6 +128,134 @@ def calibrate_copy(mib):
                       "launches": 5}))
 
 
+def _median_time(fn, reps):
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts))
+
+
+def run_configs(with_cpu, reps=5):
+    """One JSON line per configuration besides the headline one: device times
+    of the build and the solves (operands resident; medians of `reps` runs
+    after one warm-up) and, with with_cpu, the oracle on a bounded sample."""
+    from scipy import sparse
+
+    from parallellocallyoptimalapproximatesparsematrixinverses_amd import _lib, problems
+    from parallellocallyoptimalapproximatesparsematrixinverses_amd.autograd import ResidualLoss
+
+    threads = min(16, os.cpu_count() or 1)
+    out = []
+
+    # config 1: 2D Poisson 32^2, MR on pattern(A), 20 iterations, PCG to 1e-8
+    A = problems.poisson2d(32)
+    M0 = problems.diag_inverse(A)
+    b = A @ np.random.default_rng(0).uniform(-1, 1, A.shape[0])
+    s = _lib.Session("mr", A, M0=M0, max_iterations=20)
+    s.set_rhs(b)
+    s.build()
+    s.pcg(1000, 1e-8)
+    tb = _median_time(s.build, reps)
+    tp = _median_time(lambda: s.pcg(1000, 1e-8), reps)
+    st, cnt = s.pcg(1000, 1e-8)
+    s.close()
+    line = {"config": "poisson2d 32^2, MR fixed pattern 20 its + PCG 1e-8", "build_s": tb, "pcg_s": tp,
+            "pcg_iterations": cnt - 1, "status": st}
+    if with_cpu:
+        from oracle import oracle
+        t = time.perf_counter()
+        r = oracle.run("mr", A, M0, None, 20, 0.0, 1.0, False, threads, fixed_pattern=True, omp_threads=threads)
+        oracle.pcg(A.tocsr(), r["M"].tocsr(), b, 1000, 1e-8, threads=threads)
+        line["cpu_baseline"] = {"value": time.perf_counter() - t, "unit": "s", "cores": threads, "kind": "port",
+                                "sample": "full workload"}
+    out.append(line)
+
+    # config 3: A = B^T B + sigma I, n = 100,000, CG with per-column top-16 dropping, 25 iterations
+    A, _, _ = problems.random_gram(100_000, seed=3, power_iterations=50)
+    Pr = problems.diag_inverse(A)
+    r = _lib.run("cg", A, None, Pr, 25, 0.0, 0.5, True, 256, column_top_k=16)
+    dev = []
+    for _ in range(3):
+        r = _lib.run("cg", A, None, Pr, 25, 0.0, 0.5, True, 256, column_top_k=16)
+        dev.append(r["device_seconds"])
+    line = {"config": "random gram n=100000 (nnz %d), CG top-16 dropping 25 its" % A.nnz,
+            "build_s_including_transfers": float(np.median(dev)), "nnz_M": int(r["M"].nnz)}
+    if with_cpu:
+        from oracle import oracle
+        t = time.perf_counter()
+        oracle.run("cg", A, None, Pr, 2, 0.0, 0.5, True, 256, column_top_k=16, omp_threads=threads)
+        t2 = time.perf_counter() - t
+        line["cpu_baseline"] = {"value": t2 / 2 * 25, "unit": "s", "cores": threads, "kind": "port",
+                                "sample": "2 of 25 iterations (%.2f s), scaled to 25" % t2}
+    out.append(line)
+
+    # config 4: 1024^2 anisotropic, inner-outer LOMR 5 x 20 FP32 bulk, 16 right-hand sides
+    A = problems.anisotropic2d(1024)
+    M0 = problems.diag_inverse(A)
+    s = _lib.Session("io_lomr", A, M0=M0, max_iterations=100, precision="fp32", inner_iterations=5)
+    s.build()
+    tb = _median_time(s.build, 3)
+    counts = []
+    t = time.perf_counter()
+    for seed in range(16):
+        bb = A @ np.random.default_rng(seed).standard_normal(A.shape[0])
+        s.set_rhs(bb)
+        st, cnt = s.pcg(50000, 1e-8)
+        counts.append(cnt - 1)
+    tp = time.perf_counter() - t
+    s.close()
+    line = {"config": "anisotropic2d 1024^2, inner-outer LOMR 5x20 FP32 bulk + 16 RHS PCG 1e-8", "build_s": tb,
+            "pcg_16_rhs_s": tp, "pcg_iterations": counts}
+    if with_cpu:
+        from oracle import oracle
+        t = time.perf_counter()
+        oracle.run("io_lomr", A, M0, None, 5, 0.0, 1.0, False, threads, inner_iterations=5, fixed_pattern=True,
+                   omp_threads=threads)
+        t5 = time.perf_counter() - t
+        line["cpu_baseline"] = {"value": t5 * 20, "unit": "s", "cores": threads, "kind": "port",
+                                "sample": "one outer iteration (5 inner, FP64, %.2f s) scaled to 20; solves not "
+                                          "included" % t5}
+    out.append(line)
+
+    # config 5: 27-point 128^3, MR FP32 bulk 10 iterations; loss and gradients in FP32 per 65,536 columns
+    import torch
+
+    A = problems.stencil27(128, seed=4)
+    M0 = problems.diag_inverse(A)
+    s = _lib.Session("mr", A, M0=M0, max_iterations=10, precision="fp32")
+    s.build()
+    tb = _median_time(s.build, 3)
+    M = s.result()["M"].tocsc()
+    s.close()
+    M.sort_indices()
+    plan = ResidualLoss(A, M, torch.float32, block_cols=65536)
+    m = torch.tensor(M.data, dtype=torch.float32, device="cuda")
+    a = torch.tensor(A.data, dtype=torch.float32, device="cuda")
+    plan.forward_value(m, a)
+    tf = _median_time(lambda: plan.forward_value(m, a), 3)
+    tg = _median_time(lambda: plan.gradients(m, a), 3)
+    info = plan.info()
+    plan.close()
+    line = {"config": "stencil27 128^3 (n=2097152, nnz %d), MR FP32 bulk 10 its; loss fwd/bwd FP32, 65536-column "
+                      "blocks" % A.nnz, "build_s": tb, "loss_forward_s": tf, "loss_backward_s": tg,
+            "nnz_R": info["nnz_r"], "blocks": info["blocks"]}
+    if with_cpu:
+        from oracle import oracle
+        A64 = problems.stencil27(64, seed=4)
+        M064 = problems.diag_inverse(A64)
+        t = time.perf_counter()
+        oracle.run("mr", A64, M064, None, 1, 0.0, 1.0, False, threads, fixed_pattern=True, omp_threads=threads)
+        t1 = time.perf_counter() - t
+        line["cpu_baseline"] = {"value": t1 * 8 * 10, "unit": "s", "cores": threads, "kind": "port",
+                                "sample": "FP64 MR, 1 iteration on 64^3 (%.2f s), scaled by 8 (size) and 10 "
+                                          "(iterations)" % t1}
+    out.append(line)
+    for line in out:
+        print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,7 +264,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 launches of an FP64 y = 1.0 * x kernel over MIB MiB (counter calibration)")
+    ap.add_argument("--configs", action="store_true",
+                    help="instead of the headline line, one line per other configuration (1, 3, 4, 5)")
     args = ap.parse_args()
+    if args.configs:
+        run_configs(not args.no_cpu_baseline)
+        return
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return

@@ -120,7 +120,7 @@ class State {
     if (!prm_.enable_dropping) return false;
     const int64_t original = X.nnz();
     auto bsq = ctx_.Alloc<double>(nb);
-    BatchSqNorms(ctx_, X, batch_first.get(), (int)nb, bsq.get());
+    BatchSqNorms(ctx_, X, batch_first.get(), (int)nb, bsq.get(), prm_.batch_ordered);
     X = ThresholdBatches(ctx_, X, bsq.get(), batch_of_col.get(), kPruneEps);
     if (prm_.top_k > 0) {
       X = ColumnTopK(ctx_, X, prm_.top_k);

@@ -84,7 +84,9 @@ void ColumnSqNorms(Ctx& ctx, const DMat& X, double* out);
 // the batches in order (methods_common.cpp:148-154).
 double BatchOrderedDot(Ctx& ctx, const DMat& X, const DMat& Y, const int* batch_first, int nbatches, bool same);
 // Per-batch sequential squared norms for the batch column ranges (device out).
-void BatchSqNorms(Ctx& ctx, const DMat& X, const int* batch_first, int nbatches, double* out);
+// ordered: sequential sum per batch in CSC order (the reference's order);
+// otherwise a deterministic block tree per batch.
+void BatchSqNorms(Ctx& ctx, const DMat& X, const int* batch_first, int nbatches, double* out, bool ordered);
 
 // ---------------------------------------------------------------- dropping
 // ApplyDroppingStrategy stage 1: keep v != 0 and (diagonal or |v| >= eps*||col||).

@@ -472,6 +472,19 @@ __global__ void BatchSqNormKernel(int nb, const int* __restrict__ first, const i
   out[b] = s;
 }
 
+// Per-batch ||X_b||^2 with one block per batch: strided per-thread sums,
+// then the block's fixed tree (deterministic, not the sequential order).
+__global__ __launch_bounds__(256) void BatchSqNormTreeKernel(const int* __restrict__ first,
+                                                             const int* __restrict__ ptr,
+                                                             const double* __restrict__ v, double* __restrict__ out) {
+  __shared__ double red[16];
+  const int b = blockIdx.x;
+  double s = 0.0;
+  for (int p = ptr[first[b]] + threadIdx.x; p < ptr[first[b + 1]]; p += blockDim.x) s += v[p] * v[p];
+  s = BlockSum<double>(s, red);
+  if (threadIdx.x == 0) out[b] = s;
+}
+
 // Per-batch sequential intersection dot over the batch's columns (CSC order).
 __global__ void BatchDotKernel(int nb, const int* __restrict__ first, const int* __restrict__ xp,
                                const int* __restrict__ xi, const double* __restrict__ xv, const int* __restrict__ yp,
@@ -540,8 +553,36 @@ __global__ void ThresholdBatchesKernel(int64_t nnz_cols, int ncols, const int* _
 // each off-diagonal entry among the column's off-diagonal entries.
 __global__ void TopKKernel(int ncols, const int* __restrict__ ptr, const int* __restrict__ idx,
                            const double* __restrict__ v, int64_t k, unsigned char* __restrict__ keep) {
+  constexpr int kStage = 1024;
+  __shared__ double sa[kStage];
+  __shared__ int si[kStage];
   const int j = blockIdx.x;
   const int b = ptr[j], e = ptr[j + 1];
+  if (e - b <= kStage) {
+    // column staged in LDS: |v| and row (the diagonal marked -1, never counted)
+    for (int p = b + threadIdx.x; p < e; p += blockDim.x) {
+      sa[p - b] = fabs(v[p]);
+      si[p - b] = idx[p] == j ? -1 : idx[p];
+    }
+    __syncthreads();
+    const int L = e - b;
+    for (int t = threadIdx.x; t < L; t += blockDim.x) {
+      const int row = si[t];
+      if (row < 0) {
+        keep[b + t] = 1;
+        continue;
+      }
+      const double a = sa[t];
+      int64_t rank = 0;
+      for (int u = 0; u < L; ++u) {
+        const int r2 = si[u];
+        const double c = sa[u];
+        rank += (r2 >= 0 && u != t && (c > a || (c == a && r2 < row))) ? 1 : 0;
+      }
+      keep[b + t] = rank < k ? 1 : 0;
+    }
+    return;
+  }
   for (int p = b + threadIdx.x; p < e; p += blockDim.x) {
     if (idx[p] == j) {
       keep[p] = 1;
@@ -1239,8 +1280,11 @@ double BatchOrderedDot(Ctx& ctx, const DMat& X, const DMat& Y, const int* batch_
   return s;
 }
 
-void BatchSqNorms(Ctx& ctx, const DMat& X, const int* batch_first, int nbatches, double* out) {
-  BatchSqNormKernel<<<Blocks(nbatches), kBlock, 0, ctx.stream>>>(nbatches, batch_first, X.ptr(), X.v(), out);
+void BatchSqNorms(Ctx& ctx, const DMat& X, const int* batch_first, int nbatches, double* out, bool ordered) {
+  if (ordered)
+    BatchSqNormKernel<<<Blocks(nbatches), kBlock, 0, ctx.stream>>>(nbatches, batch_first, X.ptr(), X.v(), out);
+  else if (nbatches > 0)
+    BatchSqNormTreeKernel<<<(unsigned)nbatches, 256, 0, ctx.stream>>>(batch_first, X.ptr(), X.v(), out);
 }
 
 DMat ThresholdColumns(Ctx& ctx, const DMat& M, double eps) {

@@ -206,3 +206,23 @@ def test_session_build_graph_replays(gpu_lib, oracle_lib, precision):
             s.build()
             s.set_kernel_timing(False)
     s.close()
+
+
+@pytest.mark.parametrize("n,per_row", [(3000, 3), (800, 10), (1500, 6)])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fixed_lomr_wide_patterns(gpu_lib, oracle_lib, n, per_row, precision):
+    """Fixed-pattern LOMR on B^T B + sigma I patterns: 32 lanes per column
+    (3000/3), columns of A longer than 64 with tiles beyond the register-held
+    entries (800/10), and S2 columns over 1024 entries, which take the
+    unfused passes (1500/6)."""
+    A, _, _ = problems.random_gram(n, per_row=per_row, seed=3)
+    Pr = problems.diag_inverse(A)
+    print(f"case: lomr wide n={n} per_row={per_row} {precision}", flush=True)
+    g = gpu_lib.run("lomr", A, Pr, None, 8, 0.0, 1.0, False, 1, fixed_pattern=True, precision=precision)
+    o = oracle_lib.run("lomr", A, Pr, None, 8, 0.0, 1.0, False, 1, fixed_pattern=True)
+    assert g["iterations"] == o["iterations"] == 8
+    if precision == "fp64":
+        assert_same_result(g, o, hist_rtol=1e-10, m_rtol=1e-12)
+    else:
+        assert rel_fro(g["M"], o["M"]) < 1e-4
+        np.testing.assert_allclose(g["history"]["residual_norm"], o["history"]["residual_norm"], rtol=1e-4)

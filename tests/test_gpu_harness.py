@@ -66,3 +66,22 @@ def test_calculate_backward_error_many_rhs(oracle_lib):
     solver.close()
     one = harness.calculate_backward_error(A, M, 200, b=b)
     assert one.converged
+
+
+def test_torch_after_library_use(gpu_lib):
+    """The package loads torch before libspai_hip.so, so torch still sees the
+    device after the library has run (one HIP runtime in the process)."""
+    import subprocess
+    import sys
+    code = (
+        "import numpy as np\n"
+        "from parallellocallyoptimalapproximatesparsematrixinverses_amd import _lib, problems\n"
+        "A = problems.poisson2d(6).tocsr()\n"
+        "assert np.allclose(_lib.matvec(A, np.ones(A.shape[0])), A @ np.ones(A.shape[0]))\n"
+        "import torch\n"
+        "assert torch.cuda.is_available()\n"
+        "print('ok')\n")
+    from pathlib import Path
+    root = Path(__file__).resolve().parent.parent
+    out = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=240)
+    assert out.returncode == 0 and out.stdout.strip().endswith("ok"), out.stderr[-2000:]
