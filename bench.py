@@ -333,7 +333,7 @@ def main():
         "speedup_fp32_vs_fp64": round(fp64["seconds"] / fp32["seconds"], 4),
         "extra_pcg_iterations_fp32": fp32["pcg_iterations"] - fp64["pcg_iterations"],
         "rel_fro_M32_vs_M64": dM,
-        "roofline": {"kernel": "LomrFused<float> (one fused LOMR iteration)", "bound": "hbm",
+        "roofline": {"kernel": "FusedPass<float, LOMR> (one fused LOMR iteration)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": pass_bytes, "avg_launch_us": round(us, 3),

@@ -135,7 +135,7 @@ struct Result spai_session_result(SpaiSession* session);
  * one SpMV with A (HIP events around a replay of the iteration graph). */
 double spai_session_pcg_iteration_us(SpaiSession* session);
 /* on != 0: later builds record HIP events, on the session's stream, around
- * each per-iteration pass of the fused LOMR build. */
+ * each per-iteration pass of the fused LOMR or MR build. */
 int64_t spai_session_set_kernel_timing(SpaiSession* session, int64_t on);
 /* Average microseconds of that pass in the last build and (launches) how many
  * passes were averaged; negative when the last build was not timed. */

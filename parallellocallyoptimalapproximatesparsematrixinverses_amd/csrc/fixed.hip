@@ -286,9 +286,10 @@ __global__ void ExclusiveScan64Serial(int n, const int64_t* __restrict__ in, int
   if (threadIdx.x == 0) out[n] = carry;
 }
 
-// ------------------------------------------------------- fused LOMR pass
+// ------------------------------------------------------- fused LOMR / MR pass
 // One kernel per LOMR iteration (locally_minimal_residual.cpp:57-114 on the
-// fixed pattern).  What an iteration does to column j -- A [Z P](:, j), the
+// fixed pattern); MR (minimal_residual.cpp:56-92) takes the same pass with
+// M' = M + alpha Z, R' = I - A M' recomputed and one direction.  What an iteration does to column j -- A [Z P](:, j), the
 // updates of R(:, j), M(:, j), P(:, j), Z(:, j), and the products
 // A [Z' P'](:, j) whose traces give the next step sizes -- reads column j
 // alone, so a wavefront takes W = 64 / GL consecutive columns (GL lanes each),
@@ -407,8 +408,8 @@ constexpr int kQInfoMask = (1 << kQInfoShift) - 1;
 // kInit: M = M0 already on S; R = I - A M, Z = Pr R on S, P = 0, and the
 // traces of A Z (the pass before iteration 1).  Otherwise one iteration with
 // the step sizes in sc: update, then the traces of the next iteration.
-template <class V, class SlotT, int GL, bool kInit>
-__global__ __launch_bounds__(kBlk) void LomrFused(
+template <class V, class SlotT, int GL, bool kInit, bool kMr>
+__global__ __launch_bounds__(kBlk) void FusedPass(
     int n, const int* __restrict__ sp, const int2* __restrict__ a0c, const uint16_t* __restrict__ qinfo,
     const uint32_t* __restrict__ sinfo, const int* __restrict__ s2p, const int* __restrict__ s2i,
     const V* __restrict__ av,
@@ -420,14 +421,18 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
   constexpr int RQ = 4;  // S2 entries of a tile held in registers per lane
   constexpr int RS = 2;  // S entries of a tile held in registers per lane
   if (flags->stop) return;
-  V d = V(0), gm = V(0), be = V(0), nd = V(0), ng = V(0);
-  if constexpr (!kInit) {
+  // LOMR: R -= delta AZ + gamma AP (kNeedR: R read); MR: R = I - A M recomputed
+  constexpr bool kLomrIter = !kInit && !kMr;
+  constexpr int NY = kLomrIter ? 2 : 1;
+  V d = V(0), gm = V(0), be = V(0), nd = V(0), ng = V(0), alpha = V(0);
+  if constexpr (kLomrIter) {
     d = V(sc[kDelta]);
     gm = V(sc[kGamma]);
     be = V(sc[kBeta]);
     nd = V(-sc[kDelta]);
     ng = V(-sc[kGamma]);
   }
+  if constexpr (kMr && !kInit) alpha = V(sc[kAlpha]);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = LaneId(), wib = threadIdx.x / kWave;
   const int g = lane / GL, gl = lane % GL;
@@ -455,9 +460,9 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
     for (int r = 0; r < RQ; ++r) {
       const int q = qa + lane + r * kWave;
       const bool in = q < qe;
-      rq[r] = in && !kInit ? R[q] : V(0);
+      rq[r] = in && kLomrIter ? R[q] : V(0);
       pq[r] = in ? prS2[q] : V(0);
-      iq[r] = in && kInit ? s2i[q] : -1;
+      iq[r] = in && !kLomrIter ? s2i[q] : -1;
       nq[r] = in ? int(qinfo[q]) : 0;
     }
     V mp[RS], zp[RS], ppv[RS], ps[RS];
@@ -468,7 +473,7 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
       const bool in = p < pe;
       mp[r] = in && !kInit ? M[p] : V(0);
       zp[r] = in && !kInit ? Z[p] : V(0);
-      ppv[r] = in && !kInit ? P[p] : V(0);
+      ppv[r] = in && kLomrIter ? P[p] : V(0);
       ps[r] = in ? prS[p] : V(0);
       np[r] = in ? sinfo[p] : 0u;
     }
@@ -477,15 +482,18 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
     V* const acc1 = mine + lmax;
     V* const yz = mine + 3 * lmax;
     V* const yp = yz + smax;
-    // product 1: A [Z P](:, j)  (kInit: A M(:, j))
+    // product 1: A [Z P](:, j)  (kInit: A M(:, j); MR: A (M + alpha Z)(:, j))
     for (int t = gl; t < L; t += GL) {
       acc0[t] = V(0);
-      if (!kInit) acc1[t] = V(0);
+      if (NY > 1) acc1[t] = V(0);
     }
     if (valid) {
       if constexpr (kInit) {
         SlotAccumulate<V, SlotT, GL, 1>(j, gl, sp, a0c, av, plan_ptr, plan, acc0, acc1, st_ac, st_y,
                                         [&](int e, V& ya, V& yb) { ya = M[e]; });
+      } else if constexpr (kMr) {
+        SlotAccumulate<V, SlotT, GL, 1>(j, gl, sp, a0c, av, plan_ptr, plan, acc0, acc1, st_ac, st_y,
+                                        [&](int e, V& ya, V& yb) { ya = M[e] + alpha * Z[e]; });
       } else {
         SlotAccumulate<V, SlotT, GL, 2>(j, gl, sp, a0c, av, plan_ptr, plan, acc0, acc1, st_ac, st_y,
                                         [&](int e, V& ya, V& yb) {
@@ -500,7 +508,7 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
       const int gg = info >> kQInfoShift, t = info & kQInfoMask;
       V* const grp = wave_lds + size_t(gg) * words;
       V rv;
-      if constexpr (kInit) {
+      if constexpr (!kLomrIter) {
         rv = -grp[t];
         if (i == j0 + gg) rv += V(1);
       } else {
@@ -519,7 +527,7 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
     }
     for (int r = RQ; qa + r * kWave < qe; ++r) {
       const int q = qa + lane + r * kWave;
-      if (q < qe) update_r(q, kInit ? V(0) : R[q], kInit ? s2i[q] : -1, int(qinfo[q]));
+      if (q < qe) update_r(q, kLomrIter ? R[q] : V(0), kLomrIter ? -1 : s2i[q], int(qinfo[q]));
     }
     __builtin_amdgcn_wave_barrier();
     // M' = M + delta Z + gamma P, P' = beta P + Z, Z' = Pr R' over the S range
@@ -527,20 +535,20 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
       const int gg = int(info >> 26), t = int((info >> 16) & 1023u), pl = int(info & 0xFFFFu);
       V* const grp = wave_lds + size_t(gg) * words;
       const V zn = pri * grp[2 * lmax + t];
-      V pn;
-      if constexpr (kInit) {
-        pn = V(0);
-      } else {
+      V pn = V(0);
+      if constexpr (kLomrIter) {
         V m = mo;
         m = m + d * zo;
         m = m + gm * po;
         M[p] = m;
         pn = po * be + zo;
+      } else if constexpr (kMr && !kInit) {
+        M[p] = mo + alpha * zo;
       }
       Z[p] = zn;
-      P[p] = pn;
+      if constexpr (!kMr) P[p] = pn;
       grp[3 * lmax + pl] = zn;
-      grp[3 * lmax + smax + pl] = pn;
+      if constexpr (!kMr) grp[3 * lmax + smax + pl] = pn;
     };
 #pragma unroll
     for (int r = 0; r < RS; ++r) {
@@ -552,23 +560,26 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
       if (p < pe) {
         if constexpr (kInit)
           update_s(p, V(0), V(0), V(0), prS[p], sinfo[p]);
+        else if constexpr (kMr)
+          update_s(p, M[p], Z[p], V(0), prS[p], sinfo[p]);
         else
           update_s(p, M[p], Z[p], P[p], prS[p], sinfo[p]);
       }
     }
     __builtin_amdgcn_wave_barrier();
-    // product 2: A [Z' P'](:, j) from LDS
+    // product 2: A [Z' P'](:, j) from LDS (MR: A Z'(:, j))
+    constexpr int NY2 = kMr ? 1 : 2;
     for (int t = gl; t < L; t += GL) {
       acc0[t] = V(0);
-      acc1[t] = V(0);
+      if (NY2 > 1) acc1[t] = V(0);
     }
     if (valid) {
       const int sj = sp[j];
-      SlotAccumulate<V, SlotT, GL, 2>(j, gl, sp, a0c, av, plan_ptr, plan, acc0, acc1, st_ac, st_y,
-                                      [&](int e, V& ya, V& yb) {
-                                        ya = yz[e - sj];
-                                        yb = yp[e - sj];
-                                      });
+      SlotAccumulate<V, SlotT, GL, NY2>(j, gl, sp, a0c, av, plan_ptr, plan, acc0, acc1, st_ac, st_y,
+                                        [&](int e, V& ya, V& yb) {
+                                          ya = yz[e - sj];
+                                          if (NY2 > 1) yb = yp[e - sj];
+                                        });
     }
     __builtin_amdgcn_wave_barrier();
     // traces of the next step (LomrProjectEpi order)
@@ -576,12 +587,19 @@ __global__ __launch_bounds__(kBlk) void LomrFused(
       const int gg = info >> kQInfoShift, t = info & kQInfoMask;
       const V* grp = wave_lds + size_t(gg) * words;
       const V rv = grp[2 * lmax + t];
-      const double daz = grp[t], dap = grp[lmax + t], dpr = pri, dprr = double(pri * rv);
-      red[0] += daz * (dpr * daz);
-      red[1] += dap * (dpr * dap);
-      red[2] += daz * (dpr * dap);
-      red[3] += dprr * daz;
-      red[4] += dprr * dap;
+      if constexpr (kMr) {
+        // MrProjectEpi order: <Pr R, Pr A Z>, ||Pr A Z||^2
+        const double tmp = double(pri * grp[t]);
+        red[0] += double(pri * rv) * tmp;
+        red[1] += tmp * tmp;
+      } else {
+        const double daz = grp[t], dap = grp[lmax + t], dpr = pri, dprr = double(pri * rv);
+        red[0] += daz * (dpr * daz);
+        red[1] += dap * (dpr * dap);
+        red[2] += daz * (dpr * dap);
+        red[3] += dprr * daz;
+        red[4] += dprr * dap;
+      }
     };
 #pragma unroll
     for (int r = 0; r < RQ; ++r) {
@@ -636,6 +654,35 @@ __global__ __launch_bounds__(1024) void LomrFusedStep(const double* __restrict__
   sc[kDelta] = delta;
   sc[kGamma] = gamma;
   sc[kBeta] = isfinite(ratio) ? ratio : 0.0;
+}
+
+// MR between passes it - 1 and it: history row it - 1, then the step
+// alpha = <Pr R, Pr A Z> / ||Pr A Z||^2 (minimal_residual.cpp:58-65).
+__global__ __launch_bounds__(1024) void MrFusedStep(const double* __restrict__ part, int G, int it, double tol,
+                                                    double* __restrict__ hist, double* __restrict__ sc,
+                                                    Flags* __restrict__ flags) {
+  if (flags->stop) return;
+  double s[6];
+  SumPartialsBlockFused<6>(part, G, s);
+  if (threadIdx.x != 0) return;
+  const double rn = sqrt(s[5]);
+  hist[it - 1] = rn;
+  flags->iterations = it - 1;
+  if (rn < tol) {
+    flags->stop = 2;
+    return;
+  }
+  const double num = s[0], den = s[1];
+  if (!(isfinite(den) && fabs(den) > 0.0)) {
+    flags->stop = 1;
+    return;
+  }
+  const double alpha = num / den;
+  if (!isfinite(alpha)) {
+    flags->stop = 1;
+    return;
+  }
+  sc[kAlpha] = alpha;
 }
 
 // Clears the stop flags (a kernel, so that recorded graphs hold no memset).
@@ -1446,7 +1493,7 @@ class FixedImpl final : public FixedProblem {
     ConvertValues<V><<<Blocks1(at_pat_->nnz), kBlk, 0, c.stream>>>(at_pat_->nnz, at_d_.get(), atv_.get());
     prv_ = c.Alloc<V>(n_);
     ConvertValues<V><<<Blocks1(n_), kBlk, 0, c.stream>>>(n_, pr_d_.get(), prv_.get());
-    if (fused_ok_ && method_ == SPAI_GLOBAL_LOMR) {
+    if (fused_ok_ && (method_ == SPAI_GLOBAL_LOMR || method_ == SPAI_GLOBAL_MR)) {
       a0c_ = c.Alloc<int2>(nnz_s_);
       SEntryA<<<Blocks1(nnz_s_), kBlk, 0, c.stream>>>(nnz_s_, s_pat_->idx.get(), A_.ptr(), a0c_.get());
       qinfo_ = c.Alloc<uint16_t>(nnz_s2_);
@@ -1573,10 +1620,11 @@ class FixedImpl final : public FixedProblem {
     return timed_launches_ > 0 ? timed_us_ : -1.0;
   }
 
-  // Global LOMR with one fused pass per iteration (LomrFused).
-  int64_t BuildLomrFused() {
+  // Global LOMR or MR with one fused pass per iteration (FusedPass).
+  int64_t BuildFused() {
     Ctx& c = ctx_;
     hipStream_t st = c.stream;
+    const bool mr = method_ == SPAI_GLOBAL_MR;
     timed_launches_ = 0;
     if (timing_ && (int64_t)events_.size() < 2 * prm_.max_iterations) {
       for (hipEvent_t e : events_) (void)hipEventDestroy(e);
@@ -1596,20 +1644,28 @@ class FixedImpl final : public FixedProblem {
     double* part = partf_.get();
     const double tol = prm_.tolerance;
     for (int64_t it = 0; it <= prm_.max_iterations; ++it) {
-      if (it > 0)
+      if (it > 0 && mr)
+        MrFusedStep<<<1, 1024, 0, st>>>(part, (int)gridf_, (int)it, tol, hist_.get(), sc_.get(), flags_.get());
+      else if (it > 0)
         LomrFusedStep<<<1, 1024, 0, st>>>(part, (int)gridf_, (int)it, tol, hist_.get(), sc_.get(), flags_.get());
-#define SPAI_FUSED_LAUNCH(SlotT, plan, GLV)                                                                       \
-  do {                                                                                                            \
-    if (it == 0)                                                                                                  \
-      LomrFused<V, SlotT, GLV, true><<<gridf_, kBlk, smemf_, st>>>(                                               \
-          (int)n_, s_pat_->ptr.get(), a0c_.get(), qinfo_.get(), sinfo_.get(), s2_pat_->ptr.get(),              \
-          s2_pat_->idx.get(), av_.get(), plan_ptr_.get(), plan, lmax_, smax_, prS_.get(), prS2_.get(), R_.get(), \
-          M_.get(), Z_.get(), P_.get(), sc_.get(), part, flags_.get());                                           \
-    else                                                                                                          \
-      LomrFused<V, SlotT, GLV, false><<<gridf_, kBlk, smemf_, st>>>(                                              \
-          (int)n_, s_pat_->ptr.get(), a0c_.get(), qinfo_.get(), sinfo_.get(), s2_pat_->ptr.get(),              \
-          s2_pat_->idx.get(), av_.get(), plan_ptr_.get(), plan, lmax_, smax_, prS_.get(), prS2_.get(), R_.get(), \
-          M_.get(), Z_.get(), P_.get(), sc_.get(), part, flags_.get());                                           \
+#define SPAI_FUSED_ONE(SlotT, plan, GLV, INIT, MR)                                                             \
+  FusedPass<V, SlotT, GLV, INIT, MR><<<gridf_, kBlk, smemf_, st>>>(                                              \
+      (int)n_, s_pat_->ptr.get(), a0c_.get(), qinfo_.get(), sinfo_.get(), s2_pat_->ptr.get(), s2_pat_->idx.get(), \
+      av_.get(), plan_ptr_.get(), plan, lmax_, smax_, prS_.get(), prS2_.get(), R_.get(), M_.get(), Z_.get(),     \
+      P_.get(), sc_.get(), part, flags_.get())
+#define SPAI_FUSED_LAUNCH(SlotT, plan, GLV)                     \
+  do {                                                          \
+    if (mr) {                                                   \
+      if (it == 0)                                              \
+        SPAI_FUSED_ONE(SlotT, plan, GLV, true, true);           \
+      else                                                      \
+        SPAI_FUSED_ONE(SlotT, plan, GLV, false, true);          \
+    } else {                                                    \
+      if (it == 0)                                              \
+        SPAI_FUSED_ONE(SlotT, plan, GLV, true, false);          \
+      else                                                      \
+        SPAI_FUSED_ONE(SlotT, plan, GLV, false, false);         \
+    }                                                           \
   } while (0)
 #define SPAI_FUSED_GL(SlotT, plan)                          \
   switch (gl_) {                                            \
@@ -1628,6 +1684,7 @@ class FixedImpl final : public FixedProblem {
       if (timed) SPAI_HIP_CHECK(hipEventRecord(events_[2 * (it - 1) + 1], st));
 #undef SPAI_FUSED_GL
 #undef SPAI_FUSED_LAUNCH
+#undef SPAI_FUSED_ONE
     }
     LomrFusedFinish<<<1, kBlk, 0, st>>>(part, (int)gridf_, (int)prm_.max_iterations, tol, hist_.get(),
                                         flags_.get());
@@ -1658,7 +1715,7 @@ class FixedImpl final : public FixedProblem {
   int64_t Build() override {
     timed_launches_ = 0;
     if (method_ >= SPAI_INNER_OUTER_MR) return BuildInnerOuter();
-    if (method_ == SPAI_GLOBAL_LOMR && use_plan_ && fused_ok_) return BuildLomrFused();
+    if ((method_ == SPAI_GLOBAL_LOMR || method_ == SPAI_GLOBAL_MR) && use_plan_ && fused_ok_) return BuildFused();
     Ctx& c = ctx_;
     hipStream_t st = c.stream;
     // M = M0 on S, P = 0

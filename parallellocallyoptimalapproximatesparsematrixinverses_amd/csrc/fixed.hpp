@@ -26,7 +26,7 @@ class FixedProblem {
   virtual const DMat& MatrixA() const = 0;
   virtual Ctx& context() = 0;
   // HIP-event timing, on the build's stream, of the per-iteration pass of the
-  // fused LOMR build (off by default).
+  // fused LOMR or MR build (off by default).
   virtual void SetKernelTiming(bool on) = 0;
   // Average microseconds of that pass over the iterations the last build ran
   // (launches receives their number); -1 when the last build did not time it.

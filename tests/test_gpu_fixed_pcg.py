@@ -156,8 +156,8 @@ def test_session_build_and_solve(gpu_lib, oracle_lib):
 
 
 def test_session_kernel_timing(gpu_lib):
-    """HIP-event timing of the fused LOMR pass: one average per build over the
-    iterations that ran; builds of other methods report none."""
+    """HIP-event timing of the fused LOMR/MR pass: one average per build over
+    the iterations that ran; CG builds report none."""
     A = problems.diffusion3d(12, seed=1)
     Pr = problems.diag_inverse(A)
     s = gpu_lib.Session("lomr", A, M0=Pr, max_iterations=7)
@@ -171,10 +171,15 @@ def test_session_kernel_timing(gpu_lib):
     s.build()
     assert s.build_kernel_us()[1] == 0
     s.close()
-    m = gpu_lib.Session("mr", A, M0=Pr, max_iterations=3)
+    m = gpu_lib.Session("cg", A, M0=Pr, max_iterations=3)
     m.set_kernel_timing(True)
     m.build()
     assert m.build_kernel_us() == (-1.0, 0)
+    m.close()
+    m = gpu_lib.Session("mr", A, M0=Pr, max_iterations=3)
+    m.set_kernel_timing(True)
+    assert m.build() == 3
+    assert m.build_kernel_us()[1] == 3
     m.close()
 
 
