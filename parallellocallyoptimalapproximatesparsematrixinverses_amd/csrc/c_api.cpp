@@ -344,6 +344,7 @@ int64_t spai_session_build(SpaiSession* s) {
       s->pcg = std::make_unique<spai::PcgSolver>(c, *s->a_csr, *s->m_csr, s->fp32);
     } else {
       spai::CopyValues(c, *s->m_csr, fresh);
+      s->pcg->RefreshPreconditioner();
       c.Sync();
     }
     return it;

@@ -215,28 +215,71 @@ __global__ __launch_bounds__(kBlk) void PcgInitFinish(const double* __restrict__
   is[kDone] = (1 < is[kMax] && 1.0 > ds[kTol]) ? 0 : 1;
 }
 
-// Operands of a thread's first row (one thread per row): loaded before the
-// kernel's prologue reduction so that their latency overlaps it.  x0/x1 are
-// the two gathered vectors at the row's first 8 columns.
-template <class MV>
-struct RowPrefetch {
-  int b, e;
-  MV v[8];
-  double x0[8], x1[8];
-  // row < 0: nothing to load (empty row)
-  __device__ __forceinline__ void Load(const int* __restrict__ ptr, const int* __restrict__ col,
-                                       const MV* __restrict__ val, const double* __restrict__ y0,
-                                       const double* __restrict__ y1, int row) {
-    b = row >= 0 ? ptr[row] : 0;
-    e = row >= 0 ? ptr[row + 1] : 0;
+// Entries of one row, as seen by one thread: entry u sits at base + u * stride
+// for u < count; a column index < 0 marks padding.
+struct RowView {
+  int base, stride, count;
+};
+
+// CSR rows.
+struct CsrRows {
+  const int* __restrict__ ptr;
+  __device__ __forceinline__ RowView Row(int r) const { return RowView{ptr[r], 1, ptr[r + 1] - ptr[r]}; }
+};
+
+// Sliced ELLPACK rows (DSell): 64-row slices, column-major inside a slice.
+struct SellRows {
+  const int* __restrict__ off;
+  __device__ __forceinline__ RowView Row(int r) const {
+    const int s = r >> 6, b = off[s];
+    return RowView{b + (r & 63), 64, (off[s + 1] - b) >> 6};
+  }
+};
+
+// sum_u val[k_u] * f(col[k_u]) over a row's entries in order, 8 loads in
+// flight at a time; u < 8 come from `first` when given (prefetched).
+template <class MV, class F>
+__device__ __forceinline__ double RowSum(const RowView rv, const int* __restrict__ col, const MV* __restrict__ val,
+                                         int u_begin, F f) {
+  double s = 0.0;
+  for (int u0 = u_begin; u0 < rv.count; u0 += 8) {
+    double t[8];
+    bool in[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const int k = b + u;
-      const bool in = k < e;
-      const int c = in ? col[k] : 0;
-      v[u] = in ? val[k] : MV(0);
-      x0[u] = in ? y0[c] : 0.0;
-      x1[u] = in ? y1[c] : 0.0;
+      const int k = rv.base + (u0 + u) * rv.stride;
+      const int c = u0 + u < rv.count ? col[k] : -1;
+      in[u] = c >= 0;
+      t[u] = in[u] ? double(val[k]) * f(c) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (in[u]) s += t[u];
+  }
+  return s;
+}
+
+// Operands of a thread's first row (one thread per row): the row's first 8
+// values and the two gathered vectors at their columns, loaded before the
+// kernel's prologue reduction so that their latency overlaps it.
+template <class MV>
+struct RowPrefetch {
+  RowView rv;
+  MV v[8];
+  double x0[8], x1[8];
+  bool in[8];
+  template <class Rows>
+  __device__ __forceinline__ void Load(const Rows& rows, const int* __restrict__ col, const MV* __restrict__ val,
+                                       const double* __restrict__ y0, const double* __restrict__ y1, int row) {
+    rv = row >= 0 ? rows.Row(row) : RowView{0, 1, 0};
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int k = rv.base + u * rv.stride;
+      const int c = u < rv.count ? col[k] : -1;
+      in[u] = c >= 0;
+      v[u] = in[u] ? val[k] : MV(0);
+      x0[u] = in[u] ? y0[c] : 0.0;
+      x1[u] = in[u] ? y1[c] : 0.0;
     }
   }
 };
@@ -244,13 +287,15 @@ struct RowPrefetch {
 // K1 of iteration with parity par: end of the previous iteration (error from
 // the previous K2's <r', r'>; at the tolerance crossing, pause for the host
 // refresh), then p_out = z + beta p_in and Ap = A p_out; partials of <p_out, Ap>.
-template <int SG>
-__global__ __launch_bounds__(kBlk) void PcgDirSpmvA(int n, int par, const int* __restrict__ ap_,
-                                                    const int* __restrict__ ac, const double* __restrict__ av,
-                                                    const double* __restrict__ z, const double* __restrict__ p_in,
-                                                    double* __restrict__ p_out, double* __restrict__ Ap,
-                                                    double* __restrict__ part, double* __restrict__ ds,
-                                                    int* __restrict__ is, double* __restrict__ hist) {
+// SG == 1: one thread per row through `rows` (CSR or SELL); SG > 1: SG lanes
+// per CSR row (rows.ptr).
+template <int SG, class Rows>
+__global__ __launch_bounds__(kBlk) void PcgDirSpmvA(int n, int par, Rows rows, const int* __restrict__ ac,
+                                                    const double* __restrict__ av, const double* __restrict__ z,
+                                                    const double* __restrict__ p_in, double* __restrict__ p_out,
+                                                    double* __restrict__ Ap, double* __restrict__ part,
+                                                    double* __restrict__ ds, int* __restrict__ is,
+                                                    double* __restrict__ hist) {
   if (is[kDone]) return;
   const int G = gridDim.x;
   const int prev = par ^ 1;
@@ -260,7 +305,7 @@ __global__ __launch_bounds__(kBlk) void PcgDirSpmvA(int n, int par, const int* _
   double z0 = 0.0, p0 = 0.0;
   if constexpr (SG == 1) {
     const bool have = row0 < n;
-    pf.Load(ap_, ac, av, z, p_in, have ? row0 : -1);
+    pf.Load(rows, ac, av, z, p_in, have ? row0 : -1);
     z0 = have ? z[row0] : 0.0;
     p0 = have ? p_in[row0] : 0.0;
   }
@@ -284,33 +329,43 @@ __global__ __launch_bounds__(kBlk) void PcgDirSpmvA(int n, int par, const int* _
   double acc = 0.0;
   for (int row = row0; row < n; row += G * groups) {
     double s, pr;
-    if (SG == 1 && row == row0) {
-      // first 8 entries from the prefetched operands, the rest as RowDotDir
-      double t[8];
+    if constexpr (SG == 1) {
+      auto dir = [&](int c) { return z[c] + beta * p_in[c]; };
+      if (row == row0) {
+        // the first 8 entries from the prefetched operands, then the rest
+        double t[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) t[u] = pf.b + u < pf.e ? pf.v[u] * (pf.x0[u] + beta * pf.x1[u]) : 0.0;
-      s = 0.0;
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (pf.b + u < pf.e) s += t[u];
-      for (int k0 = pf.b + 8; k0 < pf.e; k0 += 8) {
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int k = k0 + u;
-          if (k < pf.e) {
-            const int c = ac[k];
-            t[u] = av[k] * (z[c] + beta * p_in[c]);
-          } else {
-            t[u] = 0.0;
-          }
-        }
+        for (int u = 0; u < 8; ++u) t[u] = pf.in[u] ? pf.v[u] * (pf.x0[u] + beta * pf.x1[u]) : 0.0;
+        s = 0.0;
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-          if (k0 + u < pf.e) s += t[u];
+          if (pf.in[u]) s += t[u];
+        if (pf.rv.count > 8) {
+          // continue the same sequential sum
+          double s2 = s;
+          for (int u0 = 8; u0 < pf.rv.count; u0 += 8) {
+            double t2[8];
+            bool in2[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const int k = pf.rv.base + (u0 + u) * pf.rv.stride;
+              const int c = u0 + u < pf.rv.count ? ac[k] : -1;
+              in2[u] = c >= 0;
+              t2[u] = in2[u] ? av[k] * dir(c) : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+              if (in2[u]) s2 += t2[u];
+          }
+          s = s2;
+        }
+        pr = z0 + beta * p0;
+      } else {
+        s = RowSum<double>(rows.Row(row), ac, av, 0, dir);
+        pr = z[row] + beta * p_in[row];
       }
-      pr = z0 + beta * p0;
     } else {
-      s = RowDotDir<SG>(ap_, ac, av, z, p_in, beta, row, sl);
+      s = RowDotDir<SG>(rows.ptr, ac, av, z, p_in, beta, row, sl);
       pr = z[row] + beta * p_in[row];
     }
     if (sl == 0) {
@@ -322,52 +377,31 @@ __global__ __launch_bounds__(kBlk) void PcgDirSpmvA(int n, int par, const int* _
   StorePartial(acc, part + kP1 * G);
 }
 
-// Row of M times r' = r - alpha Ap, r' formed per entry (same order as RowDot).
+// Row of M times r' = r - alpha Ap, r' formed per entry (SG > 1, CSR).
 template <int SG, class MV>
 __device__ __forceinline__ double RowDotUpdated(const int* __restrict__ ptr, const int* __restrict__ col,
                                                 const MV* __restrict__ val, const double* __restrict__ r,
                                                 const double* __restrict__ Ap, double alpha, int row, int sl) {
   double s = 0.0;
-  if constexpr (SG == 1) {
-    const int b = ptr[row], e = ptr[row + 1];
-    for (int k0 = b; k0 < e; k0 += 8) {
-      double t[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int k = k0 + u;
-        if (k < e) {
-          const int c = col[k];
-          t[u] = double(val[k]) * (r[c] - alpha * Ap[c]);
-        } else {
-          t[u] = 0.0;
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (k0 + u < e) s += t[u];
-    }
-  } else {
-    const int e = ptr[row + 1];
-    for (int k = ptr[row] + sl; k < e; k += SG) {
-      const int c = col[k];
-      s += double(val[k]) * (r[c] - alpha * Ap[c]);
-    }
-#pragma unroll
-    for (int off = SG / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, SG);
+  const int e = ptr[row + 1];
+  for (int k = ptr[row] + sl; k < e; k += SG) {
+    const int c = col[k];
+    s += double(val[k]) * (r[c] - alpha * Ap[c]);
   }
+#pragma unroll
+  for (int off = SG / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, SG);
   return s;
 }
 
 // K2: alpha (backward_error.pyx:102-110), x += alpha p, r' = r - alpha Ap,
 // z = M r'; partials of <r', r'> and <r', z> (backward_error.pyx:111-124).
-template <int SG, class MV>
-__global__ __launch_bounds__(kBlk) void PcgUpdateSpmvM(int n, int par, const int* __restrict__ mp,
-                                                       const int* __restrict__ mc, const MV* __restrict__ mv,
-                                                       double* __restrict__ x, const double* __restrict__ r_in,
-                                                       double* __restrict__ r_out, const double* __restrict__ p,
-                                                       const double* __restrict__ Ap, double* __restrict__ z,
-                                                       double* __restrict__ part, double* __restrict__ ds,
-                                                       int* __restrict__ is) {
+template <int SG, class MV, class Rows>
+__global__ __launch_bounds__(kBlk) void PcgUpdateSpmvM(int n, int par, Rows rows, const int* __restrict__ mc,
+                                                       const MV* __restrict__ mv, double* __restrict__ x,
+                                                       const double* __restrict__ r_in, double* __restrict__ r_out,
+                                                       const double* __restrict__ p, const double* __restrict__ Ap,
+                                                       double* __restrict__ z, double* __restrict__ part,
+                                                       double* __restrict__ ds, int* __restrict__ is) {
   if (is[kDone]) return;
   const int G = gridDim.x;
   const int row0 = XcdBlock() * (kBlk / SG) + threadIdx.x / SG;
@@ -375,7 +409,7 @@ __global__ __launch_bounds__(kBlk) void PcgUpdateSpmvM(int n, int par, const int
   double x0 = 0.0, p0 = 0.0, r0 = 0.0, a0 = 0.0;
   if constexpr (SG == 1) {
     const bool have = row0 < n;
-    pf.Load(mp, mc, mv, r_in, Ap, have ? row0 : -1);
+    pf.Load(rows, mc, mv, r_in, Ap, have ? row0 : -1);
     x0 = have ? x[row0] : 0.0;
     p0 = have ? p[row0] : 0.0;
     r0 = have ? r_in[row0] : 0.0;
@@ -401,34 +435,39 @@ __global__ __launch_bounds__(kBlk) void PcgUpdateSpmvM(int n, int par, const int
   double rr = 0.0, rz = 0.0;
   for (int row = row0; row < n; row += G * groups) {
     double zs, xv, ri;
-    if (SG == 1 && row == row0) {
-      double t[8];
+    if constexpr (SG == 1) {
+      auto upd = [&](int c) { return r_in[c] - alpha * Ap[c]; };
+      if (row == row0) {
+        double t[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
-        t[u] = pf.b + u < pf.e ? double(pf.v[u]) * (pf.x0[u] - alpha * pf.x1[u]) : 0.0;
-      zs = 0.0;
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (pf.b + u < pf.e) zs += t[u];
-      for (int k0 = pf.b + 8; k0 < pf.e; k0 += 8) {
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int k = k0 + u;
-          if (k < pf.e) {
-            const int c = mc[k];
-            t[u] = double(mv[k]) * (r_in[c] - alpha * Ap[c]);
-          } else {
-            t[u] = 0.0;
-          }
-        }
+        for (int u = 0; u < 8; ++u) t[u] = pf.in[u] ? double(pf.v[u]) * (pf.x0[u] - alpha * pf.x1[u]) : 0.0;
+        zs = 0.0;
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-          if (k0 + u < pf.e) zs += t[u];
+          if (pf.in[u]) zs += t[u];
+        for (int u0 = 8; u0 < pf.rv.count; u0 += 8) {
+          double t2[8];
+          bool in2[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int k = pf.rv.base + (u0 + u) * pf.rv.stride;
+            const int c = u0 + u < pf.rv.count ? mc[k] : -1;
+            in2[u] = c >= 0;
+            t2[u] = in2[u] ? double(mv[k]) * upd(c) : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (in2[u]) zs += t2[u];
+        }
+        xv = x0 + alpha * p0;
+        ri = r0 - alpha * a0;
+      } else {
+        zs = RowSum<MV>(rows.Row(row), mc, mv, 0, upd);
+        xv = x[row] + alpha * p[row];
+        ri = r_in[row] - alpha * Ap[row];
       }
-      xv = x0 + alpha * p0;
-      ri = r0 - alpha * a0;
     } else {
-      zs = RowDotUpdated<SG, MV>(mp, mc, mv, r_in, Ap, alpha, row, sl);
+      zs = RowDotUpdated<SG, MV>(rows.ptr, mc, mv, r_in, Ap, alpha, row, sl);
       xv = x[row] + alpha * p[row];
       ri = r_in[row] - alpha * Ap[row];
     }
@@ -547,6 +586,36 @@ __global__ __launch_bounds__(kBlk) void PcgSpmvA(int n, const int* __restrict__ 
   StorePartial(acc, part + kP1 * gridDim.x);
 }
 
+// SELL construction: per-slice width (x 64 entries), then the slots.
+__global__ void SellWidths(int64_t n, int64_t nslices, const int* __restrict__ ptr, int* __restrict__ slots) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= nslices) return;
+  int w = 0;
+  const int64_t r1 = min(n, (s + 1) * 64);
+  for (int64_t r = s * 64; r < r1; ++r) w = max(w, ptr[r + 1] - ptr[r]);
+  slots[s] = w * 64;
+}
+
+__global__ void SellFill(int64_t n, int64_t nslices, const int* __restrict__ off, const int* __restrict__ ptr,
+                         const int* __restrict__ col, int* __restrict__ scol, int* __restrict__ ssrc) {
+  const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= nslices * 64) return;
+  const int s = int(r >> 6), l = int(r & 63);
+  const int b = off[s], w = (off[s + 1] - b) >> 6;
+  const int rb = r < n ? ptr[r] : 0, len = r < n ? ptr[r + 1] - rb : 0;
+  for (int u = 0; u < w; ++u) {
+    const int k = b + u * 64 + l;
+    scol[k] = u < len ? col[rb + u] : -1;
+    ssrc[k] = u < len ? rb + u : -1;
+  }
+}
+
+template <class T>
+__global__ void SellGather(int64_t len, const int* __restrict__ src, const T* __restrict__ v, T* __restrict__ out) {
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < len) out[k] = src[k] >= 0 ? v[src[k]] : T(0);
+}
+
 __global__ void CsrFromInt64(int64_t n, const int64_t* __restrict__ in, int64_t base, int* __restrict__ out) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) out[i] = int(in[i] - base);
@@ -634,6 +703,33 @@ void CopyValues(Ctx& ctx, DCsr& dst, const DCsr& src) {
     ToFloat<<<(unsigned)((dst.nnz + 255) / 256), 256, 0, ctx.stream>>>(dst.nnz, dst.val.get(), dst.val32.get());
 }
 
+DSell SellFromCsr(Ctx& ctx, const DCsr& m, bool fp32) {
+  DSell s;
+  s.n = m.n;
+  s.nslices = (m.n + 63) / 64;
+  auto slots = ctx.Alloc<int>(std::max<int64_t>(s.nslices, 1));
+  SellWidths<<<(unsigned)((s.nslices + 255) / 256), 256, 0, ctx.stream>>>(m.n, s.nslices, m.ptr.get(), slots.get());
+  s.off = ctx.Alloc<int>(s.nslices + 1);
+  s.len = ScanCounts(ctx, slots.get(), s.nslices, s.off.get());
+  if (s.len >= (int64_t(1) << 31)) throw std::runtime_error("SELL copy exceeds the int32 range");
+  s.col = ctx.Alloc<int>(std::max<int64_t>(s.len, 1));
+  s.src = ctx.Alloc<int>(std::max<int64_t>(s.len, 1));
+  SellFill<<<(unsigned)((s.nslices * 64 + 255) / 256), 256, 0, ctx.stream>>>(m.n, s.nslices, s.off.get(),
+                                                                            m.ptr.get(), m.col.get(), s.col.get(),
+                                                                            s.src.get());
+  s.val = ctx.Alloc<double>(std::max<int64_t>(s.len, 1));
+  if (fp32) s.val32 = ctx.Alloc<float>(std::max<int64_t>(s.len, 1));
+  SellValues(ctx, s, m);
+  return s;
+}
+
+void SellValues(Ctx& ctx, DSell& s, const DCsr& m) {
+  if (s.len == 0) return;
+  const unsigned g = (unsigned)((s.len + 255) / 256);
+  SellGather<double><<<g, 256, 0, ctx.stream>>>(s.len, s.src.get(), m.val.get(), s.val.get());
+  if (s.val32.get()) SellGather<float><<<g, 256, 0, ctx.stream>>>(s.len, s.src.get(), m.val32.get(), s.val32.get());
+}
+
 PcgSolver::PcgSolver(Ctx& ctx, const DCsr& A, const DCsr& M, bool m_fp32)
     : ctx_(ctx), A_(A), M_(M), m_fp32_(m_fp32), n_(A.n) {
   if (M.n != A.n) throw std::invalid_argument("A and M must have the same shape");
@@ -647,6 +743,16 @@ PcgSolver::PcgSolver(Ctx& ctx, const DCsr& A, const DCsr& M, bool m_fp32)
   part_ = ctx.Alloc<double>(size_t(kNumP) * grid_);
   scal_ = ctx.Alloc<double>(kNumD);
   ist_ = ctx.Alloc<int>(kNumI);
+  // one thread per row: the iteration kernels read SELL copies (coalesced)
+  sell_a_ = SubgroupFor(A.n, A.nnz) == 1;
+  sell_m_ = SubgroupFor(M.n, M.nnz) == 1;
+  if (sell_a_) a_sell_ = SellFromCsr(ctx, A, false);
+  if (sell_m_) m_sell_ = SellFromCsr(ctx, M, m_fp32);
+  ctx.Sync();
+}
+
+void PcgSolver::RefreshPreconditioner() {
+  if (sell_m_) SellValues(ctx_, m_sell_, M_);
 }
 
 PcgSolver::~PcgSolver() { DropGraphs(); }
@@ -686,18 +792,31 @@ void PcgSolver::Record(int first_parity, int64_t iterations) {
     double* p_out = p_.get() + size_t(par ^ 1) * n_;
     const double* r_in = r_.get() + size_t(par) * n_;
     double* r_out = r_.get() + size_t(par ^ 1) * n_;
-    SPAI_SG_DISPATCH(sga, (PcgDirSpmvA<SG><<<grid_, kBlk, 0, st>>>(n, par, A_.ptr.get(), A_.col.get(),
-                                                                   A_.val.get(), z_.get(), p_in, p_out, ap_.get(),
-                                                                   part_.get(), scal_.get(), ist_.get(),
-                                                                   hist_.get())));
-    if (m_fp32_) {
-      SPAI_SG_DISPATCH(sgm, (PcgUpdateSpmvM<SG, float><<<grid_, kBlk, 0, st>>>(
-                                n, par, M_.ptr.get(), M_.col.get(), M_.val32.get(), x_.get(), r_in, r_out, p_out,
-                                ap_.get(), z_.get(), part_.get(), scal_.get(), ist_.get())));
+    if (sell_a_) {
+      PcgDirSpmvA<1, SellRows><<<grid_, kBlk, 0, st>>>(n, par, SellRows{a_sell_.off.get()}, a_sell_.col.get(),
+                                                       a_sell_.val.get(), z_.get(), p_in, p_out, ap_.get(),
+                                                       part_.get(), scal_.get(), ist_.get(), hist_.get());
     } else {
-      SPAI_SG_DISPATCH(sgm, (PcgUpdateSpmvM<SG, double><<<grid_, kBlk, 0, st>>>(
-                                n, par, M_.ptr.get(), M_.col.get(), M_.val.get(), x_.get(), r_in, r_out, p_out,
-                                ap_.get(), z_.get(), part_.get(), scal_.get(), ist_.get())));
+      SPAI_SG_DISPATCH(sga, (PcgDirSpmvA<SG, CsrRows><<<grid_, kBlk, 0, st>>>(
+                                n, par, CsrRows{A_.ptr.get()}, A_.col.get(), A_.val.get(), z_.get(), p_in, p_out,
+                                ap_.get(), part_.get(), scal_.get(), ist_.get(), hist_.get())));
+    }
+    if (sell_m_ && m_fp32_) {
+      PcgUpdateSpmvM<1, float, SellRows><<<grid_, kBlk, 0, st>>>(
+          n, par, SellRows{m_sell_.off.get()}, m_sell_.col.get(), m_sell_.val32.get(), x_.get(), r_in, r_out, p_out,
+          ap_.get(), z_.get(), part_.get(), scal_.get(), ist_.get());
+    } else if (sell_m_) {
+      PcgUpdateSpmvM<1, double, SellRows><<<grid_, kBlk, 0, st>>>(
+          n, par, SellRows{m_sell_.off.get()}, m_sell_.col.get(), m_sell_.val.get(), x_.get(), r_in, r_out, p_out,
+          ap_.get(), z_.get(), part_.get(), scal_.get(), ist_.get());
+    } else if (m_fp32_) {
+      SPAI_SG_DISPATCH(sgm, (PcgUpdateSpmvM<SG, float, CsrRows><<<grid_, kBlk, 0, st>>>(
+                                n, par, CsrRows{M_.ptr.get()}, M_.col.get(), M_.val32.get(), x_.get(), r_in, r_out,
+                                p_out, ap_.get(), z_.get(), part_.get(), scal_.get(), ist_.get())));
+    } else {
+      SPAI_SG_DISPATCH(sgm, (PcgUpdateSpmvM<SG, double, CsrRows><<<grid_, kBlk, 0, st>>>(
+                                n, par, CsrRows{M_.ptr.get()}, M_.col.get(), M_.val.get(), x_.get(), r_in, r_out,
+                                p_out, ap_.get(), z_.get(), part_.get(), scal_.get(), ist_.get())));
     }
   }
 }

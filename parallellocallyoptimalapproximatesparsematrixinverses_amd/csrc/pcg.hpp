@@ -17,6 +17,19 @@ struct DCsr {
   DBuf<float> val32;
 };
 
+// Sliced ELLPACK copy of a CSR matrix for one-thread-per-row SpMV: slices of
+// 64 consecutive rows (one wavefront), each slice width = its longest row,
+// entries stored column-major inside a slice (entry u of row r at
+// off[s] + 64 u + (r mod 64)), so the u-th loads of a wavefront are
+// contiguous.  Padding has col = -1.  src[k] is the CSR position of slot k
+// (-1 for padding), used to refresh the values.
+struct DSell {
+  int64_t n = 0, nslices = 0, len = 0;
+  DBuf<int> off, col, src;
+  DBuf<double> val;
+  DBuf<float> val32;
+};
+
 DCsr CsrFromHost(Ctx& ctx, int64_t n, int64_t nnz, const int64_t* ptr, const int64_t* col, const double* val);
 // CSR of the matrix stored as CSC in m (i.e. the transpose of its CSC arrays).
 DCsr CsrFromCsc(Ctx& ctx, const DMat& m);
@@ -24,6 +37,10 @@ void MakeFp32Copy(Ctx& ctx, DCsr& m);
 // dst's values (and FP32 copy, if it has one) from src of the same pattern;
 // dst's buffers stay where they are, so graphs recorded on them stay valid.
 void CopyValues(Ctx& ctx, DCsr& dst, const DCsr& src);
+// SELL layout of m; values (and an FP32 copy when fp32) gathered from m.
+DSell SellFromCsr(Ctx& ctx, const DCsr& m, bool fp32);
+// Re-gathers the values of s from m (same pattern).
+void SellValues(Ctx& ctx, DSell& s, const DCsr& m);
 // y = A x (methods_cython/backward_error.pyx:27-34 _multiply).
 void SpMV(Ctx& ctx, const DCsr& A, const double* x, double* y);
 
@@ -35,6 +52,9 @@ class PcgSolver {
   int Solve(const double* b_dev, int64_t max_iterations, double tolerance, int64_t* count);
   void History(double* host, int64_t count);
   const double* X() const { return x_.get(); }
+  // After M's values changed in place (same pattern): refresh the solver's
+  // copies of them.  Recorded graphs stay valid.
+  void RefreshPreconditioner();
   // Average device time of one iteration's A-SpMV kernel over the last solve
   // (microseconds, from HIP events around a standalone replay).
   double TimeSpmvA(int reps);
@@ -48,6 +68,8 @@ class PcgSolver {
   bool m_fp32_;
   int64_t n_;
   unsigned grid_ = 1;
+  bool sell_a_ = false, sell_m_ = false;  // one-thread-per-row iteration kernels on SELL copies
+  DSell a_sell_, m_sell_;
   DBuf<double> x_, r_, z_, p_, ap_, part_, hist_;
   DBuf<double> scal_;
   DBuf<int> ist_;
