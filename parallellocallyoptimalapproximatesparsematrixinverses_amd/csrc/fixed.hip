@@ -32,7 +32,7 @@ namespace {
 
 constexpr int kBlk = 256;
 constexpr int kMaxGrid = 2048;
-constexpr int kMaxFusedGrid = 2048;
+constexpr int kMaxFusedGrid = 1024;
 
 // Scalar slots in device memory.
 enum Slot : int {

@@ -35,7 +35,7 @@ namespace spai {
 
 namespace {
 
-constexpr int kBlk = 256;
+constexpr int kBlk = 512;
 constexpr int kChunk = 16;  // iterations per captured graph (even)
 
 enum DSlot : int { kBnorm = 0, kTol, kRz0, kRz1, kErrPre, kNumD };
