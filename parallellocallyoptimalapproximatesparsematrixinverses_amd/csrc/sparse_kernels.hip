@@ -319,28 +319,25 @@ __global__ __launch_bounds__(kNumWaves* kWave) void NumericGustavson(
 
 // ------------------------------------------------------------------ filters
 // Keep flags -> per-column counts.
-__global__ void CountKept(int ncols, const int* __restrict__ ptr, const unsigned char* __restrict__ keep,
-                          int* __restrict__ counts) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ncols) return;
-  int c = 0;
-  for (int p = ptr[j]; p < ptr[j + 1]; ++p) c += keep[p];
-  counts[j] = c;
+__global__ void KeepToInt(int64_t nnz, const unsigned char* __restrict__ keep, int* __restrict__ flags) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < nnz) flags[i] = keep[i];
 }
 
-__global__ void FillKept(int ncols, const int* __restrict__ ptr, const int* __restrict__ idx,
-                         const double* __restrict__ val, const unsigned char* __restrict__ keep,
-                         const int* __restrict__ optr, int* __restrict__ oidx, double* __restrict__ oval) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ncols) return;
-  int o = optr[j];
-  for (int p = ptr[j]; p < ptr[j + 1]; ++p) {
-    if (keep[p]) {
-      oidx[o] = idx[p];
-      oval[o] = val[p];
-      ++o;
-    }
-  }
+// out[j] = pos[ptr[j]] for j <= ncols.
+__global__ void GatherPtr(int64_t m, const int* __restrict__ ptr, const int* __restrict__ pos, int* __restrict__ out) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j < m) out[j] = pos[ptr[j] - ptr[0]];
+}
+
+__global__ void ScatterKept(int64_t nnz, const int* __restrict__ idx, const double* __restrict__ val,
+                            const unsigned char* __restrict__ keep, const int* __restrict__ pos,
+                            int* __restrict__ oidx, double* __restrict__ oval) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nnz || !keep[i]) return;
+  const int o = pos[i];
+  oidx[o] = idx[i];
+  oval[o] = val[i];
 }
 
 __global__ void FlagNonZero(int64_t nnz, const double* __restrict__ val, unsigned char* __restrict__ keep,
@@ -551,51 +548,74 @@ __global__ void ThresholdBatchesKernel(int64_t nnz_cols, int ncols, const int* _
 
 // Per-column top-k by |v| (diagonal kept, ties keep the smaller row): rank of
 // each off-diagonal entry among the column's off-diagonal entries.
-__global__ void TopKKernel(int ncols, const int* __restrict__ ptr, const int* __restrict__ idx,
-                           const double* __restrict__ v, int64_t k, unsigned char* __restrict__ keep) {
-  constexpr int kStage = 1024;
-  __shared__ double sa[kStage];
-  __shared__ int si[kStage];
+// Per-column top-k by |v| with the diagonal always kept; ties keep the
+// smaller row.  One block per column: an 8-pass radix select over the
+// order-preserving keys of |v| finds the k-th largest key T and how many
+// entries equal to T are taken; entries above T are kept, and an entry equal
+// to T is kept when fewer than that many equal entries have a smaller row.
+__global__ __launch_bounds__(256) void TopKKernel(int ncols, const int* __restrict__ ptr, const int* __restrict__ idx,
+                                                  const double* __restrict__ v, int64_t k,
+                                                  unsigned char* __restrict__ keep) {
+  __shared__ unsigned hist[256];
+  __shared__ uint64_t s_prefix;
+  __shared__ int s_target;
   const int j = blockIdx.x;
   const int b = ptr[j], e = ptr[j + 1];
-  if (e - b <= kStage) {
-    // column staged in LDS: |v| and row (the diagonal marked -1, never counted)
-    for (int p = b + threadIdx.x; p < e; p += blockDim.x) {
-      sa[p - b] = fabs(v[p]);
-      si[p - b] = idx[p] == j ? -1 : idx[p];
-    }
-    __syncthreads();
-    const int L = e - b;
-    for (int t = threadIdx.x; t < L; t += blockDim.x) {
-      const int row = si[t];
-      if (row < 0) {
-        keep[b + t] = 1;
-        continue;
-      }
-      const double a = sa[t];
-      int64_t rank = 0;
-      for (int u = 0; u < L; ++u) {
-        const int r2 = si[u];
-        const double c = sa[u];
-        rank += (r2 >= 0 && u != t && (c > a || (c == a && r2 < row))) ? 1 : 0;
-      }
-      keep[b + t] = rank < k ? 1 : 0;
-    }
+  // off-diagonal entries of the column
+  int offd = 0;
+  for (int p = b + threadIdx.x; p < e; p += blockDim.x) offd += idx[p] != j ? 1 : 0;
+  __shared__ int red[16];
+  offd = BlockSum<int>(offd, red);
+  if (offd <= k) {
+    for (int p = b + threadIdx.x; p < e; p += blockDim.x) keep[p] = 1;
     return;
   }
+  if (threadIdx.x == 0) {
+    s_prefix = 0;
+    s_target = int(k);  // rank of the wanted key counted from the largest
+  }
+  uint64_t mask = 0;
+  for (int shift = 56; shift >= 0; shift -= 8) {
+    for (int t = threadIdx.x; t < 256; t += blockDim.x) hist[t] = 0;
+    __syncthreads();
+    const uint64_t prefix = s_prefix;
+    for (int p = b + threadIdx.x; p < e; p += blockDim.x) {
+      if (idx[p] == j) continue;
+      const uint64_t key = OrderedKey(fabs(v[p]));
+      if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int target = s_target;
+      int bin = 255;
+      for (; bin > 0; --bin) {
+        const int c = int(hist[bin]);
+        if (c >= target) break;
+        target -= c;
+      }
+      s_target = target;
+      s_prefix = prefix | (uint64_t(bin) << shift);
+    }
+    mask |= uint64_t(255) << shift;
+    __syncthreads();
+  }
+  const uint64_t T = s_prefix;
+  const int take = s_target;  // how many entries with key == T are kept
   for (int p = b + threadIdx.x; p < e; p += blockDim.x) {
-    if (idx[p] == j) {
+    const int row = idx[p];
+    if (row == j) {
       keep[p] = 1;
       continue;
     }
-    const double a = fabs(v[p]);
-    int64_t rank = 0;
-    for (int q = b; q < e; ++q) {
-      if (q == p || idx[q] == j) continue;
-      const double c = fabs(v[q]);
-      if (c > a || (c == a && idx[q] < idx[p])) ++rank;
+    const uint64_t key = OrderedKey(fabs(v[p]));
+    unsigned char kp = key > T ? 1 : 0;
+    if (key == T) {
+      int before = 0;
+      for (int q = b; q < e; ++q)
+        if (idx[q] != j && idx[q] < row && OrderedKey(fabs(v[q])) == T) ++before;
+      kp = before < take ? 1 : 0;
     }
-    keep[p] = rank < k ? 1 : 0;
+    keep[p] = kp;
   }
 }
 
@@ -1064,17 +1084,23 @@ namespace {
 
 // New matrix made of the entries of X whose keep flag is set.
 DMat Filter(Ctx& ctx, const DMat& X, const unsigned char* keep) {
-  const int64_t n = X.cols();
-  auto counts = ctx.Alloc<int>(n > 0 ? n : 1);
-  CountKept<<<Blocks(n), kBlock, 0, ctx.stream>>>((int)n, X.ptr(), keep, counts.get());
+  // stream compaction over all entries: exclusive scan of the keep flags
+  // gives every kept entry its output position, and the scan at the column
+  // starts gives the output column pointers
+  const int64_t n = X.cols(), nnz = X.nnz();
   DMat out;
   out.pat = MakePattern(ctx, X.rows(), n, 0);
-  const int64_t total = ScanCounts(ctx, counts.get(), n, out.pat->ptr.get());
+  auto flags = ctx.Alloc<int>(nnz > 0 ? nnz : 1);
+  auto pos = ctx.Alloc<int>(nnz + 1);
+  if (nnz > 0) KeepToInt<<<Blocks(nnz), kBlock, 0, ctx.stream>>>(nnz, keep, flags.get());
+  const int64_t total = ScanCounts(ctx, flags.get(), nnz, pos.get());
   out.pat->nnz = total;
+  GatherPtr<<<Blocks(n + 1), kBlock, 0, ctx.stream>>>(n + 1, X.ptr(), pos.get(), out.pat->ptr.get());
   out.pat->idx = ctx.Alloc<int>(total > 0 ? total : 1);
   out.val = ctx.Alloc<double>(total > 0 ? total : 1);
-  FillKept<<<Blocks(n), kBlock, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), X.v(), keep, out.pat->ptr.get(),
-                                                 out.pat->idx.get(), out.val.get());
+  if (nnz > 0)
+    ScatterKept<<<Blocks(nnz), kBlock, 0, ctx.stream>>>(nnz, X.idx(), X.v(), keep, pos.get(), out.pat->idx.get(),
+                                                        out.val.get());
   return out;
 }
 
@@ -1306,7 +1332,7 @@ DMat ThresholdBatches(Ctx& ctx, const DMat& X, const double* batch_sq, const int
 DMat ColumnTopK(Ctx& ctx, const DMat& X, int64_t k) {
   const int64_t n = X.cols(), nnz = X.nnz();
   auto keep = ctx.Alloc<unsigned char>(nnz > 0 ? nnz : 1);
-  if (n > 0) TopKKernel<<<(unsigned)n, 64, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), X.v(), k, keep.get());
+  if (n > 0) TopKKernel<<<(unsigned)n, 256, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), X.v(), k, keep.get());
   return Filter(ctx, X, keep.get());
 }
 
