@@ -32,7 +32,7 @@ namespace {
 
 constexpr int kBlk = 256;
 constexpr int kMaxGrid = 2048;
-constexpr int kMaxFusedGrid = 1024;
+constexpr int kMaxFusedGrid = 2048;
 
 // Scalar slots in device memory.
 enum Slot : int {
@@ -301,68 +301,108 @@ __global__ void ExclusiveScan64Serial(int n, const int64_t* __restrict__ in, int
 // R/S-sized reads and writes of a tile run over the tile's contiguous range
 // with all 64 lanes, so stores cover whole lines.
 
-// Per-group LDS region of the fused pass, in values of vsize bytes:
-// accumulators (2 x lmax), R' of the column (lmax), Z' and P' of the column
-// (2 x smax), then the staging of SlotAccumulate (GL x (int2 + 2 values)),
-// 16-byte aligned.
-__host__ __device__ constexpr int FusedGroupWords(int lmax, int smax, int gl, int vsize) {
-  return (((3 * lmax + 2 * smax) * vsize + 15) / 16 * 16 + gl * (8 + 2 * vsize) + 15) / 16 * 16 / vsize;
-}
-__host__ __device__ constexpr int FusedStageOffset(int lmax, int smax, int vsize) {
-  return ((3 * lmax + 2 * smax) * vsize + 15) / 16 * 16 / vsize;
+// Wavefront-local LDS of the fused pass (byte offsets from the wavefront's
+// region), every array 16-byte aligned:
+//   acc    V2[W * lmax]  the products on the tile's S2 range (two sums side by
+//                        side), indexed by S2 position minus the tile's first;
+//   rr     V [W * lmax]  R' on the tile's S2 range, same indexing;
+//   yzp    V2[W * smax]  (Z', P') on the tile's S range, indexed likewise;
+//   stage  step operands of a product, step-major ([GL][W] int2, then [GL][W]
+//          V2), so the W columns of one step read W distinct banks;
+//   stash  FusedTerm[smax][64]: with the stash on, product 1 records every
+//          lane's (A value, slot) of every step and product 2 reads them back
+//          instead of loading the plan and A again.  Product 2 then needs no
+//          staging, so the staging overlays rr and yzp (unused in product 1).
+__host__ __device__ constexpr int Align16(int b) { return (b + 15) / 16 * 16; }
+
+struct FusedLayout {
+  int acc, rr, yzp, stage, stage_y, stash, bytes;
+};
+
+__host__ __device__ constexpr FusedLayout MakeFusedLayout(int lmax, int smax, int gl, int vsize, bool stash) {
+  const int w = kWave / gl;
+  const int rr = Align16(2 * w * lmax * vsize);
+  const int yzp = rr + Align16(w * lmax * vsize);
+  const int free_end = yzp + Align16(2 * w * smax * vsize);
+  const int stage = stash ? rr : free_end;
+  const int stage_y = stage + Align16(kWave * 8);
+  const int stage_end = stage_y + Align16(kWave * 2 * vsize);
+  const int stash_at = stage_end > free_end ? stage_end : free_end;
+  const int bytes = stash ? stash_at + Align16(smax * kWave * 2 * vsize) : stage_end;
+  return FusedLayout{0, rr, yzp, stage, stage_y, stash_at, bytes};
 }
 
-
-// Column-wise Gustavson accumulation of A [y0 y1](:, j) into acc0/acc1, in
-// the order of SlotProduct (S entries of the column in order, k ascending).
-// YGet(e, ya, yb) returns S entry e's values.  The group stages up to GL S
-// entries -- (A(:, k) start, length) and (ya, yb) -- in LDS, then every step
-// (one S entry, column k of A) reads them with one broadcast LDS load per
-// array; when |A(:, k)| <= GL lane gl takes the term u = gl, and the plan
-// slots and A values of up to kPre steps are loaded before any of them is
-// added, so a column waits for one load latency per kPre steps.
 template <class V>
 struct alignas(2 * sizeof(V)) V2 {
   V a, b;
 };
 
-template <class V, class SlotT, int GL, int NY, class YGet>
-__device__ __forceinline__ void SlotAccumulate(int j, int gl, const int* __restrict__ sp, const int2* __restrict__ a0c,
-                                               const V* __restrict__ av, const int64_t* __restrict__ plan_ptr,
-                                               const SlotT* __restrict__ plan, V* acc0, V* acc1, int2* st_ac,
-                                               V2<V>* st_y, YGet yget) {
-  constexpr int kPre = 4;
+// One term of product 1 as product 2 reuses it; slot -1: the lane had no term.
+template <class V>
+struct alignas(2 * sizeof(V)) FusedTerm {
+  V a;
+  int slot;
+};
+
+// acc[slot] += a * (y.a, y.b) (NY == 1: the first sum only), as one LDS read
+// and one write per sum pair.
+template <class V, int NY>
+__device__ __forceinline__ void AddTerm(V2<V>* acc, int slot, V a, V2<V> y) {
+  if constexpr (NY > 1) {
+    V2<V> t = acc[slot];
+    t.a += a * y.a;
+    t.b += a * y.b;
+    acc[slot] = t;
+  } else {
+    acc[slot].a += a * y.a;
+  }
+}
+
+// Column-wise Gustavson accumulation of A [y0 y1](:, j) into acc (column j's
+// S2 range), in the order of SlotProduct (S entries of the column in order, k
+// ascending).  YGet(e, ya, yb) returns S entry e's values.  The group stages
+// up to GL S entries -- (A(:, k) start, length) and (ya, yb) -- in LDS, then
+// every step (one S entry, column k of A) reads them with one broadcast LDS
+// load; when |A(:, k)| <= GL lane gl takes the term u = gl, and the plan slots
+// and A values of up to kPre steps are loaded before any of them is added, so
+// a column waits for one load latency per kPre steps.  kStash: every lane's
+// (A value, slot) of every step goes to stash[step][lane] (no column of A is
+// longer than GL then).
+template <class V, class SlotT, int GL, int NY, bool kStash, class YGet>
+__device__ __forceinline__ void FusedProduct1(int j, int g, int gl, const int* __restrict__ sp,
+                                              const int2* __restrict__ a0c, const V* __restrict__ av,
+                                              const int64_t* __restrict__ plan_ptr, const SlotT* __restrict__ plan,
+                                              V2<V>* acc, int2* st_ac, V2<V>* st_y, FusedTerm<V>* stash,
+                                              YGet yget) {
+  constexpr int W = kWave / GL;
+  constexpr int kPre = 8;
+  const int lane = g * GL + gl;
   uint32_t pp = uint32_t(plan_ptr[j]);
   const int e0 = sp[j], e1 = sp[j + 1];
   for (int eb = e0; eb < e1; eb += GL) {
     if (eb + gl < e1) {
       V ya = V(0), yb = V(0);
       yget(eb + gl, ya, yb);
-      st_ac[gl] = a0c[eb + gl];
-      st_y[gl] = V2<V>{ya, yb};
+      st_ac[gl * W + g] = a0c[eb + gl];
+      st_y[gl * W + g] = V2<V>{ya, yb};
     }
     __builtin_amdgcn_wave_barrier();
     const int ns = min(GL, e1 - eb);
     for (int s0 = 0; s0 < ns; s0 += kPre) {
-      const int cnt = min(kPre, ns - s0);
       SlotT sl[kPre];
-      V a[kPre], ya[kPre], yb[kPre];
-      int cc[kPre];
+      V a[kPre];
+      uint32_t act = 0;  // bit s: the lane has a term in step s0 + s
       uint32_t off = pp;
       bool wide = false;
 #pragma unroll
       for (int s = 0; s < kPre; ++s) {
-        cc[s] = 0;
         sl[s] = SlotT(0);
-        a[s] = ya[s] = yb[s] = V(0);
-        if (s < cnt) {
-          const int2 ac = st_ac[s0 + s];
-          const V2<V> y = st_y[s0 + s];
-          ya[s] = y.a;
-          yb[s] = y.b;
-          cc[s] = ac.y;
-          wide = wide || ac.y > GL;
+        a[s] = V(0);
+        if (s0 + s < ns) {
+          const int2 ac = st_ac[(s0 + s) * W + g];
+          if constexpr (!kStash) wide = wide || ac.y > GL;
           if (gl < ac.y) {
+            act |= 1u << s;
             sl[s] = plan[off + uint32_t(gl)];
             a[s] = av[uint32_t(ac.x + gl)];
           }
@@ -372,23 +412,19 @@ __device__ __forceinline__ void SlotAccumulate(int j, int gl, const int* __restr
       if (!wide) {
 #pragma unroll
         for (int s = 0; s < kPre; ++s) {
-          if (s < cnt && gl < cc[s]) {
-            acc0[sl[s]] += a[s] * ya[s];
-            if (NY > 1) acc1[sl[s]] += a[s] * yb[s];
+          if (s0 + s < ns) {
+            const bool act_s = (act >> s) & 1u;
+            if constexpr (kStash) stash[(eb - e0 + s0 + s) * kWave + lane] = FusedTerm<V>{a[s], act_s ? int(sl[s]) : -1};
+            if (act_s) AddTerm<V, NY>(acc, int(sl[s]), a[s], st_y[(s0 + s) * W + g]);
           }
         }
       } else {
         // a column of A longer than GL: lanes stride through each step
         uint32_t o2 = pp;
-        for (int s = 0; s < cnt; ++s) {
-          const int2 ac = st_ac[s0 + s];
-          const V2<V> y = st_y[s0 + s];
-          for (int u = gl; u < ac.y; u += GL) {
-            const int slot = plan[o2 + uint32_t(u)];
-            const V av_u = av[uint32_t(ac.x + u)];
-            acc0[slot] += av_u * y.a;
-            if (NY > 1) acc1[slot] += av_u * y.b;
-          }
+        for (int s = 0; s < min(kPre, ns - s0); ++s) {
+          const int2 ac = st_ac[(s0 + s) * W + g];
+          const V2<V> y = st_y[(s0 + s) * W + g];
+          for (int u = gl; u < ac.y; u += GL) AddTerm<V, NY>(acc, int(plan[o2 + uint32_t(u)]), av[uint32_t(ac.x + u)], y);
           o2 += uint32_t(ac.y);
         }
       }
@@ -398,21 +434,40 @@ __device__ __forceinline__ void SlotAccumulate(int j, int gl, const int* __restr
   }
 }
 
-// Tile-local position tables of the fused pass.  qinfo[q] (S2 entry q of
-// column j): (j mod W) << kQInfoShift | slot of q in S2(:, j).  sinfo[p] (S
-// entry p of column j): (j mod W) << 26 | slot of p in S2(:, j) << 16 |
-// position of p in S(:, j).  W = 64 / GL columns per wavefront tile.
-constexpr int kQInfoShift = 10;
-constexpr int kQInfoMask = (1 << kQInfoShift) - 1;
+// Product 2 from the stash: steps in the order of product 1, the step's Y
+// values (Z', P') of S entry s of the column read from LDS.
+template <class V, int NY>
+__device__ __forceinline__ void FusedProduct2Stash(int ns, int lane, const FusedTerm<V>* stash, const V2<V>* y,
+                                                   V2<V>* acc) {
+  constexpr int kPre = 4;
+  for (int s0 = 0; s0 < ns; s0 += kPre) {
+    FusedTerm<V> t[kPre];
+    V2<V> yv[kPre];
+#pragma unroll
+    for (int s = 0; s < kPre; ++s) {
+      t[s] = FusedTerm<V>{V(0), -1};
+      yv[s] = V2<V>{V(0), V(0)};
+      if (s0 + s < ns) {
+        t[s] = stash[(s0 + s) * kWave + lane];
+        yv[s] = y[s0 + s];
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < kPre; ++s)
+      if (t[s].slot >= 0) AddTerm<V, NY>(acc, t[s].slot, t[s].a, yv[s]);
+  }
+}
 
 // kInit: M = M0 already on S; R = I - A M, Z = Pr R on S, P = 0, and the
 // traces of A Z (the pass before iteration 1).  Otherwise one iteration with
 // the step sizes in sc: update, then the traces of the next iteration.
-template <class V, class SlotT, int GL, bool kInit, bool kMr>
-__global__ __launch_bounds__(kBlk) void FusedPass(
-    int n, const int* __restrict__ sp, const int2* __restrict__ a0c, const uint16_t* __restrict__ qinfo,
-    const uint32_t* __restrict__ sinfo, const int* __restrict__ s2p, const int* __restrict__ s2i,
-    const V* __restrict__ av,
+// stq[p] (S entry p of column j): position of p's row in S2(:, j), counted
+// from the first S2 entry of j's tile.  s2d[j]: position of the diagonal in
+// S2(:, j).
+template <class V, class SlotT, int GL, bool kInit, bool kMr, bool kStash>
+__global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
+    int n, const int* __restrict__ sp, const int2* __restrict__ a0c, const uint16_t* __restrict__ stq,
+    const int* __restrict__ s2d, const int* __restrict__ s2p, const V* __restrict__ av,
     const int64_t* __restrict__ plan_ptr, const SlotT* __restrict__ plan, int lmax, int smax,
     const V* __restrict__ prS, const V* __restrict__ prS2, V* __restrict__ R, V* __restrict__ M, V* __restrict__ Z,
     V* __restrict__ P, const double* __restrict__ sc, double* __restrict__ part_out,
@@ -421,7 +476,7 @@ __global__ __launch_bounds__(kBlk) void FusedPass(
   constexpr int RQ = 4;  // S2 entries of a tile held in registers per lane
   constexpr int RS = 2;  // S entries of a tile held in registers per lane
   if (flags->stop) return;
-  // LOMR: R -= delta AZ + gamma AP (kNeedR: R read); MR: R = I - A M recomputed
+  // LOMR: R -= delta AZ + gamma AP (R read); MR: R = I - A M recomputed
   constexpr bool kLomrIter = !kInit && !kMr;
   constexpr int NY = kLomrIter ? 2 : 1;
   V d = V(0), gm = V(0), be = V(0), nd = V(0), ng = V(0), alpha = V(0);
@@ -436,11 +491,14 @@ __global__ __launch_bounds__(kBlk) void FusedPass(
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = LaneId(), wib = threadIdx.x / kWave;
   const int g = lane / GL, gl = lane % GL;
-  const int words = FusedGroupWords(lmax, smax, GL, int(sizeof(V)));
-  V* const wave_lds = reinterpret_cast<V*>(smem_raw) + size_t(wib) * W * words;
-  V* const mine = wave_lds + size_t(g) * words;
-  int2* const st_ac = reinterpret_cast<int2*>(mine + FusedStageOffset(lmax, smax, int(sizeof(V))));
-  V2<V>* const st_y = reinterpret_cast<V2<V>*>(st_ac + GL);
+  const FusedLayout lay = MakeFusedLayout(lmax, smax, GL, int(sizeof(V)), kStash);
+  unsigned char* const wave_lds = smem_raw + size_t(wib) * size_t(lay.bytes);
+  V2<V>* const acc = reinterpret_cast<V2<V>*>(wave_lds + lay.acc);
+  V* const rr = reinterpret_cast<V*>(wave_lds + lay.rr);
+  V2<V>* const yzp = reinterpret_cast<V2<V>*>(wave_lds + lay.yzp);
+  int2* const st_ac = reinterpret_cast<int2*>(wave_lds + lay.stage);
+  V2<V>* const st_y = reinterpret_cast<V2<V>*>(wave_lds + lay.stage_y);
+  FusedTerm<V>* const stash = reinterpret_cast<FusedTerm<V>*>(wave_lds + lay.stash);
   double red[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   const int ntiles = (n + W - 1) / W;
   const int wstride = gridDim.x * (kBlk / kWave);
@@ -453,20 +511,17 @@ __global__ __launch_bounds__(kBlk) void FusedPass(
     const int pa = __builtin_amdgcn_readfirstlane(sp[j0]), pe = __builtin_amdgcn_readfirstlane(sp[je]);
     // operands of the updates, issued before the product so their latency
     // overlaps it: R and Pr(i) on the lane's S2 entries, M, Z, P, Pr(i) and
-    // the S2 position on its S entries
+    // the tile's S2 position of the row on its S entries
     V rq[RQ], pq[RQ];
-    int iq[RQ], nq[RQ];
 #pragma unroll
     for (int r = 0; r < RQ; ++r) {
       const int q = qa + lane + r * kWave;
       const bool in = q < qe;
       rq[r] = in && kLomrIter ? R[q] : V(0);
       pq[r] = in ? prS2[q] : V(0);
-      iq[r] = in && !kLomrIter ? s2i[q] : -1;
-      nq[r] = in ? int(qinfo[q]) : 0;
     }
     V mp[RS], zp[RS], ppv[RS], ps[RS];
-    uint32_t np[RS];
+    int tp[RS];
 #pragma unroll
     for (int r = 0; r < RS; ++r) {
       const int p = pa + lane + r * kWave;
@@ -475,66 +530,68 @@ __global__ __launch_bounds__(kBlk) void FusedPass(
       zp[r] = in && !kInit ? Z[p] : V(0);
       ppv[r] = in && kLomrIter ? P[p] : V(0);
       ps[r] = in ? prS[p] : V(0);
-      np[r] = in ? sinfo[p] : 0u;
+      tp[r] = in ? int(stq[p]) : 0;
     }
-    const int L = valid ? s2p[j + 1] - s2p[j] : 0;
-    V* const acc0 = mine;
-    V* const acc1 = mine + lmax;
-    V* const yz = mine + 3 * lmax;
-    V* const yp = yz + smax;
+    const int cb2 = valid ? s2p[j] - qa : 0;  // column j's S2 range in acc / rr
+    const int cb = valid ? sp[j] - pa : 0;    // column j's S range in yzp
+    const int ncs = valid ? sp[j + 1] - sp[j] : 0;
+    const int dj = (!kLomrIter && valid && gl == 0) ? s2d[j] : 0;
     // product 1: A [Z P](:, j)  (kInit: A M(:, j); MR: A (M + alpha Z)(:, j))
-    for (int t = gl; t < L; t += GL) {
-      acc0[t] = V(0);
-      if (NY > 1) acc1[t] = V(0);
-    }
-    if (valid) {
-      if constexpr (kInit) {
-        SlotAccumulate<V, SlotT, GL, 1>(j, gl, sp, a0c, av, plan_ptr, plan, acc0, acc1, st_ac, st_y,
-                                        [&](int e, V& ya, V& yb) { ya = M[e]; });
-      } else if constexpr (kMr) {
-        SlotAccumulate<V, SlotT, GL, 1>(j, gl, sp, a0c, av, plan_ptr, plan, acc0, acc1, st_ac, st_y,
-                                        [&](int e, V& ya, V& yb) { ya = M[e] + alpha * Z[e]; });
-      } else {
-        SlotAccumulate<V, SlotT, GL, 2>(j, gl, sp, a0c, av, plan_ptr, plan, acc0, acc1, st_ac, st_y,
-                                        [&](int e, V& ya, V& yb) {
-                                          ya = Z[e];
-                                          yb = P[e];
-                                        });
-      }
+    for (int t = lane; t < qe - qa; t += kWave) {
+      if constexpr (NY > 1)
+        acc[t] = V2<V>{V(0), V(0)};
+      else
+        acc[t].a = V(0);
     }
     __builtin_amdgcn_wave_barrier();
-    // R' = R - delta A Z - gamma A P over the tile's S2 range (kInit: I - A M)
-    auto update_r = [&](int q, V rold, int i, int info) {
-      const int gg = info >> kQInfoShift, t = info & kQInfoMask;
-      V* const grp = wave_lds + size_t(gg) * words;
+    if (valid) {
+      if constexpr (kInit) {
+        FusedProduct1<V, SlotT, GL, 1, kStash>(j, g, gl, sp, a0c, av, plan_ptr, plan, acc + cb2, st_ac, st_y, stash,
+                                               [&](int e, V& ya, V& yb) { ya = M[e]; });
+      } else if constexpr (kMr) {
+        FusedProduct1<V, SlotT, GL, 1, kStash>(j, g, gl, sp, a0c, av, plan_ptr, plan, acc + cb2, st_ac, st_y, stash,
+                                               [&](int e, V& ya, V& yb) { ya = M[e] + alpha * Z[e]; });
+      } else {
+        FusedProduct1<V, SlotT, GL, 2, kStash>(j, g, gl, sp, a0c, av, plan_ptr, plan, acc + cb2, st_ac, st_y, stash,
+                                               [&](int e, V& ya, V& yb) {
+                                                 ya = Z[e];
+                                                 yb = P[e];
+                                               });
+      }
+      // R = I - A M: the diagonal's sum becomes (A M)(j, j) - 1, so that
+      // R = -sum below rounds as 1 - (A M)(j, j)
+      if (!kLomrIter && gl == 0) acc[cb2 + dj].a -= V(1);
+    }
+    __builtin_amdgcn_wave_barrier();
+    // R' = R - delta A Z - gamma A P over the tile's S2 range (kInit, MR: I - A M)
+    auto update_r = [&](int q, V rold) {
+      const int t = q - qa;
       V rv;
       if constexpr (!kLomrIter) {
-        rv = -grp[t];
-        if (i == j0 + gg) rv += V(1);
+        rv = -acc[t].a;
       } else {
+        const V2<V> s = acc[t];
         rv = rold;
-        rv = rv + nd * grp[t];
-        rv = rv + ng * grp[lmax + t];
+        rv = rv + nd * s.a;
+        rv = rv + ng * s.b;
       }
       R[q] = rv;
       red[5] += double(rv) * double(rv);
-      grp[2 * lmax + t] = rv;
+      rr[t] = rv;
     };
 #pragma unroll
     for (int r = 0; r < RQ; ++r) {
       const int q = qa + lane + r * kWave;
-      if (q < qe) update_r(q, rq[r], iq[r], nq[r]);
+      if (q < qe) update_r(q, rq[r]);
     }
     for (int r = RQ; qa + r * kWave < qe; ++r) {
       const int q = qa + lane + r * kWave;
-      if (q < qe) update_r(q, kLomrIter ? R[q] : V(0), kLomrIter ? -1 : s2i[q], int(qinfo[q]));
+      if (q < qe) update_r(q, kLomrIter ? R[q] : V(0));
     }
     __builtin_amdgcn_wave_barrier();
     // M' = M + delta Z + gamma P, P' = beta P + Z, Z' = Pr R' over the S range
-    auto update_s = [&](int p, V mo, V zo, V po, V pri, uint32_t info) {
-      const int gg = int(info >> 26), t = int((info >> 16) & 1023u), pl = int(info & 0xFFFFu);
-      V* const grp = wave_lds + size_t(gg) * words;
-      const V zn = pri * grp[2 * lmax + t];
+    auto update_s = [&](int p, V mo, V zo, V po, V pri, int tq) {
+      const V zn = pri * rr[tq];
       V pn = V(0);
       if constexpr (kLomrIter) {
         V m = mo;
@@ -547,53 +604,60 @@ __global__ __launch_bounds__(kBlk) void FusedPass(
       }
       Z[p] = zn;
       if constexpr (!kMr) P[p] = pn;
-      grp[3 * lmax + pl] = zn;
-      if constexpr (!kMr) grp[3 * lmax + smax + pl] = pn;
+      yzp[p - pa] = V2<V>{zn, pn};
     };
 #pragma unroll
     for (int r = 0; r < RS; ++r) {
       const int p = pa + lane + r * kWave;
-      if (p < pe) update_s(p, mp[r], zp[r], ppv[r], ps[r], np[r]);
+      if (p < pe) update_s(p, mp[r], zp[r], ppv[r], ps[r], tp[r]);
     }
     for (int r = RS; pa + r * kWave < pe; ++r) {
       const int p = pa + lane + r * kWave;
       if (p < pe) {
         if constexpr (kInit)
-          update_s(p, V(0), V(0), V(0), prS[p], sinfo[p]);
+          update_s(p, V(0), V(0), V(0), prS[p], int(stq[p]));
         else if constexpr (kMr)
-          update_s(p, M[p], Z[p], V(0), prS[p], sinfo[p]);
+          update_s(p, M[p], Z[p], V(0), prS[p], int(stq[p]));
         else
-          update_s(p, M[p], Z[p], P[p], prS[p], sinfo[p]);
+          update_s(p, M[p], Z[p], P[p], prS[p], int(stq[p]));
       }
     }
     __builtin_amdgcn_wave_barrier();
     // product 2: A [Z' P'](:, j) from LDS (MR: A Z'(:, j))
     constexpr int NY2 = kMr ? 1 : 2;
-    for (int t = gl; t < L; t += GL) {
-      acc0[t] = V(0);
-      if (NY2 > 1) acc1[t] = V(0);
+    for (int t = lane; t < qe - qa; t += kWave) {
+      if constexpr (NY2 > 1)
+        acc[t] = V2<V>{V(0), V(0)};
+      else
+        acc[t].a = V(0);
     }
+    __builtin_amdgcn_wave_barrier();
     if (valid) {
-      const int sj = sp[j];
-      SlotAccumulate<V, SlotT, GL, NY2>(j, gl, sp, a0c, av, plan_ptr, plan, acc0, acc1, st_ac, st_y,
-                                        [&](int e, V& ya, V& yb) {
-                                          ya = yz[e - sj];
-                                          if (NY2 > 1) yb = yp[e - sj];
-                                        });
+      if constexpr (kStash) {
+        FusedProduct2Stash<V, NY2>(ncs, lane, stash, yzp + cb, acc + cb2);
+      } else {
+        const V2<V>* const ycol = yzp + cb - sp[j];
+        FusedProduct1<V, SlotT, GL, NY2, false>(j, g, gl, sp, a0c, av, plan_ptr, plan, acc + cb2, st_ac, st_y,
+                                                stash, [&](int e, V& ya, V& yb) {
+                                                  const V2<V> y = ycol[e];
+                                                  ya = y.a;
+                                                  yb = y.b;
+                                                });
+      }
     }
     __builtin_amdgcn_wave_barrier();
     // traces of the next step (LomrProjectEpi order)
-    auto traces = [&](int q, V pri, int info) {
-      const int gg = info >> kQInfoShift, t = info & kQInfoMask;
-      const V* grp = wave_lds + size_t(gg) * words;
-      const V rv = grp[2 * lmax + t];
+    auto traces = [&](int q, V pri) {
+      const int t = q - qa;
+      const V2<V> s = acc[t];
+      const V rv = rr[t];
       if constexpr (kMr) {
         // MrProjectEpi order: <Pr R, Pr A Z>, ||Pr A Z||^2
-        const double tmp = double(pri * grp[t]);
+        const double tmp = double(pri * s.a);
         red[0] += double(pri * rv) * tmp;
         red[1] += tmp * tmp;
       } else {
-        const double daz = grp[t], dap = grp[lmax + t], dpr = pri, dprr = double(pri * rv);
+        const double daz = s.a, dap = s.b, dpr = pri, dprr = double(pri * rv);
         red[0] += daz * (dpr * daz);
         red[1] += dap * (dpr * dap);
         red[2] += daz * (dpr * dap);
@@ -604,11 +668,11 @@ __global__ __launch_bounds__(kBlk) void FusedPass(
 #pragma unroll
     for (int r = 0; r < RQ; ++r) {
       const int q = qa + lane + r * kWave;
-      if (q < qe) traces(q, pq[r], nq[r]);
+      if (q < qe) traces(q, pq[r]);
     }
     for (int r = RQ; qa + r * kWave < qe; ++r) {
       const int q = qa + lane + r * kWave;
-      if (q < qe) traces(q, prS2[q], int(qinfo[q]));
+      if (q < qe) traces(q, prS2[q]);
     }
     __builtin_amdgcn_wave_barrier();
   }
@@ -1320,17 +1384,19 @@ __global__ void GatherRows(int64_t nnz, const int* __restrict__ rows, const V* _
   if (e < nnz) out[e] = pr[rows[e]];
 }
 
+// Tile tables of the fused pass (W = 64 / GL columns per tile): stq[p], for
+// every S entry p of column j, is the position of p's row in S2(:, j) counted
+// from the first S2 entry of j's tile; s2d[j] is the position of the diagonal
+// in S2(:, j) (S2 holds I).
 __global__ void TileInfo(int64_t nnz2, int W, const int* __restrict__ s2_to_s, const int* __restrict__ s2col,
-                         const int* __restrict__ s2p, const int* __restrict__ sp, uint16_t* __restrict__ qinfo,
-                         uint32_t* __restrict__ sinfo) {
+                         const int* __restrict__ s2i, const int* __restrict__ s2p, uint16_t* __restrict__ stq,
+                         int* __restrict__ s2d) {
   const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (q >= nnz2) return;
   const int j = s2col[q];
-  const int t = int(q - s2p[j]);
-  const int g = j % W;
-  qinfo[q] = uint16_t((g << kQInfoShift) | t);
   const int p = s2_to_s[q];
-  if (p >= 0) sinfo[p] = (uint32_t(g) << 26) | (uint32_t(t) << 16) | uint32_t(p - sp[j]);
+  if (p >= 0) stq[p] = uint16_t(q - s2p[j - j % W]);
+  if (s2i[q] == j) s2d[j] = int(q - s2p[j]);
 }
 
 unsigned GridFor(int64_t n) {
@@ -1460,13 +1526,31 @@ class FixedImpl final : public FixedProblem {
         gridp_ = (unsigned)std::min<int64_t>(std::max<int64_t>((n_ + groups - 1) / groups, 1), kMaxGrid);
         // fused LOMR pass: W = 64 / GL columns per wavefront
         smax_ = std::max(h[2], 1);
-        smemf_ = size_t(kBlk / gl_) * size_t(FusedGroupWords(lmax_, smax_, gl_, int(sizeof(V)))) * sizeof(V);
-        // LDS under 64 KiB with the kernel's static part; tile tables need
-        // S2 columns of at most 1024 entries and S columns of at most 65536
-        fused_ok_ = smemf_ <= 60 * 1024 && lmax_ <= 1024 && smax_ <= 65536 && total < (int64_t(1) << 32);
+        // LDS under 64 KiB with the kernel's static part; the tile tables
+        // need tiles of at most 65535 S2 entries.  The stash (product 2
+        // reuses product 1's terms) needs every column of A within GL and
+        // is taken while it leaves room for 4 blocks per CU (160 KiB).
+        const int vsize = int(sizeof(V));
+        const size_t waves = kBlk / kWave;
+        const size_t with = waves * size_t(MakeFusedLayout(lmax_, smax_, gl_, vsize, true).bytes);
+        const size_t without = waves * size_t(MakeFusedLayout(lmax_, smax_, gl_, vsize, false).bytes);
+        stash_ = maxcol_ <= gl_ && with <= 40 * 1024;
+        smemf_ = stash_ ? with : without;
+        fused_ok_ = smemf_ <= 60 * 1024 && int64_t(kWave / gl_) * lmax_ <= 65535 && total < (int64_t(1) << 32);
+        // the pass walks its tiles with a grid that the device holds
+        // resident at once (blocks per CU of the iteration kernel x CUs)
         const int64_t ntiles = (n_ + (kWave / gl_) - 1) / (kWave / gl_);
+        int64_t resident = kMaxFusedGrid;
+        if (fused_ok_) {
+          int dev = 0, cus = 0;
+          SPAI_HIP_CHECK(hipGetDevice(&dev));
+          SPAI_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+          const int per_cu = FusedBlocksPerCu(method_ == SPAI_GLOBAL_MR);
+          if (per_cu < 1) throw std::runtime_error("fused pass: no block fits on a CU");
+          resident = std::min<int64_t>(int64_t(per_cu) * cus, kMaxFusedGrid);
+        }
         gridf_ = (unsigned)std::min<int64_t>(std::max<int64_t>((ntiles + kBlk / kWave - 1) / (kBlk / kWave), 1),
-                                             kMaxFusedGrid);
+                                             resident);
       }
       av_ = c.Alloc<V>(A_.nnz());
       ConvertValues<V><<<Blocks1(A_.nnz()), kBlk, 0, c.stream>>>(A_.nnz(), A_.v(), av_.get());
@@ -1496,11 +1580,11 @@ class FixedImpl final : public FixedProblem {
     if (fused_ok_ && (method_ == SPAI_GLOBAL_LOMR || method_ == SPAI_GLOBAL_MR)) {
       a0c_ = c.Alloc<int2>(nnz_s_);
       SEntryA<<<Blocks1(nnz_s_), kBlk, 0, c.stream>>>(nnz_s_, s_pat_->idx.get(), A_.ptr(), a0c_.get());
-      qinfo_ = c.Alloc<uint16_t>(nnz_s2_);
-      sinfo_ = c.Alloc<uint32_t>(nnz_s_);
+      stq_ = c.Alloc<uint16_t>(nnz_s_);
+      s2d_ = c.Alloc<int>(n_);
       TileInfo<<<Blocks1(nnz_s2_), kBlk, 0, c.stream>>>(nnz_s2_, kWave / gl_, s2_to_s_.get(), s2col_.get(),
-                                                       s2_pat_->ptr.get(), s_pat_->ptr.get(), qinfo_.get(),
-                                                       sinfo_.get());
+                                                       s2_pat_->idx.get(), s2_pat_->ptr.get(), stq_.get(),
+                                                       s2d_.get());
       prS_ = c.Alloc<V>(nnz_s_);
       GatherRows<V><<<Blocks1(nnz_s_), kBlk, 0, c.stream>>>(nnz_s_, s_pat_->idx.get(), prv_.get(), prS_.get());
       prS2_ = c.Alloc<V>(nnz_s2_);
@@ -1648,11 +1732,18 @@ class FixedImpl final : public FixedProblem {
         MrFusedStep<<<1, 1024, 0, st>>>(part, (int)gridf_, (int)it, tol, hist_.get(), sc_.get(), flags_.get());
       else if (it > 0)
         LomrFusedStep<<<1, 1024, 0, st>>>(part, (int)gridf_, (int)it, tol, hist_.get(), sc_.get(), flags_.get());
-#define SPAI_FUSED_ONE(SlotT, plan, GLV, INIT, MR)                                                             \
-  FusedPass<V, SlotT, GLV, INIT, MR><<<gridf_, kBlk, smemf_, st>>>(                                              \
-      (int)n_, s_pat_->ptr.get(), a0c_.get(), qinfo_.get(), sinfo_.get(), s2_pat_->ptr.get(), s2_pat_->idx.get(), \
-      av_.get(), plan_ptr_.get(), plan, lmax_, smax_, prS_.get(), prS2_.get(), R_.get(), M_.get(), Z_.get(),     \
-      P_.get(), sc_.get(), part, flags_.get())
+#define SPAI_FUSED_STASH(SlotT, plan, GLV, INIT, MR, STASH)                                                   \
+  FusedPass<V, SlotT, GLV, INIT, MR, STASH><<<gridf_, kBlk, smemf_, st>>>(                                     \
+      (int)n_, s_pat_->ptr.get(), a0c_.get(), stq_.get(), s2d_.get(), s2_pat_->ptr.get(), av_.get(),           \
+      plan_ptr_.get(), plan, lmax_, smax_, prS_.get(), prS2_.get(), R_.get(), M_.get(), Z_.get(), P_.get(),    \
+      sc_.get(), part, flags_.get())
+#define SPAI_FUSED_ONE(SlotT, plan, GLV, INIT, MR)          \
+  do {                                                      \
+    if (stash_)                                             \
+      SPAI_FUSED_STASH(SlotT, plan, GLV, INIT, MR, true);   \
+    else                                                    \
+      SPAI_FUSED_STASH(SlotT, plan, GLV, INIT, MR, false);  \
+  } while (0)
 #define SPAI_FUSED_LAUNCH(SlotT, plan, GLV)                     \
   do {                                                          \
     if (mr) {                                                   \
@@ -1685,6 +1776,7 @@ class FixedImpl final : public FixedProblem {
 #undef SPAI_FUSED_GL
 #undef SPAI_FUSED_LAUNCH
 #undef SPAI_FUSED_ONE
+#undef SPAI_FUSED_STASH
     }
     LomrFusedFinish<<<1, kBlk, 0, st>>>(part, (int)gridf_, (int)prm_.max_iterations, tol, hist_.get(),
                                         flags_.get());
@@ -1710,6 +1802,38 @@ class FixedImpl final : public FixedProblem {
       timed_us_ = f.iterations > 0 ? total * 1e3 / double(f.iterations) : 0.0;
     }
     return f.iterations;
+  }
+
+  // Blocks of the fused iteration kernel (the configuration the build
+  // launches: slot width, GL, stash, method) resident per CU at once.
+  int FusedBlocksPerCu(bool mr) const {
+    int nb = 0;
+#define SPAI_OCC(SlotT, GLV)                                                                                     \
+  do {                                                                                                           \
+    const void* f = nullptr;                                                                                     \
+    if (mr)                                                                                                      \
+      f = stash_ ? reinterpret_cast<const void*>(&FusedPass<V, SlotT, GLV, false, true, true>)                  \
+                 : reinterpret_cast<const void*>(&FusedPass<V, SlotT, GLV, false, true, false>);                \
+    else                                                                                                         \
+      f = stash_ ? reinterpret_cast<const void*>(&FusedPass<V, SlotT, GLV, false, false, true>)                 \
+                 : reinterpret_cast<const void*>(&FusedPass<V, SlotT, GLV, false, false, false>);               \
+    SPAI_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kBlk, smemf_));                         \
+  } while (0)
+#define SPAI_OCC_GL(SlotT)                  \
+  switch (gl_) {                            \
+    case 8: SPAI_OCC(SlotT, 8); break;      \
+    case 16: SPAI_OCC(SlotT, 16); break;    \
+    case 32: SPAI_OCC(SlotT, 32); break;    \
+    default: SPAI_OCC(SlotT, 64); break;    \
+  }
+    if (slot16_) {
+      SPAI_OCC_GL(uint16_t);
+    } else {
+      SPAI_OCC_GL(uint8_t);
+    }
+#undef SPAI_OCC_GL
+#undef SPAI_OCC
+    return nb;
   }
 
   int64_t Build() override {
@@ -1868,8 +1992,9 @@ class FixedImpl final : public FixedProblem {
   unsigned gridf_ = 1;
   DBuf<double> partf_;
   DBuf<int2> a0c_;
-  DBuf<uint16_t> qinfo_;
-  DBuf<uint32_t> sinfo_;
+  bool stash_ = false;
+  DBuf<uint16_t> stq_;
+  DBuf<int> s2d_;
   DBuf<V> prS_, prS2_;
   size_t smem_ = 0;
   bool slot16_ = false, use_plan_ = false;
