@@ -358,34 +358,40 @@ __device__ __forceinline__ void AddTerm(V2<V>* acc, int slot, V a, V2<V> y) {
   }
 }
 
+// Stages S entries [eb, eb + GL) of a column for FusedProduct1: lane gl
+// writes entry eb + gl's (A(:, k) start, length) and Y values.
+template <class V, int GL, class YGet>
+__device__ __forceinline__ void FusedStage(int g, int gl, int eb, int e1, const int2* __restrict__ a0c, int2* st_ac,
+                                           V2<V>* st_y, YGet yget) {
+  constexpr int W = kWave / GL;
+  if (eb + gl < e1) {
+    V ya = V(0), yb = V(0);
+    yget(eb + gl, ya, yb);
+    st_ac[gl * W + g] = a0c[eb + gl];
+    st_y[gl * W + g] = V2<V>{ya, yb};
+  }
+}
+
 // Column-wise Gustavson accumulation of A [y0 y1](:, j) into acc (column j's
 // S2 range), in the order of SlotProduct (S entries of the column in order, k
 // ascending).  YGet(e, ya, yb) returns S entry e's values.  The group stages
-// up to GL S entries -- (A(:, k) start, length) and (ya, yb) -- in LDS, then
-// every step (one S entry, column k of A) reads them with one broadcast LDS
-// load; when |A(:, k)| <= GL lane gl takes the term u = gl, and the plan slots
-// and A values of up to kPre steps are loaded before any of them is added, so
-// a column waits for one load latency per kPre steps.  kStash: every lane's
-// (A value, slot) of every step goes to stash[step][lane] (no column of A is
-// longer than GL then).
+// up to GL S entries -- (A(:, k) start, length) and (ya, yb) -- in LDS (the
+// caller stages the first GL), then every step (one S entry, column k of A)
+// reads them with one broadcast LDS load; when |A(:, k)| <= GL lane gl takes
+// the term u = gl, and the plan slots and A values of up to kPre steps are
+// loaded before any of them is added, so a column waits for one load latency
+// per kPre steps.  The column is S entries [e0, e1) with plan terms from pp
+// on.  kStash: every lane's (A value, slot) of every step goes to
+// stash[step][lane] (no column of A is longer than GL then).
 template <class V, class SlotT, int GL, int NY, bool kStash, class YGet>
-__device__ __forceinline__ void FusedProduct1(int j, int g, int gl, const int* __restrict__ sp,
+__device__ __forceinline__ void FusedProduct1(int g, int gl, int e0, int e1, uint32_t pp,
                                               const int2* __restrict__ a0c, const V* __restrict__ av,
-                                              const int64_t* __restrict__ plan_ptr, const SlotT* __restrict__ plan,
-                                              V2<V>* acc, int2* st_ac, V2<V>* st_y, FusedTerm<V>* stash,
-                                              YGet yget) {
+                                              const SlotT* __restrict__ plan, V2<V>* acc, int2* st_ac,
+                                              V2<V>* st_y, FusedTerm<V>* stash, YGet yget) {
   constexpr int W = kWave / GL;
   constexpr int kPre = 8;
   const int lane = g * GL + gl;
-  uint32_t pp = uint32_t(plan_ptr[j]);
-  const int e0 = sp[j], e1 = sp[j + 1];
   for (int eb = e0; eb < e1; eb += GL) {
-    if (eb + gl < e1) {
-      V ya = V(0), yb = V(0);
-      yget(eb + gl, ya, yb);
-      st_ac[gl * W + g] = a0c[eb + gl];
-      st_y[gl * W + g] = V2<V>{ya, yb};
-    }
     __builtin_amdgcn_wave_barrier();
     const int ns = min(GL, e1 - eb);
     for (int s0 = 0; s0 < ns; s0 += kPre) {
@@ -431,6 +437,7 @@ __device__ __forceinline__ void FusedProduct1(int j, int g, int gl, const int* _
       pp = off;
     }
     __builtin_amdgcn_wave_barrier();
+    if (eb + GL < e1) FusedStage<V, GL>(g, gl, eb + GL, e1, a0c, st_ac, st_y, yget);
   }
 }
 
@@ -489,7 +496,7 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
   }
   if constexpr (kMr && !kInit) alpha = V(sc[kAlpha]);
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int lane = LaneId(), wib = threadIdx.x / kWave;
+  const int lane = LaneId(), wib = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int g = lane / GL, gl = lane % GL;
   const FusedLayout lay = MakeFusedLayout(lmax, smax, GL, int(sizeof(V)), kStash);
   unsigned char* const wave_lds = smem_raw + size_t(wib) * size_t(lay.bytes);
@@ -502,13 +509,64 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
   double red[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   const int ntiles = (n + W - 1) / W;
   const int wstride = gridDim.x * (kBlk / kWave);
-  for (int tile = blockIdx.x * (kBlk / kWave) + wib; tile < ntiles; tile += wstride) {
+  // Product 1's Y values of S entry e
+  auto yget1 = [&](int e, V& ya, V& yb) {
+    if constexpr (kInit) {
+      ya = M[e];
+    } else if constexpr (kMr) {
+      ya = M[e] + alpha * Z[e];
+    } else {
+      ya = Z[e];
+      yb = P[e];
+    }
+  };
+  // What a tile needs before its loads can start -- the tile's S2 and S
+  // bounds and, per group, the column's S range, first plan term, first S2
+  // entry and diagonal -- and the lane's operands of product 1's first step
+  // chunk are loaded one tile ahead, so they arrive during the previous
+  // tile's updates and product 2.
+  int nqa = 0, nqe = 0, npa = 0, npe = 0, ns2j = 0, ne0 = 0, ne1 = 0, ndj = 0;
+  uint32_t npp = 0;
+  auto load_tile = [&](int t) {
+    const int j0 = t * W, jj = j0 + g, je = min(j0 + W, n);
+    nqa = s2p[j0];
+    nqe = s2p[je];
+    npa = sp[j0];
+    npe = sp[je];
+    const bool ok = jj < n;
+    ns2j = ok ? s2p[jj] : 0;
+    ne0 = ok ? sp[jj] : 0;
+    ne1 = ok ? sp[jj + 1] : 0;
+    npp = ok ? uint32_t(plan_ptr[jj]) : 0u;
+    ndj = (!kLomrIter && ok && gl == 0) ? s2d[jj] : 0;
+  };
+  int2 nac = make_int2(0, 0);
+  V2<V> ny = V2<V>{V(0), V(0)};
+  auto load_stage = [&]() {
+    nac = make_int2(0, 0);
+    ny = V2<V>{V(0), V(0)};
+    if (ne0 + gl < ne1) {
+      nac = a0c[ne0 + gl];
+      yget1(ne0 + gl, ny.a, ny.b);
+    }
+  };
+  int tile = blockIdx.x * (kBlk / kWave) + wib;
+  if (tile < ntiles) {
+    load_tile(tile);
+    load_stage();
+  }
+  for (; tile < ntiles; tile += wstride) {
     const int j0 = tile * W;
     const int j = j0 + g;
     const bool valid = j < n;
-    const int je = min(j0 + W, n);
-    const int qa = __builtin_amdgcn_readfirstlane(s2p[j0]), qe = __builtin_amdgcn_readfirstlane(s2p[je]);
-    const int pa = __builtin_amdgcn_readfirstlane(sp[j0]), pe = __builtin_amdgcn_readfirstlane(sp[je]);
+    const int qa = __builtin_amdgcn_readfirstlane(nqa), qe = __builtin_amdgcn_readfirstlane(nqe);
+    const int pa = __builtin_amdgcn_readfirstlane(npa), pe = __builtin_amdgcn_readfirstlane(npe);
+    const int cb2 = ns2j - qa;  // column j's S2 range in acc / rr
+    const int cb = ne0 - pa;    // column j's S range in yzp
+    const int e0 = ne0, e1 = ne1, dj = ndj;
+    const uint32_t pp = npp;
+    const int2 pre_ac = nac;
+    const V2<V> pre_y = ny;
     // operands of the updates, issued before the product so their latency
     // overlaps it: R and Pr(i) on the lane's S2 entries, M, Z, P, Pr(i) and
     // the tile's S2 position of the row on its S entries
@@ -532,10 +590,7 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
       ps[r] = in ? prS[p] : V(0);
       tp[r] = in ? int(stq[p]) : 0;
     }
-    const int cb2 = valid ? s2p[j] - qa : 0;  // column j's S2 range in acc / rr
-    const int cb = valid ? sp[j] - pa : 0;    // column j's S range in yzp
-    const int ncs = valid ? sp[j + 1] - sp[j] : 0;
-    const int dj = (!kLomrIter && valid && gl == 0) ? s2d[j] : 0;
+    const int ncs = e1 - e0;
     // product 1: A [Z P](:, j)  (kInit: A M(:, j); MR: A (M + alpha Z)(:, j))
     for (int t = lane; t < qe - qa; t += kWave) {
       if constexpr (NY > 1)
@@ -545,19 +600,12 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
     }
     __builtin_amdgcn_wave_barrier();
     if (valid) {
-      if constexpr (kInit) {
-        FusedProduct1<V, SlotT, GL, 1, kStash>(j, g, gl, sp, a0c, av, plan_ptr, plan, acc + cb2, st_ac, st_y, stash,
-                                               [&](int e, V& ya, V& yb) { ya = M[e]; });
-      } else if constexpr (kMr) {
-        FusedProduct1<V, SlotT, GL, 1, kStash>(j, g, gl, sp, a0c, av, plan_ptr, plan, acc + cb2, st_ac, st_y, stash,
-                                               [&](int e, V& ya, V& yb) { ya = M[e] + alpha * Z[e]; });
-      } else {
-        FusedProduct1<V, SlotT, GL, 2, kStash>(j, g, gl, sp, a0c, av, plan_ptr, plan, acc + cb2, st_ac, st_y, stash,
-                                               [&](int e, V& ya, V& yb) {
-                                                 ya = Z[e];
-                                                 yb = P[e];
-                                               });
+      if (e0 + gl < e1) {
+        st_ac[gl * W + g] = pre_ac;
+        st_y[gl * W + g] = pre_y;
       }
+      FusedProduct1<V, SlotT, GL, NY, kStash>(g, gl, e0, e1, pp, a0c, av, plan, acc + cb2, st_ac, st_y, stash,
+                                              yget1);
       // R = I - A M: the diagonal's sum becomes (A M)(j, j) - 1, so that
       // R = -sum below rounds as 1 - (A M)(j, j)
       if (!kLomrIter && gl == 0) acc[cb2 + dj].a -= V(1);
@@ -589,6 +637,8 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
       if (q < qe) update_r(q, kLomrIter ? R[q] : V(0));
     }
     __builtin_amdgcn_wave_barrier();
+    const int next = tile + wstride;
+    if (next < ntiles) load_tile(next);
     // M' = M + delta Z + gamma P, P' = beta P + Z, Z' = Pr R' over the S range
     auto update_s = [&](int p, V mo, V zo, V po, V pri, int tq) {
       const V zn = pri * rr[tq];
@@ -636,16 +686,19 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
       if constexpr (kStash) {
         FusedProduct2Stash<V, NY2>(ncs, lane, stash, yzp + cb, acc + cb2);
       } else {
-        const V2<V>* const ycol = yzp + cb - sp[j];
-        FusedProduct1<V, SlotT, GL, NY2, false>(j, g, gl, sp, a0c, av, plan_ptr, plan, acc + cb2, st_ac, st_y,
-                                                stash, [&](int e, V& ya, V& yb) {
-                                                  const V2<V> y = ycol[e];
-                                                  ya = y.a;
-                                                  yb = y.b;
-                                                });
+        const V2<V>* const ycol = yzp + cb - e0;
+        auto yget2 = [&](int e, V& ya, V& yb) {
+          const V2<V> y = ycol[e];
+          ya = y.a;
+          yb = y.b;
+        };
+        FusedStage<V, GL>(g, gl, e0, e1, a0c, st_ac, st_y, yget2);
+        FusedProduct1<V, SlotT, GL, NY2, false>(g, gl, e0, e1, pp, a0c, av, plan, acc + cb2, st_ac, st_y, stash,
+                                                yget2);
       }
     }
     __builtin_amdgcn_wave_barrier();
+    if (next < ntiles) load_stage();
     // traces of the next step (LomrProjectEpi order)
     auto traces = [&](int q, V pri) {
       const int t = q - qa;
