@@ -321,8 +321,11 @@ struct FusedLayout {
 
 __host__ __device__ constexpr FusedLayout MakeFusedLayout(int lmax, int smax, int gl, int vsize, bool stash) {
   const int w = kWave / gl;
-  const int rr = Align16(2 * w * lmax * vsize);
-  const int yzp = rr + Align16(w * lmax * vsize);
+  // S2 positions count from the tile's first entry rounded down to 16 bytes,
+  // and the update loops cover whole 16-byte chunks: 2 chunks of slack
+  const int lq = w * lmax + 2 * (16 / vsize);
+  const int rr = Align16(2 * lq * vsize);
+  const int yzp = rr + Align16(lq * vsize);
   const int free_end = yzp + Align16(2 * w * smax * vsize);
   const int stage = stash ? rr : free_end;
   const int stage_y = stage + Align16(kWave * 8);
@@ -335,6 +338,17 @@ __host__ __device__ constexpr FusedLayout MakeFusedLayout(int lmax, int smax, in
 template <class V>
 struct alignas(2 * sizeof(V)) V2 {
   V a, b;
+};
+
+// 16 bytes of consecutive S2 entries (values, or pairs of sums): the unit of
+// the update and trace loops, one 16-byte access per array.
+template <class V>
+struct alignas(16) VecQ {
+  V v[16 / sizeof(V)];
+};
+template <class V>
+struct alignas(16) VecQ2 {
+  V2<V> v[16 / sizeof(V)];
 };
 
 // One term of product 1 as product 2 reuses it; slot -1: the lane had no term.
@@ -469,8 +483,9 @@ __device__ __forceinline__ void FusedProduct2Stash(int ns, int lane, const Fused
 // traces of A Z (the pass before iteration 1).  Otherwise one iteration with
 // the step sizes in sc: update, then the traces of the next iteration.
 // stq[p] (S entry p of column j): position of p's row in S2(:, j), counted
-// from the first S2 entry of j's tile.  s2d[j]: position of the diagonal in
-// S2(:, j).
+// from the first S2 entry of j's tile rounded down to a multiple of 16 bytes
+// (kVec entries).  s2d[j]: position of the diagonal in S2(:, j).  R and prS2
+// hold kVec - 1 entries of padding past the end, so whole chunks load.
 template <class V, class SlotT, int GL, bool kInit, bool kMr, bool kStash>
 __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
     int n, const int* __restrict__ sp, const int2* __restrict__ a0c, const uint16_t* __restrict__ stq,
@@ -480,8 +495,9 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
     V* __restrict__ P, const double* __restrict__ sc, double* __restrict__ part_out,
     const Flags* __restrict__ flags) {
   constexpr int W = kWave / GL;
-  constexpr int RQ = 4;  // S2 entries of a tile held in registers per lane
-  constexpr int RS = 2;  // S entries of a tile held in registers per lane
+  constexpr int kVec = 16 / int(sizeof(V));  // S2 entries per chunk
+  constexpr int RQ = sizeof(V) == 4 ? 1 : 2;  // S2 chunks of a tile held in registers per lane
+  constexpr int RS = 2;                        // S entries of a tile held in registers per lane
   if (flags->stop) return;
   // LOMR: R -= delta AZ + gamma AP (R read); MR: R = I - A M recomputed
   constexpr bool kLomrIter = !kInit && !kMr;
@@ -561,22 +577,26 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
     const bool valid = j < n;
     const int qa = __builtin_amdgcn_readfirstlane(nqa), qe = __builtin_amdgcn_readfirstlane(nqe);
     const int pa = __builtin_amdgcn_readfirstlane(npa), pe = __builtin_amdgcn_readfirstlane(npe);
-    const int cb2 = ns2j - qa;  // column j's S2 range in acc / rr
-    const int cb = ne0 - pa;    // column j's S range in yzp
+    const int qal = qa & ~(kVec - 1);
+    const int cb2 = ns2j - qal;  // column j's S2 range in acc / rr
+    const int cb = ne0 - pa;     // column j's S range in yzp
     const int e0 = ne0, e1 = ne1, dj = ndj;
     const uint32_t pp = npp;
     const int2 pre_ac = nac;
     const V2<V> pre_y = ny;
     // operands of the updates, issued before the product so their latency
-    // overlaps it: R and Pr(i) on the lane's S2 entries, M, Z, P, Pr(i) and
+    // overlaps it: R and Pr(i) on the lane's S2 chunks, M, Z, P, Pr(i) and
     // the tile's S2 position of the row on its S entries
-    V rq[RQ], pq[RQ];
+    VecQ<V> rq[RQ], pq[RQ];
 #pragma unroll
     for (int r = 0; r < RQ; ++r) {
-      const int q = qa + lane + r * kWave;
-      const bool in = q < qe;
-      rq[r] = in && kLomrIter ? R[q] : V(0);
-      pq[r] = in ? prS2[q] : V(0);
+      const int qb = qal + (lane + r * kWave) * kVec;
+      rq[r] = VecQ<V>{};
+      pq[r] = VecQ<V>{};
+      if (qb < qe) {
+        if constexpr (kLomrIter) rq[r] = *reinterpret_cast<const VecQ<V>*>(R + qb);
+        pq[r] = *reinterpret_cast<const VecQ<V>*>(prS2 + qb);
+      }
     }
     V mp[RS], zp[RS], ppv[RS], ps[RS];
     int tp[RS];
@@ -592,7 +612,7 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
     }
     const int ncs = e1 - e0;
     // product 1: A [Z P](:, j)  (kInit: A M(:, j); MR: A (M + alpha Z)(:, j))
-    for (int t = lane; t < qe - qa; t += kWave) {
+    for (int t = lane; t < qe - qal; t += kWave) {
       if constexpr (NY > 1)
         acc[t] = V2<V>{V(0), V(0)};
       else
@@ -611,30 +631,43 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
       if (!kLomrIter && gl == 0) acc[cb2 + dj].a -= V(1);
     }
     __builtin_amdgcn_wave_barrier();
-    // R' = R - delta A Z - gamma A P over the tile's S2 range (kInit, MR: I - A M)
-    auto update_r = [&](int q, V rold) {
-      const int t = q - qa;
-      V rv;
-      if constexpr (!kLomrIter) {
-        rv = -acc[t].a;
-      } else {
-        const V2<V> s = acc[t];
-        rv = rold;
-        rv = rv + nd * s.a;
-        rv = rv + ng * s.b;
+    // R' = R - delta A Z - gamma A P over the tile's S2 range (kInit, MR:
+    // I - A M), one chunk of kVec entries per lane; entries outside [qa, qe)
+    // are computed in LDS slack and neither stored nor summed
+    auto update_r = [&](int qb, const VecQ<V>& rold) {
+      const int t0 = qb - qal;
+      const VecQ2<V> s = *reinterpret_cast<const VecQ2<V>*>(acc + t0);
+      VecQ<V> out;
+#pragma unroll
+      for (int v = 0; v < kVec; ++v) {
+        V rv;
+        if constexpr (!kLomrIter) {
+          rv = -s.v[v].a;
+        } else {
+          rv = rold.v[v];
+          rv = rv + nd * s.v[v].a;
+          rv = rv + ng * s.v[v].b;
+        }
+        out.v[v] = rv;
+        if (qb + v >= qa && qb + v < qe) red[5] += double(rv) * double(rv);
       }
-      R[q] = rv;
-      red[5] += double(rv) * double(rv);
-      rr[t] = rv;
+      *reinterpret_cast<VecQ<V>*>(rr + t0) = out;
+      if (qb >= qa && qb + kVec <= qe) {
+        *reinterpret_cast<VecQ<V>*>(R + qb) = out;
+      } else {
+#pragma unroll
+        for (int v = 0; v < kVec; ++v)
+          if (qb + v >= qa && qb + v < qe) R[qb + v] = out.v[v];
+      }
     };
 #pragma unroll
     for (int r = 0; r < RQ; ++r) {
-      const int q = qa + lane + r * kWave;
-      if (q < qe) update_r(q, rq[r]);
+      const int qb = qal + (lane + r * kWave) * kVec;
+      if (qb < qe) update_r(qb, rq[r]);
     }
-    for (int r = RQ; qa + r * kWave < qe; ++r) {
-      const int q = qa + lane + r * kWave;
-      if (q < qe) update_r(q, kLomrIter ? R[q] : V(0));
+    for (int r = RQ; qal + r * kWave * kVec < qe; ++r) {
+      const int qb = qal + (lane + r * kWave) * kVec;
+      if (qb < qe) update_r(qb, kLomrIter ? *reinterpret_cast<const VecQ<V>*>(R + qb) : VecQ<V>{});
     }
     __builtin_amdgcn_wave_barrier();
     const int next = tile + wstride;
@@ -675,7 +708,7 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
     __builtin_amdgcn_wave_barrier();
     // product 2: A [Z' P'](:, j) from LDS (MR: A Z'(:, j))
     constexpr int NY2 = kMr ? 1 : 2;
-    for (int t = lane; t < qe - qa; t += kWave) {
+    for (int t = lane; t < qe - qal; t += kWave) {
       if constexpr (NY2 > 1)
         acc[t] = V2<V>{V(0), V(0)};
       else
@@ -699,33 +732,37 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
     }
     __builtin_amdgcn_wave_barrier();
     if (next < ntiles) load_stage();
-    // traces of the next step (LomrProjectEpi order)
-    auto traces = [&](int q, V pri) {
-      const int t = q - qa;
-      const V2<V> s = acc[t];
-      const V rv = rr[t];
-      if constexpr (kMr) {
-        // MrProjectEpi order: <Pr R, Pr A Z>, ||Pr A Z||^2
-        const double tmp = double(pri * s.a);
-        red[0] += double(pri * rv) * tmp;
-        red[1] += tmp * tmp;
-      } else {
-        const double daz = s.a, dap = s.b, dpr = pri, dprr = double(pri * rv);
-        red[0] += daz * (dpr * daz);
-        red[1] += dap * (dpr * dap);
-        red[2] += daz * (dpr * dap);
-        red[3] += dprr * daz;
-        red[4] += dprr * dap;
+    // traces of the next step (LomrProjectEpi order), by chunks as above
+    auto traces = [&](int qb, const VecQ<V>& pri) {
+      const int t0 = qb - qal;
+      const VecQ2<V> s = *reinterpret_cast<const VecQ2<V>*>(acc + t0);
+      const VecQ<V> rv = *reinterpret_cast<const VecQ<V>*>(rr + t0);
+#pragma unroll
+      for (int v = 0; v < kVec; ++v) {
+        if (qb + v < qa || qb + v >= qe) continue;
+        if constexpr (kMr) {
+          // MrProjectEpi order: <Pr R, Pr A Z>, ||Pr A Z||^2
+          const double tmp = double(pri.v[v] * s.v[v].a);
+          red[0] += double(pri.v[v] * rv.v[v]) * tmp;
+          red[1] += tmp * tmp;
+        } else {
+          const double daz = s.v[v].a, dap = s.v[v].b, dpr = pri.v[v], dprr = double(pri.v[v] * rv.v[v]);
+          red[0] += daz * (dpr * daz);
+          red[1] += dap * (dpr * dap);
+          red[2] += daz * (dpr * dap);
+          red[3] += dprr * daz;
+          red[4] += dprr * dap;
+        }
       }
     };
 #pragma unroll
     for (int r = 0; r < RQ; ++r) {
-      const int q = qa + lane + r * kWave;
-      if (q < qe) traces(q, pq[r]);
+      const int qb = qal + (lane + r * kWave) * kVec;
+      if (qb < qe) traces(qb, pq[r]);
     }
-    for (int r = RQ; qa + r * kWave < qe; ++r) {
-      const int q = qa + lane + r * kWave;
-      if (q < qe) traces(q, prS2[q]);
+    for (int r = RQ; qal + r * kWave * kVec < qe; ++r) {
+      const int qb = qal + (lane + r * kWave) * kVec;
+      if (qb < qe) traces(qb, *reinterpret_cast<const VecQ<V>*>(prS2 + qb));
     }
     __builtin_amdgcn_wave_barrier();
   }
@@ -1439,16 +1476,16 @@ __global__ void GatherRows(int64_t nnz, const int* __restrict__ rows, const V* _
 
 // Tile tables of the fused pass (W = 64 / GL columns per tile): stq[p], for
 // every S entry p of column j, is the position of p's row in S2(:, j) counted
-// from the first S2 entry of j's tile; s2d[j] is the position of the diagonal
-// in S2(:, j) (S2 holds I).
-__global__ void TileInfo(int64_t nnz2, int W, const int* __restrict__ s2_to_s, const int* __restrict__ s2col,
+// from the first S2 entry of j's tile rounded down to a multiple of vec; s2d[j]
+// is the position of the diagonal in S2(:, j) (S2 holds I).
+__global__ void TileInfo(int64_t nnz2, int W, int vec, const int* __restrict__ s2_to_s, const int* __restrict__ s2col,
                          const int* __restrict__ s2i, const int* __restrict__ s2p, uint16_t* __restrict__ stq,
                          int* __restrict__ s2d) {
   const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (q >= nnz2) return;
   const int j = s2col[q];
   const int p = s2_to_s[q];
-  if (p >= 0) stq[p] = uint16_t(q - s2p[j - j % W]);
+  if (p >= 0) stq[p] = uint16_t(q - (s2p[j - j % W] & ~(vec - 1)));
   if (s2i[q] == j) s2d[j] = int(q - s2p[j]);
 }
 
@@ -1589,7 +1626,8 @@ class FixedImpl final : public FixedProblem {
         const size_t without = waves * size_t(MakeFusedLayout(lmax_, smax_, gl_, vsize, false).bytes);
         stash_ = maxcol_ <= gl_ && with <= 40 * 1024;
         smemf_ = stash_ ? with : without;
-        fused_ok_ = smemf_ <= 60 * 1024 && int64_t(kWave / gl_) * lmax_ <= 65535 && total < (int64_t(1) << 32);
+        fused_ok_ = smemf_ <= 60 * 1024 && int64_t(kWave / gl_) * lmax_ + 16 <= 65535 &&
+                    total < (int64_t(1) << 32);
         // the pass walks its tiles with a grid that the device holds
         // resident at once (blocks per CU of the iteration kernel x CUs)
         const int64_t ntiles = (n_ + (kWave / gl_) - 1) / (kWave / gl_);
@@ -1635,18 +1673,19 @@ class FixedImpl final : public FixedProblem {
       SEntryA<<<Blocks1(nnz_s_), kBlk, 0, c.stream>>>(nnz_s_, s_pat_->idx.get(), A_.ptr(), a0c_.get());
       stq_ = c.Alloc<uint16_t>(nnz_s_);
       s2d_ = c.Alloc<int>(n_);
-      TileInfo<<<Blocks1(nnz_s2_), kBlk, 0, c.stream>>>(nnz_s2_, kWave / gl_, s2_to_s_.get(), s2col_.get(),
+      TileInfo<<<Blocks1(nnz_s2_), kBlk, 0, c.stream>>>(nnz_s2_, kWave / gl_, 16 / int(sizeof(V)), s2_to_s_.get(),
+                                                       s2col_.get(),
                                                        s2_pat_->idx.get(), s2_pat_->ptr.get(), stq_.get(),
                                                        s2d_.get());
       prS_ = c.Alloc<V>(nnz_s_);
       GatherRows<V><<<Blocks1(nnz_s_), kBlk, 0, c.stream>>>(nnz_s_, s_pat_->idx.get(), prv_.get(), prS_.get());
-      prS2_ = c.Alloc<V>(nnz_s2_);
+      prS2_ = c.Alloc<V>(nnz_s2_ + 16);  // padding: the fused pass loads whole 16-byte chunks
       GatherRows<V><<<Blocks1(nnz_s2_), kBlk, 0, c.stream>>>(nnz_s2_, s2_pat_->idx.get(), prv_.get(), prS2_.get());
     }
     M_ = c.Alloc<V>(nnz_s_);
     Z_ = c.Alloc<V>(nnz_s_);
     P_ = c.Alloc<V>(nnz_s_);
-    R_ = c.Alloc<V>(nnz_s2_);
+    R_ = c.Alloc<V>(nnz_s2_ + 16);  // padding: the fused pass loads whole 16-byte chunks
     AZ_ = c.Alloc<V>(nnz_s2_);
     AP_ = c.Alloc<V>(nnz_s2_);
     grid2_ = GridFor(nnz_s2_);
