@@ -100,7 +100,7 @@ __global__ void WriteLast(int* __restrict__ out, int64_t n, const int64_t* __res
 }
 
 // ------------------------------------------------------------------ SpGEMM
-constexpr int kEscCap = 2048;     // expansion keys per wavefront (LDS sort tier)
+constexpr int kEscCap = 4096;     // expansion keys per wavefront (LDS tiers)
 constexpr int kSymWaves = 2;      // wavefronts per block in the symbolic kernel
 constexpr int kNumCap = 1024;     // output rows per column held in LDS (numeric)
 constexpr int kNumWaves = 4;
@@ -125,9 +125,25 @@ __global__ __launch_bounds__(kBlock) void ExpansionSizes(int ncols, const int* _
   }
 }
 
-// Expand-sort-compress symbolic phase, one wavefront per output column:
-// gathers the rows of X(:, k) for every k in Y(:, j) into LDS, bitonic-sorts
-// them and counts (fill == 0) or writes (fill == 1) the distinct rows.
+// Symbolic phase, one wavefront per output column whose expansion fits the
+// LDS tier: counts (fill == 0) or writes in row order (fill == 1) the
+// distinct rows of X(:, k) over k in Y(:, j) (and j itself when add_identity).
+// X's columns are sorted, so their first and last rows bound the column's row
+// range.  When that range spans fewer than 32 kEscCap rows, the rows are
+// marked in an LDS bitmap of the range and read back in order (popcounts per
+// lane, a wave scan, then the set bits); otherwise the expansion is gathered
+// into LDS, bitonic-sorted and compressed.
+__device__ __forceinline__ int WaveMinInt(int v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+__device__ __forceinline__ int WaveMaxInt(int v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+
 __global__ __launch_bounds__(kSymWaves* kWave) void SymbolicEsc(
     int ncols, const int* __restrict__ xptr, const int* __restrict__ xidx, const int* __restrict__ yptr,
     const int* __restrict__ yidx, const int64_t* __restrict__ upper, int add_identity, int fill,
@@ -140,8 +156,27 @@ __global__ __launch_bounds__(kSymWaves* kWave) void SymbolicEsc(
   if (j >= ncols) return;
   if (upper[j] > kEscCap) return;
   int* K = keys[w];
-  int off = 0;
   const int ye = yptr[j + 1];
+  // row range of the expansion
+  int lo_r = add_identity ? j : INT_MAX, hi_r = add_identity ? j : -1;
+  for (int p = yptr[j] + lane; p < ye; p += kWave) {
+    const int k = yidx[p];
+    const int b = xptr[k], e = xptr[k + 1];
+    if (e > b) {
+      lo_r = min(lo_r, xidx[b]);
+      hi_r = max(hi_r, xidx[e - 1]);
+    }
+  }
+  lo_r = WaveMinInt(lo_r);
+  hi_r = WaveMaxInt(hi_r);
+  const bool bitmap = hi_r >= lo_r && hi_r - lo_r < 32 * kEscCap;
+  const int nwords = bitmap ? ((hi_r - lo_r) >> 5) + 1 : 0;
+  if (bitmap) {
+    for (int i = lane; i < nwords; i += kWave) K[i] = 0;
+    WaveFence();
+  }
+  // gather: every lane takes expansion entries e = lane, lane + 64, ...
+  int off = 0;
   for (int p0 = yptr[j]; p0 < ye; p0 += kWave) {
     const int p = p0 + lane;
     int b = 0, c = 0;
@@ -164,10 +199,39 @@ __global__ __launch_bounds__(kSymWaves* kWave) void SymbolicEsc(
         else
           hi = mid;
       }
-      K[off + e] = xidx[beg[w][lo] + (e - pre[w][lo])];
+      const int row = xidx[beg[w][lo] + (e - pre[w][lo])];
+      if (bitmap) {
+        const int r = row - lo_r;
+        atomicOr(&K[r >> 5], int(1u << (r & 31)));
+      } else {
+        K[off + e] = row;
+      }
     }
     off += tot;
     WaveFence();
+  }
+  if (bitmap) {
+    if (add_identity && lane == 0) atomicOr(&K[(j - lo_r) >> 5], int(1u << ((j - lo_r) & 31)));
+    WaveFence();
+    const int per = (nwords + kWave - 1) / kWave;
+    const int w0 = min(nwords, lane * per), w1 = min(nwords, w0 + per);
+    int cnt = 0;
+    for (int i = w0; i < w1; ++i) cnt += __popc(uint32_t(K[i]));
+    if (!fill) {
+      cnt = WaveSumInt(cnt);
+      if (lane == 0) counts[j] = cnt;
+    } else {
+      int o = cptr[j] + WaveInclusiveScan(cnt) - cnt;
+      for (int i = w0; i < w1; ++i) {
+        uint32_t word = uint32_t(K[i]);
+        while (word) {
+          const int bit = __ffs(word) - 1;
+          cidx[o++] = lo_r + i * 32 + bit;
+          word &= word - 1u;
+        }
+      }
+    }
+    return;
   }
   if (add_identity) {
     if (lane == 0) K[off] = j;
@@ -391,61 +455,160 @@ __device__ __forceinline__ double MergeRight(int op, double r, double a, double 
   }
 }
 
-// Union of two sorted columns, one thread per column (Eigen's binary
-// iterator order).  fill == 0 counts kept entries; fill == 1 writes them.
-__global__ void MergeKernel(int ncols, const int* __restrict__ lp, const int* __restrict__ li,
-                            const double* __restrict__ lv, const int* __restrict__ rp, const int* __restrict__ ri,
-                            const double* __restrict__ rv, int op, double a, double b, int prune, int fill,
-                            int* __restrict__ counts, const int* __restrict__ optr, int* __restrict__ oidx,
-                            double* __restrict__ oval) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ncols) return;
-  int x = lp[j], xe = lp[j + 1], y = rp[j], ye = rp[j + 1];
-  int o = fill ? optr[j] : 0;
-  int c = 0;
-  while (x < xe || y < ye) {
-    int row;
-    double v;
-    if (x < xe && y < ye && li[x] == ri[y]) {
-      row = li[x];
-      v = MergeBoth(op, lv[x], rv[y], a, b);
-      ++x;
-      ++y;
-    } else if (x < xe && (y >= ye || li[x] < ri[y])) {
-      row = li[x];
-      v = MergeLeft(op, lv[x], a, b);
-      ++x;
-    } else {
-      row = ri[y];
-      v = MergeRight(op, rv[y], a, b);
-      ++y;
-    }
-    if (prune && !(fabs(v) > 0.0) && !isnan(v)) continue;
-    if (fill) {
-      oidx[o] = row;
-      oval[o] = v;
-      ++o;
-    }
-    ++c;
+// Merge path of two sorted columns with ties taken from the left first: the
+// number of left entries among the first d entries of the merged sequence.
+__device__ __forceinline__ int MergePathSplit(const int* __restrict__ li, int L, const int* __restrict__ ri, int R,
+                                              int d) {
+  int lo = max(0, d - R), hi = min(d, L);
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (ri[d - mid - 1] < li[mid])
+      hi = mid;
+    else
+      lo = mid + 1;
   }
-  if (!fill) counts[j] = c;
+  return lo;
+}
+
+// Union of two sorted columns, one wavefront per column: lane l walks
+// positions [l c, (l + 1) c) of the merged sequence (c = ceil((L + R) / 64)),
+// found by a merge-path search.  An entry present in both columns is one
+// union entry, handled by the lane that holds its left copy; the lane after
+// skips the right copy.  The union entries and their values are those of
+// the sequential merge (Eigen's binary iterator order), so fill == 1 writes
+// the same column as a thread-per-column merge would.  fill == 0 counts kept
+// entries.
+constexpr int kMergeWaves = 4;
+__global__ __launch_bounds__(kMergeWaves* kWave) void MergeWave(
+    int ncols, const int* __restrict__ lp, const int* __restrict__ li, const double* __restrict__ lv,
+    const int* __restrict__ rp, const int* __restrict__ ri, const double* __restrict__ rv, int op, double a, double b,
+    int prune, int fill, int* __restrict__ counts, const int* __restrict__ optr, int* __restrict__ oidx,
+    double* __restrict__ oval) {
+  const int lane = LaneId();
+  const int j = blockIdx.x * kMergeWaves + WaveInBlock();
+  if (j >= ncols) return;
+  const int xb = lp[j], L = lp[j + 1] - xb, yb = rp[j], R = rp[j + 1] - yb;
+  const int* const lx = li + xb;
+  const int* const ry = ri + yb;
+  const int T = L + R;
+  const int c = (T + kWave - 1) / kWave;
+  const int d0 = min(T, lane * c), d1 = min(T, d0 + c);
+  int x0 = MergePathSplit(lx, L, ry, R, d0), y0 = d0 - x0;
+  if (d0 < d1 && x0 > 0 && y0 < R && ry[y0] == lx[x0 - 1]) ++y0;  // right copy of the previous lane's last entry
+  auto walk = [&](bool write, int base) {
+    int x = x0, y = y0, kept = 0;
+    while (x + y < d1) {
+      int row;
+      double v;
+      if (x < L && (y >= R || lx[x] <= ry[y])) {
+        row = lx[x];
+        if (y < R && lx[x] == ry[y]) {
+          v = MergeBoth(op, lv[xb + x], rv[yb + y], a, b);
+          ++y;
+        } else {
+          v = MergeLeft(op, lv[xb + x], a, b);
+        }
+        ++x;
+      } else {
+        row = ry[y];
+        v = MergeRight(op, rv[yb + y], a, b);
+        ++y;
+      }
+      if (prune && !(fabs(v) > 0.0) && !isnan(v)) continue;
+      if (write) {
+        oidx[base + kept] = row;
+        oval[base + kept] = v;
+      }
+      ++kept;
+    }
+    return kept;
+  };
+  const int kept = walk(false, 0);
+  if (!fill) {
+    const int total = WaveSumInt(kept);
+    if (lane == 0) counts[j] = total;
+  } else {
+    const int excl = WaveInclusiveScan(kept) - kept;
+    walk(true, optr[j] + excl);
+  }
 }
 
 // ------------------------------------------------------------------ reductions
-// Per-column intersection dot (or squared norm when Y == X) into out[j].
-__global__ void ColumnDotKernel(int ncols, const int* __restrict__ xp, const int* __restrict__ xi,
-                                const double* __restrict__ xv, const int* __restrict__ yp,
-                                const int* __restrict__ yi, const double* __restrict__ yv, int same,
-                                double* __restrict__ out) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+// Per-column intersection dot (same == 0) or squared norm (same == 1) into
+// out[j], one wavefront per column.  The products are formed in parallel (the
+// matches of an intersection by merge-path segments, as in MergeWave), placed
+// in LDS in the column's row order, and added by one lane in that order: the
+// order of the sequential loop (ColumnDot), so out[j] is bit-identical to it.
+// An intersection with more than kDotCap matches takes ColumnDot in lane 0.
+constexpr int kDotWaves = 4;
+constexpr int kDotCap = 1024;
+__global__ __launch_bounds__(kDotWaves* kWave) void ColumnDotWave(int ncols, const int* __restrict__ xp,
+                                                                  const int* __restrict__ xi,
+                                                                  const double* __restrict__ xv,
+                                                                  const int* __restrict__ yp,
+                                                                  const int* __restrict__ yi,
+                                                                  const double* __restrict__ yv, int same,
+                                                                  double* __restrict__ out) {
+  __shared__ double prod[kDotWaves][kDotCap];
+  const int lane = LaneId(), w = WaveInBlock();
+  const int j = blockIdx.x * kDotWaves + w;
   if (j >= ncols) return;
-  double s = 0.0;
+  const int xb = xp[j], L = xp[j + 1] - xb;
+  double* const P = prod[w];
   if (same) {
-    for (int p = xp[j]; p < xp[j + 1]; ++p) s += xv[p] * xv[p];
-  } else {
-    s = ColumnDot(xi, xv, xp[j], xp[j + 1], yi, yv, yp[j], yp[j + 1]);
+    double s = 0.0;
+    for (int e0 = 0; e0 < L; e0 += kDotCap) {
+      const int m = min(kDotCap, L - e0);
+      for (int e = lane; e < m; e += kWave) {
+        const double v = xv[xb + e0 + e];
+        P[e] = v * v;
+      }
+      WaveFence();
+      if (lane == 0)
+        for (int e = 0; e < m; ++e) s += P[e];
+      WaveFence();
+    }
+    if (lane == 0) out[j] = s;
+    return;
   }
-  out[j] = s;
+  const int yb = yp[j], R = yp[j + 1] - yb;
+  const int* const lx = xi + xb;
+  const int* const ry = yi + yb;
+  const int T = L + R;
+  const int c = (T + kWave - 1) / kWave;
+  const int d0 = min(T, lane * c), d1 = min(T, d0 + c);
+  int x0 = MergePathSplit(lx, L, ry, R, d0), y0 = d0 - x0;
+  if (d0 < d1 && x0 > 0 && y0 < R && ry[y0] == lx[x0 - 1]) ++y0;  // matched by the previous lane
+  auto walk = [&](bool write, int base) {
+    int x = x0, y = y0, k = 0;
+    while (x + y < d1) {
+      if (x < L && (y >= R || lx[x] <= ry[y])) {
+        if (y < R && lx[x] == ry[y]) {
+          if (write) P[base + k] = xv[xb + x] * yv[yb + y];
+          ++k;
+          ++y;
+        }
+        ++x;
+      } else {
+        ++y;
+      }
+    }
+    return k;
+  };
+  const int k = walk(false, 0);
+  const int incl = WaveInclusiveScan(k);
+  const int total = __shfl(incl, kWave - 1, kWave);
+  if (total > kDotCap) {
+    if (lane == 0) out[j] = ColumnDot(xi, xv, xb, xb + L, yi, yv, yb, yb + R);
+    return;
+  }
+  walk(true, incl - k);
+  WaveFence();
+  if (lane == 0) {
+    double s = 0.0;
+    for (int e = 0; e < total; ++e) s += P[e];
+    out[j] = s;
+  }
 }
 
 // Deterministic sum of v[0..n): per-block partials, then one block.
@@ -546,62 +709,70 @@ __global__ void ThresholdBatchesKernel(int64_t nnz_cols, int ncols, const int* _
   for (int p = ptr[j] + threadIdx.x; p < ptr[j + 1]; p += blockDim.x) keep[p] = !(fabs(v[p]) <= thr) ? 1 : 0;
 }
 
-// Per-column top-k by |v| (diagonal kept, ties keep the smaller row): rank of
-// each off-diagonal entry among the column's off-diagonal entries.
-// Per-column top-k by |v| with the diagonal always kept; ties keep the
-// smaller row.  One block per column: an 8-pass radix select over the
-// order-preserving keys of |v| finds the k-th largest key T and how many
-// entries equal to T are taken; entries above T are kept, and an entry equal
-// to T is kept when fewer than that many equal entries have a smaller row.
-__global__ __launch_bounds__(256) void TopKKernel(int ncols, const int* __restrict__ ptr, const int* __restrict__ idx,
-                                                  const double* __restrict__ v, int64_t k,
-                                                  unsigned char* __restrict__ keep) {
-  __shared__ unsigned hist[256];
-  __shared__ uint64_t s_prefix;
-  __shared__ int s_target;
-  const int j = blockIdx.x;
+// Per-column top-k by |v| (ties: smaller row first) with the diagonal always
+// kept, one wavefront per column: an 8-bit radix select over the
+// order-preserving keys of |v| (8 passes, a 256-bin LDS histogram per pass,
+// the crossing bin found by a wave scan over 4 bins per lane), then one pass
+// that keeps keys above the threshold key and the first `take` keys equal to
+// it in row order.
+constexpr int kTopKWaves = 4;
+__global__ __launch_bounds__(kTopKWaves* kWave) void TopKWave(int ncols, const int* __restrict__ ptr,
+                                                              const int* __restrict__ idx,
+                                                              const double* __restrict__ v, int64_t k,
+                                                              unsigned char* __restrict__ keep) {
+  __shared__ unsigned hist[kTopKWaves][256];
+  const int lane = LaneId(), w = WaveInBlock();
+  const int j = blockIdx.x * kTopKWaves + w;
+  if (j >= ncols) return;
+  unsigned* const H = hist[w];
   const int b = ptr[j], e = ptr[j + 1];
-  // off-diagonal entries of the column
   int offd = 0;
-  for (int p = b + threadIdx.x; p < e; p += blockDim.x) offd += idx[p] != j ? 1 : 0;
-  __shared__ int red[16];
-  offd = BlockSum<int>(offd, red);
+  for (int p = b + lane; p < e; p += kWave) offd += idx[p] != j ? 1 : 0;
+  offd = WaveSumInt(offd);
   if (offd <= k) {
-    for (int p = b + threadIdx.x; p < e; p += blockDim.x) keep[p] = 1;
+    for (int p = b + lane; p < e; p += kWave) keep[p] = 1;
     return;
   }
-  if (threadIdx.x == 0) {
-    s_prefix = 0;
-    s_target = int(k);  // rank of the wanted key counted from the largest
-  }
-  uint64_t mask = 0;
+  uint64_t prefix = 0, mask = 0;
+  int target = int(k);  // rank of the wanted key counted from the largest
   for (int shift = 56; shift >= 0; shift -= 8) {
-    for (int t = threadIdx.x; t < 256; t += blockDim.x) hist[t] = 0;
-    __syncthreads();
-    const uint64_t prefix = s_prefix;
-    for (int p = b + threadIdx.x; p < e; p += blockDim.x) {
+    for (int t = lane; t < 256; t += kWave) H[t] = 0;
+    WaveFence();
+    for (int p = b + lane; p < e; p += kWave) {
       if (idx[p] == j) continue;
       const uint64_t key = OrderedKey(fabs(v[p]));
-      if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+      if ((key & mask) == prefix) atomicAdd(&H[(key >> shift) & 255u], 1u);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int target = s_target;
-      int bin = 255;
-      for (; bin > 0; --bin) {
-        const int c = int(hist[bin]);
-        if (c >= target) break;
-        target -= c;
+    WaveFence();
+    int c[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) c[i] = int(H[4 * lane + i]);
+    const int mine = c[0] + c[1] + c[2] + c[3];
+    const int incl = WaveInclusiveScan(mine);
+    const int above = __shfl(incl, kWave - 1, kWave) - incl;  // keys in the bins of higher lanes
+    const bool here = above < target && target <= above + mine;
+    int bin = 0, rank = 0;
+    if (here) {
+      int t = target - above;
+      for (int i = 3; i >= 0; --i) {
+        if (c[i] >= t) {
+          bin = 4 * lane + i;
+          rank = t;
+          break;
+        }
+        t -= c[i];
       }
-      s_target = target;
-      s_prefix = prefix | (uint64_t(bin) << shift);
     }
+    const int src = __ffsll((unsigned long long)__ballot(here)) - 1;
+    bin = __shfl(bin, src, kWave);
+    target = __shfl(rank, src, kWave);
+    prefix |= uint64_t(bin) << shift;
     mask |= uint64_t(255) << shift;
-    __syncthreads();
+    WaveFence();
   }
-  const uint64_t T = s_prefix;
-  const int take = s_target;  // how many entries with key == T are kept
-  for (int p = b + threadIdx.x; p < e; p += blockDim.x) {
+  const uint64_t T = prefix;
+  const int take = target;  // how many entries with key == T are kept
+  for (int p = b + lane; p < e; p += kWave) {
     const int row = idx[p];
     if (row == j) {
       keep[p] = 1;
@@ -1200,18 +1371,19 @@ DMat Merge(Ctx& ctx, const DMat& L, const DMat& R, MergeOp op, double a, double 
   if (L.rows() != R.rows() || L.cols() != R.cols()) throw std::invalid_argument("Sparse matrices must have the same shape");
   const int64_t n = L.cols();
   auto counts = ctx.Alloc<int>(n > 0 ? n : 1);
-  MergeKernel<<<Blocks(n), kBlock, 0, ctx.stream>>>((int)n, L.ptr(), L.idx(), L.v(), R.ptr(), R.idx(), R.v(),
-                                                    (int)op, a, b, prune_zero ? 1 : 0, 0, counts.get(), nullptr,
-                                                    nullptr, nullptr);
+  const unsigned mg = Blocks(n, kMergeWaves);
+  MergeWave<<<mg, kMergeWaves * kWave, 0, ctx.stream>>>((int)n, L.ptr(), L.idx(), L.v(), R.ptr(), R.idx(), R.v(),
+                                                        (int)op, a, b, prune_zero ? 1 : 0, 0, counts.get(), nullptr,
+                                                        nullptr, nullptr);
   DMat C;
   C.pat = MakePattern(ctx, L.rows(), n, 0);
   const int64_t total = ScanCounts(ctx, counts.get(), n, C.pat->ptr.get());
   C.pat->nnz = total;
   C.pat->idx = ctx.Alloc<int>(total > 0 ? total : 1);
   C.val = ctx.Alloc<double>(total > 0 ? total : 1);
-  MergeKernel<<<Blocks(n), kBlock, 0, ctx.stream>>>((int)n, L.ptr(), L.idx(), L.v(), R.ptr(), R.idx(), R.v(),
-                                                    (int)op, a, b, prune_zero ? 1 : 0, 1, nullptr, C.pat->ptr.get(),
-                                                    C.pat->idx.get(), C.val.get());
+  MergeWave<<<mg, kMergeWaves * kWave, 0, ctx.stream>>>((int)n, L.ptr(), L.idx(), L.v(), R.ptr(), R.idx(), R.v(),
+                                                        (int)op, a, b, prune_zero ? 1 : 0, 1, nullptr,
+                                                        C.pat->ptr.get(), C.pat->idx.get(), C.val.get());
   return C;
 }
 
@@ -1270,14 +1442,14 @@ double SumDevice(Ctx& ctx, const double* v, int64_t n) {
 
 void ColumnDots(Ctx& ctx, const DMat& X, const DMat& Y, double* out) {
   const int64_t n = X.cols();
-  ColumnDotKernel<<<Blocks(n), kBlock, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), X.v(), Y.ptr(), Y.idx(), Y.v(), 0,
-                                                        out);
+  ColumnDotWave<<<Blocks(n, kDotWaves), kDotWaves * kWave, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), X.v(), Y.ptr(),
+                                                                            Y.idx(), Y.v(), 0, out);
 }
 
 void ColumnSqNorms(Ctx& ctx, const DMat& X, double* out) {
   const int64_t n = X.cols();
-  ColumnDotKernel<<<Blocks(n), kBlock, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), X.v(), X.ptr(), X.idx(), X.v(), 1,
-                                                        out);
+  ColumnDotWave<<<Blocks(n, kDotWaves), kDotWaves * kWave, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), X.v(), X.ptr(),
+                                                                            X.idx(), X.v(), 1, out);
 }
 
 double FrobDot(Ctx& ctx, const DMat& X, const DMat& Y) {
@@ -1332,7 +1504,9 @@ DMat ThresholdBatches(Ctx& ctx, const DMat& X, const double* batch_sq, const int
 DMat ColumnTopK(Ctx& ctx, const DMat& X, int64_t k) {
   const int64_t n = X.cols(), nnz = X.nnz();
   auto keep = ctx.Alloc<unsigned char>(nnz > 0 ? nnz : 1);
-  if (n > 0) TopKKernel<<<(unsigned)n, 256, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), X.v(), k, keep.get());
+  if (n > 0)
+    TopKWave<<<Blocks(n, kTopKWaves), kTopKWaves * kWave, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), X.v(), k,
+                                                                          keep.get());
   return Filter(ctx, X, keep.get());
 }
 
