@@ -6,6 +6,8 @@ Layout:
   _lib.py          ctypes bindings of the extension entry points
   harness.py       backward-error harness (prepared PCG system, RHS recipe)
   autograd.py      torch.autograd.Function for ||I - A M||_F^2
+  datasets.py      sample-dataset study (MatrixMarket loading, construction
+                   and backward-error tables of scripts/common.py)
 
 One HIP runtime per process: libspai_hip.so needs libamdhip64, and the dynamic
 loader reuses a copy that is already loaded.  PyTorch ships its own; if

@@ -7,6 +7,10 @@ data files (run in a container where the reference tree is mounted).
   * <name>-<mode>-d<density>-convergence.csv: the first rows of the reference's
     per-iteration ||R_i||_F table for the seven method variants
     (scripts/results/results_sample/<mode>/max_iter_*/max_density_*/<name>/*-convergence.csv).
+  * <name>-<mode>-d<density>-{construction_status,backward_error,solver_status}.csv
+    (PIPELINES only): the reference's end-of-build statuses, the PCG
+    relative-residual histories with its preconditioners (150-entry limit) and
+    the solver statuses, copied whole.
 """
 import csv
 import glob
@@ -21,7 +25,8 @@ REF = Path("/root/reference/scripts")
 OUT = Path(__file__).resolve().parent
 ROWS = 41
 
-DATASETS = ["tri100eigs4k", "bcsstk21"]
+DATASETS = ["tri100eigs4k", "bcsstk21", "msc04515"]
+PIPELINES = [("tri100eigs4k", "no_dropping", "1.000")]
 
 
 def main():
@@ -42,6 +47,14 @@ def main():
                 rows = list(csv.reader(fh))
             with open(OUT / f"{name}-{mode}-d{dens}-convergence.csv", "w", newline="") as fh:
                 csv.writer(fh).writerows(rows[:ROWS + 1])
+    for name, mode, dens in PIPELINES:
+        for f in sorted(glob.glob(str(REF / f"results/results_sample/{mode}/max_iter_*/max_density_{dens}/{name}/*.csv"))):
+            kind = Path(f).stem.split("-")[-1]
+            if kind in ("construction_status", "backward_error", "solver_status"):
+                with open(f) as fh:
+                    rows = list(csv.reader(fh))
+                with open(OUT / f"{name}-{mode}-d{dens}-{kind}.csv", "w", newline="") as fh:
+                    csv.writer(fh).writerows(rows)
     return 0
 
 

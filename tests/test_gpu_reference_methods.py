@@ -108,7 +108,8 @@ def test_random_spd_batch_ordered_identical(gpu_lib, oracle_lib, method, droppin
 
 
 GOLDEN = [("tri100eigs4k", "no_dropping", "1.000", 8), ("tri100eigs4k", "dropping", "0.030", 8),
-          ("bcsstk21", "no_dropping", "1.000", 4), ("bcsstk21", "dropping", "0.030", 4)]
+          ("bcsstk21", "no_dropping", "1.000", 4), ("bcsstk21", "dropping", "0.030", 4),
+          ("msc04515", "dropping", "0.030", 6)]
 
 
 @pytest.mark.parametrize("name,mode,density,iters", GOLDEN)

@@ -25,6 +25,7 @@ CASES = [
     ("tri100eigs4k", "dropping", "0.030", 12),
     ("bcsstk21", "no_dropping", "1.000", 6),
     ("bcsstk21", "dropping", "0.030", 6),
+    ("msc04515", "dropping", "0.030", 4),
 ]
 
 
