@@ -351,6 +351,17 @@ struct alignas(16) VecQ2 {
   V2<V> v[16 / sizeof(V)];
 };
 
+// The lane's terms of the last step batch of product 1 (bit s of act: the
+// lane has a term in step s of the batch).  A column of at most kFusedPre
+// steps has one batch, so product 2 takes its terms from these registers.
+constexpr int kFusedPre = 8;
+template <class V, class SlotT>
+struct FusedTerms {
+  SlotT sl[kFusedPre];
+  V a[kFusedPre];
+  uint32_t act;
+};
+
 // One term of product 1 as product 2 reuses it; slot -1: the lane had no term.
 template <class V>
 struct alignas(2 * sizeof(V)) FusedTerm {
@@ -395,15 +406,17 @@ __device__ __forceinline__ void FusedStage(int g, int gl, int eb, int e1, const 
 // the term u = gl, and the plan slots and A values of up to kPre steps are
 // loaded before any of them is added, so a column waits for one load latency
 // per kPre steps.  The column is S entries [e0, e1) with plan terms from pp
-// on.  kStash: every lane's (A value, slot) of every step goes to
-// stash[step][lane] (no column of A is longer than GL then).
+// on.  kStash and stash_write: every lane's (A value, slot) of every step
+// goes to stash[step][lane] (no column of A is longer than GL then).  `last`
+// receives the lane's terms of the last batch.
 template <class V, class SlotT, int GL, int NY, bool kStash, class YGet>
 __device__ __forceinline__ void FusedProduct1(int g, int gl, int e0, int e1, uint32_t pp,
                                               const int2* __restrict__ a0c, const V* __restrict__ av,
                                               const SlotT* __restrict__ plan, V2<V>* acc, int2* st_ac,
-                                              V2<V>* st_y, FusedTerm<V>* stash, YGet yget) {
+                                              V2<V>* st_y, FusedTerm<V>* stash, bool stash_write,
+                                              FusedTerms<V, SlotT>& last, YGet yget) {
   constexpr int W = kWave / GL;
-  constexpr int kPre = 8;
+  constexpr int kPre = kFusedPre;
   const int lane = g * GL + gl;
   for (int eb = e0; eb < e1; eb += GL) {
     __builtin_amdgcn_wave_barrier();
@@ -434,10 +447,18 @@ __device__ __forceinline__ void FusedProduct1(int g, int gl, int e0, int e1, uin
         for (int s = 0; s < kPre; ++s) {
           if (s0 + s < ns) {
             const bool act_s = (act >> s) & 1u;
-            if constexpr (kStash) stash[(eb - e0 + s0 + s) * kWave + lane] = FusedTerm<V>{a[s], act_s ? int(sl[s]) : -1};
+            if constexpr (kStash) {
+              if (stash_write) stash[(eb - e0 + s0 + s) * kWave + lane] = FusedTerm<V>{a[s], act_s ? int(sl[s]) : -1};
+            }
             if (act_s) AddTerm<V, NY>(acc, int(sl[s]), a[s], st_y[(s0 + s) * W + g]);
           }
         }
+#pragma unroll
+        for (int s = 0; s < kPre; ++s) {
+          last.sl[s] = sl[s];
+          last.a[s] = a[s];
+        }
+        last.act = act;
       } else {
         // a column of A longer than GL: lanes stride through each step
         uint32_t o2 = pp;
@@ -493,7 +514,7 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
     const int64_t* __restrict__ plan_ptr, const SlotT* __restrict__ plan, int lmax, int smax,
     const V* __restrict__ prS, const V* __restrict__ prS2, V* __restrict__ R, V* __restrict__ M, V* __restrict__ Z,
     V* __restrict__ P, const double* __restrict__ sc, double* __restrict__ part_out,
-    const Flags* __restrict__ flags) {
+    const Flags* __restrict__ flags, int narrow) {
   constexpr int W = kWave / GL;
   constexpr int kVec = 16 / int(sizeof(V));  // S2 entries per chunk
   constexpr int RQ = sizeof(V) == 4 ? 1 : 2;  // S2 chunks of a tile held in registers per lane
@@ -611,6 +632,12 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
       tp[r] = in ? int(stq[p]) : 0;
     }
     const int ncs = e1 - e0;
+    // product 2 takes a one-batch column's terms from product 1's registers
+    // (FP64 LOMR keeps 16 doubles of direction state already; holding the
+    // terms as well spills, so it reloads or stashes them)
+    constexpr bool kRegTerms = !(sizeof(V) == 8 && kLomrIter);
+    const bool reg2 = kRegTerms && narrow && ncs <= kFusedPre;
+    FusedTerms<V, SlotT> terms;
     // product 1: A [Z P](:, j)  (kInit: A M(:, j); MR: A (M + alpha Z)(:, j))
     for (int t = lane; t < qe - qal; t += kWave) {
       if constexpr (NY > 1)
@@ -625,7 +652,7 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
         st_y[gl * W + g] = pre_y;
       }
       FusedProduct1<V, SlotT, GL, NY, kStash>(g, gl, e0, e1, pp, a0c, av, plan, acc + cb2, st_ac, st_y, stash,
-                                              yget1);
+                                              !reg2, terms, yget1);
       // R = I - A M: the diagonal's sum becomes (A M)(j, j) - 1, so that
       // R = -sum below rounds as 1 - (A M)(j, j)
       if (!kLomrIter && gl == 0) acc[cb2 + dj].a -= V(1);
@@ -716,7 +743,12 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
     }
     __builtin_amdgcn_wave_barrier();
     if (valid) {
-      if constexpr (kStash) {
+      if (reg2) {
+        // one batch: product 1's terms are still in registers
+#pragma unroll
+        for (int s = 0; s < kFusedPre; ++s)
+          if ((terms.act >> s) & 1u) AddTerm<V, NY2>(acc + cb2, int(terms.sl[s]), terms.a[s], yzp[cb + s]);
+      } else if constexpr (kStash) {
         FusedProduct2Stash<V, NY2>(ncs, lane, stash, yzp + cb, acc + cb2);
       } else {
         const V2<V>* const ycol = yzp + cb - e0;
@@ -727,7 +759,7 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
         };
         FusedStage<V, GL>(g, gl, e0, e1, a0c, st_ac, st_y, yget2);
         FusedProduct1<V, SlotT, GL, NY2, false>(g, gl, e0, e1, pp, a0c, av, plan, acc + cb2, st_ac, st_y, stash,
-                                                yget2);
+                                                false, terms, yget2);
       }
     }
     __builtin_amdgcn_wave_barrier();
@@ -1828,7 +1860,7 @@ class FixedImpl final : public FixedProblem {
   FusedPass<V, SlotT, GLV, INIT, MR, STASH><<<gridf_, kBlk, smemf_, st>>>(                                     \
       (int)n_, s_pat_->ptr.get(), a0c_.get(), stq_.get(), s2d_.get(), s2_pat_->ptr.get(), av_.get(),           \
       plan_ptr_.get(), plan, lmax_, smax_, prS_.get(), prS2_.get(), R_.get(), M_.get(), Z_.get(), P_.get(),    \
-      sc_.get(), part, flags_.get())
+      sc_.get(), part, flags_.get(), maxcol_ <= gl_ ? 1 : 0)
 #define SPAI_FUSED_ONE(SlotT, plan, GLV, INIT, MR)          \
   do {                                                      \
     if (stash_)                                             \
