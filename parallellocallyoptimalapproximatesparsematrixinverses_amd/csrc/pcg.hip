@@ -221,14 +221,21 @@ struct RowView {
   int base, stride, count;
 };
 
+// Entries past the end of a row: kPadded rows can be read there (the arrays
+// hold kSellPad entries of padding, so reads of 8 consecutive entries never
+// leave them); others are only read inside the row.
+constexpr int kSellPad = 8 * 64;
+
 // CSR rows.
 struct CsrRows {
+  static constexpr bool kPadded = false;
   const int* __restrict__ ptr;
   __device__ __forceinline__ RowView Row(int r) const { return RowView{ptr[r], 1, ptr[r + 1] - ptr[r]}; }
 };
 
 // Sliced ELLPACK rows (DSell): 64-row slices, column-major inside a slice.
 struct SellRows {
+  static constexpr bool kPadded = true;
   const int* __restrict__ off;
   __device__ __forceinline__ RowView Row(int r) const {
     const int s = r >> 6, b = off[s];
@@ -238,23 +245,51 @@ struct SellRows {
 
 // sum_u val[k_u] * f(col[k_u]) over a row's entries in order, 8 loads in
 // flight at a time; u < 8 come from `first` when given (prefetched).
-template <class MV, class F>
+// kEager: the 8 column indices and values of a group are loaded first, then
+// the gathers at max(c, 0) (in bounds for padding too), all without branches
+// on loaded data, so the loads of a group are in flight together; this holds
+// more registers.  Otherwise each entry's loads wait for its column index
+// (fewer registers, more waves).
+template <class MV, bool kPadded, bool kEager, class F>
 __device__ __forceinline__ double RowSum(const RowView rv, const int* __restrict__ col, const MV* __restrict__ val,
-                                         int u_begin, F f) {
-  double s = 0.0;
+                                         int u_begin, F f, double s = 0.0) {
   for (int u0 = u_begin; u0 < rv.count; u0 += 8) {
-    double t[8];
-    bool in[8];
+    if constexpr (!kEager) {
+      double t[8];
+      bool in[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = rv.base + (u0 + u) * rv.stride;
+        const int c = u0 + u < rv.count ? col[k] : -1;
+        in[u] = c >= 0;
+        t[u] = in[u] ? double(val[k]) * f(c) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (in[u]) s += t[u];
+      continue;
+    }
+    int c[8];
+    MV v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int k = rv.base + (u0 + u) * rv.stride;
-      const int c = u0 + u < rv.count ? col[k] : -1;
-      in[u] = c >= 0;
-      t[u] = in[u] ? double(val[k]) * f(c) : 0.0;
+      const bool have = u0 + u < rv.count;
+      if constexpr (kPadded) {
+        c[u] = col[k];
+        v[u] = val[k];
+        c[u] = have ? c[u] : -1;
+      } else {
+        c[u] = have ? col[k] : -1;
+        v[u] = have ? val[k] : MV(0);
+      }
     }
+    double t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = double(v[u]) * f(max(c[u], 0));
 #pragma unroll
     for (int u = 0; u < 8; ++u)
-      if (in[u]) s += t[u];
+      if (c[u] >= 0) s += t[u];
   }
   return s;
 }
@@ -268,18 +303,42 @@ struct RowPrefetch {
   MV v[8];
   double x0[8], x1[8];
   bool in[8];
-  template <class Rows>
+  template <bool kEager, class Rows>
   __device__ __forceinline__ void Load(const Rows& rows, const int* __restrict__ col, const MV* __restrict__ val,
                                        const double* __restrict__ y0, const double* __restrict__ y1, int row) {
     rv = row >= 0 ? rows.Row(row) : RowView{0, 1, 0};
+    if constexpr (!kEager) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = rv.base + u * rv.stride;
+        const int c = u < rv.count ? col[k] : -1;
+        in[u] = c >= 0;
+        v[u] = in[u] ? val[k] : MV(0);
+        x0[u] = in[u] ? y0[c] : 0.0;
+        x1[u] = in[u] ? y1[c] : 0.0;
+      }
+      return;
+    }
+    int c[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int k = rv.base + u * rv.stride;
-      const int c = u < rv.count ? col[k] : -1;
-      in[u] = c >= 0;
-      v[u] = in[u] ? val[k] : MV(0);
-      x0[u] = in[u] ? y0[c] : 0.0;
-      x1[u] = in[u] ? y1[c] : 0.0;
+      const bool have = u < rv.count;
+      if constexpr (Rows::kPadded) {
+        c[u] = col[k];
+        v[u] = val[k];
+        c[u] = have ? c[u] : -1;
+      } else {
+        c[u] = have ? col[k] : -1;
+        v[u] = have ? val[k] : MV(0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      in[u] = c[u] >= 0;
+      const int cc = max(c[u], 0);
+      x0[u] = y0[cc];
+      x1[u] = y1[cc];
     }
   }
 };
@@ -305,7 +364,7 @@ __global__ __launch_bounds__(kBlk) void PcgDirSpmvA(int n, int par, Rows rows, c
   double z0 = 0.0, p0 = 0.0;
   if constexpr (SG == 1) {
     const bool have = row0 < n;
-    pf.Load(rows, ac, av, z, p_in, have ? row0 : -1);
+    pf.template Load<false>(rows, ac, av, z, p_in, have ? row0 : -1);
     z0 = have ? z[row0] : 0.0;
     p0 = have ? p_in[row0] : 0.0;
   }
@@ -340,28 +399,11 @@ __global__ __launch_bounds__(kBlk) void PcgDirSpmvA(int n, int par, Rows rows, c
 #pragma unroll
         for (int u = 0; u < 8; ++u)
           if (pf.in[u]) s += t[u];
-        if (pf.rv.count > 8) {
-          // continue the same sequential sum
-          double s2 = s;
-          for (int u0 = 8; u0 < pf.rv.count; u0 += 8) {
-            double t2[8];
-            bool in2[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-              const int k = pf.rv.base + (u0 + u) * pf.rv.stride;
-              const int c = u0 + u < pf.rv.count ? ac[k] : -1;
-              in2[u] = c >= 0;
-              t2[u] = in2[u] ? av[k] * dir(c) : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-              if (in2[u]) s2 += t2[u];
-          }
-          s = s2;
-        }
+        // entries past the first 8 continue the same sequential sum
+        if (pf.rv.count > 8) s = RowSum<double, Rows::kPadded, false>(pf.rv, ac, av, 8, dir, s);
         pr = z0 + beta * p0;
       } else {
-        s = RowSum<double>(rows.Row(row), ac, av, 0, dir);
+        s = RowSum<double, Rows::kPadded, false>(rows.Row(row), ac, av, 0, dir);
         pr = z[row] + beta * p_in[row];
       }
     } else {
@@ -403,13 +445,15 @@ __global__ __launch_bounds__(kBlk) void PcgUpdateSpmvM(int n, int par, Rows rows
                                                        double* __restrict__ z, double* __restrict__ part,
                                                        double* __restrict__ ds, int* __restrict__ is) {
   if (is[kDone]) return;
+  // FP32 values leave room for the registers of eager loads; FP64 ones do not
+  constexpr bool kEagerM = sizeof(MV) == 4 && Rows::kPadded;
   const int G = gridDim.x;
   const int row0 = XcdBlock() * (kBlk / SG) + threadIdx.x / SG;
   RowPrefetch<MV> pf;
   double x0 = 0.0, p0 = 0.0, r0 = 0.0, a0 = 0.0;
   if constexpr (SG == 1) {
     const bool have = row0 < n;
-    pf.Load(rows, mc, mv, r_in, Ap, have ? row0 : -1);
+    pf.template Load<kEagerM>(rows, mc, mv, r_in, Ap, have ? row0 : -1);
     x0 = have ? x[row0] : 0.0;
     p0 = have ? p[row0] : 0.0;
     r0 = have ? r_in[row0] : 0.0;
@@ -445,24 +489,11 @@ __global__ __launch_bounds__(kBlk) void PcgUpdateSpmvM(int n, int par, Rows rows
 #pragma unroll
         for (int u = 0; u < 8; ++u)
           if (pf.in[u]) zs += t[u];
-        for (int u0 = 8; u0 < pf.rv.count; u0 += 8) {
-          double t2[8];
-          bool in2[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const int k = pf.rv.base + (u0 + u) * pf.rv.stride;
-            const int c = u0 + u < pf.rv.count ? mc[k] : -1;
-            in2[u] = c >= 0;
-            t2[u] = in2[u] ? double(mv[k]) * upd(c) : 0.0;
-          }
-#pragma unroll
-          for (int u = 0; u < 8; ++u)
-            if (in2[u]) zs += t2[u];
-        }
+        if (pf.rv.count > 8) zs = RowSum<MV, Rows::kPadded, kEagerM>(pf.rv, mc, mv, 8, upd, zs);
         xv = x0 + alpha * p0;
         ri = r0 - alpha * a0;
       } else {
-        zs = RowSum<MV>(rows.Row(row), mc, mv, 0, upd);
+        zs = RowSum<MV, Rows::kPadded, kEagerM>(rows.Row(row), mc, mv, 0, upd);
         xv = x[row] + alpha * p[row];
         ri = r_in[row] - alpha * Ap[row];
       }
@@ -712,13 +743,21 @@ DSell SellFromCsr(Ctx& ctx, const DCsr& m, bool fp32) {
   s.off = ctx.Alloc<int>(s.nslices + 1);
   s.len = ScanCounts(ctx, slots.get(), s.nslices, s.off.get());
   if (s.len >= (int64_t(1) << 31)) throw std::runtime_error("SELL copy exceeds the int32 range");
-  s.col = ctx.Alloc<int>(std::max<int64_t>(s.len, 1));
-  s.src = ctx.Alloc<int>(std::max<int64_t>(s.len, 1));
+  // kSellPad entries of padding (col = src = -1, values 0): a thread reads 8
+  // entries of its row at a time, also past the row's end
+  s.col = ctx.Alloc<int>(s.len + kSellPad);
+  s.src = ctx.Alloc<int>(s.len + kSellPad);
+  SPAI_HIP_CHECK(hipMemsetAsync(s.col.get() + s.len, 0xFF, sizeof(int) * kSellPad, ctx.stream));
+  SPAI_HIP_CHECK(hipMemsetAsync(s.src.get() + s.len, 0xFF, sizeof(int) * kSellPad, ctx.stream));
   SellFill<<<(unsigned)((s.nslices * 64 + 255) / 256), 256, 0, ctx.stream>>>(m.n, s.nslices, s.off.get(),
                                                                             m.ptr.get(), m.col.get(), s.col.get(),
                                                                             s.src.get());
-  s.val = ctx.Alloc<double>(std::max<int64_t>(s.len, 1));
-  if (fp32) s.val32 = ctx.Alloc<float>(std::max<int64_t>(s.len, 1));
+  s.val = ctx.Alloc<double>(s.len + kSellPad);
+  SPAI_HIP_CHECK(hipMemsetAsync(s.val.get() + s.len, 0, sizeof(double) * kSellPad, ctx.stream));
+  if (fp32) {
+    s.val32 = ctx.Alloc<float>(s.len + kSellPad);
+    SPAI_HIP_CHECK(hipMemsetAsync(s.val32.get() + s.len, 0, sizeof(float) * kSellPad, ctx.stream));
+  }
   SellValues(ctx, s, m);
   return s;
 }
