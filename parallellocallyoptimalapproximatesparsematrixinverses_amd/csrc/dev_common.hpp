@@ -76,6 +76,30 @@ __device__ T BlockSum(T v, T* scratch /* >= 16 entries in LDS */) {
   return t;
 }
 
+// Two block sums in one pass: each value is summed in BlockSum's order (so
+// the results equal two BlockSum calls) with one set of barriers.
+template <class T>
+__device__ void BlockSum2(T& a, T& b, T* scratch /* >= 32 entries in LDS */) {
+  a = WaveSum(a);
+  b = WaveSum(b);
+  const int w = WaveInBlock();
+  const int nw = (blockDim.x + kWave - 1) / kWave;
+  __syncthreads();
+  if (LaneId() == 0) {
+    scratch[w] = a;
+    scratch[16 + w] = b;
+  }
+  __syncthreads();
+  T ta = 0, tb = 0;
+  for (int i = 0; i < nw; ++i) {
+    ta += scratch[i];
+    tb += scratch[16 + i];
+  }
+  __syncthreads();
+  a = ta;
+  b = tb;
+}
+
 // First index in [lo, hi) of sorted a[] with a[i] >= key.
 template <class I>
 __device__ __forceinline__ int LowerBound(const I* a, int lo, int hi, I key) {

@@ -129,6 +129,30 @@ __device__ __forceinline__ void StorePartial(double v, double* __restrict__ part
   if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
 
+// Block sums of a and b written to pa[blockIdx.x] and pb[blockIdx.x].
+__device__ __forceinline__ void StorePartial2(double a, double b, double* __restrict__ pa, double* __restrict__ pb) {
+  __shared__ double red[32];
+  BlockSum2<double>(a, b, red);
+  if (threadIdx.x == 0) {
+    pa[blockIdx.x] = a;
+    pb[blockIdx.x] = b;
+  }
+}
+
+// Every block: the fixed-order sums of two arrays of G partials at once
+// (each equal to SumPartials of it).
+__device__ __forceinline__ void SumPartials2(const double* __restrict__ pa, const double* __restrict__ pb, int G,
+                                             double& a, double& b) {
+  __shared__ double red[32];
+  a = 0.0;
+  b = 0.0;
+  for (int i = threadIdx.x; i < G; i += blockDim.x) {
+    a += pa[i];
+    b += pb[i];
+  }
+  BlockSum2<double>(a, b, red);
+}
+
 // Every block: fixed-order sum of the previous kernel's G partials.
 __device__ __forceinline__ double SumPartials(const double* __restrict__ part, int G) {
   __shared__ double red[16];
@@ -370,7 +394,9 @@ __global__ __launch_bounds__(kBlk) void PcgDirSpmvA(int n, int par, Rows rows, c
   }
   double beta = 0.0;
   if (is[prev ? kPend1 : kPend0]) {
-    const double err = sqrt(SumPartials(part + kP2 * G, G)) / ds[kBnorm];
+    double rr = 0.0, r_z = 0.0;
+    SumPartials2(part + kP2 * G, part + kP4 * G, G, rr, r_z);
+    const double err = sqrt(rr) / ds[kBnorm];
     if (err <= ds[kTol]) {
       if (writer) {
         ds[kErrPre] = err;
@@ -379,7 +405,6 @@ __global__ __launch_bounds__(kBlk) void PcgDirSpmvA(int n, int par, Rows rows, c
       }
       return;
     }
-    const double r_z = SumPartials(part + kP4 * G, G);
     bool stop = false;
     beta = EndIteration(prev, r_z, err, false, ds, is, hist, stop, writer);
     if (stop) return;
@@ -510,8 +535,7 @@ __global__ __launch_bounds__(kBlk) void PcgUpdateSpmvM(int n, int par, Rows rows
       rz += ri * zs;
     }
   }
-  StorePartial(rr, part + kP2 * G);
-  StorePartial(rz, part + kP4 * G);
+  StorePartial2(rr, rz, part + kP2 * G, part + kP4 * G);
 }
 
 // Host-launched refresh (backward_error.pyx:117-122): r = b - A x; <r, r>.
