@@ -13,6 +13,8 @@ roofline: the dominant kernel of the FP32-bulk step, the fused LOMR pass (one
 iteration), timed with HIP events on the session's stream over one extra build
 after the timed steps; algorithmic bytes per launch are stated in main().
 roofline_pcg_spmv: the PCG's A-SpMV, timed by a standalone replay.
+spgemm: the reference-semantics two-phase SpGEMM C = A A (the product A M
+  with M on A's pattern), GB/s of algorithmic bytes and GFLOP/s.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
@@ -309,6 +311,13 @@ def main():
     if prof2.exists():
         spmv_traffic = json.loads(prof2.read_text()).get("bytes_per_launch")
 
+    # the reference-semantics SpGEMM (symbolic + numeric phases, FP64): C = A A,
+    # i.e. A M with M on A's pattern; algorithmic bytes 3 (n+1) 4 + 12 (nnz(A)
+    # + nnz(A) + nnz(C)), operands resident, mean of 10 products
+    C, spgemm_s = _lib.spgemm(A, A, repeats=10)
+    spgemm_bytes = 3 * (n + 1) * 4 + 12 * (nnz + nnz + C.nnz)
+    spgemm_flops = 2 * int(np.diff(A.indptr)[A.indices].sum())
+
     threads = min(16, os.cpu_count() or 1)
     cpu = None if args.no_cpu_baseline else cpu_baseline(A, M0, b, threads)
 
@@ -343,6 +352,11 @@ def main():
                               "unit": "GB/s", "frac": round(spmv_bytes / (spmv_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
                               "traffic": spmv_traffic, "algorithmic_bytes_per_launch": spmv_bytes,
                               "avg_launch_us": round(spmv_us, 3)},
+        "spgemm": {"kernel": "SpGemm C = A A (symbolic: LDS bitmap tier; numeric: Gustavson in LDS), FP64",
+                   "seconds": round(spgemm_s, 7), "nnz_C": int(C.nnz), "algorithmic_bytes": spgemm_bytes,
+                   "achieved_GBs": round(spgemm_bytes / spgemm_s / 1e9, 1),
+                   "frac": round(spgemm_bytes / spgemm_s / 1e9 / HBM_PEAK_GBS, 4),
+                   "GFLOPs": round(spgemm_flops / spgemm_s / 1e9, 1)},
         "cpu_baseline": cpu,
         "device": _lib.device_name(),
     }

@@ -88,6 +88,18 @@ int64_t spai_pcg(struct CsrData A, struct CsrData M, const double* b, int64_t ma
 /* y = A x on the device; replaces methods_cython/backward_error.pyx:56-66 (matvec). */
 int64_t spai_matvec(struct CsrData A, const double* x, double* y);
 
+/* One product of the reference-semantics path on the device: C = X Y, or
+ * R = I - X Y when residual != 0 (CalculateResidualForBatch), with exact zeros
+ * pruned when prune_zero != 0.  Replaces the per-batch product of
+ * src/common/state_manager.cpp:47-66 (MultiplyBatches) and
+ * src/common/dropping.cpp:97-104: the pattern is the one Eigen's conservative
+ * product gives and the values sum in the same order.  X and Y are CSC with
+ * sorted rows; the result, in the same form, is released by free_mem().
+ * seconds_out (nullable) receives the mean wall time of one product over
+ * `repeats` runs after one warm-up, operands resident on the device. */
+struct Result spai_spgemm(struct MatrixData X, struct MatrixData Y, int64_t residual, int64_t prune_zero,
+                          int64_t repeats, double* seconds_out);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* spai_last_error(void);
 

@@ -83,6 +83,8 @@ def load():
     L.free_mem.argtypes = [ctypes.POINTER(Result)]
     L.spai_run_ex.restype = Result
     L.spai_run_ex.argtypes = [ctypes.c_int64, MatrixData, MatrixData, MatrixData, MethodParams, SpaiExtParams, F64P]
+    L.spai_spgemm.restype = Result
+    L.spai_spgemm.argtypes = [MatrixData, MatrixData, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, F64P]
     L.spai_pcg.restype = ctypes.c_int64
     L.spai_pcg.argtypes = [CsrData, CsrData, F64P, ctypes.c_int64, ctypes.c_double, F64P, I64P, F64P, ctypes.c_int64]
     L.spai_matvec.restype = ctypes.c_int64
@@ -211,6 +213,18 @@ def run(method: str, A, M0=None, Pr=None, max_iterations=100, tolerance=1e-9, ma
     out = result_to_python(res)
     out["device_seconds"] = seconds.value
     return out
+
+
+def spgemm(X, Y, residual=False, prune_zero=False, repeats=1):
+    """C = X Y (or I - X Y) by the device SpGEMM of the reference-semantics
+    path; returns (C as CSC, mean seconds of one product over `repeats`)."""
+    L = load()
+    X_csc, Y_csc = as_csc(X), as_csc(Y)
+    seconds = ctypes.c_double(0.0)
+    res = L.spai_spgemm(matrix_data(X_csc), matrix_data(Y_csc), 1 if residual else 0, 1 if prune_zero else 0,
+                        int(repeats), ctypes.byref(seconds))
+    out = result_to_python(res)
+    return out["M"], seconds.value
 
 
 def pcg(A, M, b, max_iterations, tolerance, m_fp32=False):

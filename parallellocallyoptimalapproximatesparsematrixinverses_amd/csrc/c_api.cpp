@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -14,6 +15,7 @@
 
 #include "engine.hpp"
 #include "fixed.hpp"
+#include "ops.hpp"
 #include "methods_c_api.h"
 #include "pcg.hpp"
 #include "spai_ext.h"
@@ -148,6 +150,39 @@ void free_mem(Result* result) {
 Result spai_run_ex(int64_t method, MatrixData A, MatrixData M0, MatrixData Pr, MethodParams params,
                    SpaiExtParams ext, double* seconds_out) {
   return RunMethod(method, A, M0, Pr, params, &ext, seconds_out);
+}
+
+Result spai_spgemm(MatrixData X, MatrixData Y, int64_t residual, int64_t prune_zero, int64_t repeats,
+                   double* seconds_out) {
+  try {
+    spai::ValidateMatrixData(X);
+    spai::ValidateMatrixData(Y);
+    if (X.cols != Y.rows) throw std::invalid_argument("Sparse matrix dimensions do not match");
+    if (residual && X.rows != Y.cols) throw std::invalid_argument("residual product needs a square result");
+    if (repeats < 1) throw std::invalid_argument("repeats must be positive");
+    spai::Ctx ctx;
+    const spai::DMat x = spai::Upload(ctx, X.rows, X.cols, X.col_offsets, X.row_indices, X.values);
+    const spai::DMat y = spai::Upload(ctx, Y.rows, Y.cols, Y.col_offsets, Y.row_indices, Y.values);
+    spai::DMat c = spai::SpGemm(ctx, x, y, prune_zero != 0, residual != 0);
+    ctx.Sync();
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int64_t r = 0; r < repeats; ++r) c = spai::SpGemm(ctx, x, y, prune_zero != 0, residual != 0);
+    ctx.Sync();
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (seconds_out) *seconds_out = secs / double(repeats);
+    spai::EngineOutput out;
+    out.M.rows = c.rows();
+    out.M.cols = c.cols();
+    out.M.ptr.resize(c.cols() + 1);
+    out.M.idx.resize(c.nnz());
+    out.M.val.resize(c.nnz());
+    spai::Download(ctx, c, out.M.ptr.data(), out.M.idx.data(), out.M.val.data());
+    return SuccessResult(out);
+  } catch (const std::exception& ex) {
+    return ErrorResult(ex.what());
+  } catch (...) {
+    return ErrorResult("Unknown failure.");
+  }
 }
 
 int64_t spai_pcg(CsrData A, CsrData M, const double* b, int64_t max_iterations, double tolerance, double* history,

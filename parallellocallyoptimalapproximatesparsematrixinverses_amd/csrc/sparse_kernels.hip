@@ -125,6 +125,125 @@ __global__ __launch_bounds__(kBlock) void ExpansionSizes(int ncols, const int* _
   }
 }
 
+// Short columns (expansion of at most kShortCap rows from at most kShortGroup
+// columns of X) take kShortGroup lanes per output column, four columns per
+// wavefront, in both phases.
+constexpr int kShortGroup = 16;
+constexpr int kShortCap = 64;  // kShortGroup lanes x 4 registers
+constexpr int kShortPerBlock = 256 / kShortGroup;
+
+__device__ __forceinline__ bool IsShortColumn(int64_t upper, int ny) {
+  return upper <= kShortCap && ny <= kShortGroup;
+}
+
+// Symbolic phase of a short column: lane gl of the column's group holds
+// expansion entries gl + 16 r (r < 4), sorted in registers by a 64-key
+// bitonic network (element i = 16 r + gl: partners 16 apart or more sit in
+// the same lane, closer ones are exchanged by shuffles within the group);
+// then the distinct rows are counted (fill == 0) or written in row order.
+__global__ __launch_bounds__(256) void SymbolicShort(int ncols, const int* __restrict__ xptr,
+                                                    const int* __restrict__ xidx, const int* __restrict__ yptr,
+                                                    const int* __restrict__ yidx, const int64_t* __restrict__ upper,
+                                                    int add_identity, int fill, int* __restrict__ counts,
+                                                    const int* __restrict__ cptr, int* __restrict__ cidx) {
+  constexpr int G = kShortGroup;
+  const int gl = threadIdx.x % G;
+  const int j = blockIdx.x * kShortPerBlock + threadIdx.x / G;
+  if (j >= ncols) return;  // whole groups leave together
+  const int y0 = yptr[j], ny = yptr[j + 1] - y0;
+  if (!IsShortColumn(upper[j], ny)) return;
+  // the group's Y entries: lane gl takes entry gl (X column start and length)
+  int b = 0, c = 0;
+  if (gl < ny) {
+    const int k = yidx[y0 + gl];
+    b = xptr[k];
+    c = xptr[k + 1] - b;
+  }
+  int incl = c;
+#pragma unroll
+  for (int off = 1; off < G; off <<= 1) {
+    const int t = __shfl_up(incl, off, G);
+    if (gl >= off) incl += t;
+  }
+  const int tot = __shfl(incl, G - 1, G);
+  const int pre = incl - c;
+  int key[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int e = gl + G * r;
+    // source lane: the last lane whose first entry is <= e
+    int lo = 0;
+#pragma unroll
+    for (int step = G / 2; step > 0; step >>= 1) {
+      const int mid = lo + step;
+      if (__shfl(pre, mid, G) <= e && mid < ny) lo = mid;
+    }
+    const int sb = __shfl(b, lo, G), sp = __shfl(pre, lo, G);
+    key[r] = INT_MAX;
+    if (e < tot)
+      key[r] = xidx[sb + (e - sp)];
+    else if (add_identity && e == tot)
+      key[r] = j;
+  }
+  // bitonic sort of the 64 keys, ascending
+#pragma unroll
+  for (int kk = 2; kk <= 64; kk <<= 1) {
+#pragma unroll
+    for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+      if (jj >= G) {
+        const int rj = jj / G;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if ((r & rj) == 0) {
+            const int i = G * r + gl;
+            const bool asc = (i & kk) == 0;
+            const int a = key[r], o = key[r | rj];
+            key[r] = asc ? min(a, o) : max(a, o);
+            key[r | rj] = asc ? max(a, o) : min(a, o);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = G * r + gl;
+          const bool asc = (i & kk) == 0;
+          const bool lower = (gl & jj) == 0;
+          const int o = __shfl_xor(key[r], jj, G);
+          key[r] = (lower == asc) ? min(key[r], o) : max(key[r], o);
+        }
+      }
+    }
+  }
+  // distinct rows: element i is new when it differs from element i - 1
+  int first[4], cnt = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    int prev = __shfl_up(key[r], 1, G);
+    const int last_prev = r > 0 ? __shfl(key[r - 1], G - 1, G) : INT_MIN;
+    if (gl == 0) prev = last_prev;
+    first[r] = key[r] != INT_MAX && (G * r + gl == 0 || key[r] != prev) ? 1 : 0;
+    cnt += first[r];
+  }
+  if (!fill) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, G);
+    if (gl == 0) counts[j] = cnt;
+    return;
+  }
+  int base = cptr[j];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    int sc = first[r];
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+      const int t = __shfl_up(sc, off, G);
+      if (gl >= off) sc += t;
+    }
+    if (first[r]) cidx[base + sc - 1] = key[r];
+    base += __shfl(sc, G - 1, G);
+  }
+}
+
 // Symbolic phase, one wavefront per output column whose expansion fits the
 // LDS tier: counts (fill == 0) or writes in row order (fill == 1) the
 // distinct rows of X(:, k) over k in Y(:, j) (and j itself when add_identity).
@@ -154,7 +273,7 @@ __global__ __launch_bounds__(kSymWaves* kWave) void SymbolicEsc(
   const int w = WaveInBlock(), lane = LaneId();
   const int j = blockIdx.x * kSymWaves + w;
   if (j >= ncols) return;
-  if (upper[j] > kEscCap) return;
+  if (upper[j] > kEscCap || IsShortColumn(upper[j], yptr[j + 1] - yptr[j])) return;
   int* K = keys[w];
   const int ye = yptr[j + 1];
   // row range of the expansion
@@ -333,12 +452,13 @@ __global__ __launch_bounds__(kNumWaves* kWave) void NumericGustavson(
     int ncols, const int* __restrict__ xptr, const int* __restrict__ xidx, const double* __restrict__ xval,
     const int* __restrict__ yptr, const int* __restrict__ yidx, const double* __restrict__ yval,
     const int* __restrict__ cptr, const int* __restrict__ cidx, double* __restrict__ cval, int residual,
-    int* __restrict__ error_flag) {
+    const int64_t* __restrict__ upper, int* __restrict__ error_flag) {
   __shared__ int rows_s[kNumWaves][kNumCap];
   __shared__ double acc_s[kNumWaves][kNumCap];
   const int w = WaveInBlock(), lane = LaneId();
   const int j = blockIdx.x * kNumWaves + w;
   if (j >= ncols) return;
+  if (IsShortColumn(upper[j], yptr[j + 1] - yptr[j])) return;
   const int cb = cptr[j];
   const int L = cptr[j + 1] - cb;
   const bool in_lds = L <= kNumCap;
@@ -373,6 +493,65 @@ __global__ __launch_bounds__(kNumWaves* kWave) void NumericGustavson(
   }
   for (int e = lane; e < L; e += kWave) {
     double v = in_lds ? acc_s[w][e] : __hip_atomic_load(&cval[cb + e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (residual) {
+      v = -v;
+      if (R[e] == j) v += 1.0;
+    }
+    cval[cb + e] = v;
+  }
+}
+
+// Numeric phase of a short column (kShortGroup lanes, at most kShortCap output
+// rows): the same contributions in the same order as NumericGustavson -- Y's
+// entries k ascending, X(:, k) scattered into the column's LDS accumulator,
+// distinct rows within one k -- so the values are identical.
+__global__ __launch_bounds__(256) void NumericShort(
+    int ncols, const int* __restrict__ xptr, const int* __restrict__ xidx, const double* __restrict__ xval,
+    const int* __restrict__ yptr, const int* __restrict__ yidx, const double* __restrict__ yval,
+    const int* __restrict__ cptr, const int* __restrict__ cidx, double* __restrict__ cval, int residual,
+    const int64_t* __restrict__ upper, int* __restrict__ error_flag) {
+  constexpr int G = kShortGroup;
+  __shared__ int rows_s[kShortPerBlock][kShortCap];
+  __shared__ double acc_s[kShortPerBlock][kShortCap];
+  const int gl = threadIdx.x % G, grp = threadIdx.x / G;
+  const int j = blockIdx.x * kShortPerBlock + grp;
+  if (j >= ncols) return;
+  const int y0 = yptr[j], ny = yptr[j + 1] - y0;
+  if (!IsShortColumn(upper[j], ny)) return;
+  const int cb = cptr[j];
+  const int L = cptr[j + 1] - cb;  // <= kShortCap: at most the expansion
+  int* const R = rows_s[grp];
+  double* const acc = acc_s[grp];
+  for (int e = gl; e < L; e += G) {
+    R[e] = cidx[cb + e];
+    acc[e] = 0.0;
+  }
+  int k = 0, b = 0, c = 0;
+  double y = 0.0;
+  if (gl < ny) {
+    k = yidx[y0 + gl];
+    y = yval[y0 + gl];
+    b = xptr[k];
+    c = xptr[k + 1] - b;
+  }
+  __builtin_amdgcn_wave_barrier();
+  for (int s = 0; s < ny; ++s) {
+    const int sb = __shfl(b, s, G), sc = __shfl(c, s, G);
+    const double sy = __shfl(y, s, G);
+    for (int u = gl; u < sc; u += G) {
+      const int i = xidx[sb + u];
+      const double x = xval[sb + u];
+      const int pos = LowerBound(R, 0, L, i);
+      if (pos >= L || R[pos] != i) {
+        atomicOr(error_flag, 1);
+        continue;
+      }
+      acc[pos] += x * sy;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  for (int e = gl; e < L; e += G) {
+    double v = acc[e];
     if (residual) {
       v = -v;
       if (R[e] == j) v += 1.0;
@@ -1308,6 +1487,9 @@ DMat SpGemm(Ctx& ctx, const DMat& X, const DMat& Y, bool prune_zero, bool residu
   DMat C;
   C.pat = MakePattern(ctx, nrows, n, 0);
   const unsigned sym_blocks = Blocks(n, kSymWaves);
+  const unsigned short_blocks = Blocks(n, kShortPerBlock);
+  SymbolicShort<<<short_blocks, 256, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), Y.ptr(), Y.idx(), upper.get(),
+                                                      residual ? 1 : 0, 0, counts.get(), nullptr, nullptr);
   SymbolicEsc<<<sym_blocks, kSymWaves * kWave, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), Y.ptr(), Y.idx(),
                                                                 upper.get(), residual ? 1 : 0, 0, counts.get(),
                                                                 nullptr, nullptr);
@@ -1326,6 +1508,9 @@ DMat SpGemm(Ctx& ctx, const DMat& X, const DMat& Y, bool prune_zero, bool residu
   C.pat->nnz = total;
   C.pat->idx = ctx.Alloc<int>(total > 0 ? total : 1);
   C.val = ctx.Alloc<double>(total > 0 ? total : 1);
+  SymbolicShort<<<short_blocks, 256, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), Y.ptr(), Y.idx(), upper.get(),
+                                                      residual ? 1 : 0, 1, nullptr, C.pat->ptr.get(),
+                                                      C.pat->idx.get());
   SymbolicEsc<<<sym_blocks, kSymWaves * kWave, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), Y.ptr(), Y.idx(),
                                                                 upper.get(), residual ? 1 : 0, 1, nullptr,
                                                                 C.pat->ptr.get(), C.pat->idx.get());
@@ -1337,9 +1522,12 @@ DMat SpGemm(Ctx& ctx, const DMat& X, const DMat& Y, bool prune_zero, bool residu
   }
   auto err = ctx.Alloc<int>(1);
   SPAI_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), ctx.stream));
+  NumericShort<<<short_blocks, 256, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), X.v(), Y.ptr(), Y.idx(), Y.v(),
+                                                     C.pat->ptr.get(), C.pat->idx.get(), C.val.get(),
+                                                     residual ? 1 : 0, upper.get(), err.get());
   NumericGustavson<<<Blocks(n, kNumWaves), kNumWaves * kWave, 0, ctx.stream>>>(
       (int)n, X.ptr(), X.idx(), X.v(), Y.ptr(), Y.idx(), Y.v(), C.pat->ptr.get(), C.pat->idx.get(), C.val.get(),
-      residual ? 1 : 0, err.get());
+      residual ? 1 : 0, upper.get(), err.get());
   int e = 0;
   ctx.ReadBack(&e, err.get(), sizeof(int));
   if (e) throw std::runtime_error("SpGEMM numeric phase found a row outside the symbolic pattern");

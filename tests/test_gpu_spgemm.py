@@ -1,0 +1,76 @@
+"""The device SpGEMM of the reference-semantics path (spai_spgemm) against the
+oracle's conservative product (src/common/state_manager.cpp:47-66, Eigen's
+conservative sparse product; src/common/dropping.cpp:97-104 for the residual
+form): the same pattern, and the same values bit for bit, since both sum the
+contributions of each row in k-ascending order.
+
+The cases reach each tier of the symbolic phase: the LDS bitmap (a column's
+rows span fewer than 131,072 rows), the LDS sort (wider span, expansion of at
+most 4096 rows) and the stamp array (longer expansion)."""
+import numpy as np
+import pytest
+from scipy import sparse
+
+from parallellocallyoptimalapproximatesparsematrixinverses_amd import problems
+
+pytestmark = pytest.mark.gpu
+
+
+def _random(rows, cols, per_col, seed):
+    rng = np.random.default_rng(seed)
+    r = np.concatenate([rng.choice(rows, size=per_col, replace=False) for _ in range(cols)])
+    c = np.repeat(np.arange(cols), per_col)
+    v = rng.standard_normal(r.size)
+    M = sparse.csc_matrix((v, (r, c)), shape=(rows, cols))
+    M.sum_duplicates()
+    M.sort_indices()
+    return M
+
+
+def _check(gpu_lib, oracle_lib, X, Y, residual=False, prune=False):
+    C, secs = gpu_lib.spgemm(X, Y, residual=residual, prune_zero=prune, repeats=2)
+    R = oracle_lib.product(X, Y, residual=residual, prune_zero=prune)
+    np.testing.assert_array_equal(C.indptr, R.indptr)
+    np.testing.assert_array_equal(C.indices, R.indices)
+    np.testing.assert_array_equal(C.data, R.data)
+    assert secs > 0.0
+
+
+@pytest.mark.parametrize("residual,prune", [(False, False), (True, False), (True, True), (False, True)])
+def test_spgemm_stencil_bitmap_tier(gpu_lib, oracle_lib, residual, prune):
+    A = problems.diffusion3d(16, seed=1)
+    print(f"case: spgemm diffusion16 residual={residual} prune={prune}", flush=True)
+    _check(gpu_lib, oracle_lib, A, A, residual, prune)
+
+
+def test_spgemm_random_sort_tier(gpu_lib, oracle_lib):
+    # rows spread over 200,000: the span exceeds the bitmap, expansions stay small
+    X = _random(200_000, 400, 40, seed=5)
+    Y = _random(400, 300, 12, seed=6)
+    print("case: spgemm random 200000x400 * 400x300", flush=True)
+    _check(gpu_lib, oracle_lib, X, Y)
+
+
+def test_spgemm_long_expansion_stamp_tier(gpu_lib, oracle_lib):
+    # 16 columns of 600 rows each per output column: expansion 9,600 > 4096
+    X = _random(20_000, 64, 600, seed=7)
+    Y = _random(64, 40, 16, seed=8)
+    print("case: spgemm random long expansion", flush=True)
+    _check(gpu_lib, oracle_lib, X, Y)
+
+
+def test_spgemm_empty_columns_and_cancellation(gpu_lib, oracle_lib):
+    # an empty column of Y, and two terms that cancel exactly (pruned when asked)
+    X = sparse.csc_matrix(np.array([[1.0, 1.0, 0.0], [2.0, -2.0, 0.0], [0.0, 0.0, 3.0]]))
+    Y = sparse.csc_matrix(np.array([[1.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0]]))
+    for prune in (False, True):
+        _check(gpu_lib, oracle_lib, X, Y, prune=prune)
+    C, _ = gpu_lib.spgemm(X, Y, prune_zero=True)
+    assert C.nnz == 2  # (0,0) = 2 and (2,2) = 3; (1,0) = 0 is pruned
+
+
+def test_spgemm_dimension_errors(gpu_lib):
+    X = sparse.csc_matrix(np.eye(3))
+    Y = sparse.csc_matrix(np.eye(4))
+    with pytest.raises(RuntimeError, match="dimensions do not match"):
+        gpu_lib.spgemm(X, Y)
