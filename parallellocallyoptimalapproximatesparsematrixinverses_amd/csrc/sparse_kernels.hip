@@ -105,26 +105,6 @@ constexpr int kSymWaves = 2;      // wavefronts per block in the symbolic kernel
 constexpr int kNumCap = 1024;     // output rows per column held in LDS (numeric)
 constexpr int kNumWaves = 4;
 
-// u_j = sum over Y(:, j) of nnz(X(:, k)) (+1 for the identity); columns whose
-// expansion exceeds the LDS tier go to `big_list`.
-__global__ __launch_bounds__(kBlock) void ExpansionSizes(int ncols, const int* __restrict__ xptr,
-                                                         const int* __restrict__ yptr, const int* __restrict__ yidx,
-                                                         int add_identity, int64_t* __restrict__ upper,
-                                                         int* __restrict__ big_list, int* __restrict__ big_count) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ncols) return;
-  int64_t u = add_identity ? 1 : 0;
-  for (int p = yptr[j]; p < yptr[j + 1]; ++p) {
-    const int k = yidx[p];
-    u += xptr[k + 1] - xptr[k];
-  }
-  upper[j] = u;
-  if (u > kEscCap) {
-    const int slot = atomicAdd(big_count, 1);
-    big_list[slot] = j;
-  }
-}
-
 // Short columns (expansion of at most kShortCap rows from at most kShortGroup
 // columns of X) take kShortGroup lanes per output column, four columns per
 // wavefront, in both phases.
@@ -136,22 +116,88 @@ __device__ __forceinline__ bool IsShortColumn(int64_t upper, int ny) {
   return upper <= kShortCap && ny <= kShortGroup;
 }
 
+// Block-wide ordered compaction helper: returns this thread's exclusive rank
+// among flagged threads and writes the block total to *total.
+__device__ int BlockRank(bool flag, int* wcount /* LDS >= 16 */, int* total) {
+  const uint64_t mask = __ballot(flag);
+  const int w = WaveInBlock();
+  const int nw = blockDim.x / kWave;
+  if (LaneId() == 0) wcount[w] = __popcll(mask);
+  __syncthreads();
+  int base = 0, t = 0;
+  for (int i = 0; i < nw; ++i) {
+    if (i < w) base += wcount[i];
+    t += wcount[i];
+  }
+  *total = t;
+  __syncthreads();
+  return base + LanePrefixCount(mask);
+}
+
+// Appends `value` of every flagged thread to list[*count ...] with one atomic
+// per block; threads keep their order within the block.  Called by all
+// threads of the block.
+__device__ void BlockAppend(bool flag, int value, int* __restrict__ list, int* __restrict__ count,
+                            int* wcount /* LDS >= 16 */, int* base_s /* LDS */) {
+  int tot = 0;
+  const int rank = BlockRank(flag, wcount, &tot);
+  if (threadIdx.x == 0 && tot > 0) *base_s = atomicAdd(count, tot);
+  __syncthreads();
+  if (flag) list[*base_s + rank] = value;
+  __syncthreads();
+}
+
+// u_j = sum over Y(:, j) of nnz(X(:, k)) (+1 for the identity), and the
+// column's tier: short columns go to lists[0], the others to lists[1] (the
+// wavefront-per-column kernels) and, when the expansion exceeds the LDS
+// tier, also to lists[2] (the stamp kernel).  counts[t] is list t's length.
+constexpr int kExpBlock = 1024;
+__global__ __launch_bounds__(kExpBlock) void ExpansionSizes(int ncols, const int* __restrict__ xptr,
+                                                            const int* __restrict__ yptr,
+                                                            const int* __restrict__ yidx, int add_identity,
+                                                            int64_t* __restrict__ upper, int* __restrict__ short_list,
+                                                            int* __restrict__ long_list, int* __restrict__ big_list,
+                                                            int* __restrict__ counts) {
+  __shared__ int wcount[kExpBlock / kWave];
+  __shared__ int base_s;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = j < ncols;
+  int64_t u = add_identity ? 1 : 0;
+  int ny = 0;
+  if (live) {
+    ny = yptr[j + 1] - yptr[j];
+    for (int p = yptr[j]; p < yptr[j + 1]; ++p) {
+      const int k = yidx[p];
+      u += xptr[k + 1] - xptr[k];
+    }
+    upper[j] = u;
+  }
+  const bool is_short = live && IsShortColumn(u, ny);
+  BlockAppend(is_short, j, short_list, counts + 0, wcount, &base_s);
+  BlockAppend(live && !is_short, j, long_list, counts + 1, wcount, &base_s);
+  BlockAppend(live && u > kEscCap, j, big_list, counts + 2, wcount, &base_s);
+}
+
 // Symbolic phase of a short column: lane gl of the column's group holds
 // expansion entries gl + 16 r (r < 4), sorted in registers by a 64-key
 // bitonic network (element i = 16 r + gl: partners 16 apart or more sit in
 // the same lane, closer ones are exchanged by shuffles within the group);
-// then the distinct rows are counted (fill == 0) or written in row order.
-__global__ __launch_bounds__(256) void SymbolicShort(int ncols, const int* __restrict__ xptr,
-                                                    const int* __restrict__ xidx, const int* __restrict__ yptr,
-                                                    const int* __restrict__ yidx, const int64_t* __restrict__ upper,
+// then the distinct rows are counted (fill == 0) or written in row order
+// (fill == 1).  With fill == 0 and a stash, the distinct rows also go to
+// stash[g kShortCap ...] (g = the column's list position), so that CopyShort
+// can place them once the column offsets are known.
+__global__ __launch_bounds__(256) void SymbolicShort(int nlist, const int* __restrict__ list,
+                                                    const int* __restrict__ xptr, const int* __restrict__ xidx,
+                                                    const int* __restrict__ yptr, const int* __restrict__ yidx,
                                                     int add_identity, int fill, int* __restrict__ counts,
-                                                    const int* __restrict__ cptr, int* __restrict__ cidx) {
+                                                    const int* __restrict__ cptr, int* __restrict__ cidx,
+                                                    int* __restrict__ stash) {
   constexpr int G = kShortGroup;
   const int gl = threadIdx.x % G;
-  const int j = blockIdx.x * kShortPerBlock + threadIdx.x / G;
-  if (j >= ncols) return;  // whole groups leave together
+  const int g = blockIdx.x * kShortPerBlock + threadIdx.x / G;
+  if (g >= nlist) return;  // whole groups leave together
+  const int j = list[g];
   const int y0 = yptr[j], ny = yptr[j + 1] - y0;
-  if (!IsShortColumn(upper[j], ny)) return;
   // the group's Y entries: lane gl takes entry gl (X column start and length)
   int b = 0, c = 0;
   if (gl < ny) {
@@ -224,13 +270,14 @@ __global__ __launch_bounds__(256) void SymbolicShort(int ncols, const int* __res
     first[r] = key[r] != INT_MAX && (G * r + gl == 0 || key[r] != prev) ? 1 : 0;
     cnt += first[r];
   }
-  if (!fill) {
+  if (!fill && stash == nullptr) {
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, G);
     if (gl == 0) counts[j] = cnt;
     return;
   }
-  int base = cptr[j];
+  int* const out = fill ? cidx + cptr[j] : stash + int64_t(g) * kShortCap;
+  int base = 0;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     int sc = first[r];
@@ -239,9 +286,25 @@ __global__ __launch_bounds__(256) void SymbolicShort(int ncols, const int* __res
       const int t = __shfl_up(sc, off, G);
       if (gl >= off) sc += t;
     }
-    if (first[r]) cidx[base + sc - 1] = key[r];
+    if (first[r]) out[base + sc - 1] = key[r];
     base += __shfl(sc, G - 1, G);
   }
+  if (!fill && gl == 0) counts[j] = base;
+}
+
+// Fill phase of the short columns after a stashing count phase: copies each
+// column's distinct rows from the stash to its place in C.
+__global__ __launch_bounds__(256) void CopyShort(int nlist, const int* __restrict__ list,
+                                                const int* __restrict__ stash, const int* __restrict__ cptr,
+                                                int* __restrict__ cidx) {
+  constexpr int G = kShortGroup;
+  const int gl = threadIdx.x % G;
+  const int g = blockIdx.x * kShortPerBlock + threadIdx.x / G;
+  if (g >= nlist) return;
+  const int j = list[g];
+  const int cb = cptr[j], L = cptr[j + 1] - cb;
+  const int* const src = stash + int64_t(g) * kShortCap;
+  for (int e = gl; e < L; e += G) cidx[cb + e] = src[e];
 }
 
 // Symbolic phase, one wavefront per output column whose expansion fits the
@@ -264,16 +327,17 @@ __device__ __forceinline__ int WaveMaxInt(int v) {
 }
 
 __global__ __launch_bounds__(kSymWaves* kWave) void SymbolicEsc(
-    int ncols, const int* __restrict__ xptr, const int* __restrict__ xidx, const int* __restrict__ yptr,
+    int nlist, const int* __restrict__ list, const int* __restrict__ xptr, const int* __restrict__ xidx, const int* __restrict__ yptr,
     const int* __restrict__ yidx, const int64_t* __restrict__ upper, int add_identity, int fill,
     int* __restrict__ counts, const int* __restrict__ cptr, int* __restrict__ cidx) {
   __shared__ int keys[kSymWaves][kEscCap];
   __shared__ int pre[kSymWaves][kWave];
   __shared__ int beg[kSymWaves][kWave];
   const int w = WaveInBlock(), lane = LaneId();
-  const int j = blockIdx.x * kSymWaves + w;
-  if (j >= ncols) return;
-  if (upper[j] > kEscCap || IsShortColumn(upper[j], yptr[j + 1] - yptr[j])) return;
+  const int g = blockIdx.x * kSymWaves + w;
+  if (g >= nlist) return;
+  const int j = list[g];
+  if (upper[j] > kEscCap) return;  // the stamp kernel's column
   int* K = keys[w];
   const int ye = yptr[j + 1];
   // row range of the expansion
@@ -393,24 +457,6 @@ __global__ __launch_bounds__(kSymWaves* kWave) void SymbolicEsc(
   }
 }
 
-// Block-wide ordered compaction helper: returns this thread's exclusive rank
-// among flagged threads and writes the block total to *total.
-__device__ int BlockRank(bool flag, int* wcount /* LDS >= 16 */, int* total) {
-  const uint64_t mask = __ballot(flag);
-  const int w = WaveInBlock();
-  const int nw = blockDim.x / kWave;
-  if (LaneId() == 0) wcount[w] = __popcll(mask);
-  __syncthreads();
-  int base = 0, t = 0;
-  for (int i = 0; i < nw; ++i) {
-    if (i < w) base += wcount[i];
-    t += wcount[i];
-  }
-  *total = t;
-  __syncthreads();
-  return base + LanePrefixCount(mask);
-}
-
 // Symbolic phase for columns whose expansion exceeds the LDS tier: a block per
 // column marks rows in a per-block stamp array (length nrows, initialised to
 // -1) and compacts them in row order.
@@ -449,16 +495,16 @@ __global__ __launch_bounds__(kBlock) void SymbolicStamp(int nlist, const int* __
 // the column's accumulator (distinct rows inside one k, so the lanes never
 // collide).  residual: value = -(sum) (+1 on the diagonal).
 __global__ __launch_bounds__(kNumWaves* kWave) void NumericGustavson(
-    int ncols, const int* __restrict__ xptr, const int* __restrict__ xidx, const double* __restrict__ xval,
-    const int* __restrict__ yptr, const int* __restrict__ yidx, const double* __restrict__ yval,
-    const int* __restrict__ cptr, const int* __restrict__ cidx, double* __restrict__ cval, int residual,
-    const int64_t* __restrict__ upper, int* __restrict__ error_flag) {
+    int nlist, const int* __restrict__ list, const int* __restrict__ xptr, const int* __restrict__ xidx,
+    const double* __restrict__ xval, const int* __restrict__ yptr, const int* __restrict__ yidx,
+    const double* __restrict__ yval, const int* __restrict__ cptr, const int* __restrict__ cidx,
+    double* __restrict__ cval, int residual, int* __restrict__ error_flag) {
   __shared__ int rows_s[kNumWaves][kNumCap];
   __shared__ double acc_s[kNumWaves][kNumCap];
   const int w = WaveInBlock(), lane = LaneId();
-  const int j = blockIdx.x * kNumWaves + w;
-  if (j >= ncols) return;
-  if (IsShortColumn(upper[j], yptr[j + 1] - yptr[j])) return;
+  const int g = blockIdx.x * kNumWaves + w;
+  if (g >= nlist) return;
+  const int j = list[g];
   const int cb = cptr[j];
   const int L = cptr[j + 1] - cb;
   const bool in_lds = L <= kNumCap;
@@ -502,22 +548,25 @@ __global__ __launch_bounds__(kNumWaves* kWave) void NumericGustavson(
 }
 
 // Numeric phase of a short column (kShortGroup lanes, at most kShortCap output
-// rows): the same contributions in the same order as NumericGustavson -- Y's
-// entries k ascending, X(:, k) scattered into the column's LDS accumulator,
-// distinct rows within one k -- so the values are identical.
+// rows): the same contributions in the same order as NumericGustavson -- each
+// output row's products x(i, k) y(k, j) added to 0 in ascending k -- so the
+// values are identical.  Expansion entry e = gl + 16 r (lane gl, register r)
+// is loaded and placed up front; the accumulation then walks Y's entries s in
+// order, adding the entries of X(:, k_s) (distinct rows, so no two lanes of a
+// step share an accumulator).
 __global__ __launch_bounds__(256) void NumericShort(
-    int ncols, const int* __restrict__ xptr, const int* __restrict__ xidx, const double* __restrict__ xval,
-    const int* __restrict__ yptr, const int* __restrict__ yidx, const double* __restrict__ yval,
-    const int* __restrict__ cptr, const int* __restrict__ cidx, double* __restrict__ cval, int residual,
-    const int64_t* __restrict__ upper, int* __restrict__ error_flag) {
+    int nlist, const int* __restrict__ list, const int* __restrict__ xptr, const int* __restrict__ xidx,
+    const double* __restrict__ xval, const int* __restrict__ yptr, const int* __restrict__ yidx,
+    const double* __restrict__ yval, const int* __restrict__ cptr, const int* __restrict__ cidx,
+    double* __restrict__ cval, int residual, int* __restrict__ error_flag) {
   constexpr int G = kShortGroup;
   __shared__ int rows_s[kShortPerBlock][kShortCap];
   __shared__ double acc_s[kShortPerBlock][kShortCap];
   const int gl = threadIdx.x % G, grp = threadIdx.x / G;
-  const int j = blockIdx.x * kShortPerBlock + grp;
-  if (j >= ncols) return;
+  const int g = blockIdx.x * kShortPerBlock + grp;
+  if (g >= nlist) return;  // whole groups leave together
+  const int j = list[g];
   const int y0 = yptr[j], ny = yptr[j + 1] - y0;
-  if (!IsShortColumn(upper[j], ny)) return;
   const int cb = cptr[j];
   const int L = cptr[j + 1] - cb;  // <= kShortCap: at most the expansion
   int* const R = rows_s[grp];
@@ -526,27 +575,62 @@ __global__ __launch_bounds__(256) void NumericShort(
     R[e] = cidx[cb + e];
     acc[e] = 0.0;
   }
-  int k = 0, b = 0, c = 0;
+  // lane gl holds Y entry gl: X column start and length, and y
+  int b = 0, c = 0;
   double y = 0.0;
   if (gl < ny) {
-    k = yidx[y0 + gl];
+    const int k = yidx[y0 + gl];
     y = yval[y0 + gl];
     b = xptr[k];
     c = xptr[k + 1] - b;
   }
+  int incl = c;
+#pragma unroll
+  for (int off = 1; off < G; off <<= 1) {
+    const int t = __shfl_up(incl, off, G);
+    if (gl >= off) incl += t;
+  }
+  const int tot = __shfl(incl, G - 1, G);
+  const int pre = incl - c;
+  int src[4], row[4];
+  double prod[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int e = gl + G * r;
+    int lo = 0;  // the last Y entry whose first expansion entry is <= e
+#pragma unroll
+    for (int step = G / 2; step > 0; step >>= 1) {
+      const int mid = lo + step;
+      if (__shfl(pre, mid, G) <= e && mid < ny) lo = mid;
+    }
+    const int sb = __shfl(b, lo, G), sp = __shfl(pre, lo, G);
+    const double sy = __shfl(y, lo, G);
+    src[r] = -1;
+    row[r] = 0;
+    prod[r] = 0.0;
+    if (e < tot) {
+      src[r] = lo;
+      row[r] = xidx[sb + (e - sp)];
+      prod[r] = xval[sb + (e - sp)] * sy;
+    }
+  }
   __builtin_amdgcn_wave_barrier();
-  for (int s = 0; s < ny; ++s) {
-    const int sb = __shfl(b, s, G), sc = __shfl(c, s, G);
-    const double sy = __shfl(y, s, G);
-    for (int u = gl; u < sc; u += G) {
-      const int i = xidx[sb + u];
-      const double x = xval[sb + u];
-      const int pos = LowerBound(R, 0, L, i);
-      if (pos >= L || R[pos] != i) {
+  int pos[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    pos[r] = 0;
+    if (src[r] >= 0) {
+      pos[r] = LowerBound(R, 0, L, row[r]);
+      if (pos[r] >= L || R[pos[r]] != row[r]) {
         atomicOr(error_flag, 1);
-        continue;
+        src[r] = -1;
       }
-      acc[pos] += x * sy;
+    }
+  }
+  for (int s = 0; s < ny; ++s) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (src[r] == s) acc[pos[r]] += prod[r];
     }
     __builtin_amdgcn_wave_barrier();
   }
@@ -1475,59 +1559,73 @@ DMat SpGemm(Ctx& ctx, const DMat& X, const DMat& Y, bool prune_zero, bool residu
   const int64_t nrows = X.rows();
   if (residual && nrows != n) throw std::invalid_argument("residual product needs a square result");
   auto upper = ctx.Alloc<int64_t>(n > 0 ? n : 1);
-  auto big_list = ctx.Alloc<int>(n > 0 ? n : 1);
-  auto big_count = ctx.Alloc<int>(1);
-  SPAI_HIP_CHECK(hipMemsetAsync(big_count.get(), 0, sizeof(int), ctx.stream));
-  ExpansionSizes<<<Blocks(n), kBlock, 0, ctx.stream>>>((int)n, X.ptr(), Y.ptr(), Y.idx(), residual ? 1 : 0,
-                                                       upper.get(), big_list.get(), big_count.get());
-  int nbig = 0;
-  ctx.ReadBack(&nbig, big_count.get(), sizeof(int));
+  auto lists = ctx.Alloc<int>(3 * (n > 0 ? n : 1));  // short, long, big column lists
+  int* const short_list = lists.get();
+  int* const long_list = short_list + (n > 0 ? n : 1);
+  int* const big_list = long_list + (n > 0 ? n : 1);
+  auto list_counts = ctx.Alloc<int>(3);
+  SPAI_HIP_CHECK(hipMemsetAsync(list_counts.get(), 0, 3 * sizeof(int), ctx.stream));
+  if (n > 0)
+    ExpansionSizes<<<Blocks(n, kExpBlock), kExpBlock, 0, ctx.stream>>>((int)n, X.ptr(), Y.ptr(), Y.idx(), residual ? 1 : 0,
+                                                         upper.get(), short_list, long_list, big_list,
+                                                         list_counts.get());
+  int nlist[3] = {0, 0, 0};
+  ctx.ReadBack(nlist, list_counts.get(), sizeof(nlist));
+  const int nshort = nlist[0], nlong = nlist[1], nbig = nlist[2];
 
   auto counts = ctx.Alloc<int>(n > 0 ? n : 1);
   DMat C;
   C.pat = MakePattern(ctx, nrows, n, 0);
-  const unsigned sym_blocks = Blocks(n, kSymWaves);
-  const unsigned short_blocks = Blocks(n, kShortPerBlock);
-  SymbolicShort<<<short_blocks, 256, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), Y.ptr(), Y.idx(), upper.get(),
-                                                      residual ? 1 : 0, 0, counts.get(), nullptr, nullptr);
-  SymbolicEsc<<<sym_blocks, kSymWaves * kWave, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), Y.ptr(), Y.idx(),
-                                                                upper.get(), residual ? 1 : 0, 0, counts.get(),
-                                                                nullptr, nullptr);
+  // Each tier is launched over its own column list, so a tier with no
+  // columns costs nothing.
+  // The short columns' count phase stashes its sorted rows (kShortCap ints a
+  // column, when that stays under 1 GiB) and the fill phase copies them.
+  const bool stash_short = nshort > 0 && int64_t(nshort) * kShortCap <= (int64_t(1) << 28);
+  DBuf<int> short_stash;
+  if (stash_short) short_stash = ctx.Alloc<int>(size_t(nshort) * kShortCap);
+  auto symbolic = [&](int fill, int* cnt, const int* cptr, int* cidx, DBuf<int>& stamp, unsigned stamp_blocks) {
+    if (nshort > 0 && fill && stash_short)
+      CopyShort<<<Blocks(nshort, kShortPerBlock), 256, 0, ctx.stream>>>(nshort, short_list, short_stash.get(),
+                                                                          cptr, cidx);
+    else if (nshort > 0)
+      SymbolicShort<<<Blocks(nshort, kShortPerBlock), 256, 0, ctx.stream>>>(
+          nshort, short_list, X.ptr(), X.idx(), Y.ptr(), Y.idx(), residual ? 1 : 0, fill, cnt, cptr, cidx,
+          stash_short ? short_stash.get() : nullptr);
+    if (nlong > nbig)
+      SymbolicEsc<<<Blocks(nlong, kSymWaves), kSymWaves * kWave, 0, ctx.stream>>>(
+          nlong, long_list, X.ptr(), X.idx(), Y.ptr(), Y.idx(), upper.get(), residual ? 1 : 0, fill, cnt, cptr,
+          cidx);
+    if (nbig > 0) {
+      SPAI_HIP_CHECK(
+          hipMemsetAsync(stamp.get(), 0xFF, sizeof(int) * size_t(stamp_blocks) * size_t(nrows), ctx.stream));
+      SymbolicStamp<<<stamp_blocks, kBlock, 0, ctx.stream>>>(nbig, big_list, (int)nrows, X.ptr(), X.idx(), Y.ptr(),
+                                                             Y.idx(), residual ? 1 : 0, fill, stamp.get(), cnt,
+                                                             cptr, cidx);
+    }
+  };
   DBuf<int> stamp;
   unsigned stamp_blocks = 0;
   if (nbig > 0) {
     const int64_t max_ws = (int64_t(1) << 28) / std::max<int64_t>(nrows, 1);  // <= 1 GiB of stamps
     stamp_blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>({(int64_t)nbig, max_ws, 1024}));
     stamp = ctx.Alloc<int>(size_t(stamp_blocks) * size_t(nrows));
-    SPAI_HIP_CHECK(hipMemsetAsync(stamp.get(), 0xFF, sizeof(int) * size_t(stamp_blocks) * size_t(nrows), ctx.stream));
-    SymbolicStamp<<<stamp_blocks, kBlock, 0, ctx.stream>>>(nbig, big_list.get(), (int)nrows, X.ptr(), X.idx(),
-                                                           Y.ptr(), Y.idx(), residual ? 1 : 0, 0, stamp.get(),
-                                                           counts.get(), nullptr, nullptr);
   }
+  symbolic(0, counts.get(), nullptr, nullptr, stamp, stamp_blocks);
   const int64_t total = ScanCounts(ctx, counts.get(), n, C.pat->ptr.get());
   C.pat->nnz = total;
   C.pat->idx = ctx.Alloc<int>(total > 0 ? total : 1);
   C.val = ctx.Alloc<double>(total > 0 ? total : 1);
-  SymbolicShort<<<short_blocks, 256, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), Y.ptr(), Y.idx(), upper.get(),
-                                                      residual ? 1 : 0, 1, nullptr, C.pat->ptr.get(),
-                                                      C.pat->idx.get());
-  SymbolicEsc<<<sym_blocks, kSymWaves * kWave, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), Y.ptr(), Y.idx(),
-                                                                upper.get(), residual ? 1 : 0, 1, nullptr,
-                                                                C.pat->ptr.get(), C.pat->idx.get());
-  if (nbig > 0) {
-    SPAI_HIP_CHECK(hipMemsetAsync(stamp.get(), 0xFF, sizeof(int) * size_t(stamp_blocks) * size_t(nrows), ctx.stream));
-    SymbolicStamp<<<stamp_blocks, kBlock, 0, ctx.stream>>>(nbig, big_list.get(), (int)nrows, X.ptr(), X.idx(),
-                                                           Y.ptr(), Y.idx(), residual ? 1 : 0, 1, stamp.get(),
-                                                           nullptr, C.pat->ptr.get(), C.pat->idx.get());
-  }
+  symbolic(1, nullptr, C.pat->ptr.get(), C.pat->idx.get(), stamp, stamp_blocks);
   auto err = ctx.Alloc<int>(1);
   SPAI_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), ctx.stream));
-  NumericShort<<<short_blocks, 256, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), X.v(), Y.ptr(), Y.idx(), Y.v(),
-                                                     C.pat->ptr.get(), C.pat->idx.get(), C.val.get(),
-                                                     residual ? 1 : 0, upper.get(), err.get());
-  NumericGustavson<<<Blocks(n, kNumWaves), kNumWaves * kWave, 0, ctx.stream>>>(
-      (int)n, X.ptr(), X.idx(), X.v(), Y.ptr(), Y.idx(), Y.v(), C.pat->ptr.get(), C.pat->idx.get(), C.val.get(),
-      residual ? 1 : 0, upper.get(), err.get());
+  if (nshort > 0)
+    NumericShort<<<Blocks(nshort, kShortPerBlock), 256, 0, ctx.stream>>>(
+        nshort, short_list, X.ptr(), X.idx(), X.v(), Y.ptr(), Y.idx(), Y.v(), C.pat->ptr.get(), C.pat->idx.get(),
+        C.val.get(), residual ? 1 : 0, err.get());
+  if (nlong > 0)
+    NumericGustavson<<<Blocks(nlong, kNumWaves), kNumWaves * kWave, 0, ctx.stream>>>(
+        nlong, long_list, X.ptr(), X.idx(), X.v(), Y.ptr(), Y.idx(), Y.v(), C.pat->ptr.get(), C.pat->idx.get(),
+        C.val.get(), residual ? 1 : 0, err.get());
   int e = 0;
   ctx.ReadBack(&e, err.get(), sizeof(int));
   if (e) throw std::runtime_error("SpGEMM numeric phase found a row outside the symbolic pattern");
