@@ -178,30 +178,23 @@ __global__ __launch_bounds__(kExpBlock) void ExpansionSizes(int ncols, const int
   BlockAppend(live && u > kEscCap, j, big_list, counts + 2, wcount, &base_s);
 }
 
-// Symbolic phase of a short column: lane gl of the column's group holds
-// expansion entries gl + 16 r (r < 4), sorted in registers by a 64-key
-// bitonic network (element i = 16 r + gl: partners 16 apart or more sit in
-// the same lane, closer ones are exchanged by shuffles within the group);
-// then the distinct rows are counted (fill == 0) or written in row order
-// (fill == 1).  With fill == 0 and a stash, the distinct rows also go to
-// stash[g kShortCap ...] (g = the column's list position), so that CopyShort
-// can place them once the column offsets are known.
-__global__ __launch_bounds__(256) void SymbolicShort(int nlist, const int* __restrict__ list,
-                                                    const int* __restrict__ xptr, const int* __restrict__ xidx,
-                                                    const int* __restrict__ yptr, const int* __restrict__ yidx,
-                                                    int add_identity, int fill, int* __restrict__ counts,
-                                                    const int* __restrict__ cptr, int* __restrict__ cidx,
-                                                    int* __restrict__ stash) {
+// A short column j is handled by a group of kShortGroup lanes.  Lane gl of
+// the group holds expansion entries e = gl + 16 r (r < 4): the row of entry e
+// in key[r] (INT_MAX past the end; j itself at e == tot with add_identity);
+// with kValues also the Y entry it comes from in src[r] (-1 when none) and
+// the product x(i, k) y(k, j) in prod[r].
+template <bool kValues>
+__device__ __forceinline__ void ShortLoad(int gl, int j, int y0, int ny, const int* __restrict__ xptr,
+                                          const int* __restrict__ xidx, const double* __restrict__ xval,
+                                          const int* __restrict__ yidx, const double* __restrict__ yval,
+                                          int add_identity, int (&key)[4], int (&src)[4], double (&prod)[4]) {
   constexpr int G = kShortGroup;
-  const int gl = threadIdx.x % G;
-  const int g = blockIdx.x * kShortPerBlock + threadIdx.x / G;
-  if (g >= nlist) return;  // whole groups leave together
-  const int j = list[g];
-  const int y0 = yptr[j], ny = yptr[j + 1] - y0;
-  // the group's Y entries: lane gl takes entry gl (X column start and length)
+  // lane gl takes Y entry gl: X column start and length (and y)
   int b = 0, c = 0;
+  double y = 0.0;
   if (gl < ny) {
     const int k = yidx[y0 + gl];
+    if (kValues) y = yval[y0 + gl];
     b = xptr[k];
     c = xptr[k + 1] - b;
   }
@@ -213,7 +206,6 @@ __global__ __launch_bounds__(256) void SymbolicShort(int nlist, const int* __res
   }
   const int tot = __shfl(incl, G - 1, G);
   const int pre = incl - c;
-  int key[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int e = gl + G * r;
@@ -226,12 +218,28 @@ __global__ __launch_bounds__(256) void SymbolicShort(int nlist, const int* __res
     }
     const int sb = __shfl(b, lo, G), sp = __shfl(pre, lo, G);
     key[r] = INT_MAX;
+    if (kValues) {
+      const double sy = __shfl(y, lo, G);
+      src[r] = -1;
+      prod[r] = 0.0;
+      if (e < tot) {
+        src[r] = lo;
+        prod[r] = xval[sb + (e - sp)] * sy;
+      }
+    }
     if (e < tot)
       key[r] = xidx[sb + (e - sp)];
     else if (add_identity && e == tot)
       key[r] = j;
   }
-  // bitonic sort of the 64 keys, ascending
+}
+
+// Bitonic sort of the group's 64 keys, ascending (element i = 16 r + gl:
+// partners 16 apart or more sit in the same lane, closer ones are exchanged
+// by shuffles within the group); then first[r] flags the first occurrence of
+// each distinct row and the return value is the lane's count of them.
+__device__ __forceinline__ int ShortSortDistinct(int gl, int (&key)[4], int (&first)[4]) {
+  constexpr int G = kShortGroup;
 #pragma unroll
   for (int kk = 2; kk <= 64; kk <<= 1) {
 #pragma unroll
@@ -260,8 +268,8 @@ __global__ __launch_bounds__(256) void SymbolicShort(int nlist, const int* __res
       }
     }
   }
-  // distinct rows: element i is new when it differs from element i - 1
-  int first[4], cnt = 0;
+  // element i is new when it differs from element i - 1
+  int cnt = 0;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     int prev = __shfl_up(key[r], 1, G);
@@ -270,13 +278,12 @@ __global__ __launch_bounds__(256) void SymbolicShort(int nlist, const int* __res
     first[r] = key[r] != INT_MAX && (G * r + gl == 0 || key[r] != prev) ? 1 : 0;
     cnt += first[r];
   }
-  if (!fill && stash == nullptr) {
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, G);
-    if (gl == 0) counts[j] = cnt;
-    return;
-  }
-  int* const out = fill ? cidx + cptr[j] : stash + int64_t(g) * kShortCap;
+  return cnt;
+}
+
+// Writes the distinct rows in order to out[0 ...]; returns their number.
+__device__ __forceinline__ int ShortWrite(int gl, const int (&key)[4], const int (&first)[4], int* out) {
+  constexpr int G = kShortGroup;
   int base = 0;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -289,22 +296,110 @@ __global__ __launch_bounds__(256) void SymbolicShort(int nlist, const int* __res
     if (first[r]) out[base + sc - 1] = key[r];
     base += __shfl(sc, G - 1, G);
   }
-  if (!fill && gl == 0) counts[j] = base;
+  return base;
 }
 
-// Fill phase of the short columns after a stashing count phase: copies each
-// column's distinct rows from the stash to its place in C.
+// Symbolic phase of the short columns: counts (fill == 0) or writes in row
+// order (fill == 1) the distinct rows of each column.
+__global__ __launch_bounds__(256) void SymbolicShort(int nlist, const int* __restrict__ list,
+                                                    const int* __restrict__ xptr, const int* __restrict__ xidx,
+                                                    const int* __restrict__ yptr, const int* __restrict__ yidx,
+                                                    int add_identity, int fill, int* __restrict__ counts,
+                                                    const int* __restrict__ cptr, int* __restrict__ cidx) {
+  constexpr int G = kShortGroup;
+  const int gl = threadIdx.x % G;
+  const int g = blockIdx.x * kShortPerBlock + threadIdx.x / G;
+  if (g >= nlist) return;  // whole groups leave together
+  const int j = list[g];
+  const int y0 = yptr[j], ny = yptr[j + 1] - y0;
+  int key[4], src[4], first[4];
+  double prod[4];
+  ShortLoad<false>(gl, j, y0, ny, xptr, xidx, nullptr, yidx, nullptr, add_identity, key, src, prod);
+  int cnt = ShortSortDistinct(gl, key, first);
+  if (fill) {
+    ShortWrite(gl, key, first, cidx + cptr[j]);
+    return;
+  }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, G);
+  if (gl == 0) counts[j] = cnt;
+}
+
+// Both phases of the short columns in one pass, for when the column offsets
+// are not known yet: the distinct rows go to rstash[g kShortCap ...] (g = the
+// column's list position), the values to vstash likewise, their number to
+// counts[j]; CopyShort then places them.  The values are those of
+// NumericGustavson: each output row's products added to 0 in ascending k (Y's
+// entries in order; the entries of one X column have distinct rows, so no two
+// lanes of a step share an accumulator).  residual: -(sum) (+1 on the
+// diagonal).
+__global__ __launch_bounds__(256) void ProductShort(int nlist, const int* __restrict__ list,
+                                                   const int* __restrict__ xptr, const int* __restrict__ xidx,
+                                                   const double* __restrict__ xval, const int* __restrict__ yptr,
+                                                   const int* __restrict__ yidx, const double* __restrict__ yval,
+                                                   int residual, int* __restrict__ counts, int* __restrict__ rstash,
+                                                   double* __restrict__ vstash) {
+  constexpr int G = kShortGroup;
+  __shared__ int rows_s[kShortPerBlock][kShortCap];
+  __shared__ double acc_s[kShortPerBlock][kShortCap];
+  const int gl = threadIdx.x % G, grp = threadIdx.x / G;
+  const int g = blockIdx.x * kShortPerBlock + grp;
+  if (g >= nlist) return;  // whole groups leave together
+  const int j = list[g];
+  const int y0 = yptr[j], ny = yptr[j + 1] - y0;
+  int key[4], src[4], first[4];
+  double prod[4];
+  ShortLoad<true>(gl, j, y0, ny, xptr, xidx, xval, yidx, yval, residual, key, src, prod);
+  int row[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) row[r] = key[r];
+  ShortSortDistinct(gl, key, first);
+  int* const R = rows_s[grp];
+  double* const acc = acc_s[grp];
+  const int L = ShortWrite(gl, key, first, R);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) acc[gl + G * r] = 0.0;
+  __builtin_amdgcn_wave_barrier();
+  int pos[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) pos[r] = src[r] >= 0 ? LowerBound(R, 0, L, row[r]) : 0;
+  for (int s = 0; s < ny; ++s) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (src[r] == s) acc[pos[r]] += prod[r];
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  int* const rout = rstash + int64_t(g) * kShortCap;
+  double* const vout = vstash + int64_t(g) * kShortCap;
+  for (int e = gl; e < L; e += G) {
+    double v = acc[e];
+    if (residual) {
+      v = -v;
+      if (R[e] == j) v += 1.0;
+    }
+    rout[e] = R[e];
+    vout[e] = v;
+  }
+  if (gl == 0) counts[j] = L;
+}
+
+// Places the short columns' stashed rows and values in C.
 __global__ __launch_bounds__(256) void CopyShort(int nlist, const int* __restrict__ list,
-                                                const int* __restrict__ stash, const int* __restrict__ cptr,
-                                                int* __restrict__ cidx) {
+                                                const int* __restrict__ rstash, const double* __restrict__ vstash,
+                                                const int* __restrict__ cptr, int* __restrict__ cidx,
+                                                double* __restrict__ cval) {
   constexpr int G = kShortGroup;
   const int gl = threadIdx.x % G;
   const int g = blockIdx.x * kShortPerBlock + threadIdx.x / G;
   if (g >= nlist) return;
   const int j = list[g];
   const int cb = cptr[j], L = cptr[j + 1] - cb;
-  const int* const src = stash + int64_t(g) * kShortCap;
-  for (int e = gl; e < L; e += G) cidx[cb + e] = src[e];
+  const int64_t sb = int64_t(g) * kShortCap;
+  for (int e = gl; e < L; e += G) {
+    cidx[cb + e] = rstash[sb + e];
+    cval[cb + e] = vstash[sb + e];
+  }
 }
 
 // Symbolic phase, one wavefront per output column whose expansion fits the
@@ -1577,20 +1672,24 @@ DMat SpGemm(Ctx& ctx, const DMat& X, const DMat& Y, bool prune_zero, bool residu
   DMat C;
   C.pat = MakePattern(ctx, nrows, n, 0);
   // Each tier is launched over its own column list, so a tier with no
-  // columns costs nothing.
-  // The short columns' count phase stashes its sorted rows (kShortCap ints a
-  // column, when that stays under 1 GiB) and the fill phase copies them.
-  const bool stash_short = nshort > 0 && int64_t(nshort) * kShortCap <= (int64_t(1) << 28);
-  DBuf<int> short_stash;
-  if (stash_short) short_stash = ctx.Alloc<int>(size_t(nshort) * kShortCap);
+  // columns costs nothing.  The short columns take both phases in one pass
+  // (ProductShort) into stashes of kShortCap rows and values a column, when
+  // those stay under 1.5 GiB, and CopyShort places them after the scan;
+  // otherwise they take the two-pass kernels like the other tiers.
+  const bool fuse_short = nshort > 0 && int64_t(nshort) * kShortCap <= (int64_t(1) << 27);
+  DBuf<int> short_rows;
+  DBuf<double> short_vals;
+  if (fuse_short) {
+    short_rows = ctx.Alloc<int>(size_t(nshort) * kShortCap);
+    short_vals = ctx.Alloc<double>(size_t(nshort) * kShortCap);
+    ProductShort<<<Blocks(nshort, kShortPerBlock), 256, 0, ctx.stream>>>(
+        nshort, short_list, X.ptr(), X.idx(), X.v(), Y.ptr(), Y.idx(), Y.v(), residual ? 1 : 0, counts.get(),
+        short_rows.get(), short_vals.get());
+  }
   auto symbolic = [&](int fill, int* cnt, const int* cptr, int* cidx, DBuf<int>& stamp, unsigned stamp_blocks) {
-    if (nshort > 0 && fill && stash_short)
-      CopyShort<<<Blocks(nshort, kShortPerBlock), 256, 0, ctx.stream>>>(nshort, short_list, short_stash.get(),
-                                                                          cptr, cidx);
-    else if (nshort > 0)
+    if (nshort > 0 && !fuse_short)
       SymbolicShort<<<Blocks(nshort, kShortPerBlock), 256, 0, ctx.stream>>>(
-          nshort, short_list, X.ptr(), X.idx(), Y.ptr(), Y.idx(), residual ? 1 : 0, fill, cnt, cptr, cidx,
-          stash_short ? short_stash.get() : nullptr);
+          nshort, short_list, X.ptr(), X.idx(), Y.ptr(), Y.idx(), residual ? 1 : 0, fill, cnt, cptr, cidx);
     if (nlong > nbig)
       SymbolicEsc<<<Blocks(nlong, kSymWaves), kSymWaves * kWave, 0, ctx.stream>>>(
           nlong, long_list, X.ptr(), X.idx(), Y.ptr(), Y.idx(), upper.get(), residual ? 1 : 0, fill, cnt, cptr,
@@ -1616,9 +1715,18 @@ DMat SpGemm(Ctx& ctx, const DMat& X, const DMat& Y, bool prune_zero, bool residu
   C.pat->idx = ctx.Alloc<int>(total > 0 ? total : 1);
   C.val = ctx.Alloc<double>(total > 0 ? total : 1);
   symbolic(1, nullptr, C.pat->ptr.get(), C.pat->idx.get(), stamp, stamp_blocks);
+  if (fuse_short)
+    CopyShort<<<Blocks(nshort, kShortPerBlock), 256, 0, ctx.stream>>>(
+        nshort, short_list, short_rows.get(), short_vals.get(), C.pat->ptr.get(), C.pat->idx.get(), C.val.get());
+  // ProductShort takes its rows from its own sort; the other numeric kernels
+  // check theirs against the symbolic pattern.
+  if (nlong == 0 && (nshort == 0 || fuse_short)) {
+    if (prune_zero) return PruneZeros(ctx, std::move(C));
+    return C;
+  }
   auto err = ctx.Alloc<int>(1);
   SPAI_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), ctx.stream));
-  if (nshort > 0)
+  if (nshort > 0 && !fuse_short)
     NumericShort<<<Blocks(nshort, kShortPerBlock), 256, 0, ctx.stream>>>(
         nshort, short_list, X.ptr(), X.idx(), X.v(), Y.ptr(), Y.idx(), Y.v(), C.pat->ptr.get(), C.pat->idx.get(),
         C.val.get(), residual ? 1 : 0, err.get());

@@ -4,9 +4,11 @@ conservative sparse product; src/common/dropping.cpp:97-104 for the residual
 form): the same pattern, and the same values bit for bit, since both sum the
 contributions of each row in k-ascending order.
 
-The cases reach each tier of the symbolic phase: the LDS bitmap (a column's
-rows span fewer than 131,072 rows), the LDS sort (wider span, expansion of at
-most 4096 rows) and the stamp array (longer expansion)."""
+The cases reach each tier: short columns (expansion of at most 64 rows from
+at most 16 columns of X: one 16-lane group sorts in registers, in one fused
+pass, or in two passes past 2,097,152 such columns), the LDS bitmap (a
+column's rows span fewer than 131,072 rows), the LDS sort (wider span,
+expansion of at most 4096 rows) and the stamp array (longer expansion)."""
 import numpy as np
 import pytest
 from scipy import sparse
@@ -57,6 +59,40 @@ def test_spgemm_long_expansion_stamp_tier(gpu_lib, oracle_lib):
     Y = _random(64, 40, 16, seed=8)
     print("case: spgemm random long expansion", flush=True)
     _check(gpu_lib, oracle_lib, X, Y)
+
+
+@pytest.mark.parametrize("residual", [False, True])
+def test_spgemm_mixed_tiers(gpu_lib, oracle_lib, residual):
+    # Y's columns hold 1..40 entries, so short and long columns interleave
+    rng = np.random.default_rng(11)
+    n = 3000
+    X = _random(n, n, 5, seed=12)
+    per = rng.integers(1, 41, size=n)
+    r = np.concatenate([rng.choice(n, size=k, replace=False) for k in per])
+    c = np.repeat(np.arange(n), per)
+    Y = sparse.csc_matrix((rng.standard_normal(r.size), (r, c)), shape=(n, n))
+    Y.sort_indices()
+    print(f"case: spgemm mixed tiers residual={residual}", flush=True)
+    _check(gpu_lib, oracle_lib, X, Y, residual=residual)
+
+
+def test_spgemm_many_short_columns_two_pass(gpu_lib, oracle_lib):
+    # 2,200,000 short columns: past the one-pass stash limit
+    n = 2_200_000
+    j = np.arange(n)
+    rng = np.random.default_rng(13)
+
+    def two_per_column(mult, add):
+        rows = np.concatenate([j, (j * mult + add) % n])  # distinct: mult*j + add - j is odd, n even
+        cols = np.concatenate([j, j])
+        M = sparse.csc_matrix((rng.standard_normal(2 * n), (rows, cols)), shape=(n, n))
+        M.sort_indices()
+        return M
+
+    X = two_per_column(7, 3)
+    Y = two_per_column(13, 5)
+    print("case: spgemm 2.2M short columns residual", flush=True)
+    _check(gpu_lib, oracle_lib, X, Y, residual=True)
 
 
 def test_spgemm_empty_columns_and_cancellation(gpu_lib, oracle_lib):
