@@ -102,6 +102,7 @@ __global__ void WriteLast(int* __restrict__ out, int64_t n, const int64_t* __res
 // ------------------------------------------------------------------ SpGEMM
 constexpr int kEscCap = 4096;     // expansion keys per wavefront (LDS tiers)
 constexpr int kSymWaves = 2;      // wavefronts per block in the symbolic kernel
+constexpr int kStashMidColumns = 4096;  // LDS-tier columns from which their count pass stashes
 constexpr int kNumCap = 1024;     // output rows per column held in LDS (numeric)
 constexpr int kNumWaves = 4;
 
@@ -147,19 +148,28 @@ __device__ void BlockAppend(bool flag, int value, int* __restrict__ list, int* _
   __syncthreads();
 }
 
+// Sizes of the column lists of one product, and the summed expansion of the
+// columns in the LDS tiers.
+struct TierCounts {
+  int n[4];  // short, long (LDS tiers and stamp), big (stamp), unused
+  unsigned long long mid_expansion;
+};
+
 // u_j = sum over Y(:, j) of nnz(X(:, k)) (+1 for the identity), and the
-// column's tier: short columns go to lists[0], the others to lists[1] (the
+// column's tier: short columns go to short_list, the others to long_list (the
 // wavefront-per-column kernels) and, when the expansion exceeds the LDS
-// tier, also to lists[2] (the stamp kernel).  counts[t] is list t's length.
+// tier, also to big_list (the stamp kernel).
 constexpr int kExpBlock = 1024;
 __global__ __launch_bounds__(kExpBlock) void ExpansionSizes(int ncols, const int* __restrict__ xptr,
                                                             const int* __restrict__ yptr,
                                                             const int* __restrict__ yidx, int add_identity,
                                                             int64_t* __restrict__ upper, int* __restrict__ short_list,
                                                             int* __restrict__ long_list, int* __restrict__ big_list,
-                                                            int* __restrict__ counts) {
+                                                            TierCounts* __restrict__ tiers) {
   __shared__ int wcount[kExpBlock / kWave];
   __shared__ int base_s;
+  __shared__ unsigned long long mid_s;
+  if (threadIdx.x == 0) mid_s = 0;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = j < ncols;
   int64_t u = add_identity ? 1 : 0;
@@ -173,9 +183,24 @@ __global__ __launch_bounds__(kExpBlock) void ExpansionSizes(int ncols, const int
     upper[j] = u;
   }
   const bool is_short = live && IsShortColumn(u, ny);
-  BlockAppend(is_short, j, short_list, counts + 0, wcount, &base_s);
-  BlockAppend(live && !is_short, j, long_list, counts + 1, wcount, &base_s);
-  BlockAppend(live && u > kEscCap, j, big_list, counts + 2, wcount, &base_s);
+  BlockAppend(is_short, j, short_list, tiers->n + 0, wcount, &base_s);
+  BlockAppend(live && !is_short, j, long_list, tiers->n + 1, wcount, &base_s);
+  BlockAppend(live && u > kEscCap, j, big_list, tiers->n + 2, wcount, &base_s);
+  const int64_t mid = live && !is_short && u <= kEscCap ? u : 0;
+  const int64_t wsum = WaveInclusiveScan64(mid);
+  if (LaneId() == kWave - 1 && wsum > 0) atomicAdd(&mid_s, (unsigned long long)wsum);
+  __syncthreads();
+  if (threadIdx.x == 0 && mid_s > 0) atomicAdd(&tiers->mid_expansion, mid_s);
+}
+
+// cnt[g] = the expansion of long-list column g when it is in the LDS tiers
+// (the stash offsets of the stashing count pass), else 0.
+__global__ void MidExpansions(int nlist, const int* __restrict__ list, const int64_t* __restrict__ upper,
+                              int* __restrict__ cnt) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nlist) return;
+  const int64_t u = upper[list[g]];
+  cnt[g] = u <= kEscCap ? int(u) : 0;
 }
 
 // A short column j is handled by a group of kShortGroup lanes.  Lane gl of
@@ -424,7 +449,8 @@ __device__ __forceinline__ int WaveMaxInt(int v) {
 __global__ __launch_bounds__(kSymWaves* kWave) void SymbolicEsc(
     int nlist, const int* __restrict__ list, const int* __restrict__ xptr, const int* __restrict__ xidx, const int* __restrict__ yptr,
     const int* __restrict__ yidx, const int64_t* __restrict__ upper, int add_identity, int fill,
-    int* __restrict__ counts, const int* __restrict__ cptr, int* __restrict__ cidx) {
+    int* __restrict__ counts, const int* __restrict__ cptr, int* __restrict__ cidx, const int* __restrict__ soff,
+    int* __restrict__ stash) {
   __shared__ int keys[kSymWaves][kEscCap];
   __shared__ int pre[kSymWaves][kWave];
   __shared__ int beg[kSymWaves][kWave];
@@ -495,16 +521,19 @@ __global__ __launch_bounds__(kSymWaves* kWave) void SymbolicEsc(
     const int w0 = min(nwords, lane * per), w1 = min(nwords, w0 + per);
     int cnt = 0;
     for (int i = w0; i < w1; ++i) cnt += __popc(uint32_t(K[i]));
-    if (!fill) {
+    if (!fill && stash == nullptr) {
       cnt = WaveSumInt(cnt);
       if (lane == 0) counts[j] = cnt;
     } else {
-      int o = cptr[j] + WaveInclusiveScan(cnt) - cnt;
+      const int incl = WaveInclusiveScan(cnt);
+      if (!fill && lane == kWave - 1) counts[j] = incl;
+      int* const out = fill ? cidx + cptr[j] : stash + soff[g];
+      int o = incl - cnt;
       for (int i = w0; i < w1; ++i) {
         uint32_t word = uint32_t(K[i]);
         while (word) {
           const int bit = __ffs(word) - 1;
-          cidx[o++] = lo_r + i * 32 + bit;
+          out[o++] = lo_r + i * 32 + bit;
           word &= word - 1u;
         }
       }
@@ -535,21 +564,38 @@ __global__ __launch_bounds__(kSymWaves* kWave) void SymbolicEsc(
       WaveFence();
     }
   }
-  if (!fill) {
+  if (!fill && stash == nullptr) {
     int cnt = 0;
     for (int i = lane; i < off; i += kWave) cnt += (i == 0 || K[i] != K[i - 1]) ? 1 : 0;
     cnt = WaveSumInt(cnt);
     if (lane == 0) counts[j] = cnt;
   } else {
-    int base = cptr[j];
+    int* const out = fill ? cidx + cptr[j] : stash + soff[g];
+    int base = 0;
     for (int i0 = 0; i0 < off; i0 += kWave) {
       const int i = i0 + lane;
       const bool flag = i < off && (i == 0 || K[i] != K[i - 1]);
       const uint64_t mask = __ballot(flag);
-      if (flag) cidx[base + LanePrefixCount(mask)] = K[i];
+      if (flag) out[base + LanePrefixCount(mask)] = K[i];
       base += __popcll(mask);
     }
+    if (!fill && lane == 0) counts[j] = base;
   }
+}
+
+// Fill phase of the LDS-tier columns after a stashing count pass: copies each
+// column's distinct rows from stash[soff[g] ...] to its place in C.
+__global__ __launch_bounds__(kBlock) void CopyMid(int nlist, const int* __restrict__ list,
+                                                  const int64_t* __restrict__ upper, const int* __restrict__ soff,
+                                                  const int* __restrict__ stash, const int* __restrict__ cptr,
+                                                  int* __restrict__ cidx) {
+  const int g = blockIdx.x * (kBlock / kWave) + WaveInBlock();
+  if (g >= nlist) return;
+  const int j = list[g];
+  if (upper[j] > kEscCap) return;  // the stamp kernel's column
+  const int cb = cptr[j], L = cptr[j + 1] - cb;
+  const int* const src = stash + soff[g];
+  for (int e = LaneId(); e < L; e += kWave) cidx[cb + e] = src[e];
 }
 
 // Symbolic phase for columns whose expansion exceeds the LDS tier: a block per
@@ -589,6 +635,15 @@ __global__ __launch_bounds__(kBlock) void SymbolicStamp(int nlist, const int* __
 // Eigen's order: Y(:, j)'s entries k ascending, each scattering X(:, k) into
 // the column's accumulator (distinct rows inside one k, so the lanes never
 // collide).  residual: value = -(sum) (+1 on the diagonal).
+//
+// Columns of at most kNumCap rows accumulate in LDS.  Y's entries are taken
+// 64 at a time (lane l holds entry l: X column start, length, y), and their
+// expansion kNumRegs * 64 entries at a time: every lane loads and places its
+// entries (e = lane + 64 r) up front, then register by register adds them
+// grouped by Y entry in ascending order, so the loads and searches of a chunk
+// overlap and only the additions are ordered.  Longer columns accumulate in
+// global memory, one Y entry at a time.
+constexpr int kNumRegs = 4;
 __global__ __launch_bounds__(kNumWaves* kWave) void NumericGustavson(
     int nlist, const int* __restrict__ list, const int* __restrict__ xptr, const int* __restrict__ xidx,
     const double* __restrict__ xval, const int* __restrict__ yptr, const int* __restrict__ yidx,
@@ -596,44 +651,113 @@ __global__ __launch_bounds__(kNumWaves* kWave) void NumericGustavson(
     double* __restrict__ cval, int residual, int* __restrict__ error_flag) {
   __shared__ int rows_s[kNumWaves][kNumCap];
   __shared__ double acc_s[kNumWaves][kNumCap];
+  __shared__ int pre_s[kNumWaves][kWave];
+  __shared__ int beg_s[kNumWaves][kWave];
+  __shared__ double y_s[kNumWaves][kWave];
   const int w = WaveInBlock(), lane = LaneId();
   const int g = blockIdx.x * kNumWaves + w;
   if (g >= nlist) return;
   const int j = list[g];
   const int cb = cptr[j];
   const int L = cptr[j + 1] - cb;
-  const bool in_lds = L <= kNumCap;
-  if (in_lds) {
-    for (int e = lane; e < L; e += kWave) {
-      rows_s[w][e] = cidx[cb + e];
-      acc_s[w][e] = 0.0;
-    }
-  } else {
+  const int y0 = yptr[j], ye = yptr[j + 1];
+  if (L > kNumCap) {
     for (int e = lane; e < L; e += kWave) cval[cb + e] = 0.0;
-  }
-  WaveFence();
-  const int* R = in_lds ? rows_s[w] : (cidx + cb);
-  for (int p = yptr[j]; p < yptr[j + 1]; ++p) {
-    const int k = yidx[p];
-    const double y = yval[p];
-    for (int q = xptr[k] + lane; q < xptr[k + 1]; q += kWave) {
-      const int i = xidx[q];
-      const double x = xval[q];
-      const int pos = LowerBound(R, 0, L, i);
-      if (pos >= L || R[pos] != i) {
-        atomicOr(error_flag, 1);
-        continue;
-      }
-      if (in_lds) {
-        acc_s[w][pos] += x * y;
-      } else {
+    WaveFence();
+    const int* R = cidx + cb;
+    for (int p = y0; p < ye; ++p) {
+      const int k = yidx[p];
+      const double y = yval[p];
+      for (int q = xptr[k] + lane; q < xptr[k + 1]; q += kWave) {
+        const int i = xidx[q];
+        const double x = xval[q];
+        const int pos = LowerBound(R, 0, L, i);
+        if (pos >= L || R[pos] != i) {
+          atomicOr(error_flag, 1);
+          continue;
+        }
         atomicAdd(&cval[cb + pos], x * y);
+      }
+      WaveFence();
+    }
+    for (int e = lane; e < L; e += kWave) {
+      double v = __hip_atomic_load(&cval[cb + e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (residual) {
+        v = -v;
+        if (R[e] == j) v += 1.0;
+      }
+      cval[cb + e] = v;
+    }
+    return;
+  }
+  int* const R = rows_s[w];
+  double* const acc = acc_s[w];
+  for (int e = lane; e < L; e += kWave) {
+    R[e] = cidx[cb + e];
+    acc[e] = 0.0;
+  }
+  for (int p0 = y0; p0 < ye; p0 += kWave) {
+    const int p = p0 + lane;
+    const int nyc = min(kWave, ye - p0);
+    int b = 0, c = 0;
+    double y = 0.0;
+    if (p < ye) {
+      const int k = yidx[p];
+      y = yval[p];
+      b = xptr[k];
+      c = xptr[k + 1] - b;
+    }
+    const int incl = WaveInclusiveScan(c);
+    const int tot = __shfl(incl, kWave - 1, kWave);
+    pre_s[w][lane] = incl - c;
+    beg_s[w][lane] = b;
+    y_s[w][lane] = y;
+    WaveFence();
+    for (int e0 = 0; e0 < tot; e0 += kNumRegs * kWave) {
+      int src[kNumRegs], pos[kNumRegs];
+      double prod[kNumRegs];
+#pragma unroll
+      for (int r = 0; r < kNumRegs; ++r) {
+        const int e = e0 + lane + kWave * r;
+        src[r] = -1;
+        pos[r] = -1;
+        prod[r] = 0.0;
+        if (e < tot) {
+          int lo = 0, hi = nyc;  // the last Y entry whose first expansion entry is <= e
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (pre_s[w][mid] <= e)
+              lo = mid;
+            else
+              hi = mid;
+          }
+          const int q = beg_s[w][lo] + (e - pre_s[w][lo]);
+          const int i = xidx[q];
+          prod[r] = xval[q] * y_s[w][lo];
+          src[r] = lo;
+          const int at = LowerBound(R, 0, L, i);
+          if (at < L && R[at] == i)
+            pos[r] = at;
+          else
+            atomicOr(error_flag, 1);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kNumRegs; ++r) {
+        const int first = e0 + kWave * r;
+        if (first >= tot) break;
+        const int s_lo = __shfl(src[r], 0, kWave);
+        const int s_hi = __shfl(src[r], min(tot - 1 - first, kWave - 1), kWave);
+        for (int sv = s_lo; sv <= s_hi; ++sv) {
+          if (src[r] == sv && pos[r] >= 0) acc[pos[r]] += prod[r];
+          WaveFence();
+        }
       }
     }
     WaveFence();
   }
   for (int e = lane; e < L; e += kWave) {
-    double v = in_lds ? acc_s[w][e] : __hip_atomic_load(&cval[cb + e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    double v = acc[e];
     if (residual) {
       v = -v;
       if (R[e] == j) v += 1.0;
@@ -1067,13 +1191,58 @@ __global__ void ThresholdBatchesKernel(int64_t nnz_cols, int ncols, const int* _
   for (int p = ptr[j] + threadIdx.x; p < ptr[j + 1]; p += blockDim.x) keep[p] = !(fabs(v[p]) <= thr) ? 1 : 0;
 }
 
+// One step of the 8-bit radix select: H holds the 256-bin histogram of the
+// candidate keys that match `prefix` on `mask`; finds the bin that holds the
+// target-th largest of them and narrows prefix, mask and target to it.
+__device__ __forceinline__ void TopKPickBin(const unsigned* H, int shift, uint64_t& prefix, uint64_t& mask,
+                                            int& target) {
+  const int lane = LaneId();
+  int c[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) c[i] = int(H[4 * lane + i]);
+  const int mine = c[0] + c[1] + c[2] + c[3];
+  const int incl = WaveInclusiveScan(mine);
+  const int above = __shfl(incl, kWave - 1, kWave) - incl;  // keys in the bins of higher lanes
+  const bool here = above < target && target <= above + mine;
+  int bin = 0, rank = 0;
+  if (here) {
+    int t = target - above;
+    for (int i = 3; i >= 0; --i) {
+      if (c[i] >= t) {
+        bin = 4 * lane + i;
+        rank = t;
+        break;
+      }
+      t -= c[i];
+    }
+  }
+  const int src = __ffsll((unsigned long long)__ballot(here)) - 1;
+  bin = __shfl(bin, src, kWave);
+  target = __shfl(rank, src, kWave);
+  prefix |= uint64_t(bin) << shift;
+  mask |= uint64_t(255) << shift;
+}
+
+// Whether the entry at p (row `row`, key `key` equal to the threshold key T)
+// is among the first `take` off-diagonal entries of column j with key T in
+// row order.
+__device__ __forceinline__ unsigned char TopKTie(const int* __restrict__ idx, const double* __restrict__ v, int b,
+                                                 int e, int j, int row, uint64_t T, int take) {
+  int before = 0;
+  for (int q = b; q < e; ++q)
+    if (idx[q] != j && idx[q] < row && OrderedKey(fabs(v[q])) == T) ++before;
+  return before < take ? 1 : 0;
+}
+
 // Per-column top-k by |v| (ties: smaller row first) with the diagonal always
 // kept, one wavefront per column: an 8-bit radix select over the
 // order-preserving keys of |v| (8 passes, a 256-bin LDS histogram per pass,
 // the crossing bin found by a wave scan over 4 bins per lane), then one pass
 // that keeps keys above the threshold key and the first `take` keys equal to
-// it in row order.
+// it in row order.  Columns of at most kTopKRegs * 64 entries keep their keys
+// in registers across the passes; longer ones re-read them each pass.
 constexpr int kTopKWaves = 4;
+constexpr int kTopKRegs = 16;
 __global__ __launch_bounds__(kTopKWaves* kWave) void TopKWave(int ncols, const int* __restrict__ ptr,
                                                               const int* __restrict__ idx,
                                                               const double* __restrict__ v, int64_t k,
@@ -1084,6 +1253,77 @@ __global__ __launch_bounds__(kTopKWaves* kWave) void TopKWave(int ncols, const i
   if (j >= ncols) return;
   unsigned* const H = hist[w];
   const int b = ptr[j], e = ptr[j + 1];
+  uint64_t prefix = 0, mask = 0;
+  int target = int(k);  // rank of the wanted key counted from the largest
+  if (e - b <= kTopKRegs * kWave) {
+    uint64_t key[kTopKRegs];
+    int row[kTopKRegs];
+    int offd = 0;
+#pragma unroll
+    for (int r = 0; r < kTopKRegs; ++r) {
+      const int p = b + lane + kWave * r;
+      row[r] = -1;
+      key[r] = 0;
+      if (p < e) {
+        row[r] = idx[p];
+        key[r] = OrderedKey(fabs(v[p]));
+        offd += row[r] != j ? 1 : 0;
+      }
+    }
+    offd = WaveSumInt(offd);
+    if (offd <= k) {
+#pragma unroll
+      for (int r = 0; r < kTopKRegs; ++r)
+        if (row[r] >= 0) keep[b + lane + kWave * r] = 1;
+      return;
+    }
+#pragma unroll 1
+    for (int shift = 56; shift >= 0; shift -= 8) {
+      for (int t = lane; t < 256; t += kWave) H[t] = 0;
+      WaveFence();
+#pragma unroll
+      for (int r = 0; r < kTopKRegs; ++r)
+        if (row[r] >= 0 && row[r] != j && (key[r] & mask) == prefix) atomicAdd(&H[(key[r] >> shift) & 255u], 1u);
+      WaveFence();
+      TopKPickBin(H, shift, prefix, mask, target);
+      WaveFence();
+    }
+    // entries with the threshold key: all kept when there are at most
+    // `target` of them (the usual single one), else ranked by row
+    bool tie[kTopKRegs];
+    int nties = 0;
+#pragma unroll
+    for (int r = 0; r < kTopKRegs; ++r) {
+      tie[r] = row[r] >= 0 && row[r] != j && key[r] == prefix;
+      nties += tie[r] ? 1 : 0;
+    }
+    nties = WaveSumInt(nties);
+    int before[kTopKRegs];
+#pragma unroll
+    for (int r = 0; r < kTopKRegs; ++r) before[r] = 0;
+    if (nties > target) {
+#pragma unroll
+      for (int rr = 0; rr < kTopKRegs; ++rr) {
+        uint64_t m = __ballot(tie[rr]);
+        while (m) {
+          const int src = __ffsll((unsigned long long)m) - 1;
+          m &= m - 1;
+          const int trow = __shfl(row[rr], src, kWave);
+#pragma unroll
+          for (int r = 0; r < kTopKRegs; ++r) before[r] += tie[r] && trow < row[r] ? 1 : 0;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kTopKRegs; ++r) {
+      if (row[r] < 0) continue;
+      const int p = b + lane + kWave * r;
+      unsigned char kp = 1;
+      if (row[r] != j) kp = key[r] > prefix || (tie[r] && before[r] < target) ? 1 : 0;
+      keep[p] = kp;
+    }
+    return;
+  }
   int offd = 0;
   for (int p = b + lane; p < e; p += kWave) offd += idx[p] != j ? 1 : 0;
   offd = WaveSumInt(offd);
@@ -1091,8 +1331,6 @@ __global__ __launch_bounds__(kTopKWaves* kWave) void TopKWave(int ncols, const i
     for (int p = b + lane; p < e; p += kWave) keep[p] = 1;
     return;
   }
-  uint64_t prefix = 0, mask = 0;
-  int target = int(k);  // rank of the wanted key counted from the largest
   for (int shift = 56; shift >= 0; shift -= 8) {
     for (int t = lane; t < 256; t += kWave) H[t] = 0;
     WaveFence();
@@ -1102,34 +1340,14 @@ __global__ __launch_bounds__(kTopKWaves* kWave) void TopKWave(int ncols, const i
       if ((key & mask) == prefix) atomicAdd(&H[(key >> shift) & 255u], 1u);
     }
     WaveFence();
-    int c[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) c[i] = int(H[4 * lane + i]);
-    const int mine = c[0] + c[1] + c[2] + c[3];
-    const int incl = WaveInclusiveScan(mine);
-    const int above = __shfl(incl, kWave - 1, kWave) - incl;  // keys in the bins of higher lanes
-    const bool here = above < target && target <= above + mine;
-    int bin = 0, rank = 0;
-    if (here) {
-      int t = target - above;
-      for (int i = 3; i >= 0; --i) {
-        if (c[i] >= t) {
-          bin = 4 * lane + i;
-          rank = t;
-          break;
-        }
-        t -= c[i];
-      }
-    }
-    const int src = __ffsll((unsigned long long)__ballot(here)) - 1;
-    bin = __shfl(bin, src, kWave);
-    target = __shfl(rank, src, kWave);
-    prefix |= uint64_t(bin) << shift;
-    mask |= uint64_t(255) << shift;
+    TopKPickBin(H, shift, prefix, mask, target);
     WaveFence();
   }
   const uint64_t T = prefix;
   const int take = target;  // how many entries with key == T are kept
+  int nties = 0;
+  for (int p = b + lane; p < e; p += kWave) nties += idx[p] != j && OrderedKey(fabs(v[p])) == T ? 1 : 0;
+  nties = WaveSumInt(nties);
   for (int p = b + lane; p < e; p += kWave) {
     const int row = idx[p];
     if (row == j) {
@@ -1138,12 +1356,7 @@ __global__ __launch_bounds__(kTopKWaves* kWave) void TopKWave(int ncols, const i
     }
     const uint64_t key = OrderedKey(fabs(v[p]));
     unsigned char kp = key > T ? 1 : 0;
-    if (key == T) {
-      int before = 0;
-      for (int q = b; q < e; ++q)
-        if (idx[q] != j && idx[q] < row && OrderedKey(fabs(v[q])) == T) ++before;
-      kp = before < take ? 1 : 0;
-    }
+    if (key == T) kp = nties <= take ? 1 : TopKTie(idx, v, b, e, j, row, T, take);
     keep[p] = kp;
   }
 }
@@ -1658,15 +1871,27 @@ DMat SpGemm(Ctx& ctx, const DMat& X, const DMat& Y, bool prune_zero, bool residu
   int* const short_list = lists.get();
   int* const long_list = short_list + (n > 0 ? n : 1);
   int* const big_list = long_list + (n > 0 ? n : 1);
-  auto list_counts = ctx.Alloc<int>(3);
-  SPAI_HIP_CHECK(hipMemsetAsync(list_counts.get(), 0, 3 * sizeof(int), ctx.stream));
+  auto tiers_d = ctx.Alloc<TierCounts>(1);
+  SPAI_HIP_CHECK(hipMemsetAsync(tiers_d.get(), 0, sizeof(TierCounts), ctx.stream));
   if (n > 0)
-    ExpansionSizes<<<Blocks(n, kExpBlock), kExpBlock, 0, ctx.stream>>>((int)n, X.ptr(), Y.ptr(), Y.idx(), residual ? 1 : 0,
-                                                         upper.get(), short_list, long_list, big_list,
-                                                         list_counts.get());
-  int nlist[3] = {0, 0, 0};
-  ctx.ReadBack(nlist, list_counts.get(), sizeof(nlist));
-  const int nshort = nlist[0], nlong = nlist[1], nbig = nlist[2];
+    ExpansionSizes<<<Blocks(n, kExpBlock), kExpBlock, 0, ctx.stream>>>((int)n, X.ptr(), Y.ptr(), Y.idx(),
+                                                                       residual ? 1 : 0, upper.get(), short_list,
+                                                                       long_list, big_list, tiers_d.get());
+  TierCounts tiers{};
+  ctx.ReadBack(&tiers, tiers_d.get(), sizeof(tiers));
+  const int nshort = tiers.n[0], nlong = tiers.n[1], nbig = tiers.n[2];
+  // Many LDS-tier columns: their count pass stashes the sorted rows (at the
+  // offsets of a scan of their expansions, when those stay under 1 GiB) and
+  // the fill pass copies them, instead of repeating the gather.
+  const bool stash_mid = nlong - nbig >= kStashMidColumns && tiers.mid_expansion <= (1ull << 28);
+  DBuf<int> mid_off, mid_stash;
+  if (stash_mid) {
+    auto mid_cnt = ctx.Alloc<int>(size_t(nlong));
+    mid_off = ctx.Alloc<int>(size_t(nlong) + 1);
+    MidExpansions<<<Blocks(nlong), kBlock, 0, ctx.stream>>>(nlong, long_list, upper.get(), mid_cnt.get());
+    const int64_t mid_total = ScanCounts(ctx, mid_cnt.get(), nlong, mid_off.get());
+    mid_stash = ctx.Alloc<int>(mid_total > 0 ? size_t(mid_total) : 1);
+  }
 
   auto counts = ctx.Alloc<int>(n > 0 ? n : 1);
   DMat C;
@@ -1690,10 +1915,13 @@ DMat SpGemm(Ctx& ctx, const DMat& X, const DMat& Y, bool prune_zero, bool residu
     if (nshort > 0 && !fuse_short)
       SymbolicShort<<<Blocks(nshort, kShortPerBlock), 256, 0, ctx.stream>>>(
           nshort, short_list, X.ptr(), X.idx(), Y.ptr(), Y.idx(), residual ? 1 : 0, fill, cnt, cptr, cidx);
-    if (nlong > nbig)
+    if (nlong > nbig && fill && stash_mid)
+      CopyMid<<<Blocks(nlong, kBlock / kWave), kBlock, 0, ctx.stream>>>(nlong, long_list, upper.get(),
+                                                                        mid_off.get(), mid_stash.get(), cptr, cidx);
+    else if (nlong > nbig)
       SymbolicEsc<<<Blocks(nlong, kSymWaves), kSymWaves * kWave, 0, ctx.stream>>>(
           nlong, long_list, X.ptr(), X.idx(), Y.ptr(), Y.idx(), upper.get(), residual ? 1 : 0, fill, cnt, cptr,
-          cidx);
+          cidx, stash_mid ? mid_off.get() : nullptr, stash_mid ? mid_stash.get() : nullptr);
     if (nbig > 0) {
       SPAI_HIP_CHECK(
           hipMemsetAsync(stamp.get(), 0xFF, sizeof(int) * size_t(stamp_blocks) * size_t(nrows), ctx.stream));

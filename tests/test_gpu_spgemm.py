@@ -76,6 +76,22 @@ def test_spgemm_mixed_tiers(gpu_lib, oracle_lib, residual):
     _check(gpu_lib, oracle_lib, X, Y, residual=residual)
 
 
+@pytest.mark.parametrize("rows,residual", [(5000, False), (5000, True), (300_000, False)])
+def test_spgemm_many_lds_tier_columns_stashed(gpu_lib, oracle_lib, rows, residual):
+    # over 4096 columns in the LDS tiers: their count pass stashes the rows and
+    # the fill pass copies them; 5000 rows use the bitmap, 300,000 the sort
+    rng = np.random.default_rng(17)
+    n = 5000 if residual else 4500
+    X = _random(rows, 5000, 20, seed=18)
+    per = rng.integers(17, 41, size=n)
+    r = np.concatenate([rng.choice(5000, size=k, replace=False) for k in per])
+    c = np.repeat(np.arange(n), per)
+    Y = sparse.csc_matrix((rng.standard_normal(r.size), (r, c)), shape=(5000, n))
+    Y.sort_indices()
+    print(f"case: spgemm stashed LDS tiers rows={rows} residual={residual}", flush=True)
+    _check(gpu_lib, oracle_lib, X, Y, residual=residual)
+
+
 def test_spgemm_many_short_columns_two_pass(gpu_lib, oracle_lib):
     # 2,200,000 short columns: past the one-pass stash limit
     n = 2_200_000
