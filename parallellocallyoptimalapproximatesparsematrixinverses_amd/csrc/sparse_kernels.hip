@@ -102,7 +102,8 @@ __global__ void WriteLast(int* __restrict__ out, int64_t n, const int64_t* __res
 // ------------------------------------------------------------------ SpGEMM
 constexpr int kEscCap = 4096;     // expansion keys per wavefront (LDS tiers)
 constexpr int kSymWaves = 2;      // wavefronts per block in the symbolic kernel
-constexpr int kStashMidColumns = 4096;  // LDS-tier columns from which their count pass stashes
+constexpr int kStashMidColumns = 4096;
+constexpr int kEscRegs = 4;  // expansion entries a lane gathers at once  // LDS-tier columns from which their count pass stashes
 constexpr int kNumCap = 1024;     // output rows per column held in LDS (numeric)
 constexpr int kNumWaves = 4;
 
@@ -451,7 +452,7 @@ __global__ __launch_bounds__(kSymWaves* kWave) void SymbolicEsc(
     const int* __restrict__ yidx, const int64_t* __restrict__ upper, int add_identity, int fill,
     int* __restrict__ counts, const int* __restrict__ cptr, int* __restrict__ cidx, const int* __restrict__ soff,
     int* __restrict__ stash) {
-  __shared__ int keys[kSymWaves][kEscCap];
+  __shared__ __align__(16) int keys[kSymWaves][kEscCap];
   __shared__ int pre[kSymWaves][kWave];
   __shared__ int beg[kSymWaves][kWave];
   const int w = WaveInBlock(), lane = LaneId();
@@ -460,10 +461,22 @@ __global__ __launch_bounds__(kSymWaves* kWave) void SymbolicEsc(
   const int j = list[g];
   if (upper[j] > kEscCap) return;  // the stamp kernel's column
   int* K = keys[w];
-  const int ye = yptr[j + 1];
+  const int y0 = yptr[j], ye = yptr[j + 1];
+  // the first 64 Y entries (most columns have no more) stay in registers:
+  // lane l holds entry y0 + l's X column start and length
+  int b1 = 0, c1 = 0;
+  if (y0 + lane < ye) {
+    const int k = yidx[y0 + lane];
+    b1 = xptr[k];
+    c1 = xptr[k + 1] - b1;
+  }
   // row range of the expansion
   int lo_r = add_identity ? j : INT_MAX, hi_r = add_identity ? j : -1;
-  for (int p = yptr[j] + lane; p < ye; p += kWave) {
+  if (c1 > 0) {
+    lo_r = min(lo_r, xidx[b1]);
+    hi_r = max(hi_r, xidx[b1 + c1 - 1]);
+  }
+  for (int p = y0 + kWave + lane; p < ye; p += kWave) {
     const int k = yidx[p];
     const int b = xptr[k], e = xptr[k + 1];
     if (e > b) {
@@ -476,39 +489,60 @@ __global__ __launch_bounds__(kSymWaves* kWave) void SymbolicEsc(
   const bool bitmap = hi_r >= lo_r && hi_r - lo_r < 32 * kEscCap;
   const int nwords = bitmap ? ((hi_r - lo_r) >> 5) + 1 : 0;
   if (bitmap) {
-    for (int i = lane; i < nwords; i += kWave) K[i] = 0;
+    for (int i = lane; i < (nwords + 3) / 4; i += kWave) reinterpret_cast<int4*>(K)[i] = make_int4(0, 0, 0, 0);
     WaveFence();
   }
-  // gather: every lane takes expansion entries e = lane, lane + 64, ...
+  // gather: the expansion entries of 64 Y entries at a time, kEscRegs per
+  // lane at once (e = e0 + lane + 64 r): the source entry by binary search,
+  // then the loads, then the marks
   int off = 0;
-  for (int p0 = yptr[j]; p0 < ye; p0 += kWave) {
-    const int p = p0 + lane;
-    int b = 0, c = 0;
-    if (p < ye) {
-      const int k = yidx[p];
-      b = xptr[k];
-      c = xptr[k + 1] - b;
+  for (int p0 = y0; p0 < ye; p0 += kWave) {
+    int b = b1, c = c1;
+    if (p0 != y0) {
+      b = 0;
+      c = 0;
+      if (p0 + lane < ye) {
+        const int k = yidx[p0 + lane];
+        b = xptr[k];
+        c = xptr[k + 1] - b;
+      }
     }
+    const int nyc = min(kWave, ye - p0);
     const int incl = WaveInclusiveScan(c);
     const int tot = __shfl(incl, kWave - 1, kWave);
     pre[w][lane] = incl - c;
     beg[w][lane] = b;
     WaveFence();
-    for (int e = lane; e < tot; e += kWave) {
-      int lo = 0, hi = kWave;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (pre[w][mid] <= e)
-          lo = mid;
-        else
-          hi = mid;
+    for (int e0 = 0; e0 < tot; e0 += kEscRegs * kWave) {
+      int q[kEscRegs];
+#pragma unroll
+      for (int r = 0; r < kEscRegs; ++r) {
+        const int e = e0 + lane + kWave * r;
+        q[r] = -1;
+        if (e < tot) {
+          int lo = 0, hi = nyc;
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (pre[w][mid] <= e)
+              lo = mid;
+            else
+              hi = mid;
+          }
+          q[r] = beg[w][lo] + (e - pre[w][lo]);
+        }
       }
-      const int row = xidx[beg[w][lo] + (e - pre[w][lo])];
-      if (bitmap) {
-        const int r = row - lo_r;
-        atomicOr(&K[r >> 5], int(1u << (r & 31)));
-      } else {
-        K[off + e] = row;
+      int row[kEscRegs];
+#pragma unroll
+      for (int r = 0; r < kEscRegs; ++r) row[r] = q[r] >= 0 ? xidx[q[r]] : 0;
+#pragma unroll
+      for (int r = 0; r < kEscRegs; ++r) {
+        if (q[r] < 0) continue;
+        if (bitmap) {
+          const int rr = row[r] - lo_r;
+          atomicOr(&K[rr >> 5], int(1u << (rr & 31)));
+        } else {
+          K[off + e0 + lane + kWave * r] = row[r];
+        }
       }
     }
     off += tot;
@@ -517,10 +551,15 @@ __global__ __launch_bounds__(kSymWaves* kWave) void SymbolicEsc(
   if (bitmap) {
     if (add_identity && lane == 0) atomicOr(&K[(j - lo_r) >> 5], int(1u << ((j - lo_r) & 31)));
     WaveFence();
-    const int per = (nwords + kWave - 1) / kWave;
+    // lane ranges of a multiple of 4 words, read 16 bytes at a time (the
+    // words past nwords up to the next multiple of 4 were zeroed)
+    const int per = ((nwords + kWave - 1) / kWave + 3) & ~3;
     const int w0 = min(nwords, lane * per), w1 = min(nwords, w0 + per);
     int cnt = 0;
-    for (int i = w0; i < w1; ++i) cnt += __popc(uint32_t(K[i]));
+    for (int i = w0; i < w1; i += 4) {
+      const int4 q = *reinterpret_cast<const int4*>(K + i);
+      cnt += __popc(uint32_t(q.x)) + __popc(uint32_t(q.y)) + __popc(uint32_t(q.z)) + __popc(uint32_t(q.w));
+    }
     if (!fill && stash == nullptr) {
       cnt = WaveSumInt(cnt);
       if (lane == 0) counts[j] = cnt;
@@ -529,12 +568,17 @@ __global__ __launch_bounds__(kSymWaves* kWave) void SymbolicEsc(
       if (!fill && lane == kWave - 1) counts[j] = incl;
       int* const out = fill ? cidx + cptr[j] : stash + soff[g];
       int o = incl - cnt;
-      for (int i = w0; i < w1; ++i) {
-        uint32_t word = uint32_t(K[i]);
-        while (word) {
-          const int bit = __ffs(word) - 1;
-          out[o++] = lo_r + i * 32 + bit;
-          word &= word - 1u;
+      for (int i = w0; i < w1; i += 4) {
+        const int4 q = *reinterpret_cast<const int4*>(K + i);
+        const int qw[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          uint32_t word = uint32_t(qw[t]);
+          while (word) {
+            const int bit = __ffs(word) - 1;
+            out[o++] = lo_r + (i + t) * 32 + bit;
+            word &= word - 1u;
+          }
         }
       }
     }
