@@ -1004,19 +1004,101 @@ __device__ __forceinline__ int MergePathSplit(const int* __restrict__ li, int L,
 // the sequential merge (Eigen's binary iterator order), so fill == 1 writes
 // the same column as a thread-per-column merge would.  fill == 0 counts kept
 // entries.
+//
+// Columns with L + R <= kMergeStage first copy both index lists into LDS and
+// do the search and the walk there; the walk then only records, for each
+// union entry, where it comes from, and the values are loaded, combined and
+// written by consecutive lanes for consecutive entries.
 constexpr int kMergeWaves = 4;
+constexpr int kMergeStage = 1024;
+// union entry record: kind (0 left only, 1 both, 2 right only), x, y
+__device__ __forceinline__ int MergeRecord(int kind, int x, int y) { return (kind << 20) | (x << 10) | y; }
+
 __global__ __launch_bounds__(kMergeWaves* kWave) void MergeWave(
     int ncols, const int* __restrict__ lp, const int* __restrict__ li, const double* __restrict__ lv,
     const int* __restrict__ rp, const int* __restrict__ ri, const double* __restrict__ rv, int op, double a, double b,
     int prune, int fill, int* __restrict__ counts, const int* __restrict__ optr, int* __restrict__ oidx,
     double* __restrict__ oval) {
-  const int lane = LaneId();
-  const int j = blockIdx.x * kMergeWaves + WaveInBlock();
+  __shared__ int stage_s[kMergeWaves][kMergeStage];
+  __shared__ int rec_s[kMergeWaves][kMergeStage];
+  const int lane = LaneId(), w = WaveInBlock();
+  const int j = blockIdx.x * kMergeWaves + w;
   if (j >= ncols) return;
   const int xb = lp[j], L = lp[j + 1] - xb, yb = rp[j], R = rp[j + 1] - yb;
+  const int T = L + R;
+  if (T <= kMergeStage) {
+    int* const S = stage_s[w];
+    int* const rec = rec_s[w];
+    for (int t = lane; t < L; t += kWave) S[t] = li[xb + t];
+    for (int t = lane; t < R; t += kWave) S[L + t] = ri[yb + t];
+    WaveFence();
+    const int* const lx = S;
+    const int* const ry = S + L;
+    const int c = (T + kWave - 1) / kWave;
+    const int d0 = min(T, lane * c), d1 = min(T, d0 + c);
+    int x0 = MergePathSplit(lx, L, ry, R, d0), y0 = d0 - x0;
+    if (d0 < d1 && x0 > 0 && y0 < R && ry[y0] == lx[x0 - 1]) ++y0;
+    auto walk = [&](bool write, int base) {
+      int x = x0, y = y0, u = 0;
+      while (x + y < d1) {
+        int kind;
+        if (x < L && (y >= R || lx[x] <= ry[y])) {
+          kind = (y < R && lx[x] == ry[y]) ? 1 : 0;
+          if (write) rec[base + u] = MergeRecord(kind, x, kind ? y : 0);
+          if (kind == 1) ++y;
+          ++x;
+        } else {
+          if (write) rec[base + u] = MergeRecord(2, 0, y);
+          ++y;
+        }
+        ++u;
+      }
+      return u;
+    };
+    const int u = walk(false, 0);
+    const int incl = WaveInclusiveScan(u);
+    const int U = __shfl(incl, kWave - 1, kWave);  // union entries
+    if (!fill && !prune) {
+      if (lane == 0) counts[j] = U;
+      return;
+    }
+    walk(true, incl - u);
+    WaveFence();
+    int base = fill ? optr[j] : 0, kept = 0;
+    for (int s0 = 0; s0 < U; s0 += kWave) {
+      const int slot = s0 + lane;
+      int row = 0;
+      double v = 0.0;
+      bool keep_it = false;
+      if (slot < U) {
+        const int rc = rec[slot];
+        const int kind = rc >> 20, x = (rc >> 10) & 1023, y = rc & 1023;
+        if (kind == 0) {
+          row = lx[x];
+          v = MergeLeft(op, lv[xb + x], a, b);
+        } else if (kind == 1) {
+          row = lx[x];
+          v = MergeBoth(op, lv[xb + x], rv[yb + y], a, b);
+        } else {
+          row = ry[y];
+          v = MergeRight(op, rv[yb + y], a, b);
+        }
+        keep_it = !(prune && !(fabs(v) > 0.0) && !isnan(v));
+      }
+      const uint64_t mask = __ballot(keep_it);
+      if (fill && keep_it) {
+        const int o = base + LanePrefixCount(mask);
+        oidx[o] = row;
+        oval[o] = v;
+      }
+      base += __popcll(mask);
+      kept += __popcll(mask);
+    }
+    if (!fill && lane == 0) counts[j] = kept;
+    return;
+  }
   const int* const lx = li + xb;
   const int* const ry = ri + yb;
-  const int T = L + R;
   const int c = (T + kWave - 1) / kWave;
   const int d0 = min(T, lane * c), d1 = min(T, d0 + c);
   int x0 = MergePathSplit(lx, L, ry, R, d0), y0 = d0 - x0;
