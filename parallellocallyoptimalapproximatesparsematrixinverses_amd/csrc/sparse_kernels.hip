@@ -941,11 +941,14 @@ __global__ void FlagNonZero(int64_t nnz, const double* __restrict__ val, unsigne
 }
 
 // ------------------------------------------------------------------ elementwise
-__global__ void DiagScaleKernel(int ncols, const int* __restrict__ ptr, const int* __restrict__ idx,
+// out = d(idx) .* yv over the stored entries ptr[0] + [0, nnz).
+__global__ void DiagScaleKernel(int64_t nnz, const int* __restrict__ ptr, const int* __restrict__ idx,
                                 const double* __restrict__ d, const double* __restrict__ yv,
                                 double* __restrict__ out) {
-  const int j = blockIdx.x;
-  for (int p = ptr[j] + threadIdx.x; p < ptr[j + 1]; p += blockDim.x) out[p] = d[idx[p]] * yv[p];
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= nnz) return;
+  const int64_t p = ptr[0] + t;
+  out[p] = d[idx[p]] * yv[p];
 }
 
 __global__ void ColScaleKernel(int ncols, const int* __restrict__ ptr, const double* __restrict__ f,
@@ -1148,8 +1151,31 @@ __global__ __launch_bounds__(kMergeWaves* kWave) void MergeWave(
 // in LDS in the column's row order, and added by one lane in that order: the
 // order of the sequential loop (ColumnDot), so out[j] is bit-identical to it.
 // An intersection with more than kDotCap matches takes ColumnDot in lane 0.
+// When L + R <= kDotCap, both index lists are first staged in the upper half
+// of the product buffer, the search and walk run there and record the
+// matches, and consecutive lanes form consecutive products.
 constexpr int kDotWaves = 4;
 constexpr int kDotCap = 1024;
+
+// s + P[0] + P[1] + ... + P[n-1], added in that order by the calling lane;
+// P is 16-byte aligned and read 8 values ahead of the additions.
+__device__ __forceinline__ double OrderedSumLds(const double* P, int n, double s) {
+  const double2* P2 = reinterpret_cast<const double2*>(P);
+  int e = 0;
+  for (; e + 8 <= n; e += 8) {
+    const double2 a = P2[e / 2], b = P2[e / 2 + 1], c = P2[e / 2 + 2], d = P2[e / 2 + 3];
+    s += a.x;
+    s += a.y;
+    s += b.x;
+    s += b.y;
+    s += c.x;
+    s += c.y;
+    s += d.x;
+    s += d.y;
+  }
+  for (; e < n; ++e) s += P[e];
+  return s;
+}
 __global__ __launch_bounds__(kDotWaves* kWave) void ColumnDotWave(int ncols, const int* __restrict__ xp,
                                                                   const int* __restrict__ xi,
                                                                   const double* __restrict__ xv,
@@ -1157,7 +1183,7 @@ __global__ __launch_bounds__(kDotWaves* kWave) void ColumnDotWave(int ncols, con
                                                                   const int* __restrict__ yi,
                                                                   const double* __restrict__ yv, int same,
                                                                   double* __restrict__ out) {
-  __shared__ double prod[kDotWaves][kDotCap];
+  __shared__ __align__(16) double prod[kDotWaves][kDotCap];
   const int lane = LaneId(), w = WaveInBlock();
   const int j = blockIdx.x * kDotWaves + w;
   if (j >= ncols) return;
@@ -1172,17 +1198,63 @@ __global__ __launch_bounds__(kDotWaves* kWave) void ColumnDotWave(int ncols, con
         P[e] = v * v;
       }
       WaveFence();
-      if (lane == 0)
-        for (int e = 0; e < m; ++e) s += P[e];
+      if (lane == 0) s = OrderedSumLds(P, m, s);
       WaveFence();
     }
     if (lane == 0) out[j] = s;
     return;
   }
   const int yb = yp[j], R = yp[j + 1] - yb;
+  const int T = L + R;
+  if (T <= kDotCap) {
+    // ints [kDotCap, 2 kDotCap) of P: the index lists; ints [0, 512): the
+    // match records (x << 10 | y), read back before products [0, 512) land
+    int* const S = reinterpret_cast<int*>(P) + kDotCap;
+    int* const rec = reinterpret_cast<int*>(P);
+    for (int t = lane; t < L; t += kWave) S[t] = xi[xb + t];
+    for (int t = lane; t < R; t += kWave) S[L + t] = yi[yb + t];
+    WaveFence();
+    const int* const lx = S;
+    const int* const ry = S + L;
+    const int c = (T + kWave - 1) / kWave;
+    const int d0 = min(T, lane * c), d1 = min(T, d0 + c);
+    int x0 = MergePathSplit(lx, L, ry, R, d0), y0 = d0 - x0;
+    if (d0 < d1 && x0 > 0 && y0 < R && ry[y0] == lx[x0 - 1]) ++y0;  // matched by the previous lane
+    auto walk = [&](bool write, int base) {
+      int x = x0, y = y0, k = 0;
+      while (x + y < d1) {
+        if (x < L && (y >= R || lx[x] <= ry[y])) {
+          if (y < R && lx[x] == ry[y]) {
+            if (write) rec[base + k] = (x << 10) | y;
+            ++k;
+            ++y;
+          }
+          ++x;
+        } else {
+          ++y;
+        }
+      }
+      return k;
+    };
+    const int k = walk(false, 0);
+    const int incl = WaveInclusiveScan(k);
+    const int total = __shfl(incl, kWave - 1, kWave);  // <= min(L, R) <= kDotCap / 2
+    walk(true, incl - k);
+    WaveFence();
+    constexpr int kRecs = kDotCap / 2 / kWave;
+    int rc[kRecs];
+#pragma unroll
+    for (int t = 0; t < kRecs; ++t) rc[t] = lane + kWave * t < total ? rec[lane + kWave * t] : -1;
+    WaveFence();
+#pragma unroll
+    for (int t = 0; t < kRecs; ++t)
+      if (rc[t] >= 0) P[lane + kWave * t] = xv[xb + (rc[t] >> 10)] * yv[yb + (rc[t] & 1023)];
+    WaveFence();
+    if (lane == 0) out[j] = OrderedSumLds(P, total, 0.0);
+    return;
+  }
   const int* const lx = xi + xb;
   const int* const ry = yi + yb;
-  const int T = L + R;
   const int c = (T + kWave - 1) / kWave;
   const int d0 = min(T, lane * c), d1 = min(T, d0 + c);
   int x0 = MergePathSplit(lx, L, ry, R, d0), y0 = d0 - x0;
@@ -1212,11 +1284,7 @@ __global__ __launch_bounds__(kDotWaves* kWave) void ColumnDotWave(int ncols, con
   }
   walk(true, incl - k);
   WaveFence();
-  if (lane == 0) {
-    double s = 0.0;
-    for (int e = 0; e < total; ++e) s += P[e];
-    out[j] = s;
-  }
+  if (lane == 0) out[j] = OrderedSumLds(P, total, 0.0);
 }
 
 // Deterministic sum of v[0..n): per-block partials, then one block.
@@ -2100,7 +2168,7 @@ DMat DiagScale(Ctx& ctx, const double* d, const DMat& Y, bool prune_zero) {
   C.pat = Y.pat;
   C.val = ctx.Alloc<double>(Y.nnz() > 0 ? Y.nnz() : 1);
   if (Y.cols() > 0 && Y.nnz() > 0)
-    DiagScaleKernel<<<(unsigned)Y.cols(), 64, 0, ctx.stream>>>((int)Y.cols(), Y.ptr(), Y.idx(), d, Y.v(), C.val.get());
+    DiagScaleKernel<<<Blocks(Y.nnz()), kBlock, 0, ctx.stream>>>(Y.nnz(), Y.ptr(), Y.idx(), d, Y.v(), C.val.get());
   if (prune_zero) return PruneZeros(ctx, std::move(C));
   return C;
 }
