@@ -8,6 +8,8 @@ Layout:
   autograd.py      torch.autograd.Function for ||I - A M||_F^2
   datasets.py      sample-dataset study (MatrixMarket loading, construction
                    and backward-error tables of scripts/common.py)
+  speedup.py       thread-speedup study format of
+                   scripts/methods_thread_speedup.py, with a device column
 
 One HIP runtime per process: libspai_hip.so needs libamdhip64, and the dynamic
 loader reuses a copy that is already loaded.  PyTorch ships its own; if

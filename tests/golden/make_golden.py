@@ -11,6 +11,10 @@ data files (run in a container where the reference tree is mounted).
     (PIPELINES only): the reference's end-of-build statuses, the PCG
     relative-residual histories with its preconditioners (150-entry limit) and
     the solver statuses, copied whole.
+  * <name>-<mode>-i<iterations>-d<density>-thread_speedup_timings.csv
+    (SPEEDUP only): the reference's CPU timings of every method variant at
+    p = 1..64 threads (scripts/results/results_parallelization_speedup/...),
+    copied whole.
 """
 import csv
 import glob
@@ -27,6 +31,7 @@ ROWS = 41
 
 DATASETS = ["tri100eigs4k", "bcsstk21", "msc04515"]
 PIPELINES = [("tri100eigs4k", "no_dropping", "1.000")]
+SPEEDUP = [("tri100eigs4k", "dropping", "25", "0.030")]
 
 
 def main():
@@ -55,6 +60,12 @@ def main():
                     rows = list(csv.reader(fh))
                 with open(OUT / f"{name}-{mode}-d{dens}-{kind}.csv", "w", newline="") as fh:
                     csv.writer(fh).writerows(rows)
+    for name, mode, its, dens in SPEEDUP:
+        src = REF / f"results/results_parallelization_speedup/{mode}/max_iter_{its}/max_density_{dens}/{name}"
+        with open(src / "all_thread_speedup_timings.csv") as fh:
+            rows = list(csv.reader(fh))
+        with open(OUT / f"{name}-{mode}-i{its}-d{dens}-thread_speedup_timings.csv", "w", newline="") as fh:
+            csv.writer(fh).writerows(rows)
     return 0
 
 
