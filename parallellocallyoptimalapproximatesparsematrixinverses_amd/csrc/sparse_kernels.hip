@@ -99,6 +99,34 @@ __global__ void WriteLast(int* __restrict__ out, int64_t n, const int64_t* __res
   out[n] = static_cast<int>(*total);
 }
 
+// The whole scan in one block for short inputs: thread t sums its run of
+// ceil(n / 1024) entries, the block scans the run sums, and each thread
+// writes its run's offsets; out[n] and *total receive the sum.  in may be out.
+constexpr int64_t kScanSmallMax = 65536;
+__global__ __launch_bounds__(1024) void ScanSmall(const int* __restrict__ in, int64_t n, int* __restrict__ out,
+                                                  int64_t* __restrict__ total) {
+  __shared__ int64_t wsum[16];
+  const int64_t per = (n + 1023) / 1024;
+  const int64_t b = min(n, int64_t(threadIdx.x) * per), e = min(n, b + per);
+  int64_t s = 0;
+  for (int64_t i = b; i < e; ++i) s += in[i];
+  const int64_t incl = WaveInclusiveScan64(s);
+  if (LaneId() == kWave - 1) wsum[WaveInBlock()] = incl;
+  __syncthreads();
+  int64_t run = incl - s;
+  for (int w = 0; w < WaveInBlock(); ++w) run += wsum[w];
+  const int64_t end = run + s;
+  for (int64_t i = b; i < e; ++i) {
+    const int v = in[i];
+    out[i] = static_cast<int>(run);
+    run += v;
+  }
+  if (threadIdx.x == blockDim.x - 1) {
+    out[n] = static_cast<int>(end);
+    *total = end;
+  }
+}
+
 // ------------------------------------------------------------------ SpGEMM
 constexpr int kEscCap = 4096;     // expansion keys per wavefront (LDS tiers)
 constexpr int kSymWaves = 2;      // wavefronts per block in the symbolic kernel
@@ -1926,6 +1954,14 @@ PatternPtr MakePattern(Ctx& ctx, int64_t rows, int64_t cols, int64_t nnz) {
 }
 
 int64_t ScanCounts(Ctx& ctx, const int* counts, int64_t n, int* ptr) {
+  if (n > 0 && n <= kScanSmallMax) {
+    auto total = ctx.Alloc<int64_t>(1);
+    ScanSmall<<<1, 1024, 0, ctx.stream>>>(counts, n, ptr, total.get());
+    int64_t t = 0;
+    ctx.ReadBack(&t, total.get(), sizeof(t));
+    if (t > kMaxDeviceNnz) throw std::runtime_error("result exceeds 2^31-1 stored entries on the device");
+    return t;
+  }
   const int64_t tiles = (n + kScanTile - 1) / kScanTile;
   auto sums = ctx.Alloc<int64_t>(tiles > 0 ? tiles : 1);
   auto total = ctx.Alloc<int64_t>(1);
