@@ -1695,9 +1695,11 @@ __global__ void ScatterTranspose(int ncols, const int* __restrict__ ptr, const i
 constexpr int kSortCap = 2048;
 
 // Sorts each segment [tptr[c], tptr[c+1]) by index (values follow).  Short
-// segments: insertion sort by one thread; medium: wavefront bitonic sort of
-// (index << 11 | slot) keys in LDS; long segments are listed for the stamp pass.
-__global__ void SortSegmentsSmall(int nseg, const int* __restrict__ tptr, int* __restrict__ tidx,
+// segments (at most 32 entries): one thread, a 32-element bitonic network in
+// registers (indices distinct, padding INT_MAX); medium: wavefront bitonic
+// sort of (index << 11 | slot) keys in LDS; long segments are listed for the
+// stamp pass.
+__global__ __launch_bounds__(kBlock) void SortSegmentsSmall(int nseg, const int* __restrict__ tptr, int* __restrict__ tidx,
                                   double* __restrict__ tval, int* __restrict__ long_list,
                                   int* __restrict__ long_count, int* __restrict__ mid_list,
                                   int* __restrict__ mid_count) {
@@ -1713,17 +1715,40 @@ __global__ void SortSegmentsSmall(int nseg, const int* __restrict__ tptr, int* _
     mid_list[atomicAdd(mid_count, 1)] = c;
     return;
   }
-  for (int i = b + 1; i < e; ++i) {
-    const int key = tidx[i];
-    const double val = tval[i];
-    int k = i - 1;
-    while (k >= b && tidx[k] > key) {
-      tidx[k + 1] = tidx[k];
-      tval[k + 1] = tval[k];
-      --k;
+  if (len <= 1) return;
+  int key[32];
+  double val[32];
+#pragma unroll
+  for (int t = 0; t < 32; ++t) {
+    key[t] = t < len ? tidx[b + t] : INT_MAX;
+    val[t] = t < len ? tval[b + t] : 0.0;
+  }
+#pragma unroll
+  for (int k = 2; k <= 32; k <<= 1) {
+#pragma unroll
+    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+#pragma unroll
+      for (int i = 0; i < 32; ++i) {
+        const int l = i ^ jj;
+        if (l > i) {
+          const bool asc = (i & k) == 0;
+          const bool swap = asc ? key[i] > key[l] : key[i] < key[l];
+          const int ki = key[i], kl = key[l];
+          const double vi = val[i], vl = val[l];
+          key[i] = swap ? kl : ki;
+          key[l] = swap ? ki : kl;
+          val[i] = swap ? vl : vi;
+          val[l] = swap ? vi : vl;
+        }
+      }
     }
-    tidx[k + 1] = key;
-    tval[k + 1] = val;
+  }
+#pragma unroll
+  for (int t = 0; t < 32; ++t) {
+    if (t < len) {
+      tidx[b + t] = key[t];
+      tval[b + t] = val[t];
+    }
   }
 }
 
