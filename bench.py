@@ -347,7 +347,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": pass_bytes, "avg_launch_us": round(us, 3),
                      "launches_timed": fp32["lomr_passes_timed"]},
-        "roofline_pcg_spmv": {"kernel": "PcgSpmvA (CSR SpMV with A + <p, Ap>), standalone replay", "bound": "hbm",
+        "roofline_pcg_spmv": {"kernel": "PcgSpmvASell (SpMV with A + <p, Ap> on the sliced-ELLPACK copy the solve reads), standalone replay", "bound": "hbm",
                               "achieved": round(spmv_bytes / (spmv_us * 1e-6) / 1e9, 1), "peak": HBM_PEAK_GBS,
                               "unit": "GB/s", "frac": round(spmv_bytes / (spmv_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
                               "traffic": spmv_traffic, "algorithmic_bytes_per_launch": spmv_bytes,

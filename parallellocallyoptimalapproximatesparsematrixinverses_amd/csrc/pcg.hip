@@ -641,6 +641,22 @@ __global__ __launch_bounds__(kBlk) void PcgSpmvA(int n, const int* __restrict__ 
   StorePartial(acc, part + kP1 * gridDim.x);
 }
 
+// The same SpMV + <p, Ap> on the sliced-ELLPACK copy of A, one thread per row
+// as the iteration kernels read it (8 column indices and values in flight,
+// then the gathers).
+__global__ __launch_bounds__(kBlk) void PcgSpmvASell(int n, SellRows rows, const int* __restrict__ ac,
+                                                     const double* __restrict__ av, const double* __restrict__ p,
+                                                     double* __restrict__ Ap, double* __restrict__ part) {
+  double acc = 0.0;
+  auto gather = [&](int c) { return p[c]; };
+  for (int row = XcdBlock() * kBlk + threadIdx.x; row < n; row += gridDim.x * kBlk) {
+    const double s = RowSum<double, SellRows::kPadded, true>(rows.Row(row), ac, av, 0, gather);
+    Ap[row] = s;
+    acc += p[row] * s;
+  }
+  StorePartial(acc, part + kP1 * gridDim.x);
+}
+
 // SELL construction: per-slice width (x 64 entries), then the slots.
 __global__ void SellWidths(int64_t n, int64_t nslices, const int* __restrict__ ptr, int* __restrict__ slots) {
   const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -974,8 +990,12 @@ double PcgSolver::TimeSpmvA(int reps) {
   for (auto& e : ev) SPAI_HIP_CHECK(hipEventCreate(&e));
   for (int i = 0; i < reps; ++i) {
     SPAI_HIP_CHECK(hipEventRecord(ev[2 * i], st));
-    SPAI_SG_DISPATCH(sga, (PcgSpmvA<SG><<<grid_, kBlk, 0, st>>>(n, A_.ptr.get(), A_.col.get(), A_.val.get(),
-                                                                x_.get(), ap_.get(), part_.get())));
+    if (sell_a_)
+      PcgSpmvASell<<<grid_, kBlk, 0, st>>>(n, SellRows{a_sell_.off.get()}, a_sell_.col.get(), a_sell_.val.get(),
+                                           x_.get(), ap_.get(), part_.get());
+    else
+      SPAI_SG_DISPATCH(sga, (PcgSpmvA<SG><<<grid_, kBlk, 0, st>>>(n, A_.ptr.get(), A_.col.get(), A_.val.get(),
+                                                                  x_.get(), ap_.get(), part_.get())));
     SPAI_HIP_CHECK(hipEventRecord(ev[2 * i + 1], st));
   }
   SPAI_HIP_CHECK(hipStreamSynchronize(st));
