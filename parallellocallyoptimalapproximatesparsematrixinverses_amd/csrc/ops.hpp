@@ -48,7 +48,10 @@ DMat Clone(Ctx& ctx, const DMat& m);
 DMat ShareClone(Ctx& ctx, const DMat& m);
 
 // Exclusive scan of int32 counts into ptr[0..n]; returns the total (blocking).
-int64_t ScanCounts(Ctx& ctx, const int* counts, int64_t n, int* ptr);
+// ptr[0..n] = exclusive scan of counts[0..n) (ptr[n] = the sum); returns the
+// sum read back to the host, or -1 without that read-back when read_total is
+// false (for callers that know it or do not need it).
+int64_t ScanCounts(Ctx& ctx, const int* counts, int64_t n, int* ptr, bool read_total = true);
 
 // ---------------------------------------------------------------- products
 // Conservative product C = X * Y (Eigen's column-wise Gustavson order:

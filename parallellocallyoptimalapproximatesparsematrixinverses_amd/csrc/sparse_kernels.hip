@@ -958,6 +958,7 @@ __global__ void ScatterKept(int64_t nnz, const int* __restrict__ idx, const doub
   oval[o] = val[i];
 }
 
+// dropped (nullable) counts the entries not kept.
 __global__ void FlagNonZero(int64_t nnz, const double* __restrict__ val, unsigned char* __restrict__ keep,
                             int* __restrict__ dropped) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -965,8 +966,12 @@ __global__ void FlagNonZero(int64_t nnz, const double* __restrict__ val, unsigne
   const double v = val[i];
   const bool k = !(fabs(v) <= 0.0);  // prune(0.0): keep |v| > 0 and NaN
   keep[i] = k ? 1 : 0;
-  if (!k) atomicAdd(dropped, 1);
+  if (!k && dropped) atomicAdd(dropped, 1);
 }
+
+// Up to this many entries, PruneZeros filters at once instead of first
+// reading back whether any entry is dropped (one host round trip less).
+constexpr int64_t kPruneFilterAlways = int64_t(1) << 20;
 
 // ------------------------------------------------------------------ elementwise
 // out = d(idx) .* yv over the stored entries ptr[0] + [0, nnz).
@@ -1978,10 +1983,11 @@ PatternPtr MakePattern(Ctx& ctx, int64_t rows, int64_t cols, int64_t nnz) {
   return p;
 }
 
-int64_t ScanCounts(Ctx& ctx, const int* counts, int64_t n, int* ptr) {
+int64_t ScanCounts(Ctx& ctx, const int* counts, int64_t n, int* ptr, bool read_total) {
   if (n > 0 && n <= kScanSmallMax) {
     auto total = ctx.Alloc<int64_t>(1);
     ScanSmall<<<1, 1024, 0, ctx.stream>>>(counts, n, ptr, total.get());
+    if (!read_total) return -1;
     int64_t t = 0;
     ctx.ReadBack(&t, total.get(), sizeof(t));
     if (t > kMaxDeviceNnz) throw std::runtime_error("result exceeds 2^31-1 stored entries on the device");
@@ -2107,6 +2113,11 @@ DMat PruneZeros(Ctx& ctx, DMat&& X) {
   const int64_t nnz = X.nnz();
   if (nnz == 0) return std::move(X);
   auto keep = ctx.Alloc<unsigned char>(nnz);
+  if (nnz <= kPruneFilterAlways) {
+    // small: filter without first reading back whether anything is dropped
+    FlagNonZero<<<Blocks(nnz), kBlock, 0, ctx.stream>>>(nnz, X.v(), keep.get(), nullptr);
+    return Filter(ctx, X, keep.get());
+  }
   auto dropped = ctx.Alloc<int>(1);
   SPAI_HIP_CHECK(hipMemsetAsync(dropped.get(), 0, sizeof(int), ctx.stream));
   FlagNonZero<<<Blocks(nnz), kBlock, 0, ctx.stream>>>(nnz, X.v(), keep.get(), dropped.get());
@@ -2272,7 +2283,7 @@ DMat Transpose(Ctx& ctx, const DMat& X) {
   DMat T;
   T.pat = MakePattern(ctx, cols, rows, nnz);
   T.val = ctx.Alloc<double>(nnz > 0 ? nnz : 1);
-  ScanCounts(ctx, counts.get(), rows, T.pat->ptr.get());
+  ScanCounts(ctx, counts.get(), rows, T.pat->ptr.get(), false);  // the sum is nnz
   if (nnz == 0) return T;
   SPAI_HIP_CHECK(hipMemsetAsync(counts.get(), 0, sizeof(int) * rows, ctx.stream));
   ScatterTranspose<<<(unsigned)cols, 64, 0, ctx.stream>>>((int)cols, X.ptr(), X.idx(), X.v(), T.pat->ptr.get(),
@@ -2405,7 +2416,7 @@ DMat DropByKeys(Ctx& ctx, const DMat& X, const uint64_t* keys, int64_t drop) {
   auto flags = ctx.Alloc<int>(nnz);
   auto scan = ctx.Alloc<int>(nnz + 1);
   EqualFlags<<<Blocks(nnz), kBlock, 0, ctx.stream>>>(nnz, keys, state.get(), flags.get());
-  ScanCounts(ctx, flags.get(), nnz, scan.get());
+  ScanCounts(ctx, flags.get(), nnz, scan.get(), false);
   auto pos = ctx.Alloc<int64_t>(1);
   SPAI_HIP_CHECK(hipMemsetAsync(pos.get(), 0xFF, sizeof(int64_t), ctx.stream));
   FindTiePosition<<<Blocks(nnz), kBlock, 0, ctx.stream>>>(nnz, flags.get(), scan.get(), state.get(), pos.get());
