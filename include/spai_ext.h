@@ -100,6 +100,14 @@ int64_t spai_matvec(struct CsrData A, const double* x, double* y);
 struct Result spai_spgemm(struct MatrixData X, struct MatrixData Y, int64_t residual, int64_t prune_zero,
                           int64_t repeats, double* seconds_out);
 
+/* Per-column top-k by |v| on the device: each column keeps its k off-diagonal
+ * entries of largest magnitude (ties: smaller row first) and its diagonal
+ * entry.  The column form of the magnitude budget of
+ * src/common/dropping.cpp:177-224 (DropForMatrixType), used by
+ * column_top_k in spai_run_ex.  X is CSC with sorted rows; the result, in the
+ * same form, is released by free_mem(). */
+struct Result spai_column_topk(struct MatrixData X, int64_t k);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* spai_last_error(void);
 

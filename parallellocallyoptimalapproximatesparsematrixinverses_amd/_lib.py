@@ -85,6 +85,8 @@ def load():
     L.spai_run_ex.argtypes = [ctypes.c_int64, MatrixData, MatrixData, MatrixData, MethodParams, SpaiExtParams, F64P]
     L.spai_spgemm.restype = Result
     L.spai_spgemm.argtypes = [MatrixData, MatrixData, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, F64P]
+    L.spai_column_topk.restype = Result
+    L.spai_column_topk.argtypes = [MatrixData, ctypes.c_int64]
     L.spai_pcg.restype = ctypes.c_int64
     L.spai_pcg.argtypes = [CsrData, CsrData, F64P, ctypes.c_int64, ctypes.c_double, F64P, I64P, F64P, ctypes.c_int64]
     L.spai_matvec.restype = ctypes.c_int64
@@ -225,6 +227,15 @@ def spgemm(X, Y, residual=False, prune_zero=False, repeats=1):
                         int(repeats), ctypes.byref(seconds))
     out = result_to_python(res)
     return out["M"], seconds.value
+
+
+def column_topk(X, k):
+    """Per-column top-k by |v| (ties: smaller row first), diagonal kept, on
+    the device; returns CSC."""
+    L = load()
+    X_csc = as_csc(X)  # keeps the arrays matrix_data points into alive across the call
+    res = L.spai_column_topk(matrix_data(X_csc), int(k))
+    return result_to_python(res)["M"]
 
 
 def pcg(A, M, b, max_iterations, tolerance, m_fp32=False):

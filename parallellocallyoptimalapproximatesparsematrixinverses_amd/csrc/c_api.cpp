@@ -185,6 +185,28 @@ Result spai_spgemm(MatrixData X, MatrixData Y, int64_t residual, int64_t prune_z
   }
 }
 
+Result spai_column_topk(MatrixData X, int64_t k) {
+  try {
+    spai::ValidateMatrixData(X);
+    if (k < 0) throw std::invalid_argument("k must be non-negative");
+    spai::Ctx ctx;
+    const spai::DMat x = spai::Upload(ctx, X.rows, X.cols, X.col_offsets, X.row_indices, X.values);
+    const spai::DMat c = spai::ColumnTopK(ctx, x, k);
+    spai::EngineOutput out;
+    out.M.rows = c.rows();
+    out.M.cols = c.cols();
+    out.M.ptr.resize(c.cols() + 1);
+    out.M.idx.resize(c.nnz());
+    out.M.val.resize(c.nnz());
+    spai::Download(ctx, c, out.M.ptr.data(), out.M.idx.data(), out.M.val.data());
+    return SuccessResult(out);
+  } catch (const std::exception& ex) {
+    return ErrorResult(ex.what());
+  } catch (...) {
+    return ErrorResult("Unknown failure.");
+  }
+}
+
 int64_t spai_pcg(CsrData A, CsrData M, const double* b, int64_t max_iterations, double tolerance, double* history,
                  int64_t* count, double* x_out, int64_t m_fp32) {
   try {
