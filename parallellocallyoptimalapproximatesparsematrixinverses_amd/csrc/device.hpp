@@ -8,6 +8,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -28,15 +29,78 @@ namespace spai {
 // Largest index the device kernels address with int32 offsets.
 constexpr int64_t kMaxDeviceNnz = (int64_t(1) << 31) - 1;
 
+// Process-wide cache of idle device blocks, per device, shared by the pools
+// of successive contexts.  A block enters it only when its pool is destroyed,
+// after the owning context has synchronised its stream, so no queued work
+// still uses it.  It holds at most kCacheBytes (blocks of at most
+// kCacheBlockBytes); the rest is freed.  Never destroyed: freeing at process
+// exit could run after the HIP runtime has shut down.
+class BlockCache {
+ public:
+  static constexpr size_t kCacheBytes = size_t(2) << 30;
+  static constexpr size_t kCacheBlockBytes = size_t(512) << 20;
+
+  static BlockCache& Get() {
+    static BlockCache* cache = new BlockCache();
+    return *cache;
+  }
+  // A cached block of [sz, 2 sz) bytes on `device` (its size in *got), or null.
+  void* Take(int device, size_t sz, size_t* got) {
+    std::lock_guard<std::mutex> lock(mu_);
+    auto& m = blocks_[device];
+    auto it = m.lower_bound(sz);
+    if (it == m.end() || it->first >= 2 * sz) return nullptr;
+    void* p = it->second;
+    *got = it->first;
+    bytes_ -= it->first;
+    m.erase(it);
+    return p;
+  }
+  // Keeps an idle block, or frees it when the cache is full.
+  void Give(int device, size_t sz, void* p) {
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      if (sz <= kCacheBlockBytes && bytes_ + sz <= kCacheBytes) {
+        blocks_[device].emplace(sz, p);
+        bytes_ += sz;
+        return;
+      }
+    }
+    (void)hipFree(p);
+  }
+  // Frees every cached block of `device` (to make room for an allocation).
+  void Drain(int device) {
+    std::multimap<size_t, void*> m;
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      m.swap(blocks_[device]);
+      for (auto& kv : m) bytes_ -= kv.first;
+    }
+    for (auto& kv : m) (void)hipFree(kv.second);
+  }
+
+ private:
+  std::mutex mu_;
+  std::map<int, std::multimap<size_t, void*>> blocks_;
+  size_t bytes_ = 0;
+};
+
 // Caching device allocator.  Every kernel of one build runs on one stream, so
 // a block released after its last use can be handed to a later launch on the
 // same stream without synchronisation (stream order makes the reuse safe).
+// Blocks come from the pool's own idle list, then from the process-wide
+// cache, then from hipMalloc; when the pool is destroyed (its context has
+// synchronised) its idle blocks go back to the cache.
 class Pool {
  public:
   Pool() = default;
   Pool(const Pool&) = delete;
   Pool& operator=(const Pool&) = delete;
-  ~Pool() { Trim(); }
+  ~Pool() {
+    for (auto& kv : free_) BlockCache::Get().Give(device_, kv.first, kv.second);
+    free_.clear();
+  }
+  void SetDevice(int device) { device_ = device; }
 
   void* Alloc(size_t bytes) {
     if (bytes == 0) bytes = 1;
@@ -47,6 +111,12 @@ class Pool {
       live_[p] = it->first;
       free_.erase(it);
       return p;
+    }
+    size_t got = 0;
+    if (void* c = BlockCache::Get().Take(device_, sz, &got)) {
+      live_[c] = got;
+      bytes_reserved_ += got;
+      return c;
     }
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, sz);
@@ -68,12 +138,14 @@ class Pool {
     live_.erase(it);
   }
 
+  // Frees the pool's idle blocks and the cache's blocks of this device.
   void Trim() {
     for (auto& kv : free_) {
       (void)hipFree(kv.second);
       bytes_reserved_ -= kv.first;
     }
     free_.clear();
+    BlockCache::Get().Drain(device_);
   }
 
   size_t BytesReserved() const { return bytes_reserved_; }
@@ -97,6 +169,7 @@ class Pool {
   std::multimap<size_t, void*> free_;
   std::unordered_map<void*, size_t> live_;
   size_t bytes_reserved_ = 0;
+  int device_ = 0;
 };
 
 // Move-only device array drawn from a Pool.

@@ -1936,6 +1936,7 @@ Ctx::Ctx() {
   hipDeviceProp_t prop;
   SPAI_HIP_CHECK(hipGetDeviceProperties(&prop, device));
   num_cus = prop.multiProcessorCount;
+  pool.SetDevice(device);
   SPAI_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   pinned_bytes = 1 << 16;
   SPAI_HIP_CHECK(hipHostMalloc(&pinned, pinned_bytes, hipHostMallocDefault));
@@ -1946,6 +1947,7 @@ Ctx::Ctx(hipStream_t external) {
   hipDeviceProp_t prop;
   SPAI_HIP_CHECK(hipGetDeviceProperties(&prop, device));
   num_cus = prop.multiProcessorCount;
+  pool.SetDevice(device);
   stream = external;
   owns_stream = false;
   pinned_bytes = 1 << 16;
