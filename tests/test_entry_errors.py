@@ -1,0 +1,50 @@
+"""Argument errors of the extension entry points, reported before any device
+work (so they run without a GPU): the reference's contract of a zeroed
+Result with error_message, surfaced by the Python bindings as RuntimeError."""
+import numpy as np
+import pytest
+from scipy import sparse
+
+from parallellocallyoptimalapproximatesparsematrixinverses_amd import _lib
+
+
+def _m(rows, cols):
+    return sparse.random(rows, cols, density=0.2, random_state=0, format="csc")
+
+
+def test_spgemm_argument_errors():
+    with pytest.raises(RuntimeError, match="dimensions do not match"):
+        _lib.spgemm(_m(4, 3), _m(4, 4))
+    with pytest.raises(RuntimeError, match="square result"):
+        _lib.spgemm(_m(5, 4), _m(4, 3), residual=True)
+    with pytest.raises(RuntimeError, match="repeats must be positive"):
+        _lib.spgemm(_m(4, 4), _m(4, 4), repeats=0)
+
+
+def test_column_topk_argument_errors():
+    with pytest.raises(RuntimeError, match="non-negative"):
+        _lib.column_topk(_m(5, 5), -1)
+
+
+def test_malformed_csc_is_rejected():
+    X = _m(6, 6)
+    bad = _CscView(X.shape, X.indptr[::-1], X.indices, X.data)  # decreasing column offsets
+    L = _lib.load()
+    res = L.spai_column_topk(_lib.matrix_data(bad), 2)
+    try:
+        assert res.error_message and b"non-decreasing" in res.error_message
+        assert res.nnz == 0 and res.rows == 0
+    finally:
+        L.free_mem(res)
+
+
+class _CscView:
+    """CSC arrays as int64/float64 without scipy's checks (to pass malformed
+    offsets through to the library)."""
+
+    def __init__(self, shape, indptr, indices, data):
+        self.shape = shape
+        self.nnz = len(data)
+        self.indptr = np.ascontiguousarray(indptr, dtype=np.int64)
+        self.indices = np.ascontiguousarray(indices, dtype=np.int64)
+        self.data = np.ascontiguousarray(data, dtype=np.float64)
