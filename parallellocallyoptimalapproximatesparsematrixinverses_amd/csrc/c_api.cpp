@@ -98,6 +98,7 @@ struct SpaiSession {
   std::unique_ptr<spai::DCsr> a_csr, m_csr;
   std::unique_ptr<spai::PcgSolver> pcg;
   spai::DBuf<double> rhs;
+  spai::DBuf<int> m_perm;  // position on S of each entry of m_csr
   bool fp32 = false;
   bool m_stale = true;
   int64_t last_count = 0;
@@ -392,15 +393,18 @@ int64_t spai_session_build(SpaiSession* s) {
     const int64_t it = s->prob->Build();
     spai::Ctx& c = s->prob->context();
     // PCG operand: CSR of M.  The pattern is fixed, so after the first build
-    // only the values are refreshed and the solver (and its recorded graphs)
-    // is kept.
-    spai::DCsr fresh = spai::CsrFromCsc(c, s->prob->MatrixM());
+    // only the values move, through the transposition kept from the first
+    // build (one gather instead of a transpose), and the solver (and its
+    // recorded graphs) is kept.
     if (!s->m_csr) {
-      s->m_csr = std::make_unique<spai::DCsr>(std::move(fresh));
+      const spai::DMat Md = s->prob->MatrixM();
+      s->m_csr = std::make_unique<spai::DCsr>(spai::CsrFromCsc(c, Md));
       if (s->fp32) spai::MakeFp32Copy(c, *s->m_csr);
+      s->m_perm = spai::TransposePositions(c, Md);
       s->pcg = std::make_unique<spai::PcgSolver>(c, *s->a_csr, *s->m_csr, s->fp32);
     } else {
-      spai::CopyValues(c, *s->m_csr, fresh);
+      s->prob->GatherM(s->m_perm.get(), s->m_csr->nnz, s->m_csr->val.get(),
+                       s->fp32 ? s->m_csr->val32.get() : nullptr);
       s->pcg->RefreshPreconditioner();
       c.Sync();
     }

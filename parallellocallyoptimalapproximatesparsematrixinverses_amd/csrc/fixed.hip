@@ -1484,6 +1484,18 @@ __global__ void ConvertValues(int64_t n, const double* __restrict__ in, V* __res
   if (i < n) out[i] = V(in[i]);
 }
 
+// out_d[i] = m[perm[i]] in FP64, out_f (nullable) the same rounded to FP32
+// (as converting to FP64 and then to FP32 rounds).
+template <class V>
+__global__ void GatherValues(int64_t len, const int* __restrict__ perm, const V* __restrict__ m,
+                             double* __restrict__ out_d, float* __restrict__ out_f) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  const double v = double(m[perm[i]]);
+  out_d[i] = v;
+  if (out_f) out_f[i] = float(v);
+}
+
 template <class V>
 __global__ void ToDouble(int64_t n, const V* __restrict__ in, double* __restrict__ out) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -2088,6 +2100,10 @@ class FixedImpl final : public FixedProblem {
     Md.val = c.Alloc<double>(nnz_s_);
     ToDouble<V><<<Blocks1(nnz_s_), kBlk, 0, c.stream>>>(nnz_s_, M_.get(), Md.val.get());
     return Md;
+  }
+
+  void GatherM(const int* perm, int64_t len, double* out_d, float* out_f) override {
+    if (len > 0) GatherValues<V><<<Blocks1(len), kBlk, 0, ctx_.stream>>>(len, perm, M_.get(), out_d, out_f);
   }
 
  private:

@@ -23,6 +23,9 @@ class FixedProblem {
   virtual void Output(EngineOutput& out) = 0;
   // Device copy of the built M as FP64 CSC on S (for the PCG).
   virtual DMat MatrixM() = 0;
+  // M's values in the order perm gives: out_d[i] = M at position perm[i] of
+  // S, and out_f (when non-null) the same rounded to FP32, for i < len.
+  virtual void GatherM(const int* perm, int64_t len, double* out_d, float* out_f) = 0;
   virtual const DMat& MatrixA() const = 0;
   virtual Ctx& context() = 0;
   // HIP-event timing, on the build's stream, of the per-iteration pass of the

@@ -75,6 +75,10 @@ DMat PruneZeros(Ctx& ctx, DMat&& X);
 // X(:, j) *= f[j] (f on the device), then drop exact zeros.
 DMat ColScale(Ctx& ctx, const DMat& X, const double* f, bool prune_zero);
 DMat Transpose(Ctx& ctx, const DMat& X);
+// For every entry of Transpose(X), in its order, the position of the same
+// entry in X (a fixed pattern's transposition, to move values without
+// transposing again).
+DBuf<int> TransposePositions(Ctx& ctx, const DMat& X);
 
 // ---------------------------------------------------------------- reductions
 double FrobDot(Ctx& ctx, const DMat& X, const DMat& Y);

@@ -2318,6 +2318,30 @@ DMat Transpose(Ctx& ctx, const DMat& X) {
 }
 
 namespace {
+__global__ void IotaValues(int64_t n, double* __restrict__ v) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = double(i);
+}
+
+__global__ void ValuesToPositions(int64_t n, const double* __restrict__ v, int* __restrict__ pos) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) pos[i] = int(v[i]);
+}
+}  // namespace
+
+DBuf<int> TransposePositions(Ctx& ctx, const DMat& X) {
+  const int64_t nnz = X.nnz();
+  DMat I;  // X's pattern with each entry's position as its value (exact in FP64)
+  I.pat = X.pat;
+  I.val = ctx.Alloc<double>(nnz > 0 ? nnz : 1);
+  if (nnz > 0) IotaValues<<<Blocks(nnz), kBlock, 0, ctx.stream>>>(nnz, I.val.get());
+  const DMat T = Transpose(ctx, I);
+  auto pos = ctx.Alloc<int>(nnz > 0 ? nnz : 1);
+  if (nnz > 0) ValuesToPositions<<<Blocks(nnz), kBlock, 0, ctx.stream>>>(nnz, T.v(), pos.get());
+  return pos;
+}
+
+namespace {
 double SumDevice(Ctx& ctx, const double* v, int64_t n) {
   const unsigned g = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (n + kBlock - 1) / kBlock), 1024);
   auto part = ctx.Alloc<double>(g);

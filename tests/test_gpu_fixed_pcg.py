@@ -150,9 +150,38 @@ def test_session_build_and_solve(gpu_lib, oracle_lib):
     assert st == so == 1
     assert abs(cnt - len(ho)) <= 1
     assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) <= 1e-8
-    # rebuild reuses the session
+    # rebuild reuses the session: M's values reach the PCG through the
+    # transposition kept from the first build, so the solve repeats exactly
     assert s.build() == 12
+    st2, cnt2 = s.pcg(5000, 1e-8)
+    hist2, x2 = s.pcg_output(cnt2)
+    assert st2 == st and cnt2 == cnt
+    np.testing.assert_array_equal(hist2, hist)
+    np.testing.assert_array_equal(x2, x)
     s.close()
+
+
+def test_session_fp32_rebuild_repeats_solve(gpu_lib):
+    """FP32-bulk session: the PCG after a rebuild (values moved by the kept
+    transposition, FP32 copy included) repeats the first solve exactly, and
+    matches a fresh session's."""
+    A = problems.diffusion3d(16, seed=3)
+    Pr = problems.diag_inverse(A)
+    b = A @ np.random.default_rng(4).standard_normal(A.shape[0])
+    print("case: session fp32 rebuild diffusion16", flush=True)
+    runs = []
+    for builds in (1, 2):
+        s = gpu_lib.Session("lomr", A, M0=Pr, max_iterations=9, precision="fp32")
+        for _ in range(builds):
+            assert s.build() == 9
+        s.set_rhs(b)
+        st, cnt = s.pcg(5000, 1e-8)
+        runs.append((st, cnt) + s.pcg_output(cnt))
+        s.close()
+    (st1, c1, h1, x1), (st2, c2, h2, x2) = runs
+    assert st1 == st2 == 1 and c1 == c2
+    np.testing.assert_array_equal(h1, h2)
+    np.testing.assert_array_equal(x1, x2)
 
 
 def test_session_kernel_timing(gpu_lib):
