@@ -657,6 +657,14 @@ __global__ __launch_bounds__(kBlk) void PcgSpmvASell(int n, SellRows rows, const
   StorePartial(acc, part + kP1 * gridDim.x);
 }
 
+// Start of a solve: every scalar slot 0 except the tolerance and the
+// iteration limit.
+__global__ void PcgSetScalars(double* __restrict__ ds, int* __restrict__ is, double tolerance, int max_iterations) {
+  const int t = threadIdx.x;
+  if (t < kNumD) ds[t] = t == kTol ? tolerance : 0.0;
+  if (t < kNumI) is[t] = t == kMax ? max_iterations : 0;
+}
+
 // SELL construction: per-slice width (x 64 entries), then the slots.
 __global__ void SellWidths(int64_t n, int64_t nslices, const int* __restrict__ ptr, int* __restrict__ slots) {
   const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -910,11 +918,8 @@ int PcgSolver::Solve(const double* b_dev, int64_t max_iterations, double toleran
     if ((int64_t)hist_.size() < max_iterations) hist_ = ctx_.Alloc<double>(max_iterations);
   }
   b_ = b_dev;
-  const double hd[kNumD] = {0.0, tolerance, 0.0, 0.0, 0.0};
-  int hi[kNumI] = {0};
-  hi[kMax] = (int)max_iterations;
-  SPAI_HIP_CHECK(hipMemcpyAsync(scal_.get(), hd, sizeof(hd), hipMemcpyHostToDevice, st));
-  SPAI_HIP_CHECK(hipMemcpyAsync(ist_.get(), hi, sizeof(hi), hipMemcpyHostToDevice, st));
+  // the solve's scalars, written by a kernel (no copy from pageable memory)
+  PcgSetScalars<<<1, kWave, 0, st>>>(scal_.get(), ist_.get(), tolerance, (int)max_iterations);
   const int sgm = SubgroupFor(M_.n, M_.nnz), sga = SubgroupFor(A_.n, A_.nnz);
   if (m_fp32_) {
     SPAI_SG_DISPATCH(sgm, (PcgInit<SG, float><<<grid_, kBlk, 0, st>>>(n, M_.ptr.get(), M_.col.get(), M_.val32.get(),
