@@ -940,8 +940,11 @@ int PcgSolver::Solve(const double* b_dev, int64_t max_iterations, double toleran
     }
     graph_key_b_ = b_dev;
   }
-  int hst[kNumI];
-  ctx_.ReadBack(hst, ist_.get(), sizeof(hst));
+  // PcgInitFinish leaves iteration 0 pending unless the limit or the
+  // tolerance ends the solve at once: the host knows which without reading
+  int hst[kNumI] = {0};
+  hst[kDone] = (1 < max_iterations && 1.0 > tolerance) ? 0 : 1;
+  hst[kIter] = 0;
   int64_t chunks = 0;
   const int64_t max_chunks = max_iterations / kChunk + 4;
   // launch as many chunks as the previous solve needed before looking at the
