@@ -17,6 +17,12 @@ struct Ctx {
   // pinned host staging for small device-to-host reads
   void* pinned = nullptr;
   size_t pinned_bytes = 0;
+  // Device error code in mapped host memory: kernels store a nonzero code on
+  // a failed consistency check (never otherwise); ReadBack and Sync raise it
+  // after their synchronisation, so such checks cost no read-back of their
+  // own.  It sits after the staging area; err_dev is its device address.
+  int* err_host = nullptr;
+  int* err_dev = nullptr;
 
   Ctx();
   // Runs on a caller-owned stream (not destroyed with the context).
@@ -33,6 +39,10 @@ struct Ctx {
   // Synchronous small device-to-host copy through pinned memory.
   void ReadBack(void* host, const void* dev, size_t bytes);
   void Sync();
+
+ private:
+  void InitHostBuffers();
+  void CheckErrorCode();
 };
 
 // ---------------------------------------------------------------- patterns

@@ -745,7 +745,7 @@ __global__ __launch_bounds__(kNumWaves* kWave) void NumericGustavson(
         const double x = xval[q];
         const int pos = LowerBound(R, 0, L, i);
         if (pos >= L || R[pos] != i) {
-          atomicOr(error_flag, 1);
+          *error_flag = 1;
           continue;
         }
         atomicAdd(&cval[cb + pos], x * y);
@@ -811,7 +811,7 @@ __global__ __launch_bounds__(kNumWaves* kWave) void NumericGustavson(
           if (at < L && R[at] == i)
             pos[r] = at;
           else
-            atomicOr(error_flag, 1);
+            *error_flag = 1;
         }
       }
 #pragma unroll
@@ -913,7 +913,7 @@ __global__ __launch_bounds__(256) void NumericShort(
     if (src[r] >= 0) {
       pos[r] = LowerBound(R, 0, L, row[r]);
       if (pos[r] >= L || R[pos[r]] != row[r]) {
-        atomicOr(error_flag, 1);
+        *error_flag = 1;
         src[r] = -1;
       }
     }
@@ -1938,8 +1938,7 @@ Ctx::Ctx() {
   num_cus = prop.multiProcessorCount;
   pool.SetDevice(device);
   SPAI_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  pinned_bytes = 1 << 16;
-  SPAI_HIP_CHECK(hipHostMalloc(&pinned, pinned_bytes, hipHostMallocDefault));
+  InitHostBuffers();
 }
 
 Ctx::Ctx(hipStream_t external) {
@@ -1950,8 +1949,24 @@ Ctx::Ctx(hipStream_t external) {
   pool.SetDevice(device);
   stream = external;
   owns_stream = false;
+  InitHostBuffers();
+}
+
+// One pinned allocation: the staging area for read-backs, then the error code.
+void Ctx::InitHostBuffers() {
   pinned_bytes = 1 << 16;
-  SPAI_HIP_CHECK(hipHostMalloc(&pinned, pinned_bytes, hipHostMallocDefault));
+  SPAI_HIP_CHECK(hipHostMalloc(&pinned, pinned_bytes + 256, hipHostMallocMapped | hipHostMallocCoherent));
+  err_host = reinterpret_cast<int*>(static_cast<char*>(pinned) + pinned_bytes);
+  *err_host = 0;
+  SPAI_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&err_dev), err_host, 0));
+}
+
+void Ctx::CheckErrorCode() {
+  const int e = *static_cast<volatile int*>(err_host);
+  if (e == 0) return;
+  *err_host = 0;
+  if (e == 1) throw std::runtime_error("SpGEMM numeric phase found a row outside the symbolic pattern");
+  throw std::runtime_error("device consistency check failed (code " + std::to_string(e) + ")");
 }
 
 Ctx::~Ctx() {
@@ -1969,9 +1984,13 @@ void Ctx::ReadBack(void* host, const void* dev, size_t bytes) {
     SPAI_HIP_CHECK(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, stream));
     SPAI_HIP_CHECK(hipStreamSynchronize(stream));
   }
+  CheckErrorCode();
 }
 
-void Ctx::Sync() { SPAI_HIP_CHECK(hipStreamSynchronize(stream)); }
+void Ctx::Sync() {
+  SPAI_HIP_CHECK(hipStreamSynchronize(stream));
+  CheckErrorCode();
+}
 
 PatternPtr MakePattern(Ctx& ctx, int64_t rows, int64_t cols, int64_t nnz) {
   if (nnz > kMaxDeviceNnz) throw std::runtime_error("matrix exceeds 2^31-1 stored entries on the device");
@@ -2215,24 +2234,16 @@ DMat SpGemm(Ctx& ctx, const DMat& X, const DMat& Y, bool prune_zero, bool residu
     CopyShort<<<Blocks(nshort, kShortPerBlock), 256, 0, ctx.stream>>>(
         nshort, short_list, short_rows.get(), short_vals.get(), C.pat->ptr.get(), C.pat->idx.get(), C.val.get());
   // ProductShort takes its rows from its own sort; the other numeric kernels
-  // check theirs against the symbolic pattern.
-  if (nlong == 0 && (nshort == 0 || fuse_short)) {
-    if (prune_zero) return PruneZeros(ctx, std::move(C));
-    return C;
-  }
-  auto err = ctx.Alloc<int>(1);
-  SPAI_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), ctx.stream));
+  // check theirs against the symbolic pattern and report a miss through the
+  // context's device error code (raised at its next synchronisation).
   if (nshort > 0 && !fuse_short)
     NumericShort<<<Blocks(nshort, kShortPerBlock), 256, 0, ctx.stream>>>(
         nshort, short_list, X.ptr(), X.idx(), X.v(), Y.ptr(), Y.idx(), Y.v(), C.pat->ptr.get(), C.pat->idx.get(),
-        C.val.get(), residual ? 1 : 0, err.get());
+        C.val.get(), residual ? 1 : 0, ctx.err_dev);
   if (nlong > 0)
     NumericGustavson<<<Blocks(nlong, kNumWaves), kNumWaves * kWave, 0, ctx.stream>>>(
         nlong, long_list, X.ptr(), X.idx(), X.v(), Y.ptr(), Y.idx(), Y.v(), C.pat->ptr.get(), C.pat->idx.get(),
-        C.val.get(), residual ? 1 : 0, err.get());
-  int e = 0;
-  ctx.ReadBack(&e, err.get(), sizeof(int));
-  if (e) throw std::runtime_error("SpGEMM numeric phase found a row outside the symbolic pattern");
+        C.val.get(), residual ? 1 : 0, ctx.err_dev);
   if (prune_zero) return PruneZeros(ctx, std::move(C));
   return C;
 }
