@@ -1343,15 +1343,31 @@ __global__ void BatchSqNormKernel(int nb, const int* __restrict__ first, const i
 
 // Per-batch ||X_b||^2 with one block per batch: strided per-thread sums,
 // then the block's fixed tree (deterministic, not the sequential order).
+// Tree form of the per-batch squared norms: each batch's entries split into
+// kBatchSplit equal ranges, one block each (block sums), then the ranges'
+// sums added in range order (BatchSqNormJoin).
+constexpr int kBatchSplit = 16;
 __global__ __launch_bounds__(256) void BatchSqNormTreeKernel(const int* __restrict__ first,
                                                              const int* __restrict__ ptr,
-                                                             const double* __restrict__ v, double* __restrict__ out) {
+                                                             const double* __restrict__ v,
+                                                             double* __restrict__ part) {
   __shared__ double red[16];
-  const int b = blockIdx.x;
+  const int b = blockIdx.x / kBatchSplit, piece = blockIdx.x % kBatchSplit;
+  const int64_t p0 = ptr[first[b]], p1 = ptr[first[b + 1]];
+  const int64_t len = (p1 - p0 + kBatchSplit - 1) / kBatchSplit;
+  const int64_t q0 = min(p1, p0 + piece * len), q1 = min(p1, q0 + len);
   double s = 0.0;
-  for (int p = ptr[first[b]] + threadIdx.x; p < ptr[first[b + 1]]; p += blockDim.x) s += v[p] * v[p];
+  for (int64_t p = q0 + threadIdx.x; p < q1; p += blockDim.x) s += v[p] * v[p];
   s = BlockSum<double>(s, red);
-  if (threadIdx.x == 0) out[b] = s;
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void BatchSqNormJoin(int nbatches, const double* __restrict__ part, double* __restrict__ out) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbatches) return;
+  double s = 0.0;
+  for (int i = 0; i < kBatchSplit; ++i) s += part[b * kBatchSplit + i];
+  out[b] = s;
 }
 
 // Per-batch sequential intersection dot over the batch's columns (CSC order).
@@ -2406,8 +2422,12 @@ double BatchOrderedDot(Ctx& ctx, const DMat& X, const DMat& Y, const int* batch_
 void BatchSqNorms(Ctx& ctx, const DMat& X, const int* batch_first, int nbatches, double* out, bool ordered) {
   if (ordered)
     BatchSqNormKernel<<<Blocks(nbatches), kBlock, 0, ctx.stream>>>(nbatches, batch_first, X.ptr(), X.v(), out);
-  else if (nbatches > 0)
-    BatchSqNormTreeKernel<<<(unsigned)nbatches, 256, 0, ctx.stream>>>(batch_first, X.ptr(), X.v(), out);
+  else if (nbatches > 0) {
+    auto part = ctx.Alloc<double>(size_t(nbatches) * kBatchSplit);
+    BatchSqNormTreeKernel<<<(unsigned)nbatches * kBatchSplit, 256, 0, ctx.stream>>>(batch_first, X.ptr(), X.v(),
+                                                                                    part.get());
+    BatchSqNormJoin<<<Blocks(nbatches), kBlock, 0, ctx.stream>>>(nbatches, part.get(), out);
+  }
 }
 
 DMat ThresholdColumns(Ctx& ctx, const DMat& M, double eps) {
