@@ -655,21 +655,6 @@ __global__ __launch_bounds__(kSymWaves* kWave) void SymbolicEsc(
   }
 }
 
-// Fill phase of the LDS-tier columns after a stashing count pass: copies each
-// column's distinct rows from stash[soff[g] ...] to its place in C.
-__global__ __launch_bounds__(kBlock) void CopyMid(int nlist, const int* __restrict__ list,
-                                                  const int64_t* __restrict__ upper, const int* __restrict__ soff,
-                                                  const int* __restrict__ stash, const int* __restrict__ cptr,
-                                                  int* __restrict__ cidx) {
-  const int g = blockIdx.x * (kBlock / kWave) + WaveInBlock();
-  if (g >= nlist) return;
-  const int j = list[g];
-  if (upper[j] > kEscCap) return;  // the stamp kernel's column
-  const int cb = cptr[j], L = cptr[j + 1] - cb;
-  const int* const src = stash + soff[g];
-  for (int e = LaneId(); e < L; e += kWave) cidx[cb + e] = src[e];
-}
-
 // Symbolic phase for columns whose expansion exceeds the LDS tier: a block per
 // column marks rows in a per-block stamp array (length nrows, initialised to
 // -1) and compacts them in row order.
@@ -719,8 +704,9 @@ constexpr int kNumRegs = 4;
 __global__ __launch_bounds__(kNumWaves* kWave) void NumericGustavson(
     int nlist, const int* __restrict__ list, const int* __restrict__ xptr, const int* __restrict__ xidx,
     const double* __restrict__ xval, const int* __restrict__ yptr, const int* __restrict__ yidx,
-    const double* __restrict__ yval, const int* __restrict__ cptr, const int* __restrict__ cidx,
-    double* __restrict__ cval, int residual, int* __restrict__ error_flag) {
+    const double* __restrict__ yval, const int* __restrict__ cptr, int* __restrict__ cidx,
+    double* __restrict__ cval, int residual, int* __restrict__ error_flag, const int64_t* __restrict__ upper,
+    const int* __restrict__ soff, const int* __restrict__ stash) {
   __shared__ int rows_s[kNumWaves][kNumCap];
   __shared__ double acc_s[kNumWaves][kNumCap];
   __shared__ int pre_s[kNumWaves][kWave];
@@ -733,10 +719,14 @@ __global__ __launch_bounds__(kNumWaves* kWave) void NumericGustavson(
   const int cb = cptr[j];
   const int L = cptr[j + 1] - cb;
   const int y0 = yptr[j], ye = yptr[j + 1];
+  // rows of an LDS-tier column whose count pass stashed them (the fill pass
+  // skipped it): read from the stash and placed in C here
+  const bool stashed = stash != nullptr && upper[j] <= kEscCap;
+  const int* const rows = stashed ? stash + soff[g] : cidx + cb;
   if (L > kNumCap) {
     for (int e = lane; e < L; e += kWave) cval[cb + e] = 0.0;
     WaveFence();
-    const int* R = cidx + cb;
+    const int* R = rows;
     for (int p = y0; p < ye; ++p) {
       const int k = yidx[p];
       const double y = yval[p];
@@ -758,6 +748,7 @@ __global__ __launch_bounds__(kNumWaves* kWave) void NumericGustavson(
         v = -v;
         if (R[e] == j) v += 1.0;
       }
+      if (stashed) cidx[cb + e] = R[e];
       cval[cb + e] = v;
     }
     return;
@@ -765,7 +756,7 @@ __global__ __launch_bounds__(kNumWaves* kWave) void NumericGustavson(
   int* const R = rows_s[w];
   double* const acc = acc_s[w];
   for (int e = lane; e < L; e += kWave) {
-    R[e] = cidx[cb + e];
+    R[e] = rows[e];
     acc[e] = 0.0;
   }
   for (int p0 = y0; p0 < ye; p0 += kWave) {
@@ -834,6 +825,7 @@ __global__ __launch_bounds__(kNumWaves* kWave) void NumericGustavson(
       v = -v;
       if (R[e] == j) v += 1.0;
     }
+    if (stashed) cidx[cb + e] = R[e];
     cval[cb + e] = v;
   }
 }
@@ -2218,10 +2210,8 @@ DMat SpGemm(Ctx& ctx, const DMat& X, const DMat& Y, bool prune_zero, bool residu
     if (nshort > 0 && !fuse_short)
       SymbolicShort<<<Blocks(nshort, kShortPerBlock), 256, 0, ctx.stream>>>(
           nshort, short_list, X.ptr(), X.idx(), Y.ptr(), Y.idx(), residual ? 1 : 0, fill, cnt, cptr, cidx);
-    if (nlong > nbig && fill && stash_mid)
-      CopyMid<<<Blocks(nlong, kBlock / kWave), kBlock, 0, ctx.stream>>>(nlong, long_list, upper.get(),
-                                                                        mid_off.get(), mid_stash.get(), cptr, cidx);
-    else if (nlong > nbig)
+    // with the stash, the numeric phase places the LDS-tier columns' rows
+    if (nlong > nbig && !(fill && stash_mid))
       SymbolicEsc<<<Blocks(nlong, kSymWaves), kSymWaves * kWave, 0, ctx.stream>>>(
           nlong, long_list, X.ptr(), X.idx(), Y.ptr(), Y.idx(), upper.get(), residual ? 1 : 0, fill, cnt, cptr,
           cidx, stash_mid ? mid_off.get() : nullptr, stash_mid ? mid_stash.get() : nullptr);
@@ -2259,7 +2249,8 @@ DMat SpGemm(Ctx& ctx, const DMat& X, const DMat& Y, bool prune_zero, bool residu
   if (nlong > 0)
     NumericGustavson<<<Blocks(nlong, kNumWaves), kNumWaves * kWave, 0, ctx.stream>>>(
         nlong, long_list, X.ptr(), X.idx(), X.v(), Y.ptr(), Y.idx(), Y.v(), C.pat->ptr.get(), C.pat->idx.get(),
-        C.val.get(), residual ? 1 : 0, ctx.err_dev);
+        C.val.get(), residual ? 1 : 0, ctx.err_dev, upper.get(), stash_mid ? mid_off.get() : nullptr,
+        stash_mid ? mid_stash.get() : nullptr);
   if (prune_zero) return PruneZeros(ctx, std::move(C));
   return C;
 }
