@@ -458,7 +458,8 @@ __global__ __launch_bounds__(256) void CopyShort(int nlist, const int* __restric
 
 // Symbolic phase, one wavefront per output column whose expansion fits the
 // LDS tier: counts (fill == 0) or writes in row order (fill == 1) the
-// distinct rows of X(:, k) over k in Y(:, j) (and j itself when add_identity).
+// distinct rows of X(:, k) over k in Y(:, j) (and j itself when add_identity);
+// with fill == 0 and a stash, also writes them to stash[soff[g] ...].
 // X's columns are sorted, so their first and last rows bound the column's row
 // range.  When that range spans fewer than 32 kEscCap rows, the rows are
 // marked in an LDS bitmap of the range and read back in order (popcounts per
@@ -2177,7 +2178,8 @@ DMat SpGemm(Ctx& ctx, const DMat& X, const DMat& Y, bool prune_zero, bool residu
   const int nshort = tiers.n[0], nlong = tiers.n[1], nbig = tiers.n[2];
   // Many LDS-tier columns: their count pass stashes the sorted rows (at the
   // offsets of a scan of their expansions, when those stay under 1 GiB) and
-  // the fill pass copies them, instead of repeating the gather.
+  // the numeric phase places them, instead of a fill pass repeating the
+  // gather.
   const bool stash_mid = nlong - nbig >= kStashMidColumns && tiers.mid_expansion <= (1ull << 28);
   DBuf<int> mid_off, mid_stash;
   if (stash_mid) {
