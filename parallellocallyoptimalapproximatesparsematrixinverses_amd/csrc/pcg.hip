@@ -36,7 +36,8 @@ namespace spai {
 namespace {
 
 constexpr int kBlk = 512;
-constexpr int kChunk = 16;  // iterations per captured graph (even)
+constexpr int kChunk = 16;     // iterations per captured graph (even)
+constexpr int kTailChunk = 2;  // iterations per graph after the refresh (even)
 
 enum DSlot : int { kBnorm = 0, kTol, kRz0, kRz1, kErrPre, kNumD };
 enum ISlot : int {
@@ -848,8 +849,12 @@ void PcgSolver::DropGraphs() {
   for (int i = 0; i < 2; ++i) {
     if (exec_[i]) (void)hipGraphExecDestroy(exec_[i]);
     if (graph_[i]) (void)hipGraphDestroy(graph_[i]);
+    if (tail_exec_[i]) (void)hipGraphExecDestroy(tail_exec_[i]);
+    if (tail_graph_[i]) (void)hipGraphDestroy(tail_graph_[i]);
     exec_[i] = nullptr;
     graph_[i] = nullptr;
+    tail_exec_[i] = nullptr;
+    tail_graph_[i] = nullptr;
   }
 }
 
@@ -937,6 +942,10 @@ int PcgSolver::Solve(const double* b_dev, int64_t max_iterations, double toleran
       Record(par, kChunk);
       SPAI_HIP_CHECK(hipStreamEndCapture(st, &graph_[par]));
       SPAI_HIP_CHECK(hipGraphInstantiate(&exec_[par], graph_[par], nullptr, nullptr, 0));
+      SPAI_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      Record(par, kTailChunk);
+      SPAI_HIP_CHECK(hipStreamEndCapture(st, &tail_graph_[par]));
+      SPAI_HIP_CHECK(hipGraphInstantiate(&tail_exec_[par], tail_graph_[par], nullptr, nullptr, 0));
     }
     graph_key_b_ = b_dev;
   }
@@ -945,17 +954,36 @@ int PcgSolver::Solve(const double* b_dev, int64_t max_iterations, double toleran
   int hst[kNumI] = {0};
   hst[kDone] = (1 < max_iterations && 1.0 > tolerance) ? 0 : 1;
   hst[kIter] = 0;
-  int64_t chunks = 0;
-  const int64_t max_chunks = max_iterations / kChunk + 4;
-  // launch as many chunks as the previous solve needed before looking at the
-  // state (chunks after the stop return at once); then one at a time
-  int64_t ahead = std::max<int64_t>(1, (last_iterations_ + kChunk - 1) / kChunk);
-  while (!hst[kDone] && chunks < max_chunks) {
-    const int64_t k = std::min<int64_t>(ahead, max_chunks - chunks);
+  // the iteration limit bounds the launched iterations: kChunk per chunk,
+  // kTailChunk per tail chunk
+  int64_t launched = 0;
+  const int64_t max_launched = max_iterations + 4 * kChunk;
+  bool tail = false, first = true;
+  int64_t ahead = 1;
+  while (!hst[kDone] && launched < max_launched) {
+    int64_t full = 0, rest = 0;
+    if (first) {
+      // as many iterations as the previous solve ran, plus the one whose K1
+      // finds the tolerance crossing (at least one chunk), whole chunks and
+      // then tail chunks, before looking at the state (iterations after the
+      // stop return at once)
+      const int64_t want = std::max<int64_t>(kChunk, last_iterations_ + 1);
+      full = want / kChunk;
+      rest = (want % kChunk + kTailChunk - 1) / kTailChunk;
+      first = false;
+    } else if (tail) {
+      // past the refresh: tail chunks, doubling each round in case the solve
+      // goes on
+      rest = ahead;
+      ahead = std::min<int64_t>(2 * ahead, 64);
+    } else {
+      full = 1;  // then one chunk at a time
+    }
     // chunks hold an even number of iterations: each starts at the same parity
-    for (int64_t c = 0; c < k; ++c) SPAI_HIP_CHECK(hipGraphLaunch(exec_[hst[kIter] & 1], st));
-    chunks += k;
-    ahead = 1;
+    const int par0 = hst[kIter] & 1;
+    for (int64_t c = 0; c < full; ++c) SPAI_HIP_CHECK(hipGraphLaunch(exec_[par0], st));
+    for (int64_t c = 0; c < rest; ++c) SPAI_HIP_CHECK(hipGraphLaunch(tail_exec_[par0], st));
+    launched += full * kChunk + rest * kTailChunk;
     ctx_.ReadBack(hst, ist_.get(), sizeof(hst));
     if (hst[kNeedRefresh]) {
       // the last K2 had parity par and left r' in buffer par ^ 1
@@ -974,6 +1002,8 @@ int PcgSolver::Solve(const double* b_dev, int64_t max_iterations, double toleran
       }
       PcgRefreshFinish<<<1, kBlk, 0, st>>>(par, part_.get(), (int)grid_, scal_.get(), ist_.get(), hist_.get());
       ctx_.ReadBack(hst, ist_.get(), sizeof(hst));
+      tail = true;  // past the tolerance crossing only a few iterations remain
+      ahead = 1;
     }
   }
   SPAI_SG_DISPATCH(sga, (PcgFinal<SG><<<grid_, kBlk, 0, st>>>(n, A_.ptr.get(), A_.col.get(), A_.val.get(), x_.get(),

@@ -76,6 +76,10 @@ class PcgSolver {
   const double* b_ = nullptr;
   hipGraph_t graph_[2] = {nullptr, nullptr};
   hipGraphExec_t exec_[2] = {nullptr, nullptr};
+  // kTailChunk-iteration graphs, launched after the tolerance-crossing
+  // refresh, when only a few iterations remain
+  hipGraph_t tail_graph_[2] = {nullptr, nullptr};
+  hipGraphExec_t tail_exec_[2] = {nullptr, nullptr};
   const double* graph_key_b_ = nullptr;
   int64_t last_iterations_ = 0;  // iterations of the previous solve, for the launch-ahead
 };
