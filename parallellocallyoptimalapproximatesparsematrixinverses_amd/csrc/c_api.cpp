@@ -390,24 +390,26 @@ void spai_session_destroy(SpaiSession* s) { delete s; }
 int64_t spai_session_build(SpaiSession* s) {
   try {
     if (!s) throw std::invalid_argument("null session");
-    const int64_t it = s->prob->Build();
     spai::Ctx& c = s->prob->context();
     // PCG operand: CSR of M.  The pattern is fixed, so after the first build
     // only the values move, through the transposition kept from the first
-    // build (one gather instead of a transpose), and the solver (and its
-    // recorded graphs) is kept.
+    // build (one gather instead of a transpose), queued before the build's
+    // final read-back; the solver (and its recorded graphs) is kept.
     if (!s->m_csr) {
+      const int64_t it = s->prob->Build();
       const spai::DMat Md = s->prob->MatrixM();
       s->m_csr = std::make_unique<spai::DCsr>(spai::CsrFromCsc(c, Md));
       if (s->fp32) spai::MakeFp32Copy(c, *s->m_csr);
       s->m_perm = spai::TransposePositions(c, Md);
       s->pcg = std::make_unique<spai::PcgSolver>(c, *s->a_csr, *s->m_csr, s->fp32);
-    } else {
+      return it;
+    }
+    const int64_t it = s->prob->BuildThen([&] {
       s->prob->GatherM(s->m_perm.get(), s->m_csr->nnz, s->m_csr->val.get(),
                        s->fp32 ? s->m_csr->val32.get() : nullptr);
       s->pcg->RefreshPreconditioner();
-      c.Sync();
-    }
+    });
+    c.Sync();
     return it;
   } catch (const std::exception& ex) {
     return Fail(ex.what());

@@ -1835,6 +1835,11 @@ class FixedImpl final : public FixedProblem {
 
   int64_t FinishFusedBuild() {
     SPAI_HIP_CHECK(hipGetLastError());
+    if (enqueue_after_) {
+      const std::function<void()>* f = enqueue_after_;
+      enqueue_after_ = nullptr;  // called here: BuildThen does not call it again
+      (*f)();
+    }
     Flags f;
     ctx_.ReadBack(&f, flags_.get(), sizeof(Flags));
     last_flags_ = f;
@@ -1976,6 +1981,22 @@ class FixedImpl final : public FixedProblem {
 #undef SPAI_OCC_GL
 #undef SPAI_OCC
     return nb;
+  }
+
+  int64_t BuildThen(const std::function<void()>& enqueue_after) override {
+    // the fused builds call it before their read-back; the others after
+    enqueue_after_ = &enqueue_after;
+    int64_t it = 0;
+    try {
+      it = Build();
+    } catch (...) {
+      enqueue_after_ = nullptr;
+      throw;
+    }
+    const bool done = enqueue_after_ == nullptr;
+    enqueue_after_ = nullptr;
+    if (!done) enqueue_after();
+    return it;
   }
 
   int64_t Build() override {
@@ -2135,6 +2156,7 @@ class FixedImpl final : public FixedProblem {
   std::vector<hipEvent_t> events_;
   double timed_us_ = 0.0;
   int64_t timed_launches_ = 0;
+  const std::function<void()>* enqueue_after_ = nullptr;  // see BuildThen
   unsigned gridf_ = 1;
   DBuf<double> partf_;
   DBuf<int2> a0c_;

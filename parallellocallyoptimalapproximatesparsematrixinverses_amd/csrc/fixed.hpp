@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <memory>
 
 #include "engine.hpp"
@@ -20,6 +21,10 @@ class FixedProblem {
   virtual ~FixedProblem() = default;
   // Runs the build from M0; returns the iterations completed.
   virtual int64_t Build() = 0;
+  // The same, calling enqueue_after() (which queues more work on the build's
+  // stream) before the build's final read-back, so that work needs no
+  // synchronisation of its own.
+  virtual int64_t BuildThen(const std::function<void()>& enqueue_after) = 0;
   virtual void Output(EngineOutput& out) = 0;
   // Device copy of the built M as FP64 CSC on S (for the PCG).
   virtual DMat MatrixM() = 0;
