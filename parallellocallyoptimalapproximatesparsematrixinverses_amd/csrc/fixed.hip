@@ -1057,25 +1057,52 @@ struct StoreTwoEpi {
 
 // Per-column step sizes of inner_outer_common.hpp:50-69: sequential column dots
 // over S2 (row order), delta = <r, az> / ||az||^2 or the 2x2 solve, and the
-// normalisation 1 / delta.  FP64 throughout.
+// normalisation 1 / delta.  FP64 throughout.  A wavefront takes 64
+// consecutive columns: their S2 entries are contiguous, so the wavefront
+// stages them through LDS kIoChunk at a time with coalesced loads, and each
+// lane adds its own column's entries from there in row order.
+constexpr int kIoChunk = 512;
 template <class V>
-__global__ void IoColumnSteps(int n, const int* __restrict__ s2p, const V* __restrict__ R, const V* __restrict__ AZ,
-                              const V* __restrict__ AQ, int lomr, int two_term, double rank_eps,
-                              double* __restrict__ delta, double* __restrict__ gamma, double* __restrict__ norm) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+__global__ __launch_bounds__(kBlk) void IoColumnSteps(int n, const int* __restrict__ s2p, const V* __restrict__ R,
+                                                      const V* __restrict__ AZ, const V* __restrict__ AQ, int lomr,
+                                                      int two_term, double rank_eps, double* __restrict__ delta,
+                                                      double* __restrict__ gamma, double* __restrict__ norm) {
+  constexpr int W = kBlk / kWave;
+  __shared__ V sR[W][kIoChunk];
+  __shared__ V sZ[W][kIoChunk];
+  __shared__ V sQ[W][kIoChunk];
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int j0 = (blockIdx.x * W + w) * kWave;
+  if (j0 >= n) return;  // the whole wavefront leaves
+  const int j = j0 + lane;
+  const int span0 = s2p[j0], span1 = s2p[min(n, j0 + kWave)];
+  const int b = j < n ? s2p[j] : span1, e = j < n ? s2p[j + 1] : span1;
   double r_az = 0.0, az_sq = 0.0, az_aq = 0.0, aq_sq = 0.0, r_aq = 0.0;
-  for (int q = s2p[j]; q < s2p[j + 1]; ++q) {
-    const double r = R[q], az = AZ[q];
-    r_az += r * az;
-    az_sq += az * az;
-    if (two_term) {
-      const double aq = AQ[q];
-      az_aq += az * aq;
-      aq_sq += aq * aq;
-      r_aq += r * aq;
+  for (int c0 = span0; c0 < span1; c0 += kIoChunk) {
+    const int c1 = min(span1, c0 + kIoChunk);
+    for (int q = c0 + lane; q < c1; q += kWave) {
+      sR[w][q - c0] = R[q];
+      sZ[w][q - c0] = AZ[q];
+      if (two_term) sQ[w][q - c0] = AQ[q];
     }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    const int qe = min(e, c1);
+    for (int q = max(b, c0); q < qe; ++q) {
+      const double r = sR[w][q - c0], az = sZ[w][q - c0];
+      r_az += r * az;
+      az_sq += az * az;
+      if (two_term) {
+        const double aq = sQ[w][q - c0];
+        az_aq += az * aq;
+        aq_sq += aq * aq;
+        r_aq += r * aq;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
   }
+  if (j >= n) return;
   double d, g;
   if (!two_term) {
     d = (isfinite(az_sq) && fabs(az_sq) > 0.0) ? r_az / az_sq : 0.0;
