@@ -167,7 +167,10 @@ __device__ __forceinline__ void SumPartialsBlockFused(const double* __restrict__
 // precomputed once for the fixed pattern ("plan", one uint8/uint16 per product
 // term, laid out in the order the terms are visited), so the iteration does no
 // searching: A's columns, the plan and Y are read contiguously.  Rows inside
-// one A(:, k) are distinct, so lanes never collide within a step.
+// one A(:, k) are distinct, so lanes never collide within a step.  With every
+// A column at most GL long (narrow) and GL <= 16, a lane holds at most one
+// term per step: the slots and values of all GL steps are loaded first and
+// the additions follow in step order.
 template <class V, class SlotT, int GL, int NY, class Epi>
 __global__ __launch_bounds__(kBlk) void SlotProduct(int n, const int* __restrict__ sp, const int* __restrict__ si,
                                                     const V* __restrict__ y0, const V* __restrict__ y1,
@@ -176,7 +179,7 @@ __global__ __launch_bounds__(kBlk) void SlotProduct(int n, const int* __restrict
                                                     const SlotT* __restrict__ plan, const int* __restrict__ s2p,
                                                     const int* __restrict__ s2i, int lmax, Epi epi,
                                                     const double* __restrict__ sc, double* __restrict__ part,
-                                                    const Flags* __restrict__ flags) {
+                                                    const Flags* __restrict__ flags, int narrow) {
   if (flags->stop) return;
   epi.Load(sc);  // epilogues that use the step sizes read them here
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -206,6 +209,32 @@ __global__ __launch_bounds__(kBlk) void SlotProduct(int n, const int* __restrict
         if (NY > 1) my_yb = y1[eb + gl];
       }
       const int ns = min(GL, e1 - eb);
+      if constexpr (GL <= 16) {
+        if (narrow) {
+          int slot_r[GL];
+          V a_r[GL], ya_r[GL], yb_r[GL];
+          bool has_r[GL];
+#pragma unroll
+          for (int st = 0; st < GL; ++st) {
+            const int a0 = __shfl(my_a0, st, GL);
+            const int c = __shfl(my_c, st, GL);
+            ya_r[st] = __shfl(my_ya, st, GL);
+            yb_r[st] = NY > 1 ? __shfl(my_yb, st, GL) : V(0);
+            has_r[st] = st < ns && gl < c;
+            slot_r[st] = has_r[st] ? int(plan[pp + gl]) : 0;
+            a_r[st] = has_r[st] ? av[a0 + gl] : V(0);
+            if (st < ns) pp += c;
+          }
+#pragma unroll
+          for (int st = 0; st < GL; ++st) {
+            if (has_r[st]) {
+              acc[slot_r[st]] += a_r[st] * ya_r[st];
+              if (NY > 1) acc[lmax + slot_r[st]] += a_r[st] * yb_r[st];
+            }
+          }
+          continue;
+        }
+      }
       for (int st = 0; st < ns; ++st) {
         const int a0 = __shfl(my_a0, st, GL);
         const int c = __shfl(my_c, st, GL);
@@ -2206,7 +2235,8 @@ class FixedImpl final : public FixedProblem {
 #define SPAI_SLOT_LAUNCH(SlotT, plan, GLV)                                                                     \
   SlotProduct<V, SlotT, GLV, NY, Epi><<<gridp_, kBlk, smem_, st>>>((int)n_, sp, si, y0, y1, A_.ptr(), av_.get(), \
                                                                    plan_ptr_.get(), plan, s2p, s2i, lmax_, e,  \
-                                                                   sc_.get(), part_.get(), flags_.get())
+                                                                   sc_.get(), part_.get(), flags_.get(),       \
+                                                                   maxcol_ <= gl_ ? 1 : 0)
 #define SPAI_SLOT_GL(SlotT, plan)             \
   switch (gl_) {                              \
     case 8: SPAI_SLOT_LAUNCH(SlotT, plan, 8); break;   \
