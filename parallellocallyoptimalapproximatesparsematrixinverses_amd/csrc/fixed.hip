@@ -1043,6 +1043,7 @@ struct CgProjectEpi {
 // Z = M_outer R sampled on S: (i, j) -> sum_k M_outer(i, k) R(k, j) over k
 // ascending (row i of M_outer = column i of S^T, gathered through stperm,
 // against column j of R on S2).
+constexpr int kOuterRow = 8, kOuterCol = 16;  // list lengths of the register path
 template <class V>
 __global__ __launch_bounds__(kBlk) void SampledOuterProduct(int64_t nnz_s, const int* __restrict__ si,
                                                             const int* __restrict__ scol, const int* __restrict__ stp,
@@ -1055,6 +1056,27 @@ __global__ __launch_bounds__(kBlk) void SampledOuterProduct(int64_t nnz_s, const
     const int i = si[p], j = scol[p];
     int a = stp[i], ae = stp[i + 1], b = s2p[j], be = s2p[j + 1];
     V acc = V(0);
+    if (ae - a <= kOuterRow && be - b <= kOuterCol) {
+      // short lists: both loaded at once, matched by comparisons, the matches
+      // added in k order (the row's order), as the merge below adds them
+      int ka[kOuterRow], mp[kOuterRow], kb[kOuterCol];
+#pragma unroll
+      for (int u = 0; u < kOuterRow; ++u) {
+        ka[u] = a + u < ae ? sti[a + u] : -1;
+        mp[u] = a + u < ae ? stperm[a + u] : 0;
+      }
+#pragma unroll
+      for (int v = 0; v < kOuterCol; ++v) kb[v] = b + v < be ? s2i[b + v] : -2;
+#pragma unroll
+      for (int u = 0; u < kOuterRow; ++u) {
+        int m = -1;
+#pragma unroll
+        for (int v = 0; v < kOuterCol; ++v) m = kb[v] == ka[u] ? v : m;
+        if (m >= 0) acc += mout[mp[u]] * R[b + m];
+      }
+      Z[p] = acc;
+      continue;
+    }
     while (a < ae && b < be) {
       const int ka = sti[a], kb = s2i[b];
       if (ka == kb) {
