@@ -146,6 +146,41 @@ __device__ __forceinline__ bool IsShortColumn(int64_t upper, int ny) {
   return upper <= kShortCap && ny <= kShortGroup;
 }
 
+// Lane exchanges inside a 16-lane group, which is one DPP row: DPP moves and
+// swizzles read other lanes' registers directly, with none of the LDS round
+// trips of a general shuffle.  Every lane of the group must be active.
+// XorLane16(v, m): the v of lane gl ^ m (m in 1, 2, 4, 8; constant after
+// unrolling).
+__device__ __forceinline__ int XorLane16(int v, int m) {
+  switch (m) {
+    case 1: return __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    case 2: return __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    case 4: {  // banks 0 and 2 (lanes 0-3, 8-11) from row_shl 4, banks 1 and 3 from row_shr 4
+      const int a = __builtin_amdgcn_update_dpp(v, v, 0x104, 0xF, 0x5, false);
+      return __builtin_amdgcn_update_dpp(a, v, 0x114, 0xF, 0xA, false);
+    }
+    default: return __builtin_amdgcn_update_dpp(v, v, 0x128, 0xF, 0xF, false); // row_ror 8
+  }
+}
+
+// The v of lane gl - N, or `fill` on lanes gl < N (DPP row_shr N).
+template <int N>
+__device__ __forceinline__ int UpLane16(int v, int fill) {
+  return __builtin_amdgcn_update_dpp(fill, v, 0x110 + N, 0xF, 0xF, false);
+}
+
+// The v of the group's last lane (swizzle: lane (gl & 16) | 15 of 32).
+__device__ __forceinline__ int LastLane16(int v) { return __builtin_amdgcn_ds_swizzle(v, (0xF << 5) | 0x10); }
+
+// Inclusive sum over the group's lanes 0..gl.
+__device__ __forceinline__ int GroupScan16(int v) {
+  v += UpLane16<1>(v, 0);
+  v += UpLane16<2>(v, 0);
+  v += UpLane16<4>(v, 0);
+  v += UpLane16<8>(v, 0);
+  return v;
+}
+
 // Block-wide ordered compaction helper: returns this thread's exclusive rank
 // among flagged threads and writes the block total to *total.
 __device__ int BlockRank(bool flag, int* wcount /* LDS >= 16 */, int* total) {
@@ -252,13 +287,8 @@ __device__ __forceinline__ void ShortLoad(int gl, int j, int y0, int ny, const i
     b = xptr[k];
     c = xptr[k + 1] - b;
   }
-  int incl = c;
-#pragma unroll
-  for (int off = 1; off < G; off <<= 1) {
-    const int t = __shfl_up(incl, off, G);
-    if (gl >= off) incl += t;
-  }
-  const int tot = __shfl(incl, G - 1, G);
+  const int incl = GroupScan16(c);
+  const int tot = LastLane16(incl);
   const int pre = incl - c;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -316,7 +346,7 @@ __device__ __forceinline__ int ShortSortDistinct(int gl, int (&key)[4], int (&fi
           const int i = G * r + gl;
           const bool asc = (i & kk) == 0;
           const bool lower = (gl & jj) == 0;
-          const int o = __shfl_xor(key[r], jj, G);
+          const int o = XorLane16(key[r], jj);
           key[r] = (lower == asc) ? min(key[r], o) : max(key[r], o);
         }
       }
@@ -326,9 +356,8 @@ __device__ __forceinline__ int ShortSortDistinct(int gl, int (&key)[4], int (&fi
   int cnt = 0;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    int prev = __shfl_up(key[r], 1, G);
-    const int last_prev = r > 0 ? __shfl(key[r - 1], G - 1, G) : INT_MIN;
-    if (gl == 0) prev = last_prev;
+    // lane 0 compares with the previous register's last lane
+    const int prev = UpLane16<1>(key[r], r > 0 ? LastLane16(key[r - 1]) : INT_MIN);
     first[r] = key[r] != INT_MAX && (G * r + gl == 0 || key[r] != prev) ? 1 : 0;
     cnt += first[r];
   }
@@ -337,18 +366,12 @@ __device__ __forceinline__ int ShortSortDistinct(int gl, int (&key)[4], int (&fi
 
 // Writes the distinct rows in order to out[0 ...]; returns their number.
 __device__ __forceinline__ int ShortWrite(int gl, const int (&key)[4], const int (&first)[4], int* out) {
-  constexpr int G = kShortGroup;
   int base = 0;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    int sc = first[r];
-#pragma unroll
-    for (int off = 1; off < G; off <<= 1) {
-      const int t = __shfl_up(sc, off, G);
-      if (gl >= off) sc += t;
-    }
+    const int sc = GroupScan16(first[r]);
     if (first[r]) out[base + sc - 1] = key[r];
-    base += __shfl(sc, G - 1, G);
+    base += LastLane16(sc);
   }
   return base;
 }
@@ -868,13 +891,8 @@ __global__ __launch_bounds__(256) void NumericShort(
     b = xptr[k];
     c = xptr[k + 1] - b;
   }
-  int incl = c;
-#pragma unroll
-  for (int off = 1; off < G; off <<= 1) {
-    const int t = __shfl_up(incl, off, G);
-    if (gl >= off) incl += t;
-  }
-  const int tot = __shfl(incl, G - 1, G);
+  const int incl = GroupScan16(c);
+  const int tot = LastLane16(incl);
   const int pre = incl - c;
   int src[4], row[4];
   double prod[4];
