@@ -93,6 +93,8 @@ __global__ __launch_bounds__(kBlock) void ScanApply(const int* __restrict__ in, 
     if (threadIdx.x == kBlock - 1) carry = excl + v;
     __syncthreads();
   }
+  // the last tile ends at n - 1, so its final carry is the total
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = static_cast<int>(carry);
 }
 
 __global__ void WriteLast(int* __restrict__ out, int64_t n, const int64_t* __restrict__ total) {
@@ -2047,11 +2049,12 @@ int64_t ScanCounts(Ctx& ctx, const int* counts, int64_t n, int* ptr, bool read_t
   if (n > 0) {
     ScanTileSums<<<Blocks(n, kScanTile), kBlock, 0, ctx.stream>>>(counts, n, sums.get());
     ScanTiles<<<1, 1024, 0, ctx.stream>>>(sums.get(), tiles, total.get());
-    ScanApply<<<Blocks(n, kScanTile), kBlock, 0, ctx.stream>>>(counts, n, sums.get(), ptr);
+    ScanApply<<<Blocks(n, kScanTile), kBlock, 0, ctx.stream>>>(counts, n, sums.get(), ptr);  // and ptr[n]
   } else {
     SPAI_HIP_CHECK(hipMemsetAsync(total.get(), 0, sizeof(int64_t), ctx.stream));
+    WriteLast<<<1, 1, 0, ctx.stream>>>(ptr, n, total.get());
   }
-  WriteLast<<<1, 1, 0, ctx.stream>>>(ptr, n, total.get());
+  if (!read_total) return -1;
   int64_t t = 0;
   ctx.ReadBack(&t, total.get(), sizeof(t));
   if (t > kMaxDeviceNnz) throw std::runtime_error("result exceeds 2^31-1 stored entries on the device");
