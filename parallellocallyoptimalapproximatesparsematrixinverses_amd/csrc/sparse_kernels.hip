@@ -241,10 +241,21 @@ __global__ __launch_bounds__(kExpBlock) void ExpansionSizes(int ncols, const int
   int64_t u = add_identity ? 1 : 0;
   int ny = 0;
   if (live) {
-    ny = yptr[j + 1] - yptr[j];
-    for (int p = yptr[j]; p < yptr[j + 1]; ++p) {
-      const int k = yidx[p];
-      u += xptr[k + 1] - xptr[k];
+    const int p0 = yptr[j], p1 = yptr[j + 1];
+    ny = p1 - p0;
+    // eight Y entries at a time: their X column bounds are loaded together
+    // rather than one dependent pair per entry (k = -1 past the column's end)
+    for (int p = p0; p < p1; p += 8) {
+      int k[8], lo[8], hi[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) k[t] = p + t < p1 ? yidx[p + t] : -1;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        lo[t] = k[t] >= 0 ? xptr[k[t]] : 0;
+        hi[t] = k[t] >= 0 ? xptr[k[t] + 1] : 0;
+      }
+#pragma unroll
+      for (int t = 0; t < 8; ++t) u += hi[t] - lo[t];
     }
     upper[j] = u;
   }
