@@ -160,6 +160,11 @@ int64_t spai_session_set_kernel_timing(SpaiSession* session, int64_t on);
 /* Average microseconds of that pass in the last build and (launches) how many
  * passes were averaged; negative when the last build was not timed. */
 double spai_session_build_kernel_us(SpaiSession* session, int64_t* launches);
+/* Launch shape of the session's fixed-pattern build, for tests and
+ * measurement: out[0] = 1 when LOMR/MR run the fused per-iteration pass, then
+ * lanes per column, columns per tile, tiles, resident-grid blocks and
+ * wavefronts per block (len entries written, at most 6).  Returns 0, or -1. */
+int64_t spai_session_info(SpaiSession* session, int64_t* out, int64_t len);
 
 /* ------------------------------------------------------------------------
  * Differentiation form: f(M, A) = ||I - A M||_F^2 on fixed patterns.

@@ -115,6 +115,8 @@ def load():
     L.spai_session_set_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int64]
     L.spai_session_build_kernel_us.restype = ctypes.c_double
     L.spai_session_build_kernel_us.argtypes = [ctypes.c_void_p, I64P]
+    L.spai_session_info.restype = ctypes.c_int64
+    L.spai_session_info.argtypes = [ctypes.c_void_p, I64P, ctypes.c_int64]
     _lib = L
     return L
 
@@ -324,6 +326,15 @@ class Session:
         n = ctypes.c_int64(0)
         us = load().spai_session_build_kernel_us(self._h, ctypes.byref(n))
         return float(us), int(n.value)
+
+    def info(self) -> dict:
+        """Launch shape of the build: fused pass used, lanes per column,
+        columns per tile, tiles, resident-grid blocks, wavefronts per block."""
+        out = (ctypes.c_int64 * 6)()
+        if load().spai_session_info(self._h, out, 6) < 0:
+            raise RuntimeError(last_error())
+        keys = ("fused", "lanes_per_column", "columns_per_tile", "tiles", "grid_blocks", "waves_per_block")
+        return {k: int(v) for k, v in zip(keys, out)}
 
     def close(self):
         if self._h:

@@ -340,6 +340,19 @@ double spai_session_build_kernel_us(SpaiSession* s, int64_t* launches) {
   return s->prob->KernelMicros(launches);
 }
 
+int64_t spai_session_info(SpaiSession* s, int64_t* out, int64_t len) {
+  try {
+    if (!s) throw std::invalid_argument("null session");
+    if (!out || len < 0) throw std::invalid_argument("null output");
+    int64_t info[6];
+    s->prob->FusedInfo(info);
+    for (int64_t i = 0; i < len && i < 6; ++i) out[i] = info[i];
+    return 0;
+  } catch (const std::exception& ex) {
+    return Fail(ex.what());
+  }
+}
+
 const char* spai_last_error(void) { return g_last_error.c_str(); }
 
 int64_t spai_device_name(char* buffer, int64_t size) {
@@ -418,6 +431,8 @@ int64_t spai_session_build(SpaiSession* s) {
 
 int64_t spai_session_set_rhs(SpaiSession* s, const double* b) {
   try {
+    if (!s) throw std::invalid_argument("null session");
+    if (!b) throw std::invalid_argument("null right-hand side");
     spai::Ctx& c = s->prob->context();
     SPAI_HIP_CHECK(hipMemcpyAsync(s->rhs.get(), b, sizeof(double) * s->a_csr->n, hipMemcpyHostToDevice, c.stream));
     c.Sync();
@@ -442,6 +457,7 @@ int64_t spai_session_pcg(SpaiSession* s, int64_t max_iterations, double toleranc
 
 int64_t spai_session_pcg_output(SpaiSession* s, double* history, double* x) {
   try {
+    if (!s || !s->pcg) throw std::invalid_argument("session has no built preconditioner");
     spai::Ctx& c = s->prob->context();
     if (history) s->pcg->History(history, s->last_count);
     if (x) {
@@ -467,6 +483,7 @@ Result spai_session_result(SpaiSession* s) {
 
 double spai_session_pcg_iteration_us(SpaiSession* s) {
   try {
+    if (!s || !s->pcg) throw std::invalid_argument("session has no built preconditioner");
     return s->pcg->TimeSpmvA(20);
   } catch (const std::exception& ex) {
     g_last_error = ex.what();

@@ -1929,6 +1929,17 @@ class FixedImpl final : public FixedProblem {
     return timed_launches_ > 0 ? timed_us_ : -1.0;
   }
 
+  void FusedInfo(int64_t out[6]) const override {
+    const bool fused = (method_ == SPAI_GLOBAL_LOMR || method_ == SPAI_GLOBAL_MR) && use_plan_ && fused_ok_;
+    const int64_t w = kWave / gl_;
+    out[0] = fused ? 1 : 0;
+    out[1] = gl_;
+    out[2] = w;
+    out[3] = (n_ + w - 1) / w;
+    out[4] = gridf_;
+    out[5] = kBlk / kWave;
+  }
+
   // Global LOMR or MR with one fused pass per iteration (FusedPass).
   int64_t BuildFused() {
     Ctx& c = ctx_;

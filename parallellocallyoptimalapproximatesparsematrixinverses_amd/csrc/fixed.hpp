@@ -39,6 +39,10 @@ class FixedProblem {
   // Average microseconds of that pass over the iterations the last build ran
   // (launches receives their number); -1 when the last build did not time it.
   virtual double KernelMicros(int64_t* launches) = 0;
+  // Launch shape of the fused pass: out[0] = 1 when the build uses it, then
+  // lanes per column (GL), columns per tile (64 / GL), tiles, blocks of the
+  // resident grid and wavefronts per block.
+  virtual void FusedInfo(int64_t out[6]) const = 0;
 };
 
 std::unique_ptr<FixedProblem> MakeFixedProblem(int64_t method, const MatrixData& A, const MatrixData* M0,

@@ -954,20 +954,23 @@ int PcgSolver::Solve(const double* b_dev, int64_t max_iterations, double toleran
   int hst[kNumI] = {0};
   hst[kDone] = (1 < max_iterations && 1.0 > tolerance) ? 0 : 1;
   hst[kIter] = 0;
-  // the iteration limit bounds the launched iterations: kChunk per chunk,
-  // kTailChunk per tail chunk
-  int64_t launched = 0;
-  const int64_t max_launched = max_iterations + 4 * kChunk;
+  // The device ends the solve by itself: every iteration it runs either
+  // appends a history entry or stops (limit, tolerance crossing pending a
+  // refresh, breakdown), so the loop runs until the device reports done and
+  // each pass of it must append at least one entry.  Launches past the stop
+  // return at once and are not counted as progress.
   bool tail = false, first = true;
   int64_t ahead = 1;
-  while (!hst[kDone] && launched < max_launched) {
+  while (!hst[kDone]) {
+    const int count_before = hst[kCountLatest];
     int64_t full = 0, rest = 0;
     if (first) {
       // as many iterations as the previous solve ran, plus the one whose K1
-      // finds the tolerance crossing (at least one chunk), whole chunks and
-      // then tail chunks, before looking at the state (iterations after the
-      // stop return at once)
-      const int64_t want = std::max<int64_t>(kChunk, last_iterations_ + 1);
+      // finds the tolerance crossing (at least one chunk, at most this
+      // solve's limit), whole chunks and then tail chunks, before looking at
+      // the state (iterations after the stop return at once)
+      const int64_t want =
+          std::min<int64_t>(max_iterations + 1, std::max<int64_t>(kChunk, last_iterations_ + 1));
       full = want / kChunk;
       rest = (want % kChunk + kTailChunk - 1) / kTailChunk;
       first = false;
@@ -983,7 +986,6 @@ int PcgSolver::Solve(const double* b_dev, int64_t max_iterations, double toleran
     const int par0 = hst[kIter] & 1;
     for (int64_t c = 0; c < full; ++c) SPAI_HIP_CHECK(hipGraphLaunch(exec_[par0], st));
     for (int64_t c = 0; c < rest; ++c) SPAI_HIP_CHECK(hipGraphLaunch(tail_exec_[par0], st));
-    launched += full * kChunk + rest * kTailChunk;
     ctx_.ReadBack(hst, ist_.get(), sizeof(hst));
     if (hst[kNeedRefresh]) {
       // the last K2 had parity par and left r' in buffer par ^ 1
@@ -1005,6 +1007,8 @@ int PcgSolver::Solve(const double* b_dev, int64_t max_iterations, double toleran
       tail = true;  // past the tolerance crossing only a few iterations remain
       ahead = 1;
     }
+    if (!hst[kDone] && hst[kCountLatest] <= count_before)
+      throw std::runtime_error("PCG: launched iterations appended no history entry");
   }
   SPAI_SG_DISPATCH(sga, (PcgFinal<SG><<<grid_, kBlk, 0, st>>>(n, A_.ptr.get(), A_.col.get(), A_.val.get(), x_.get(),
                                                               b_dev, part_.get())));

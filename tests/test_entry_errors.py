@@ -48,3 +48,27 @@ class _CscView:
         self.indptr = np.ascontiguousarray(indptr, dtype=np.int64)
         self.indices = np.ascontiguousarray(indices, dtype=np.int64)
         self.data = np.ascontiguousarray(data, dtype=np.float64)
+
+
+def test_session_entry_points_reject_null_session():
+    """Every session entry point answers a null session with -1 and a message
+    (no host crash), before any device work."""
+    import ctypes
+    L = _lib.load()
+    f64 = ctypes.POINTER(ctypes.c_double)
+    buf = (ctypes.c_double * 4)()
+    cnt = ctypes.c_int64(0)
+    out = (ctypes.c_int64 * 6)()
+    calls = [
+        lambda: L.spai_session_build(None),
+        lambda: L.spai_session_set_rhs(None, ctypes.cast(buf, f64)),
+        lambda: L.spai_session_pcg(None, 10, 1e-8, ctypes.byref(cnt)),
+        lambda: L.spai_session_pcg_output(None, ctypes.cast(buf, f64), ctypes.cast(buf, f64)),
+        lambda: L.spai_session_info(None, out, 6),
+    ]
+    for call in calls:
+        assert call() == -1
+        assert _lib.last_error() in ("null session", "session has no built preconditioner")
+    assert L.spai_session_pcg_iteration_us(None) < 0
+    assert _lib.last_error() == "session has no built preconditioner"
+    assert L.spai_session_build_kernel_us(None, ctypes.byref(cnt)) < 0

@@ -293,3 +293,31 @@ def test_fixed_nine_point_product2_paths(gpu_lib, oracle_lib, method, precision)
     else:
         assert rel_fro(g["M"], o["M"]) < 1e-4
         np.testing.assert_allclose(g["history"]["residual_norm"], o["history"]["residual_norm"], rtol=1e-4)
+
+
+@pytest.mark.parametrize("m_fp32", [False, True])
+def test_prepared_pcg_smaller_limit_after_failed_refreshes(gpu_lib, oracle_lib, m_fp32):
+    """One prepared system, two solves: the first to a limit of 2000, the
+    second to 300.  The tolerance 1e-17 lies below the attainable accuracy, so
+    the recursive residual keeps crossing it while the refreshed true residual
+    stays above it, and the solve goes on with beta = 0 after each refresh
+    (backward_error.pyx:101-133).  Both solves must run to their own limit
+    (status 0, history of exactly `limit` entries) as the oracle's do; the
+    second solve must not stop early because the first ran longer."""
+    from parallellocallyoptimalapproximatesparsematrixinverses_amd import harness
+    A = problems.diffusion3d(12, seed=1)
+    Pr = problems.diag_inverse(A)
+    M = oracle_lib.run("lomr", A, Pr, None, 10, 0.0, 1.0, False, 1, fixed_pattern=True)["M"].tocsr()
+    b = A @ np.random.default_rng(2).standard_normal(A.shape[0])
+    system = harness._DeviceSystem(A.tocsr(), M, m_fp32=m_fp32)
+    try:
+        for limit in (2000, 300, 2000):
+            print(f"case: prepared pcg limit {limit} m_fp32={m_fp32}", flush=True)
+            hg, sg, _ = system.solve(b, limit, 1e-17)
+            ho, so, _ = oracle_lib.pcg(A.tocsr(), M, b, limit, 1e-17)
+            assert sg == so == 0
+            assert len(hg) == len(ho) == limit
+            np.testing.assert_allclose(hg[:30], ho[:30], rtol=1e-4 if m_fp32 else 1e-8)
+            assert np.all(hg[100:] < 1e-14)
+    finally:
+        system.close()
