@@ -108,6 +108,27 @@ struct Result spai_spgemm(struct MatrixData X, struct MatrixData Y, int64_t resi
  * same form, is released by free_mem(). */
 struct Result spai_column_topk(struct MatrixData X, int64_t k);
 
+/* Per-column top-k that allocates nothing: the same result as
+ * spai_column_topk, on caller-owned device arrays.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under
+ * stream capture, serving loops, harnesses that forbid hipMalloc).
+ *   X: CSC on the device, int32 column offsets[cols + 1] and row indices[nnz]
+ *      (rows sorted in each column), FP64 values[nnz].
+ *   out_*: device arrays, offsets[cols + 1] and room for nnz entries (the
+ *      result never has more entries than X); the result's nnz is
+ *      out_col_offsets[cols] once the work completes.
+ * Limits: nnz <= 16,777,216 and cols <= 4,194,304 (larger sizes are rejected
+ * with an error); the work uses a 16 MiB device array that each GPU holds
+ * from its first call until the process ends.  The work is enqueued on
+ * `stream` (a hipStream_t, 0 = null stream) without synchronisation, so
+ * errors inside kernels surface at the caller's next synchronisation.  Calls
+ * that overlap on one GPU share that array and race.  Returns 0, or -1 with
+ * spai_last_error() set. */
+int64_t spai_column_topk_noalloc(int64_t rows, int64_t cols, int64_t nnz, const int32_t* col_offsets_dev,
+                                 const int32_t* row_indices_dev, const double* values_dev, int64_t k,
+                                 int32_t* out_col_offsets_dev, int32_t* out_row_indices_dev, double* out_values_dev,
+                                 uint64_t stream);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* spai_last_error(void);
 

@@ -208,6 +208,26 @@ Result spai_column_topk(MatrixData X, int64_t k) {
   }
 }
 
+int64_t spai_column_topk_noalloc(int64_t rows, int64_t cols, int64_t nnz, const int32_t* col_offsets_dev,
+                                 const int32_t* row_indices_dev, const double* values_dev, int64_t k,
+                                 int32_t* out_col_offsets_dev, int32_t* out_row_indices_dev, double* out_values_dev,
+                                 uint64_t stream) {
+  try {
+    spai::CheckTopKNoAllocSizes(rows, cols, nnz, k);
+    if (!col_offsets_dev || !out_col_offsets_dev) throw std::invalid_argument("null column offsets");
+    if (nnz > 0 && (!row_indices_dev || !values_dev || !out_row_indices_dev || !out_values_dev))
+      throw std::invalid_argument("null entry arrays");
+    spai::ColumnTopKNoAlloc(reinterpret_cast<hipStream_t>(stream), rows, cols, nnz, col_offsets_dev,
+                            row_indices_dev, values_dev, k, out_col_offsets_dev, out_row_indices_dev,
+                            out_values_dev);
+    return 0;
+  } catch (const std::exception& ex) {
+    return Fail(ex.what());
+  } catch (...) {
+    return Fail("Unknown failure.");
+  }
+}
+
 int64_t spai_pcg(CsrData A, CsrData M, const double* b, int64_t max_iterations, double tolerance, double* history,
                  int64_t* count, double* x_out, int64_t m_fp32) {
   try {

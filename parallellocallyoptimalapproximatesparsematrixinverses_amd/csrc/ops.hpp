@@ -113,6 +113,19 @@ DMat ThresholdColumns(Ctx& ctx, const DMat& M, double eps);
 DMat ThresholdBatches(Ctx& ctx, const DMat& X, const double* batch_sq, const int* batch_of_col, double eps);
 // Per-column top-k: keep the diagonal and the k largest |v| off the diagonal.
 DMat ColumnTopK(Ctx& ctx, const DMat& X, int64_t k);
+// The same on caller-owned device arrays, allocating nothing: the keep flags
+// and scan tile sums live in a file-scope device array of
+// kTopKNoAllocScratchBytes (one per GPU).  Enqueued on `stream`, no
+// synchronisation; out_ptr[cols] holds the result's nnz when it completes.
+// Inputs must be CSC with int32 offsets/rows sorted per column; the outputs
+// need cols + 1 offsets and room for nnz entries.  Calls that overlap on one
+// GPU race on the shared scratch.
+constexpr int64_t kTopKNoAllocMaxNnz = int64_t(1) << 24;
+constexpr int64_t kTopKNoAllocMaxCols = int64_t(1) << 22;
+constexpr size_t kTopKNoAllocScratchBytes = size_t(kTopKNoAllocMaxNnz) + 4 * 4096 + 256;
+void CheckTopKNoAllocSizes(int64_t rows, int64_t cols, int64_t nnz, int64_t k);
+void ColumnTopKNoAlloc(hipStream_t stream, int64_t rows, int64_t cols, int64_t nnz, const int* ptr, const int* idx,
+                       const double* v, int64_t k, int* out_ptr, int* out_idx, double* out_v);
 // Global select: among candidates (drop keys in CSC order), drops the
 // `drop` smallest under (key, position); candidates are the off-diagonal
 // entries (score mode, scores m^2 c_k + 2 m (A R)_kl) or every entry (|v| mode).

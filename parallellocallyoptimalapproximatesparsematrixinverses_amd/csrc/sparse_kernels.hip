@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -2540,6 +2541,196 @@ bool IsFullDiagonal(Ctx& ctx, const DMat& X) {
   int b = 0;
   ctx.ReadBack(&b, bad.get(), sizeof(int));
   return b == 0;
+}
+
+}  // namespace spai
+
+// ------------------------------------------------- no-allocation top-k
+// Device memory of the no-allocation per-column top-k: the keep flag of every
+// input entry, then the per-tile sums of the column-count scan.  One copy per
+// GPU, held until the process ends; calls on one GPU share it.
+static __device__ __align__(256) unsigned char scratch[spai::kTopKNoAllocScratchBytes];
+
+namespace spai {
+
+namespace {
+
+constexpr int kNaTile = 1024;  // column counts per scan tile (256 threads x 4)
+constexpr int64_t kNaFlagBytes = kTopKNoAllocMaxNnz;
+static_assert(kNaFlagBytes % 256 == 0, "tile sums must stay aligned");
+static_assert(kTopKNoAllocScratchBytes >= kNaFlagBytes + sizeof(int) * (kTopKNoAllocMaxCols / kNaTile + 1),
+              "scratch too small for the tile sums");
+
+// out_ptr[j + 1] = kept entries of column j (one wavefront per column);
+// out_ptr[0] = 0.
+__global__ __launch_bounds__(kBlock) void NaKeptCounts(int ncols, const int* __restrict__ ptr,
+                                                       const unsigned char* __restrict__ keep,
+                                                       int* __restrict__ out_ptr) {
+  const int j = blockIdx.x * (kBlock / kWave) + WaveInBlock();
+  if (blockIdx.x == 0 && threadIdx.x == 0) out_ptr[0] = 0;
+  if (j >= ncols) return;
+  int c = 0;
+  for (int p = ptr[j] + LaneId(); p < ptr[j + 1]; p += kWave) c += keep[p];
+  c = WaveSumInt(c);
+  if (LaneId() == 0) out_ptr[j + 1] = c;
+}
+
+// Inclusive block scan of 4 values per thread (256 threads); returns the
+// exclusive prefix of this thread's first value and sets total.
+__device__ __forceinline__ int NaBlockScan4(int (&v)[4], int& total) {
+  __shared__ int wsum[kBlock / kWave];
+  int t = v[0] + v[1] + v[2] + v[3];
+  // inclusive scan over the wavefront
+  const int lane = LaneId(), w = WaveInBlock();
+  int x = t;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int y = __shfl_up(x, d, kWave);
+    if (lane >= d) x += y;
+  }
+  if (lane == kWave - 1) wsum[w] = x;
+  __syncthreads();
+  int before = 0;
+  total = 0;
+#pragma unroll
+  for (int q = 0; q < kBlock / kWave; ++q) {
+    before += q < w ? wsum[q] : 0;
+    total += wsum[q];
+  }
+  __syncthreads();
+  return before + x - t;
+}
+
+// tiles[t] = sum of out_ptr[1 + t * kNaTile, ...) over the tile.
+__global__ __launch_bounds__(kBlock) void NaTileSums(int ncols, const int* __restrict__ out_ptr,
+                                                     int* __restrict__ tiles) {
+  const int base = blockIdx.x * kNaTile + threadIdx.x * 4;
+  int v[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i] = base + i < ncols ? out_ptr[1 + base + i] : 0;
+  int total = 0;
+  NaBlockScan4(v, total);
+  if (threadIdx.x == 0) tiles[blockIdx.x] = total;
+}
+
+// Exclusive scan of the tile sums in place (one block, ntiles <= 4097).
+__global__ __launch_bounds__(kBlock) void NaScanTiles(int ntiles, int* __restrict__ tiles) {
+  int carry = 0;
+  for (int base = 0; base < ntiles; base += kNaTile) {
+    const int i0 = base + threadIdx.x * 4;
+    int v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = i0 + i < ntiles ? tiles[i0 + i] : 0;
+    int total = 0;
+    int run = carry + NaBlockScan4(v, total);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i0 + i < ntiles) tiles[i0 + i] = run;
+      run += v[i];
+    }
+    carry += total;
+  }
+}
+
+// out_ptr[1 + i] becomes the inclusive prefix sum of the counts.
+__global__ __launch_bounds__(kBlock) void NaApplyScan(int ncols, int* __restrict__ out_ptr,
+                                                      const int* __restrict__ tiles) {
+  const int base = blockIdx.x * kNaTile + threadIdx.x * 4;
+  int v[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i] = base + i < ncols ? out_ptr[1 + base + i] : 0;
+  int total = 0;
+  int run = tiles[blockIdx.x] + NaBlockScan4(v, total);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    run += v[i];
+    if (base + i < ncols) out_ptr[1 + base + i] = run;
+  }
+}
+
+// Kept entries of column j to out[out_ptr[j] ...) in input order (one
+// wavefront per column, 64 entries per ballot).
+__global__ __launch_bounds__(kBlock) void NaScatterKept(int ncols, const int* __restrict__ ptr,
+                                                        const int* __restrict__ idx, const double* __restrict__ v,
+                                                        const unsigned char* __restrict__ keep,
+                                                        const int* __restrict__ out_ptr, int* __restrict__ out_idx,
+                                                        double* __restrict__ out_v) {
+  const int j = blockIdx.x * (kBlock / kWave) + WaveInBlock();
+  if (j >= ncols) return;
+  const int lane = LaneId();
+  const uint64_t below = (uint64_t(1) << lane) - 1;
+  int dst = out_ptr[j];
+  for (int p0 = ptr[j]; p0 < ptr[j + 1]; p0 += kWave) {
+    const int p = p0 + lane;
+    const bool k = p < ptr[j + 1] && keep[p];
+    const uint64_t m = __ballot(k);
+    if (k) {
+      const int q = dst + __popcll(m & below);
+      out_idx[q] = idx[p];
+      out_v[q] = v[p];
+    }
+    dst += __popcll(m);
+  }
+}
+
+// Device address of `scratch` on each GPU, looked up once per device with
+// that device current (each GPU holds its own copy at its own address).
+std::mutex g_na_mu;
+void* g_na_ptr[64] = {nullptr};
+
+unsigned char* NoAllocScratch() {
+  int dev = 0;
+  SPAI_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) throw std::runtime_error("device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_na_ptr[dev] = p;
+  }
+  return static_cast<unsigned char*>(g_na_ptr[dev]);
+}
+
+}  // namespace
+
+void CheckTopKNoAllocSizes(int64_t rows, int64_t cols, int64_t nnz, int64_t k) {
+  if (rows < 0 || cols < 0 || nnz < 0) throw std::invalid_argument("rows, cols and nnz must be non-negative");
+  if (k < 0) throw std::invalid_argument("k must be non-negative");
+  if (rows > INT_MAX) throw std::invalid_argument("rows must fit a 32-bit row index");
+  if (cols > kTopKNoAllocMaxCols)
+    throw std::invalid_argument("no-allocation top-k: cols " + std::to_string(cols) + " exceeds the maximum " +
+                                std::to_string(kTopKNoAllocMaxCols));
+  if (nnz > kTopKNoAllocMaxNnz)
+    throw std::invalid_argument("no-allocation top-k: nnz " + std::to_string(nnz) + " exceeds the maximum " +
+                                std::to_string(kTopKNoAllocMaxNnz));
+}
+
+void ColumnTopKNoAlloc(hipStream_t stream, int64_t rows, int64_t cols, int64_t nnz, const int* ptr, const int* idx,
+                       const double* v, int64_t k, int* out_ptr, int* out_idx, double* out_v) {
+  CheckTopKNoAllocSizes(rows, cols, nnz, k);
+  unsigned char* const base = NoAllocScratch();
+  unsigned char* const keep = base;
+  int* const tiles = reinterpret_cast<int*>(base + kNaFlagBytes);
+  const int n = int(cols);
+  if (n == 0) {
+    // an empty matrix still gets its one column offset
+    NaKeptCounts<<<1, kBlock, 0, stream>>>(0, ptr, keep, out_ptr);
+    SPAI_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  // every flag of the input's entries is written by TopKWave, every tile sum
+  // by NaTileSums: nothing left from an earlier call is read
+  TopKWave<<<Blocks(n, kTopKWaves), kTopKWaves * kWave, 0, stream>>>(n, ptr, idx, v, k, keep);
+  const unsigned wave_blocks = Blocks(n, kBlock / kWave);
+  NaKeptCounts<<<wave_blocks, kBlock, 0, stream>>>(n, ptr, keep, out_ptr);
+  const int ntiles = (n + kNaTile - 1) / kNaTile;
+  NaTileSums<<<ntiles, kBlock, 0, stream>>>(n, out_ptr, tiles);
+  NaScanTiles<<<1, kBlock, 0, stream>>>(ntiles, tiles);
+  NaApplyScan<<<ntiles, kBlock, 0, stream>>>(n, out_ptr, tiles);
+  NaScatterKept<<<wave_blocks, kBlock, 0, stream>>>(n, ptr, idx, v, keep, out_ptr, out_idx, out_v);
+  SPAI_HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace spai

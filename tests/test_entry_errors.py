@@ -72,3 +72,21 @@ def test_session_entry_points_reject_null_session():
     assert L.spai_session_pcg_iteration_us(None) < 0
     assert _lib.last_error() == "session has no built preconditioner"
     assert L.spai_session_build_kernel_us(None, ctypes.byref(cnt)) < 0
+
+
+def test_column_topk_noalloc_rejects_sizes_beyond_its_scratch():
+    """The no-allocation top-k refuses inputs beyond its fixed device scratch
+    (16,777,216 entries, 4,194,304 columns) before touching the device."""
+    import ctypes
+    L = _lib.load()
+    f = L.spai_column_topk_noalloc
+    f.restype = ctypes.c_int64
+    f.argtypes = [ctypes.c_int64] * 3 + [ctypes.c_void_p] * 3 + [ctypes.c_int64] + [ctypes.c_void_p] * 3 + [
+        ctypes.c_uint64]
+    dummy = ctypes.c_void_p(16)  # never dereferenced: the size check comes first
+    assert f(10, 10, (1 << 24) + 1, dummy, dummy, dummy, 2, dummy, dummy, dummy, 0) == -1
+    assert "exceeds the maximum 16777216" in _lib.last_error()
+    assert f(10, (1 << 22) + 1, 10, dummy, dummy, dummy, 2, dummy, dummy, dummy, 0) == -1
+    assert "exceeds the maximum 4194304" in _lib.last_error()
+    assert f(10, 10, 10, dummy, dummy, dummy, -1, dummy, dummy, dummy, 0) == -1
+    assert "k must be non-negative" in _lib.last_error()
