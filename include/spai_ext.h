@@ -129,6 +129,14 @@ int64_t spai_column_topk_noalloc(int64_t rows, int64_t cols, int64_t nnz, const 
                                  int32_t* out_col_offsets_dev, int32_t* out_row_indices_dev, double* out_values_dev,
                                  uint64_t stream);
 
+/* Achievable HBM bandwidth of this device, for stating the sparse kernels'
+ * rates as a fraction of it: median GB/s over `repeats` launches of a
+ * streaming read (sum), copy (y = x) and triad (a = b + s c) on FP64 arrays of
+ * bytes_per_array bytes each (use sizes well beyond the caches, e.g. 2 GiB),
+ * written to gbs_out[0..2].  No counterpart in the reference.  Returns 0, or
+ * -1 with spai_last_error() set. */
+int64_t spai_stream_bandwidth(int64_t bytes_per_array, int64_t repeats, double* gbs_out);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* spai_last_error(void);
 

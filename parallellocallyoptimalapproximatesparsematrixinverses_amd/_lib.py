@@ -115,6 +115,8 @@ def load():
     L.spai_session_set_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int64]
     L.spai_session_build_kernel_us.restype = ctypes.c_double
     L.spai_session_build_kernel_us.argtypes = [ctypes.c_void_p, I64P]
+    L.spai_stream_bandwidth.restype = ctypes.c_int64
+    L.spai_stream_bandwidth.argtypes = [ctypes.c_int64, ctypes.c_int64, F64P]
     L.spai_session_info.restype = ctypes.c_int64
     L.spai_session_info.argtypes = [ctypes.c_void_p, I64P, ctypes.c_int64]
     _lib = L
@@ -123,6 +125,14 @@ def load():
 
 def last_error() -> str:
     return load().spai_last_error().decode("utf-8", "replace")
+
+
+def stream_bandwidth(bytes_per_array=2 << 30, repeats=20) -> dict:
+    """Achievable HBM GB/s of the device: streaming read, copy and triad."""
+    out = (ctypes.c_double * 3)()
+    if load().spai_stream_bandwidth(int(bytes_per_array), int(repeats), out) < 0:
+        raise RuntimeError(last_error())
+    return {"read": out[0], "copy": out[1], "triad": out[2]}
 
 
 def device_name() -> str:

@@ -373,6 +373,16 @@ int64_t spai_session_info(SpaiSession* s, int64_t* out, int64_t len) {
   }
 }
 
+int64_t spai_stream_bandwidth(int64_t bytes_per_array, int64_t repeats, double* gbs_out) {
+  try {
+    if (!gbs_out) throw std::invalid_argument("null output");
+    spai::StreamBandwidth(bytes_per_array, repeats, gbs_out);
+    return 0;
+  } catch (const std::exception& ex) {
+    return Fail(ex.what());
+  }
+}
+
 const char* spai_last_error(void) { return g_last_error.c_str(); }
 
 int64_t spai_device_name(char* buffer, int64_t size) {

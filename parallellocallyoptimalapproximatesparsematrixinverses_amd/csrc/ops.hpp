@@ -133,6 +133,9 @@ DMat DropSmallestByScore(Ctx& ctx, const DMat& M, const DMat& AR, const double* 
 DMat DropSmallestByAbs(Ctx& ctx, const DMat& X, int64_t drop);
 
 // ---------------------------------------------------------------- misc
+// Achievable streaming bandwidth (GB/s, median over `repeats` launches):
+// out[0] read, out[1] copy, out[2] triad, on arrays of bytes_per_array.
+void StreamBandwidth(int64_t bytes_per_array, int64_t repeats, double out[3]);
 void FillValue(Ctx& ctx, double* p, int64_t n, double v);
 // Inner-outer per-column step sizes (inner_outer_common.hpp:50-69): delta,
 // gamma and the normalisations 1/delta from the column dots.  two_term selects
