@@ -437,12 +437,14 @@ EngineParams ToEngineParams(const MethodParams& p, const SpaiExtParams* ext) {
 }
 
 EngineOutput RunEngine(int64_t method, const MatrixData& A, const MatrixData* M0, const MatrixData* Pr,
-                       const EngineParams& params) {
+                       const EngineParams& params, bool check_csc) {
   if (method < 0 || method > 4) throw std::invalid_argument("unknown method");
   // GetInputParams (methods_c_api.cpp:42-57): A, then M0, then Pr.
-  ValidateCsc(A);
-  if (M0) ValidateCsc(*M0);
-  if (Pr) ValidateCsc(*Pr);
+  if (check_csc) {
+    ValidateCsc(A);
+    if (M0) ValidateCsc(*M0);
+    if (Pr) ValidateCsc(*Pr);
+  }
   const bool inner_outer = method >= SPAI_INNER_OUTER_MR;
   if (inner_outer && params.inner_iterations <= 0) throw std::invalid_argument("inner_iterations must be positive");
   if (params.fp32 && !params.fixed_pattern) throw std::invalid_argument("precision = 1 requires fixed_pattern = 1");

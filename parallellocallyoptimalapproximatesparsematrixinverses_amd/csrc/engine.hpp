@@ -43,8 +43,11 @@ struct EngineOutput {
 void ValidateMatrixData(const MatrixData& d);
 
 // Runs one of the five methods (SpaiMethod) on the current HIP device.
+// check_csc: validate the CSC arrays as the C ABI does
+// (src/common/methods_c_api.cpp:16-29); the C++ interface passes false, its
+// matrices being well formed by construction.
 EngineOutput RunEngine(int64_t method, const MatrixData& A, const MatrixData* M0, const MatrixData* Pr,
-                       const EngineParams& params);
+                       const EngineParams& params, bool check_csc = true);
 
 EngineParams ToEngineParams(const MethodParams& p, const SpaiExtParams* ext);
 

@@ -5,7 +5,6 @@ against the library, and the Cython mirror imports.  No compute calls."""
 import ctypes
 import re
 import shutil
-import subprocess
 from pathlib import Path
 
 import pytest
@@ -75,14 +74,16 @@ int main(void) {
 
 
 @pytest.mark.skipif(not LIB.exists() or shutil.which("gcc") is None, reason="needs gcc and the built library")
-def test_headers_compile_as_c_and_link(tmp_path):
+def test_headers_compile_as_c_and_link(tmp_path, spawn):
     src = tmp_path / "abi.c"
     src.write_text(C_PROGRAM)
     exe = tmp_path / "abi"
-    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", str(ROOT / "include"), str(src), "-o", str(exe),
-                    "-L", str(PKG), "-lspai_hip", f"-Wl,-rpath,{PKG}"], check=True)
-    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
-    assert out == ["1", "48", "64", "32", "88", "80"]
+    rc, _, err = spawn(["gcc", "-std=c99", "-Wall", "-Werror", "-I", ROOT / "include", src, "-o", exe,
+                        "-L", PKG, "-lspai_hip", f"-Wl,-rpath,{PKG}"])
+    assert rc == 0, err
+    rc, out, err = spawn([exe])
+    assert rc == 0, err
+    assert out.split() == ["1", "48", "64", "32", "88", "80"]
 
 
 def test_cython_mirror_imports():

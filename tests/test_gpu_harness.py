@@ -68,13 +68,11 @@ def test_calculate_backward_error_many_rhs(oracle_lib):
     assert one.converged
 
 
-@pytest.mark.skip(reason="once, the pytest process itself took a segmentation fault inside subprocess.run "
-                         "(fork/exec of the child) here; the cause is not shown, so the spawn from the GPU "
-                         "test process stays off until it is")
-def test_torch_after_library_use(gpu_lib):
+def test_torch_after_library_use(gpu_lib, spawn):
     """The package loads torch before libspai_hip.so, so torch still sees the
-    device after the library has run (one HIP runtime in the process)."""
-    import subprocess
+    device after the library has run (one HIP runtime in the process).  The
+    child is started by the session's launcher process, not by this GPU test
+    process (tests/spawner.py)."""
     import sys
     code = (
         "import numpy as np\n"
@@ -86,5 +84,6 @@ def test_torch_after_library_use(gpu_lib):
         "print('ok')\n")
     from pathlib import Path
     root = Path(__file__).resolve().parent.parent
-    out = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=240)
-    assert out.returncode == 0 and out.stdout.strip().endswith("ok"), out.stderr[-2000:]
+    print("case: child process, library then torch", flush=True)
+    rc, out, err = spawn([sys.executable, "-c", code], cwd=root, timeout=240)
+    assert rc == 0 and out.strip().endswith("ok"), err[-2000:]
