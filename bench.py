@@ -424,7 +424,7 @@ def main():
                               "frac_of_achievable": round(spmv_bytes / (spmv_us * 1e-6) / 1e9 / achievable, 4),
                               "traffic": spmv_traffic, "algorithmic_bytes_per_launch": spmv_bytes,
                               "avg_launch_us": round(spmv_us, 3)},
-        "spgemm": {"kernel": "SpGemm C = A A (every column short: ProductShort + CopyShort), FP64",
+        "spgemm": {"kernel": "SpGemm C = A A (every column short: ExpansionSizes, MergeCount, scan, MergeNumeric), FP64",
                    "seconds": round(spgemm_s, 7), "nnz_C": int(C.nnz), "algorithmic_bytes": spgemm_bytes,
                    "achieved_GBs": round(spgemm_bytes / spgemm_s / 1e9, 1),
                    "frac": round(spgemm_bytes / spgemm_s / 1e9 / HBM_PEAK_GBS, 4),

@@ -16,6 +16,7 @@ struct Ctx {
   int num_cus = 256;
   // pinned host staging for small device-to-host reads
   void* pinned = nullptr;
+  void* pinned_dev = nullptr;  // its device address (mapped)
   size_t pinned_bytes = 0;
   // Device error code in mapped host memory: kernels store a nonzero code on
   // a failed consistency check (never otherwise); ReadBack and Sync raise it
@@ -38,6 +39,8 @@ struct Ctx {
   }
   // Synchronous small device-to-host copy through pinned memory.
   void ReadBack(void* host, const void* dev, size_t bytes);
+  // Two small reads (at most 256 bytes each) with one synchronisation.
+  void ReadBack2(void* h1, const void* d1, size_t b1, void* h2, const void* d2, size_t b2);
   void Sync();
 
  private:
@@ -57,11 +60,12 @@ DMat Clone(Ctx& ctx, const DMat& m);
 // Values-only copy sharing the pattern.
 DMat ShareClone(Ctx& ctx, const DMat& m);
 
-// Exclusive scan of int32 counts into ptr[0..n]; returns the total (blocking).
 // ptr[0..n] = exclusive scan of counts[0..n) (ptr[n] = the sum); returns the
 // sum read back to the host, or -1 without that read-back when read_total is
-// false (for callers that know it or do not need it).
-int64_t ScanCounts(Ctx& ctx, const int* counts, int64_t n, int* ptr, bool read_total = true);
+// false (for callers that know it, do not need it, or read it later from
+// total_dev, which then receives the sum as int64).
+int64_t ScanCounts(Ctx& ctx, const int* counts, int64_t n, int* ptr, bool read_total = true,
+                   int64_t* total_dev = nullptr);
 
 // ---------------------------------------------------------------- products
 // Conservative product C = X * Y (Eigen's column-wise Gustavson order:

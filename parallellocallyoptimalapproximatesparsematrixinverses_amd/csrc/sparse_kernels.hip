@@ -138,49 +138,24 @@ constexpr int kEscRegs = 4;  // expansion entries a lane gathers at once  // LDS
 constexpr int kNumCap = 1024;     // output rows per column held in LDS (numeric)
 constexpr int kNumWaves = 4;
 
-// Short columns (expansion of at most kShortCap rows from at most kShortGroup
-// columns of X) take kShortGroup lanes per output column, four columns per
-// wavefront, in both phases.
-constexpr int kShortGroup = 16;
-constexpr int kShortCap = 64;  // kShortGroup lanes x 4 registers
-constexpr int kShortPerBlock = 256 / kShortGroup;
+// Short columns (at most kTmLists entries in Y(:, j) and an expansion of at
+// most kTmCap rows, counting the identity's) are merged by one thread each;
+// the other columns take the wavefront-per-column tiers.
+constexpr int kTmLists = 16;
+constexpr int kTmCap = 64;
 
 __device__ __forceinline__ bool IsShortColumn(int64_t upper, int ny) {
-  return upper <= kShortCap && ny <= kShortGroup;
+  return upper <= kTmCap && ny <= kTmLists;
 }
 
-// Lane exchanges inside a 16-lane group, which is one DPP row: DPP moves and
-// swizzles read other lanes' registers directly, with none of the LDS round
-// trips of a general shuffle.  Every lane of the group must be active.
-// XorLane16(v, m): the v of lane gl ^ m (m in 1, 2, 4, 8; constant after
-// unrolling).
-__device__ __forceinline__ int XorLane16(int v, int m) {
-  switch (m) {
-    case 1: return __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
-    case 2: return __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
-    case 4: {  // banks 0 and 2 (lanes 0-3, 8-11) from row_shl 4, banks 1 and 3 from row_shr 4
-      const int a = __builtin_amdgcn_update_dpp(v, v, 0x104, 0xF, 0x5, false);
-      return __builtin_amdgcn_update_dpp(a, v, 0x114, 0xF, 0xA, false);
-    }
-    default: return __builtin_amdgcn_update_dpp(v, v, 0x128, 0xF, 0xF, false); // row_ror 8
-  }
+__device__ __forceinline__ int WaveMinInt(int v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, kWave));
+  return v;
 }
-
-// The v of lane gl - N, or `fill` on lanes gl < N (DPP row_shr N).
-template <int N>
-__device__ __forceinline__ int UpLane16(int v, int fill) {
-  return __builtin_amdgcn_update_dpp(fill, v, 0x110 + N, 0xF, 0xF, false);
-}
-
-// The v of the group's last lane (swizzle: lane (gl & 16) | 15 of 32).
-__device__ __forceinline__ int LastLane16(int v) { return __builtin_amdgcn_ds_swizzle(v, (0xF << 5) | 0x10); }
-
-// Inclusive sum over the group's lanes 0..gl.
-__device__ __forceinline__ int GroupScan16(int v) {
-  v += UpLane16<1>(v, 0);
-  v += UpLane16<2>(v, 0);
-  v += UpLane16<4>(v, 0);
-  v += UpLane16<8>(v, 0);
+__device__ __forceinline__ int WaveMaxInt(int v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
   return v;
 }
 
@@ -215,28 +190,98 @@ __device__ void BlockAppend(bool flag, int value, int* __restrict__ list, int* _
   __syncthreads();
 }
 
-// Sizes of the column lists of one product, and the summed expansion of the
-// columns in the LDS tiers.
+// Sizes of the column lists of one product, the largest Y column among the
+// short columns, and the summed expansion of the columns in the LDS tiers.
 struct TierCounts {
-  int n[4];  // short, long (LDS tiers and stamp), big (stamp), unused
+  int n[4];  // short, long (LDS tiers and stamp), big (stamp), largest short-column |Y(:, j)|
   unsigned long long mid_expansion;
 };
 
-// u_j = sum over Y(:, j) of nnz(X(:, k)) (+1 for the identity), and the
-// column's tier: short columns go to short_list, the others to long_list (the
+// k-way merge of one short column by one thread: the union, in ascending
+// row order, of the sorted row lists X(:, k) over the entries k of Y(:, j)
+// (list b is X(:, k_b), at most NL of them) and, with add_identity, of {j}.
+// Each list keeps its next unread position, its end and a queue of its next
+// D rows (and values), loaded D merge steps ahead of their use so that a step
+// does not wait on the loads of the step before it (INT_MAX past the end).
+template <int NL>
+__device__ __forceinline__ void OpenLists(int y0, int ny, const int* __restrict__ yidx, const int* __restrict__ xptr,
+                                          int (&pos)[NL], int (&end)[NL]) {
+  int k[NL];
+#pragma unroll
+  for (int b = 0; b < NL; ++b) k[b] = b < ny ? yidx[y0 + b] : -1;
+#pragma unroll
+  for (int b = 0; b < NL; ++b) {
+    pos[b] = k[b] >= 0 ? xptr[k[b]] : 0;
+    end[b] = k[b] >= 0 ? xptr[k[b] + 1] : 0;
+  }
+}
+
+template <int NL, int D, bool kValues>
+struct MergeQueues {
+  int pos[NL], end[NL];
+  int row[D][NL];
+  double val[D][NL];
+  __device__ __forceinline__ void Fill(const int* __restrict__ xidx, const double* __restrict__ xval) {
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+#pragma unroll
+      for (int b = 0; b < NL; ++b) {
+        const bool h = pos[b] + d < end[b];
+        row[d][b] = h ? xidx[pos[b] + d] : INT_MAX;
+        if (kValues) val[d][b] = h ? xval[pos[b] + d] : 0.0;
+      }
+  }
+  // Smallest head row (ident included).
+  __device__ __forceinline__ int Min(int ident) const {
+    int m = ident;
+#pragma unroll
+    for (int b = 0; b < NL; ++b) m = min(m, row[0][b]);
+    return m;
+  }
+  // Pops every head equal to m, adding its value times y[b] to acc in list
+  // order when kValues.
+  __device__ __forceinline__ void Pop(int m, const int* __restrict__ xidx, const double* __restrict__ xval,
+                                      const double (&y)[NL], double& acc) {
+#pragma unroll
+    for (int b = 0; b < NL; ++b) {
+      if (row[0][b] == m) {
+        if (kValues) acc += val[0][b] * y[b];
+        ++pos[b];
+#pragma unroll
+        for (int d = 0; d + 1 < D; ++d) {
+          row[d][b] = row[d + 1][b];
+          if (kValues) val[d][b] = val[d + 1][b];
+        }
+        const bool h = pos[b] + D - 1 < end[b];
+        row[D - 1][b] = h ? xidx[pos[b] + D - 1] : INT_MAX;
+        if (kValues) val[D - 1][b] = h ? xval[pos[b] + D - 1] : 0.0;
+      }
+    }
+  }
+};
+
+// u_j = sum over Y(:, j) of nnz(X(:, k)) (+1 for the identity), one thread
+// per column, and the column's tier: short columns are counted (and their
+// largest |Y(:, j)| kept), the others go to long_list (the
 // wavefront-per-column kernels) and, when the expansion exceeds the LDS
 // tier, also to big_list (the stamp kernel).
 constexpr int kExpBlock = 1024;
 __global__ __launch_bounds__(kExpBlock) void ExpansionSizes(int ncols, const int* __restrict__ xptr,
                                                             const int* __restrict__ yptr,
                                                             const int* __restrict__ yidx, int add_identity,
-                                                            int64_t* __restrict__ upper, int* __restrict__ short_list,
-                                                            int* __restrict__ long_list, int* __restrict__ big_list,
+                                                            int64_t* __restrict__ upper, int* __restrict__ long_list,
+                                                            int* __restrict__ big_list,
                                                             TierCounts* __restrict__ tiers) {
   __shared__ int wcount[kExpBlock / kWave];
   __shared__ int base_s;
   __shared__ unsigned long long mid_s;
-  if (threadIdx.x == 0) mid_s = 0;
+  __shared__ int short_s, ny_s;
+  if (threadIdx.x == 0) {
+    mid_s = 0;
+    short_s = 0;
+    ny_s = 0;
+  }
+  __syncthreads();
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = j < ncols;
   int64_t u = add_identity ? 1 : 0;
@@ -261,15 +306,139 @@ __global__ __launch_bounds__(kExpBlock) void ExpansionSizes(int ncols, const int
     upper[j] = u;
   }
   const bool is_short = live && IsShortColumn(u, ny);
-  BlockAppend(is_short, j, short_list, tiers->n + 0, wcount, &base_s);
+  // short-column count and largest short |Y(:, j)|: one LDS atomic per
+  // wavefront, one global atomic per block
+  const uint64_t sm = __ballot(is_short);
+  const int wmax = WaveMaxInt(is_short ? ny : 0);
+  if (LaneId() == 0 && sm) {
+    atomicAdd(&short_s, __popcll(sm));
+    atomicMax(&ny_s, wmax);
+  }
   BlockAppend(live && !is_short, j, long_list, tiers->n + 1, wcount, &base_s);
   BlockAppend(live && u > kEscCap, j, big_list, tiers->n + 2, wcount, &base_s);
   const int64_t mid = live && !is_short && u <= kEscCap ? u : 0;
   const int64_t wsum = WaveInclusiveScan64(mid);
   if (LaneId() == kWave - 1 && wsum > 0) atomicAdd(&mid_s, (unsigned long long)wsum);
   __syncthreads();
-  if (threadIdx.x == 0 && mid_s > 0) atomicAdd(&tiers->mid_expansion, mid_s);
+  if (threadIdx.x == 0) {
+    if (mid_s > 0) atomicAdd(&tiers->mid_expansion, mid_s);
+    if (short_s > 0) {
+      atomicAdd(&tiers->n[0], short_s);
+      atomicMax(&tiers->n[3], ny_s);
+    }
+  }
 }
+
+constexpr int kMergeDepth = 3;     // rows loaded ahead per list (count)
+constexpr int kMergeDepthNum = 2;  // rows and values loaded ahead per list (numeric)
+
+// Symbolic phase of the short columns, one thread per column: counts[j] =
+// the number of distinct rows of the merge (0 for the other columns).
+template <int NL>
+__global__ __launch_bounds__(256) void MergeCount(int ncols, const int* __restrict__ xptr,
+                                                  const int* __restrict__ xidx, const int* __restrict__ yptr,
+                                                  const int* __restrict__ yidx, const int64_t* __restrict__ upper,
+                                                  int add_identity, int* __restrict__ counts) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncols) return;
+  const int y0 = yptr[j], ny = yptr[j + 1] - y0;
+  if (ny > NL || !IsShortColumn(upper[j], ny)) {
+    counts[j] = 0;  // a long column: its count comes from the wavefront tiers
+    return;
+  }
+  MergeQueues<NL, kMergeDepth, false> q;
+  OpenLists<NL>(y0, ny, yidx, xptr, q.pos, q.end);
+  q.Fill(xidx, nullptr);
+  const double y[NL] = {};
+  double acc = 0.0;
+  int ident = add_identity ? j : INT_MAX;
+  int cnt = 0;
+  for (;;) {
+    const int m = q.Min(ident);
+    if (m == INT_MAX) break;
+    ++cnt;
+    if (ident == m) ident = INT_MAX;
+    q.Pop(m, xidx, nullptr, y, acc);
+  }
+  counts[j] = cnt;
+}
+
+// Numeric phase of the short columns, one thread per column: the same merge,
+// writing each distinct row and its value to C in row order.  A row's value
+// is its products x(i, k) y(k, j) added to 0 in ascending k -- the lists in
+// order -- as NumericGustavson adds them.  residual: -(sum) (+1 on the
+// diagonal, which the identity list inserts when the product lacks it).
+// NL >= the largest short |Y(:, j)|; long columns produce nothing here.
+// Outputs go through LDS, kMergeChunk merge steps at a time: each lane
+// stages its next entries, then eight lanes write one column's chunk, so a
+// store instruction covers eight contiguous runs instead of 64 scattered
+// entries.
+constexpr int kMergeChunk = 8;
+constexpr int kNumMergeWaves = 4;
+template <int NL>
+__global__ __launch_bounds__(kNumMergeWaves* kWave) void MergeNumeric(
+    int ncols, const int* __restrict__ xptr, const int* __restrict__ xidx, const double* __restrict__ xval,
+    const int* __restrict__ yptr, const int* __restrict__ yidx, const double* __restrict__ yval,
+    const int64_t* __restrict__ upper, int residual, const int* __restrict__ cptr, int* __restrict__ cidx,
+    double* __restrict__ cval) {
+  __shared__ int sidx_all[kNumMergeWaves][kWave * kMergeChunk];
+  __shared__ double sval_all[kNumMergeWaves][kWave * kMergeChunk];
+  int* const sidx = sidx_all[WaveInBlock()];
+  double* const sval = sval_all[WaveInBlock()];
+  const int lane = LaneId();
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  bool active = false;
+  int y0 = 0, ny = 0;
+  if (j < ncols) {
+    y0 = yptr[j];
+    ny = yptr[j + 1] - y0;
+    active = ny <= NL && IsShortColumn(upper[j], ny);
+  }
+  if (!__ballot(active)) return;  // whole wavefront
+  MergeQueues<NL, kMergeDepthNum, true> q;
+  double y[NL];
+  OpenLists<NL>(y0, active ? ny : 0, yidx, xptr, q.pos, q.end);
+#pragma unroll
+  for (int b = 0; b < NL; ++b) y[b] = b < ny && active ? yval[y0 + b] : 0.0;
+  q.Fill(xidx, xval);
+  int ident = residual && active ? j : INT_MAX;
+  int out = active ? cptr[j] : 0;
+  for (;;) {
+    // up to kMergeChunk merge steps into this lane's LDS slots
+    int cnt = 0;
+    for (; cnt < kMergeChunk; ++cnt) {
+      const int m = q.Min(ident);
+      if (m == INT_MAX) break;
+      double acc = 0.0;
+      q.Pop(m, xidx, xval, y, acc);
+      if (ident == m) ident = INT_MAX;
+      double v = acc;
+      if (residual) {
+        v = -v;
+        if (m == j) v += 1.0;
+      }
+      sidx[lane * kMergeChunk + cnt] = m;
+      sval[lane * kMergeChunk + cnt] = v;
+    }
+    WaveFence();
+    // eight lanes per column chunk: lane 8 r + e writes entry e of the chunk
+    // of column lane (8 r' + lane / 8) in round r'
+#pragma unroll
+    for (int r = 0; r < kWave / 8; ++r) {
+      const int src = r * 8 + lane / 8, e = lane & 7;
+      const int c = __shfl(cnt, src, kWave), o = __shfl(out, src, kWave);
+      if (e < c) {
+        cidx[o + e] = sidx[src * kMergeChunk + e];
+        cval[o + e] = sval[src * kMergeChunk + e];
+      }
+    }
+    out += cnt;
+    if (!__ballot(cnt == kMergeChunk && q.Min(ident) != INT_MAX)) break;
+    WaveFence();
+  }
+}
+
+__global__ void ZeroTiers(TierCounts* __restrict__ t) { *t = TierCounts{}; }
 
 // cnt[g] = the expansion of long-list column g when it is in the LDS tiers
 // (the stash offsets of the stashing count pass), else 0.
@@ -281,218 +450,6 @@ __global__ void MidExpansions(int nlist, const int* __restrict__ list, const int
   cnt[g] = u <= kEscCap ? int(u) : 0;
 }
 
-// A short column j is handled by a group of kShortGroup lanes.  Lane gl of
-// the group holds expansion entries e = gl + 16 r (r < 4): the row of entry e
-// in key[r] (INT_MAX past the end; j itself at e == tot with add_identity);
-// with kValues also the Y entry it comes from in src[r] (-1 when none) and
-// the product x(i, k) y(k, j) in prod[r].
-template <bool kValues>
-__device__ __forceinline__ void ShortLoad(int gl, int j, int y0, int ny, const int* __restrict__ xptr,
-                                          const int* __restrict__ xidx, const double* __restrict__ xval,
-                                          const int* __restrict__ yidx, const double* __restrict__ yval,
-                                          int add_identity, int (&key)[4], int (&src)[4], double (&prod)[4]) {
-  constexpr int G = kShortGroup;
-  // lane gl takes Y entry gl: X column start and length (and y)
-  int b = 0, c = 0;
-  double y = 0.0;
-  if (gl < ny) {
-    const int k = yidx[y0 + gl];
-    if (kValues) y = yval[y0 + gl];
-    b = xptr[k];
-    c = xptr[k + 1] - b;
-  }
-  const int incl = GroupScan16(c);
-  const int tot = LastLane16(incl);
-  const int pre = incl - c;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int e = gl + G * r;
-    // source lane: the last lane whose first entry is <= e
-    int lo = 0;
-#pragma unroll
-    for (int step = G / 2; step > 0; step >>= 1) {
-      const int mid = lo + step;
-      if (__shfl(pre, mid, G) <= e && mid < ny) lo = mid;
-    }
-    const int sb = __shfl(b, lo, G), sp = __shfl(pre, lo, G);
-    key[r] = INT_MAX;
-    if (kValues) {
-      const double sy = __shfl(y, lo, G);
-      src[r] = -1;
-      prod[r] = 0.0;
-      if (e < tot) {
-        src[r] = lo;
-        prod[r] = xval[sb + (e - sp)] * sy;
-      }
-    }
-    if (e < tot)
-      key[r] = xidx[sb + (e - sp)];
-    else if (add_identity && e == tot)
-      key[r] = j;
-  }
-}
-
-// Bitonic sort of the group's 64 keys, ascending (element i = 16 r + gl:
-// partners 16 apart or more sit in the same lane, closer ones are exchanged
-// by shuffles within the group); then first[r] flags the first occurrence of
-// each distinct row and the return value is the lane's count of them.
-__device__ __forceinline__ int ShortSortDistinct(int gl, int (&key)[4], int (&first)[4]) {
-  constexpr int G = kShortGroup;
-#pragma unroll
-  for (int kk = 2; kk <= 64; kk <<= 1) {
-#pragma unroll
-    for (int jj = kk >> 1; jj > 0; jj >>= 1) {
-      if (jj >= G) {
-        const int rj = jj / G;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          if ((r & rj) == 0) {
-            const int i = G * r + gl;
-            const bool asc = (i & kk) == 0;
-            const int a = key[r], o = key[r | rj];
-            key[r] = asc ? min(a, o) : max(a, o);
-            key[r | rj] = asc ? max(a, o) : min(a, o);
-          }
-        }
-      } else {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = G * r + gl;
-          const bool asc = (i & kk) == 0;
-          const bool lower = (gl & jj) == 0;
-          const int o = XorLane16(key[r], jj);
-          key[r] = (lower == asc) ? min(key[r], o) : max(key[r], o);
-        }
-      }
-    }
-  }
-  // element i is new when it differs from element i - 1
-  int cnt = 0;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    // lane 0 compares with the previous register's last lane
-    const int prev = UpLane16<1>(key[r], r > 0 ? LastLane16(key[r - 1]) : INT_MIN);
-    first[r] = key[r] != INT_MAX && (G * r + gl == 0 || key[r] != prev) ? 1 : 0;
-    cnt += first[r];
-  }
-  return cnt;
-}
-
-// Writes the distinct rows in order to out[0 ...]; returns their number.
-__device__ __forceinline__ int ShortWrite(int gl, const int (&key)[4], const int (&first)[4], int* out) {
-  int base = 0;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int sc = GroupScan16(first[r]);
-    if (first[r]) out[base + sc - 1] = key[r];
-    base += LastLane16(sc);
-  }
-  return base;
-}
-
-// Symbolic phase of the short columns: counts (fill == 0) or writes in row
-// order (fill == 1) the distinct rows of each column.
-__global__ __launch_bounds__(256) void SymbolicShort(int nlist, const int* __restrict__ list,
-                                                    const int* __restrict__ xptr, const int* __restrict__ xidx,
-                                                    const int* __restrict__ yptr, const int* __restrict__ yidx,
-                                                    int add_identity, int fill, int* __restrict__ counts,
-                                                    const int* __restrict__ cptr, int* __restrict__ cidx) {
-  constexpr int G = kShortGroup;
-  const int gl = threadIdx.x % G;
-  const int g = blockIdx.x * kShortPerBlock + threadIdx.x / G;
-  if (g >= nlist) return;  // whole groups leave together
-  const int j = list[g];
-  const int y0 = yptr[j], ny = yptr[j + 1] - y0;
-  int key[4], src[4], first[4];
-  double prod[4];
-  ShortLoad<false>(gl, j, y0, ny, xptr, xidx, nullptr, yidx, nullptr, add_identity, key, src, prod);
-  int cnt = ShortSortDistinct(gl, key, first);
-  if (fill) {
-    ShortWrite(gl, key, first, cidx + cptr[j]);
-    return;
-  }
-#pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, G);
-  if (gl == 0) counts[j] = cnt;
-}
-
-// Both phases of the short columns in one pass, for when the column offsets
-// are not known yet: the distinct rows go to rstash[g kShortCap ...] (g = the
-// column's list position), the values to vstash likewise, their number to
-// counts[j]; CopyShort then places them.  The values are those of
-// NumericGustavson: each output row's products added to 0 in ascending k (Y's
-// entries in order; the entries of one X column have distinct rows, so no two
-// lanes of a step share an accumulator).  residual: -(sum) (+1 on the
-// diagonal).
-__global__ __launch_bounds__(256) void ProductShort(int nlist, const int* __restrict__ list,
-                                                   const int* __restrict__ xptr, const int* __restrict__ xidx,
-                                                   const double* __restrict__ xval, const int* __restrict__ yptr,
-                                                   const int* __restrict__ yidx, const double* __restrict__ yval,
-                                                   int residual, int* __restrict__ counts, int* __restrict__ rstash,
-                                                   double* __restrict__ vstash) {
-  constexpr int G = kShortGroup;
-  __shared__ int rows_s[kShortPerBlock][kShortCap];
-  __shared__ double acc_s[kShortPerBlock][kShortCap];
-  const int gl = threadIdx.x % G, grp = threadIdx.x / G;
-  const int g = blockIdx.x * kShortPerBlock + grp;
-  if (g >= nlist) return;  // whole groups leave together
-  const int j = list[g];
-  const int y0 = yptr[j], ny = yptr[j + 1] - y0;
-  int key[4], src[4], first[4];
-  double prod[4];
-  ShortLoad<true>(gl, j, y0, ny, xptr, xidx, xval, yidx, yval, residual, key, src, prod);
-  int row[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) row[r] = key[r];
-  ShortSortDistinct(gl, key, first);
-  int* const R = rows_s[grp];
-  double* const acc = acc_s[grp];
-  const int L = ShortWrite(gl, key, first, R);
-#pragma unroll
-  for (int r = 0; r < 4; ++r) acc[gl + G * r] = 0.0;
-  __builtin_amdgcn_wave_barrier();
-  int pos[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) pos[r] = src[r] >= 0 ? LowerBound(R, 0, L, row[r]) : 0;
-  for (int s = 0; s < ny; ++s) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (src[r] == s) acc[pos[r]] += prod[r];
-    }
-    __builtin_amdgcn_wave_barrier();
-  }
-  int* const rout = rstash + int64_t(g) * kShortCap;
-  double* const vout = vstash + int64_t(g) * kShortCap;
-  for (int e = gl; e < L; e += G) {
-    double v = acc[e];
-    if (residual) {
-      v = -v;
-      if (R[e] == j) v += 1.0;
-    }
-    rout[e] = R[e];
-    vout[e] = v;
-  }
-  if (gl == 0) counts[j] = L;
-}
-
-// Places the short columns' stashed rows and values in C.
-__global__ __launch_bounds__(256) void CopyShort(int nlist, const int* __restrict__ list,
-                                                const int* __restrict__ rstash, const double* __restrict__ vstash,
-                                                const int* __restrict__ cptr, int* __restrict__ cidx,
-                                                double* __restrict__ cval) {
-  constexpr int G = kShortGroup;
-  const int gl = threadIdx.x % G;
-  const int g = blockIdx.x * kShortPerBlock + threadIdx.x / G;
-  if (g >= nlist) return;
-  const int j = list[g];
-  const int cb = cptr[j], L = cptr[j + 1] - cb;
-  const int64_t sb = int64_t(g) * kShortCap;
-  for (int e = gl; e < L; e += G) {
-    cidx[cb + e] = rstash[sb + e];
-    cval[cb + e] = vstash[sb + e];
-  }
-}
-
 // Symbolic phase, one wavefront per output column whose expansion fits the
 // LDS tier: counts (fill == 0) or writes in row order (fill == 1) the
 // distinct rows of X(:, k) over k in Y(:, j) (and j itself when add_identity);
@@ -502,16 +459,6 @@ __global__ __launch_bounds__(256) void CopyShort(int nlist, const int* __restric
 // marked in an LDS bitmap of the range and read back in order (popcounts per
 // lane, a wave scan, then the set bits); otherwise the expansion is gathered
 // into LDS, bitonic-sorted and compressed.
-__device__ __forceinline__ int WaveMinInt(int v) {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, kWave));
-  return v;
-}
-__device__ __forceinline__ int WaveMaxInt(int v) {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
-  return v;
-}
 
 __global__ __launch_bounds__(kSymWaves* kWave) void SymbolicEsc(
     int nlist, const int* __restrict__ list, const int* __restrict__ xptr, const int* __restrict__ xidx, const int* __restrict__ yptr,
@@ -864,98 +811,6 @@ __global__ __launch_bounds__(kNumWaves* kWave) void NumericGustavson(
       if (R[e] == j) v += 1.0;
     }
     if (stashed) cidx[cb + e] = R[e];
-    cval[cb + e] = v;
-  }
-}
-
-// Numeric phase of a short column (kShortGroup lanes, at most kShortCap output
-// rows): the same contributions in the same order as NumericGustavson -- each
-// output row's products x(i, k) y(k, j) added to 0 in ascending k -- so the
-// values are identical.  Expansion entry e = gl + 16 r (lane gl, register r)
-// is loaded and placed up front; the accumulation then walks Y's entries s in
-// order, adding the entries of X(:, k_s) (distinct rows, so no two lanes of a
-// step share an accumulator).
-__global__ __launch_bounds__(256) void NumericShort(
-    int nlist, const int* __restrict__ list, const int* __restrict__ xptr, const int* __restrict__ xidx,
-    const double* __restrict__ xval, const int* __restrict__ yptr, const int* __restrict__ yidx,
-    const double* __restrict__ yval, const int* __restrict__ cptr, const int* __restrict__ cidx,
-    double* __restrict__ cval, int residual, int* __restrict__ error_flag) {
-  constexpr int G = kShortGroup;
-  __shared__ int rows_s[kShortPerBlock][kShortCap];
-  __shared__ double acc_s[kShortPerBlock][kShortCap];
-  const int gl = threadIdx.x % G, grp = threadIdx.x / G;
-  const int g = blockIdx.x * kShortPerBlock + grp;
-  if (g >= nlist) return;  // whole groups leave together
-  const int j = list[g];
-  const int y0 = yptr[j], ny = yptr[j + 1] - y0;
-  const int cb = cptr[j];
-  const int L = cptr[j + 1] - cb;  // <= kShortCap: at most the expansion
-  int* const R = rows_s[grp];
-  double* const acc = acc_s[grp];
-  for (int e = gl; e < L; e += G) {
-    R[e] = cidx[cb + e];
-    acc[e] = 0.0;
-  }
-  // lane gl holds Y entry gl: X column start and length, and y
-  int b = 0, c = 0;
-  double y = 0.0;
-  if (gl < ny) {
-    const int k = yidx[y0 + gl];
-    y = yval[y0 + gl];
-    b = xptr[k];
-    c = xptr[k + 1] - b;
-  }
-  const int incl = GroupScan16(c);
-  const int tot = LastLane16(incl);
-  const int pre = incl - c;
-  int src[4], row[4];
-  double prod[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int e = gl + G * r;
-    int lo = 0;  // the last Y entry whose first expansion entry is <= e
-#pragma unroll
-    for (int step = G / 2; step > 0; step >>= 1) {
-      const int mid = lo + step;
-      if (__shfl(pre, mid, G) <= e && mid < ny) lo = mid;
-    }
-    const int sb = __shfl(b, lo, G), sp = __shfl(pre, lo, G);
-    const double sy = __shfl(y, lo, G);
-    src[r] = -1;
-    row[r] = 0;
-    prod[r] = 0.0;
-    if (e < tot) {
-      src[r] = lo;
-      row[r] = xidx[sb + (e - sp)];
-      prod[r] = xval[sb + (e - sp)] * sy;
-    }
-  }
-  __builtin_amdgcn_wave_barrier();
-  int pos[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    pos[r] = 0;
-    if (src[r] >= 0) {
-      pos[r] = LowerBound(R, 0, L, row[r]);
-      if (pos[r] >= L || R[pos[r]] != row[r]) {
-        *error_flag = 1;
-        src[r] = -1;
-      }
-    }
-  }
-  for (int s = 0; s < ny; ++s) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (src[r] == s) acc[pos[r]] += prod[r];
-    }
-    __builtin_amdgcn_wave_barrier();
-  }
-  for (int e = gl; e < L; e += G) {
-    double v = acc[e];
-    if (residual) {
-      v = -v;
-      if (R[e] == j) v += 1.0;
-    }
     cval[cb + e] = v;
   }
 }
@@ -1993,6 +1848,14 @@ Ctx::Ctx(hipStream_t external) {
   InitHostBuffers();
 }
 
+// out[0 .. na) = a, out[64 .. 64 + nb) = b (words; na, nb <= 64).
+__global__ void GatherWords(const unsigned* __restrict__ a, int na, const unsigned* __restrict__ b, int nb,
+                            unsigned* __restrict__ out) {
+  const int t = threadIdx.x;
+  if (t < na) out[t] = a[t];
+  if (t >= 64 && t - 64 < nb) out[t] = b[t - 64];
+}
+
 // One pinned allocation: the staging area for read-backs, then the error code.
 void Ctx::InitHostBuffers() {
   pinned_bytes = 1 << 16;
@@ -2000,6 +1863,7 @@ void Ctx::InitHostBuffers() {
   err_host = reinterpret_cast<int*>(static_cast<char*>(pinned) + pinned_bytes);
   *err_host = 0;
   SPAI_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&err_dev), err_host, 0));
+  SPAI_HIP_CHECK(hipHostGetDevicePointer(&pinned_dev, pinned, 0));
 }
 
 void Ctx::CheckErrorCode() {
@@ -2028,6 +1892,18 @@ void Ctx::ReadBack(void* host, const void* dev, size_t bytes) {
   CheckErrorCode();
 }
 
+void Ctx::ReadBack2(void* h1, const void* d1, size_t b1, void* h2, const void* d2, size_t b2) {
+  if (b1 > 256 || b2 > 256 || b1 % 4 || b2 % 4) throw std::invalid_argument("ReadBack2 reads whole words, at most 256 bytes a piece");
+  char* const p = static_cast<char*>(pinned);
+  // one small kernel stores both pieces into the mapped staging area
+  GatherWords<<<1, 128, 0, stream>>>(static_cast<const unsigned*>(d1), int(b1 / 4), static_cast<const unsigned*>(d2),
+                                     int(b2 / 4), reinterpret_cast<unsigned*>(pinned_dev));
+  SPAI_HIP_CHECK(hipStreamSynchronize(stream));
+  std::memcpy(h1, p, b1);
+  std::memcpy(h2, p + 256, b2);
+  CheckErrorCode();
+}
+
 void Ctx::Sync() {
   SPAI_HIP_CHECK(hipStreamSynchronize(stream));
   CheckErrorCode();
@@ -2045,10 +1921,10 @@ PatternPtr MakePattern(Ctx& ctx, int64_t rows, int64_t cols, int64_t nnz) {
   return p;
 }
 
-int64_t ScanCounts(Ctx& ctx, const int* counts, int64_t n, int* ptr, bool read_total) {
+int64_t ScanCounts(Ctx& ctx, const int* counts, int64_t n, int* ptr, bool read_total, int64_t* total_dev) {
   if (n > 0 && n <= kScanSmallMax) {
     auto total = ctx.Alloc<int64_t>(1);
-    ScanSmall<<<1, 1024, 0, ctx.stream>>>(counts, n, ptr, total.get());
+    ScanSmall<<<1, 1024, 0, ctx.stream>>>(counts, n, ptr, total_dev ? total_dev : total.get());
     if (!read_total) return -1;
     int64_t t = 0;
     ctx.ReadBack(&t, total.get(), sizeof(t));
@@ -2057,18 +1933,19 @@ int64_t ScanCounts(Ctx& ctx, const int* counts, int64_t n, int* ptr, bool read_t
   }
   const int64_t tiles = (n + kScanTile - 1) / kScanTile;
   auto sums = ctx.Alloc<int64_t>(tiles > 0 ? tiles : 1);
-  auto total = ctx.Alloc<int64_t>(1);
+  auto total_buf = ctx.Alloc<int64_t>(1);
+  int64_t* const total = total_dev ? total_dev : total_buf.get();
   if (n > 0) {
     ScanTileSums<<<Blocks(n, kScanTile), kBlock, 0, ctx.stream>>>(counts, n, sums.get());
-    ScanTiles<<<1, 1024, 0, ctx.stream>>>(sums.get(), tiles, total.get());
+    ScanTiles<<<1, 1024, 0, ctx.stream>>>(sums.get(), tiles, total);
     ScanApply<<<Blocks(n, kScanTile), kBlock, 0, ctx.stream>>>(counts, n, sums.get(), ptr);  // and ptr[n]
   } else {
-    SPAI_HIP_CHECK(hipMemsetAsync(total.get(), 0, sizeof(int64_t), ctx.stream));
-    WriteLast<<<1, 1, 0, ctx.stream>>>(ptr, n, total.get());
+    SPAI_HIP_CHECK(hipMemsetAsync(total, 0, sizeof(int64_t), ctx.stream));
+    WriteLast<<<1, 1, 0, ctx.stream>>>(ptr, n, total);
   }
   if (!read_total) return -1;
   int64_t t = 0;
-  ctx.ReadBack(&t, total.get(), sizeof(t));
+  ctx.ReadBack(&t, total, sizeof(t));
   if (t > kMaxDeviceNnz) throw std::runtime_error("result exceeds 2^31-1 stored entries on the device");
   return t;
 }
@@ -2196,19 +2073,32 @@ DMat SpGemm(Ctx& ctx, const DMat& X, const DMat& Y, bool prune_zero, bool residu
   const int64_t nrows = X.rows();
   if (residual && nrows != n) throw std::invalid_argument("residual product needs a square result");
   auto upper = ctx.Alloc<int64_t>(n > 0 ? n : 1);
-  auto lists = ctx.Alloc<int>(3 * (n > 0 ? n : 1));  // short, long, big column lists
-  int* const short_list = lists.get();
-  int* const long_list = short_list + (n > 0 ? n : 1);
+  auto counts = ctx.Alloc<int>(n > 0 ? n : 1);
+  auto lists = ctx.Alloc<int>(2 * (n > 0 ? n : 1));  // long, big column lists
+  int* const long_list = lists.get();
   int* const big_list = long_list + (n > 0 ? n : 1);
   auto tiers_d = ctx.Alloc<TierCounts>(1);
-  SPAI_HIP_CHECK(hipMemsetAsync(tiers_d.get(), 0, sizeof(TierCounts), ctx.stream));
+  ZeroTiers<<<1, 1, 0, ctx.stream>>>(tiers_d.get());
   if (n > 0)
     ExpansionSizes<<<Blocks(n, kExpBlock), kExpBlock, 0, ctx.stream>>>((int)n, X.ptr(), Y.ptr(), Y.idx(),
-                                                                       residual ? 1 : 0, upper.get(), short_list,
-                                                                       long_list, big_list, tiers_d.get());
+                                                                       residual ? 1 : 0, upper.get(), long_list,
+                                                                       big_list, tiers_d.get());
+  // symbolic phase of the short columns, then the scan, before reading
+  // anything back: when every column is short (the usual case) the tiers and
+  // the total come back together in one round trip
+  if (n > 0)
+    MergeCount<kTmLists><<<Blocks(n, 256), 256, 0, ctx.stream>>>((int)n, X.ptr(), X.idx(), Y.ptr(), Y.idx(),
+                                                                 upper.get(), residual ? 1 : 0, counts.get());
+  DMat C;
+  C.pat = MakePattern(ctx, nrows, n, 0);
+  auto total_d = ctx.Alloc<int64_t>(1);
+  ScanCounts(ctx, counts.get(), n, C.pat->ptr.get(), false, total_d.get());
   TierCounts tiers{};
-  ctx.ReadBack(&tiers, tiers_d.get(), sizeof(tiers));
-  const int nshort = tiers.n[0], nlong = tiers.n[1], nbig = tiers.n[2];
+  int64_t total = 0;
+  ctx.ReadBack2(&tiers, tiers_d.get(), sizeof(tiers), &total, total_d.get(), sizeof(total));
+  const int nshort = tiers.n[0], nlong = tiers.n[1], nbig = tiers.n[2], short_ny = tiers.n[3];
+  // the numeric merge's list count: the smallest that holds every short column
+  const int merge_lists = short_ny <= 4 ? 4 : short_ny <= 8 ? 8 : 16;
   // Many LDS-tier columns: their count pass stashes the sorted rows (at the
   // offsets of a scan of their expansions, when those stay under 1 GiB) and
   // the numeric phase places them, instead of a fill pass repeating the
@@ -2223,28 +2113,9 @@ DMat SpGemm(Ctx& ctx, const DMat& X, const DMat& Y, bool prune_zero, bool residu
     mid_stash = ctx.Alloc<int>(mid_total > 0 ? size_t(mid_total) : 1);
   }
 
-  auto counts = ctx.Alloc<int>(n > 0 ? n : 1);
-  DMat C;
-  C.pat = MakePattern(ctx, nrows, n, 0);
-  // Each tier is launched over its own column list, so a tier with no
-  // columns costs nothing.  The short columns take both phases in one pass
-  // (ProductShort) into stashes of kShortCap rows and values a column, when
-  // those stay under 1.5 GiB, and CopyShort places them after the scan;
-  // otherwise they take the two-pass kernels like the other tiers.
-  const bool fuse_short = nshort > 0 && int64_t(nshort) * kShortCap <= (int64_t(1) << 27);
-  DBuf<int> short_rows;
-  DBuf<double> short_vals;
-  if (fuse_short) {
-    short_rows = ctx.Alloc<int>(size_t(nshort) * kShortCap);
-    short_vals = ctx.Alloc<double>(size_t(nshort) * kShortCap);
-    ProductShort<<<Blocks(nshort, kShortPerBlock), 256, 0, ctx.stream>>>(
-        nshort, short_list, X.ptr(), X.idx(), X.v(), Y.ptr(), Y.idx(), Y.v(), residual ? 1 : 0, counts.get(),
-        short_rows.get(), short_vals.get());
-  }
+  // Each long tier is launched over its own column list, so a tier with no
+  // columns costs nothing.
   auto symbolic = [&](int fill, int* cnt, const int* cptr, int* cidx, DBuf<int>& stamp, unsigned stamp_blocks) {
-    if (nshort > 0 && !fuse_short)
-      SymbolicShort<<<Blocks(nshort, kShortPerBlock), 256, 0, ctx.stream>>>(
-          nshort, short_list, X.ptr(), X.idx(), Y.ptr(), Y.idx(), residual ? 1 : 0, fill, cnt, cptr, cidx);
     // with the stash, the numeric phase places the LDS-tier columns' rows
     if (nlong > nbig && !(fill && stash_mid))
       SymbolicEsc<<<Blocks(nlong, kSymWaves), kSymWaves * kWave, 0, ctx.stream>>>(
@@ -2265,22 +2136,34 @@ DMat SpGemm(Ctx& ctx, const DMat& X, const DMat& Y, bool prune_zero, bool residu
     stamp_blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>({(int64_t)nbig, max_ws, 1024}));
     stamp = ctx.Alloc<int>(size_t(stamp_blocks) * size_t(nrows));
   }
-  symbolic(0, counts.get(), nullptr, nullptr, stamp, stamp_blocks);
-  const int64_t total = ScanCounts(ctx, counts.get(), n, C.pat->ptr.get());
+  if (nlong > 0) {
+    // the long columns' counts, then the scan again
+    symbolic(0, counts.get(), nullptr, nullptr, stamp, stamp_blocks);
+    total = ScanCounts(ctx, counts.get(), n, C.pat->ptr.get());
+  } else if (total > kMaxDeviceNnz) {
+    throw std::runtime_error("result exceeds 2^31-1 stored entries on the device");
+  }
   C.pat->nnz = total;
   C.pat->idx = ctx.Alloc<int>(total > 0 ? total : 1);
   C.val = ctx.Alloc<double>(total > 0 ? total : 1);
-  symbolic(1, nullptr, C.pat->ptr.get(), C.pat->idx.get(), stamp, stamp_blocks);
-  if (fuse_short)
-    CopyShort<<<Blocks(nshort, kShortPerBlock), 256, 0, ctx.stream>>>(
-        nshort, short_list, short_rows.get(), short_vals.get(), C.pat->ptr.get(), C.pat->idx.get(), C.val.get());
-  // ProductShort takes its rows from its own sort; the other numeric kernels
-  // check theirs against the symbolic pattern and report a miss through the
-  // context's device error code (raised at its next synchronisation).
-  if (nshort > 0 && !fuse_short)
-    NumericShort<<<Blocks(nshort, kShortPerBlock), 256, 0, ctx.stream>>>(
-        nshort, short_list, X.ptr(), X.idx(), X.v(), Y.ptr(), Y.idx(), Y.v(), C.pat->ptr.get(), C.pat->idx.get(),
-        C.val.get(), residual ? 1 : 0, ctx.err_dev);
+  if (nlong > 0) symbolic(1, nullptr, C.pat->ptr.get(), C.pat->idx.get(), stamp, stamp_blocks);
+  // numeric phase: the short columns write their rows and values in place;
+  // the long kernels check their rows against the symbolic pattern and
+  // report a miss through the context's device error code (raised at its
+  // next synchronisation)
+  if (nshort > 0) {
+#define SPAI_MERGE_NUMERIC(NL)                                                                                 \
+  MergeNumeric<NL><<<Blocks(n, kNumMergeWaves * kWave), kNumMergeWaves * kWave, 0, ctx.stream>>>(                    \
+      (int)n, X.ptr(), X.idx(), X.v(), Y.ptr(), Y.idx(), Y.v(), upper.get(), residual ? 1 : 0, C.pat->ptr.get(), \
+      C.pat->idx.get(), C.val.get())
+    if (merge_lists == 4)
+      SPAI_MERGE_NUMERIC(4);
+    else if (merge_lists == 8)
+      SPAI_MERGE_NUMERIC(8);
+    else
+      SPAI_MERGE_NUMERIC(16);
+#undef SPAI_MERGE_NUMERIC
+  }
   if (nlong > 0)
     NumericGustavson<<<Blocks(nlong, kNumWaves), kNumWaves * kWave, 0, ctx.stream>>>(
         nlong, long_list, X.ptr(), X.idx(), X.v(), Y.ptr(), Y.idx(), Y.v(), C.pat->ptr.get(), C.pat->idx.get(),
