@@ -210,6 +210,9 @@ typedef struct SpaiLossPlan SpaiLossPlan;
 SpaiLossPlan* spai_loss_plan_create(struct MatrixData A, struct MatrixData M, int64_t precision, int64_t block_cols,
                                     uint64_t stream);
 void spai_loss_plan_destroy(SpaiLossPlan* plan);
+/* Later forward and backward calls run on `stream` (a hipStream_t, 0 = null
+ * stream), e.g. the caller's current stream at each call. */
+int64_t spai_loss_set_stream(SpaiLossPlan* plan, uint64_t stream);
 /* out4: nnz(pattern(A M) | I), largest block of R, number of blocks, nnz(M). */
 int64_t spai_loss_plan_info(SpaiLossPlan* plan, int64_t* out4);
 int64_t spai_loss_forward(SpaiLossPlan* plan, const void* m_values_dev, const void* a_values_dev, double* loss_out);

@@ -29,6 +29,8 @@ def _load():
                                             ctypes.c_uint64]
         L.spai_loss_plan_destroy.restype = None
         L.spai_loss_plan_destroy.argtypes = [ctypes.c_void_p]
+        L.spai_loss_set_stream.restype = ctypes.c_int64
+        L.spai_loss_set_stream.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
         L.spai_loss_plan_info.restype = ctypes.c_int64
         L.spai_loss_plan_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]
         L.spai_loss_forward.restype = ctypes.c_int64
@@ -55,7 +57,11 @@ class ResidualLoss:
             raise ValueError("dtype must be torch.float32 or torch.float64")
         L = _load()
         self.dtype = dtype
-        self.device = torch.device(device)
+        device = torch.device(device)
+        if device.type != "cuda":
+            raise ValueError("the plan runs on a HIP device")
+        # the plan's device memory lives on the device current at creation
+        self.device = torch.device("cuda", device.index if device.index is not None else torch.cuda.current_device())
         self.A = _lib.as_csc(A_pattern)
         self.M = _lib.as_csc(M_pattern, self.A.shape)
         stream = torch.cuda.current_stream(self.device).cuda_stream
@@ -70,13 +76,20 @@ class ResidualLoss:
         return {"nnz_r": out[0], "max_block_entries": out[1], "blocks": out[2], "nnz_m": out[3]}
 
     def _check(self, t, n):
-        if t.dtype != self.dtype or t.device.type != "cuda" or t.numel() != n or not t.is_contiguous():
-            raise ValueError(f"expected a contiguous {self.dtype} device tensor of {n} values")
+        if t.dtype != self.dtype or t.device != self.device or t.numel() != n or not t.is_contiguous():
+            raise ValueError(f"expected a contiguous {self.dtype} tensor of {n} values on {self.device}")
+
+    def _use_current_stream(self, L):
+        # the kernels run on the caller's current stream, ordered after the
+        # work that produced the value tensors
+        if L.spai_loss_set_stream(self._h, torch.cuda.current_stream(self.device).cuda_stream) < 0:
+            raise RuntimeError(_error(L))
 
     def forward_value(self, m_values, a_values) -> float:
         self._check(m_values, self.M.nnz)
         self._check(a_values, self.A.nnz)
         L = _load()
+        self._use_current_stream(L)
         out = ctypes.c_double(0.0)
         if L.spai_loss_forward(self._h, m_values.data_ptr(), a_values.data_ptr(), ctypes.byref(out)) < 0:
             raise RuntimeError(_error(L))
@@ -86,6 +99,7 @@ class ResidualLoss:
         self._check(m_values, self.M.nnz)
         self._check(a_values, self.A.nnz)
         L = _load()
+        self._use_current_stream(L)
         gm = torch.empty_like(m_values)
         ga = torch.empty_like(a_values)
         if L.spai_loss_backward(self._h, m_values.data_ptr(), a_values.data_ptr(), float(scale), gm.data_ptr(),

@@ -23,6 +23,7 @@
 
 #include "dev_common.hpp"
 #include "fixed.hpp"
+#include "slot_plan.hpp"
 #include "ls2x2.hpp"
 #include "ops.hpp"
 
@@ -257,62 +258,6 @@ __global__ __launch_bounds__(kBlk) void SlotProduct(int n, const int* __restrict
     }
   }
   StorePartials<Epi::K>(red, part);
-}
-
-// Plan of the slot product: slot of every (j, s, e) term, visited order.
-template <class SlotT>
-__global__ void BuildPlan(int n, const int* __restrict__ sp, const int* __restrict__ si, const int* __restrict__ ap,
-                          const int* __restrict__ ai, const int* __restrict__ s2p, const int* __restrict__ s2i,
-                          const int64_t* __restrict__ plan_ptr, SlotT* __restrict__ plan,
-                          int* __restrict__ error_flag) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  int64_t pp = plan_ptr[j];
-  const int qb = s2p[j], qe = s2p[j + 1];
-  for (int e = sp[j]; e < sp[j + 1]; ++e) {
-    const int k = si[e];
-    for (int t = ap[k]; t < ap[k + 1]; ++t) {
-      const int q = FindSorted(s2i, qb, qe, ai[t]);
-      if (q < 0) atomicOr(error_flag, 1);
-      plan[pp++] = SlotT(q < 0 ? 0 : q - qb);
-    }
-  }
-}
-
-// Number of product terms of each column (u_j) and the longest A column.
-__global__ void PlanSizes(int n, const int* __restrict__ sp, const int* __restrict__ si, const int* __restrict__ ap,
-                          const int* __restrict__ s2p, int64_t* __restrict__ terms, int* __restrict__ ext) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  int64_t u = 0;
-  for (int e = sp[j]; e < sp[j + 1]; ++e) u += ap[si[e] + 1] - ap[si[e]];
-  terms[j] = u;
-  atomicMax(&ext[0], ap[j + 1] - ap[j]);    // longest column of A
-  atomicMax(&ext[1], s2p[j + 1] - s2p[j]);  // longest column of S2
-  atomicMax(&ext[2], sp[j + 1] - sp[j]);    // longest column of S
-}
-
-__global__ void ExclusiveScan64Serial(int n, const int64_t* __restrict__ in, int64_t* __restrict__ out) {
-  // one block: chunked scan, carry kept by thread 0
-  __shared__ int64_t wsum[16];
-  __shared__ int64_t carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int b0 = 0; b0 < n; b0 += blockDim.x) {
-    const int i = b0 + threadIdx.x;
-    const int64_t v = i < n ? in[i] : 0;
-    const int64_t incl = WaveInclusiveScan64(v);
-    if (LaneId() == kWave - 1) wsum[WaveInBlock()] = incl;
-    __syncthreads();
-    int64_t wb = 0;
-    for (int w = 0; w < WaveInBlock(); ++w) wb += wsum[w];
-    const int64_t ex = carry + wb + incl - v;
-    __syncthreads();
-    if (i < n) out[i] = ex;
-    if (threadIdx.x == blockDim.x - 1) carry = ex + v;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[n] = carry;
 }
 
 // ------------------------------------------------------- fused LOMR / MR pass
@@ -1706,40 +1651,20 @@ class FixedImpl final : public FixedProblem {
     if (e) throw std::runtime_error("fixed pattern S is not contained in pattern(A S)");
     // slot plan of the products (fixed pattern)
     {
-      auto terms = c.Alloc<int64_t>(n_);
-      auto ext = c.Alloc<int>(3);
-      SPAI_HIP_CHECK(hipMemsetAsync(ext.get(), 0, 3 * sizeof(int), c.stream));
-      PlanSizes<<<Blocks1(n_), kBlk, 0, c.stream>>>((int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), A_.ptr(),
-                                                    s2_pat_->ptr.get(), terms.get(), ext.get());
-      plan_ptr_ = c.Alloc<int64_t>(n_ + 1);
-      ExclusiveScan64Serial<<<1, 1024, 0, c.stream>>>((int)n_, terms.get(), plan_ptr_.get());
-      int h[3];
-      c.ReadBack(h, ext.get(), sizeof(h));
-      int64_t total = 0;
-      c.ReadBack(&total, plan_ptr_.get() + n_, sizeof(int64_t));
+      SlotPlan plan = BuildSlotPlan(c, n_, s_pat_->ptr.get(), s_pat_->idx.get(), A_.ptr(), A_.idx(),
+                                    s2_pat_->ptr.get(), s2_pat_->idx.get());
+      const int64_t total = plan.terms;
+      const int h[3] = {plan.max_a_col, plan.max_s2_col, plan.max_s_col};
+      plan_ptr_ = std::move(plan.ptr);
       maxcol_ = std::max(h[0], 1);
       lmax_ = std::max(h[1], 1);
       gl_ = maxcol_ <= 8 ? 8 : maxcol_ <= 16 ? 16 : maxcol_ <= 32 ? 32 : 64;
       smem_ = size_t(kBlk / gl_) * 2 * size_t(lmax_) * sizeof(V);
-      slot16_ = lmax_ > 256;
-      use_plan_ = lmax_ <= 65535 && smem_ <= 64 * 1024;
+      slot16_ = plan.slot16;
+      use_plan_ = plan.built && smem_ <= 64 * 1024;
       if (use_plan_) {
-        auto err2 = c.Alloc<int>(1);
-        SPAI_HIP_CHECK(hipMemsetAsync(err2.get(), 0, sizeof(int), c.stream));
-        if (slot16_) {
-          plan16_ = c.Alloc<uint16_t>(std::max<int64_t>(total, 1));
-          BuildPlan<uint16_t><<<Blocks1(n_), kBlk, 0, c.stream>>>(
-              (int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), A_.ptr(), A_.idx(), s2_pat_->ptr.get(),
-              s2_pat_->idx.get(), plan_ptr_.get(), plan16_.get(), err2.get());
-        } else {
-          plan8_ = c.Alloc<uint8_t>(std::max<int64_t>(total, 1));
-          BuildPlan<uint8_t><<<Blocks1(n_), kBlk, 0, c.stream>>>(
-              (int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), A_.ptr(), A_.idx(), s2_pat_->ptr.get(),
-              s2_pat_->idx.get(), plan_ptr_.get(), plan8_.get(), err2.get());
-        }
-        int e2 = 0;
-        c.ReadBack(&e2, err2.get(), sizeof(int));
-        if (e2) throw std::runtime_error("product term outside pattern(A S)");
+        plan8_ = std::move(plan.plan8);
+        plan16_ = std::move(plan.plan16);
         const int64_t groups = kBlk / gl_;
         gridp_ = (unsigned)std::min<int64_t>(std::max<int64_t>((n_ + groups - 1) / groups, 1), kMaxGrid);
         // fused LOMR pass: W = 64 / GL columns per wavefront

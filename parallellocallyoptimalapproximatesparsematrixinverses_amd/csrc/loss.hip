@@ -3,17 +3,24 @@
 // with R = I - A M.  The first is the sampled product that the reference's drop
 // score uses (src/common/dropping.cpp:19-29,145: (A R)_kl for symmetric A).
 //
-// R is formed per block of consecutive columns and never stored whole: the
-// forward pass sums ||R_block||^2; the backward pass re-forms each block, writes
-// the gradient entries of M in the block's columns and adds the block's share
-// of R M^T into an FP64 accumulator per entry of A (blocks in column order, so
-// the result is deterministic).  Values are FP64 or FP32 ("FP32 bulk": FP32
-// products and storage, FP64 reductions and gradient accumulation).
+// Every product walks the terms A(i, k) M(k, j) (k over pattern(M)(:, j),
+// i over A(:, k)) with the slot plan of the fixed patterns: the position of
+// row i in S2(:, j) = pattern(A M) | I for each term, built once per plan.
+// R is formed per block of consecutive columns, column by column in LDS, and
+// never stored whole: the forward pass sums ||R||^2; the backward pass also
+// computes df/dM on the column's M entries from the LDS copy (the terms of
+// the product again, now a dot per entry of M), stores the block of R, and a
+// second kernel adds the block's share of R M^T into an FP64 accumulator per
+// entry of A (blocks in column order, so the result is deterministic).
+// Values are FP64 or FP32 ("FP32 bulk": FP32 products and storage of R, FP64
+// reductions and gradient accumulation).
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
+#include <type_traits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -21,6 +28,7 @@
 
 #include "dev_common.hpp"
 #include "ops.hpp"
+#include "slot_plan.hpp"
 #include "spai_ext.h"
 
 namespace spai {
@@ -28,118 +36,199 @@ namespace {
 
 constexpr int kBlk = 256;
 
-// Sorted-intersection dot of x[xb, xe) (rows xi) and y[yb, ye) (rows yi).
-template <class TX, class TY>
-__device__ __forceinline__ double Intersect(const int* __restrict__ xi, const TX* __restrict__ xv, int xb, int xe,
-                                            const int* __restrict__ yi, const TY* __restrict__ yv, int yb, int ye) {
-  double s = 0.0;
-  while (xb < xe && yb < ye) {
-    const int a = xi[xb], b = yi[yb];
-    if (a == b) {
-      s += double(xv[xb]) * double(yv[yb]);
-      ++xb;
-      ++yb;
-    } else if (a < b) {
-      ++xb;
-    } else {
-      ++yb;
-    }
+// Offsets of the product terms inside each column's plan: for the S entry e
+// = (k, l), poff[e] = the number of terms of column l before those of A(:, k).
+__global__ void TermOffsets(int n, const int* __restrict__ sp, const int* __restrict__ si,
+                            const int* __restrict__ ap, int* __restrict__ poff) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n) return;
+  int off = 0;
+  for (int e = sp[l]; e < sp[l + 1]; ++e) {
+    poff[e] = off;
+    const int k = si[e];
+    off += ap[k + 1] - ap[k];
   }
-  return s;
 }
 
-// R(i, j) = [i == j] - sum_k A(i, k) M(k, j) for the S2 entries q in
-// [q0, q1): A^T column i (rows k, values gathered through atperm) against
-// column j of M (rows k), k ascending.
-template <class V>
-__global__ __launch_bounds__(kBlk) void ResidualBlock(int64_t q0, int64_t q1, const int* __restrict__ s2i,
-                                                      const int* __restrict__ s2c, const int* __restrict__ atp,
-                                                      const int* __restrict__ ati, const int* __restrict__ atperm,
-                                                      const V* __restrict__ av, const int* __restrict__ sp,
-                                                      const int* __restrict__ si, const V* __restrict__ mv,
-                                                      V* __restrict__ rblk, double* __restrict__ part) {
+// Sum over the GL lanes of a group (a fixed tree, so deterministic).
+template <int GL>
+__device__ __forceinline__ double GroupSum(double v) {
+#pragma unroll
+  for (int off = GL / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, GL);
+  return v;
+}
+
+// Columns j in [j0, j1) of R = I - A M, GL lanes per column (a power of two
+// at least the longest column of A): R(:, j) is accumulated on S2(:, j) in
+// LDS by the slot plan -- for each S entry (k, m) in order, the lanes add
+// A(u, k) m at the slot of row u -- then ||R(:, j)||^2 goes to the block's
+// FP64 partial.  kBack: R(:, j) is also stored in the block buffer rblk (at
+// the column's S2 offset minus q_base), and df/dM(k, j) = -2 scale
+// sum_u A(u, k) R(u, j) for every S entry of the column: the same terms
+// walked again, each a product with R read back from LDS at its slot, summed
+// over the group's lanes in FP64.
+template <class V, class SlotT, int GL, bool kBack>
+__global__ __launch_bounds__(kBlk) void LossColumns(int j0, int j1, const int* __restrict__ sp,
+                                                    const int* __restrict__ si, const V* __restrict__ mv,
+                                                    const int* __restrict__ ap, const V* __restrict__ av,
+                                                    const int64_t* __restrict__ plan_ptr,
+                                                    const SlotT* __restrict__ plan, const int* __restrict__ s2p,
+                                                    const int* __restrict__ s2i, int64_t q_base, int lmax,
+                                                    V* __restrict__ rblk, double scale, V* __restrict__ gm,
+                                                    double* __restrict__ part) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ double red_s[16];
+  const int groups = blockDim.x / GL;
+  const int g = threadIdx.x / GL, gl = threadIdx.x % GL;
+  V* const acc = reinterpret_cast<V*>(lds_raw) + size_t(g) * lmax;
+  double red = 0.0;
+  for (int j = j0 + blockIdx.x * groups + g; j < j1; j += gridDim.x * groups) {
+    const int q0 = s2p[j], L = s2p[j + 1] - q0;
+    for (int t = gl; t < L; t += GL) acc[t] = V(0);
+    WaveFence();
+    const int e0 = sp[j], e1 = sp[j + 1];
+    int64_t pp = plan_ptr[j];
+    for (int eb = e0; eb < e1; eb += GL) {
+      int my_a0 = 0, my_c = 0;
+      V my_m = V(0);
+      if (eb + gl < e1) {
+        const int k = si[eb + gl];
+        my_a0 = ap[k];
+        my_c = ap[k + 1] - my_a0;
+        my_m = mv[eb + gl];
+      }
+      const int ns = min(GL, e1 - eb);
+      for (int st = 0; st < ns; ++st) {
+        const int a0 = __shfl(my_a0, st, GL), c = __shfl(my_c, st, GL);
+        const V m = __shfl(my_m, st, GL);
+        // distinct rows within one column of A: no two lanes share a slot
+        for (int u = gl; u < c; u += GL) acc[plan[pp + u]] += av[a0 + u] * m;
+        pp += c;
+      }
+    }
+    WaveFence();
+    for (int t = gl; t < L; t += GL) {
+      V r = -acc[t];
+      if (s2i[q0 + t] == j) r += V(1);
+      red += double(r) * double(r);
+      if (kBack) {
+        acc[t] = r;
+        rblk[int64_t(q0) - q_base + t] = r;
+      }
+    }
+    if (kBack) {
+      WaveFence();
+      pp = plan_ptr[j];
+      for (int eb = e0; eb < e1; eb += GL) {
+        int my_a0 = 0, my_c = 0;
+        if (eb + gl < e1) {
+          const int k = si[eb + gl];
+          my_a0 = ap[k];
+          my_c = ap[k + 1] - my_a0;
+        }
+        const int ns = min(GL, e1 - eb);
+        double mine = 0.0;  // lane st keeps the dot of step st
+        for (int st = 0; st < ns; ++st) {
+          const int a0 = __shfl(my_a0, st, GL), c = __shfl(my_c, st, GL);
+          double s = 0.0;
+          for (int u = gl; u < c; u += GL) s += double(av[a0 + u]) * double(acc[plan[pp + u]]);
+          s = GroupSum<GL>(s);
+          if (gl == st) mine = s;
+          pp += c;
+        }
+        if (gl < ns) gm[eb + gl] = V(-2.0 * scale * mine);
+      }
+    }
+    WaveFence();
+  }
+  red = BlockSum<double>(red, red_s);
+  if (threadIdx.x == 0) part[blockIdx.x] = red;
+}
+
+// df/dA for the A columns k in [k0, k1), contributions of the R block's
+// columns [j0, j1): acc(u, k) += sum_l M(k, l) R(u, l) over the entries l of
+// row k of S (column k of S^T) inside the block, l ascending.  R(u, l) is
+// the block buffer at the slot the plan gives the term (l, k, u), found
+// through the term offset of the S entry (k, l).  GL lanes per column of A,
+// lane gl holds rows gl + GL r (r < kRegs); FP64 products and sums; blocks
+// add in column order, so the result is deterministic.
+constexpr int kGradARegs = 4;
+template <class V, class SlotT, int GL>
+__global__ __launch_bounds__(kBlk) void LossGradA(int k0, int k1, int j0, int j1, const int* __restrict__ ap,
+                                                  const int* __restrict__ stp, const int* __restrict__ sti,
+                                                  const int* __restrict__ stperm, const V* __restrict__ mv,
+                                                  const int64_t* __restrict__ plan_ptr,
+                                                  const SlotT* __restrict__ plan, const int* __restrict__ poff,
+                                                  const int* __restrict__ s2p, int64_t q_base,
+                                                  const V* __restrict__ rblk, double* __restrict__ acc_a) {
+  const int groups = blockDim.x / GL;
+  const int g = threadIdx.x / GL, gl = threadIdx.x % GL;
+  for (int k = k0 + blockIdx.x * groups + g; k < k1; k += gridDim.x * groups) {
+    const int a0 = ap[k], c = ap[k + 1] - a0;
+    const int x = LowerBound(sti, stp[k], stp[k + 1], j0);
+    const int xe = LowerBound(sti, x, stp[k + 1], j1);
+    double acc[kGradARegs];
+#pragma unroll
+    for (int r = 0; r < kGradARegs; ++r) acc[r] = 0.0;
+    for (int tb = x; tb < xe; tb += GL) {
+      V my_m = V(0);
+      int64_t my_off = 0, my_rb = 0;
+      if (tb + gl < xe) {
+        const int p = stperm[tb + gl], l = sti[tb + gl];
+        my_m = mv[p];
+        my_off = plan_ptr[l] + poff[p];
+        my_rb = int64_t(s2p[l]) - q_base;
+      }
+      const int ns = min(GL, xe - tb);
+      for (int st = 0; st < ns; ++st) {
+        const double m = double(__shfl(my_m, st, GL));
+        const int64_t off = __shfl(my_off, st, GL), rb = __shfl(my_rb, st, GL);
+#pragma unroll
+        for (int r = 0; r < kGradARegs; ++r) {
+          const int u = gl + GL * r;
+          if (u < c) acc[r] += m * double(rblk[rb + int64_t(plan[off + u])]);
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kGradARegs; ++r) {
+      const int u = gl + GL * r;
+      if (u < c) acc_a[a0 + u] += acc[r];
+    }
+  }
+}
+
+// Row range of S over columns [first[b], first[b + 1]) of each block b:
+// the columns k of A that a block's R reaches in LossGradA.
+__global__ void BlockRowRanges(int n, const int* __restrict__ sp, const int* __restrict__ si, int block_cols,
+                               int* __restrict__ kmin, int* __restrict__ kmax) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int b = j / block_cols;
+  int lo = INT_MAX, hi = -1;
+  for (int e = sp[j]; e < sp[j + 1]; ++e) {
+    lo = min(lo, si[e]);
+    hi = max(hi, si[e]);
+  }
+  if (hi >= 0) {
+    atomicMin(&kmin[b], lo);
+    atomicMax(&kmax[b], hi);
+  }
+}
+
+// Deterministic sum of the per-block partials (one block).
+__global__ void SumLoss(const double* __restrict__ part, int np, double* __restrict__ out) {
   __shared__ double red[16];
-  double acc = 0.0;
-  for (int64_t q = q0 + int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < q1; q += int64_t(gridDim.x) * blockDim.x) {
-    const int i = s2i[q], j = s2c[q];
-    V s = V(0);
-    int a = atp[i], ae = atp[i + 1], b = sp[j], be = sp[j + 1];
-    while (a < ae && b < be) {
-      const int ka = ati[a], kb = si[b];
-      if (ka == kb) {
-        s += av[atperm[a]] * mv[b];
-        ++a;
-        ++b;
-      } else if (ka < kb) {
-        ++a;
-      } else {
-        ++b;
-      }
-    }
-    V r = -s;
-    if (i == j) r += V(1);
-    rblk[q - q0] = r;
-    acc += double(r) * double(r);
-  }
-  acc = BlockSum<double>(acc, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = acc;
-}
-
-// dM(k, l) = -2 scale * sum_i A(i, k) R(i, l) for the S entries p in [p0, p1).
-template <class V>
-__global__ __launch_bounds__(kBlk) void GradMBlock(int64_t p0, int64_t p1, int64_t q0, const int* __restrict__ si,
-                                                   const int* __restrict__ scol, const int* __restrict__ ap,
-                                                   const int* __restrict__ ai, const V* __restrict__ av,
-                                                   const int* __restrict__ s2p, const int* __restrict__ s2i,
-                                                   const V* __restrict__ rblk, double scale, V* __restrict__ gm) {
-  for (int64_t p = p0 + int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < p1; p += int64_t(gridDim.x) * blockDim.x) {
-    const int k = si[p], l = scol[p];
-    const int rb = s2p[l], re = s2p[l + 1];
-    const double s = Intersect(ai, av, ap[k], ap[k + 1], s2i, rblk - q0, rb, re);
-    gm[p] = V(-2.0 * scale * s);
-  }
-}
-
-// acc(i, k) += sum_{j in [j0, j1)} R(i, j) M(k, j) for every entry (i, k) of A:
-// row i of R (column i of S2^T, columns j) against row k of M (column k of S^T).
-template <class V>
-__global__ __launch_bounds__(kBlk) void GradABlock(int64_t nnz_a, int j0, int j1, int64_t q0, const int* __restrict__ ai,
-                                                   const int* __restrict__ acol, const int* __restrict__ s2tp,
-                                                   const int* __restrict__ s2ti, const int* __restrict__ s2tperm,
-                                                   const V* __restrict__ rblk, const int* __restrict__ stp,
-                                                   const int* __restrict__ sti, const int* __restrict__ stperm,
-                                                   const V* __restrict__ mv, double* __restrict__ acc) {
-  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < nnz_a; e += int64_t(gridDim.x) * blockDim.x) {
-    const int i = ai[e], k = acol[e];
-    int x = LowerBound(s2ti, s2tp[i], s2tp[i + 1], j0), xe = LowerBound(s2ti, x, s2tp[i + 1], j1);
-    int y = LowerBound(sti, stp[k], stp[k + 1], j0), ye = LowerBound(sti, y, stp[k + 1], j1);
-    double s = 0.0;
-    while (x < xe && y < ye) {
-      const int a = s2ti[x], b = sti[y];
-      if (a == b) {
-        s += double(rblk[s2tperm[x] - q0]) * double(mv[stperm[y]]);
-        ++x;
-        ++y;
-      } else if (a < b) {
-        ++x;
-      } else {
-        ++y;
-      }
-    }
-    acc[e] += s;
-  }
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[i];
+  s = BlockSum<double>(s, red);
+  if (threadIdx.x == 0) *out = s;
 }
 
 template <class V>
 __global__ void ScaleOut(int64_t n, const double* __restrict__ acc, double scale, V* __restrict__ out) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) out[i] = V(-2.0 * scale * acc[i]);
-}
-
-__global__ void ColOf(int n, const int* __restrict__ ptr, int* __restrict__ col) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  for (int p = ptr[j]; p < ptr[j + 1]; ++p) col[p] = j;
 }
 
 __global__ void Iota(int64_t n, double* __restrict__ v) {
@@ -150,12 +239,6 @@ __global__ void Iota(int64_t n, double* __restrict__ v) {
 __global__ void ToInt(int64_t n, const double* __restrict__ v, int* __restrict__ out) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) out[i] = int(v[i]);
-}
-
-// Grid of the grid-stride kernels (capped; partial sums fit `part`).
-unsigned GridOf(int64_t n) {
-  int64_t g = (n + kBlk - 1) / kBlk;
-  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 4096));
 }
 
 // Grid of the one-thread-per-element kernels: every element needs a thread.
@@ -173,12 +256,19 @@ struct SpaiLossPlan {
   int64_t n = 0, block_cols = 0;
   bool fp32 = false;
   spai::DMat A, S, S2;                 // patterns (values unused)
-  spai::PatternPtr AT, S2T, ST;        // transposed patterns
-  spai::DBuf<int> atperm, s2tperm, stperm, s2col, scol, acol;
+  spai::SlotPlan plan;                 // slots of the terms of A S on S2
+  spai::DBuf<int> poff;                // per S entry: its first term inside its column's plan
+  spai::PatternPtr ST;                 // S^T
+  spai::DBuf<int> stperm;              // S^T entry -> S entry
   spai::DBuf<char> rblk;               // one column block of R
-  spai::DBuf<double> part, acc;
+  spai::DBuf<double> part, acc, loss;
   int64_t max_block_entries = 0;
+  int gl = 8, lmax = 1, block_threads = spai::kBlk;
   std::vector<int> blocks;             // column boundaries
+  std::vector<int64_t> q_base;         // first S2 entry of each block
+  std::vector<int> kmin, kmax;         // rows of S in each block's columns
+  std::vector<unsigned> grid;          // column-kernel blocks of each column block
+  std::vector<int> part_off;           // their partials' offsets
 };
 
 namespace {
@@ -202,48 +292,63 @@ spai::PatternPtr TransposeWithPerm(spai::Ctx& c, const spai::DMat& X, spai::DBuf
   return T.pat;
 }
 
+template <class V, class SlotT, int GL>
+void LaunchBlock(SpaiLossPlan* p, size_t b, const V* m, const V* a, bool backward, double scale, V* gm) {
+  using namespace spai;
+  Ctx& c = *p->ctx;
+  const int j0 = p->blocks[b], j1 = p->blocks[b + 1];
+  const SlotT* plan = std::is_same<SlotT, uint8_t>::value ? reinterpret_cast<const SlotT*>(p->plan.plan8.get())
+                                                          : reinterpret_cast<const SlotT*>(p->plan.plan16.get());
+  V* rblk = reinterpret_cast<V*>(p->rblk.get());
+  const size_t lds = size_t(p->block_threads / GL) * size_t(p->lmax) * sizeof(V);
+  double* part = p->part.get() + p->part_off[b];
+  if (backward)
+    LossColumns<V, SlotT, GL, true><<<p->grid[b], p->block_threads, lds, c.stream>>>(
+        j0, j1, p->S.ptr(), p->S.idx(), m, p->A.ptr(), a, p->plan.ptr.get(), plan, p->S2.ptr(), p->S2.idx(),
+        p->q_base[b], p->lmax, rblk, scale, gm, part);
+  else
+    LossColumns<V, SlotT, GL, false><<<p->grid[b], p->block_threads, lds, c.stream>>>(
+        j0, j1, p->S.ptr(), p->S.idx(), m, p->A.ptr(), a, p->plan.ptr.get(), plan, p->S2.ptr(), p->S2.idx(),
+        p->q_base[b], p->lmax, rblk, scale, gm, part);
+  if (backward && p->kmax[b] >= p->kmin[b]) {
+    const int k0 = p->kmin[b], k1 = p->kmax[b] + 1;
+    const int groups = p->block_threads / GL;
+    const unsigned ga = (unsigned)std::max<int64_t>(1, std::min<int64_t>((k1 - k0 + groups - 1) / groups, 8192));
+    LossGradA<V, SlotT, GL><<<ga, p->block_threads, 0, c.stream>>>(
+        k0, k1, j0, j1, p->A.ptr(), p->ST->ptr.get(), p->ST->idx.get(), p->stperm.get(), m, p->plan.ptr.get(), plan,
+        p->poff.get(), p->S2.ptr(), p->q_base[b], rblk, p->acc.get());
+  }
+}
+
+template <class V, class SlotT>
+void LaunchBlockGl(SpaiLossPlan* p, size_t b, const V* m, const V* a, bool backward, double scale, V* gm) {
+  switch (p->gl) {
+    case 8: LaunchBlock<V, SlotT, 8>(p, b, m, a, backward, scale, gm); break;
+    case 16: LaunchBlock<V, SlotT, 16>(p, b, m, a, backward, scale, gm); break;
+    case 32: LaunchBlock<V, SlotT, 32>(p, b, m, a, backward, scale, gm); break;
+    default: LaunchBlock<V, SlotT, 64>(p, b, m, a, backward, scale, gm); break;
+  }
+}
+
+// Forward (loss to *loss) or forward and backward (gradients to gm, ga),
+// blocks of R in column order on the plan's stream.
 template <class V>
 int64_t Forward(SpaiLossPlan* p, const V* m, const V* a, double* loss, bool backward, double scale, V* gm, V* ga) {
   using namespace spai;
   Ctx& c = *p->ctx;
   hipStream_t st = c.stream;
-  V* rblk = reinterpret_cast<V*>(p->rblk.get());
-  const int* s2p = p->S2.ptr();
   if (backward) SPAI_HIP_CHECK(hipMemsetAsync(p->acc.get(), 0, sizeof(double) * p->A.nnz(), st));
-  double total = 0.0;
   for (size_t b = 0; b + 1 < p->blocks.size(); ++b) {
-    const int j0 = p->blocks[b], j1 = p->blocks[b + 1];
-    int qq[2];
-    SPAI_HIP_CHECK(hipMemcpyAsync(qq, s2p + j0, sizeof(int), hipMemcpyDeviceToHost, st));
-    SPAI_HIP_CHECK(hipMemcpyAsync(qq + 1, s2p + j1, sizeof(int), hipMemcpyDeviceToHost, st));
-    SPAI_HIP_CHECK(hipStreamSynchronize(st));
-    const int64_t q0 = qq[0], q1 = qq[1];
-    const unsigned g = GridOf(q1 - q0);
-    ResidualBlock<V><<<g, kBlk, 0, st>>>(q0, q1, p->S2.idx(), p->s2col.get(), p->AT->ptr.get(), p->AT->idx.get(),
-                                         p->atperm.get(), a, p->S.ptr(), p->S.idx(), m, rblk, p->part.get());
-    std::vector<double> h(g);
-    SPAI_HIP_CHECK(hipMemcpyAsync(h.data(), p->part.get(), sizeof(double) * g, hipMemcpyDeviceToHost, st));
-    SPAI_HIP_CHECK(hipStreamSynchronize(st));
-    double bs = 0.0;
-    for (double v : h) bs += v;
-    total += bs;
-    if (backward) {
-      int pp[2];
-      SPAI_HIP_CHECK(hipMemcpyAsync(pp, p->S.ptr() + j0, sizeof(int), hipMemcpyDeviceToHost, st));
-      SPAI_HIP_CHECK(hipMemcpyAsync(pp + 1, p->S.ptr() + j1, sizeof(int), hipMemcpyDeviceToHost, st));
-      SPAI_HIP_CHECK(hipStreamSynchronize(st));
-      GradMBlock<V><<<GridOf(pp[1] - pp[0]), kBlk, 0, st>>>(pp[0], pp[1], q0, p->S.idx(), p->scol.get(), p->A.ptr(),
-                                                            p->A.idx(), a, s2p, p->S2.idx(), rblk, scale, gm);
-      GradABlock<V><<<GridOf(p->A.nnz()), kBlk, 0, st>>>(p->A.nnz(), j0, j1, q0, p->A.idx(), p->acol.get(),
-                                                         p->S2T->ptr.get(), p->S2T->idx.get(), p->s2tperm.get(), rblk,
-                                                         p->ST->ptr.get(), p->ST->idx.get(), p->stperm.get(), m,
-                                                         p->acc.get());
-    }
+    if (p->plan.slot16)
+      LaunchBlockGl<V, uint16_t>(p, b, m, a, backward, scale, gm);
+    else
+      LaunchBlockGl<V, uint8_t>(p, b, m, a, backward, scale, gm);
   }
-  if (backward)
-    ScaleOut<V><<<GridAll(p->A.nnz()), kBlk, 0, st>>>(p->A.nnz(), p->acc.get(), scale, ga);
+  if (backward) ScaleOut<V><<<GridAll(p->A.nnz()), kBlk, 0, st>>>(p->A.nnz(), p->acc.get(), scale, ga);
+  SumLoss<<<1, kBlk, 0, st>>>(p->part.get(), p->part_off.back(), p->loss.get());
   SPAI_HIP_CHECK(hipGetLastError());
-  SPAI_HIP_CHECK(hipStreamSynchronize(st));
+  double total = 0.0;
+  c.ReadBack(&total, p->loss.get(), sizeof(double));
   if (loss) *loss = total;
   return 0;
 }
@@ -268,25 +373,57 @@ SpaiLossPlan* spai_loss_plan_create(MatrixData A, MatrixData M, int64_t precisio
     p->S = Upload(c, M.rows, M.cols, M.col_offsets, M.row_indices, M.values);
     DMat S2 = SpGemm(c, p->A, p->S, false, true);
     p->S2 = std::move(S2);
-    p->AT = TransposeWithPerm(c, p->A, p->atperm);
-    p->S2T = TransposeWithPerm(c, p->S2, p->s2tperm);
+    p->plan = BuildSlotPlan(c, p->n, p->S.ptr(), p->S.idx(), p->A.ptr(), p->A.idx(), p->S2.ptr(), p->S2.idx());
+    if (!p->plan.built) throw std::invalid_argument("the differentiation form needs columns of A M of at most 65535 entries");
+    if (p->plan.max_a_col > 64 * kGradARegs)
+      throw std::invalid_argument("the differentiation form needs columns of A of at most 256 entries");
+    p->gl = p->plan.max_a_col <= 8 ? 8 : p->plan.max_a_col <= 16 ? 16 : p->plan.max_a_col <= 32 ? 32 : 64;
+    p->lmax = std::max(p->plan.max_s2_col, 1);
+    // R columns in LDS: fewer lanes per block when 256 would not fit
+    const size_t vsize = p->fp32 ? 4 : 8;
+    p->block_threads = kBlk;
+    while (p->block_threads > p->gl && size_t(p->block_threads / p->gl) * size_t(p->lmax) * vsize > 48 * 1024)
+      p->block_threads /= 2;
+    if (size_t(p->block_threads / p->gl) * size_t(p->lmax) * vsize > 64 * 1024)
+      throw std::invalid_argument("the differentiation form needs columns of A M within 64 KiB of LDS");
+    p->poff = c.Alloc<int>(std::max<int64_t>(p->S.nnz(), 1));
+    TermOffsets<<<GridAll(p->n), kBlk, 0, c.stream>>>((int)p->n, p->S.ptr(), p->S.idx(), p->A.ptr(), p->poff.get());
     p->ST = TransposeWithPerm(c, p->S, p->stperm);
-    p->s2col = c.Alloc<int>(std::max<int64_t>(p->S2.nnz(), 1));
-    p->scol = c.Alloc<int>(std::max<int64_t>(p->S.nnz(), 1));
-    p->acol = c.Alloc<int>(std::max<int64_t>(p->A.nnz(), 1));
-    ColOf<<<GridAll(p->n), kBlk, 0, c.stream>>>((int)p->n, p->S2.ptr(), p->s2col.get());
-    ColOf<<<GridAll(p->n), kBlk, 0, c.stream>>>((int)p->n, p->S.ptr(), p->scol.get());
-    ColOf<<<GridAll(p->n), kBlk, 0, c.stream>>>((int)p->n, p->A.ptr(), p->acol.get());
-    std::vector<int> s2p(p->n + 1);
-    SPAI_HIP_CHECK(hipMemcpyAsync(s2p.data(), p->S2.ptr(), sizeof(int) * (p->n + 1), hipMemcpyDeviceToHost, c.stream));
-    c.Sync();
     for (int64_t j = 0; j < p->n; j += p->block_cols) p->blocks.push_back((int)j);
     p->blocks.push_back((int)p->n);
-    for (size_t b = 0; b + 1 < p->blocks.size(); ++b)
-      p->max_block_entries =
-          std::max<int64_t>(p->max_block_entries, s2p[p->blocks[b + 1]] - s2p[p->blocks[b]]);
-    p->rblk = c.Alloc<char>(std::max<int64_t>(p->max_block_entries, 1) * (p->fp32 ? 4 : 8));
-    p->part = c.Alloc<double>(4096);
+    const size_t nb = p->blocks.size() - 1;
+    // per block: S2 offsets, the A columns its R reaches, the column grid
+    std::vector<int> s2p(p->n + 1);
+    SPAI_HIP_CHECK(hipMemcpyAsync(s2p.data(), p->S2.ptr(), sizeof(int) * (p->n + 1), hipMemcpyDeviceToHost, c.stream));
+    auto kr = c.Alloc<int>(2 * nb);
+    {
+      std::vector<int> init(2 * nb);
+      for (size_t b = 0; b < nb; ++b) {
+        init[b] = INT_MAX;
+        init[nb + b] = -1;
+      }
+      SPAI_HIP_CHECK(hipMemcpyAsync(kr.get(), init.data(), sizeof(int) * 2 * nb, hipMemcpyHostToDevice, c.stream));
+      BlockRowRanges<<<GridAll(p->n), kBlk, 0, c.stream>>>((int)p->n, p->S.ptr(), p->S.idx(), (int)p->block_cols,
+                                                           kr.get(), kr.get() + nb);
+      std::vector<int> h(2 * nb);
+      c.ReadBack(h.data(), kr.get(), sizeof(int) * 2 * nb);
+      p->kmin.assign(h.begin(), h.begin() + nb);
+      p->kmax.assign(h.begin() + nb, h.end());
+    }
+    const int groups = p->block_threads / p->gl;
+    int total_parts = 0;
+    for (size_t b = 0; b < nb; ++b) {
+      p->q_base.push_back(s2p[p->blocks[b]]);
+      p->max_block_entries = std::max<int64_t>(p->max_block_entries, s2p[p->blocks[b + 1]] - s2p[p->blocks[b]]);
+      const int64_t cols = p->blocks[b + 1] - p->blocks[b];
+      p->grid.push_back((unsigned)std::max<int64_t>(1, std::min<int64_t>((cols + groups - 1) / groups, 2048)));
+      p->part_off.push_back(total_parts);
+      total_parts += (int)p->grid.back();
+    }
+    p->part_off.push_back(total_parts);
+    p->rblk = c.Alloc<char>(std::max<int64_t>(p->max_block_entries, 1) * vsize);
+    p->part = c.Alloc<double>(std::max(total_parts, 1));
+    p->loss = c.Alloc<double>(1);
     p->acc = c.Alloc<double>(std::max<int64_t>(p->A.nnz(), 1));
     c.Sync();
     return p.release();
@@ -294,6 +431,12 @@ SpaiLossPlan* spai_loss_plan_create(MatrixData A, MatrixData M, int64_t precisio
     g_loss_error = ex.what();
     return nullptr;
   }
+}
+
+int64_t spai_loss_set_stream(SpaiLossPlan* p, uint64_t stream) {
+  if (!p) return LossFail("null plan");
+  p->ctx->stream = reinterpret_cast<hipStream_t>(stream);
+  return 0;
 }
 
 void spai_loss_plan_destroy(SpaiLossPlan* p) { delete p; }

@@ -1871,6 +1871,7 @@ void Ctx::CheckErrorCode() {
   if (e == 0) return;
   *err_host = 0;
   if (e == 1) throw std::runtime_error("SpGEMM numeric phase found a row outside the symbolic pattern");
+  if (e == 2) throw std::runtime_error("product term outside pattern(A S)");
   throw std::runtime_error("device consistency check failed (code " + std::to_string(e) + ")");
 }
 

@@ -135,3 +135,25 @@ def test_fp32_against_fp64():
     g32 = p32.gradients(dev(M.data, torch.float32), dev(A.data, torch.float32))
     for a, b in zip(g32, g64):
         assert torch.linalg.norm(a.double() - b) / torch.linalg.norm(b) < 1e-5
+
+
+def test_values_written_on_another_stream(small):
+    """The kernels run on the caller's current stream: value tensors produced on
+    a side stream and used under it give the default stream's results."""
+    from parallellocallyoptimalapproximatesparsematrixinverses_amd.autograd import ResidualLoss
+    A, M = small
+    plan = ResidualLoss(A, M, torch.float64)
+    f_ref = plan.forward_value(dev(M.data), dev(A.data))
+    gm_ref, ga_ref = plan.gradients(dev(M.data), dev(A.data))
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        m = torch.zeros(M.nnz, dtype=torch.float64, device="cuda")
+        a = torch.zeros(A.nnz, dtype=torch.float64, device="cuda")
+        m.add_(dev(M.data))
+        a.add_(dev(A.data))
+        f = plan.forward_value(m, a)
+        gm, ga = plan.gradients(m, a)
+    side.synchronize()
+    assert f == f_ref
+    torch.testing.assert_close(gm, gm_ref, rtol=0, atol=0)
+    torch.testing.assert_close(ga, ga_ref, rtol=0, atol=0)
