@@ -96,14 +96,16 @@ def time_form(lib, A, M0, b, precision, steps, warmup):
 def cold_step(lib, A, M0, b, precision):
     """One step from nothing on the device, through the C ABI: create the
     session (uploads, symbolic phase, plan), upload b, build, solve, and
-    download M and x.  Returns (seconds, pcg iterations)."""
+    download M (C-ABI Result arrays) and x.  A and M0 are canonical CSC with
+    int64 indices, which the calls read in place, as a C caller hands over its
+    arrays.  Returns (seconds, pcg iterations)."""
     t = time.perf_counter()
     s = lib.Session("lomr", A, M0=M0, max_iterations=ITERATIONS, precision=precision)
     s.set_rhs(b)
     s.build()
     st, cnt = s.pcg(PCG_LIMIT, TOL)
     hist, x = s.pcg_output(cnt)
-    s.result()
+    s.download_result()
     s.close()
     return time.perf_counter() - t, cnt - 1
 

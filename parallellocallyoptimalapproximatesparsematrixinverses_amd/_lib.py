@@ -143,8 +143,20 @@ def device_name() -> str:
     return buf.value.decode()
 
 
+def _is_canonical(m) -> bool:
+    return (sparse.isspmatrix_csc(m) and m.dtype == np.float64 and m.indptr.dtype == np.int64
+            and m.indices.dtype == np.int64 and m.indptr.flags.c_contiguous and m.indices.flags.c_contiguous
+            and m.data.flags.c_contiguous and m.has_canonical_format)
+
+
 def as_csc(matrix, shape=None) -> sparse.csc_matrix:
-    """Canonical CSC with int64 indices (methods_cython/methods.pyx:74-87 semantics)."""
+    """Canonical CSC with int64 indices (methods_cython/methods.pyx:74-87
+    semantics).  A matrix already in that form is used as it is (the library
+    only reads it); anything else is copied and canonicalised."""
+    if _is_canonical(matrix):
+        if shape is not None and matrix.shape != tuple(shape):
+            raise ValueError("matrix shape does not match A")
+        return matrix
     csc = sparse.csc_matrix(matrix, dtype=np.float64, copy=True)
     if shape is not None and csc.shape != tuple(shape):
         raise ValueError("matrix shape does not match A")
@@ -320,6 +332,19 @@ class Session:
 
     def result(self) -> dict:
         return result_to_python(load().spai_session_result(self._h))
+
+    def download_result(self) -> int:
+        """The C-ABI result call alone (M and the history copied to host
+        arrays, then released), without building Python objects; returns
+        nnz(M)."""
+        L = load()
+        res = L.spai_session_result(self._h)
+        try:
+            if res.error_message:
+                raise RuntimeError(res.error_message.decode("utf-8", "replace"))
+            return int(res.nnz)
+        finally:
+            L.free_mem(ctypes.byref(res))
 
     def spmv_a_us(self) -> float:
         v = load().spai_session_pcg_iteration_us(self._h)

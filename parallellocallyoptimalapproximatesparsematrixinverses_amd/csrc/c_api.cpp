@@ -31,7 +31,9 @@ Result ErrorResult(const std::string& msg) {
   return out;
 }
 
-Result SuccessResult(const spai::EngineOutput& o) {
+// Hands M's arrays over to the Result (free_mem releases them with delete[])
+// and copies the history.
+Result SuccessResult(spai::EngineOutput& o) {
   Result r{};
   r.rows = o.M.rows;
   r.cols = o.M.cols;
@@ -39,24 +41,16 @@ Result SuccessResult(const spai::EngineOutput& o) {
   r.iterations = o.iterations;
   r.converged = o.converged ? 1 : 0;
   r.history_size = (int64_t)o.history.size();
-  try {
-    r.col_offsets = new int64_t[r.cols + 1];
-    r.row_indices = new int64_t[r.nnz > 0 ? r.nnz : 1];
-    r.values = new double[r.nnz > 0 ? r.nnz : 1];
-    r.history = new SingleIterationData[r.history_size > 0 ? r.history_size : 1];
-  } catch (...) {
-    delete[] r.col_offsets;
-    delete[] r.row_indices;
-    delete[] r.values;
-    delete[] r.history;
-    throw;
+  if (o.M.ptr.size() != size_t(r.cols + 1)) throw std::logic_error("result offsets have the wrong length");
+  if (r.nnz == 0) {
+    o.M.idx.resize(1);
+    o.M.val.resize(1);
   }
-  std::memcpy(r.col_offsets, o.M.ptr.data(), sizeof(int64_t) * (r.cols + 1));
-  if (r.nnz) {
-    std::memcpy(r.row_indices, o.M.idx.data(), sizeof(int64_t) * r.nnz);
-    std::memcpy(r.values, o.M.val.data(), sizeof(double) * r.nnz);
-  }
+  r.history = new SingleIterationData[r.history_size > 0 ? r.history_size : 1];
   for (int64_t i = 0; i < r.history_size; ++i) r.history[i] = o.history[i];
+  r.col_offsets = o.M.ptr.release();
+  r.row_indices = o.M.idx.release();
+  r.values = o.M.val.release();
   return r;
 }
 
@@ -64,8 +58,8 @@ Result RunMethod(int64_t method, const MatrixData& A, const MatrixData& M0, cons
                  const MethodParams& p, const SpaiExtParams* ext, double* seconds) {
   try {
     const spai::EngineParams ep = spai::ToEngineParams(p, ext);
-    const auto out = spai::RunEngine(method, A, p.use_initial_m ? &M0 : nullptr,
-                                     p.use_preconditioner ? &Pr : nullptr, ep);
+    auto out = spai::RunEngine(method, A, p.use_initial_m ? &M0 : nullptr, p.use_preconditioner ? &Pr : nullptr,
+                               ep);
     if (seconds) *seconds = out.device_seconds;
     return SuccessResult(out);
   } catch (const std::exception& ex) {

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -23,10 +24,33 @@ struct EngineParams {
   bool batch_ordered = false;
 };
 
+// Host array allocated with new[] and left uninitialised (it is filled by a
+// download), so the C ABI can hand it to the caller as a Result array, which
+// free_mem() releases with delete[].
+template <class T>
+class HostArray {
+ public:
+  void resize(size_t n) {
+    p_.reset(n > 0 ? new T[n] : nullptr);
+    n_ = n;
+  }
+  T* data() { return p_.get(); }
+  const T* data() const { return p_.get(); }
+  size_t size() const { return n_; }
+  T* release() {
+    n_ = 0;
+    return p_.release();
+  }
+
+ private:
+  std::unique_ptr<T[]> p_;
+  size_t n_ = 0;
+};
+
 struct HostMatrix {
   int64_t rows = 0, cols = 0;
-  std::vector<int64_t> ptr, idx;
-  std::vector<double> val;
+  HostArray<int64_t> ptr, idx;
+  HostArray<double> val;
 };
 
 struct EngineOutput {

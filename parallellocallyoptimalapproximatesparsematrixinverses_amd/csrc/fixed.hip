@@ -1471,10 +1471,36 @@ __global__ void MapSubPattern(int ncols, const int* __restrict__ sp, const int* 
   for (int p = sp[j]; p < sp[j + 1]; ++p) {
     const int q = FindSorted(s2i, s2p[j], s2p[j + 1], si[p]);
     if (q < 0) {
-      atomicOr(error_flag, 1);
+      *error_flag = 5;  // the context's error code (mapped host memory)
       continue;
     }
     s2_to_s[q] = p;
+  }
+}
+
+// Facts that decide how S and S2 are formed, and the checks of Pr =
+// diag(A)^-1 (methods_common.cpp:50-63) on d = diag(A), inverted in place:
+// facts[0] = 1 when a column of A lacks its diagonal entry, facts[1] = 1 when
+// M0 has an entry off the diagonal, facts[2] = 1 when a diagonal entry is
+// not finite and positive, facts[3] = 1 when its inverse is not finite.
+__global__ void PatternFacts(int n, const int* __restrict__ ap, const int* __restrict__ ai,
+                             const int* __restrict__ mp, const int* __restrict__ mi, double* __restrict__ d,
+                             int* __restrict__ facts) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  if (FindSorted(ai, ap[j], ap[j + 1], j) < 0) facts[0] = 1;
+  if (mp)
+    for (int e = mp[j]; e < mp[j + 1]; ++e)
+      if (mi[e] != j) facts[1] = 1;
+  if (d) {
+    const double v = d[j];
+    if (!isfinite(v) || v <= 0.0) {
+      facts[2] = 1;
+    } else {
+      const double r = 1.0 / v;
+      if (!isfinite(r)) facts[3] = 1;
+      d[j] = r;
+    }
   }
 }
 
@@ -1608,47 +1634,50 @@ class FixedImpl final : public FixedProblem {
       if (!IsFullDiagonal(c, Pr)) throw std::invalid_argument("fixed_pattern requires a diagonal Pr");
       ExtractDiagonal(c, Pr, pr_d_.get());
     } else {
-      std::vector<double> d(n_);
       ExtractDiagonal(c, A_, pr_d_.get());
-      c.ReadBack(d.data(), pr_d_.get(), sizeof(double) * n_);
-      for (double& v : d) {
-        if (!std::isfinite(v) || v <= 0.0)
-          throw std::invalid_argument("A has an invalid diagonal entry; provide Pr explicitly");
-        v = 1.0 / v;
-        if (!std::isfinite(v))
-          throw std::invalid_argument("A diagonal cannot be inverted safely; provide Pr explicitly");
-      }
-      SPAI_HIP_CHECK(hipMemcpyAsync(pr_d_.get(), d.data(), sizeof(double) * n_, hipMemcpyHostToDevice, c.stream));
-      c.Sync();
     }
-    // S = pattern(A) | pattern(M0) | I
-    DMat I = Identity(c, n_);
-    DMat S = Merge(c, A_, I, MergeOp::kCopyLeft, 0, 0, false);
-    if (M0h) {
-      m0_ = Upload(c, M0h->rows, M0h->cols, M0h->col_offsets, M0h->row_indices, M0h->values);
-      S = Merge(c, S, m0_, MergeOp::kCopyLeft, 0, 0, false);
+    if (M0h) m0_ = Upload(c, M0h->rows, M0h->cols, M0h->col_offsets, M0h->row_indices, M0h->values);
+    // one read-back decides the patterns and checks the diagonal
+    int facts[4] = {0, 0, 0, 0};
+    {
+      auto fd = c.Alloc<int>(4);
+      SPAI_HIP_CHECK(hipMemsetAsync(fd.get(), 0, 4 * sizeof(int), c.stream));
+      PatternFacts<<<Blocks1(n_), kBlk, 0, c.stream>>>((int)n_, A_.ptr(), A_.idx(), M0h ? m0_.ptr() : nullptr,
+                                                       M0h ? m0_.idx() : nullptr, Prh ? nullptr : pr_d_.get(),
+                                                       fd.get());
+      c.ReadBack(facts, fd.get(), sizeof(facts));
+    }
+    if (facts[2]) throw std::invalid_argument("A has an invalid diagonal entry; provide Pr explicitly");
+    if (facts[3]) throw std::invalid_argument("A diagonal cannot be inverted safely; provide Pr explicitly");
+    // S = pattern(A) | pattern(M0) | I and S2 = pattern(A S) | S | I.  When A
+    // has its whole diagonal and M0 (if given) none off it, S is pattern(A)
+    // and pattern(A S) already holds S (each (i, j) of S meets A(i, i)), so
+    // the unions are skipped.
+    if (!M0h) m0_ = Identity(c, n_);
+    if (!facts[0] && !facts[1]) {
+      s_pat_ = A_.pat;
+      DMat AS = SpGemm(c, A_, A_, false, true);  // only the pattern is kept
+      s2_pat_ = AS.pat;
     } else {
-      m0_ = std::move(I);
+      DMat I = Identity(c, n_);
+      DMat S = Merge(c, A_, I, MergeOp::kCopyLeft, 0, 0, false);
+      if (M0h) S = Merge(c, S, m0_, MergeOp::kCopyLeft, 0, 0, false);
+      s_pat_ = S.pat;
+      DMat AS = SpGemm(c, A_, S, false, true);
+      DMat S2 = Merge(c, AS, S, MergeOp::kCopyLeft, 0, 0, false);
+      s2_pat_ = S2.pat;
     }
-    s_pat_ = S.pat;
-    // S2 = pattern(A S) | S | I
-    DMat AS = SpGemm(c, A_, S, false, true);
-    DMat S2 = Merge(c, AS, S, MergeOp::kCopyLeft, 0, 0, false);
-    s2_pat_ = S2.pat;
     nnz_s_ = s_pat_->nnz;
     nnz_s2_ = s2_pat_->nnz;
     s2col_ = c.Alloc<int>(nnz_s2_);
     ColumnOfEntry<<<Blocks1(n_), kBlk, 0, c.stream>>>((int)n_, s2_pat_->ptr.get(), s2col_.get());
     s2_to_s_ = c.Alloc<int>(nnz_s2_);
     SPAI_HIP_CHECK(hipMemsetAsync(s2_to_s_.get(), 0xFF, sizeof(int) * nnz_s2_, c.stream));
-    auto err = c.Alloc<int>(1);
-    SPAI_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), c.stream));
+    // a miss is reported through the context's error code at its next
+    // synchronisation
     MapSubPattern<<<Blocks1(n_), kBlk, 0, c.stream>>>((int)n_, s_pat_->ptr.get(), s_pat_->idx.get(),
                                                       s2_pat_->ptr.get(), s2_pat_->idx.get(), s2_to_s_.get(),
-                                                      err.get());
-    int e = 0;
-    c.ReadBack(&e, err.get(), sizeof(int));
-    if (e) throw std::runtime_error("fixed pattern S is not contained in pattern(A S)");
+                                                      c.err_dev);
     // slot plan of the products (fixed pattern)
     {
       SlotPlan plan = BuildSlotPlan(c, n_, s_pat_->ptr.get(), s_pat_->idx.get(), A_.ptr(), A_.idx(),

@@ -238,7 +238,11 @@ mc::OutputResult Run(int64_t method, const char* name, const mc::InputParams& p,
   history.reserve(out.history.size());
   for (const ::SingleIterationData& h : out.history)
     history.emplace_back(h.iteration, h.residual_norm, h.density_m, h.density_direction);
-  mc::SparseMatrix M(out.M.rows, out.M.cols, std::move(out.M.ptr), std::move(out.M.idx), std::move(out.M.val));
+  const int64_t nnz = int64_t(out.M.idx.size());
+  std::vector<int64_t> ptr(out.M.ptr.data(), out.M.ptr.data() + out.M.ptr.size());
+  std::vector<int64_t> idx(out.M.idx.data(), out.M.idx.data() + nnz);
+  std::vector<double> val(out.M.val.data(), out.M.val.data() + nnz);
+  mc::SparseMatrix M(out.M.rows, out.M.cols, std::move(ptr), std::move(idx), std::move(val));
   return mc::OutputResult(std::move(M), out.iterations, out.converged, name, std::move(history));
 }
 

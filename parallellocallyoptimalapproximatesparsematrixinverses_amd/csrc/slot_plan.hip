@@ -22,7 +22,7 @@ __device__ __forceinline__ int WaveMax(int v) {
 
 // terms[j] = sum over S(:, j) of |A(:, k)|, one thread per column (eight S
 // entries' A column bounds loaded together); ext = the longest column of A,
-// of S2 and of S (one atomic per wavefront).
+// of S2 and of S (one atomic per block).
 __global__ __launch_bounds__(kPlanBlock) void PlanTerms(int n, const int* __restrict__ sp, const int* __restrict__ si,
                                                         const int* __restrict__ ap, const int* __restrict__ s2p,
                                                         int64_t* __restrict__ terms, int* __restrict__ ext) {
@@ -48,14 +48,20 @@ __global__ __launch_bounds__(kPlanBlock) void PlanTerms(int n, const int* __rest
     ls2 = s2p[j + 1] - s2p[j];
     ls = p1 - p0;
   }
+  // block maxima in LDS, then one atomic per block and value
+  __shared__ int bmax[3];
+  if (threadIdx.x < 3) bmax[threadIdx.x] = 0;
+  __syncthreads();
   la = WaveMax(la);
   ls2 = WaveMax(ls2);
   ls = WaveMax(ls);
   if (LaneId() == 0) {
-    atomicMax(&ext[0], la);
-    atomicMax(&ext[1], ls2);
-    atomicMax(&ext[2], ls);
+    atomicMax(&bmax[0], la);
+    atomicMax(&bmax[1], ls2);
+    atomicMax(&bmax[2], ls);
   }
+  __syncthreads();
+  if (threadIdx.x < 3) atomicMax(&ext[threadIdx.x], bmax[threadIdx.x]);
 }
 
 // ---- int64 exclusive scan: tile sums, one block over the tile sums, apply

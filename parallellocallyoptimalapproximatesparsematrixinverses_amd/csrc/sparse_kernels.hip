@@ -1872,6 +1872,7 @@ void Ctx::CheckErrorCode() {
   *err_host = 0;
   if (e == 1) throw std::runtime_error("SpGEMM numeric phase found a row outside the symbolic pattern");
   if (e == 2) throw std::runtime_error("product term outside pattern(A S)");
+  if (e == 5) throw std::runtime_error("fixed pattern S is not contained in pattern(A S)");
   throw std::runtime_error("device consistency check failed (code " + std::to_string(e) + ")");
 }
 
