@@ -38,16 +38,35 @@ constexpr int kBlk = 256;
 
 // Offsets of the product terms inside each column's plan: for the S entry e
 // = (k, l), poff[e] = the number of terms of column l before those of A(:, k).
+// One thread per column, the A column bounds of eight S entries loaded
+// together.
 __global__ void TermOffsets(int n, const int* __restrict__ sp, const int* __restrict__ si,
                             const int* __restrict__ ap, int* __restrict__ poff) {
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= n) return;
+  const int e0 = sp[l], e1 = sp[l + 1];
   int off = 0;
-  for (int e = sp[l]; e < sp[l + 1]; ++e) {
-    poff[e] = off;
-    const int k = si[e];
-    off += ap[k + 1] - ap[k];
+  for (int e = e0; e < e1; e += 8) {
+    int k[8], len[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) k[t] = e + t < e1 ? si[e + t] : -1;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) len[t] = k[t] >= 0 ? ap[k[t] + 1] - ap[k[t]] : 0;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      if (e + t < e1) poff[e + t] = off;
+      off += len[t];
+    }
   }
+}
+
+// Position of row j inside S2(:, j) for every column (-1 when absent).
+__global__ void DiagSlots(int n, const int* __restrict__ s2p, const int* __restrict__ s2i, int* __restrict__ s2d) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int q0 = s2p[j], q1 = s2p[j + 1];
+  const int p = LowerBound(s2i, q0, q1, j);
+  s2d[j] = p < q1 && s2i[p] == j ? p - q0 : -1;
 }
 
 // Sum over the GL lanes of a group (a fixed tree, so deterministic).
@@ -66,21 +85,35 @@ __device__ __forceinline__ double GroupSum(double v) {
 // the column's S2 offset minus q_base), and df/dM(k, j) = -2 scale
 // sum_u A(u, k) R(u, j) for every S entry of the column: the same terms
 // walked again, each a product with R read back from LDS at its slot, summed
-// over the group's lanes in FP64.
-template <class V, class SlotT, int GL, bool kBack>
+// over the group's lanes in FP64.  kNarrow: every column of A has at most GL
+// entries (one term per lane and step).  The S entries' A column bounds and M
+// values are staged in LDS per chunk of GL entries, so that a step reads them
+// with one broadcast load.
+constexpr int kLossBatch = 8;  // steps whose loads are in flight together
+
+// One step of a column's product: the S entry's A column (start, length) and
+// its M value, staged in LDS so that a step reads it with one broadcast load.
+template <class V>
+struct alignas(16) LossStep {
+  int a0, c;
+  V m;
+};
+template <class V, class SlotT, int GL, bool kBack, bool kNarrow>
 __global__ __launch_bounds__(kBlk) void LossColumns(int j0, int j1, const int* __restrict__ sp,
                                                     const int* __restrict__ si, const V* __restrict__ mv,
                                                     const int* __restrict__ ap, const V* __restrict__ av,
                                                     const int64_t* __restrict__ plan_ptr,
                                                     const SlotT* __restrict__ plan, const int* __restrict__ s2p,
-                                                    const int* __restrict__ s2i, int64_t q_base, int lmax,
+                                                    const int* __restrict__ s2d, int64_t q_base, int lmax,
                                                     V* __restrict__ rblk, double scale, V* __restrict__ gm,
                                                     double* __restrict__ part) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   __shared__ double red_s[16];
   const int groups = blockDim.x / GL;
   const int g = threadIdx.x / GL, gl = threadIdx.x % GL;
-  V* const acc = reinterpret_cast<V*>(lds_raw) + size_t(g) * lmax;
+  // LDS: the groups' step tables, then their R columns (lmax entries each)
+  LossStep<V>* const steps = reinterpret_cast<LossStep<V>*>(lds_raw) + size_t(g) * GL;
+  V* const acc = reinterpret_cast<V*>(lds_raw + size_t(groups) * GL * sizeof(LossStep<V>)) + size_t(g) * lmax;
   double red = 0.0;
   for (int j = j0 + blockIdx.x * groups + g; j < j1; j += gridDim.x * groups) {
     const int q0 = s2p[j], L = s2p[j + 1] - q0;
@@ -89,27 +122,49 @@ __global__ __launch_bounds__(kBlk) void LossColumns(int j0, int j1, const int* _
     const int e0 = sp[j], e1 = sp[j + 1];
     int64_t pp = plan_ptr[j];
     for (int eb = e0; eb < e1; eb += GL) {
-      int my_a0 = 0, my_c = 0;
-      V my_m = V(0);
+      // the chunk's steps: lane gl stages S entry eb + gl
       if (eb + gl < e1) {
         const int k = si[eb + gl];
-        my_a0 = ap[k];
-        my_c = ap[k + 1] - my_a0;
-        my_m = mv[eb + gl];
+        const int a0 = ap[k];
+        steps[gl] = LossStep<V>{a0, ap[k + 1] - a0, mv[eb + gl]};
       }
+      WaveFence();
       const int ns = min(GL, e1 - eb);
-      for (int st = 0; st < ns; ++st) {
-        const int a0 = __shfl(my_a0, st, GL), c = __shfl(my_c, st, GL);
-        const V m = __shfl(my_m, st, GL);
-        // distinct rows within one column of A: no two lanes share a slot
-        for (int u = gl; u < c; u += GL) acc[plan[pp + u]] += av[a0 + u] * m;
-        pp += c;
+      if (kNarrow) {
+        // every column of A within GL: lane gl holds term gl of a step; the
+        // slots and values of kLossBatch steps are loaded before their
+        // additions, which then follow in step order
+        for (int sb = 0; sb < ns; sb += kLossBatch) {
+          int slot[kLossBatch];
+          V prod[kLossBatch];
+#pragma unroll
+          for (int b = 0; b < kLossBatch; ++b) {
+            const int st = sb + b;
+            const LossStep<V> sd = steps[st < ns ? st : 0];
+            const bool has = st < ns && gl < sd.c;
+            slot[b] = has ? int(plan[pp + gl]) : -1;
+            prod[b] = has ? av[sd.a0 + gl] * sd.m : V(0);
+            if (st < ns) pp += sd.c;
+          }
+#pragma unroll
+          for (int b = 0; b < kLossBatch; ++b)
+            if (slot[b] >= 0) acc[slot[b]] += prod[b];
+        }
+      } else {
+        for (int st = 0; st < ns; ++st) {
+          const LossStep<V> sd = steps[st];
+          // distinct rows within one column of A: no two lanes share a slot
+          for (int u = gl; u < sd.c; u += GL) acc[plan[pp + u]] += av[sd.a0 + u] * sd.m;
+          pp += sd.c;
+        }
       }
+      WaveFence();
     }
     WaveFence();
+    const int dj = s2d[j];  // slot of the diagonal entry in S2(:, j)
     for (int t = gl; t < L; t += GL) {
       V r = -acc[t];
-      if (s2i[q0 + t] == j) r += V(1);
+      if (t == dj) r += V(1);
       red += double(r) * double(r);
       if (kBack) {
         acc[t] = r;
@@ -120,23 +175,47 @@ __global__ __launch_bounds__(kBlk) void LossColumns(int j0, int j1, const int* _
       WaveFence();
       pp = plan_ptr[j];
       for (int eb = e0; eb < e1; eb += GL) {
-        int my_a0 = 0, my_c = 0;
         if (eb + gl < e1) {
           const int k = si[eb + gl];
-          my_a0 = ap[k];
-          my_c = ap[k + 1] - my_a0;
+          const int a0 = ap[k];
+          steps[gl] = LossStep<V>{a0, ap[k + 1] - a0, V(0)};
         }
+        WaveFence();
         const int ns = min(GL, e1 - eb);
-        double mine = 0.0;  // lane st keeps the dot of step st
-        for (int st = 0; st < ns; ++st) {
-          const int a0 = __shfl(my_a0, st, GL), c = __shfl(my_c, st, GL);
-          double s = 0.0;
-          for (int u = gl; u < c; u += GL) s += double(av[a0 + u]) * double(acc[plan[pp + u]]);
-          s = GroupSum<GL>(s);
-          if (gl == st) mine = s;
-          pp += c;
+        if (kNarrow) {
+          // kLossBatch dots at a time: the loads first, then the lane sums
+          for (int sb = 0; sb < ns; sb += kLossBatch) {
+            double s[kLossBatch];
+#pragma unroll
+            for (int b = 0; b < kLossBatch; ++b) {
+              const int st = sb + b;
+              const LossStep<V> sd = steps[st < ns ? st : 0];
+              const bool has = st < ns && gl < sd.c;
+              const int slot = has ? int(plan[pp + gl]) : 0;
+              s[b] = has ? double(av[sd.a0 + gl]) * double(acc[slot]) : 0.0;
+              if (st < ns) pp += sd.c;
+            }
+            double mine = 0.0;
+#pragma unroll
+            for (int b = 0; b < kLossBatch; ++b) {
+              const double t = GroupSum<GL>(s[b]);
+              if (gl == b) mine = t;
+            }
+            if (gl < kLossBatch && sb + gl < ns) gm[eb + sb + gl] = V(-2.0 * scale * mine);
+          }
+        } else {
+          double mine = 0.0;  // lane st keeps the dot of step st
+          for (int st = 0; st < ns; ++st) {
+            const LossStep<V> sd = steps[st];
+            double s = 0.0;
+            for (int u = gl; u < sd.c; u += GL) s += double(av[sd.a0 + u]) * double(acc[plan[pp + u]]);
+            s = GroupSum<GL>(s);
+            if (gl == st) mine = s;
+            pp += sd.c;
+          }
+          if (gl < ns) gm[eb + gl] = V(-2.0 * scale * mine);
         }
-        if (gl < ns) gm[eb + gl] = V(-2.0 * scale * mine);
+        WaveFence();
       }
     }
     WaveFence();
@@ -153,6 +232,16 @@ __global__ __launch_bounds__(kBlk) void LossColumns(int j0, int j1, const int* _
 // lane gl holds rows gl + GL r (r < kRegs); FP64 products and sums; blocks
 // add in column order, so the result is deterministic.
 constexpr int kGradARegs = 4;
+
+// One step of LossGradA: the term offset of the S entry (k, l) in the plan,
+// the S2 offset of column l inside the R block, and M(k, l).
+template <class V>
+struct alignas(8) GradStep {
+  int64_t off;
+  int rb;
+  V m;
+};
+
 template <class V, class SlotT, int GL>
 __global__ __launch_bounds__(kBlk) void LossGradA(int k0, int k1, int j0, int j1, const int* __restrict__ ap,
                                                   const int* __restrict__ stp, const int* __restrict__ sti,
@@ -161,8 +250,10 @@ __global__ __launch_bounds__(kBlk) void LossGradA(int k0, int k1, int j0, int j1
                                                   const SlotT* __restrict__ plan, const int* __restrict__ poff,
                                                   const int* __restrict__ s2p, int64_t q_base,
                                                   const V* __restrict__ rblk, double* __restrict__ acc_a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
   const int groups = blockDim.x / GL;
   const int g = threadIdx.x / GL, gl = threadIdx.x % GL;
+  GradStep<V>* const steps = reinterpret_cast<GradStep<V>*>(lds_raw) + size_t(g) * GL;
   for (int k = k0 + blockIdx.x * groups + g; k < k1; k += gridDim.x * groups) {
     const int a0 = ap[k], c = ap[k + 1] - a0;
     const int x = LowerBound(sti, stp[k], stp[k + 1], j0);
@@ -171,24 +262,42 @@ __global__ __launch_bounds__(kBlk) void LossGradA(int k0, int k1, int j0, int j1
 #pragma unroll
     for (int r = 0; r < kGradARegs; ++r) acc[r] = 0.0;
     for (int tb = x; tb < xe; tb += GL) {
-      V my_m = V(0);
-      int64_t my_off = 0, my_rb = 0;
       if (tb + gl < xe) {
         const int p = stperm[tb + gl], l = sti[tb + gl];
-        my_m = mv[p];
-        my_off = plan_ptr[l] + poff[p];
-        my_rb = int64_t(s2p[l]) - q_base;
+        steps[gl] = GradStep<V>{plan_ptr[l] + poff[p], int(int64_t(s2p[l]) - q_base), mv[p]};
       }
+      WaveFence();
       const int ns = min(GL, xe - tb);
-      for (int st = 0; st < ns; ++st) {
-        const double m = double(__shfl(my_m, st, GL));
-        const int64_t off = __shfl(my_off, st, GL), rb = __shfl(my_rb, st, GL);
+      if (c <= GL) {
+        // one row per lane: the slots, then the R values of kLossBatch
+        // steps are loaded before their additions (in step order)
+        for (int sb = 0; sb < ns; sb += kLossBatch) {
+          int at[kLossBatch];
+          double m[kLossBatch], rv[kLossBatch];
 #pragma unroll
-        for (int r = 0; r < kGradARegs; ++r) {
-          const int u = gl + GL * r;
-          if (u < c) acc[r] += m * double(rblk[rb + int64_t(plan[off + u])]);
+          for (int b = 0; b < kLossBatch; ++b) {
+            const int st = sb + b;
+            const GradStep<V> sd = steps[st < ns ? st : 0];
+            const bool has = st < ns && gl < c;
+            m[b] = double(sd.m);
+            at[b] = has ? sd.rb + int(plan[sd.off + gl]) : -1;
+          }
+#pragma unroll
+          for (int b = 0; b < kLossBatch; ++b) rv[b] = at[b] >= 0 ? double(rblk[at[b]]) : 0.0;
+#pragma unroll
+          for (int b = 0; b < kLossBatch; ++b) acc[0] += m[b] * rv[b];
+        }
+      } else {
+        for (int st = 0; st < ns; ++st) {
+          const GradStep<V> sd = steps[st];
+#pragma unroll
+          for (int r = 0; r < kGradARegs; ++r) {
+            const int u = gl + GL * r;
+            if (u < c) acc[r] += double(sd.m) * double(rblk[sd.rb + int(plan[sd.off + u])]);
+          }
         }
       }
+      WaveFence();
     }
 #pragma unroll
     for (int r = 0; r < kGradARegs; ++r) {
@@ -198,29 +307,67 @@ __global__ __launch_bounds__(kBlk) void LossGradA(int k0, int k1, int j0, int j1
   }
 }
 
-// Row range of S over columns [first[b], first[b + 1]) of each block b:
-// the columns k of A that a block's R reaches in LossGradA.
-__global__ void BlockRowRanges(int n, const int* __restrict__ sp, const int* __restrict__ si, int block_cols,
-                               int* __restrict__ kmin, int* __restrict__ kmax) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  const int b = j / block_cols;
-  int lo = INT_MAX, hi = -1;
-  for (int e = sp[j]; e < sp[j + 1]; ++e) {
-    lo = min(lo, si[e]);
-    hi = max(hi, si[e]);
-  }
-  if (hi >= 0) {
-    atomicMin(&kmin[b], lo);
-    atomicMax(&kmax[b], hi);
+// Row range of S over the columns of each block b of block_cols columns:
+// the columns k of A that a block's R reaches in LossGradA.  blockIdx.y
+// strides over the column blocks, blockIdx.x over chunks of one block's
+// columns; one atomic pair per workgroup.
+__global__ __launch_bounds__(kBlk) void BlockRowRanges(int n, const int* __restrict__ sp, const int* __restrict__ si,
+                                                      int block_cols, int nb, int* __restrict__ kmin,
+                                                      int* __restrict__ kmax) {
+  __shared__ int s_lo[kBlk / kWave], s_hi[kBlk / kWave];
+  for (int b = blockIdx.y; b < nb; b += gridDim.y) {
+    const int64_t c0 = int64_t(b) * block_cols, c1 = std::min<int64_t>(n, c0 + block_cols);
+    int lo = INT_MAX, hi = -1;
+    for (int64_t j = c0 + int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < c1;
+         j += int64_t(gridDim.x) * blockDim.x) {
+      const int e0 = sp[j], e1 = sp[j + 1];
+      if (e1 > e0) {  // rows sorted inside a column
+        lo = min(lo, si[e0]);
+        hi = max(hi, si[e1 - 1]);
+      }
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      lo = min(lo, __shfl_xor(lo, off, kWave));
+      hi = max(hi, __shfl_xor(hi, off, kWave));
+    }
+    if (LaneId() == 0) {
+      s_lo[WaveInBlock()] = lo;
+      s_hi[WaveInBlock()] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < int(blockDim.x) / kWave; ++w) {
+        lo = min(lo, s_lo[w]);
+        hi = max(hi, s_hi[w]);
+      }
+      if (hi >= 0) {
+        atomicMin(&kmin[b], lo);
+        atomicMax(&kmax[b], hi);
+      }
+    }
+    __syncthreads();
   }
 }
 
-// Deterministic sum of the per-block partials (one block).
-__global__ void SumLoss(const double* __restrict__ part, int np, double* __restrict__ out) {
+// Deterministic sum of the per-block partials (one block of kSumThreads;
+// each thread adds its strided partials in index order, eight loads at a
+// time).
+constexpr int kSumThreads = 1024;
+__global__ __launch_bounds__(kSumThreads) void SumLoss(const double* __restrict__ part, int np,
+                                                      double* __restrict__ out) {
   __shared__ double red[16];
   double s = 0.0;
-  for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[i];
+  for (int i0 = 0; i0 < np; i0 += 8 * kSumThreads) {
+    double v[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int i = i0 + t * kSumThreads + int(threadIdx.x);
+      v[t] = i < np ? part[i] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) s += v[t];
+  }
   s = BlockSum<double>(s, red);
   if (threadIdx.x == 0) *out = s;
 }
@@ -258,6 +405,7 @@ struct SpaiLossPlan {
   spai::DMat A, S, S2;                 // patterns (values unused)
   spai::SlotPlan plan;                 // slots of the terms of A S on S2
   spai::DBuf<int> poff;                // per S entry: its first term inside its column's plan
+  spai::DBuf<int> s2d;                 // per column j: the slot of row j in S2(:, j)
   spai::PatternPtr ST;                 // S^T
   spai::DBuf<int> stperm;              // S^T entry -> S entry
   spai::DBuf<char> rblk;               // one column block of R
@@ -300,21 +448,31 @@ void LaunchBlock(SpaiLossPlan* p, size_t b, const V* m, const V* a, bool backwar
   const SlotT* plan = std::is_same<SlotT, uint8_t>::value ? reinterpret_cast<const SlotT*>(p->plan.plan8.get())
                                                           : reinterpret_cast<const SlotT*>(p->plan.plan16.get());
   V* rblk = reinterpret_cast<V*>(p->rblk.get());
-  const size_t lds = size_t(p->block_threads / GL) * size_t(p->lmax) * sizeof(V);
+  const size_t lds = size_t(p->block_threads / GL) * (GL * sizeof(LossStep<V>) + size_t(p->lmax) * sizeof(V));
   double* part = p->part.get() + p->part_off[b];
-  if (backward)
-    LossColumns<V, SlotT, GL, true><<<p->grid[b], p->block_threads, lds, c.stream>>>(
-        j0, j1, p->S.ptr(), p->S.idx(), m, p->A.ptr(), a, p->plan.ptr.get(), plan, p->S2.ptr(), p->S2.idx(),
-        p->q_base[b], p->lmax, rblk, scale, gm, part);
-  else
-    LossColumns<V, SlotT, GL, false><<<p->grid[b], p->block_threads, lds, c.stream>>>(
-        j0, j1, p->S.ptr(), p->S.idx(), m, p->A.ptr(), a, p->plan.ptr.get(), plan, p->S2.ptr(), p->S2.idx(),
-        p->q_base[b], p->lmax, rblk, scale, gm, part);
+#define SPAI_LOSS_COLUMNS(BACK, NARROW)                                                                       \
+  LossColumns<V, SlotT, GL, BACK, NARROW><<<p->grid[b], p->block_threads, lds, c.stream>>>(                     \
+      j0, j1, p->S.ptr(), p->S.idx(), m, p->A.ptr(), a, p->plan.ptr.get(), plan, p->S2.ptr(), p->s2d.get(), \
+      p->q_base[b], p->lmax, rblk, scale, gm, part)
+  const bool narrow = p->plan.max_a_col <= GL;
+  if (backward) {
+    if (narrow)
+      SPAI_LOSS_COLUMNS(true, true);
+    else
+      SPAI_LOSS_COLUMNS(true, false);
+  } else {
+    if (narrow)
+      SPAI_LOSS_COLUMNS(false, true);
+    else
+      SPAI_LOSS_COLUMNS(false, false);
+  }
+#undef SPAI_LOSS_COLUMNS
   if (backward && p->kmax[b] >= p->kmin[b]) {
     const int k0 = p->kmin[b], k1 = p->kmax[b] + 1;
     const int groups = p->block_threads / GL;
     const unsigned ga = (unsigned)std::max<int64_t>(1, std::min<int64_t>((k1 - k0 + groups - 1) / groups, 8192));
-    LossGradA<V, SlotT, GL><<<ga, p->block_threads, 0, c.stream>>>(
+    const size_t lds_a = size_t(p->block_threads) * sizeof(GradStep<V>);
+    LossGradA<V, SlotT, GL><<<ga, p->block_threads, lds_a, c.stream>>>(
         k0, k1, j0, j1, p->A.ptr(), p->ST->ptr.get(), p->ST->idx.get(), p->stperm.get(), m, p->plan.ptr.get(), plan,
         p->poff.get(), p->S2.ptr(), p->q_base[b], rblk, p->acc.get());
   }
@@ -345,7 +503,7 @@ int64_t Forward(SpaiLossPlan* p, const V* m, const V* a, double* loss, bool back
       LaunchBlockGl<V, uint8_t>(p, b, m, a, backward, scale, gm);
   }
   if (backward) ScaleOut<V><<<GridAll(p->A.nnz()), kBlk, 0, st>>>(p->A.nnz(), p->acc.get(), scale, ga);
-  SumLoss<<<1, kBlk, 0, st>>>(p->part.get(), p->part_off.back(), p->loss.get());
+  SumLoss<<<1, kSumThreads, 0, st>>>(p->part.get(), p->part_off.back(), p->loss.get());
   SPAI_HIP_CHECK(hipGetLastError());
   double total = 0.0;
   c.ReadBack(&total, p->loss.get(), sizeof(double));
@@ -379,15 +537,20 @@ SpaiLossPlan* spai_loss_plan_create(MatrixData A, MatrixData M, int64_t precisio
       throw std::invalid_argument("the differentiation form needs columns of A of at most 256 entries");
     p->gl = p->plan.max_a_col <= 8 ? 8 : p->plan.max_a_col <= 16 ? 16 : p->plan.max_a_col <= 32 ? 32 : 64;
     p->lmax = std::max(p->plan.max_s2_col, 1);
-    // R columns in LDS: fewer lanes per block when 256 would not fit
+    // step tables and R columns in LDS: fewer lanes per block when 256
+    // would not fit
     const size_t vsize = p->fp32 ? 4 : 8;
+    const size_t per_group = size_t(p->gl) * (p->fp32 ? sizeof(LossStep<float>) : sizeof(LossStep<double>)) +
+                             size_t(p->lmax) * vsize;
     p->block_threads = kBlk;
-    while (p->block_threads > p->gl && size_t(p->block_threads / p->gl) * size_t(p->lmax) * vsize > 48 * 1024)
+    while (p->block_threads > p->gl && size_t(p->block_threads / p->gl) * per_group > 48 * 1024)
       p->block_threads /= 2;
-    if (size_t(p->block_threads / p->gl) * size_t(p->lmax) * vsize > 64 * 1024)
+    if (size_t(p->block_threads / p->gl) * per_group > 64 * 1024)
       throw std::invalid_argument("the differentiation form needs columns of A M within 64 KiB of LDS");
     p->poff = c.Alloc<int>(std::max<int64_t>(p->S.nnz(), 1));
     TermOffsets<<<GridAll(p->n), kBlk, 0, c.stream>>>((int)p->n, p->S.ptr(), p->S.idx(), p->A.ptr(), p->poff.get());
+    p->s2d = c.Alloc<int>(size_t(p->n));
+    DiagSlots<<<GridAll(p->n), kBlk, 0, c.stream>>>((int)p->n, p->S2.ptr(), p->S2.idx(), p->s2d.get());
     p->ST = TransposeWithPerm(c, p->S, p->stperm);
     for (int64_t j = 0; j < p->n; j += p->block_cols) p->blocks.push_back((int)j);
     p->blocks.push_back((int)p->n);
@@ -403,8 +566,10 @@ SpaiLossPlan* spai_loss_plan_create(MatrixData A, MatrixData M, int64_t precisio
         init[nb + b] = -1;
       }
       SPAI_HIP_CHECK(hipMemcpyAsync(kr.get(), init.data(), sizeof(int) * 2 * nb, hipMemcpyHostToDevice, c.stream));
-      BlockRowRanges<<<GridAll(p->n), kBlk, 0, c.stream>>>((int)p->n, p->S.ptr(), p->S.idx(), (int)p->block_cols,
-                                                           kr.get(), kr.get() + nb);
+      const dim3 grr((unsigned)std::min<int64_t>((p->block_cols + kBlk - 1) / kBlk, 16),
+                     (unsigned)std::min<size_t>(nb, 65535));
+      BlockRowRanges<<<grr, kBlk, 0, c.stream>>>((int)p->n, p->S.ptr(), p->S.idx(), (int)p->block_cols, (int)nb,
+                                                 kr.get(), kr.get() + nb);
       std::vector<int> h(2 * nb);
       c.ReadBack(h.data(), kr.get(), sizeof(int) * 2 * nb);
       p->kmin.assign(h.begin(), h.begin() + nb);
