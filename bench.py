@@ -110,6 +110,38 @@ def cold_step(lib, A, M0, b, precision):
     return time.perf_counter() - t, cnt - 1
 
 
+def fused_pass_bytes(n, nnz_a, nnz_s, nnz_s2, sv, method):
+    """Algorithmic bytes of one fused iteration pass (int32 indices and
+    pointers, s_v bytes per value): A once ((n+1) 4 + nnz(A) (4 + s_v)), the
+    patterns of S and S2 once ((n+1) 4 + nnz 4 each); LOMR reads and writes R
+    on S2 and M, Z, P on S (2 nnz(S2) s_v + 6 nnz(S) s_v); MR writes R
+    (recomputed as I - A M) and reads and writes M and Z (nnz(S2) s_v +
+    4 nnz(S) s_v)."""
+    base = (n + 1) * 4 + nnz_a * (4 + sv) + (n + 1) * 4 + nnz_s * 4 + (n + 1) * 4 + nnz_s2 * 4
+    if method == "lomr":
+        return base + 2 * nnz_s2 * sv + 6 * nnz_s * sv
+    return base + nnz_s2 * sv + 4 * nnz_s * sv
+
+
+def pcg_iteration_bytes(n, nnz_a, nnz_m, m_bytes):
+    """Algorithmic bytes of one PCG iteration: the SpMV with A (FP64 values,
+    int32 indices: (n+1) 4 + 12 nnz(A)), the SpMV with M ((n+1) 4 +
+    nnz(M) (4 + m_bytes)), and the FP64 vectors -- p = z + beta p with A p
+    reads z, p and writes p, Ap; the update reads x, p, r, Ap and writes x,
+    r, z -- 11 n doubles."""
+    return 2 * (n + 1) * 4 + 12 * nnz_a + nnz_m * (4 + m_bytes) + 11 * 8 * n
+
+
+def roof(bytes_, seconds, achievable=None):
+    """GB/s of algorithmic bytes and the fractions of HBM peak (and of the
+    in-repo measured streaming bandwidth, when given)."""
+    gbs = bytes_ / seconds / 1e9
+    out = {"algorithmic_bytes": int(bytes_), "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    if achievable:
+        out["frac_of_achievable"] = round(gbs / achievable, 4)
+    return out
+
+
 def s2_nnz(A):
     """nnz of pattern(A S) | S | I with S = pattern(A) (Boolean product)."""
     from scipy import sparse
@@ -197,122 +229,183 @@ def _median_time(fn, reps):
     return float(np.median(ts))
 
 
-def run_configs(with_cpu, reps=5):
+def _fused_kernel_roof(s, method, n, nnz_a, sv, achievable):
+    """The session's per-iteration fused pass: average HIP-event time over
+    one more build, and its algorithmic bytes (fused_pass_bytes)."""
+    info = s.info()
+    s.set_kernel_timing(True)
+    s.build()
+    us, launches = s.build_kernel_us()
+    s.set_kernel_timing(False)
+    if not info["fused"] or us <= 0:
+        return None
+    out = {"kernel": "FusedPass (one %s iteration)" % method.upper(), "avg_launch_us": round(us, 3),
+           "launches_timed": launches}
+    out.update(roof(fused_pass_bytes(n, nnz_a, info["nnz_s"], info["nnz_s2"], sv, method), us * 1e-6, achievable))
+    return out
+
+
+def run_configs(with_cpu, reps=5, only=None):
     """One JSON line per configuration besides the headline one: device times
     of the build and the solves (operands resident; medians of `reps` runs
-    after one warm-up) and, with with_cpu, the oracle on a bounded sample."""
-    from scipy import sparse
-
+    after one warm-up), the dominant kernels' GB/s of algorithmic bytes with
+    their fractions of HBM peak and of the measured streaming bandwidth, and,
+    with with_cpu, the oracle on a bounded sample."""
     from parallellocallyoptimalapproximatesparsematrixinverses_amd import _lib, problems
     from parallellocallyoptimalapproximatesparsematrixinverses_amd.autograd import ResidualLoss
 
     threads = host_threads()
-    out = []
+    bw = _lib.stream_bandwidth(2 << 30, 20)
+    achievable = max(bw["copy"], bw["triad"])
+    print(json.dumps({"achievable_bandwidth": {"copy_GBs": round(bw["copy"], 1), "triad_GBs": round(bw["triad"], 1),
+                                               "read_GBs": round(bw["read"], 1),
+                                               "arrays": "FP64, 2 GiB each, median of 20 launches"}}), flush=True)
+    want = set(only) if only else {1, 3, 4, 5}
 
-    # config 1: 2D Poisson 32^2, MR on pattern(A), 20 iterations, PCG to 1e-8
-    A = problems.poisson2d(32)
-    M0 = problems.diag_inverse(A)
-    b = A @ np.random.default_rng(0).uniform(-1, 1, A.shape[0])
-    s = _lib.Session("mr", A, M0=M0, max_iterations=20)
-    s.set_rhs(b)
-    s.build()
-    s.pcg(1000, 1e-8)
-    tb = _median_time(s.build, reps)
-    tp = _median_time(lambda: s.pcg(1000, 1e-8), reps)
-    st, cnt = s.pcg(1000, 1e-8)
-    s.close()
-    line = {"config": "poisson2d 32^2, MR fixed pattern 20 its + PCG 1e-8", "build_s": tb, "pcg_s": tp,
-            "pcg_iterations": cnt - 1, "status": st}
-    if with_cpu:
-        from oracle import oracle
-        t = time.perf_counter()
-        r = oracle.run("mr", A, M0, None, 20, 0.0, 1.0, False, threads, fixed_pattern=True, omp_threads=threads)
-        oracle.pcg(A.tocsr(), r["M"].tocsr(), b, 1000, 1e-8, threads=threads)
-        line["cpu_baseline"] = {"value": time.perf_counter() - t, "unit": "s", "cores": threads, "kind": "port",
-                                "sample": "full workload"}
-    out.append(line)
+    if 1 in want:
+        # config 1: 2D Poisson 32^2, MR on pattern(A), 20 iterations, PCG to 1e-8
+        A = problems.poisson2d(32)
+        M0 = problems.diag_inverse(A)
+        b = A @ np.random.default_rng(0).uniform(-1, 1, A.shape[0])
+        s = _lib.Session("mr", A, M0=M0, max_iterations=20)
+        s.set_rhs(b)
+        s.build()
+        s.pcg(1000, 1e-8)
+        tb = _median_time(s.build, reps)
+        tp = _median_time(lambda: s.pcg(1000, 1e-8), reps)
+        st, cnt = s.pcg(1000, 1e-8)
+        n = A.shape[0]
+        fused = _fused_kernel_roof(s, "mr", n, A.nnz, 8, achievable)
+        nnz_m = s.info()["nnz_s"]
+        s.close()
+        pcg_roof = {"kernel": "PCG iteration (two kernels; solve time / iterations)",
+                    "avg_iteration_us": round(tp / max(cnt - 1, 1) * 1e6, 3)}
+        pcg_roof.update(roof(pcg_iteration_bytes(n, A.nnz, nnz_m, 8), tp / max(cnt - 1, 1), achievable))
+        line = {"config": "poisson2d 32^2, MR fixed pattern 20 its + PCG 1e-8", "build_s": tb, "pcg_s": tp,
+                "pcg_iterations": cnt - 1, "status": st, "roofline_build_pass": fused, "roofline_pcg": pcg_roof,
+                "note": "n = 1,024: every kernel is launch-bound, far from any bandwidth bound"}
+        if with_cpu:
+            from oracle import oracle
+            t = time.perf_counter()
+            r = oracle.run("mr", A, M0, None, 20, 0.0, 1.0, False, threads, fixed_pattern=True, omp_threads=threads)
+            oracle.pcg(A.tocsr(), r["M"].tocsr(), b, 1000, 1e-8, threads=threads)
+            line["cpu_baseline"] = {"value": time.perf_counter() - t, "unit": "s", "cores": threads, "kind": "port",
+                                    "sample": "full workload"}
+        print(json.dumps(line), flush=True)
 
-    # config 3: A = B^T B + sigma I, n = 100,000, CG with per-column top-16 dropping, 25 iterations
-    A, _, _ = problems.random_gram(100_000, seed=3, power_iterations=50)
-    Pr = problems.diag_inverse(A)
-    r = _lib.run("cg", A, None, Pr, 25, 0.0, 0.5, True, 256, column_top_k=16)
-    dev = []
-    for _ in range(3):
+    if 3 in want:
+        # config 3: A = B^T B + sigma I, n = 100,000, CG with per-column top-16 dropping, 25 iterations
+        A, _, _ = problems.random_gram(100_000, seed=3, power_iterations=50)
+        Pr = problems.diag_inverse(A)
         r = _lib.run("cg", A, None, Pr, 25, 0.0, 0.5, True, 256, column_top_k=16)
-        dev.append(r["device_seconds"])
-    line = {"config": "random gram n=100000 (nnz %d), CG top-16 dropping 25 its" % A.nnz,
-            "build_s_including_transfers": float(np.median(dev)), "nnz_M": int(r["M"].nnz)}
-    if with_cpu:
-        from oracle import oracle
-        t = time.perf_counter()
-        oracle.run("cg", A, None, Pr, 2, 0.0, 0.5, True, 256, column_top_k=16, omp_threads=threads)
-        t2 = time.perf_counter() - t
-        line["cpu_baseline"] = {"value": t2 / 2 * 25, "unit": "s", "cores": threads, "kind": "port",
-                                "sample": "2 of 25 iterations (%.2f s), scaled to 25" % t2}
-    out.append(line)
+        dev = []
+        for _ in range(3):
+            r = _lib.run("cg", A, None, Pr, 25, 0.0, 0.5, True, 256, column_top_k=16)
+            dev.append(r["device_seconds"])
+        # the dynamic build's dominant kernel: the two-phase SpGEMM, here A M
+        # with the final M (FP64; 3 (n+1) 4 + 12 (nnz(A) + nnz(M) + nnz(C)))
+        n = A.shape[0]
+        C, sg_s = _lib.spgemm(A, r["M"], repeats=10)
+        sg = {"kernel": "SpGemm C = A M (final M; symbolic + numeric phases), FP64", "seconds": round(sg_s, 7),
+              "nnz_C": int(C.nnz)}
+        sg.update(roof(3 * (n + 1) * 4 + 12 * (A.nnz + r["M"].nnz + C.nnz), sg_s, achievable))
+        line = {"config": "random gram n=100000 (nnz %d), CG top-16 dropping 25 its" % A.nnz,
+                "build_s_including_transfers": float(np.median(dev)), "nnz_M": int(r["M"].nnz), "spgemm": sg}
+        if with_cpu:
+            from oracle import oracle
+            t = time.perf_counter()
+            oracle.run("cg", A, None, Pr, 2, 0.0, 0.5, True, 256, column_top_k=16, omp_threads=threads)
+            t2 = time.perf_counter() - t
+            line["cpu_baseline"] = {"value": t2 / 2 * 25, "unit": "s", "cores": threads, "kind": "port",
+                                    "sample": "2 of 25 iterations (%.2f s), scaled to 25" % t2}
+        print(json.dumps(line), flush=True)
 
-    # config 4: 1024^2 anisotropic, inner-outer LOMR 5 x 20 FP32 bulk, 16 right-hand sides
-    A = problems.anisotropic2d(1024)
-    M0 = problems.diag_inverse(A)
-    s = _lib.Session("io_lomr", A, M0=M0, max_iterations=100, precision="fp32", inner_iterations=5)
-    s.build()
-    tb = _median_time(s.build, 3)
-    counts = []
-    t = time.perf_counter()
-    for seed in range(16):
-        bb = A @ np.random.default_rng(seed).standard_normal(A.shape[0])
-        s.set_rhs(bb)
-        st, cnt = s.pcg(50000, 1e-8)
-        counts.append(cnt - 1)
-    tp = time.perf_counter() - t
-    s.close()
-    line = {"config": "anisotropic2d 1024^2, inner-outer LOMR 5x20 FP32 bulk + 16 RHS PCG 1e-8", "build_s": tb,
-            "pcg_16_rhs_s": tp, "pcg_iterations": counts}
-    if with_cpu:
-        from oracle import oracle
+    if 4 in want:
+        # config 4: 1024^2 anisotropic, inner-outer LOMR 5 x 20 FP32 bulk, 16 right-hand sides
+        A = problems.anisotropic2d(1024)
+        M0 = problems.diag_inverse(A)
+        s = _lib.Session("io_lomr", A, M0=M0, max_iterations=100, precision="fp32", inner_iterations=5)
+        s.build()
+        tb = _median_time(s.build, 3)
+        counts = []
         t = time.perf_counter()
-        oracle.run("io_lomr", A, M0, None, 5, 0.0, 1.0, False, threads, inner_iterations=5, fixed_pattern=True,
-                   omp_threads=threads)
-        t5 = time.perf_counter() - t
-        line["cpu_baseline"] = {"value": t5 * 20, "unit": "s", "cores": threads, "kind": "port",
-                                "sample": "one outer iteration (5 inner, FP64, %.2f s) scaled to 20; solves not "
-                                          "included" % t5}
-    out.append(line)
+        for seed in range(16):
+            bb = A @ np.random.default_rng(seed).standard_normal(A.shape[0])
+            s.set_rhs(bb)
+            st, cnt = s.pcg(50000, 1e-8)
+            counts.append(cnt - 1)
+        tp = time.perf_counter() - t
+        n = A.shape[0]
+        nnz_m = s.info()["nnz_s"]
+        spmv_us = s.spmv_a_us()
+        s.close()
+        per_it = tp / sum(counts)
+        pcg_roof = {"kernel": "PCG iteration (two kernels, FP32 M; 16 solves' time / their iterations, "
+                              "host set-up of each right-hand side included)",
+                    "avg_iteration_us": round(per_it * 1e6, 3)}
+        pcg_roof.update(roof(pcg_iteration_bytes(n, A.nnz, nnz_m, 4), per_it, achievable))
+        spmv_roof = {"kernel": "SpMV with A + <p, Ap> (sliced ELLPACK, FP64), standalone replay",
+                     "avg_launch_us": round(spmv_us, 3)}
+        spmv_roof.update(roof((n + 1) * 4 + 12 * A.nnz + 16 * n, spmv_us * 1e-6, achievable))
+        line = {"config": "anisotropic2d 1024^2, inner-outer LOMR 5x20 FP32 bulk + 16 RHS PCG 1e-8", "build_s": tb,
+                "pcg_16_rhs_s": tp, "pcg_iterations": counts, "roofline_pcg": pcg_roof, "roofline_spmv_a": spmv_roof}
+        if with_cpu:
+            from oracle import oracle
+            t = time.perf_counter()
+            oracle.run("io_lomr", A, M0, None, 5, 0.0, 1.0, False, threads, inner_iterations=5, fixed_pattern=True,
+                       omp_threads=threads)
+            t5 = time.perf_counter() - t
+            line["cpu_baseline"] = {"value": t5 * 20, "unit": "s", "cores": threads, "kind": "port",
+                                    "sample": "one outer iteration (5 inner, FP64, %.2f s) scaled to 20; solves not "
+                                              "included" % t5}
+        print(json.dumps(line), flush=True)
 
-    # config 5: 27-point 128^3, MR FP32 bulk 10 iterations; loss and gradients in FP32 per 65,536 columns
-    import torch
+    if 5 in want:
+        # config 5: 27-point 128^3, MR FP32 bulk 10 iterations; loss and gradients in FP32 per 65,536 columns
+        import torch
 
-    A = problems.stencil27(128, seed=4)
-    M0 = problems.diag_inverse(A)
-    s = _lib.Session("mr", A, M0=M0, max_iterations=10, precision="fp32")
-    s.build()
-    tb = _median_time(s.build, 3)
-    M = s.result()["M"].tocsc()
-    s.close()
-    M.sort_indices()
-    plan = ResidualLoss(A, M, torch.float32, block_cols=65536)
-    m = torch.tensor(M.data, dtype=torch.float32, device="cuda")
-    a = torch.tensor(A.data, dtype=torch.float32, device="cuda")
-    plan.forward_value(m, a)
-    tf = _median_time(lambda: plan.forward_value(m, a), 3)
-    tg = _median_time(lambda: plan.gradients(m, a), 3)
-    info = plan.info()
-    plan.close()
-    line = {"config": "stencil27 128^3 (n=2097152, nnz %d), MR FP32 bulk 10 its; loss fwd/bwd FP32, 65536-column "
-                      "blocks" % A.nnz, "build_s": tb, "loss_forward_s": tf, "loss_backward_s": tg,
-            "nnz_R": info["nnz_r"], "blocks": info["blocks"]}
-    if with_cpu:
-        from oracle import oracle
-        A64 = problems.stencil27(64, seed=4)
-        M064 = problems.diag_inverse(A64)
-        t = time.perf_counter()
-        oracle.run("mr", A64, M064, None, 1, 0.0, 1.0, False, threads, fixed_pattern=True, omp_threads=threads)
-        t1 = time.perf_counter() - t
-        line["cpu_baseline"] = {"value": t1 * 8 * 10, "unit": "s", "cores": threads, "kind": "port",
-                                "sample": "FP64 MR, 1 iteration on 64^3 (%.2f s), scaled by 8 (size) and 10 "
-                                          "(iterations)" % t1}
-    out.append(line)
-    for line in out:
+        A = problems.stencil27(128, seed=4)
+        M0 = problems.diag_inverse(A)
+        s = _lib.Session("mr", A, M0=M0, max_iterations=10, precision="fp32")
+        s.build()
+        tb = _median_time(s.build, 3)
+        n = A.shape[0]
+        fused = _fused_kernel_roof(s, "mr", n, A.nnz, 4, achievable)
+        M = s.result()["M"].tocsc()
+        s.close()
+        M.sort_indices()
+        plan = ResidualLoss(A, M, torch.float32, block_cols=65536)
+        m = torch.tensor(M.data, dtype=torch.float32, device="cuda")
+        a = torch.tensor(A.data, dtype=torch.float32, device="cuda")
+        plan.forward_value(m, a)
+        tf = _median_time(lambda: plan.forward_value(m, a), 3)
+        tg = _median_time(lambda: plan.gradients(m, a), 3)
+        info = plan.info()
+        plan.close()
+        # loss algorithmic bytes (FP32 values, int32 indices): forward reads A
+        # and M (patterns and values) and the pattern of R (S2); backward also
+        # writes and reads back each R block and writes df/dM and df/dA
+        nnz_r, nnz_m = info["nnz_r"], info["nnz_m"]
+        fwd = (n + 1) * 4 + A.nnz * 8 + (n + 1) * 4 + nnz_m * 8 + (n + 1) * 4 + nnz_r * 4
+        bwd = fwd + 2 * nnz_r * 4 + nnz_m * 4 + A.nnz * 4
+        lf = {"kernel": "loss forward (LossColumns per 65,536-column block)"}
+        lf.update(roof(fwd, tf, achievable))
+        lb = {"kernel": "loss backward (LossColumns with df/dM, then LossGradA per block)"}
+        lb.update(roof(bwd, tg, achievable))
+        line = {"config": "stencil27 128^3 (n=2097152, nnz %d), MR FP32 bulk 10 its; loss fwd/bwd FP32, 65536-column "
+                          "blocks" % A.nnz, "build_s": tb, "loss_forward_s": tf, "loss_backward_s": tg,
+                "nnz_R": nnz_r, "blocks": info["blocks"], "roofline_build_pass": fused, "roofline_loss_forward": lf,
+                "roofline_loss_backward": lb}
+        if with_cpu:
+            from oracle import oracle
+            A64 = problems.stencil27(64, seed=4)
+            M064 = problems.diag_inverse(A64)
+            t = time.perf_counter()
+            oracle.run("mr", A64, M064, None, 1, 0.0, 1.0, False, threads, fixed_pattern=True, omp_threads=threads)
+            t1 = time.perf_counter() - t
+            line["cpu_baseline"] = {"value": t1 * 8 * 10, "unit": "s", "cores": threads, "kind": "port",
+                                    "sample": "FP64 MR, 1 iteration on 64^3 (%.2f s), scaled by 8 (size) and 10 "
+                                              "(iterations)" % t1}
         print(json.dumps(line), flush=True)
 
 
@@ -326,9 +419,12 @@ def main():
                     help="only run 5 launches of an FP64 y = 1.0 * x kernel over MIB MiB (counter calibration)")
     ap.add_argument("--configs", action="store_true",
                     help="instead of the headline line, one line per other configuration (1, 3, 4, 5)")
+    ap.add_argument("--only", type=str, default="",
+                    help="with --configs: comma-separated configuration numbers to run (default all)")
     args = ap.parse_args()
     if args.configs:
-        run_configs(not args.no_cpu_baseline)
+        only = [int(x) for x in args.only.split(",") if x.strip()]
+        run_configs(not args.no_cpu_baseline, only=only)
         return
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
@@ -353,15 +449,10 @@ def main():
     dM = float(np.linalg.norm((fp32["M"] - fp64["M"]).data) / np.linalg.norm(fp64["M"].data))
 
     # roofline of the dominant kernel of the FP32-bulk step: the fused LOMR
-    # pass (one iteration).  Algorithmic bytes per launch, int32 indices and
-    # pointers, s_v = 4: A once ((n+1)*4 + nnz(A)*(4+4)), the patterns of S
-    # and S2 once ((n+1)*4 + nnz*4 each), R read and written (2*nnz(S2)*4),
-    # M, Z, P read and written (6*nnz(S)*4).
+    # pass (one iteration), algorithmic bytes of fused_pass_bytes with s_v = 4
     nnz_s = nnz  # S = pattern(A) | pattern(M0) | I = pattern(A) here
     nnz_s2 = s2_nnz(A)
-    sv = 4
-    pass_bytes = ((n + 1) * 4 + nnz * (4 + sv) + (n + 1) * 4 + nnz_s * 4 + (n + 1) * 4 + nnz_s2 * 4
-                  + 2 * nnz_s2 * sv + 6 * nnz_s * sv)
+    pass_bytes = fused_pass_bytes(n, nnz, nnz_s, nnz_s2, 4, "lomr")
     us = fp32["lomr_pass_us"]
     achieved = pass_bytes / (us * 1e-6) / 1e9
     traffic = None

@@ -191,8 +191,10 @@ int64_t spai_session_set_kernel_timing(SpaiSession* session, int64_t on);
 double spai_session_build_kernel_us(SpaiSession* session, int64_t* launches);
 /* Launch shape of the session's fixed-pattern build, for tests and
  * measurement: out[0] = 1 when LOMR/MR run the fused per-iteration pass, then
- * lanes per column, columns per tile, tiles, resident-grid blocks and
- * wavefronts per block (len entries written, at most 6).  Returns 0, or -1. */
+ * lanes per column, columns per tile, tiles, resident-grid blocks,
+ * wavefronts per block, nnz of the iterate pattern S and nnz of the product
+ * pattern S2 = pattern(A S) | S | I (len entries written, at most 8).
+ * Returns 0, or -1. */
 int64_t spai_session_info(SpaiSession* session, int64_t* out, int64_t len);
 
 /* ------------------------------------------------------------------------

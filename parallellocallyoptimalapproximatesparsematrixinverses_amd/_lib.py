@@ -364,11 +364,13 @@ class Session:
 
     def info(self) -> dict:
         """Launch shape of the build: fused pass used, lanes per column,
-        columns per tile, tiles, resident-grid blocks, wavefronts per block."""
-        out = (ctypes.c_int64 * 6)()
-        if load().spai_session_info(self._h, out, 6) < 0:
+        columns per tile, tiles, resident-grid blocks, wavefronts per block;
+        nnz of the patterns S (M's) and S2 (pattern(A S) | S | I)."""
+        out = (ctypes.c_int64 * 8)()
+        if load().spai_session_info(self._h, out, 8) < 0:
             raise RuntimeError(last_error())
-        keys = ("fused", "lanes_per_column", "columns_per_tile", "tiles", "grid_blocks", "waves_per_block")
+        keys = ("fused", "lanes_per_column", "columns_per_tile", "tiles", "grid_blocks", "waves_per_block", "nnz_s",
+                "nnz_s2")
         return {k: int(v) for k, v in zip(keys, out)}
 
     def close(self):

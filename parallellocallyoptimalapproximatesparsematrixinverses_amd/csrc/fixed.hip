@@ -1505,14 +1505,19 @@ __global__ void PatternFacts(int n, const int* __restrict__ ap, const int* __res
 }
 
 // Values of X sampled on the pattern S (0 where X has no entry).
+// kSampleLanes lanes per column: a column's entries go to consecutive lanes,
+// so a wavefront's stores cover contiguous ranges of out.
+constexpr int kSampleLanes = 8;
 template <class V>
 __global__ void SampleOnPattern(int ncols, const int* __restrict__ sp, const int* __restrict__ si,
                                 const int* __restrict__ xp, const int* __restrict__ xi, const double* __restrict__ xv,
                                 V* __restrict__ out) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int j = int(t / kSampleLanes), gl = int(t % kSampleLanes);
   if (j >= ncols) return;
-  for (int p = sp[j]; p < sp[j + 1]; ++p) {
-    const int q = FindSorted(xi, xp[j], xp[j + 1], si[p]);
+  const int x0 = xp[j], x1 = xp[j + 1];
+  for (int p = sp[j] + gl; p < sp[j + 1]; p += kSampleLanes) {
+    const int q = FindSorted(xi, x0, x1, si[p]);
     out[p] = q >= 0 ? V(xv[q]) : V(0);
   }
 }
@@ -1801,8 +1806,8 @@ class FixedImpl final : public FixedProblem {
     hipStream_t st = c.stream;
     const bool lomr = method_ == SPAI_INNER_OUTER_LOMR;
     const int* map = s2_to_s_.get();
-    SampleOnPattern<V><<<Blocks1(n_), kBlk, 0, st>>>((int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), m0_.ptr(),
-                                                     m0_.idx(), m0_.v(), M_.get());
+    SampleOnPattern<V><<<Blocks1(n_ * kSampleLanes), kBlk, 0, st>>>(
+        (int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), m0_.ptr(), m0_.idx(), m0_.v(), M_.get());
     SPAI_HIP_CHECK(hipMemsetAsync(flags_.get(), 0, sizeof(Flags), st));
     const double tol = prm_.tolerance;
     const double thr = std::max(10.0 * tol, 64.0 * 2.220446049250313e-16 * std::sqrt((double)n_));
@@ -1883,7 +1888,7 @@ class FixedImpl final : public FixedProblem {
     return timed_launches_ > 0 ? timed_us_ : -1.0;
   }
 
-  void FusedInfo(int64_t out[6]) const override {
+  void FusedInfo(int64_t out[8]) const override {
     const bool fused = (method_ == SPAI_GLOBAL_LOMR || method_ == SPAI_GLOBAL_MR) && use_plan_ && fused_ok_;
     const int64_t w = kWave / gl_;
     out[0] = fused ? 1 : 0;
@@ -1892,6 +1897,8 @@ class FixedImpl final : public FixedProblem {
     out[3] = (n_ + w - 1) / w;
     out[4] = gridf_;
     out[5] = kBlk / kWave;
+    out[6] = nnz_s_;
+    out[7] = nnz_s2_;
   }
 
   // Global LOMR or MR with one fused pass per iteration (FusedPass).
@@ -1912,8 +1919,8 @@ class FixedImpl final : public FixedProblem {
       return FinishFusedBuild();
     }
     if (record) SPAI_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    SampleOnPattern<V><<<Blocks1(n_), kBlk, 0, st>>>((int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), m0_.ptr(),
-                                                     m0_.idx(), m0_.v(), M_.get());
+    SampleOnPattern<V><<<Blocks1(n_ * kSampleLanes), kBlk, 0, st>>>(
+        (int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), m0_.ptr(), m0_.idx(), m0_.v(), M_.get());
     ResetFlags<<<1, 1, 0, st>>>(flags_.get());
     double* part = partf_.get();
     const double tol = prm_.tolerance;
@@ -2049,8 +2056,8 @@ class FixedImpl final : public FixedProblem {
     Ctx& c = ctx_;
     hipStream_t st = c.stream;
     // M = M0 on S, P = 0
-    SampleOnPattern<V><<<Blocks1(n_), kBlk, 0, st>>>((int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), m0_.ptr(),
-                                                     m0_.idx(), m0_.v(), M_.get());
+    SampleOnPattern<V><<<Blocks1(n_ * kSampleLanes), kBlk, 0, st>>>(
+        (int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), m0_.ptr(), m0_.idx(), m0_.v(), M_.get());
     FillZero<V><<<Blocks1(nnz_s_), kBlk, 0, st>>>(nnz_s_, P_.get());
     SPAI_HIP_CHECK(hipMemsetAsync(flags_.get(), 0, sizeof(Flags), st));
     const int* s2r = s2_pat_->idx.get();

@@ -42,7 +42,7 @@ class FixedProblem {
   // Launch shape of the fused pass: out[0] = 1 when the build uses it, then
   // lanes per column (GL), columns per tile (64 / GL), tiles, blocks of the
   // resident grid and wavefronts per block.
-  virtual void FusedInfo(int64_t out[6]) const = 0;
+  virtual void FusedInfo(int64_t out[8]) const = 0;
 };
 
 std::unique_ptr<FixedProblem> MakeFixedProblem(int64_t method, const MatrixData& A, const MatrixData* M0,
