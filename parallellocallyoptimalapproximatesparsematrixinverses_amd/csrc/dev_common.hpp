@@ -120,11 +120,10 @@ __device__ __forceinline__ int FindSorted(const I* a, int lo, int hi, I key) {
   return (p < hi && a[p] == key) ? p : -1;
 }
 
-// Intersection dot of two sorted sparse columns, summed in row order
+// Intersection dot of two sorted sparse columns, added to s in row order
 // (sequential, as Eigen's cwiseProduct().sum()).
 __device__ __forceinline__ double ColumnDot(const int* xi, const double* xv, int xb, int xe, const int* yi,
-                                            const double* yv, int yb, int ye) {
-  double s = 0.0;
+                                            const double* yv, int yb, int ye, double s = 0.0) {
   while (xb < xe && yb < ye) {
     const int a = xi[xb], b = yi[yb];
     if (a == b) {

@@ -148,6 +148,10 @@ __device__ __forceinline__ bool IsShortColumn(int64_t upper, int ny) {
   return upper <= kTmCap && ny <= kTmLists;
 }
 
+// A column's rows fit the LDS bitmap of the symbolic tier (one bit per row
+// of its range, kEscCap words).
+__device__ __forceinline__ bool BitmapRange(int lo_r, int hi_r) { return hi_r >= lo_r && hi_r - lo_r < 32 * kEscCap; }
+
 __device__ __forceinline__ int WaveMinInt(int v) {
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, kWave));
@@ -193,7 +197,7 @@ __device__ void BlockAppend(bool flag, int value, int* __restrict__ list, int* _
 // Sizes of the column lists of one product, the largest Y column among the
 // short columns, and the summed expansion of the columns in the LDS tiers.
 struct TierCounts {
-  int n[4];  // short, long (LDS tiers and stamp), big (stamp), largest short-column |Y(:, j)|
+  int n[4];  // short, long (LDS tiers and stamp), big (expansion over kEscCap), largest short-column |Y(:, j)|
   unsigned long long mid_expansion;
 };
 
@@ -472,7 +476,10 @@ __global__ __launch_bounds__(kSymWaves* kWave) void SymbolicEsc(
   const int g = blockIdx.x * kSymWaves + w;
   if (g >= nlist) return;
   const int j = list[g];
-  if (upper[j] > kEscCap) return;  // the stamp kernel's column
+  // an expansion over kEscCap is not stashed (the fill pass writes its rows)
+  const bool wide = upper[j] > kEscCap;
+  if (fill && stash != nullptr && !wide) return;  // its rows are in the stash already
+  if (!fill && wide) stash = nullptr;
   int* K = keys[w];
   const int y0 = yptr[j], ye = yptr[j + 1];
   // the first 64 Y entries (most columns have no more) stay in registers:
@@ -499,7 +506,8 @@ __global__ __launch_bounds__(kSymWaves* kWave) void SymbolicEsc(
   }
   lo_r = WaveMinInt(lo_r);
   hi_r = WaveMaxInt(hi_r);
-  const bool bitmap = hi_r >= lo_r && hi_r - lo_r < 32 * kEscCap;
+  const bool bitmap = BitmapRange(lo_r, hi_r);
+  if (wide && !bitmap) return;  // the stamp kernel's column
   const int nwords = bitmap ? ((hi_r - lo_r) >> 5) + 1 : 0;
   if (bitmap) {
     for (int i = lane; i < (nwords + 3) / 4; i += kWave) reinterpret_cast<int4*>(K)[i] = make_int4(0, 0, 0, 0);
@@ -640,6 +648,34 @@ __global__ __launch_bounds__(kSymWaves* kWave) void SymbolicEsc(
   }
 }
 
+// The columns whose expansion exceeds kEscCap, one wavefront each: those
+// whose row range fits the LDS bitmap stay with SymbolicEsc; the others are
+// appended to stamp_list (count in *nstamp) for the stamp kernel.
+__global__ __launch_bounds__(kSymWaves* kWave) void BigColumnRanges(int nbig, const int* __restrict__ big_list,
+                                                                    const int* __restrict__ xptr,
+                                                                    const int* __restrict__ xidx,
+                                                                    const int* __restrict__ yptr,
+                                                                    const int* __restrict__ yidx, int add_identity,
+                                                                    int* __restrict__ stamp_list,
+                                                                    int* __restrict__ nstamp) {
+  const int lane = LaneId();
+  const int g = blockIdx.x * kSymWaves + WaveInBlock();
+  if (g >= nbig) return;
+  const int j = big_list[g];
+  int lo_r = add_identity ? j : INT_MAX, hi_r = add_identity ? j : -1;
+  for (int p = yptr[j] + lane; p < yptr[j + 1]; p += kWave) {
+    const int k = yidx[p];
+    const int b = xptr[k], e = xptr[k + 1];
+    if (e > b) {
+      lo_r = min(lo_r, xidx[b]);
+      hi_r = max(hi_r, xidx[e - 1]);
+    }
+  }
+  lo_r = WaveMinInt(lo_r);
+  hi_r = WaveMaxInt(hi_r);
+  if (lane == 0 && !BitmapRange(lo_r, hi_r)) stamp_list[atomicAdd(nstamp, 1)] = j;
+}
+
 // Symbolic phase for columns whose expansion exceeds the LDS tier: a block per
 // column marks rows in a per-block stamp array (length nrows, initialised to
 // -1) and compacts them in row order.
@@ -685,30 +721,40 @@ __global__ __launch_bounds__(kBlock) void SymbolicStamp(int nlist, const int* __
 // grouped by Y entry in ascending order, so the loads and searches of a chunk
 // overlap and only the additions are ordered.  Longer columns accumulate in
 // global memory, one Y entry at a time.
+//
+// Two instances cover the long columns: kCap = kNumCap with kNumWaves
+// wavefronts per block, and kCap = kNumCapWide with one (48 KiB of LDS) for
+// the columns of kNumCap < L <= kNumCapWide rows.  Each is launched over the
+// whole list and skips the columns of the other (L <= min_len: the smaller
+// instance's; L > kCap: the global path, run by the instance with
+// global_long).
 constexpr int kNumRegs = 4;
-__global__ __launch_bounds__(kNumWaves* kWave) void NumericGustavson(
+constexpr int kNumCapWide = 4096;
+template <int kCap, int kWaves>
+__global__ __launch_bounds__(kWaves* kWave) void NumericGustavson(
     int nlist, const int* __restrict__ list, const int* __restrict__ xptr, const int* __restrict__ xidx,
     const double* __restrict__ xval, const int* __restrict__ yptr, const int* __restrict__ yidx,
     const double* __restrict__ yval, const int* __restrict__ cptr, int* __restrict__ cidx,
     double* __restrict__ cval, int residual, int* __restrict__ error_flag, const int64_t* __restrict__ upper,
-    const int* __restrict__ soff, const int* __restrict__ stash) {
-  __shared__ int rows_s[kNumWaves][kNumCap];
-  __shared__ double acc_s[kNumWaves][kNumCap];
-  __shared__ int pre_s[kNumWaves][kWave];
-  __shared__ int beg_s[kNumWaves][kWave];
-  __shared__ double y_s[kNumWaves][kWave];
+    const int* __restrict__ soff, const int* __restrict__ stash, int min_len, int global_long) {
+  __shared__ int rows_s[kWaves][kCap];
+  __shared__ double acc_s[kWaves][kCap];
+  __shared__ int pre_s[kWaves][kWave];
+  __shared__ int beg_s[kWaves][kWave];
+  __shared__ double y_s[kWaves][kWave];
   const int w = WaveInBlock(), lane = LaneId();
-  const int g = blockIdx.x * kNumWaves + w;
+  const int g = blockIdx.x * kWaves + w;
   if (g >= nlist) return;
   const int j = list[g];
   const int cb = cptr[j];
   const int L = cptr[j + 1] - cb;
+  if (L <= min_len || (L > kCap && !global_long)) return;  // the other instance's column
   const int y0 = yptr[j], ye = yptr[j + 1];
   // rows of an LDS-tier column whose count pass stashed them (the fill pass
   // skipped it): read from the stash and placed in C here
   const bool stashed = stash != nullptr && upper[j] <= kEscCap;
   const int* const rows = stashed ? stash + soff[g] : cidx + cb;
-  if (L > kNumCap) {
+  if (L > kCap) {
     for (int e = lane; e < L; e += kWave) cval[cb + e] = 0.0;
     WaveFence();
     const int* R = rows;
@@ -1089,34 +1135,14 @@ __device__ __forceinline__ double OrderedSumLds(const double* P, int n, double s
   for (; e < n; ++e) s += P[e];
   return s;
 }
-__global__ __launch_bounds__(kDotWaves* kWave) void ColumnDotWave(int ncols, const int* __restrict__ xp,
-                                                                  const int* __restrict__ xi,
-                                                                  const double* __restrict__ xv,
-                                                                  const int* __restrict__ yp,
-                                                                  const int* __restrict__ yi,
-                                                                  const double* __restrict__ yv, int same,
-                                                                  double* __restrict__ out) {
-  __shared__ __align__(16) double prod[kDotWaves][kDotCap];
-  const int lane = LaneId(), w = WaveInBlock();
-  const int j = blockIdx.x * kDotWaves + w;
-  if (j >= ncols) return;
+// The matched products of column j of X .* Y, in the column's row order,
+// into P (kDotCap doubles of the calling wavefront's LDS); returns their
+// count, or -1 when there are more than kDotCap (nothing usable in P).
+__device__ __forceinline__ int ColumnProducts(int j, int lane, const int* __restrict__ xp, const int* __restrict__ xi,
+                                              const double* __restrict__ xv, const int* __restrict__ yp,
+                                              const int* __restrict__ yi, const double* __restrict__ yv,
+                                              double* P) {
   const int xb = xp[j], L = xp[j + 1] - xb;
-  double* const P = prod[w];
-  if (same) {
-    double s = 0.0;
-    for (int e0 = 0; e0 < L; e0 += kDotCap) {
-      const int m = min(kDotCap, L - e0);
-      for (int e = lane; e < m; e += kWave) {
-        const double v = xv[xb + e0 + e];
-        P[e] = v * v;
-      }
-      WaveFence();
-      if (lane == 0) s = OrderedSumLds(P, m, s);
-      WaveFence();
-    }
-    if (lane == 0) out[j] = s;
-    return;
-  }
   const int yb = yp[j], R = yp[j + 1] - yb;
   const int T = L + R;
   if (T <= kDotCap) {
@@ -1163,8 +1189,7 @@ __global__ __launch_bounds__(kDotWaves* kWave) void ColumnDotWave(int ncols, con
     for (int t = 0; t < kRecs; ++t)
       if (rc[t] >= 0) P[lane + kWave * t] = xv[xb + (rc[t] >> 10)] * yv[yb + (rc[t] & 1023)];
     WaveFence();
-    if (lane == 0) out[j] = OrderedSumLds(P, total, 0.0);
-    return;
+    return total;
   }
   const int* const lx = xi + xb;
   const int* const ry = yi + yb;
@@ -1191,13 +1216,101 @@ __global__ __launch_bounds__(kDotWaves* kWave) void ColumnDotWave(int ncols, con
   const int k = walk(false, 0);
   const int incl = WaveInclusiveScan(k);
   const int total = __shfl(incl, kWave - 1, kWave);
-  if (total > kDotCap) {
-    if (lane == 0) out[j] = ColumnDot(xi, xv, xb, xb + L, yi, yv, yb, yb + R);
-    return;
-  }
+  if (total > kDotCap) return -1;
   walk(true, incl - k);
   WaveFence();
-  if (lane == 0) out[j] = OrderedSumLds(P, total, 0.0);
+  return total;
+}
+
+__global__ __launch_bounds__(kDotWaves* kWave) void ColumnDotWave(int ncols, const int* __restrict__ xp,
+                                                                  const int* __restrict__ xi,
+                                                                  const double* __restrict__ xv,
+                                                                  const int* __restrict__ yp,
+                                                                  const int* __restrict__ yi,
+                                                                  const double* __restrict__ yv, int same,
+                                                                  double* __restrict__ out) {
+  __shared__ __align__(16) double prod[kDotWaves][kDotCap];
+  const int lane = LaneId(), w = WaveInBlock();
+  const int j = blockIdx.x * kDotWaves + w;
+  if (j >= ncols) return;
+  double* const P = prod[w];
+  if (same) {
+    const int xb = xp[j], L = xp[j + 1] - xb;
+    double s = 0.0;
+    for (int e0 = 0; e0 < L; e0 += kDotCap) {
+      const int m = min(kDotCap, L - e0);
+      for (int e = lane; e < m; e += kWave) {
+        const double v = xv[xb + e0 + e];
+        P[e] = v * v;
+      }
+      WaveFence();
+      if (lane == 0) s = OrderedSumLds(P, m, s);
+      WaveFence();
+    }
+    if (lane == 0) out[j] = s;
+    return;
+  }
+  const int total = ColumnProducts(j, lane, xp, xi, xv, yp, yi, yv, P);
+  if (lane == 0)
+    out[j] = total >= 0 ? OrderedSumLds(P, total, 0.0)
+                        : ColumnDot(xi, xv, xp[j], xp[j + 1], yi, yv, yp[j], yp[j + 1]);
+}
+
+// Batch-ordered sums, one wavefront per column batch b (columns
+// [first[b], first[b + 1])): the batch's terms added one after another in
+// CSC order from 0.0, as the reference's sequential per-batch loop does, so
+// out[b] is bit-identical to it.  The terms are formed by all lanes and
+// staged in LDS; lane 0 adds them (the only sequential part).
+//
+// Squared norm: the batch's entries are one contiguous range, taken in
+// chunks of kDotCap values.  Each lane holds kSqPer values of a chunk in
+// registers; the next chunk is loaded before lane 0 adds the current one, so
+// its load latency hides behind the additions.
+constexpr int kSqPer = kDotCap / kWave;
+__global__ __launch_bounds__(kWave) void BatchSqWave(const int* __restrict__ first, const int* __restrict__ ptr,
+                                                     const double* __restrict__ v, double* __restrict__ out) {
+  __shared__ __align__(16) double P[kDotCap];
+  const int lane = LaneId(), b = blockIdx.x;
+  const int p0 = ptr[first[b]], p1 = ptr[first[b + 1]];
+  double r[kSqPer];
+  auto load = [&](int c0) {
+#pragma unroll
+    for (int t = 0; t < kSqPer; ++t) {
+      const int p = c0 + lane + kWave * t;
+      r[t] = p < p1 ? v[p] : 0.0;
+    }
+  };
+  double s = 0.0;
+  load(p0);
+  for (int c0 = p0; c0 < p1; c0 += kDotCap) {
+    const int m = min(kDotCap, p1 - c0);
+#pragma unroll
+    for (int t = 0; t < kSqPer; ++t) P[lane + kWave * t] = r[t] * r[t];
+    WaveFence();
+    if (c0 + kDotCap < p1) load(c0 + kDotCap);
+    if (lane == 0) s = OrderedSumLds(P, m, s);
+    WaveFence();
+  }
+  if (lane == 0) out[b] = s;
+}
+
+// Intersection dot: column by column, each column's matched products formed
+// in parallel (ColumnProducts) and added by lane 0 to the running sum; a
+// column with more than kDotCap matches is added by lane 0 sequentially.
+__global__ __launch_bounds__(kWave) void BatchDotWave(const int* __restrict__ first, const int* __restrict__ xp,
+                                                      const int* __restrict__ xi, const double* __restrict__ xv,
+                                                      const int* __restrict__ yp, const int* __restrict__ yi,
+                                                      const double* __restrict__ yv, double* __restrict__ out) {
+  __shared__ __align__(16) double P[kDotCap];
+  const int lane = LaneId(), b = blockIdx.x;
+  double s = 0.0;
+  for (int j = first[b]; j < first[b + 1]; ++j) {
+    const int total = ColumnProducts(j, lane, xp, xi, xv, yp, yi, yv, P);
+    if (lane == 0)
+      s = total >= 0 ? OrderedSumLds(P, total, s) : ColumnDot(xi, xv, xp[j], xp[j + 1], yi, yv, yp[j], yp[j + 1], s);
+    WaveFence();
+  }
+  if (lane == 0) out[b] = s;
 }
 
 // Deterministic sum of v[0..n): per-block partials, then one block.
@@ -1209,16 +1322,6 @@ __global__ __launch_bounds__(kBlock) void SumPartials(const double* __restrict__
     s += v[i];
   s = BlockSum<double>(s, red);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
-}
-
-// Sequential squared norm of each batch's column range (entries in CSC order).
-__global__ void BatchSqNormKernel(int nb, const int* __restrict__ first, const int* __restrict__ ptr,
-                                  const double* __restrict__ v, double* __restrict__ out) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
-  double s = 0.0;
-  for (int p = ptr[first[b]]; p < ptr[first[b + 1]]; ++p) s += v[p] * v[p];
-  out[b] = s;
 }
 
 // Per-batch ||X_b||^2 with one block per batch: strided per-thread sums,
@@ -1247,35 +1350,6 @@ __global__ void BatchSqNormJoin(int nbatches, const double* __restrict__ part, d
   if (b >= nbatches) return;
   double s = 0.0;
   for (int i = 0; i < kBatchSplit; ++i) s += part[b * kBatchSplit + i];
-  out[b] = s;
-}
-
-// Per-batch sequential intersection dot over the batch's columns (CSC order).
-__global__ void BatchDotKernel(int nb, const int* __restrict__ first, const int* __restrict__ xp,
-                               const int* __restrict__ xi, const double* __restrict__ xv, const int* __restrict__ yp,
-                               const int* __restrict__ yi, const double* __restrict__ yv, int same,
-                               double* __restrict__ out) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
-  double s = 0.0;
-  for (int j = first[b]; j < first[b + 1]; ++j) {
-    if (same) {
-      for (int p = xp[j]; p < xp[j + 1]; ++p) s += xv[p] * xv[p];
-    } else {
-      int x = xp[j], xe = xp[j + 1], y = yp[j], ye = yp[j + 1];
-      while (x < xe && y < ye) {
-        if (xi[x] == yi[y]) {
-          s += xv[x] * yv[y];
-          ++x;
-          ++y;
-        } else if (xi[x] < yi[y]) {
-          ++x;
-        } else {
-          ++y;
-        }
-      }
-    }
-  }
   out[b] = s;
 }
 
@@ -2115,27 +2189,42 @@ DMat SpGemm(Ctx& ctx, const DMat& X, const DMat& Y, bool prune_zero, bool residu
     mid_stash = ctx.Alloc<int>(mid_total > 0 ? size_t(mid_total) : 1);
   }
 
+  // The large-expansion columns whose rows fit the LDS bitmap stay in the
+  // wavefront tier; only the others take the stamp kernel.  Their order in
+  // stamp_list does not matter: each column's result depends on it alone.
+  int nstamp = 0;
+  DBuf<int> stamp_list;
+  if (nbig > 0) {
+    stamp_list = ctx.Alloc<int>(size_t(nbig) + 1);
+    int* const nstamp_d = stamp_list.get() + nbig;
+    SPAI_HIP_CHECK(hipMemsetAsync(nstamp_d, 0, sizeof(int), ctx.stream));
+    BigColumnRanges<<<Blocks(nbig, kSymWaves), kSymWaves * kWave, 0, ctx.stream>>>(
+        nbig, big_list, X.ptr(), X.idx(), Y.ptr(), Y.idx(), residual ? 1 : 0, stamp_list.get(), nstamp_d);
+    ctx.ReadBack(&nstamp, nstamp_d, sizeof(int));
+  }
+  const int nwide = nbig - nstamp;
   // Each long tier is launched over its own column list, so a tier with no
   // columns costs nothing.
   auto symbolic = [&](int fill, int* cnt, const int* cptr, int* cidx, DBuf<int>& stamp, unsigned stamp_blocks) {
     // with the stash, the numeric phase places the LDS-tier columns' rows
-    if (nlong > nbig && !(fill && stash_mid))
+    // and the fill pass only writes the wide (bitmap, unstashed) columns
+    if (nlong > nstamp && !(fill && stash_mid && nwide == 0))
       SymbolicEsc<<<Blocks(nlong, kSymWaves), kSymWaves * kWave, 0, ctx.stream>>>(
           nlong, long_list, X.ptr(), X.idx(), Y.ptr(), Y.idx(), upper.get(), residual ? 1 : 0, fill, cnt, cptr,
           cidx, stash_mid ? mid_off.get() : nullptr, stash_mid ? mid_stash.get() : nullptr);
-    if (nbig > 0) {
+    if (nstamp > 0) {
       SPAI_HIP_CHECK(
           hipMemsetAsync(stamp.get(), 0xFF, sizeof(int) * size_t(stamp_blocks) * size_t(nrows), ctx.stream));
-      SymbolicStamp<<<stamp_blocks, kBlock, 0, ctx.stream>>>(nbig, big_list, (int)nrows, X.ptr(), X.idx(), Y.ptr(),
+      SymbolicStamp<<<stamp_blocks, kBlock, 0, ctx.stream>>>(nstamp, stamp_list.get(), (int)nrows, X.ptr(), X.idx(), Y.ptr(),
                                                              Y.idx(), residual ? 1 : 0, fill, stamp.get(), cnt,
                                                              cptr, cidx);
     }
   };
   DBuf<int> stamp;
   unsigned stamp_blocks = 0;
-  if (nbig > 0) {
+  if (nstamp > 0) {
     const int64_t max_ws = (int64_t(1) << 28) / std::max<int64_t>(nrows, 1);  // <= 1 GiB of stamps
-    stamp_blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>({(int64_t)nbig, max_ws, 1024}));
+    stamp_blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>({(int64_t)nstamp, max_ws, 1024}));
     stamp = ctx.Alloc<int>(size_t(stamp_blocks) * size_t(nrows));
   }
   if (nlong > 0) {
@@ -2166,11 +2255,18 @@ DMat SpGemm(Ctx& ctx, const DMat& X, const DMat& Y, bool prune_zero, bool residu
       SPAI_MERGE_NUMERIC(16);
 #undef SPAI_MERGE_NUMERIC
   }
-  if (nlong > 0)
-    NumericGustavson<<<Blocks(nlong, kNumWaves), kNumWaves * kWave, 0, ctx.stream>>>(
+  if (nlong > 0) {
+    NumericGustavson<kNumCap, kNumWaves><<<Blocks(nlong, kNumWaves), kNumWaves * kWave, 0, ctx.stream>>>(
         nlong, long_list, X.ptr(), X.idx(), X.v(), Y.ptr(), Y.idx(), Y.v(), C.pat->ptr.get(), C.pat->idx.get(),
         C.val.get(), residual ? 1 : 0, ctx.err_dev, upper.get(), stash_mid ? mid_off.get() : nullptr,
-        stash_mid ? mid_stash.get() : nullptr);
+        stash_mid ? mid_stash.get() : nullptr, -1, 0);
+    // the columns of more than kNumCap rows (an expansion over kNumCap)
+    if (nbig > 0 || total > int64_t(kNumCap))
+      NumericGustavson<kNumCapWide, 1><<<(unsigned)nlong, kWave, 0, ctx.stream>>>(
+          nlong, long_list, X.ptr(), X.idx(), X.v(), Y.ptr(), Y.idx(), Y.v(), C.pat->ptr.get(), C.pat->idx.get(),
+          C.val.get(), residual ? 1 : 0, ctx.err_dev, upper.get(), stash_mid ? mid_off.get() : nullptr,
+          stash_mid ? mid_stash.get() : nullptr, kNumCap, 1);
+  }
   if (prune_zero) return PruneZeros(ctx, std::move(C));
   return C;
 }
@@ -2322,8 +2418,13 @@ double SqNorm(Ctx& ctx, const DMat& X) {
 double BatchOrderedDot(Ctx& ctx, const DMat& X, const DMat& Y, const int* batch_first, int nbatches, bool same) {
   if (X.rows() != Y.rows() || X.cols() != Y.cols()) throw std::invalid_argument("Sparse matrices must have the same shape");
   auto part = ctx.Alloc<double>(nbatches + 1);
-  BatchDotKernel<<<Blocks(nbatches, 64), 64, 0, ctx.stream>>>(nbatches, batch_first, X.ptr(), X.idx(), X.v(), Y.ptr(),
-                                                              Y.idx(), Y.v(), same ? 1 : 0, part.get());
+  if (nbatches > 0) {
+    if (same)
+      BatchSqWave<<<(unsigned)nbatches, kWave, 0, ctx.stream>>>(batch_first, X.ptr(), X.v(), part.get());
+    else
+      BatchDotWave<<<(unsigned)nbatches, kWave, 0, ctx.stream>>>(batch_first, X.ptr(), X.idx(), X.v(), Y.ptr(),
+                                                                  Y.idx(), Y.v(), part.get());
+  }
   SequentialSum<<<1, 64, 0, ctx.stream>>>(nbatches, part.get(), part.get() + nbatches);
   double s = 0.0;
   ctx.ReadBack(&s, part.get() + nbatches, sizeof(double));
@@ -2331,9 +2432,9 @@ double BatchOrderedDot(Ctx& ctx, const DMat& X, const DMat& Y, const int* batch_
 }
 
 void BatchSqNorms(Ctx& ctx, const DMat& X, const int* batch_first, int nbatches, double* out, bool ordered) {
-  if (ordered)
-    BatchSqNormKernel<<<Blocks(nbatches), kBlock, 0, ctx.stream>>>(nbatches, batch_first, X.ptr(), X.v(), out);
-  else if (nbatches > 0) {
+  if (ordered) {
+    if (nbatches > 0) BatchSqWave<<<(unsigned)nbatches, kWave, 0, ctx.stream>>>(batch_first, X.ptr(), X.v(), out);
+  } else if (nbatches > 0) {
     auto part = ctx.Alloc<double>(size_t(nbatches) * kBatchSplit);
     BatchSqNormTreeKernel<<<(unsigned)nbatches * kBatchSplit, 256, 0, ctx.stream>>>(batch_first, X.ptr(), X.v(),
                                                                                     part.get());

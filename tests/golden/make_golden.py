@@ -3,7 +3,10 @@ data files (run in a container where the reference tree is mounted).
 
   * <name>.npz: the sample matrix (scripts/sample_datasets/<name>-dataset.mtx),
     canonicalised as scripts/common.py:396-400 does (CSC, duplicates summed,
-    zeros eliminated, indices sorted).
+    zeros eliminated, indices sorted).  The larger matrices (LOWER) are stored
+    as their lower triangle with int32 indices and a "lower" flag; they are
+    exactly symmetric, and tests/golden_data.py rebuilds the full matrix as
+    L + strict_lower(L)^T, which adds no two values.
   * <name>-<mode>-d<density>-convergence.csv: the first rows of the reference's
     per-iteration ||R_i||_F table for the seven method variants
     (scripts/results/results_sample/<mode>/max_iter_*/max_density_*/<name>/*-convergence.csv).
@@ -30,18 +33,28 @@ OUT = Path(__file__).resolve().parent
 ROWS = 41
 
 DATASETS = ["tri100eigs4k", "bcsstk21", "msc04515"]
-PIPELINES = [("tri100eigs4k", "no_dropping", "1.000")]
+LOWER = ["Poisson32k", "wathen100", "4bw100eigs20k", "4bw100eigs20k2", "bundle1", "rand20k", "rand20k2"]
+PIPELINES = [("tri100eigs4k", "no_dropping", "1.000"), ("Poisson32k", "dropping", "0.030")]
 SPEEDUP = [("tri100eigs4k", "dropping", "25", "0.030")]
 
 
 def main():
-    for name in DATASETS:
+    for name in DATASETS + LOWER:
         A = mmread(REF / "sample_datasets" / f"{name}-dataset.mtx").tocsc().astype(np.float64)
         A.sum_duplicates()
         A.eliminate_zeros()
         A.sort_indices()
-        np.savez_compressed(OUT / f"{name}.npz", indptr=A.indptr.astype(np.int64),
-                            indices=A.indices.astype(np.int64), data=A.data, shape=np.array(A.shape))
+        if name in LOWER:
+            if abs(A - A.T).max() != 0.0:
+                raise ValueError(f"{name} is not exactly symmetric")
+            L = sparse.tril(A).tocsc()
+            L.sort_indices()
+            np.savez_compressed(OUT / f"{name}.npz", indptr=L.indptr.astype(np.int32),
+                                indices=L.indices.astype(np.int32), data=L.data, shape=np.array(L.shape),
+                                lower=np.array(1))
+        else:
+            np.savez_compressed(OUT / f"{name}.npz", indptr=A.indptr.astype(np.int64),
+                                indices=A.indices.astype(np.int64), data=A.data, shape=np.array(A.shape))
         for f in sorted(glob.glob(str(REF / f"results/results_sample/*/max_iter_*/max_density_*/{name}/*-convergence.csv"))):
             if "no_dropping" in f and "max_density_0.030" in f:
                 continue  # written by an earlier revision (see tests/test_oracle_golden.py)

@@ -12,8 +12,16 @@ VARIANTS = [("cg", 2), ("mr", 2), ("lomr", 2), ("io_mr", 2), ("io_lomr", 2), ("i
 
 
 def matrix(name):
+    """The canonical CSC matrix (sorted indices).  Fixtures flagged "lower" hold
+    the lower triangle of an exactly symmetric matrix; the full matrix is
+    L + strict_lower(L)^T, whose two terms share no position."""
     z = np.load(GOLDEN / f"{name}.npz", allow_pickle=False)
-    return sparse.csc_matrix((z["data"], z["indices"], z["indptr"]), shape=tuple(z["shape"]))
+    A = sparse.csc_matrix((z["data"], z["indices"].astype(np.int64), z["indptr"].astype(np.int64)),
+                          shape=tuple(z["shape"]))
+    if "lower" in z.files:
+        A = (A + sparse.tril(A, k=-1).T).tocsc()
+        A.sort_indices()
+    return A
 
 
 def convergence(name, mode, density):

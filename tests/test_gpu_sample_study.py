@@ -15,9 +15,15 @@ reductions) must give:
   that converges in one update ends at roundoff level, where only the status
   and the tolerance are compared.
 
-tri100eigs4k without dropping (max_density 1.0, 500 iterations).  The
-inner-outer variants run 500 iterations at density 0.34 (about 70 s each on
-one GPU) and are left to the per-iteration history tests.
+Two pipelines:
+
+* tri100eigs4k without dropping (max_density 1.0, 500 iterations).  The
+  inner-outer variants run 500 iterations at density 0.34 (about 70 s each on
+  one GPU) and are left to the per-iteration history tests.
+* Poisson32k with dropping (max_density 0.03, 100 iterations), the Global
+  methods: 24, 31 and 29 PCG updates.  Its inner-outer builds form products
+  M_outer R with near-dense columns at the density cap and are left to the
+  per-iteration history tests (tests/test_gpu_sample_histories.py).
 """
 import csv
 
@@ -28,28 +34,33 @@ import golden_data as gd
 
 pytestmark = pytest.mark.gpu
 
-NAME, MODE, DENSITY, ITERATIONS = "tri100eigs4k", "no_dropping", "1.000", 500
+# (matrix, mode, max_density, iterations)
+PIPELINES = [("tri100eigs4k", "no_dropping", "1.000", 500), ("Poisson32k", "dropping", "0.030", 100)]
 
 
-def _table(kind):
-    with open(gd.GOLDEN / f"{NAME}-{MODE}-d{DENSITY}-{kind}.csv") as fh:
+def _table(pipe, kind):
+    name, mode, density, _ = pipe
+    with open(gd.GOLDEN / f"{name}-{mode}-d{density}-{kind}.csv") as fh:
         return list(csv.DictReader(fh))
 
 
-def _histories():
-    with open(gd.GOLDEN / f"{NAME}-{MODE}-d{DENSITY}-backward_error.csv") as fh:
+def _histories(pipe):
+    name, mode, density, _ = pipe
+    with open(gd.GOLDEN / f"{name}-{mode}-d{density}-backward_error.csv") as fh:
         rows = list(csv.reader(fh))
     return [np.array([float(r[1 + v]) for r in rows[1:] if r[1 + v] != ""]) for v in range(len(rows[0]) - 1)]
 
 
+@pytest.mark.parametrize("pipe", PIPELINES, ids=[p[0] for p in PIPELINES])
 @pytest.mark.parametrize("variant", [0, 1, 2])
-def test_device_study_matches_reference_tables(gpu_lib, variant):
+def test_device_study_matches_reference_tables(gpu_lib, pipe, variant):
     from parallellocallyoptimalapproximatesparsematrixinverses_amd import datasets
+    name, mode, density, iterations = pipe
     label = datasets.VARIANTS[variant][0]
-    print(f"case: {NAME} {MODE} d{DENSITY} {label}", flush=True)
-    study = datasets.run_study(gd.matrix(NAME), ITERATIONS, float(DENSITY), MODE == "dropping", tolerance=1e-9,
+    print(f"case: {name} {mode} d{density} {label}", flush=True)
+    study = datasets.run_study(gd.matrix(name), iterations, float(density), mode == "dropping", tolerance=1e-9,
                                num_threads=256, variants=[datasets.VARIANTS[variant]])
-    got, ref = study.construction[0], _table("construction_status")[variant]
+    got, ref = study.construction[0], _table(pipe, "construction_status")[variant]
     assert got["method"] == ref["method"]
     assert got["iterations"] == int(ref["iterations"])
     assert got["reported_residual_norm"] == float(ref["reported_residual_norm"])
@@ -59,14 +70,14 @@ def test_device_study_matches_reference_tables(gpu_lib, variant):
     assert got["density"] == float(ref["density"])
     assert str(got["converged"]) == ref["converged"]
 
-    sgot, sref = study.solver_status[0], _table("solver_status")[variant]
+    sgot, sref = study.solver_status[0], _table(pipe, "solver_status")[variant]
     assert sgot["method"] == sref["method"]
     assert sgot["history_limit"] == int(sref["history_limit"])
     assert sgot["active_threads"] == int(sref["active_threads"])
     assert sgot["solver_updates"] == int(sref["solver_updates"])
     assert sgot["status"] == sref["status"]
     assert str(sgot["converged"]) == sref["converged"]
-    h, r = study.backward_error[0][0][1], _histories()[variant]
+    h, r = study.backward_error[0][0][1], _histories(pipe)[variant]
     assert len(h) == len(r)
     if sgot["converged"]:
         np.testing.assert_allclose(h[:-1], r[:-1], rtol=1e-7)

@@ -7,8 +7,11 @@ contributions of each row in k-ascending order.
 The cases reach each tier: short columns (expansion of at most 64 rows from
 at most 16 columns of X: one 16-lane group sorts in registers, in one fused
 pass, or in two passes past 2,097,152 such columns), the LDS bitmap (a
-column's rows span fewer than 131,072 rows), the LDS sort (wider span,
-expansion of at most 4096 rows) and the stamp array (longer expansion)."""
+column's rows span fewer than 131,072 rows, at any expansion), the LDS sort
+(wider span, expansion of at most 4096 rows) and the stamp array (wider span,
+longer expansion).  The numeric phase holds columns of up to 1024 rows in the
+LDS of a 4-wavefront block, up to 4096 in that of a 1-wavefront block, and
+accumulates longer ones in global memory."""
 import numpy as np
 import pytest
 from scipy import sparse
@@ -53,11 +56,49 @@ def test_spgemm_random_sort_tier(gpu_lib, oracle_lib):
     _check(gpu_lib, oracle_lib, X, Y)
 
 
-def test_spgemm_long_expansion_stamp_tier(gpu_lib, oracle_lib):
+def test_spgemm_long_expansion_bitmap_tier(gpu_lib, oracle_lib):
     # 16 columns of 600 rows each per output column: expansion 9,600 > 4096
+    # within 20,000 rows (the bitmap); about 7,500 rows per output column
+    # (the numeric phase's global path)
     X = _random(20_000, 64, 600, seed=7)
     Y = _random(64, 40, 16, seed=8)
-    print("case: spgemm random long expansion", flush=True)
+    print("case: spgemm random long expansion, bitmap", flush=True)
+    _check(gpu_lib, oracle_lib, X, Y)
+
+
+def test_spgemm_long_expansion_stamp_tier(gpu_lib, oracle_lib):
+    # expansion 9,600 spread over 300,000 rows: the stamp array
+    X = _random(300_000, 64, 600, seed=7)
+    Y = _random(64, 40, 16, seed=8)
+    print("case: spgemm random long expansion, stamp", flush=True)
+    _check(gpu_lib, oracle_lib, X, Y)
+
+
+@pytest.mark.parametrize("residual", [False, True])
+def test_spgemm_wide_columns_in_lds(gpu_lib, oracle_lib, residual):
+    # expansion 4,800 > 4096 within 5,000 rows: the bitmap, and about 3,100
+    # rows per output column (the one-wavefront LDS instance of the numeric
+    # phase)
+    X = _random(5000, 5000, 300, seed=21)
+    Y = _random(5000, 5000 if residual else 300, 16, seed=22)
+    print(f"case: spgemm wide columns in LDS residual={residual}", flush=True)
+    _check(gpu_lib, oracle_lib, X, Y, residual=residual)
+
+
+def test_spgemm_wide_columns_with_stashed_lds_tier(gpu_lib, oracle_lib):
+    # over 4096 LDS-tier columns (their rows stashed by the count pass) mixed
+    # with columns of expansion over 4096 in the bitmap (rows written by the
+    # fill pass)
+    rng = np.random.default_rng(23)
+    n = 4800
+    X = _random(5000, 5000, 20, seed=24)
+    per = rng.integers(17, 41, size=n)
+    per[::97] = 260  # expansion 5,200
+    r = np.concatenate([rng.choice(5000, size=k, replace=False) for k in per])
+    c = np.repeat(np.arange(n), per)
+    Y = sparse.csc_matrix((rng.standard_normal(r.size), (r, c)), shape=(5000, n))
+    Y.sort_indices()
+    print("case: spgemm wide columns with stashed LDS tier", flush=True)
     _check(gpu_lib, oracle_lib, X, Y)
 
 
