@@ -19,7 +19,7 @@ cold: the same step from nothing on the device, once per form: session
   creation (upload of A and M0, S/S2 symbolic phase, slot plan, PCG operands),
   right-hand side upload, the first build (graph recording included), the
   solve (a fresh iteration schedule) and the download of M and x; median of
-  three.  `value` stays the warm step (operands resident).
+  five.  `value` stays the warm step (operands resident).
 achievable_bandwidth: streaming read / copy / triad kernels of this library
   on 2 GiB arrays; every roofline object also states its fraction of the best
   of copy and triad.
@@ -441,7 +441,7 @@ def main():
     fp32 = time_form(_lib, A, M0, b, "fp32", args.steps, args.warmup)
     cold = {}
     for form in ("fp64", "fp32"):
-        runs = [cold_step(_lib, A, M0, b, form) for _ in range(3)]
+        runs = [cold_step(_lib, A, M0, b, form) for _ in range(5)]
         cold[form] = {"seconds": round(float(np.median([r[0] for r in runs])), 6),
                       "runs_s": [round(r[0], 6) for r in runs], "pcg_iterations": runs[-1][1]}
     bw = _lib.stream_bandwidth(2 << 30, 20)

@@ -235,14 +235,13 @@ int64_t spai_pcg(CsrData A, CsrData M, const double* b, int64_t max_iterations, 
     spai::DCsr m = spai::CsrFromHost(ctx, n, M.nnz, M.row_offsets, M.col_indices, M.values);
     if (m_fp32) spai::MakeFp32Copy(ctx, m);
     auto bd = ctx.Alloc<double>(n);
-    SPAI_HIP_CHECK(hipMemcpyAsync(bd.get(), b, sizeof(double) * n, hipMemcpyHostToDevice, ctx.stream));
+    spai::CopyToDevice(ctx, bd.get(), b, sizeof(double) * n);
     spai::PcgSolver solver(ctx, a, m, m_fp32 != 0);
     int64_t cnt = 0;
     const int status = solver.Solve(bd.get(), max_iterations, tolerance, &cnt);
     solver.History(history, cnt);
     if (x_out) {
-      SPAI_HIP_CHECK(hipMemcpyAsync(x_out, solver.X(), sizeof(double) * n, hipMemcpyDeviceToHost, ctx.stream));
-      ctx.Sync();
+      spai::CopyToHost(ctx, x_out, solver.X(), sizeof(double) * n);
     }
     *count = cnt;
     return status;
@@ -303,13 +302,12 @@ int64_t spai_pcg_solve(SpaiPcgSystem* s, const double* b, int64_t max_iterations
   try {
     if (!s) throw std::invalid_argument("null PCG system");
     if (max_iterations <= 0) throw std::invalid_argument("n, max_iterations and threads must be positive");
-    SPAI_HIP_CHECK(hipMemcpyAsync(s->b.get(), b, sizeof(double) * s->n, hipMemcpyHostToDevice, s->ctx.stream));
+    spai::CopyToDevice(s->ctx, s->b.get(), b, sizeof(double) * s->n);
     int64_t cnt = 0;
     const int status = s->solver->Solve(s->b.get(), max_iterations, tolerance, &cnt);
     if (history) s->solver->History(history, cnt);
     if (x_out) {
-      SPAI_HIP_CHECK(
-          hipMemcpyAsync(x_out, s->solver->X(), sizeof(double) * s->n, hipMemcpyDeviceToHost, s->ctx.stream));
+      spai::CopyToHost(s->ctx, x_out, s->solver->X(), sizeof(double) * s->n);
     }
     s->ctx.Sync();
     if (count) *count = cnt;
@@ -324,9 +322,9 @@ int64_t spai_pcg_solve(SpaiPcgSystem* s, const double* b, int64_t max_iterations
 int64_t spai_pcg_matvec(SpaiPcgSystem* s, const double* x, double* y) {
   try {
     if (!s) throw std::invalid_argument("null PCG system");
-    SPAI_HIP_CHECK(hipMemcpyAsync(s->x.get(), x, sizeof(double) * s->n, hipMemcpyHostToDevice, s->ctx.stream));
+    spai::CopyToDevice(s->ctx, s->x.get(), x, sizeof(double) * s->n);
     spai::SpMV(s->ctx, *s->a, s->x.get(), s->y.get());
-    SPAI_HIP_CHECK(hipMemcpyAsync(y, s->y.get(), sizeof(double) * s->n, hipMemcpyDeviceToHost, s->ctx.stream));
+    spai::CopyToHost(s->ctx, y, s->y.get(), sizeof(double) * s->n);
     s->ctx.Sync();
     return 0;
   } catch (const std::exception& ex) {
@@ -458,7 +456,7 @@ int64_t spai_session_set_rhs(SpaiSession* s, const double* b) {
     if (!s) throw std::invalid_argument("null session");
     if (!b) throw std::invalid_argument("null right-hand side");
     spai::Ctx& c = s->prob->context();
-    SPAI_HIP_CHECK(hipMemcpyAsync(s->rhs.get(), b, sizeof(double) * s->a_csr->n, hipMemcpyHostToDevice, c.stream));
+    spai::CopyToDevice(c, s->rhs.get(), b, sizeof(double) * s->a_csr->n);
     c.Sync();
     return 0;
   } catch (const std::exception& ex) {
@@ -485,8 +483,7 @@ int64_t spai_session_pcg_output(SpaiSession* s, double* history, double* x) {
     spai::Ctx& c = s->prob->context();
     if (history) s->pcg->History(history, s->last_count);
     if (x) {
-      SPAI_HIP_CHECK(
-          hipMemcpyAsync(x, s->pcg->X(), sizeof(double) * s->a_csr->n, hipMemcpyDeviceToHost, c.stream));
+      spai::CopyToHost(c, x, s->pcg->X(), sizeof(double) * s->a_csr->n);
       c.Sync();
     }
     return 0;

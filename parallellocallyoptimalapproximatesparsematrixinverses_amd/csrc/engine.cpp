@@ -6,6 +6,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <limits>
 #include <memory>
@@ -404,15 +405,24 @@ void RunInnerOuter(State& s, EngineOutput& out, bool lomr) {
   }
 }
 
+// The checks run over slices on the host worker threads; the first failing
+// check in the reference's order (offsets, then row indices) is reported.
 void ValidateCsc(const MatrixData& d) {
   if (d.rows <= 0 || d.cols <= 0 || d.nnz <= 0)
     throw std::invalid_argument("Sparse matrix dimensions and nnz must be positive");
-  for (int64_t i = 0; i < d.cols; ++i)
-    if (d.col_offsets[i] > d.col_offsets[i + 1])
-      throw std::invalid_argument("Sparse matrix column offsets must be non-decreasing");
-  for (int64_t i = 0; i < d.nnz; ++i)
-    if (d.row_indices[i] < 0 || d.row_indices[i] >= d.rows)
-      throw std::invalid_argument("Sparse matrix row idx is out of range");
+  std::atomic<bool> bad_offsets{false}, bad_rows{false};
+  HostParallelFor(d.cols, [&](int64_t b, int64_t e) {
+    bool bad = false;
+    for (int64_t i = b; i < e; ++i) bad |= d.col_offsets[i] > d.col_offsets[i + 1];
+    if (bad) bad_offsets.store(true);
+  });
+  if (bad_offsets.load()) throw std::invalid_argument("Sparse matrix column offsets must be non-decreasing");
+  HostParallelFor(d.nnz, [&](int64_t b, int64_t e) {
+    bool bad = false;
+    for (int64_t i = b; i < e; ++i) bad |= d.row_indices[i] < 0 || d.row_indices[i] >= d.rows;
+    if (bad) bad_rows.store(true);
+  });
+  if (bad_rows.load()) throw std::invalid_argument("Sparse matrix row idx is out of range");
 }
 
 }  // namespace

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 #include "device.hpp"
@@ -54,6 +55,20 @@ PatternPtr MakePattern(Ctx& ctx, int64_t rows, int64_t cols, int64_t nnz);
 DMat Upload(Ctx& ctx, int64_t rows, int64_t cols, const int64_t* ptr, const int64_t* idx, const double* val);
 // Download into host arrays (ptr: cols + 1, idx/val: nnz).
 void Download(Ctx& ctx, const DMat& m, int64_t* ptr, int64_t* idx, double* val);
+
+// Copies between pageable host arrays and device memory through pinned
+// chunks, the host side split over worker threads (transfer.cpp).  Each
+// returns when the copy is complete; the ones to the host are queued after
+// the work already on ctx.stream.
+void CopyToDevice(Ctx& ctx, void* dst, const void* src, size_t bytes);
+// dst[i] = int(src[i] - sub); throws when a value leaves [0, 2^31 - 1].
+void NarrowToDevice(Ctx& ctx, int* dst, const int64_t* src, int64_t n, int64_t sub);
+void CopyToHost(Ctx& ctx, void* dst, const void* src, size_t bytes);
+// dst[i] = src[i] (int32 on the device, int64 on the host).
+void WidenToHost(Ctx& ctx, int64_t* dst, const int* src, int64_t n);
+// f(begin, end) over slices of [0, n) on the library's host worker threads
+// (and the calling thread); returns when every slice is done.
+void HostParallelFor(int64_t n, const std::function<void(int64_t, int64_t)>& f);
 DMat EmptyMatrix(Ctx& ctx, int64_t rows, int64_t cols);
 DMat Identity(Ctx& ctx, int64_t n);
 DMat Clone(Ctx& ctx, const DMat& m);

@@ -696,11 +696,6 @@ __global__ void SellGather(int64_t len, const int* __restrict__ src, const T* __
   if (k < len) out[k] = src[k] >= 0 ? v[src[k]] : T(0);
 }
 
-__global__ void CsrFromInt64(int64_t n, const int64_t* __restrict__ in, int64_t base, int* __restrict__ out) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = int(in[i] - base);
-}
-
 __global__ void ToFloat(int64_t n, const double* __restrict__ in, float* __restrict__ out) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) out[i] = float(in[i]);
@@ -744,14 +739,10 @@ DCsr CsrFromHost(Ctx& ctx, int64_t n, int64_t nnz, const int64_t* ptr, const int
   m.ptr = ctx.Alloc<int>(n + 1);
   m.col = ctx.Alloc<int>(std::max<int64_t>(m.nnz, 1));
   m.val = ctx.Alloc<double>(std::max<int64_t>(m.nnz, 1));
-  auto tmp = ctx.Alloc<int64_t>(std::max<int64_t>(n + 1, m.nnz));
-  SPAI_HIP_CHECK(hipMemcpyAsync(tmp.get(), ptr, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, ctx.stream));
-  CsrFromInt64<<<(unsigned)((n + 1 + 255) / 256), 256, 0, ctx.stream>>>(n + 1, tmp.get(), base, m.ptr.get());
-  ctx.Sync();
+  NarrowToDevice(ctx, m.ptr.get(), ptr, n + 1, base);
   if (m.nnz > 0) {
-    SPAI_HIP_CHECK(hipMemcpyAsync(tmp.get(), col + base, sizeof(int64_t) * m.nnz, hipMemcpyHostToDevice, ctx.stream));
-    CsrFromInt64<<<(unsigned)((m.nnz + 255) / 256), 256, 0, ctx.stream>>>(m.nnz, tmp.get(), 0, m.col.get());
-    SPAI_HIP_CHECK(hipMemcpyAsync(m.val.get(), val + base, sizeof(double) * m.nnz, hipMemcpyHostToDevice, ctx.stream));
+    NarrowToDevice(ctx, m.col.get(), col + base, m.nnz, 0);
+    CopyToDevice(ctx, m.val.get(), val + base, sizeof(double) * size_t(m.nnz));
   }
   ctx.Sync();
   return m;

@@ -1832,18 +1832,6 @@ __global__ void IdentityKernel(int n, int* __restrict__ ptr, int* __restrict__ i
   }
 }
 
-__global__ void Int64ToInt32(int64_t n, const int64_t* __restrict__ in, int64_t base, int* __restrict__ out) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  out[i] = static_cast<int>(in[i] - base);
-}
-
-__global__ void Int32ToInt64(int64_t n, const int* __restrict__ in, int64_t* __restrict__ out) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  out[i] = in[i];
-}
-
 __global__ void FillKernel(int64_t n, double* __restrict__ p, double v) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -2032,30 +2020,20 @@ DMat Upload(Ctx& ctx, int64_t rows, int64_t cols, const int64_t* ptr, const int6
   DMat m;
   m.pat = MakePattern(ctx, rows, cols, nnz);
   m.val = ctx.Alloc<double>(nnz > 0 ? nnz : 1);
-  auto tmp = ctx.Alloc<int64_t>(std::max<int64_t>(cols + 1, nnz) + 1);
-  SPAI_HIP_CHECK(hipMemcpyAsync(tmp.get(), ptr, sizeof(int64_t) * (cols + 1), hipMemcpyHostToDevice, ctx.stream));
-  Int64ToInt32<<<Blocks(cols + 1), kBlock, 0, ctx.stream>>>(cols + 1, tmp.get(), base, m.pat->ptr.get());
+  NarrowToDevice(ctx, m.pat->ptr.get(), ptr, cols + 1, base);
   if (nnz > 0) {
-    auto tmp2 = ctx.Alloc<int64_t>(nnz);
-    SPAI_HIP_CHECK(hipMemcpyAsync(tmp2.get(), idx + base, sizeof(int64_t) * nnz, hipMemcpyHostToDevice, ctx.stream));
-    Int64ToInt32<<<Blocks(nnz), kBlock, 0, ctx.stream>>>(nnz, tmp2.get(), 0, m.pat->idx.get());
-    SPAI_HIP_CHECK(hipMemcpyAsync(m.val.get(), val + base, sizeof(double) * nnz, hipMemcpyHostToDevice, ctx.stream));
-    ctx.Sync();  // host buffers may be released by the caller after return
+    NarrowToDevice(ctx, m.pat->idx.get(), idx + base, nnz, 0);
+    CopyToDevice(ctx, m.val.get(), val + base, sizeof(double) * size_t(nnz));
   }
-  ctx.Sync();
   return m;
 }
 
 void Download(Ctx& ctx, const DMat& m, int64_t* ptr, int64_t* idx, double* val) {
   const int64_t cols = m.cols(), nnz = m.nnz();
-  auto t = ctx.Alloc<int64_t>(std::max<int64_t>(cols + 1, nnz));
-  Int32ToInt64<<<Blocks(cols + 1), kBlock, 0, ctx.stream>>>(cols + 1, m.ptr(), t.get());
-  SPAI_HIP_CHECK(hipMemcpyAsync(ptr, t.get(), sizeof(int64_t) * (cols + 1), hipMemcpyDeviceToHost, ctx.stream));
-  ctx.Sync();
+  WidenToHost(ctx, ptr, m.ptr(), cols + 1);
   if (nnz > 0) {
-    Int32ToInt64<<<Blocks(nnz), kBlock, 0, ctx.stream>>>(nnz, m.idx(), t.get());
-    SPAI_HIP_CHECK(hipMemcpyAsync(idx, t.get(), sizeof(int64_t) * nnz, hipMemcpyDeviceToHost, ctx.stream));
-    SPAI_HIP_CHECK(hipMemcpyAsync(val, m.v(), sizeof(double) * nnz, hipMemcpyDeviceToHost, ctx.stream));
+    WidenToHost(ctx, idx, m.idx(), nnz);
+    CopyToHost(ctx, val, m.v(), sizeof(double) * size_t(nnz));
   }
   ctx.Sync();
 }
