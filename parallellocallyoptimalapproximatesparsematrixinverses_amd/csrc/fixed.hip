@@ -482,7 +482,14 @@ __device__ __forceinline__ void FusedProduct2Stash(int ns, int lane, const Fused
 // from the first S2 entry of j's tile rounded down to a multiple of 16 bytes
 // (kVec entries).  s2d[j]: position of the diagonal in S2(:, j).  R and prS2
 // hold kVec - 1 entries of padding past the end, so whole chunks load.
-template <class V, class SlotT, int GL, bool kInit, bool kMr, bool kStash>
+// kTerms: how product 2 gets product 1's terms.  kTermsReload: from
+// registers for columns of at most kFusedPre steps (narrow), else re-staged
+// from a0c with the plan and A values loaded again; kTermsStash: from
+// registers or the LDS stash; kTermsRegs: every column has at most kFusedPre
+// steps and every column of A at most GL entries, so always from registers
+// (no stash in LDS, no re-staging code).
+constexpr int kTermsReload = 0, kTermsStash = 1, kTermsRegs = 2;
+template <class V, class SlotT, int GL, bool kInit, bool kMr, int kTerms>
 __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
     int n, const int* __restrict__ sp, const int2* __restrict__ a0c, const uint16_t* __restrict__ stq,
     const int* __restrict__ s2d, const int* __restrict__ s2p, const V* __restrict__ av,
@@ -499,6 +506,8 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
   constexpr int kVec = 16 / int(sizeof(V));  // S2 entries per chunk
   constexpr int RQ = sizeof(V) == 4 ? 1 : 2;  // S2 chunks of a tile held in registers per lane
   constexpr int RS = 2;                        // S entries of a tile held in registers per lane
+  constexpr bool kStash = kTerms == kTermsStash;
+  constexpr bool kRegsOnly = kTerms == kTermsRegs;
   if (flags->stop) return;
   // LOMR: R -= delta AZ + gamma AP (R read); MR: R = I - A M recomputed
   constexpr bool kLomrIter = !kInit && !kMr;
@@ -616,7 +625,8 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
     // (FP64 LOMR keeps 16 doubles of direction state already; holding the
     // terms as well spills, so it reloads or stashes them)
     constexpr bool kRegTerms = !(sizeof(V) == 8 && kLomrIter);
-    const bool reg2 = kRegTerms && narrow && ncs <= kFusedPre;
+    static_assert(!kRegsOnly || kRegTerms, "kTermsRegs needs product 1's terms kept in registers");
+    const bool reg2 = kRegTerms && (kRegsOnly || (narrow && ncs <= kFusedPre));
     FusedTerms<V, SlotT> terms;
     // product 1: A [Z P](:, j)  (kInit: A M(:, j); MR: A (M + alpha Z)(:, j))
     for (int t = lane; t < qe - qal; t += kWave) {
@@ -631,8 +641,9 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
         st_ac[gl * W + g] = pre_ac;
         st_y[gl * W + g] = pre_y;
       }
-      FusedProduct1<V, SlotT, GL, NY, kStash>(g, gl, e0, e1, pp, a0c, av, plan, acc + cb2, st_ac, st_y, stash,
-                                              !reg2, terms, yget1);
+      // (with kTermsRegs no column of A is longer than GL, as with the stash)
+      FusedProduct1<V, SlotT, GL, NY, kStash || kRegsOnly>(g, gl, e0, e1, pp, a0c, av, plan, acc + cb2, st_ac, st_y,
+                                                           stash, kStash && !reg2, terms, yget1);
       // R = I - A M: the diagonal's sum becomes (A M)(j, j) - 1, so that
       // R = -sum below rounds as 1 - (A M)(j, j)
       if (!kLomrIter && gl == 0) acc[cb2 + dj].a -= V(1);
@@ -730,7 +741,7 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
           if ((terms.act >> s) & 1u) AddTerm<V, NY2>(acc + cb2, int(terms.sl[s]), terms.a[s], yzp[cb + s]);
       } else if constexpr (kStash) {
         FusedProduct2Stash<V, NY2>(ncs, lane, stash, yzp + cb, acc + cb2);
-      } else {
+      } else if constexpr (!kRegsOnly) {
         const V2<V>* const ycol = yzp + cb - e0;
         auto yget2 = [&](int e, V& ya, V& yb) {
           const V2<V> y = ycol[e];
@@ -1711,8 +1722,12 @@ class FixedImpl final : public FixedProblem {
         const size_t waves = kBlk / kWave;
         const size_t with = waves * size_t(MakeFusedLayout(lmax_, smax_, gl_, vsize, true).bytes);
         const size_t without = waves * size_t(MakeFusedLayout(lmax_, smax_, gl_, vsize, false).bytes);
-        stash_ = maxcol_ <= gl_ && with <= 40 * 1024;
-        smemf_ = stash_ ? with : without;
+        // every column within kFusedPre steps and every column of A within
+        // GL (FP32, byte slots, GL <= 32): product 2 always from registers
+        const bool regs_only =
+            sizeof(V) == 4 && maxcol_ <= gl_ && smax_ <= kFusedPre && !plan.slot16 && gl_ <= 32;
+        terms_ = regs_only ? kTermsRegs : (maxcol_ <= gl_ && with <= 40 * 1024) ? kTermsStash : kTermsReload;
+        smemf_ = terms_ == kTermsStash ? with : without;
         fused_ok_ = smemf_ <= 60 * 1024 && int64_t(kWave / gl_) * lmax_ + 16 <= 65535 &&
                     total < (int64_t(1) << 32);
         // the pass walks its tiles with a grid that the device holds
@@ -1934,12 +1949,18 @@ class FixedImpl final : public FixedProblem {
       (int)n_, s_pat_->ptr.get(), a0c_.get(), stq_.get(), s2d_.get(), s2_pat_->ptr.get(), av_.get(),           \
       plan_ptr_.get(), plan, lmax_, smax_, prS_.get(), prS2_.get(), R_.get(), M_.get(), Z_.get(), P_.get(),    \
       sc_.get(), part, flags_.get(), maxcol_ <= gl_ ? 1 : 0)
-#define SPAI_FUSED_ONE(SlotT, plan, GLV, INIT, MR)          \
-  do {                                                      \
-    if (stash_)                                             \
-      SPAI_FUSED_STASH(SlotT, plan, GLV, INIT, MR, true);   \
-    else                                                    \
-      SPAI_FUSED_STASH(SlotT, plan, GLV, INIT, MR, false);  \
+#define SPAI_FUSED_ONE(SlotT, plan, GLV, INIT, MR)                          \
+  do {                                                                      \
+    if constexpr (sizeof(V) == 4 && sizeof(SlotT) == 1 && GLV <= 32) {     \
+      if (terms_ == kTermsRegs) {                                           \
+        SPAI_FUSED_STASH(SlotT, plan, GLV, INIT, MR, kTermsRegs);           \
+        break;                                                              \
+      }                                                                     \
+    }                                                                       \
+    if (terms_ == kTermsStash)                                              \
+      SPAI_FUSED_STASH(SlotT, plan, GLV, INIT, MR, kTermsStash);            \
+    else                                                                    \
+      SPAI_FUSED_STASH(SlotT, plan, GLV, INIT, MR, kTermsReload);           \
   } while (0)
 #define SPAI_FUSED_LAUNCH(SlotT, plan, GLV)                     \
   do {                                                          \
@@ -2008,12 +2029,21 @@ class FixedImpl final : public FixedProblem {
 #define SPAI_OCC(SlotT, GLV)                                                                                     \
   do {                                                                                                           \
     const void* f = nullptr;                                                                                     \
-    if (mr)                                                                                                      \
-      f = stash_ ? reinterpret_cast<const void*>(&FusedPass<V, SlotT, GLV, false, true, true>)                  \
-                 : reinterpret_cast<const void*>(&FusedPass<V, SlotT, GLV, false, true, false>);                \
-    else                                                                                                         \
-      f = stash_ ? reinterpret_cast<const void*>(&FusedPass<V, SlotT, GLV, false, false, true>)                 \
-                 : reinterpret_cast<const void*>(&FusedPass<V, SlotT, GLV, false, false, false>);               \
+    if constexpr (sizeof(V) == 4 && sizeof(SlotT) == 1 && GLV <= 32) {                                          \
+      if (terms_ == kTermsRegs)                                                                                  \
+        f = mr ? reinterpret_cast<const void*>(&FusedPass<V, SlotT, GLV, false, true, kTermsRegs>)              \
+               : reinterpret_cast<const void*>(&FusedPass<V, SlotT, GLV, false, false, kTermsRegs>);            \
+    }                                                                                                            \
+    if (f == nullptr) {                                                                                          \
+      if (mr)                                                                                                    \
+        f = terms_ == kTermsStash                                                                                \
+                ? reinterpret_cast<const void*>(&FusedPass<V, SlotT, GLV, false, true, kTermsStash>)            \
+                : reinterpret_cast<const void*>(&FusedPass<V, SlotT, GLV, false, true, kTermsReload>);          \
+      else                                                                                                       \
+        f = terms_ == kTermsStash                                                                                \
+                ? reinterpret_cast<const void*>(&FusedPass<V, SlotT, GLV, false, false, kTermsStash>)           \
+                : reinterpret_cast<const void*>(&FusedPass<V, SlotT, GLV, false, false, kTermsReload>);         \
+    }                                                                                                            \
     SPAI_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kBlk, smemf_));                         \
   } while (0)
 #define SPAI_OCC_GL(SlotT)                  \
@@ -2210,7 +2240,7 @@ class FixedImpl final : public FixedProblem {
   unsigned gridf_ = 1;
   DBuf<double> partf_;
   DBuf<int2> a0c_;
-  bool stash_ = false;
+  int terms_ = kTermsReload;  // how the fused pass's product 2 gets its terms
   DBuf<uint16_t> stq_;
   DBuf<int> s2d_;
   DBuf<V> prS_, prS2_;

@@ -6,10 +6,10 @@ The fused per-iteration pass gives each wavefront tiles of 64/GL consecutive
 columns (GL lanes per column) and walks them with the grid the device holds
 resident at once.  A wavefront with more than one tile loads the next tile's
 bounds and first step chunk while it finishes the current one; that path only
-runs above (resident wavefronts x 64/GL) columns -- about 41k columns for
-GL = 8 in FP32 on 256 CUs.  The cases:
+runs above (resident wavefronts x 64/GL) columns -- at most 65,536 columns
+for GL = 8 on 256 CUs (8 wavefronts per SIMD).  The cases:
 
-  diffusion3d(36)   n = 46,656,  7-point, GL = 8   (just above the threshold)
+  diffusion3d(42)   n = 74,088,  7-point, GL = 8   (above the threshold at any occupancy)
   stencil27(24)     n = 13,824, 27-point, GL = 32
   diffusion3d(64)   n = 262,144, the benchmark workload (6-8 tiles per wave)
 
@@ -32,7 +32,7 @@ pytestmark = pytest.mark.gpu
 _THREADS = 16
 
 _CASES = {
-    "diffusion36": lambda: problems.diffusion3d(36, seed=1),
+    "diffusion42": lambda: problems.diffusion3d(42, seed=1),
     "stencil27_24": lambda: problems.stencil27(24, seed=4),
     "diffusion64": lambda: problems.diffusion3d(64, seed=1),
 }
@@ -69,7 +69,7 @@ def _session_build(gpu_lib, name, method, iters, precision):
     return s, it, s.result(), info
 
 
-@pytest.mark.parametrize("name", ["diffusion36", "stencil27_24", "diffusion64"])
+@pytest.mark.parametrize("name", ["diffusion42", "stencil27_24", "diffusion64"])
 @pytest.mark.parametrize("method", ["mr", "lomr"])
 def test_multitile_fp64_matches_oracle(gpu_lib, oracle_lib, name, method):
     iters = 6
@@ -81,7 +81,7 @@ def test_multitile_fp64_matches_oracle(gpu_lib, oracle_lib, name, method):
     assert_same_result(g, o, hist_rtol=1e-10, m_rtol=1e-12)
 
 
-@pytest.mark.parametrize("name", ["diffusion36", "stencil27_24", "diffusion64"])
+@pytest.mark.parametrize("name", ["diffusion42", "stencil27_24", "diffusion64"])
 @pytest.mark.parametrize("method", ["mr", "lomr"])
 def test_multitile_fp32_against_fp64_oracle(gpu_lib, oracle_lib, name, method):
     iters = 6
