@@ -145,6 +145,10 @@ def lib():
                                      ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                      ctypes.POINTER(ctypes.c_double)]
         L.oracle_solve_ls2x2.argtypes = [ctypes.POINTER(ctypes.c_double)] * 3
+        L.oracle_solve_ls2x2_rank_eps.argtypes = [ctypes.POINTER(ctypes.c_double)] * 3 + [ctypes.c_double]
+        L.oracle_run_rank_eps.restype = Result
+        L.oracle_run_rank_eps.argtypes = [ctypes.c_int64, MatrixData, MatrixData, MatrixData, MethodParams,
+                                          SpaiExtParams, ctypes.c_int64, ctypes.c_double]
         L.oracle_column_topk.restype = ctypes.c_int64
         L.oracle_column_topk.argtypes = [MatrixData, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64),
                                          ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]
@@ -202,8 +206,11 @@ def result_to_python(res: Result, free):
 
 def run(method: str, A, M0=None, Pr=None, max_iterations=100, tolerance=1e-9, max_density=0.03,
         enable_dropping=True, num_threads=1, inner_iterations=2, fixed_pattern=0, column_top_k=0,
-        omp_threads=0):
-    """Runs one method on the CPU oracle; returns dict(M, iterations, converged, history)."""
+        omp_threads=0, rank_eps=None):
+    """Runs one method on the CPU oracle; returns dict(M, iterations, converged, history).
+    rank_eps: the rank threshold of the inner-outer LOMR 2x2 solves (default
+    DBL_EPSILON, the reference's); sqrt(FLT_EPSILON) restates the device's FP32
+    form."""
     keep = []
     A_csc = as_csc(A)
     mA = _md(A_csc, keep)
@@ -214,7 +221,10 @@ def run(method: str, A, M0=None, Pr=None, max_iterations=100, tolerance=1e-9, ma
                           int(inner_iterations))
     ext = SpaiExtParams(int(fixed_pattern), 0, int(column_top_k))
     L = lib()
-    res = L.oracle_run(METHODS[method], mA, mM, mP, params, ext, int(omp_threads))
+    if rank_eps is None:
+        res = L.oracle_run(METHODS[method], mA, mM, mP, params, ext, int(omp_threads))
+    else:
+        res = L.oracle_run_rank_eps(METHODS[method], mA, mM, mP, params, ext, int(omp_threads), float(rank_eps))
     return result_to_python(res, L.oracle_free)
 
 
@@ -250,12 +260,16 @@ def column_topk(X, k) -> sparse.csc_matrix:
     return sparse.csc_matrix((val[:nnz], idx[:nnz], ptr), shape=X.shape)
 
 
-def solve_ls2x2(m, x):
+def solve_ls2x2(m, x, rank_eps=None):
     m = np.ascontiguousarray(m, dtype=np.float64).reshape(4)
     x = np.ascontiguousarray(x, dtype=np.float64).reshape(2)
     out = np.zeros(2)
     dp = ctypes.POINTER(ctypes.c_double)
-    lib().oracle_solve_ls2x2(m.ctypes.data_as(dp), x.ctypes.data_as(dp), out.ctypes.data_as(dp))
+    if rank_eps is None:
+        lib().oracle_solve_ls2x2(m.ctypes.data_as(dp), x.ctypes.data_as(dp), out.ctypes.data_as(dp))
+    else:
+        lib().oracle_solve_ls2x2_rank_eps(m.ctypes.data_as(dp), x.ctypes.data_as(dp), out.ctypes.data_as(dp),
+                                          float(rank_eps))
     return out
 
 

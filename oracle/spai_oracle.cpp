@@ -371,7 +371,12 @@ void ApplyHouse2(double e, double tau, double& v0, double& v1) {
   v1 -= (tau * e) * tmp;
 }
 
-void SolveLS2x2(const double m[2][2], const double x[2], double out[2]) {
+// rank_eps replaces eps in the final rank decision only (|r_ii| > maxpivot *
+// 2 * rank_eps); Eigen's value, DBL_EPSILON, is the default.  A larger value
+// restates the FP32 inner-outer form of the device library, which takes
+// pivots below sqrt(FLT_EPSILON) as zero.
+void SolveLS2x2(const double m[2][2], const double x[2], double out[2],
+                double rank_eps = std::numeric_limits<double>::epsilon()) {
   out[0] = out[1] = 0.0;
   for (int i = 0; i < 2; ++i) {
     if (!std::isfinite(x[i])) return;
@@ -438,7 +443,7 @@ void SolveLS2x2(const double m[2][2], const double x[2], double out[2]) {
     if (nonzero_pivots == 2 && big_sq < threshold_helper * 1.0) nonzero_pivots = 1;
     maxpivot = std::max(maxpivot, std::fabs(a[1][1]));
   }
-  const double premultiplied = maxpivot * (eps * 2.0);
+  const double premultiplied = maxpivot * (rank_eps * 2.0);
   int rank = 0;
   for (int i = 0; i < nonzero_pivots; ++i) rank += (std::fabs(a[i][i]) > premultiplied) ? 1 : 0;
   if (rank == 0) {
@@ -490,6 +495,7 @@ struct Params {
   // extensions
   bool fixed_pattern = false;
   i64 top_k = 0;
+  double rank_eps = std::numeric_limits<double>::epsilon();  // inner-outer 2x2 rank threshold
 };
 
 struct History {
@@ -1111,7 +1117,7 @@ Output RunInnerOuter(const Params& prm, i64 threads, bool lomr, i64 inner_iterat
             const double g[2][2] = {{ColSqNorm(s.AZ, j), coupling}, {coupling, ColSqNorm(AQ, j)}};
             const double x[2] = {ColDot(s.R, s.AZ, j), ColDot(s.R, AQ, j)};
             double c[2];
-            SolveLS2x2(g, x, c);
+            SolveLS2x2(g, x, c, prm.rank_eps);
             delta[j] = c[0];
             gamma[j] = c[1];
           }
@@ -1229,9 +1235,11 @@ Result SuccessResult(const Output& o) {
   return r;
 }
 
-Result Run(int64_t method, MatrixData A, MatrixData M0, MatrixData Pr, MethodParams cp, SpaiExtParams ext) {
+Result Run(int64_t method, MatrixData A, MatrixData M0, MatrixData Pr, MethodParams cp, SpaiExtParams ext,
+           double rank_eps = std::numeric_limits<double>::epsilon()) {
   try {
     Params p;
+    p.rank_eps = rank_eps;
     p.A = FromData(A);
     if (cp.use_initial_m) p.M0 = FromData(M0);
     if (cp.use_preconditioner) p.Pr = FromData(Pr);
@@ -1266,6 +1274,14 @@ Result oracle_run(int64_t method, MatrixData A, MatrixData M0, MatrixData Pr, Me
                   int64_t omp_threads) {
   if (omp_threads > 0) omp_set_num_threads((int)omp_threads);
   return Run(method, A, M0, Pr, params, ext);
+}
+
+// oracle_run with the rank threshold of the inner-outer 2x2 solves
+// (SolveLS2x2's rank_eps) set to rank_eps.
+Result oracle_run_rank_eps(int64_t method, MatrixData A, MatrixData M0, MatrixData Pr, MethodParams params,
+                           SpaiExtParams ext, int64_t omp_threads, double rank_eps) {
+  if (omp_threads > 0) omp_set_num_threads((int)omp_threads);
+  return Run(method, A, M0, Pr, params, ext, rank_eps);
 }
 
 void oracle_free(Result* r) {
@@ -1321,6 +1337,11 @@ int64_t oracle_column_topk(MatrixData X, int64_t k, int64_t* out_ptr, int64_t* o
 void oracle_solve_ls2x2(const double* m4, const double* x2, double* out2) {
   const double m[2][2] = {{m4[0], m4[1]}, {m4[2], m4[3]}};
   SolveLS2x2(m, x2, out2);
+}
+
+void oracle_solve_ls2x2_rank_eps(const double* m4, const double* x2, double* out2, double rank_eps) {
+  const double m[2][2] = {{m4[0], m4[1]}, {m4[2], m4[3]}};
+  SolveLS2x2(m, x2, out2, rank_eps);
 }
 
 // backward_error.pyx:16-34 (_row_dot/_multiply): y = A x for CSR A.

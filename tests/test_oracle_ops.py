@@ -124,6 +124,38 @@ def test_ls2x2_rank_one_is_scaled_minimum_norm(oracle_lib):
     np.testing.assert_allclose(G @ got, x, rtol=1e-12)
 
 
+@pytest.mark.parametrize("rho", [1e-5, 1e-7])
+def test_ls2x2_rank_eps_takes_small_pivots_as_zero(oracle_lib, rho):
+    """The FP32 inner-outer LOMR form solves its per-column 2x2 Grams with the
+    rank threshold rank_eps = sqrt(FLT_EPSILON) instead of DBL_EPSILON.  Two
+    directions a and q = t a + rho e (e orthogonal to a) give a Gram whose
+    scaled second pivot is about (rho |e| / (t |a|))^2, between the two
+    thresholds here.  With sqrt(FLT_EPSILON) the solve is the scaled
+    minimum-norm solution of the exact rank-1 Gram (rho = 0) up to O(rho);
+    with DBL_EPSILON it is the full-rank solution, which resolves the e
+    component of r with the coefficient gamma = (r.e) / (rho e.e): large, and
+    cancelled in delta."""
+    a = np.array([1.0, 2.0, 2.0, -1.0])
+    e = np.array([2.0, -1.0, 0.0, 0.0])
+    assert a @ e == 0.0
+    t = 0.5
+    q = t * a + rho * e
+    r = np.array([1.0, 0.3, -0.2, 0.5])
+    G = np.array([[a @ a, a @ q], [a @ q, q @ q]])
+    x = np.array([r @ a, r @ q])
+    y32 = oracle_lib.solve_ls2x2(G, x, rank_eps=np.sqrt(np.finfo(np.float32).eps))
+    G1 = np.array([[a @ a, t * (a @ a)], [t * (a @ a), t * t * (a @ a)]])
+    x1 = np.array([r @ a, t * (r @ a)])
+    np.testing.assert_allclose(y32, scaled_min_norm(G1, x1), rtol=10 * rho)
+    # the rank-1 combination moves along a alone: y32[0] a + y32[1] q ~ (r.a / a.a) a
+    np.testing.assert_allclose(y32[0] * a + y32[1] * q, (r @ a) / (a @ a) * a, rtol=10 * rho, atol=1e-12)
+    y64 = oracle_lib.solve_ls2x2(G, x)
+    np.testing.assert_allclose(y64[1], (r @ e) / (rho * (e @ e)), rtol=0.05)
+    assert abs(y64[1]) > 1e3 * abs(y32[1])
+    # the default threshold is the reference's: same result as rank_eps = DBL_EPSILON
+    np.testing.assert_array_equal(y64, oracle_lib.solve_ls2x2(G, x, rank_eps=np.finfo(np.float64).eps))
+
+
 @pytest.mark.parametrize("bad", [np.nan, np.inf])
 def test_ls2x2_nonfinite_gives_zero(oracle_lib, bad):
     np.testing.assert_array_equal(oracle_lib.solve_ls2x2([[1.0, 0.0], [0.0, bad]], [1.0, 1.0]), [0.0, 0.0])
