@@ -220,6 +220,20 @@ def calibrate_copy(mib):
                       "launches": 5}))
 
 
+def spgemm_only(repeats):
+    """Only the standalone SpGEMM of the headline line (C = A A on the 64^3
+    diffusion matrix, symbolic and numeric phases, operands resident):
+    `repeats` products after one warm-up, for kernel traces of that product
+    alone."""
+    from parallellocallyoptimalapproximatesparsematrixinverses_amd import _lib
+
+    A, _, _ = problem()
+    _lib.spgemm(A, A, repeats=1)
+    C, s = _lib.spgemm(A, A, repeats=repeats)
+    print(json.dumps({"spgemm": "C = A A, diffusion3d 64^3", "repeats": repeats, "seconds_per_product": s,
+                      "nnz_C": int(C.nnz)}))
+
+
 def _median_time(fn, reps):
     ts = []
     for _ in range(reps):
@@ -417,6 +431,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 launches of an FP64 y = 1.0 * x kernel over MIB MiB (counter calibration)")
+    ap.add_argument("--spgemm", type=int, default=0, metavar="REPEATS",
+                    help="only run REPEATS standalone SpGEMM products C = A A (for kernel traces)")
     ap.add_argument("--configs", action="store_true",
                     help="instead of the headline line, one line per other configuration (1, 3, 4, 5)")
     ap.add_argument("--only", type=str, default="",
@@ -428,6 +444,9 @@ def main():
         return
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.spgemm:
+        spgemm_only(args.spgemm)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1"}))

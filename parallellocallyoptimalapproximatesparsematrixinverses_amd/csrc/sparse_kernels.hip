@@ -102,31 +102,57 @@ __global__ void WriteLast(int* __restrict__ out, int64_t n, const int64_t* __res
   out[n] = static_cast<int>(*total);
 }
 
-// The whole scan in one block for short inputs: thread t sums its run of
-// ceil(n / 1024) entries, the block scans the run sums, and each thread
-// writes its run's offsets; out[n] and *total receive the sum.  in may be out.
+// The whole scan in one block for short inputs.  Wavefront w owns the
+// contiguous segment [w seg, (w + 1) seg) and reads it in coalesced 64-entry
+// chunks, kScanBatch chunks in flight: first the segment sums (the block adds
+// the earlier segments' sums), then the scan of each chunk with a running
+// carry.  out[n] and *total receive the sum.  in may be out: a chunk is read
+// before any entry at or after it is written (each lane writes only its own
+// entries, and later chunks are loaded ahead of the current chunk's writes).
 constexpr int64_t kScanSmallMax = 65536;
-__global__ __launch_bounds__(1024) void ScanSmall(const int* __restrict__ in, int64_t n, int* __restrict__ out,
-                                                  int64_t* __restrict__ total) {
+constexpr int kScanBatch = 8;
+__global__ __launch_bounds__(1024) void ScanSmall(const int* in, int64_t n, int* out, int64_t* __restrict__ total) {
   __shared__ int64_t wsum[16];
-  const int64_t per = (n + 1023) / 1024;
-  const int64_t b = min(n, int64_t(threadIdx.x) * per), e = min(n, b + per);
+  const int w = WaveInBlock(), lane = LaneId();
+  const int64_t seg = (n + 15) / 16;
+  const int64_t b = min(n, int64_t(w) * seg), e = min(n, b + seg);
   int64_t s = 0;
-  for (int64_t i = b; i < e; ++i) s += in[i];
-  const int64_t incl = WaveInclusiveScan64(s);
-  if (LaneId() == kWave - 1) wsum[WaveInBlock()] = incl;
-  __syncthreads();
-  int64_t run = incl - s;
-  for (int w = 0; w < WaveInBlock(); ++w) run += wsum[w];
-  const int64_t end = run + s;
-  for (int64_t i = b; i < e; ++i) {
-    const int v = in[i];
-    out[i] = static_cast<int>(run);
-    run += v;
+  for (int64_t i0 = b; i0 < e; i0 += kScanBatch * kWave) {
+    int v[kScanBatch];
+#pragma unroll
+    for (int t = 0; t < kScanBatch; ++t) {
+      const int64_t i = i0 + t * kWave + lane;
+      v[t] = i < e ? in[i] : 0;
+    }
+#pragma unroll
+    for (int t = 0; t < kScanBatch; ++t) s += v[t];
   }
-  if (threadIdx.x == blockDim.x - 1) {
-    out[n] = static_cast<int>(end);
-    *total = end;
+  s = WaveSum<int64_t>(s);
+  if (lane == 0) wsum[w] = s;
+  __syncthreads();
+  int64_t run = 0, all = 0;
+  for (int k = 0; k < 16; ++k) {
+    if (k < w) run += wsum[k];
+    all += wsum[k];
+  }
+  for (int64_t i0 = b; i0 < e; i0 += kScanBatch * kWave) {
+    int v[kScanBatch];
+#pragma unroll
+    for (int t = 0; t < kScanBatch; ++t) {
+      const int64_t i = i0 + t * kWave + lane;
+      v[t] = i < e ? in[i] : 0;
+    }
+#pragma unroll
+    for (int t = 0; t < kScanBatch; ++t) {
+      const int64_t i = i0 + t * kWave + lane;
+      const int64_t incl = WaveInclusiveScan64(v[t]);
+      if (i < e) out[i] = static_cast<int>(run + incl - v[t]);
+      run += __shfl(incl, kWave - 1, kWave);
+    }
+  }
+  if (threadIdx.x == 0) {
+    out[n] = static_cast<int>(all);
+    *total = all;
   }
 }
 
