@@ -80,6 +80,7 @@ def time_form(lib, A, M0, b, precision, steps, warmup):
                true_rel_residual=float(np.linalg.norm(b - A @ x) / np.linalg.norm(b)))
     out["M"] = s.result()["M"]
     out["spmv_a_us"] = s.spmv_a_us()
+    out["pcg_layouts"] = dict(zip(("A", "M"), s.pcg_layouts()))
     out["launch_shape"] = s.info()
     # kernel timing: the same build once more with HIP events recorded on the
     # session's stream around every per-iteration pass (outside the timed
@@ -478,11 +479,20 @@ def main():
     prof = ROOT / "profiles" / "lomr_fused_diffusion64_traffic.json"
     if prof.exists():
         traffic = json.loads(prof.read_text()).get("fp32_bytes_per_launch")
-    # the PCG's A-SpMV, (n+1)*4 + nnz*(4+8) + 2*8*n bytes per launch
-    spmv_bytes = (n + 1) * 4 + nnz * 12 + 16 * n
+    # the PCG's A-SpMV + <p, Ap>: on the diagonal layout the solve reads,
+    # nd*n*8 (values) + 4n (row masks) + 2*8*n (p read, Ap written) bytes per
+    # launch; on CSR/SELL (n+1)*4 + nnz*(4+8) + 2*8*n
+    csr_spmv_bytes = (n + 1) * 4 + nnz * 12 + 16 * n
+    a_coo = A.tocoo()
+    n_diagonals = int(np.unique(a_coo.col.astype(np.int64) - a_coo.row).size)
+    spmv_layout = fp32["pcg_layouts"]["A"]
+    spmv_bytes = n_diagonals * n * 8 + 4 * n + 16 * n if spmv_layout == "dia" else csr_spmv_bytes
+    spmv_kernel = {"dia": f"PcgSpmvADia (SpMV with A + <p, Ap> on the {n_diagonals}-diagonal layout the solve reads)",
+                   "sell": "PcgSpmvASell (SpMV with A + <p, Ap> on the sliced-ELLPACK copy the solve reads)",
+                   "csr": "PcgSpmvA (SpMV with A + <p, Ap> on CSR)"}[spmv_layout]
     spmv_us = fp32["spmv_a_us"]
     spmv_traffic = None
-    prof2 = ROOT / "profiles" / "pcg_spmv_a_diffusion64_traffic.json"
+    prof2 = ROOT / "profiles" / f"pcg_spmv_a_{spmv_layout}_diffusion64_traffic.json"
     if prof2.exists():
         spmv_traffic = json.loads(prof2.read_text()).get("bytes_per_launch")
 
@@ -530,11 +540,12 @@ def main():
                      "frac_of_achievable": round(achieved / achievable, 4),
                      "algorithmic_bytes_per_launch": pass_bytes, "avg_launch_us": round(us, 3),
                      "launches_timed": fp32["lomr_passes_timed"]},
-        "roofline_pcg_spmv": {"kernel": "PcgSpmvASell (SpMV with A + <p, Ap> on the sliced-ELLPACK copy the solve reads), standalone replay", "bound": "hbm",
+        "roofline_pcg_spmv": {"kernel": spmv_kernel + ", standalone replay", "bound": "hbm",
                               "achieved": round(spmv_bytes / (spmv_us * 1e-6) / 1e9, 1), "peak": HBM_PEAK_GBS,
                               "unit": "GB/s", "frac": round(spmv_bytes / (spmv_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
                               "frac_of_achievable": round(spmv_bytes / (spmv_us * 1e-6) / 1e9 / achievable, 4),
                               "traffic": spmv_traffic, "algorithmic_bytes_per_launch": spmv_bytes,
+                              "csr_equivalent_GBs": round(csr_spmv_bytes / (spmv_us * 1e-6) / 1e9, 1),
                               "avg_launch_us": round(spmv_us, 3)},
         "spgemm": {"kernel": "SpGemm C = A A (every column short: ExpansionSizes, MergeCount, scan, MergeNumeric), FP64",
                    "seconds": round(spgemm_s, 7), "nnz_C": int(C.nnz), "algorithmic_bytes": spgemm_bytes,

@@ -159,6 +159,9 @@ int64_t spai_pcg_solve(SpaiPcgSystem* system, const double* b, int64_t max_itera
                        double* history, int64_t* count, double* x_out);
 /* y = A x with the prepared A (row dots in column order, as spai_matvec). */
 int64_t spai_pcg_matvec(SpaiPcgSystem* system, const double* x, double* y);
+/* Row layouts of the prepared system's PCG iteration kernels (as
+ * spai_session_pcg_layouts). */
+int64_t spai_pcg_layouts(SpaiPcgSystem* system, int64_t* out);
 
 /* ------------------------------------------------------------------------
  * Device-resident session: A, M0, Pr and the right-hand sides are uploaded once,
@@ -183,6 +186,12 @@ struct Result spai_session_result(SpaiSession* session);
 /* Per-kernel timing of the last PCG iteration loop: average microseconds of
  * one SpMV with A (HIP events around a replay of the iteration graph). */
 double spai_session_pcg_iteration_us(SpaiSession* session);
+/* Row layouts of the session's PCG iteration kernels, for tests and
+ * measurement: out[0] for A, out[1] for M; 0 = CSR, 1 = sliced ELLPACK,
+ * 2 = diagonals.  The environment variable SPAI_PCG_LAYOUT ("csr", "sell",
+ * "auto") caps the choice when the solver is created.  No counterpart in the
+ * reference (its PCG is a CSR loop, methods_cython/backward_error.pyx:16-34). */
+int64_t spai_session_pcg_layouts(SpaiSession* session, int64_t* out);
 /* on != 0: later builds record HIP events, on the session's stream, around
  * each per-iteration pass of the fused LOMR or MR build. */
 int64_t spai_session_set_kernel_timing(SpaiSession* session, int64_t on);

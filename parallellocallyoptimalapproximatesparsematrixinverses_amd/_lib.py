@@ -23,6 +23,8 @@ EXT_ENTRY_POINTS = ("spai_run_ex", "spai_pcg", "spai_matvec", "spai_last_error",
                     "spai_session_pcg", "spai_session_pcg_output", "spai_session_result",
                     "spai_session_pcg_iteration_us")
 
+PCG_LAYOUTS = ("csr", "sell", "dia")
+
 I64P = ctypes.POINTER(ctypes.c_int64)
 F64P = ctypes.POINTER(ctypes.c_double)
 
@@ -111,6 +113,8 @@ def load():
     L.spai_session_result.argtypes = [ctypes.c_void_p]
     L.spai_session_pcg_iteration_us.restype = ctypes.c_double
     L.spai_session_pcg_iteration_us.argtypes = [ctypes.c_void_p]
+    L.spai_session_pcg_layouts.restype = ctypes.c_int64
+    L.spai_session_pcg_layouts.argtypes = [ctypes.c_void_p, I64P]
     L.spai_session_set_kernel_timing.restype = ctypes.c_int64
     L.spai_session_set_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int64]
     L.spai_session_build_kernel_us.restype = ctypes.c_double
@@ -351,6 +355,13 @@ class Session:
         if v < 0:
             raise RuntimeError(last_error())
         return float(v)
+
+    def pcg_layouts(self):
+        """Row layouts of the PCG iteration kernels for (A, M): "csr", "sell" or "dia"."""
+        out = (ctypes.c_int64 * 2)()
+        if load().spai_session_pcg_layouts(self._h, out) < 0:
+            raise RuntimeError(last_error())
+        return tuple(PCG_LAYOUTS[int(v)] for v in out)
 
     def set_kernel_timing(self, on=True):
         if load().spai_session_set_kernel_timing(self._h, 1 if on else 0) < 0:

@@ -512,4 +512,28 @@ double spai_session_pcg_iteration_us(SpaiSession* s) {
   }
 }
 
+int64_t spai_session_pcg_layouts(SpaiSession* s, int64_t* out) {
+  try {
+    if (!s || !s->pcg) throw std::invalid_argument("session has no built preconditioner");
+    if (!out) throw std::invalid_argument("null output array");
+    out[0] = int64_t(s->pcg->LayoutA());
+    out[1] = int64_t(s->pcg->LayoutM());
+    return 0;
+  } catch (const std::exception& ex) {
+    return Fail(ex.what());
+  }
+}
+
+int64_t spai_pcg_layouts(SpaiPcgSystem* s, int64_t* out) {
+  try {
+    if (!s) throw std::invalid_argument("null PCG system");
+    if (!out) throw std::invalid_argument("null output array");
+    out[0] = int64_t(s->solver->LayoutA());
+    out[1] = int64_t(s->solver->LayoutM());
+    return 0;
+  } catch (const std::exception& ex) {
+    return Fail(ex.what());
+  }
+}
+
 }  // extern "C"

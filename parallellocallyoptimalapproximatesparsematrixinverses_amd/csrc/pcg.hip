@@ -27,6 +27,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <stdexcept>
 
 #include "dev_common.hpp"
@@ -174,17 +176,19 @@ __device__ __forceinline__ int XcdBlock() {
 // End of an iteration (backward_error.pyx:123-133,100): beta, history entry,
 // counters, status.  Returns beta; sets stop when the loop must end.  Writes
 // happen in block 0 only; slots written are never read by the same kernel.
-__device__ __forceinline__ double EndIteration(int prev, double r_z_new, double err, bool refreshed,
-                                               double* __restrict__ ds, int* __restrict__ is,
-                                               double* __restrict__ hist, bool& stop, bool writer) {
-  const double beta = refreshed ? 0.0 : r_z_new / ds[prev ? kRz1 : kRz0];
-  const int count = is[prev ? kCount1 : kCount0];
+// rz_prev, count, max_it and tol are the slots kRz/kCount of parity prev,
+// kMax and kTol as the kernel read them.
+__device__ __forceinline__ double EndIteration(int prev, double r_z_new, double err, bool refreshed, double rz_prev,
+                                               int count, int max_it, double tol, double* __restrict__ ds,
+                                               int* __restrict__ is, double* __restrict__ hist, bool& stop,
+                                               bool writer) {
+  const double beta = refreshed ? 0.0 : r_z_new / rz_prev;
   int status = 0;
   if (!Finite(err))
     status = 3;
   else if (!Finite(beta))
     status = 2;
-  stop = status != 0 || !(count + 1 < is[kMax] && err > ds[kTol]);
+  stop = status != 0 || !(count + 1 < max_it && err > tol);
   if (writer) {
     hist[count] = err;
     is[prev ? kCount0 : kCount1] = count + 1;
@@ -194,6 +198,117 @@ __device__ __forceinline__ double EndIteration(int prev, double r_z_new, double 
     if (stop) is[kDone] = 1;
   }
   return beta;
+}
+
+// A thread's share of G <= kMaxPartials * kBlk block partials (entries
+// threadIdx.x + u kBlk), loaded without waiting on anything.
+constexpr int kMaxPartials = 4;
+__device__ __forceinline__ void LoadPartials(const double* __restrict__ part, int G, double (&v)[kMaxPartials]) {
+#pragma unroll
+  for (int u = 0; u < kMaxPartials; ++u) {
+    const int i = threadIdx.x + u * kBlk;
+    v[u] = i < G ? part[i] : 0.0;
+  }
+}
+
+// This thread's entries of a partial array in SumPartials' order.
+__device__ __forceinline__ double AddPartials(const double (&v)[kMaxPartials], int G) {
+  double s = 0.0;
+#pragma unroll
+  for (int u = 0; u < kMaxPartials; ++u)
+    if (int(threadIdx.x) + u * kBlk < G) s += v[u];
+  return s;
+}
+
+// What K1's prologue reads, loaded at kernel start, none waiting on another
+// (the state slots of the previous iteration and the previous K2's partials
+// of <r', r'> and <r', z>), so the prologue costs one memory round trip.
+struct DirLoads {
+  int done, pend, count, max_it;
+  double bnorm, tol, rz_prev;
+  double rr[kMaxPartials], rz[kMaxPartials];
+  __device__ __forceinline__ void Load(int par, const double* __restrict__ part, int G, const double* __restrict__ ds,
+                                       const int* __restrict__ is) {
+    const int prev = par ^ 1;
+    done = is[kDone];
+    pend = is[prev ? kPend1 : kPend0];
+    count = is[prev ? kCount1 : kCount0];
+    max_it = is[kMax];
+    bnorm = ds[kBnorm];
+    tol = ds[kTol];
+    rz_prev = ds[prev ? kRz1 : kRz0];
+    LoadPartials(part + kP2 * G, G, rr);
+    LoadPartials(part + kP4 * G, G, rz);
+  }
+};
+
+// Prologue of K1 (iteration with parity par): when the previous iteration is
+// pending, its error from the previous K2's <r', r'> partials, the tolerance
+// crossing (pause for the host refresh) or its end (beta, history, stop
+// rules).  Returns false when the kernel must return without work.
+__device__ __forceinline__ bool DirPrologue(int par, const DirLoads& L, int G, double* __restrict__ ds,
+                                            int* __restrict__ is, double* __restrict__ hist, double& beta) {
+  const int prev = par ^ 1;
+  const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
+  beta = 0.0;
+  if (L.done) return false;
+  if (!L.pend) return true;
+  __shared__ double red[32];
+  double rr = AddPartials(L.rr, G), r_z = AddPartials(L.rz, G);
+  BlockSum2<double>(rr, r_z, red);
+  const double err = sqrt(rr) / L.bnorm;
+  if (err <= L.tol) {
+    if (writer) {
+      ds[kErrPre] = err;
+      is[kNeedRefresh] = 1;
+      is[kDone] = 1;
+    }
+    return false;
+  }
+  bool stop = false;
+  beta = EndIteration(prev, r_z, err, false, L.rz_prev, L.count, L.max_it, L.tol, ds, is, hist, stop, writer);
+  return !stop;
+}
+
+// What K2's prologue reads, loaded at kernel start (K1's <p, Ap> partials and
+// the state slots).
+struct UpdateLoads {
+  int done, iter;
+  double rz;
+  double pap[kMaxPartials];
+  __device__ __forceinline__ void Load(int par, const double* __restrict__ part, int G, const double* __restrict__ ds,
+                                       const int* __restrict__ is) {
+    done = is[kDone];
+    iter = is[kIter];
+    rz = ds[par ? kRz1 : kRz0];
+    LoadPartials(part + kP1 * G, G, pap);
+  }
+};
+
+// Prologue of K2: alpha = <r, z> / <p, Ap> from K1's partials
+// (backward_error.pyx:102-110); breakdown ends the solve with status 2.
+// Returns false when the kernel must return without work.
+__device__ __forceinline__ bool UpdatePrologue(int par, const UpdateLoads& L, int G, int* __restrict__ is,
+                                               double& alpha) {
+  alpha = 0.0;
+  if (L.done) return false;
+  __shared__ double red[16];
+  const double den = BlockSum<double>(AddPartials(L.pap, G), red);
+  const double r_z = L.rz;
+  bool brk = !Finite(r_z) || !Finite(den) || r_z == 0.0 || den == 0.0;
+  alpha = brk ? 0.0 : r_z / den;
+  brk = brk || !Finite(alpha);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    is[par ? kPend0 : kPend1] = 0;  // consumed by K1 of this iteration
+    if (brk) {
+      is[kStatus] = 2;
+      is[kDone] = 1;
+    } else {
+      is[kIter] = L.iter + 1;
+      is[par ? kPend1 : kPend0] = 1;
+    }
+  }
+  return !brk;
 }
 
 // r = b, x = 0, z = M b, p = 0; partials of <r, r> and <r, z>.
@@ -380,10 +495,9 @@ __global__ __launch_bounds__(kBlk) void PcgDirSpmvA(int n, int par, Rows rows, c
                                                     double* __restrict__ Ap, double* __restrict__ part,
                                                     double* __restrict__ ds, int* __restrict__ is,
                                                     double* __restrict__ hist) {
-  if (is[kDone]) return;
   const int G = gridDim.x;
-  const int prev = par ^ 1;
-  const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
+  DirLoads L;
+  L.Load(par, part, G, ds, is);
   const int row0 = XcdBlock() * (kBlk / SG) + threadIdx.x / SG;
   RowPrefetch<double> pf;
   double z0 = 0.0, p0 = 0.0;
@@ -394,22 +508,7 @@ __global__ __launch_bounds__(kBlk) void PcgDirSpmvA(int n, int par, Rows rows, c
     p0 = have ? p_in[row0] : 0.0;
   }
   double beta = 0.0;
-  if (is[prev ? kPend1 : kPend0]) {
-    double rr = 0.0, r_z = 0.0;
-    SumPartials2(part + kP2 * G, part + kP4 * G, G, rr, r_z);
-    const double err = sqrt(rr) / ds[kBnorm];
-    if (err <= ds[kTol]) {
-      if (writer) {
-        ds[kErrPre] = err;
-        is[kNeedRefresh] = 1;
-        is[kDone] = 1;
-      }
-      return;
-    }
-    bool stop = false;
-    beta = EndIteration(prev, r_z, err, false, ds, is, hist, stop, writer);
-    if (stop) return;
-  }
+  if (!DirPrologue(par, L, G, ds, is, hist, beta)) return;
   const int groups = kBlk / SG, sl = threadIdx.x % SG;
   double acc = 0.0;
   for (int row = row0; row < n; row += G * groups) {
@@ -470,10 +569,11 @@ __global__ __launch_bounds__(kBlk) void PcgUpdateSpmvM(int n, int par, Rows rows
                                                        const double* __restrict__ p, const double* __restrict__ Ap,
                                                        double* __restrict__ z, double* __restrict__ part,
                                                        double* __restrict__ ds, int* __restrict__ is) {
-  if (is[kDone]) return;
   // FP32 values leave room for the registers of eager loads; FP64 ones do not
   constexpr bool kEagerM = sizeof(MV) == 4 && Rows::kPadded;
   const int G = gridDim.x;
+  UpdateLoads L;
+  L.Load(par, part, G, ds, is);
   const int row0 = XcdBlock() * (kBlk / SG) + threadIdx.x / SG;
   RowPrefetch<MV> pf;
   double x0 = 0.0, p0 = 0.0, r0 = 0.0, a0 = 0.0;
@@ -485,22 +585,8 @@ __global__ __launch_bounds__(kBlk) void PcgUpdateSpmvM(int n, int par, Rows rows
     r0 = have ? r_in[row0] : 0.0;
     a0 = have ? Ap[row0] : 0.0;
   }
-  const double den = SumPartials(part + kP1 * G, G);
-  const double r_z = ds[par ? kRz1 : kRz0];
-  bool brk = !Finite(r_z) || !Finite(den) || r_z == 0.0 || den == 0.0;
-  const double alpha = brk ? 0.0 : r_z / den;
-  brk = brk || !Finite(alpha);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    is[par ? kPend0 : kPend1] = 0;  // consumed by K1 of this iteration
-    if (brk) {
-      is[kStatus] = 2;
-      is[kDone] = 1;
-    } else {
-      is[kIter] = is[kIter] + 1;
-      is[par ? kPend1 : kPend0] = 1;
-    }
-  }
-  if (brk) return;
+  double alpha;
+  if (!UpdatePrologue(par, L, G, is, alpha)) return;
   const int groups = kBlk / SG, sl = threadIdx.x % SG;
   double rr = 0.0, rz = 0.0;
   for (int row = row0; row < n; row += G * groups) {
@@ -537,6 +623,219 @@ __global__ __launch_bounds__(kBlk) void PcgUpdateSpmvM(int n, int par, Rows rows
     }
   }
   StorePartial2(rr, rz, part + kP2 * G, part + kP4 * G);
+}
+
+// Diagonal layout as the kernels take it: the offsets by value (kernel
+// arguments, loaded with the others at wave start), the row masks and the
+// diagonal count.
+struct DiaView {
+  int off[kDiaMaxDiagonals];
+  const unsigned* __restrict__ mask;
+  int nd;
+};
+
+// Operands of diagonals [D0, D0 + 8) of one row: the values and NV vectors at
+// row + off[d], the column clamped into [0, n) (an entry the row does not
+// have is loaded but never enters a sum).  No address depends on a load, so
+// all of them are in flight together.
+template <class MV, int NV>
+struct DiaChunk {
+  MV v[8];
+  double x[NV][8];
+  template <int D0>
+  __device__ __forceinline__ void Load(const DiaView& dv, const MV* __restrict__ val, const double* __restrict__ y0,
+                                       const double* __restrict__ y1, int row, int n) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (D0 + u < dv.nd) {
+        const int c = min(max(row + dv.off[D0 + u], 0), n - 1);
+        v[u] = val[size_t(D0 + u) * size_t(n) + row];
+        x[0][u] = y0[c];
+        if constexpr (NV > 1) x[1][u] = y1[c];
+      } else {
+        v[u] = MV(0);
+#pragma unroll
+        for (int w = 0; w < NV; ++w) x[w][u] = 0.0;
+      }
+    }
+  }
+  // s + sum of v[u] * f(x[0][u], x[NV-1][u]) over the diagonals the row has,
+  // in diagonal order (= column order, the sequential order of the CSR row)
+  template <int D0, class F>
+  __device__ __forceinline__ double Sum(unsigned m, int nd, F f, double s) const {
+    double t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = double(v[u]) * f(x[0][u], x[NV - 1][u]);
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (D0 + u < nd && ((m >> (D0 + u)) & 1u)) s += t[u];
+    return s;
+  }
+};
+
+// Row sum over diagonals 8.. of a row (the first chunk is summed by the
+// caller): chunk by chunk, each chunk's loads issued together.
+template <class MV, int NV, class F>
+__device__ __forceinline__ double DiaRest(const DiaView& dv, const MV* __restrict__ val,
+                                          const double* __restrict__ y0, const double* __restrict__ y1, int row,
+                                          int n, unsigned m, F f, double s) {
+  static_assert(kDiaMaxDiagonals == 32, "chunks below cover 32 diagonals");
+  if (dv.nd > 8) {
+    DiaChunk<MV, NV> c;
+    c.template Load<8>(dv, val, y0, y1, row, n);
+    s = c.template Sum<8>(m, dv.nd, f, s);
+  }
+  if (dv.nd > 16) {
+    DiaChunk<MV, NV> c;
+    c.template Load<16>(dv, val, y0, y1, row, n);
+    s = c.template Sum<16>(m, dv.nd, f, s);
+  }
+  if (dv.nd > 24) {
+    DiaChunk<MV, NV> c;
+    c.template Load<24>(dv, val, y0, y1, row, n);
+    s = c.template Sum<24>(m, dv.nd, f, s);
+  }
+  return s;
+}
+
+// K1 on the diagonal layout of A (one thread per row): the same work as
+// PcgDirSpmvA, with the first row's operands loaded before the prologue.
+template <int kChunks>
+__global__ __launch_bounds__(kBlk) void PcgDirSpmvADia(int n, int par, DiaView dv, const double* __restrict__ av,
+                                                       const double* __restrict__ z, const double* __restrict__ p_in,
+                                                       double* __restrict__ p_out, double* __restrict__ Ap,
+                                                       double* __restrict__ part, double* __restrict__ ds,
+                                                       int* __restrict__ is, double* __restrict__ hist) {
+  const int G = gridDim.x;
+  DirLoads L;
+  L.Load(par, part, G, ds, is);
+  const int row0 = XcdBlock() * kBlk + threadIdx.x;
+  DiaChunk<double, 2> c0;
+  unsigned m0 = 0;
+  double z0 = 0.0, p0 = 0.0;
+  if (row0 < n) {
+    c0.Load<0>(dv, av, z, p_in, row0, n);
+    m0 = dv.mask[row0];
+    z0 = z[row0];
+    p0 = p_in[row0];
+  }
+  double beta;
+  if (!DirPrologue(par, L, G, ds, is, hist, beta)) return;
+  auto dir = [&](double zc, double pc) { return zc + beta * pc; };
+  double acc = 0.0;
+  for (int row = row0; row < n; row += G * kBlk) {
+    if (row != row0) {
+      c0.Load<0>(dv, av, z, p_in, row, n);
+      m0 = dv.mask[row];
+      z0 = z[row];
+      p0 = p_in[row];
+    }
+    double s = c0.Sum<0>(m0, dv.nd, dir, 0.0);
+    if constexpr (kChunks > 1) s = DiaRest<double, 2>(dv, av, z, p_in, row, n, m0, dir, s);
+    const double pr = z0 + beta * p0;
+    p_out[row] = pr;
+    Ap[row] = s;
+    acc += pr * s;
+  }
+  StorePartial(acc, part + kP1 * G);
+}
+
+// K2 on the diagonal layout of M (one thread per row): the same work as
+// PcgUpdateSpmvM, with the first row's operands loaded before the prologue.
+template <class MV, int kChunks>
+__global__ __launch_bounds__(kBlk) void PcgUpdateSpmvMDia(int n, int par, DiaView dv, const MV* __restrict__ mv,
+                                                          double* __restrict__ x, const double* __restrict__ r_in,
+                                                          double* __restrict__ r_out, const double* __restrict__ p,
+                                                          const double* __restrict__ Ap, double* __restrict__ z,
+                                                          double* __restrict__ part, double* __restrict__ ds,
+                                                          int* __restrict__ is) {
+  const int G = gridDim.x;
+  UpdateLoads L;
+  L.Load(par, part, G, ds, is);
+  const int row0 = XcdBlock() * kBlk + threadIdx.x;
+  DiaChunk<MV, 2> c0;
+  unsigned m0 = 0;
+  double x0 = 0.0, p0 = 0.0, r0 = 0.0, a0 = 0.0;
+  if (row0 < n) {
+    c0.template Load<0>(dv, mv, r_in, Ap, row0, n);
+    m0 = dv.mask[row0];
+    x0 = x[row0];
+    p0 = p[row0];
+    r0 = r_in[row0];
+    a0 = Ap[row0];
+  }
+  double alpha;
+  if (!UpdatePrologue(par, L, G, is, alpha)) return;
+  auto upd = [&](double rc, double ac) { return rc - alpha * ac; };
+  double rr = 0.0, rz = 0.0;
+  for (int row = row0; row < n; row += G * kBlk) {
+    if (row != row0) {
+      c0.template Load<0>(dv, mv, r_in, Ap, row, n);
+      m0 = dv.mask[row];
+      x0 = x[row];
+      p0 = p[row];
+      r0 = r_in[row];
+      a0 = Ap[row];
+    }
+    double zs = c0.template Sum<0>(m0, dv.nd, upd, 0.0);
+    if constexpr (kChunks > 1) zs = DiaRest<MV, 2>(dv, mv, r_in, Ap, row, n, m0, upd, zs);
+    const double ri = r0 - alpha * a0;
+    x[row] = x0 + alpha * p0;
+    r_out[row] = ri;
+    z[row] = zs;
+    rr += ri * ri;
+    rz += ri * zs;
+  }
+  StorePartial2(rr, rz, part + kP2 * G, part + kP4 * G);
+}
+
+// Plain Ap = A p with <p, Ap> partials on the diagonal layout (the solve's
+// A-SpMV replayed on its own for timing).
+template <int kChunks>
+__global__ __launch_bounds__(kBlk) void PcgSpmvADia(int n, DiaView dv, const double* __restrict__ av,
+                                                    const double* __restrict__ p, double* __restrict__ Ap,
+                                                    double* __restrict__ part) {
+  auto id = [](double v, double) { return v; };
+  double acc = 0.0;
+  for (int row = XcdBlock() * kBlk + threadIdx.x; row < n; row += gridDim.x * kBlk) {
+    DiaChunk<double, 1> c;
+    c.Load<0>(dv, av, p, p, row, n);
+    const unsigned m = dv.mask[row];
+    double s = c.Sum<0>(m, dv.nd, id, 0.0);
+    if constexpr (kChunks > 1) s = DiaRest<double, 1>(dv, av, p, p, row, n, m, id, s);
+    Ap[row] = s;
+    acc += p[row] * s;
+  }
+  StorePartial(acc, part + kP1 * gridDim.x);
+}
+
+// Diagonal layout construction: flag[col - row + n - 1] = 1 for every entry.
+__global__ void DiaMark(int n, const int* __restrict__ ptr, const int* __restrict__ col, int* __restrict__ flag) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  for (int k = ptr[row]; k < ptr[row + 1]; ++k) flag[int64_t(col[k]) - row + n - 1] = 1;
+}
+
+// off[pos[i]] = i - (n - 1) for every flagged i (pos: exclusive scan of flag).
+__global__ void DiaOffsets(int64_t len, int n, const int* __restrict__ flag, const int* __restrict__ pos,
+                           int* __restrict__ off) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < len && flag[i]) off[pos[i]] = int(i - (n - 1));
+}
+
+// Per row: the CSR position of each diagonal slot (-1 for none) and the mask.
+__global__ void DiaFill(int n, int nd, const int* __restrict__ ptr, const int* __restrict__ col,
+                        const int* __restrict__ pos, int* __restrict__ src, unsigned* __restrict__ mask) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  for (int d = 0; d < nd; ++d) src[size_t(d) * n + row] = -1;
+  unsigned m = 0;
+  for (int k = ptr[row]; k < ptr[row + 1]; ++k) {
+    const int d = pos[int64_t(col[k]) - row + n - 1];
+    src[size_t(d) * n + row] = k;
+    m |= 1u << d;
+  }
+  mask[row] = m;
 }
 
 // Host-launched refresh (backward_error.pyx:117-122): r = b - A x; <r, r>.
@@ -589,7 +888,8 @@ __global__ __launch_bounds__(kBlk) void PcgRefreshFinish(int par, const double* 
   is[kDone] = 0;
   is[par ? kPend1 : kPend0] = 0;  // finished here; K1 of the next iteration starts with beta = 0
   bool stop = false;
-  EndIteration(par, r_z, ds[kErrPre], true, ds, is, hist, stop, true);
+  EndIteration(par, r_z, ds[kErrPre], true, 0.0, is[par ? kCount1 : kCount0], is[kMax], ds[kTol], ds, is, hist,
+               stop, true);
 }
 
 // Final true residual (backward_error.pyx:136-139): partials of ||b - A x||^2.
@@ -809,11 +1109,69 @@ void SellValues(Ctx& ctx, DSell& s, const DCsr& m) {
   if (s.val32.get()) SellGather<float><<<g, 256, 0, ctx.stream>>>(s.len, s.src.get(), m.val32.get(), s.val32.get());
 }
 
+DDia DiaFromCsr(Ctx& ctx, const DCsr& m, bool fp32) {
+  DDia d;
+  d.n = m.n;
+  if (m.n <= 0 || m.nnz <= 0 || m.n > (int64_t(1) << 29)) return d;
+  const int n = (int)m.n;
+  const int64_t len = 2 * m.n - 1;  // offsets -(n-1) .. n-1
+  auto flag = ctx.Alloc<int>(len);
+  auto pos = ctx.Alloc<int>(len + 1);
+  SPAI_HIP_CHECK(hipMemsetAsync(flag.get(), 0, sizeof(int) * len, ctx.stream));
+  const unsigned g = (unsigned)((m.n + 255) / 256);
+  DiaMark<<<g, 256, 0, ctx.stream>>>(n, m.ptr.get(), m.col.get(), flag.get());
+  const int64_t nd = ScanCounts(ctx, flag.get(), len, pos.get());
+  if (nd > kDiaMaxDiagonals || double(nd) * double(m.n) > 1.5 * double(m.nnz)) return d;
+  auto off = ctx.Alloc<int>(nd);
+  DiaOffsets<<<(unsigned)((len + 255) / 256), 256, 0, ctx.stream>>>(len, n, flag.get(), pos.get(), off.get());
+  d.off.resize(nd);
+  ctx.ReadBack(d.off.data(), off.get(), sizeof(int) * nd);
+  d.src = ctx.Alloc<int>(nd * m.n);
+  d.mask = ctx.Alloc<unsigned>(m.n);
+  DiaFill<<<g, 256, 0, ctx.stream>>>(n, (int)nd, m.ptr.get(), m.col.get(), pos.get(), d.src.get(), d.mask.get());
+  d.val = ctx.Alloc<double>(nd * m.n);
+  if (fp32) d.val32 = ctx.Alloc<float>(nd * m.n);
+  d.nd = nd;
+  DiaValues(ctx, d, m);
+  return d;
+}
+
+void DiaValues(Ctx& ctx, DDia& d, const DCsr& m) {
+  const int64_t len = d.nd * d.n;
+  if (len == 0) return;
+  const unsigned g = (unsigned)((len + 255) / 256);
+  SellGather<double><<<g, 256, 0, ctx.stream>>>(len, d.src.get(), m.val.get(), d.val.get());
+  if (d.val32.get()) SellGather<float><<<g, 256, 0, ctx.stream>>>(len, d.src.get(), m.val32.get(), d.val32.get());
+}
+
+namespace {
+
+DiaView ViewOf(const DDia& d) {
+  DiaView v{};
+  for (int64_t i = 0; i < d.nd; ++i) v.off[i] = d.off[i];
+  v.mask = d.mask.get();
+  v.nd = (int)d.nd;
+  return v;
+}
+
+// The cap on the layout choice from SPAI_PCG_LAYOUT ("csr", "sell", "auto").
+PcgLayout LayoutCap() {
+  const char* e = std::getenv("SPAI_PCG_LAYOUT");
+  if (!e || !*e || std::strcmp(e, "auto") == 0) return PcgLayout::kDia;
+  if (std::strcmp(e, "sell") == 0) return PcgLayout::kSell;
+  if (std::strcmp(e, "csr") == 0) return PcgLayout::kCsr;
+  throw std::invalid_argument("SPAI_PCG_LAYOUT must be csr, sell or auto");
+}
+
+}  // namespace
+
 PcgSolver::PcgSolver(Ctx& ctx, const DCsr& A, const DCsr& M, bool m_fp32)
     : ctx_(ctx), A_(A), M_(M), m_fp32_(m_fp32), n_(A.n) {
   if (M.n != A.n) throw std::invalid_argument("A and M must have the same shape");
   if (m_fp32 && !M.val32.get()) throw std::invalid_argument("FP32 copy of M missing");
   grid_ = Grid(n_, kBlk, ctx.num_cus);
+  // the iteration kernels' prologues hold at most kMaxPartials partials a thread
+  if (grid_ > unsigned(kMaxPartials * kBlk)) grid_ = unsigned(kMaxPartials * kBlk);
   x_ = ctx.Alloc<double>(n_);
   r_ = ctx.Alloc<double>(2 * n_);
   z_ = ctx.Alloc<double>(n_);
@@ -822,16 +1180,29 @@ PcgSolver::PcgSolver(Ctx& ctx, const DCsr& A, const DCsr& M, bool m_fp32)
   part_ = ctx.Alloc<double>(size_t(kNumP) * grid_);
   scal_ = ctx.Alloc<double>(kNumD);
   ist_ = ctx.Alloc<int>(kNumI);
-  // one thread per row: the iteration kernels read SELL copies (coalesced)
-  sell_a_ = SubgroupFor(A.n, A.nnz) == 1;
-  sell_m_ = SubgroupFor(M.n, M.nnz) == 1;
-  if (sell_a_) a_sell_ = SellFromCsr(ctx, A, false);
-  if (sell_m_) m_sell_ = SellFromCsr(ctx, M, m_fp32);
+  // the iteration kernels' row layouts: diagonals (no index loads) when the
+  // matrix lies on few diagonals, else SELL copies for short rows
+  // (coalesced, one thread per row), else CSR with lanes per row
+  const PcgLayout cap = LayoutCap();
+  auto choose = [&](const DCsr& m, bool fp32, DSell& sell, DDia& dia) {
+    if (cap == PcgLayout::kDia) {
+      dia = DiaFromCsr(ctx, m, fp32);
+      if (dia.nd > 0) return PcgLayout::kDia;
+    }
+    if (cap != PcgLayout::kCsr && SubgroupFor(m.n, m.nnz) == 1) {
+      sell = SellFromCsr(ctx, m, fp32);
+      return PcgLayout::kSell;
+    }
+    return PcgLayout::kCsr;
+  };
+  a_layout_ = choose(A, false, a_sell_, a_dia_);
+  m_layout_ = choose(M, m_fp32, m_sell_, m_dia_);
   ctx.Sync();
 }
 
 void PcgSolver::RefreshPreconditioner() {
-  if (sell_m_) SellValues(ctx_, m_sell_, M_);
+  if (m_layout_ == PcgLayout::kSell) SellValues(ctx_, m_sell_, M_);
+  if (m_layout_ == PcgLayout::kDia) DiaValues(ctx_, m_dia_, M_);
 }
 
 PcgSolver::~PcgSolver() { DropGraphs(); }
@@ -865,6 +1236,16 @@ void PcgSolver::DropGraphs() {
     } break;                      \
   }
 
+// Diagonal kernels: one chunk of 8 diagonals (nd <= 8) or up to four.
+#define SPAI_DIA_DISPATCH(nd, ...)   \
+  if ((nd) <= 8) {                   \
+    constexpr int kChunks = 1;       \
+    __VA_ARGS__;                     \
+  } else {                           \
+    constexpr int kChunks = 4;       \
+    __VA_ARGS__;                     \
+  }
+
 void PcgSolver::Record(int first_parity, int64_t iterations) {
   const int sga = SubgroupFor(A_.n, A_.nnz), sgm = SubgroupFor(M_.n, M_.nnz);
   hipStream_t st = ctx_.stream;
@@ -875,7 +1256,11 @@ void PcgSolver::Record(int first_parity, int64_t iterations) {
     double* p_out = p_.get() + size_t(par ^ 1) * n_;
     const double* r_in = r_.get() + size_t(par) * n_;
     double* r_out = r_.get() + size_t(par ^ 1) * n_;
-    if (sell_a_) {
+    if (a_layout_ == PcgLayout::kDia) {
+      SPAI_DIA_DISPATCH(a_dia_.nd, (PcgDirSpmvADia<kChunks><<<grid_, kBlk, 0, st>>>(
+                                       n, par, ViewOf(a_dia_), a_dia_.val.get(), z_.get(), p_in, p_out, ap_.get(),
+                                       part_.get(), scal_.get(), ist_.get(), hist_.get())));
+    } else if (a_layout_ == PcgLayout::kSell) {
       PcgDirSpmvA<1, SellRows><<<grid_, kBlk, 0, st>>>(n, par, SellRows{a_sell_.off.get()}, a_sell_.col.get(),
                                                        a_sell_.val.get(), z_.get(), p_in, p_out, ap_.get(),
                                                        part_.get(), scal_.get(), ist_.get(), hist_.get());
@@ -884,11 +1269,19 @@ void PcgSolver::Record(int first_parity, int64_t iterations) {
                                 n, par, CsrRows{A_.ptr.get()}, A_.col.get(), A_.val.get(), z_.get(), p_in, p_out,
                                 ap_.get(), part_.get(), scal_.get(), ist_.get(), hist_.get())));
     }
-    if (sell_m_ && m_fp32_) {
+    if (m_layout_ == PcgLayout::kDia && m_fp32_) {
+      SPAI_DIA_DISPATCH(m_dia_.nd, (PcgUpdateSpmvMDia<float, kChunks><<<grid_, kBlk, 0, st>>>(
+                                       n, par, ViewOf(m_dia_), m_dia_.val32.get(), x_.get(), r_in, r_out, p_out,
+                                       ap_.get(), z_.get(), part_.get(), scal_.get(), ist_.get())));
+    } else if (m_layout_ == PcgLayout::kDia) {
+      SPAI_DIA_DISPATCH(m_dia_.nd, (PcgUpdateSpmvMDia<double, kChunks><<<grid_, kBlk, 0, st>>>(
+                                       n, par, ViewOf(m_dia_), m_dia_.val.get(), x_.get(), r_in, r_out, p_out,
+                                       ap_.get(), z_.get(), part_.get(), scal_.get(), ist_.get())));
+    } else if (m_layout_ == PcgLayout::kSell && m_fp32_) {
       PcgUpdateSpmvM<1, float, SellRows><<<grid_, kBlk, 0, st>>>(
           n, par, SellRows{m_sell_.off.get()}, m_sell_.col.get(), m_sell_.val32.get(), x_.get(), r_in, r_out, p_out,
           ap_.get(), z_.get(), part_.get(), scal_.get(), ist_.get());
-    } else if (sell_m_) {
+    } else if (m_layout_ == PcgLayout::kSell) {
       PcgUpdateSpmvM<1, double, SellRows><<<grid_, kBlk, 0, st>>>(
           n, par, SellRows{m_sell_.off.get()}, m_sell_.col.get(), m_sell_.val.get(), x_.get(), r_in, r_out, p_out,
           ap_.get(), z_.get(), part_.get(), scal_.get(), ist_.get());
@@ -1023,7 +1416,10 @@ double PcgSolver::TimeSpmvA(int reps) {
   for (auto& e : ev) SPAI_HIP_CHECK(hipEventCreate(&e));
   for (int i = 0; i < reps; ++i) {
     SPAI_HIP_CHECK(hipEventRecord(ev[2 * i], st));
-    if (sell_a_)
+    if (a_layout_ == PcgLayout::kDia) {
+      SPAI_DIA_DISPATCH(a_dia_.nd, (PcgSpmvADia<kChunks><<<grid_, kBlk, 0, st>>>(
+                                       n, ViewOf(a_dia_), a_dia_.val.get(), x_.get(), ap_.get(), part_.get())));
+    } else if (a_layout_ == PcgLayout::kSell)
       PcgSpmvASell<<<grid_, kBlk, 0, st>>>(n, SellRows{a_sell_.off.get()}, a_sell_.col.get(), a_sell_.val.get(),
                                            x_.get(), ap_.get(), part_.get());
     else

@@ -30,6 +30,23 @@ struct DSell {
   DBuf<float> val32;
 };
 
+// Diagonal layout of a CSR matrix whose entries lie on few distinct diagonals
+// (grid stencils): diagonal d has offset off[d] = col - row (ascending), and
+// its value in row r sits at d * n + r (0 where the row has no entry there);
+// bit d of mask[r] says whether row r has one.  One thread per row reads it
+// with every address known before any load returns (values, mask and the
+// vector at r + off[d]), and the loads of a wavefront are contiguous.  src[k]
+// is the CSR position of slot k (-1 for none), used to refresh the values.
+constexpr int kDiaMaxDiagonals = 32;
+struct DDia {
+  int64_t n = 0, nd = 0;  // nd = 0: the matrix does not take this layout
+  std::vector<int> off;   // host copy (the kernels take it as an argument)
+  DBuf<int> src;
+  DBuf<unsigned> mask;
+  DBuf<double> val;
+  DBuf<float> val32;
+};
+
 DCsr CsrFromHost(Ctx& ctx, int64_t n, int64_t nnz, const int64_t* ptr, const int64_t* col, const double* val);
 // CSR of the matrix stored as CSC in m (i.e. the transpose of its CSC arrays).
 DCsr CsrFromCsc(Ctx& ctx, const DMat& m);
@@ -41,6 +58,17 @@ void CopyValues(Ctx& ctx, DCsr& dst, const DCsr& src);
 DSell SellFromCsr(Ctx& ctx, const DCsr& m, bool fp32);
 // Re-gathers the values of s from m (same pattern).
 void SellValues(Ctx& ctx, DSell& s, const DCsr& m);
+// Diagonal layout of m when its entries lie on at most kDiaMaxDiagonals
+// diagonals and the layout has at most 1.5 nnz slots (else nd = 0); values
+// (and an FP32 copy when fp32) gathered from m.
+DDia DiaFromCsr(Ctx& ctx, const DCsr& m, bool fp32);
+// Re-gathers the values of d from m (same pattern).
+void DiaValues(Ctx& ctx, DDia& d, const DCsr& m);
+// Row layout of the PCG iteration kernels: CSR (SG lanes per row), sliced
+// ELLPACK or diagonals (one thread per row).  The environment variable
+// SPAI_PCG_LAYOUT caps the choice: "csr", "sell", or "auto" (the default:
+// diagonals when the matrix takes them, else SELL for short rows, else CSR).
+enum class PcgLayout : int { kCsr = 0, kSell = 1, kDia = 2 };
 // y = A x (methods_cython/backward_error.pyx:27-34 _multiply).
 void SpMV(Ctx& ctx, const DCsr& A, const double* x, double* y);
 
@@ -58,6 +86,8 @@ class PcgSolver {
   // Average device time of one iteration's A-SpMV kernel over the last solve
   // (microseconds, from HIP events around a standalone replay).
   double TimeSpmvA(int reps);
+  PcgLayout LayoutA() const { return a_layout_; }
+  PcgLayout LayoutM() const { return m_layout_; }
 
  private:
   void Record(int first_parity, int64_t iterations);
@@ -68,8 +98,9 @@ class PcgSolver {
   bool m_fp32_;
   int64_t n_;
   unsigned grid_ = 1;
-  bool sell_a_ = false, sell_m_ = false;  // one-thread-per-row iteration kernels on SELL copies
+  PcgLayout a_layout_ = PcgLayout::kCsr, m_layout_ = PcgLayout::kCsr;
   DSell a_sell_, m_sell_;
+  DDia a_dia_, m_dia_;
   DBuf<double> x_, r_, z_, p_, ap_, part_, hist_;
   DBuf<double> scal_;
   DBuf<int> ist_;

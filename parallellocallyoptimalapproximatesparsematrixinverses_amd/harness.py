@@ -44,6 +44,8 @@ def _declare(L):
                                  ctypes.POINTER(ctypes.c_int64), _lib.F64P]
     L.spai_pcg_matvec.restype = ctypes.c_int64
     L.spai_pcg_matvec.argtypes = [ctypes.c_void_p, _lib.F64P, _lib.F64P]
+    L.spai_pcg_layouts.restype = ctypes.c_int64
+    L.spai_pcg_layouts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]
     L._harness_declared = True
     return L
 
@@ -100,6 +102,13 @@ class _DeviceSystem:
         if L.spai_pcg_matvec(self._h, x.ctypes.data_as(_lib.F64P), y.ctypes.data_as(_lib.F64P)) < 0:
             raise ValueError(_lib.last_error())
         return y
+
+    def layouts(self):
+        """Row layouts of the PCG iteration kernels for (A, M): "csr", "sell" or "dia"."""
+        out = (ctypes.c_int64 * 2)()
+        if _declare(_lib.load()).spai_pcg_layouts(self._h, out) < 0:
+            raise ValueError(_lib.last_error())
+        return tuple(_lib.PCG_LAYOUTS[int(v)] for v in out)
 
     def close(self):
         if getattr(self, "_h", None):
