@@ -183,8 +183,8 @@ int64_t spai_session_pcg(SpaiSession* session, int64_t max_iterations, double to
 int64_t spai_session_pcg_output(SpaiSession* session, double* history, double* x);
 /* Copies the built M and its history to a host Result (release with free_mem). */
 struct Result spai_session_result(SpaiSession* session);
-/* Per-kernel timing of the last PCG iteration loop: average microseconds of
- * one SpMV with A (HIP events around a replay of the iteration graph). */
+/* Average microseconds of one SpMV with A (plus <p, Ap>) in the layout the
+ * session's PCG reads: one event pair around 20 back-to-back launches. */
 double spai_session_pcg_iteration_us(SpaiSession* session);
 /* Row layouts of the session's PCG iteration kernels, for tests and
  * measurement: out[0] for A, out[1] for M; 0 = CSR, 1 = sliced ELLPACK,

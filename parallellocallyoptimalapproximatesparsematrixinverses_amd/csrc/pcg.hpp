@@ -83,8 +83,8 @@ class PcgSolver {
   // After M's values changed in place (same pattern): refresh the solver's
   // copies of them.  Recorded graphs stay valid.
   void RefreshPreconditioner();
-  // Average device time of one iteration's A-SpMV kernel over the last solve
-  // (microseconds, from HIP events around a standalone replay).
+  // Average device time of the A-SpMV kernel in the solve's layout
+  // (microseconds; one event pair around `reps` back-to-back launches).
   double TimeSpmvA(int reps);
   PcgLayout LayoutA() const { return a_layout_; }
   PcgLayout LayoutM() const { return m_layout_; }
