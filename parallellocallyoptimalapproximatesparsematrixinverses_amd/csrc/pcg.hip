@@ -201,14 +201,13 @@ __device__ __forceinline__ double EndIteration(int prev, double r_z_new, double 
 }
 
 // A thread's share of G <= kMaxPartials * kBlk block partials (entries
-// threadIdx.x + u kBlk), loaded without waiting on anything.
+// threadIdx.x + u kBlk), loaded without waiting on anything: the loads take
+// no per-lane branch (lanes past G read entry G - 1, which AddPartials
+// skips), so none of them is waited on before the others are issued.
 constexpr int kMaxPartials = 4;
 __device__ __forceinline__ void LoadPartials(const double* __restrict__ part, int G, double (&v)[kMaxPartials]) {
 #pragma unroll
-  for (int u = 0; u < kMaxPartials; ++u) {
-    const int i = threadIdx.x + u * kBlk;
-    v[u] = i < G ? part[i] : 0.0;
-  }
+  for (int u = 0; u < kMaxPartials; ++u) v[u] = part[min(int(threadIdx.x) + u * kBlk, G - 1)];
 }
 
 // This thread's entries of a partial array in SumPartials' order.
