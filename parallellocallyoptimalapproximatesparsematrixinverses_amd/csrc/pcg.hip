@@ -53,6 +53,8 @@ enum ISlot : int {
   kPend1,
   kNeedRefresh,
   kIter,
+  kFinalDone,
+  kRefreshes,
   kNumI
 };
 // partial-sum arrays, G doubles each
@@ -308,31 +310,6 @@ __device__ __forceinline__ bool UpdatePrologue(int par, const UpdateLoads& L, in
     }
   }
   return !brk;
-}
-
-// r = b, x = 0, z = M b, p = 0; partials of <r, r> and <r, z>.
-template <int SG, class MV>
-__global__ __launch_bounds__(kBlk) void PcgInit(int n, const int* __restrict__ mp, const int* __restrict__ mc,
-                                                const MV* __restrict__ mv, const double* __restrict__ b,
-                                                double* __restrict__ x, double* __restrict__ r,
-                                                double* __restrict__ z, double* __restrict__ p0,
-                                                double* __restrict__ part) {
-  const int groups = kBlk / SG, sl = threadIdx.x % SG;
-  double rr = 0.0, rz = 0.0;
-  for (int row = blockIdx.x * groups + threadIdx.x / SG; row < n; row += gridDim.x * groups) {
-    const double zr = RowDot<SG, MV>(mp, mc, mv, b, row, sl);
-    if (sl == 0) {
-      const double br = b[row];
-      r[row] = br;
-      x[row] = 0.0;
-      z[row] = zr;
-      p0[row] = 0.0;
-      rr += br * br;
-      rz += br * zr;
-    }
-  }
-  StorePartial(rr, part + kPA * gridDim.x);
-  StorePartial(rz, part + kPB * gridDim.x);
 }
 
 // ||b||, r.z, history[0] = 1 (backward_error.pyx:91-100).
@@ -837,16 +814,75 @@ __global__ void DiaFill(int n, int nd, const int* __restrict__ ptr, const int* _
   mask[row] = m;
 }
 
-// Host-launched refresh (backward_error.pyx:117-122): r = b - A x; <r, r>.
-template <int SG>
-__global__ __launch_bounds__(kBlk) void PcgRefresh(int n, const int* __restrict__ ap_, const int* __restrict__ ac,
-                                                   const double* __restrict__ av, const double* __restrict__ x,
-                                                   const double* __restrict__ b, double* __restrict__ r,
-                                                   double* __restrict__ part) {
+// Row dot products of the once-per-solve kernels (start, refresh, final
+// residual): CSR with SG lanes per row, or the diagonal layout with one
+// thread per row.  Both add a row's entries in column order.
+template <int SG, class MV>
+struct CsrDot {
+  static constexpr int kSG = SG;
+  const int* __restrict__ ptr;
+  const int* __restrict__ col;
+  const MV* __restrict__ val;
+  __device__ __forceinline__ double operator()(const double* __restrict__ x, int row, int sl, int) const {
+    return RowDot<SG, MV>(ptr, col, val, x, row, sl);
+  }
+};
+
+template <class MV, int kChunks>
+struct DiaDot {
+  static constexpr int kSG = 1;
+  DiaView dv;
+  const MV* __restrict__ val;
+  __device__ __forceinline__ double operator()(const double* __restrict__ x, int row, int, int n) const {
+    auto id = [](double v, double) { return v; };
+    DiaChunk<MV, 1> c;
+    c.template Load<0>(dv, val, x, x, row, n);
+    const unsigned m = dv.mask[row];
+    double s = c.template Sum<0>(m, dv.nd, id, 0.0);
+    if constexpr (kChunks > 1) s = DiaRest<MV, 1>(dv, val, x, x, row, n, m, id, s);
+    return s;
+  }
+};
+
+// r = b, x = 0, z = M b, p = 0; partials of <r, r> and <r, z>.
+template <class Dot>
+__global__ __launch_bounds__(kBlk) void PcgInit(int n, Dot m, const double* __restrict__ b, double* __restrict__ x,
+                                                double* __restrict__ r, double* __restrict__ z,
+                                                double* __restrict__ p0, double* __restrict__ part) {
+  constexpr int SG = Dot::kSG;
+  const int groups = kBlk / SG, sl = threadIdx.x % SG;
+  double rr = 0.0, rz = 0.0;
+  for (int row = blockIdx.x * groups + threadIdx.x / SG; row < n; row += gridDim.x * groups) {
+    const double zr = m(b, row, sl, n);
+    if (sl == 0) {
+      const double br = b[row];
+      r[row] = br;
+      x[row] = 0.0;
+      z[row] = zr;
+      p0[row] = 0.0;
+      rr += br * br;
+      rz += br * zr;
+    }
+  }
+  StorePartial(rr, part + kPA * gridDim.x);
+  StorePartial(rz, part + kPB * gridDim.x);
+}
+
+// The refresh of backward_error.pyx:116-122, enqueued after every batch of
+// iterations and run only when K1 found the tolerance crossing (kNeedRefresh):
+// r = b - A x, <r, r>.  The last K2 had parity (kIter - 1) & 1 and left r' in
+// buffer ((kIter - 1) & 1) ^ 1, which is refreshed in place.
+template <class Dot>
+__global__ __launch_bounds__(kBlk) void PcgRefresh(int n, Dot a, const double* __restrict__ x,
+                                                   const double* __restrict__ b, double* __restrict__ r2,
+                                                   double* __restrict__ part, const int* __restrict__ is) {
+  if (!is[kNeedRefresh]) return;
+  constexpr int SG = Dot::kSG;
+  double* __restrict__ r = r2 + size_t(((is[kIter] - 1) & 1) ^ 1) * size_t(n);
   const int groups = kBlk / SG, sl = threadIdx.x % SG;
   double acc = 0.0;
   for (int row = blockIdx.x * groups + threadIdx.x / SG; row < n; row += gridDim.x * groups) {
-    const double s = RowDot<SG, double>(ap_, ac, av, x, row, sl);
+    const double s = a(x, row, sl, n);
     if (sl == 0) {
       const double ri = b[row] - s;
       r[row] = ri;
@@ -857,18 +893,20 @@ __global__ __launch_bounds__(kBlk) void PcgRefresh(int n, const int* __restrict_
 }
 
 // After the refresh: error from the true residual, z = M r, <r, z>.
-template <int SG, class MV>
-__global__ __launch_bounds__(kBlk) void PcgRefreshSpmvM(int n, const int* __restrict__ mp, const int* __restrict__ mc,
-                                                        const MV* __restrict__ mv, const double* __restrict__ r,
+template <class Dot>
+__global__ __launch_bounds__(kBlk) void PcgRefreshSpmvM(int n, Dot m, const double* __restrict__ r2,
                                                         double* __restrict__ z, double* __restrict__ part,
-                                                        double* __restrict__ ds) {
+                                                        double* __restrict__ ds, const int* __restrict__ is) {
+  if (!is[kNeedRefresh]) return;
+  constexpr int SG = Dot::kSG;
+  const double* __restrict__ r = r2 + size_t(((is[kIter] - 1) & 1) ^ 1) * size_t(n);
   const int G = gridDim.x;
   const double err = sqrt(SumPartials(part + kPR * G, G)) / ds[kBnorm];
   if (blockIdx.x == 0 && threadIdx.x == 0) ds[kErrPre] = err;
   const int groups = kBlk / SG, sl = threadIdx.x % SG;
   double acc = 0.0;
   for (int row = blockIdx.x * groups + threadIdx.x / SG; row < n; row += gridDim.x * groups) {
-    const double s = RowDot<SG, MV>(mp, mc, mv, r, row, sl);
+    const double s = m(r, row, sl, n);
     if (sl == 0) {
       z[row] = s;
       acc += r[row] * s;
@@ -878,28 +916,39 @@ __global__ __launch_bounds__(kBlk) void PcgRefreshSpmvM(int n, const int* __rest
 }
 
 // End of the refreshed iteration (beta = 0); resumes the loop unless it stops.
-__global__ __launch_bounds__(kBlk) void PcgRefreshFinish(int par, const double* __restrict__ part, int G,
+__global__ __launch_bounds__(kBlk) void PcgRefreshFinish(const double* __restrict__ part, int G,
                                                          double* __restrict__ ds, int* __restrict__ is,
                                                          double* __restrict__ hist) {
+  if (!is[kNeedRefresh]) return;
+  const int par = (is[kIter] - 1) & 1;
   const double r_z = SumPartials(part + kP4 * G, G);
   if (threadIdx.x != 0) return;
   is[kNeedRefresh] = 0;
   is[kDone] = 0;
+  is[kRefreshes] = is[kRefreshes] + 1;
   is[par ? kPend1 : kPend0] = 0;  // finished here; K1 of the next iteration starts with beta = 0
   bool stop = false;
   EndIteration(par, r_z, ds[kErrPre], true, 0.0, is[par ? kCount1 : kCount0], is[kMax], ds[kTol], ds, is, hist,
                stop, true);
 }
 
-// Final true residual (backward_error.pyx:136-139): partials of ||b - A x||^2.
-template <int SG>
-__global__ __launch_bounds__(kBlk) void PcgFinal(int n, const int* __restrict__ ap_, const int* __restrict__ ac,
-                                                 const double* __restrict__ av, const double* __restrict__ x,
-                                                 const double* __restrict__ b, double* __restrict__ part) {
+// Final true residual (backward_error.pyx:136-139): partials of ||b - A x||^2,
+// enqueued after every batch and run once the solve has ended (done, no
+// refresh pending, not yet finished).
+__device__ __forceinline__ bool FinalDue(const int* __restrict__ is) {
+  return is[kDone] && !is[kNeedRefresh] && !is[kFinalDone];
+}
+
+template <class Dot>
+__global__ __launch_bounds__(kBlk) void PcgFinal(int n, Dot a, const double* __restrict__ x,
+                                                 const double* __restrict__ b, double* __restrict__ part,
+                                                 const int* __restrict__ is) {
+  if (!FinalDue(is)) return;
+  constexpr int SG = Dot::kSG;
   const int groups = kBlk / SG, sl = threadIdx.x % SG;
   double acc = 0.0;
   for (int row = blockIdx.x * groups + threadIdx.x / SG; row < n; row += gridDim.x * groups) {
-    const double s = RowDot<SG, double>(ap_, ac, av, x, row, sl);
+    const double s = a(x, row, sl, n);
     if (sl == 0) {
       const double ri = b[row] - s;
       acc += ri * ri;
@@ -909,10 +958,11 @@ __global__ __launch_bounds__(kBlk) void PcgFinal(int n, const int* __restrict__ 
 }
 
 // Overwrites the last history entry with the true residual; status 1 / 3
-// (backward_error.pyx:139-143).
+// (backward_error.pyx:139-143); marks the solve finished.
 __global__ __launch_bounds__(kBlk) void PcgFinalFinish(const double* __restrict__ part, int G,
                                                        const double* __restrict__ ds, int* __restrict__ is,
                                                        double* __restrict__ hist) {
+  if (!FinalDue(is)) return;
   const double rr = SumPartials(part + kPF * G, G);
   if (threadIdx.x != 0) return;
   const int count = is[kCountLatest];
@@ -922,6 +972,7 @@ __global__ __launch_bounds__(kBlk) void PcgFinalFinish(const double* __restrict_
     is[kStatus] = 3;
   else if (h <= ds[kTol])
     is[kStatus] = 1;
+  is[kFinalDone] = 1;
 }
 
 // Plain Ap = A p with <p, Ap> partials (timing of the SpMV itself).
@@ -1162,6 +1213,25 @@ PcgLayout LayoutCap() {
   throw std::invalid_argument("SPAI_PCG_LAYOUT must be csr, sell or auto");
 }
 
+// f(dot): the row-dot policy of matrix m for the once-per-solve kernels
+// (diagonals when the solver reads m in that layout, else CSR with the lanes
+// per row of its mean row length); val / dval are its CSR / diagonal values.
+template <class MV, class F>
+void WithDot(const DCsr& m, const MV* val, const DDia& d, const MV* dval, PcgLayout layout, F&& f) {
+  if (layout == PcgLayout::kDia) {
+    if (d.nd <= 8)
+      f(DiaDot<MV, 1>{ViewOf(d), dval});
+    else
+      f(DiaDot<MV, 4>{ViewOf(d), dval});
+    return;
+  }
+  switch (SubgroupFor(m.n, m.nnz)) {
+    case 1: f(CsrDot<1, MV>{m.ptr.get(), m.col.get(), val}); break;
+    case 16: f(CsrDot<16, MV>{m.ptr.get(), m.col.get(), val}); break;
+    default: f(CsrDot<32, MV>{m.ptr.get(), m.col.get(), val}); break;
+  }
+}
+
 }  // namespace
 
 PcgSolver::PcgSolver(Ctx& ctx, const DCsr& A, const DCsr& M, bool m_fp32)
@@ -1308,16 +1378,16 @@ int PcgSolver::Solve(const double* b_dev, int64_t max_iterations, double toleran
   b_ = b_dev;
   // the solve's scalars, written by a kernel (no copy from pageable memory)
   PcgSetScalars<<<1, kWave, 0, st>>>(scal_.get(), ist_.get(), tolerance, (int)max_iterations);
-  const int sgm = SubgroupFor(M_.n, M_.nnz), sga = SubgroupFor(A_.n, A_.nnz);
-  if (m_fp32_) {
-    SPAI_SG_DISPATCH(sgm, (PcgInit<SG, float><<<grid_, kBlk, 0, st>>>(n, M_.ptr.get(), M_.col.get(), M_.val32.get(),
-                                                                      b_dev, x_.get(), r_.get(), z_.get(), p_.get(),
-                                                                      part_.get())));
-  } else {
-    SPAI_SG_DISPATCH(sgm, (PcgInit<SG, double><<<grid_, kBlk, 0, st>>>(n, M_.ptr.get(), M_.col.get(), M_.val.get(),
-                                                                       b_dev, x_.get(), r_.get(), z_.get(),
-                                                                       p_.get(), part_.get())));
-  }
+  auto with_a = [&](auto&& f) { WithDot<double>(A_, A_.val.get(), a_dia_, a_dia_.val.get(), a_layout_, f); };
+  auto with_m = [&](auto&& f) {
+    if (m_fp32_)
+      WithDot<float>(M_, M_.val32.get(), m_dia_, m_dia_.val32.get(), m_layout_, f);
+    else
+      WithDot<double>(M_, M_.val.get(), m_dia_, m_dia_.val.get(), m_layout_, f);
+  };
+  with_m([&](auto dot) {
+    PcgInit<decltype(dot)><<<grid_, kBlk, 0, st>>>(n, dot, b_dev, x_.get(), r_.get(), z_.get(), p_.get(), part_.get());
+  });
   PcgInitFinish<<<1, kBlk, 0, st>>>(part_.get(), (int)grid_, scal_.get(), ist_.get(), hist_.get());
   if (!exec_[0]) {
     for (int par = 0; par < 2; ++par) {
@@ -1339,65 +1409,61 @@ int PcgSolver::Solve(const double* b_dev, int64_t max_iterations, double toleran
   hst[kIter] = 0;
   // The device ends the solve by itself: every iteration it runs either
   // appends a history entry or stops (limit, tolerance crossing pending a
-  // refresh, breakdown), so the loop runs until the device reports done and
-  // each pass of it must append at least one entry.  Launches past the stop
-  // return at once and are not counted as progress.
+  // refresh, breakdown).  Each round launches a batch of iterations, then the
+  // refresh (runs only when K1 paused at the tolerance crossing) and the final
+  // true residual (runs only once the solve has ended), then reads the state
+  // back once: a solve whose crossing falls in its first batch and ends at
+  // the refresh takes one read-back.  Launches past the stop return at once.
   bool tail = false, first = true;
   int64_t ahead = 1;
-  while (!hst[kDone]) {
-    const int count_before = hst[kCountLatest];
-    int64_t full = 0, rest = 0;
-    if (first) {
-      // as many iterations as the previous solve ran, plus the one whose K1
-      // finds the tolerance crossing (at least one chunk, at most this
-      // solve's limit), whole chunks and then tail chunks, before looking at
-      // the state (iterations after the stop return at once)
-      const int64_t want =
-          std::min<int64_t>(max_iterations + 1, std::max<int64_t>(kChunk, last_iterations_ + 1));
-      full = want / kChunk;
-      rest = (want % kChunk + kTailChunk - 1) / kTailChunk;
-      first = false;
-    } else if (tail) {
-      // past the refresh: tail chunks, doubling each round in case the solve
-      // goes on
-      rest = ahead;
-      ahead = std::min<int64_t>(2 * ahead, 64);
-    } else {
-      full = 1;  // then one chunk at a time
-    }
-    // chunks hold an even number of iterations: each starts at the same parity
-    const int par0 = hst[kIter] & 1;
-    for (int64_t c = 0; c < full; ++c) SPAI_HIP_CHECK(hipGraphLaunch(exec_[par0], st));
-    for (int64_t c = 0; c < rest; ++c) SPAI_HIP_CHECK(hipGraphLaunch(tail_exec_[par0], st));
-    ctx_.ReadBack(hst, ist_.get(), sizeof(hst));
-    if (hst[kNeedRefresh]) {
-      // the last K2 had parity par and left r' in buffer par ^ 1
-      const int par = (hst[kIter] - 1) & 1;
-      double* r_cur = r_.get() + size_t(par ^ 1) * n_;
-      SPAI_SG_DISPATCH(sga, (PcgRefresh<SG><<<grid_, kBlk, 0, st>>>(n, A_.ptr.get(), A_.col.get(), A_.val.get(),
-                                                                    x_.get(), b_dev, r_cur, part_.get())));
-      if (m_fp32_) {
-        SPAI_SG_DISPATCH(sgm, (PcgRefreshSpmvM<SG, float><<<grid_, kBlk, 0, st>>>(
-                                  n, M_.ptr.get(), M_.col.get(), M_.val32.get(), r_cur, z_.get(), part_.get(),
-                                  scal_.get())));
+  for (;;) {
+    const int count_before = hst[kCountLatest], refreshes_before = hst[kRefreshes];
+    if (!hst[kDone]) {
+      int64_t full = 0, rest = 0;
+      if (first) {
+        // as many iterations as the previous solve ran, plus the one whose K1
+        // finds the tolerance crossing (at least one chunk, at most this
+        // solve's limit), whole chunks and then tail chunks
+        const int64_t want =
+            std::min<int64_t>(max_iterations + 1, std::max<int64_t>(kChunk, last_iterations_ + 1));
+        full = want / kChunk;
+        rest = (want % kChunk + kTailChunk - 1) / kTailChunk;
+      } else if (tail) {
+        // past the refresh: tail chunks, doubling each round in case the
+        // solve goes on
+        rest = ahead;
+        ahead = std::min<int64_t>(2 * ahead, 64);
       } else {
-        SPAI_SG_DISPATCH(sgm, (PcgRefreshSpmvM<SG, double><<<grid_, kBlk, 0, st>>>(
-                                  n, M_.ptr.get(), M_.col.get(), M_.val.get(), r_cur, z_.get(), part_.get(),
-                                  scal_.get())));
+        full = 1;  // then one chunk at a time
       }
-      PcgRefreshFinish<<<1, kBlk, 0, st>>>(par, part_.get(), (int)grid_, scal_.get(), ist_.get(), hist_.get());
-      ctx_.ReadBack(hst, ist_.get(), sizeof(hst));
+      // chunks hold an even number of iterations: each starts at the same parity
+      const int par0 = hst[kIter] & 1;
+      for (int64_t c = 0; c < full; ++c) SPAI_HIP_CHECK(hipGraphLaunch(exec_[par0], st));
+      for (int64_t c = 0; c < rest; ++c) SPAI_HIP_CHECK(hipGraphLaunch(tail_exec_[par0], st));
+    }
+    first = false;
+    with_a([&](auto dot) {
+      PcgRefresh<decltype(dot)><<<grid_, kBlk, 0, st>>>(n, dot, x_.get(), b_dev, r_.get(), part_.get(), ist_.get());
+    });
+    with_m([&](auto dot) {
+      PcgRefreshSpmvM<decltype(dot)><<<grid_, kBlk, 0, st>>>(n, dot, r_.get(), z_.get(), part_.get(), scal_.get(),
+                                                            ist_.get());
+    });
+    PcgRefreshFinish<<<1, kBlk, 0, st>>>(part_.get(), (int)grid_, scal_.get(), ist_.get(), hist_.get());
+    with_a([&](auto dot) {
+      PcgFinal<decltype(dot)><<<grid_, kBlk, 0, st>>>(n, dot, x_.get(), b_dev, part_.get(), ist_.get());
+    });
+    PcgFinalFinish<<<1, kBlk, 0, st>>>(part_.get(), (int)grid_, scal_.get(), ist_.get(), hist_.get());
+    SPAI_HIP_CHECK(hipGetLastError());
+    ctx_.ReadBack(hst, ist_.get(), sizeof(hst));
+    if (hst[kFinalDone]) break;
+    if (hst[kRefreshes] != refreshes_before) {
       tail = true;  // past the tolerance crossing only a few iterations remain
       ahead = 1;
     }
-    if (!hst[kDone] && hst[kCountLatest] <= count_before)
+    if (hst[kCountLatest] <= count_before)
       throw std::runtime_error("PCG: launched iterations appended no history entry");
   }
-  SPAI_SG_DISPATCH(sga, (PcgFinal<SG><<<grid_, kBlk, 0, st>>>(n, A_.ptr.get(), A_.col.get(), A_.val.get(), x_.get(),
-                                                              b_dev, part_.get())));
-  PcgFinalFinish<<<1, kBlk, 0, st>>>(part_.get(), (int)grid_, scal_.get(), ist_.get(), hist_.get());
-  SPAI_HIP_CHECK(hipGetLastError());
-  ctx_.ReadBack(hst, ist_.get(), sizeof(hst));
   *count = hst[kCountLatest];
   last_iterations_ = hst[kIter];
   return hst[kStatus];
