@@ -124,13 +124,35 @@ def fused_pass_bytes(n, nnz_a, nnz_s, nnz_s2, sv, method):
     return base + nnz_s2 * sv + 4 * nnz_s * sv
 
 
-def pcg_iteration_bytes(n, nnz_a, nnz_m, m_bytes):
-    """Algorithmic bytes of one PCG iteration: the SpMV with A (FP64 values,
-    int32 indices: (n+1) 4 + 12 nnz(A)), the SpMV with M ((n+1) 4 +
-    nnz(M) (4 + m_bytes)), and the FP64 vectors -- p = z + beta p with A p
-    reads z, p and writes p, Ap; the update reads x, p, r, Ap and writes x,
-    r, z -- 11 n doubles."""
-    return 2 * (n + 1) * 4 + 12 * nnz_a + nnz_m * (4 + m_bytes) + 11 * 8 * n
+def n_diagonals(A):
+    """Distinct diagonals (col - row) holding entries of A."""
+    c = A.tocoo()
+    return int(np.unique(c.col.astype(np.int64) - c.row).size)
+
+
+def matrix_bytes(n, nnz, v_bytes, nd=None):
+    """Bytes of one pass over a matrix in the PCG's layout: diagonals (nd
+    diagonals of n values and a 4-byte row mask) or CSR/SELL (int32 offsets
+    and indices)."""
+    if nd is not None:
+        return nd * n * v_bytes + 4 * n
+    return (n + 1) * 4 + nnz * (4 + v_bytes)
+
+
+def pcg_iteration_bytes(n, nnz_a, nnz_m, m_bytes, nd_a=None, nd_m=None):
+    """Algorithmic bytes of one PCG iteration: one pass over A (FP64 values)
+    and one over M (m_bytes values), in the layouts the solve reads (nd_a /
+    nd_m diagonals, or CSR when None), and the FP64 vectors -- p = z + beta p
+    with A p reads z, p and writes p, Ap; the update reads x, p, r, Ap and
+    writes x, r, z -- 11 n doubles."""
+    return matrix_bytes(n, nnz_a, 8, nd_a) + matrix_bytes(n, nnz_m, m_bytes, nd_m) + 11 * 8 * n
+
+
+def pcg_layout_diagonals(A, layouts):
+    """(nd_a, nd_m) for pcg_iteration_bytes from a session's PCG layouts; M
+    lies on pattern(A) | I here, which is pattern(A) for these matrices."""
+    nd = n_diagonals(A)
+    return (nd if layouts[0] == "dia" else None), (nd if layouts[1] == "dia" else None)
 
 
 def roof(bytes_, seconds, achievable=None):
@@ -292,10 +314,12 @@ def run_configs(with_cpu, reps=5, only=None):
         n = A.shape[0]
         fused = _fused_kernel_roof(s, "mr", n, A.nnz, 8, achievable)
         nnz_m = s.info()["nnz_s"]
+        layouts = s.pcg_layouts()
         s.close()
-        pcg_roof = {"kernel": "PCG iteration (two kernels; solve time / iterations)",
+        pcg_roof = {"kernel": f"PCG iteration (two kernels, A and M in layouts {layouts}; solve time / iterations)",
                     "avg_iteration_us": round(tp / max(cnt - 1, 1) * 1e6, 3)}
-        pcg_roof.update(roof(pcg_iteration_bytes(n, A.nnz, nnz_m, 8), tp / max(cnt - 1, 1), achievable))
+        pcg_roof.update(roof(pcg_iteration_bytes(n, A.nnz, nnz_m, 8, *pcg_layout_diagonals(A, layouts)),
+                             tp / max(cnt - 1, 1), achievable))
         line = {"config": "poisson2d 32^2, MR fixed pattern 20 its + PCG 1e-8", "build_s": tb, "pcg_s": tp,
                 "pcg_iterations": cnt - 1, "status": st, "roofline_build_pass": fused, "roofline_pcg": pcg_roof,
                 "note": "n = 1,024: every kernel is launch-bound, far from any bandwidth bound"}
@@ -353,15 +377,17 @@ def run_configs(with_cpu, reps=5, only=None):
         n = A.shape[0]
         nnz_m = s.info()["nnz_s"]
         spmv_us = s.spmv_a_us()
+        layouts = s.pcg_layouts()
         s.close()
+        nd_a, nd_m = pcg_layout_diagonals(A, layouts)
         per_it = tp / sum(counts)
-        pcg_roof = {"kernel": "PCG iteration (two kernels, FP32 M; 16 solves' time / their iterations, "
-                              "host set-up of each right-hand side included)",
+        pcg_roof = {"kernel": f"PCG iteration (two kernels, FP32 M, A and M in layouts {layouts}; 16 solves' "
+                              "time / their iterations, host set-up of each right-hand side included)",
                     "avg_iteration_us": round(per_it * 1e6, 3)}
-        pcg_roof.update(roof(pcg_iteration_bytes(n, A.nnz, nnz_m, 4), per_it, achievable))
-        spmv_roof = {"kernel": "SpMV with A + <p, Ap> (sliced ELLPACK, FP64), standalone replay",
+        pcg_roof.update(roof(pcg_iteration_bytes(n, A.nnz, nnz_m, 4, nd_a, nd_m), per_it, achievable))
+        spmv_roof = {"kernel": f"SpMV with A + <p, Ap> (A in layout {layouts[0]}, FP64), standalone replay",
                      "avg_launch_us": round(spmv_us, 3)}
-        spmv_roof.update(roof((n + 1) * 4 + 12 * A.nnz + 16 * n, spmv_us * 1e-6, achievable))
+        spmv_roof.update(roof(matrix_bytes(n, A.nnz, 8, nd_a) + 16 * n, spmv_us * 1e-6, achievable))
         line = {"config": "anisotropic2d 1024^2, inner-outer LOMR 5x20 FP32 bulk + 16 RHS PCG 1e-8", "build_s": tb,
                 "pcg_16_rhs_s": tp, "pcg_iterations": counts, "roofline_pcg": pcg_roof, "roofline_spmv_a": spmv_roof}
         if with_cpu:
@@ -483,14 +509,22 @@ def main():
     # nd*n*8 (values) + 4n (row masks) + 2*8*n (p read, Ap written) bytes per
     # launch; on CSR/SELL (n+1)*4 + nnz*(4+8) + 2*8*n
     csr_spmv_bytes = (n + 1) * 4 + nnz * 12 + 16 * n
-    a_coo = A.tocoo()
-    n_diagonals = int(np.unique(a_coo.col.astype(np.int64) - a_coo.row).size)
+    nd_a = n_diagonals(A)
     spmv_layout = fp32["pcg_layouts"]["A"]
-    spmv_bytes = n_diagonals * n * 8 + 4 * n + 16 * n if spmv_layout == "dia" else csr_spmv_bytes
-    spmv_kernel = {"dia": f"PcgSpmvADia (SpMV with A + <p, Ap> on the {n_diagonals}-diagonal layout the solve reads)",
+    spmv_bytes = matrix_bytes(n, nnz, 8, nd_a if spmv_layout == "dia" else None) + 16 * n
+    spmv_kernel = {"dia": f"PcgSpmvADia (SpMV with A + <p, Ap> on the {nd_a}-diagonal layout the solve reads)",
                    "sell": "PcgSpmvASell (SpMV with A + <p, Ap> on the sliced-ELLPACK copy the solve reads)",
                    "csr": "PcgSpmvA (SpMV with A + <p, Ap> on CSR)"}[spmv_layout]
     spmv_us = fp32["spmv_a_us"]
+    # one PCG iteration of the FP32-bulk solve (K1 + K2, FP32 M) from the
+    # solve's time over its iterations (the per-solve start, refresh, final
+    # residual and read-back included)
+    lay = (fp32["pcg_layouts"]["A"], fp32["pcg_layouts"]["M"])
+    it_s = fp32["pcg_s"] / max(fp32["pcg_iterations"], 1)
+    pcg_it_roof = {"kernel": f"PCG iteration (PcgDirSpmvA + PcgUpdateSpmvM, FP32 M, layouts {lay}), "
+                             "solve time / iterations", "bound": "hbm", "avg_iteration_us": round(it_s * 1e6, 3)}
+    pcg_it_roof.update(roof(pcg_iteration_bytes(n, nnz, nnz, 4, *pcg_layout_diagonals(A, lay)), it_s,
+                            max(bw["copy"], bw["triad"])))
     spmv_traffic = None
     prof2 = ROOT / "profiles" / f"pcg_spmv_a_{spmv_layout}_diffusion64_traffic.json"
     if prof2.exists():
@@ -547,6 +581,7 @@ def main():
                               "traffic": spmv_traffic, "algorithmic_bytes_per_launch": spmv_bytes,
                               "csr_equivalent_GBs": round(csr_spmv_bytes / (spmv_us * 1e-6) / 1e9, 1),
                               "avg_launch_us": round(spmv_us, 3)},
+        "roofline_pcg_iteration": pcg_it_roof,
         "spgemm": {"kernel": "SpGemm C = A A (every column short: ExpansionSizes, MergeCount, scan, MergeNumeric), FP64",
                    "seconds": round(spgemm_s, 7), "nnz_C": int(C.nnz), "algorithmic_bytes": spgemm_bytes,
                    "achieved_GBs": round(spgemm_bytes / spgemm_s / 1e9, 1),
