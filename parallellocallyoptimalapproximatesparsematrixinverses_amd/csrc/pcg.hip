@@ -37,7 +37,7 @@ namespace spai {
 
 namespace {
 
-constexpr int kBlk = 512;
+constexpr int kBlk = 256;
 constexpr int kChunk = 16;     // iterations per captured graph (even)
 constexpr int kTailChunk = 2;  // iterations per graph after the refresh (even)
 
@@ -206,7 +206,7 @@ __device__ __forceinline__ double EndIteration(int prev, double r_z_new, double 
 // threadIdx.x + u kBlk), loaded without waiting on anything: the loads take
 // no per-lane branch (lanes past G read entry G - 1, which AddPartials
 // skips), so none of them is waited on before the others are issued.
-constexpr int kMaxPartials = 4;
+constexpr int kMaxPartials = 2;
 __device__ __forceinline__ void LoadPartials(const double* __restrict__ part, int G, double (&v)[kMaxPartials]) {
 #pragma unroll
   for (int u = 0; u < kMaxPartials; ++u) v[u] = part[min(int(threadIdx.x) + u * kBlk, G - 1)];
@@ -1239,7 +1239,11 @@ PcgSolver::PcgSolver(Ctx& ctx, const DCsr& A, const DCsr& M, bool m_fp32)
   if (M.n != A.n) throw std::invalid_argument("A and M must have the same shape");
   if (m_fp32 && !M.val32.get()) throw std::invalid_argument("FP32 copy of M missing");
   grid_ = Grid(n_, kBlk, ctx.num_cus);
-  // the iteration kernels' prologues hold at most kMaxPartials partials a thread
+  // At most kMaxPartials * kBlk = 512 blocks: the iteration kernels'
+  // prologues sum every block's partial, and fewer, fuller blocks (two rows a
+  // thread once n > 131,072) measured faster on the 64^3 solve than one row
+  // a thread: 1.76 against 1.89 ms with 1,024 blocks of 256 threads and
+  // 2.03 ms with 512 blocks of 512.
   if (grid_ > unsigned(kMaxPartials * kBlk)) grid_ = unsigned(kMaxPartials * kBlk);
   x_ = ctx.Alloc<double>(n_);
   r_ = ctx.Alloc<double>(2 * n_);

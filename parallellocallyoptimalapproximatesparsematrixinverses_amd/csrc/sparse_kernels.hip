@@ -362,20 +362,11 @@ __global__ __launch_bounds__(kExpBlock) void ExpansionSizes(int ncols, const int
 constexpr int kMergeDepth = 3;     // rows loaded ahead per list (count)
 constexpr int kMergeDepthNum = 2;  // rows and values loaded ahead per list (numeric)
 
-// Symbolic phase of the short columns, one thread per column: counts[j] =
-// the number of distinct rows of the merge (0 for the other columns).
+// Distinct rows of the k-way merge of column j's NL (or fewer) lists.
 template <int NL>
-__global__ __launch_bounds__(256) void MergeCount(int ncols, const int* __restrict__ xptr,
-                                                  const int* __restrict__ xidx, const int* __restrict__ yptr,
-                                                  const int* __restrict__ yidx, const int64_t* __restrict__ upper,
-                                                  int add_identity, int* __restrict__ counts) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ncols) return;
-  const int y0 = yptr[j], ny = yptr[j + 1] - y0;
-  if (ny > NL || !IsShortColumn(upper[j], ny)) {
-    counts[j] = 0;  // a long column: its count comes from the wavefront tiers
-    return;
-  }
+__device__ __forceinline__ int CountMerge(int j, int y0, int ny, const int* __restrict__ xptr,
+                                          const int* __restrict__ xidx, const int* __restrict__ yidx,
+                                          int add_identity) {
   MergeQueues<NL, kMergeDepth, false> q;
   OpenLists<NL>(y0, ny, yidx, xptr, q.pos, q.end);
   q.Fill(xidx, nullptr);
@@ -390,6 +381,34 @@ __global__ __launch_bounds__(256) void MergeCount(int ncols, const int* __restri
     if (ident == m) ident = INT_MAX;
     q.Pop(m, xidx, nullptr, y, acc);
   }
+  return cnt;
+}
+
+// Symbolic phase of the short columns, one thread per column: counts[j] =
+// the number of distinct rows of the merge (0 for the other columns).  Each
+// wavefront merges with the smallest list count (4, 8 or NL) that holds its
+// short columns, so a stencil product's 7-list columns do not walk 16 lists.
+template <int NL>
+__global__ __launch_bounds__(256) void MergeCount(int ncols, const int* __restrict__ xptr,
+                                                  const int* __restrict__ xidx, const int* __restrict__ yptr,
+                                                  const int* __restrict__ yidx, const int64_t* __restrict__ upper,
+                                                  int add_identity, int* __restrict__ counts) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = j < ncols;
+  const int y0 = live ? yptr[j] : 0, ny = live ? yptr[j + 1] - y0 : 0;
+  const bool active = live && ny <= NL && IsShortColumn(upper[j], ny);
+  const int wmax = WaveMaxInt(active ? ny : 0);
+  if (!active) {
+    if (live) counts[j] = 0;  // a long column: its count comes from the wavefront tiers
+    return;
+  }
+  int cnt;
+  if (wmax <= 4)
+    cnt = CountMerge<4>(j, y0, ny, xptr, xidx, yidx, add_identity);
+  else if (wmax <= 8 || NL <= 8)
+    cnt = CountMerge<(NL < 8 ? NL : 8)>(j, y0, ny, xptr, xidx, yidx, add_identity);
+  else
+    cnt = CountMerge<NL>(j, y0, ny, xptr, xidx, yidx, add_identity);
   counts[j] = cnt;
 }
 
