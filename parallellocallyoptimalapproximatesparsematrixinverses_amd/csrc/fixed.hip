@@ -533,8 +533,21 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
   V2<V>* const st_y = reinterpret_cast<V2<V>*>(wave_lds + lay.stage_y);
   FusedTerm<V>* const stash = reinterpret_cast<FusedTerm<V>*>(wave_lds + lay.stash);
   double red[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  const int ntiles = (n + W - 1) / W;
-  const int wstride = gridDim.x * (kBlk / kWave);
+  // Tiles: when the grid is a multiple of 8, the blocks of each XCD (blocks
+  // b with b mod 8 = x, as workgroups are dealt to the XCDs round robin)
+  // walk one contiguous eighth of the tiles, so that the columns of A and
+  // the operand entries that neighbouring tiles share stay in one L2.
+  // ntiles is the end of this block's tile range.
+  constexpr int kWpb = kBlk / kWave;
+  const int ntiles_all = (n + W - 1) / W;
+  int ntiles = ntiles_all, wstride = int(gridDim.x) * kWpb, tile_first = int(blockIdx.x) * kWpb + wib;
+  if ((gridDim.x & 7u) == 0) {
+    const int x = int(blockIdx.x & 7u);
+    const int tile_begin = int(int64_t(ntiles_all) * x / 8);
+    ntiles = int(int64_t(ntiles_all) * (x + 1) / 8);
+    wstride = int(gridDim.x >> 3) * kWpb;
+    tile_first = tile_begin + int(blockIdx.x >> 3) * kWpb + wib;
+  }
   // Product 1's Y values of S entry e
   auto yget1 = [&](int e, V& ya, V& yb) {
     if constexpr (kInit) {
@@ -576,7 +589,7 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
       yget1(ne0 + gl, ny.a, ny.b);
     }
   };
-  int tile = blockIdx.x * (kBlk / kWave) + wib;
+  int tile = tile_first;
   if (tile < ntiles) {
     load_tile(tile);
     load_stage();
