@@ -124,6 +124,31 @@ def fused_pass_bytes(n, nnz_a, nnz_s, nnz_s2, sv, method):
     return base + nnz_s2 * sv + 4 * nnz_s * sv
 
 
+def dia_pass_bytes(n, nd_a, nd_s, sv):
+    """Algorithmic bytes of one LOMR iteration on the diagonal pass (s_v bytes
+    per value, every slot of the layout counted): A's values once (nd_a n
+    s_v), S's column masks (4 n), M, Z and P read and written (6 nd_s n s_v)
+    and Pr once (n s_v); R' is formed from M' in registers and never stored."""
+    return nd_a * n * sv + 4 * n + 6 * nd_s * n * sv + n * sv
+
+
+def lomr_pass_bytes(n, nnz_a, info, sv, method):
+    """Algorithmic bytes of the per-iteration pass the session's build runs
+    (info = Session.info()): the diagonal pass or the general fused pass."""
+    if info["fused"] == 2:
+        return dia_pass_bytes(n, info["diagonals_a"], info["diagonals_s"], sv)
+    return fused_pass_bytes(n, nnz_a, info["nnz_s"], info["nnz_s2"], sv, method)
+
+
+def lomr_pass_kernel(info, method, precision):
+    """Name of the per-iteration pass kernel of a session's build."""
+    if info["fused"] == 2:
+        dims = {3: 1, 5: 2, 7: 3}.get(info["diagonals_s"], "?")
+        return (f"DiaLomrPass<{precision}, {dims}-D stencil> (one LOMR iteration on the diagonal layout: "
+                f"{info['diagonals_s']} diagonals in A and S, R' = I - A M' formed in registers)")
+    return "FusedPass<%s, %s> (one fused %s iteration)" % (precision, method.upper(), method.upper())
+
+
 def n_diagonals(A):
     """Distinct diagonals (col - row) holding entries of A."""
     c = A.tocoo()
@@ -163,15 +188,6 @@ def roof(bytes_, seconds, achievable=None):
     if achievable:
         out["frac_of_achievable"] = round(gbs / achievable, 4)
     return out
-
-
-def s2_nnz(A):
-    """nnz of pattern(A S) | S | I with S = pattern(A) (Boolean product)."""
-    from scipy import sparse
-
-    B = sparse.csc_matrix((np.ones(A.nnz), A.indices, A.indptr), shape=A.shape)
-    C = (B @ B + B + sparse.identity(A.shape[0], format="csc")).tocsc()
-    return int(C.nnz)
 
 
 def cpu_model():
@@ -268,7 +284,7 @@ def _median_time(fn, reps):
 
 def _fused_kernel_roof(s, method, n, nnz_a, sv, achievable):
     """The session's per-iteration fused pass: average HIP-event time over
-    one more build, and its algorithmic bytes (fused_pass_bytes)."""
+    one more build, and its algorithmic bytes (lomr_pass_bytes)."""
     info = s.info()
     s.set_kernel_timing(True)
     s.build()
@@ -276,9 +292,9 @@ def _fused_kernel_roof(s, method, n, nnz_a, sv, achievable):
     s.set_kernel_timing(False)
     if not info["fused"] or us <= 0:
         return None
-    out = {"kernel": "FusedPass (one %s iteration)" % method.upper(), "avg_launch_us": round(us, 3),
+    out = {"kernel": lomr_pass_kernel(info, method, "float" if sv == 4 else "double"), "avg_launch_us": round(us, 3),
            "launches_timed": launches}
-    out.update(roof(fused_pass_bytes(n, nnz_a, info["nnz_s"], info["nnz_s2"], sv, method), us * 1e-6, achievable))
+    out.update(roof(lomr_pass_bytes(n, nnz_a, info, sv, method), us * 1e-6, achievable))
     return out
 
 
@@ -494,15 +510,17 @@ def main():
     achievable = max(bw["copy"], bw["triad"])
     dM = float(np.linalg.norm((fp32["M"] - fp64["M"]).data) / np.linalg.norm(fp64["M"].data))
 
-    # roofline of the dominant kernel of the FP32-bulk step: the fused LOMR
-    # pass (one iteration), algorithmic bytes of fused_pass_bytes with s_v = 4
-    nnz_s = nnz  # S = pattern(A) | pattern(M0) | I = pattern(A) here
-    nnz_s2 = s2_nnz(A)
-    pass_bytes = fused_pass_bytes(n, nnz, nnz_s, nnz_s2, 4, "lomr")
+    # roofline of the dominant kernel of the FP32-bulk step: the LOMR pass
+    # (one iteration; the diagonal pass on this stencil), algorithmic bytes of
+    # lomr_pass_bytes with s_v = 4
+    shape = fp32["launch_shape"]
+    pass_bytes = lomr_pass_bytes(n, nnz, shape, 4, "lomr")
+    pass_kernel = lomr_pass_kernel(shape, "lomr", "float")
     us = fp32["lomr_pass_us"]
     achieved = pass_bytes / (us * 1e-6) / 1e9
     traffic = None
-    prof = ROOT / "profiles" / "lomr_fused_diffusion64_traffic.json"
+    prof = ROOT / "profiles" / ("lomr_dia_diffusion64_traffic.json" if shape["fused"] == 2
+                                else "lomr_fused_diffusion64_traffic.json")
     if prof.exists():
         traffic = json.loads(prof.read_text()).get("fp32_bytes_per_launch")
     # the PCG's A-SpMV + <p, Ap>: on the diagonal layout the solve reads,
@@ -525,6 +543,14 @@ def main():
                              "solve time / iterations", "bound": "hbm", "avg_iteration_us": round(it_s * 1e6, 3)}
     pcg_it_roof.update(roof(pcg_iteration_bytes(n, nnz, nnz, 4, *pcg_layout_diagonals(A, lay)), it_s,
                             max(bw["copy"], bw["triad"])))
+    # the FP32-over-FP64 step ratio with every kernel at the HBM roofline:
+    # algorithmic bytes of the two steps (LOMR passes + PCG iterations)
+    nd_pair = pcg_layout_diagonals(A, lay)
+    step_bytes = {}
+    for form, res, sv in (("fp32", fp32, 4), ("fp64", fp64, 8)):
+        step_bytes[form] = ((res["spai_iterations"] + 1) * lomr_pass_bytes(n, nnz, res["launch_shape"], sv, "lomr")
+                            + res["pcg_iterations"] * pcg_iteration_bytes(n, nnz, nnz, sv, *nd_pair))
+    ratio_bound = step_bytes["fp64"] / step_bytes["fp32"]
     spmv_traffic = None
     prof2 = ROOT / "profiles" / f"pcg_spmv_a_{spmv_layout}_diffusion64_traffic.json"
     if prof2.exists():
@@ -563,12 +589,13 @@ def main():
                  "speedup_fp32_vs_fp64": round(cold["fp64"]["seconds"] / cold["fp32"]["seconds"], 4)},
         "speedup_target": {"target": 1.5, "met_warm": fp64["seconds"] / fp32["seconds"] >= 1.5,
                            "met_cold": cold["fp64"]["seconds"] / cold["fp32"]["seconds"] >= 1.5,
-                           "bound_at_roofline": 1.4,
+                           "bound_at_roofline": round(ratio_bound, 3),
                            "note": "the PCG recurrence is FP64 in both forms (only M is FP32); with every kernel "
-                                   "at the HBM roofline the step ratio is at most about 1.4"},
+                                   "at the HBM roofline the step ratio is the ratio of the two steps' algorithmic "
+                                   "bytes (LOMR passes + PCG iterations), bound_at_roofline"},
         "extra_pcg_iterations_fp32": fp32["pcg_iterations"] - fp64["pcg_iterations"],
         "rel_fro_M32_vs_M64": dM,
-        "roofline": {"kernel": "FusedPass<float, LOMR> (one fused LOMR iteration)", "bound": "hbm",
+        "roofline": {"kernel": pass_kernel, "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "frac_of_achievable": round(achieved / achievable, 4),

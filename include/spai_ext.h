@@ -199,10 +199,12 @@ int64_t spai_session_set_kernel_timing(SpaiSession* session, int64_t on);
  * passes were averaged; negative when the last build was not timed. */
 double spai_session_build_kernel_us(SpaiSession* session, int64_t* launches);
 /* Launch shape of the session's fixed-pattern build, for tests and
- * measurement: out[0] = 1 when LOMR/MR run the fused per-iteration pass, then
- * lanes per column, columns per tile, tiles, resident-grid blocks,
- * wavefronts per block, nnz of the iterate pattern S and nnz of the product
- * pattern S2 = pattern(A S) | S | I (len entries written, at most 8).
+ * measurement: out[0] = 1 when LOMR/MR run the fused per-iteration pass, 2
+ * when LOMR runs the diagonal pass (A, S and S2 on few diagonals), 0
+ * otherwise; then lanes per column, columns per tile, tiles, resident-grid
+ * blocks, wavefronts per block, nnz of the iterate pattern S, nnz of the
+ * product pattern S2 = pattern(A S) | S | I, and the diagonals of A, S and
+ * S2 on the diagonal pass (0 without it); len entries written, at most 11.
  * Returns 0, or -1. */
 int64_t spai_session_info(SpaiSession* session, int64_t* out, int64_t len);
 

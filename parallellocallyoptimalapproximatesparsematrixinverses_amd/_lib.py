@@ -374,14 +374,16 @@ class Session:
         return float(us), int(n.value)
 
     def info(self) -> dict:
-        """Launch shape of the build: fused pass used, lanes per column,
-        columns per tile, tiles, resident-grid blocks, wavefronts per block;
-        nnz of the patterns S (M's) and S2 (pattern(A S) | S | I)."""
-        out = (ctypes.c_int64 * 8)()
-        if load().spai_session_info(self._h, out, 8) < 0:
-            raise RuntimeError(last_error())
+        """Launch shape of the build: per-iteration pass (1 fused, 2
+        diagonal, 0 neither), lanes per column, columns per tile, tiles,
+        resident-grid blocks, wavefronts per block; nnz of the patterns S
+        (M's) and S2 (pattern(A S) | S | I); diagonals of A, S and S2 on the
+        diagonal pass (0 otherwise)."""
         keys = ("fused", "lanes_per_column", "columns_per_tile", "tiles", "grid_blocks", "waves_per_block", "nnz_s",
-                "nnz_s2")
+                "nnz_s2", "diagonals_a", "diagonals_s", "diagonals_s2")
+        out = (ctypes.c_int64 * len(keys))()
+        if load().spai_session_info(self._h, out, len(keys)) < 0:
+            raise RuntimeError(last_error())
         return {k: int(v) for k, v in zip(keys, out)}
 
     def close(self):

@@ -356,9 +356,9 @@ int64_t spai_session_info(SpaiSession* s, int64_t* out, int64_t len) {
   try {
     if (!s) throw std::invalid_argument("null session");
     if (!out || len < 0) throw std::invalid_argument("null output");
-    int64_t info[8];
+    int64_t info[spai::kFixedInfoLen];
     s->prob->FusedInfo(info);
-    for (int64_t i = 0; i < len && i < 8; ++i) out[i] = info[i];
+    for (int64_t i = 0; i < len && i < spai::kFixedInfoLen; ++i) out[i] = info[i];
     return 0;
   } catch (const std::exception& ex) {
     return Fail(ex.what());

@@ -17,6 +17,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <stdexcept>
 #include <type_traits>
 #include <vector>
@@ -1629,6 +1631,267 @@ unsigned Blocks1(int64_t n) {
   return (unsigned)std::max<int64_t>(g, 1);
 }
 
+// ------------------------------------------------------------ diagonal pass
+// LOMR on a fixed pattern whose matrices are a grid Laplacian's: A and S on
+// the diagonals 0, +-1, +-m, +-m^2 (up to the grid's dimension; o = col -
+// row, ascending) and S2 on all their pairwise sums.  Every matrix is held
+// by columns and diagonals: entry (j - o_d, j) sits at d n + j, 0 where the
+// column has no entry there (a bit per diagonal of S's column mask says which
+// are entries).  Entry (i, j) of A Y (Y on S) is the sum over the S entries
+// (k, j) of A(i, k) Y(k, j); with the diagonals in this order the S2 diagonal
+// that pair (S diagonal es, A diagonal ea) lands on is a constant of the
+// shape, so one thread per column forms its products with every index known
+// at compile time, every load address known before any load returns, and
+// the loads of a wavefront contiguous.  Absent entries are exact zeros:
+// a product term with one adds nothing, and no mask is needed for A or S2.
+constexpr int kDpS = 8;    // at most this many diagonals in A and in S
+constexpr int kDpS2 = 32;  // at most this many in S2
+
+struct DiaShape {
+  int nd = 0, nd2 = 0;
+  long long off[kDpS] = {};          // diagonals of A and S, with a stand-in grid side
+  long long off2[kDpS2] = {};        // S2: their pairwise sums, ascending
+  signed char t[kDpS][kDpS] = {};    // S2 diagonal of off[es] + off[ea]
+  signed char s2OfS[kDpS] = {};      // S2 diagonal of off[es]
+  signed char sOfS2[kDpS2] = {};     // S diagonal of off2[t] (-1: none)
+};
+
+// Shape of a dims-dimensional Laplacian stencil (dims = 1, 2, 3).  The side
+// 2^12 stands in for the real one: the order of the sums is the same for any
+// side of at least 5, and the host checks the real offsets against it.
+constexpr DiaShape MakeDiaShape(int dims) {
+  DiaShape s{};
+  long long o[kDpS] = {};
+  int n = 0;
+  o[n++] = 0;
+  long long unit = 1;
+  for (int d = 0; d < dims; ++d) {
+    o[n++] = unit;
+    o[n++] = -unit;
+    unit *= 4096;
+  }
+  for (int i = 1; i < n; ++i)
+    for (int k = i; k > 0 && o[k - 1] > o[k]; --k) {
+      const long long x = o[k];
+      o[k] = o[k - 1];
+      o[k - 1] = x;
+    }
+  long long u[kDpS * kDpS] = {};
+  int m = 0;
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < n; ++k) u[m++] = o[i] + o[k];
+  for (int i = 1; i < m; ++i)
+    for (int k = i; k > 0 && u[k - 1] > u[k]; --k) {
+      const long long x = u[k];
+      u[k] = u[k - 1];
+      u[k - 1] = x;
+    }
+  int n2 = 0;
+  for (int i = 0; i < m; ++i)
+    if (n2 == 0 || u[i] != s.off2[n2 - 1]) s.off2[n2++] = u[i];
+  s.nd = n;
+  s.nd2 = n2;
+  for (int k = 0; k < n2; ++k) s.sOfS2[k] = -1;
+  for (int i = 0; i < n; ++i) {
+    s.off[i] = o[i];
+    for (int k = 0; k < n2; ++k) {
+      if (s.off2[k] == o[i]) {
+        s.s2OfS[i] = (signed char)k;
+        s.sOfS2[k] = (signed char)i;
+      }
+      for (int e = 0; e < n; ++e)
+        if (s.off2[k] == o[i] + o[e]) s.t[i][e] = (signed char)k;
+    }
+  }
+  return s;
+}
+
+// The real offsets of S (= A's) and S2 (kernel arguments).
+struct DiaPassArgs {
+  int offS[kDpS];
+  int offS2[kDpS2];
+};
+
+// One LOMR iteration (kInit: R = I - A M, Z = Pr R on S, P = 0, and the
+// first traces) on the diagonal layout, the work of FusedPass: M' = M +
+// delta Z + gamma P, P' = beta P + Z on S; R' = I - A M' on S2 (||R'||^2
+// into partial 5); Z' = Pr R' on S; then A [Z' P'] and the traces of the next
+// step (LomrProjectEpi order).  R' is formed from M' rather than updated as
+// R - delta A Z - gamma A P (locally_minimal_residual.cpp:92-107): the two
+// are equal in exact arithmetic, and R, 25 diagonals for a 3-D stencil, is
+// then never stored -- the pass reads and writes only M, Z and P.  The terms
+// of each S2 entry are added with S rows k ascending (S diagonals
+// descending), the order of the reference's products.
+template <class V, bool kInit, int kDims>
+__global__ __launch_bounds__(kBlk) void DiaLomrPass(int n, DiaPassArgs g, const unsigned* __restrict__ maskS,
+                                                    const V* __restrict__ av, const V* __restrict__ pr,
+                                                    V* __restrict__ M, V* __restrict__ Z, V* __restrict__ P,
+                                                    const double* __restrict__ sc, double* __restrict__ part_out,
+                                                    const Flags* __restrict__ flags) {
+#pragma clang fp contract(fast)
+  constexpr DiaShape sh = MakeDiaShape(kDims);
+  constexpr int ND = sh.nd, ND2 = sh.nd2;
+  if (flags->stop) return;
+  V d = V(0), gm = V(0), be = V(0);
+  if constexpr (!kInit) {
+    d = V(sc[kDelta]);
+    gm = V(sc[kGamma]);
+    be = V(sc[kBeta]);
+  }
+  double red[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const size_t ns = size_t(n);
+  for (int j = blockIdx.x * kBlk + threadIdx.x; j < n; j += gridDim.x * kBlk) {
+    const unsigned mS = maskS[j];
+    int kk[ND];  // S rows j - off (clamped: out of range only where S has no entry)
+    V mv[ND], pn[ND], zn[ND];
+#pragma unroll
+    for (int e = 0; e < ND; ++e) {
+      kk[e] = min(max(j - g.offS[e], 0), n - 1);
+      const size_t at = size_t(e) * ns + size_t(j);
+      mv[e] = M[at];
+      pn[e] = V(0);
+      if constexpr (!kInit) {
+        const V zo = Z[at], po = P[at];
+        V m = mv[e];
+        m = m + d * zo;
+        m = m + gm * po;
+        mv[e] = m;
+        M[at] = m;
+        pn[e] = po * be + zo;
+      }
+    }
+    // Z' = Pr R' on S: R' = I - A M' on the S2 diagonals that are S's
+#pragma unroll
+    for (int t = 0; t < ND2; ++t) {
+      if (sh.sOfS2[t] < 0) continue;
+      V am = V(0);
+#pragma unroll
+      for (int es = ND - 1; es >= 0; --es)
+#pragma unroll
+        for (int ea = 0; ea < ND; ++ea)
+          if (sh.t[es][ea] == t) am += av[size_t(ea) * ns + size_t(kk[es])] * mv[es];
+      V r = -am;
+      if (sh.off2[t] == 0) r += V(1);
+#pragma unroll
+      for (int e = 0; e < ND; ++e)
+        if (sh.sOfS2[t] == e) zn[e] = ((mS >> e) & 1u) ? pr[kk[e]] * r : V(0);
+    }
+#pragma unroll
+    for (int e = 0; e < ND; ++e) {
+      const size_t at = size_t(e) * ns + size_t(j);
+      Z[at] = zn[e];
+      P[at] = pn[e];
+    }
+    // R' = I - A M' on all of S2 (the same terms in the same order, so the
+    // same values on S's diagonals), ||R'||^2, A [Z' P'] and the traces
+    // (LomrProjectEpi order).  A's values are read again (from cache): an
+    // opaque copy of the pointer keeps the compiler from holding the first
+    // loop's in registers.
+    const V* av2 = av;
+    asm volatile("" : "+v"(av2));
+#pragma unroll
+    for (int t = 0; t < ND2; ++t) {
+      V am = V(0), az = V(0), ap = V(0);
+#pragma unroll
+      for (int es = ND - 1; es >= 0; --es)
+#pragma unroll
+        for (int ea = 0; ea < ND; ++ea)
+          if (sh.t[es][ea] == t) {
+            const V a = av2[size_t(ea) * ns + size_t(kk[es])];
+            am += a * mv[es];
+            az += a * zn[es];
+            ap += a * pn[es];
+          }
+      V r = -am;
+      if (sh.off2[t] == 0) r += V(1);
+      red[5] += double(r) * double(r);
+      const V p2 = pr[min(max(j - g.offS2[t], 0), n - 1)];
+      const double daz = az, dap = ap, dpr = p2, dprr = double(p2 * r);
+      red[0] += daz * (dpr * daz);
+      red[1] += dap * (dpr * dap);
+      red[2] += daz * (dpr * dap);
+      red[3] += dprr * daz;
+      red[4] += dprr * dap;
+    }
+  }
+  StorePartials<6>(red, part_out);
+}
+
+// Diagonal layout construction (columns): flag[j - i + n - 1] = 1 for every
+// entry (i, j) of a CSC pattern.
+__global__ void ColDiaMark(int n, const int* __restrict__ ptr, const int* __restrict__ idx, int* __restrict__ flag) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  for (int p = ptr[j]; p < ptr[j + 1]; ++p) flag[int64_t(j) - idx[p] + n - 1] = 1;
+}
+
+// off[pos[i]] = i - (n - 1) for every flagged i.
+__global__ void ColDiaOffsets(int64_t len, int n, const int* __restrict__ flag, const int* __restrict__ pos,
+                              int* __restrict__ off) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < len && flag[i]) off[pos[i]] = int(i - (n - 1));
+}
+
+// Per column: the CSC position of each diagonal slot (-1 for none) and the mask.
+__global__ void ColDiaFill(int n, int nd, const int* __restrict__ ptr, const int* __restrict__ idx,
+                           const int* __restrict__ pos, int* __restrict__ src, unsigned* __restrict__ mask) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  for (int e = 0; e < nd; ++e) src[size_t(e) * n + j] = -1;
+  unsigned m = 0;
+  for (int p = ptr[j]; p < ptr[j + 1]; ++p) {
+    const int e = pos[int64_t(j) - idx[p] + n - 1];
+    src[size_t(e) * n + j] = p;
+    m |= 1u << e;
+  }
+  mask[j] = m;
+}
+
+// out[k] = src[k] >= 0 ? v[src[k]] : 0 (CSC values into diagonal slots).
+template <class VI, class VO>
+__global__ void ToDiagonals(int64_t len, const int* __restrict__ src, const VI* __restrict__ v, VO* __restrict__ out) {
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < len) out[k] = src[k] >= 0 ? VO(v[src[k]]) : VO(0);
+}
+
+// out[src[k]] = v[k] for the slots that hold an entry (diagonals back to CSC).
+template <class V>
+__global__ void FromDiagonals(int64_t len, const int* __restrict__ src, const V* __restrict__ v, V* __restrict__ out) {
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < len && src[k] >= 0) out[src[k]] = v[k];
+}
+
+// Column-diagonal layout of a CSC pattern: offsets (host), masks and slot
+// sources (device); nd = 0 when it has more than max_nd diagonals or more
+// than 1.5 nnz slots.
+struct ColDia {
+  int nd = 0;
+  std::vector<int> off;
+  DBuf<int> src;
+  DBuf<unsigned> mask;
+};
+
+ColDia MakeColDia(Ctx& c, int64_t n, const Pattern& pat, int max_nd) {
+  ColDia out;
+  const int64_t len = 2 * n - 1;
+  auto flag = c.Alloc<int>(len);
+  auto pos = c.Alloc<int>(len + 1);
+  SPAI_HIP_CHECK(hipMemsetAsync(flag.get(), 0, sizeof(int) * len, c.stream));
+  ColDiaMark<<<Blocks1(n), kBlk, 0, c.stream>>>((int)n, pat.ptr.get(), pat.idx.get(), flag.get());
+  const int64_t nd = ScanCounts(c, flag.get(), len, pos.get());
+  if (nd > max_nd || double(nd) * double(n) > 1.5 * double(pat.nnz)) return out;
+  auto off = c.Alloc<int>(nd);
+  ColDiaOffsets<<<Blocks1(len), kBlk, 0, c.stream>>>(len, (int)n, flag.get(), pos.get(), off.get());
+  out.off.resize(nd);
+  c.ReadBack(out.off.data(), off.get(), sizeof(int) * nd);
+  out.src = c.Alloc<int>(nd * n);
+  out.mask = c.Alloc<unsigned>(n);
+  ColDiaFill<<<Blocks1(n), kBlk, 0, c.stream>>>((int)n, (int)nd, pat.ptr.get(), pat.idx.get(), pos.get(),
+                                                out.src.get(), out.mask.get());
+  out.nd = (int)nd;
+  return out;
+}
+
 template <class V>
 class FixedImpl final : public FixedProblem {
  public:
@@ -1797,6 +2060,7 @@ class FixedImpl final : public FixedProblem {
       prS2_ = c.Alloc<V>(nnz_s2_ + 16);  // padding: the fused pass loads whole 16-byte chunks
       GatherRows<V><<<Blocks1(nnz_s2_), kBlk, 0, c.stream>>>(nnz_s2_, s2_pat_->idx.get(), prv_.get(), prS2_.get());
     }
+    if (method_ == SPAI_GLOBAL_LOMR) SetUpDiaPass();
     M_ = c.Alloc<V>(nnz_s_);
     Z_ = c.Alloc<V>(nnz_s_);
     P_ = c.Alloc<V>(nnz_s_);
@@ -1916,20 +2180,76 @@ class FixedImpl final : public FixedProblem {
     return timed_launches_ > 0 ? timed_us_ : -1.0;
   }
 
-  void FusedInfo(int64_t out[8]) const override {
+  void FusedInfo(int64_t out[kFixedInfoLen]) const override {
     const bool fused = (method_ == SPAI_GLOBAL_LOMR || method_ == SPAI_GLOBAL_MR) && use_plan_ && fused_ok_;
-    const int64_t w = kWave / gl_;
-    out[0] = fused ? 1 : 0;
-    out[1] = gl_;
+    const int64_t w = dia_ ? kWave : kWave / gl_;
+    out[0] = dia_ ? 2 : fused ? 1 : 0;
+    out[1] = dia_ ? 1 : gl_;
     out[2] = w;
     out[3] = (n_ + w - 1) / w;
-    out[4] = gridf_;
+    out[4] = dia_ ? gridd_ : gridf_;
     out[5] = kBlk / kWave;
     out[6] = nnz_s_;
     out[7] = nnz_s2_;
+    out[8] = dia_ ? dnd_ : 0;
+    out[9] = dia_ ? dnd_ : 0;
+    out[10] = dia_ ? dnd2_ : 0;
   }
 
-  // Global LOMR or MR with one fused pass per iteration (FusedPass).
+  // The diagonal pass (DiaLomrPass) for Global LOMR when A and S lie on the
+  // diagonals of a 1-, 2- or 3-dimensional Laplacian stencil and S2 on all
+  // their sums, in the order of MakeDiaShape, each layout with at most 1.5
+  // nnz slots.  The environment variable SPAI_FIXED_LAYOUT ("csr", or "auto"
+  // the default) can keep the general fused pass.
+  void SetUpDiaPass() {
+    const char* env = std::getenv("SPAI_FIXED_LAYOUT");
+    if (env && *env && std::strcmp(env, "auto") != 0) {
+      if (std::strcmp(env, "csr") == 0) return;
+      throw std::invalid_argument("SPAI_FIXED_LAYOUT must be csr or auto");
+    }
+    Ctx& c = ctx_;
+    ColDia a = MakeColDia(c, n_, *A_.pat, kDpS);
+    if (!a.nd) return;
+    ColDia s = MakeColDia(c, n_, *s_pat_, kDpS);
+    if (s.nd != a.nd || s.off != a.off) return;
+    ColDia s2 = MakeColDia(c, n_, *s2_pat_, kDpS2);
+    if (!s2.nd) return;
+    int dims = 0;
+    for (int k = 1; k <= 3 && !dims; ++k)
+      if (MatchesShape(MakeDiaShape(k), s.off, s2.off)) dims = k;
+    if (!dims) return;
+    for (int e = 0; e < s.nd; ++e) dargs_.offS[e] = s.off[e];
+    for (int t = 0; t < s2.nd; ++t) dargs_.offS2[t] = s2.off[t];
+    const int64_t la = int64_t(a.nd) * n_, ls = int64_t(s.nd) * n_;
+    dav_ = c.Alloc<V>(la);
+    ToDiagonals<double, V><<<Blocks1(la), kBlk, 0, c.stream>>>(la, a.src.get(), A_.v(), dav_.get());
+    dM_ = c.Alloc<V>(ls);
+    dZ_ = c.Alloc<V>(ls);
+    dP_ = c.Alloc<V>(ls);
+    dmask_s_ = std::move(s.mask);
+    dsrc_s_ = std::move(s.src);
+    ddims_ = dims;
+    dnd_ = s.nd;
+    dnd2_ = s2.nd;
+    gridd_ = (unsigned)std::min<int64_t>(std::max<int64_t>((n_ + kBlk - 1) / kBlk, 1), kMaxFusedGrid);
+    dia_ = true;
+  }
+
+  // Whether the offsets of S (ascending, 0 in the middle) and S2 pair up as
+  // in shape sh: off[es] + off[ea] is S2 diagonal sh.t[es][ea] for all es, ea.
+  static bool MatchesShape(const DiaShape& sh, const std::vector<int>& off, const std::vector<int>& off2) {
+    if ((int)off.size() != sh.nd || (int)off2.size() != sh.nd2 || off[sh.nd / 2] != 0) return false;
+    for (int es = 0; es < sh.nd; ++es)
+      for (int ea = 0; ea < sh.nd; ++ea) {
+        const int64_t sum = int64_t(off[es]) + off[ea];
+        const int t = sh.t[es][ea];
+        if (off2[t] != sum) return false;
+      }
+    return true;
+  }
+
+  // Global LOMR or MR with one fused pass per iteration (FusedPass, or
+  // DiaLomrPass on the diagonal layout).
   int64_t BuildFused() {
     Ctx& c = ctx_;
     hipStream_t st = c.stream;
@@ -1949,14 +2269,17 @@ class FixedImpl final : public FixedProblem {
     if (record) SPAI_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     SampleOnPattern<V><<<Blocks1(n_ * kSampleLanes), kBlk, 0, st>>>(
         (int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), m0_.ptr(), m0_.idx(), m0_.v(), M_.get());
+    const int64_t ls = int64_t(dnd_) * n_;
+    if (dia_) ToDiagonals<V, V><<<Blocks1(ls), kBlk, 0, st>>>(ls, dsrc_s_.get(), M_.get(), dM_.get());
     ResetFlags<<<1, 1, 0, st>>>(flags_.get());
     double* part = partf_.get();
     const double tol = prm_.tolerance;
+    const int G = int(dia_ ? gridd_ : gridf_);
     for (int64_t it = 0; it <= prm_.max_iterations; ++it) {
       if (it > 0 && mr)
-        MrFusedStep<<<1, 256, 0, st>>>(part, (int)gridf_, (int)it, tol, hist_.get(), sc_.get(), flags_.get());
+        MrFusedStep<<<1, 256, 0, st>>>(part, G, (int)it, tol, hist_.get(), sc_.get(), flags_.get());
       else if (it > 0)
-        LomrFusedStep<<<1, 256, 0, st>>>(part, (int)gridf_, (int)it, tol, hist_.get(), sc_.get(), flags_.get());
+        LomrFusedStep<<<1, 256, 0, st>>>(part, G, (int)it, tol, hist_.get(), sc_.get(), flags_.get());
 #define SPAI_FUSED_STASH(SlotT, plan, GLV, INIT, MR, STASH)                                                   \
   FusedPass<V, SlotT, GLV, INIT, MR, STASH><<<gridf_, kBlk, smemf_, st>>>(                                     \
       (int)n_, s_pat_->ptr.get(), a0c_.get(), stq_.get(), s2d_.get(), s2_pat_->ptr.get(), av_.get(),           \
@@ -1998,7 +2321,12 @@ class FixedImpl final : public FixedProblem {
   }
       const bool timed = timing_ && it > 0;
       if (timed) SPAI_HIP_CHECK(hipEventRecord(events_[2 * (it - 1)], st));
-      if (slot16_) {
+      if (dia_) {
+        if (it == 0)
+          LaunchDiaPass<true>();
+        else
+          LaunchDiaPass<false>();
+      } else if (slot16_) {
         SPAI_FUSED_GL(uint16_t, plan16_.get());
       } else {
         SPAI_FUSED_GL(uint8_t, plan8_.get());
@@ -2009,8 +2337,8 @@ class FixedImpl final : public FixedProblem {
 #undef SPAI_FUSED_ONE
 #undef SPAI_FUSED_STASH
     }
-    LomrFusedFinish<<<1, kBlk, 0, st>>>(part, (int)gridf_, (int)prm_.max_iterations, tol, hist_.get(),
-                                        flags_.get());
+    LomrFusedFinish<<<1, kBlk, 0, st>>>(part, G, (int)prm_.max_iterations, tol, hist_.get(), flags_.get());
+    if (dia_) FromDiagonals<V><<<Blocks1(ls), kBlk, 0, st>>>(ls, dsrc_s_.get(), dM_.get(), M_.get());
     if (record) {
       SPAI_HIP_CHECK(hipStreamEndCapture(st, &build_graph_));
       SPAI_HIP_CHECK(hipGraphInstantiate(&build_exec_, build_graph_, nullptr, nullptr, 0));
@@ -2033,6 +2361,20 @@ class FixedImpl final : public FixedProblem {
       timed_us_ = f.iterations > 0 ? total * 1e3 / double(f.iterations) : 0.0;
     }
     return f.iterations;
+  }
+
+  template <bool kInit>
+  void LaunchDiaPass() {
+#define SPAI_DIA_PASS(DIMS)                                                                                \
+  DiaLomrPass<V, kInit, DIMS><<<gridd_, kBlk, 0, ctx_.stream>>>((int)n_, dargs_, dmask_s_.get(), dav_.get(), \
+                                                                prv_.get(), dM_.get(), dZ_.get(), dP_.get(),  \
+                                                                sc_.get(), partf_.get(), flags_.get())
+    switch (ddims_) {
+      case 1: SPAI_DIA_PASS(1); break;
+      case 2: SPAI_DIA_PASS(2); break;
+      default: SPAI_DIA_PASS(3); break;
+    }
+#undef SPAI_DIA_PASS
   }
 
   // Blocks of the fused iteration kernel (the configuration the build
@@ -2095,7 +2437,8 @@ class FixedImpl final : public FixedProblem {
   int64_t Build() override {
     timed_launches_ = 0;
     if (method_ >= SPAI_INNER_OUTER_MR) return BuildInnerOuter();
-    if ((method_ == SPAI_GLOBAL_LOMR || method_ == SPAI_GLOBAL_MR) && use_plan_ && fused_ok_) return BuildFused();
+    if (dia_ || ((method_ == SPAI_GLOBAL_LOMR || method_ == SPAI_GLOBAL_MR) && use_plan_ && fused_ok_))
+      return BuildFused();
     Ctx& c = ctx_;
     hipStream_t st = c.stream;
     // M = M0 on S, P = 0
@@ -2260,6 +2603,15 @@ class FixedImpl final : public FixedProblem {
   size_t smem_ = 0;
   bool slot16_ = false, use_plan_ = false;
   unsigned gridp_ = 1;
+  // diagonal pass: stencil dimensions, diagonals of S and S2, their
+  // offsets, S's column masks and slot sources, values on diagonals
+  bool dia_ = false;
+  unsigned gridd_ = 1;
+  int ddims_ = 0, dnd_ = 0, dnd2_ = 0;
+  DiaPassArgs dargs_{};
+  DBuf<unsigned> dmask_s_;
+  DBuf<int> dsrc_s_;
+  DBuf<V> dav_, dM_, dZ_, dP_;
 
   // Launches the slot product with epilogue e over Y = (y0, y1).
   template <int NY, class Epi>

@@ -16,6 +16,8 @@ EngineOutput RunFixedEngine(int64_t method, const MatrixData& A, const MatrixDat
 
 // Device-resident fixed-pattern problem: A, S and S2 = pattern(A S) | S | I
 // set up once, builds replayable with every operand in HBM.
+constexpr int kFixedInfoLen = 11;
+
 class FixedProblem {
  public:
   virtual ~FixedProblem() = default;
@@ -39,10 +41,12 @@ class FixedProblem {
   // Average microseconds of that pass over the iterations the last build ran
   // (launches receives their number); -1 when the last build did not time it.
   virtual double KernelMicros(int64_t* launches) = 0;
-  // Launch shape of the fused pass: out[0] = 1 when the build uses it, then
-  // lanes per column (GL), columns per tile (64 / GL), tiles, blocks of the
-  // resident grid and wavefronts per block.
-  virtual void FusedInfo(int64_t out[8]) const = 0;
+  // Launch shape of the per-iteration pass: out[0] = 1 when the build uses
+  // the fused pass, 2 the diagonal pass, 0 neither; then lanes per column
+  // (GL), columns per tile (64 / GL), tiles, blocks of the resident grid,
+  // wavefronts per block, nnz of S and of S2, and the diagonals of A, S and
+  // S2 that the diagonal pass reads (0 without it).
+  virtual void FusedInfo(int64_t out[kFixedInfoLen]) const = 0;
 };
 
 std::unique_ptr<FixedProblem> MakeFixedProblem(int64_t method, const MatrixData& A, const MatrixData* M0,

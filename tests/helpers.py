@@ -1,4 +1,7 @@
 """Shared comparisons for the parity tests."""
+import contextlib
+import os
+
 import numpy as np
 from scipy import sparse
 
@@ -55,3 +58,17 @@ def dense_loss_reference(A, M):
     gm = np.array([GM[M.indices[p], j] for j in range(M.shape[1]) for p in range(M.indptr[j], M.indptr[j + 1])])
     ga = np.array([GA[A.indices[p], j] for j in range(A.shape[1]) for p in range(A.indptr[j], A.indptr[j + 1])])
     return float((R * R).sum()), gm, ga
+
+
+@contextlib.contextmanager
+def env_set(name, value):
+    """Environment variable name = value inside the block (restored after)."""
+    old = os.environ.get(name)
+    os.environ[name] = value
+    try:
+        yield
+    finally:
+        if old is None:
+            del os.environ[name]
+        else:
+            os.environ[name] = old
