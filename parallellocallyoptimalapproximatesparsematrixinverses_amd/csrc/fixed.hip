@@ -807,22 +807,22 @@ __global__ __launch_bounds__(kBlk, sizeof(V) == 4 ? 5 : 4) void FusedPass(
   StorePartials<6>(red, part_out);
 }
 
-// Between passes it - 1 and it: history row it - 1 (result_operations.cpp:7-67,
-// converged iff ||R|| < tol), then GetDeltaGamma (locally_minimal_residual.cpp:36-50)
-// and the direction ratio gamma / delta for iteration it.
-__global__ __launch_bounds__(256) void LomrFusedStep(const double* __restrict__ part, int G, int it, double tol,
-                                                      double* __restrict__ hist, double* __restrict__ sc,
-                                                      Flags* __restrict__ flags) {
-  if (flags->stop) return;
-  double s[6];
-  SumPartialsBlockFused<6>(part, G, s);
-  if (threadIdx.x != 0) return;
-  const double rn = sqrt(s[5]);
-  hist[it - 1] = rn;
-  flags->iterations = it - 1;
-  if (rn < tol) {
-    flags->stop = 2;
-    return;
+// Threads of the one-block kernels between the fused passes.
+constexpr int kStepBlk = 256;
+
+// Between passes it - 1 and it, from the sums s of the pass's partials:
+// ||R|| of row it - 1 (result_operations.cpp:7-67, converged iff ||R|| <
+// tol: stop 2), then GetDeltaGamma (locally_minimal_residual.cpp:36-50) and
+// the direction ratio gamma / delta for iteration it (breakdown: stop 1).
+struct LomrStep {
+  int stop;
+  double rn, delta, gamma, beta;
+};
+__device__ LomrStep LomrStepValues(const double (&s)[6], int it, double tol) {
+  LomrStep r{0, sqrt(s[5]), 0.0, 0.0, 0.0};
+  if (r.rn < tol) {
+    r.stop = 2;
+    return r;
   }
   const bool first = it == 1;
   const double az_praz = s[0], ap_prap = first ? 0.0 : s[1], az_prap = first ? 0.0 : s[2], z_az = s[3],
@@ -830,8 +830,8 @@ __global__ __launch_bounds__(256) void LomrFusedStep(const double* __restrict__ 
   double delta = 0.0, gamma = 0.0;
   if (first) {
     if (!(isfinite(az_praz) && fabs(az_praz) > 0.0)) {
-      flags->stop = 1;
-      return;
+      r.stop = 1;
+      return r;
     }
     delta = z_az / az_praz;
     gamma = 1.0;
@@ -839,18 +839,55 @@ __global__ __launch_bounds__(256) void LomrFusedStep(const double* __restrict__ 
     SolveLeastSquares2x2(az_praz, az_prap, az_prap, ap_prap, z_az, z_ap, delta, gamma);
   }
   if (!isfinite(delta) || !isfinite(gamma)) {
-    flags->stop = 1;
-    return;
+    r.stop = 1;
+    return r;
   }
   const double ratio = delta != 0.0 ? gamma / delta : 0.0;
-  sc[kDelta] = delta;
-  sc[kGamma] = gamma;
-  sc[kBeta] = isfinite(ratio) ? ratio : 0.0;
+  r.delta = delta;
+  r.gamma = gamma;
+  r.beta = isfinite(ratio) ? ratio : 0.0;
+  return r;
+}
+
+// LomrStepValues with its results stored: history row it - 1, the iteration
+// count and stop flag, and the step values (thread 0 of one block).
+__device__ void LomrStepFromSums(const double (&s)[6], int it, double tol, double* __restrict__ hist,
+                                 double* __restrict__ sc, Flags* __restrict__ flags) {
+  const LomrStep r = LomrStepValues(s, it, tol);
+  hist[it - 1] = r.rn;
+  flags->iterations = it - 1;
+  if (r.stop) {
+    flags->stop = r.stop;
+    return;
+  }
+  sc[kDelta] = r.delta;
+  sc[kGamma] = r.gamma;
+  sc[kBeta] = r.beta;
+}
+
+// History row it of the last pass (AppendResidual), from its summed partials.
+__device__ void LomrFinishFromSums(const double (&s)[6], int it, double tol, double* __restrict__ hist,
+                                   Flags* __restrict__ flags) {
+  const double rn = sqrt(s[5]);
+  hist[it] = rn;
+  flags->iterations = it;
+  if (rn < tol) flags->stop = 2;
+}
+
+// LomrStepFromSums as a kernel of its own, between the fused passes.
+__global__ __launch_bounds__(kStepBlk) void LomrFusedStep(const double* __restrict__ part, int G, int it, double tol,
+                                                      double* __restrict__ hist, double* __restrict__ sc,
+                                                      Flags* __restrict__ flags) {
+  if (flags->stop) return;
+  double s[6];
+  SumPartialsBlockFused<6>(part, G, s);
+  if (threadIdx.x != 0) return;
+  LomrStepFromSums(s, it, tol, hist, sc, flags);
 }
 
 // MR between passes it - 1 and it: history row it - 1, then the step
 // alpha = <Pr R, Pr A Z> / ||Pr A Z||^2 (minimal_residual.cpp:58-65).
-__global__ __launch_bounds__(256) void MrFusedStep(const double* __restrict__ part, int G, int it, double tol,
+__global__ __launch_bounds__(kStepBlk) void MrFusedStep(const double* __restrict__ part, int G, int it, double tol,
                                                     double* __restrict__ hist, double* __restrict__ sc,
                                                     Flags* __restrict__ flags) {
   if (flags->stop) return;
@@ -884,16 +921,13 @@ __global__ void ResetFlags(Flags* __restrict__ flags) {
 }
 
 // History row of the last pass (AppendResidual).
-__global__ __launch_bounds__(kBlk) void LomrFusedFinish(const double* __restrict__ part, int G, int it, double tol,
+__global__ __launch_bounds__(kStepBlk) void LomrFusedFinish(const double* __restrict__ part, int G, int it, double tol,
                                                         double* __restrict__ hist, Flags* __restrict__ flags) {
   if (flags->stop) return;
   double s[6];
   SumPartialsBlock<6>(part, G, s);
   if (threadIdx.x != 0) return;
-  const double rn = sqrt(s[5]);
-  hist[it] = rn;
-  flags->iterations = it;
-  if (rn < tol) flags->stop = 2;
+  LomrFinishFromSums(s, it, tol, hist, flags);
 }
 
 // R = I - A M; Z = Pr R on S (mode 1: P = Z and <R, Z>).
@@ -1712,6 +1746,49 @@ struct DiaPassArgs {
   int offS2[kDpS2];
 };
 
+// The diagonal pass's grid is at most kDiaMaxGrid blocks, so that a block of
+// kBlk threads sums the previous pass's partials with every load in flight
+// at once (kDiaMaxGrid / kBlk per thread and quantity, branch-free: the
+// clamped extra loads are not added).
+constexpr int kDiaMaxGrid = 1024;
+constexpr int kDiaPartials = kDiaMaxGrid / kBlk;
+
+// Whether the diagonal pass computes the LOMR step itself (FP32: one kernel
+// per iteration) or reads it from LomrFusedStep (FP64, where the pass holds
+// more registers and the separate step kernel measured faster).
+template <class V>
+constexpr bool DiaSelfStep() {
+  return sizeof(V) == 4;
+}
+
+// Sums of the 6 quantities over G <= kDiaMaxGrid block partials, the same in
+// every block that calls it (one fixed order).
+__device__ __forceinline__ void SumDiaPartials(const double* __restrict__ part, int G, double (&out)[6]) {
+  __shared__ double red[6][kBlk / kWave];
+  double v[6][kDiaPartials];
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+#pragma unroll
+    for (int u = 0; u < kDiaPartials; ++u) v[k][u] = part[k * G + min(int(threadIdx.x) + u * kBlk, G - 1)];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    double t = 0.0;
+#pragma unroll
+    for (int u = 0; u < kDiaPartials; ++u)
+      if (int(threadIdx.x) + u * kBlk < G) t += v[k][u];
+    t = WaveSum(t);
+    if (LaneId() == 0) red[k][WaveInBlock()] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBlk / kWave; ++w) t += red[k][w];
+    out[k] = t;
+  }
+}
+
 // One LOMR iteration (kInit: R = I - A M, Z = Pr R on S, P = 0, and the
 // first traces) on the diagonal layout, the work of FusedPass: M' = M +
 // delta Z + gamma P, P' = beta P + Z on S; R' = I - A M' on S2 (||R'||^2
@@ -1723,23 +1800,52 @@ struct DiaPassArgs {
 // of each S2 entry are added with S rows k ascending (S diagonals
 // descending), the order of the reference's products.
 template <class V, bool kInit, int kDims>
-__global__ __launch_bounds__(kBlk) void DiaLomrPass(int n, DiaPassArgs g, const unsigned* __restrict__ maskS,
+__global__ __launch_bounds__(kBlk) void DiaLomrPass(int n, int64_t ld, DiaPassArgs g, const unsigned* __restrict__ maskS,
                                                     const V* __restrict__ av, const V* __restrict__ pr,
                                                     V* __restrict__ M, V* __restrict__ Z, V* __restrict__ P,
-                                                    const double* __restrict__ sc, double* __restrict__ part_out,
-                                                    const Flags* __restrict__ flags) {
+                                                    const double* __restrict__ sc,
+                                                    const double* __restrict__ part_in, int g_in, int it,
+                                                    double tol, double* __restrict__ hist,
+                                                    double* __restrict__ part_out, Flags* __restrict__ flags) {
 #pragma clang fp contract(fast)
   constexpr DiaShape sh = MakeDiaShape(kDims);
   constexpr int ND = sh.nd, ND2 = sh.nd2;
   if (flags->stop) return;
   V d = V(0), gm = V(0), be = V(0);
-  if constexpr (!kInit) {
+  if constexpr (!kInit && !DiaSelfStep<V>()) {
     d = V(sc[kDelta]);
     gm = V(sc[kGamma]);
     be = V(sc[kBeta]);
+  } else if constexpr (!kInit) {
+    // The step of iteration it (LomrFusedStep) from the previous pass's
+    // partials, in every block: the same sums in the same order, so the same
+    // values everywhere.  Block 0 stores the history row, the iteration count
+    // and the stop flag (a block of this pass that reads the flag once set
+    // stops, as it would from its own values).
+    __shared__ double step[3];
+    __shared__ int stop;
+    double s[6];
+    SumDiaPartials(part_in, g_in, s);
+    if (threadIdx.x == 0) {
+      const LomrStep r = LomrStepValues(s, it, tol);
+      if (blockIdx.x == 0) {
+        hist[it - 1] = r.rn;
+        flags->iterations = it - 1;
+        if (r.stop) flags->stop = r.stop;
+      }
+      step[0] = r.delta;
+      step[1] = r.gamma;
+      step[2] = r.beta;
+      stop = r.stop;
+    }
+    __syncthreads();
+    if (stop) return;
+    d = V(step[0]);
+    gm = V(step[1]);
+    be = V(step[2]);
   }
   double red[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  const size_t ns = size_t(n);
+  const size_t ns = size_t(ld);
   for (int j = blockIdx.x * kBlk + threadIdx.x; j < n; j += gridDim.x * kBlk) {
     const unsigned mS = maskS[j];
     int kk[ND];  // S rows j - off (clamped: out of range only where S has no entry)
@@ -1847,19 +1953,29 @@ __global__ void ColDiaFill(int n, int nd, const int* __restrict__ ptr, const int
   mask[j] = m;
 }
 
-// out[k] = src[k] >= 0 ? v[src[k]] : 0 (CSC values into diagonal slots).
+// Diagonal slot k = d n + j of a layout with leading dimension ld (d ld + j).
+__device__ __forceinline__ int64_t DiaSlot(int64_t k, int n, int64_t ld) { return (k / n) * ld + k % n; }
+
+// out[slot k] = src[k] >= 0 ? v[src[k]] : 0 (CSC values into diagonal slots).
 template <class VI, class VO>
-__global__ void ToDiagonals(int64_t len, const int* __restrict__ src, const VI* __restrict__ v, VO* __restrict__ out) {
+__global__ void ToDiagonals(int64_t len, int n, int64_t ld, const int* __restrict__ src, const VI* __restrict__ v,
+                            VO* __restrict__ out) {
   const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (k < len) out[k] = src[k] >= 0 ? VO(v[src[k]]) : VO(0);
+  if (k < len) out[DiaSlot(k, n, ld)] = src[k] >= 0 ? VO(v[src[k]]) : VO(0);
 }
 
 // out[src[k]] = v[k] for the slots that hold an entry (diagonals back to CSC).
 template <class V>
-__global__ void FromDiagonals(int64_t len, const int* __restrict__ src, const V* __restrict__ v, V* __restrict__ out) {
+__global__ void FromDiagonals(int64_t len, int n, int64_t ld, const int* __restrict__ src, const V* __restrict__ v,
+                              V* __restrict__ out) {
   const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (k < len && src[k] >= 0) out[src[k]] = v[k];
+  if (k < len && src[k] >= 0) out[src[k]] = v[DiaSlot(k, n, ld)];
 }
+
+// Leading dimension of the diagonal pass's arrays: n rounded up to 64 values
+// plus kDiaPad, so that the diagonals do not start a power of two apart.
+constexpr int64_t kDiaPad = 0;
+int64_t DiaStride(int64_t n) { return (n + 63) / 64 * 64 + kDiaPad; }
 
 // Column-diagonal layout of a CSC pattern: offsets (host), masks and slot
 // sources (device); nd = 0 when it has more than max_nd diagonals or more
@@ -2220,18 +2336,20 @@ class FixedImpl final : public FixedProblem {
     if (!dims) return;
     for (int e = 0; e < s.nd; ++e) dargs_.offS[e] = s.off[e];
     for (int t = 0; t < s2.nd; ++t) dargs_.offS2[t] = s2.off[t];
-    const int64_t la = int64_t(a.nd) * n_, ls = int64_t(s.nd) * n_;
-    dav_ = c.Alloc<V>(la);
-    ToDiagonals<double, V><<<Blocks1(la), kBlk, 0, c.stream>>>(la, a.src.get(), A_.v(), dav_.get());
-    dM_ = c.Alloc<V>(ls);
-    dZ_ = c.Alloc<V>(ls);
-    dP_ = c.Alloc<V>(ls);
+    const int64_t la = int64_t(a.nd) * n_;
+    dld_ = DiaStride(n_);
+    dav_ = c.Alloc<V>(int64_t(a.nd) * dld_);
+    ToDiagonals<double, V><<<Blocks1(la), kBlk, 0, c.stream>>>(la, (int)n_, dld_, a.src.get(), A_.v(), dav_.get());
+    dM_ = c.Alloc<V>(int64_t(s.nd) * dld_);
+    dZ_ = c.Alloc<V>(int64_t(s.nd) * dld_);
+    dP_ = c.Alloc<V>(int64_t(s.nd) * dld_);
     dmask_s_ = std::move(s.mask);
     dsrc_s_ = std::move(s.src);
+    partf2_ = c.Alloc<double>(6 * size_t(kMaxFusedGrid));
     ddims_ = dims;
     dnd_ = s.nd;
     dnd2_ = s2.nd;
-    gridd_ = (unsigned)std::min<int64_t>(std::max<int64_t>((n_ + kBlk - 1) / kBlk, 1), kMaxFusedGrid);
+    gridd_ = (unsigned)std::min<int64_t>(std::max<int64_t>((n_ + kBlk - 1) / kBlk, 1), kDiaMaxGrid);
     dia_ = true;
   }
 
@@ -2270,16 +2388,21 @@ class FixedImpl final : public FixedProblem {
     SampleOnPattern<V><<<Blocks1(n_ * kSampleLanes), kBlk, 0, st>>>(
         (int)n_, s_pat_->ptr.get(), s_pat_->idx.get(), m0_.ptr(), m0_.idx(), m0_.v(), M_.get());
     const int64_t ls = int64_t(dnd_) * n_;
-    if (dia_) ToDiagonals<V, V><<<Blocks1(ls), kBlk, 0, st>>>(ls, dsrc_s_.get(), M_.get(), dM_.get());
+    if (dia_)
+      ToDiagonals<V, V><<<Blocks1(ls), kBlk, 0, st>>>(ls, (int)n_, dld_, dsrc_s_.get(), M_.get(), dM_.get());
     ResetFlags<<<1, 1, 0, st>>>(flags_.get());
     double* part = partf_.get();
     const double tol = prm_.tolerance;
     const int G = int(dia_ ? gridd_ : gridf_);
+    // the diagonal pass writes its partials to two buffers in turn (the FP32
+    // pass computes the step itself, from the partials of the pass before)
+    double* const pbuf[2] = {partf_.get(), partf2_.get()};
     for (int64_t it = 0; it <= prm_.max_iterations; ++it) {
       if (it > 0 && mr)
-        MrFusedStep<<<1, 256, 0, st>>>(part, G, (int)it, tol, hist_.get(), sc_.get(), flags_.get());
-      else if (it > 0)
-        LomrFusedStep<<<1, 256, 0, st>>>(part, G, (int)it, tol, hist_.get(), sc_.get(), flags_.get());
+        MrFusedStep<<<1, kStepBlk, 0, st>>>(part, G, (int)it, tol, hist_.get(), sc_.get(), flags_.get());
+      else if (it > 0 && !(dia_ && DiaSelfStep<V>()))
+        LomrFusedStep<<<1, kStepBlk, 0, st>>>(dia_ ? pbuf[(it - 1) & 1] : part, G, (int)it, tol, hist_.get(),
+                                              sc_.get(), flags_.get());
 #define SPAI_FUSED_STASH(SlotT, plan, GLV, INIT, MR, STASH)                                                   \
   FusedPass<V, SlotT, GLV, INIT, MR, STASH><<<gridf_, kBlk, smemf_, st>>>(                                     \
       (int)n_, s_pat_->ptr.get(), a0c_.get(), stq_.get(), s2d_.get(), s2_pat_->ptr.get(), av_.get(),           \
@@ -2323,9 +2446,9 @@ class FixedImpl final : public FixedProblem {
       if (timed) SPAI_HIP_CHECK(hipEventRecord(events_[2 * (it - 1)], st));
       if (dia_) {
         if (it == 0)
-          LaunchDiaPass<true>();
+          LaunchDiaPass<true>(0, nullptr, pbuf[0]);
         else
-          LaunchDiaPass<false>();
+          LaunchDiaPass<false>((int)it, pbuf[(it - 1) & 1], pbuf[it & 1]);
       } else if (slot16_) {
         SPAI_FUSED_GL(uint16_t, plan16_.get());
       } else {
@@ -2337,8 +2460,9 @@ class FixedImpl final : public FixedProblem {
 #undef SPAI_FUSED_ONE
 #undef SPAI_FUSED_STASH
     }
-    LomrFusedFinish<<<1, kBlk, 0, st>>>(part, G, (int)prm_.max_iterations, tol, hist_.get(), flags_.get());
-    if (dia_) FromDiagonals<V><<<Blocks1(ls), kBlk, 0, st>>>(ls, dsrc_s_.get(), dM_.get(), M_.get());
+    LomrFusedFinish<<<1, kStepBlk, 0, st>>>(dia_ ? pbuf[prm_.max_iterations & 1] : part, G,
+                                            (int)prm_.max_iterations, tol, hist_.get(), flags_.get());
+    if (dia_) FromDiagonals<V><<<Blocks1(ls), kBlk, 0, st>>>(ls, (int)n_, dld_, dsrc_s_.get(), dM_.get(), M_.get());
     if (record) {
       SPAI_HIP_CHECK(hipStreamEndCapture(st, &build_graph_));
       SPAI_HIP_CHECK(hipGraphInstantiate(&build_exec_, build_graph_, nullptr, nullptr, 0));
@@ -2364,11 +2488,11 @@ class FixedImpl final : public FixedProblem {
   }
 
   template <bool kInit>
-  void LaunchDiaPass() {
-#define SPAI_DIA_PASS(DIMS)                                                                                \
-  DiaLomrPass<V, kInit, DIMS><<<gridd_, kBlk, 0, ctx_.stream>>>((int)n_, dargs_, dmask_s_.get(), dav_.get(), \
-                                                                prv_.get(), dM_.get(), dZ_.get(), dP_.get(),  \
-                                                                sc_.get(), partf_.get(), flags_.get())
+  void LaunchDiaPass(int it, const double* part_in, double* part_out) {
+#define SPAI_DIA_PASS(DIMS)                                                                                   \
+  DiaLomrPass<V, kInit, DIMS><<<gridd_, kBlk, 0, ctx_.stream>>>(                                              \
+      (int)n_, dld_, dargs_, dmask_s_.get(), dav_.get(), prv_.get(), dM_.get(), dZ_.get(), dP_.get(), sc_.get(), \
+      part_in, (int)gridd_, it, prm_.tolerance, hist_.get(), part_out, flags_.get())
     switch (ddims_) {
       case 1: SPAI_DIA_PASS(1); break;
       case 2: SPAI_DIA_PASS(2); break;
@@ -2608,9 +2732,11 @@ class FixedImpl final : public FixedProblem {
   bool dia_ = false;
   unsigned gridd_ = 1;
   int ddims_ = 0, dnd_ = 0, dnd2_ = 0;
+  int64_t dld_ = 0;  // leading dimension of the diagonal arrays
   DiaPassArgs dargs_{};
   DBuf<unsigned> dmask_s_;
   DBuf<int> dsrc_s_;
+  DBuf<double> partf2_;  // the diagonal pass's second partials buffer
   DBuf<V> dav_, dM_, dZ_, dP_;
 
   // Launches the slot product with epilogue e over Y = (y0, y1).
