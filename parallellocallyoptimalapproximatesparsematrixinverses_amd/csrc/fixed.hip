@@ -1753,14 +1753,6 @@ struct DiaPassArgs {
 constexpr int kDiaMaxGrid = 1024;
 constexpr int kDiaPartials = kDiaMaxGrid / kBlk;
 
-// Whether the diagonal pass computes the LOMR step itself (FP32: one kernel
-// per iteration) or reads it from LomrFusedStep (FP64, where the pass holds
-// more registers and the separate step kernel measured faster).
-template <class V>
-constexpr bool DiaSelfStep() {
-  return sizeof(V) == 4;
-}
-
 // Sums of the 6 quantities over G <= kDiaMaxGrid block partials, the same in
 // every block that calls it (one fixed order).
 __device__ __forceinline__ void SumDiaPartials(const double* __restrict__ part, int G, double (&out)[6]) {
@@ -1798,25 +1790,40 @@ __device__ __forceinline__ void SumDiaPartials(const double* __restrict__ part, 
 // are equal in exact arithmetic, and R, 25 diagonals for a 3-D stencil, is
 // then never stored -- the pass reads and writes only M, Z and P.  The terms
 // of each S2 entry are added with S rows k ascending (S diagonals
-// descending), the order of the reference's products.
-template <class V, bool kInit, int kDims>
+// descending), the order of the reference's products.  A pass after the
+// first also does LomrFusedStep's work (history row, stop decision, the step
+// of its iteration) from the previous pass's partials, so no kernel runs
+// between passes.
+template <class V, bool kInit, int kDims, bool kOneCol>
 __global__ __launch_bounds__(kBlk) void DiaLomrPass(int n, int64_t ld, DiaPassArgs g, const unsigned* __restrict__ maskS,
                                                     const V* __restrict__ av, const V* __restrict__ pr,
                                                     V* __restrict__ M, V* __restrict__ Z, V* __restrict__ P,
-                                                    const double* __restrict__ sc,
                                                     const double* __restrict__ part_in, int g_in, int it,
                                                     double tol, double* __restrict__ hist,
                                                     double* __restrict__ part_out, Flags* __restrict__ flags) {
 #pragma clang fp contract(fast)
   constexpr DiaShape sh = MakeDiaShape(kDims);
   constexpr int ND = sh.nd, ND2 = sh.nd2;
+  const size_t ns = size_t(ld);
+  const int j0 = blockIdx.x * kBlk + threadIdx.x;
+  // a column's mask and M, Z, P; with kOneCol (the grid covers n, one column
+  // a thread) loaded before the stop flag and the step values arrive
+  unsigned mS = 0;
+  V mo[ND], zo[ND], po[ND];
+  auto load = [&](int j) {
+    mS = maskS[j];
+#pragma unroll
+    for (int e = 0; e < ND; ++e) {
+      const size_t at = size_t(e) * ns + size_t(j);
+      mo[e] = M[at];
+      zo[e] = kInit ? V(0) : Z[at];
+      po[e] = kInit ? V(0) : P[at];
+    }
+  };
+  if constexpr (kOneCol) load(min(j0, n - 1));
   if (flags->stop) return;
   V d = V(0), gm = V(0), be = V(0);
-  if constexpr (!kInit && !DiaSelfStep<V>()) {
-    d = V(sc[kDelta]);
-    gm = V(sc[kGamma]);
-    be = V(sc[kBeta]);
-  } else if constexpr (!kInit) {
+  if constexpr (!kInit) {
     // The step of iteration it (LomrFusedStep) from the previous pass's
     // partials, in every block: the same sums in the same order, so the same
     // values everywhere.  Block 0 stores the history row, the iteration count
@@ -1845,25 +1852,23 @@ __global__ __launch_bounds__(kBlk) void DiaLomrPass(int n, int64_t ld, DiaPassAr
     be = V(step[2]);
   }
   double red[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  const size_t ns = size_t(ld);
-  for (int j = blockIdx.x * kBlk + threadIdx.x; j < n; j += gridDim.x * kBlk) {
-    const unsigned mS = maskS[j];
+  for (int j = j0; j < n; j += gridDim.x * kBlk) {
+    if constexpr (!kOneCol) load(j);
     int kk[ND];  // S rows j - off (clamped: out of range only where S has no entry)
     V mv[ND], pn[ND], zn[ND];
 #pragma unroll
     for (int e = 0; e < ND; ++e) {
       kk[e] = min(max(j - g.offS[e], 0), n - 1);
       const size_t at = size_t(e) * ns + size_t(j);
-      mv[e] = M[at];
+      mv[e] = mo[e];
       pn[e] = V(0);
       if constexpr (!kInit) {
-        const V zo = Z[at], po = P[at];
         V m = mv[e];
-        m = m + d * zo;
-        m = m + gm * po;
+        m = m + d * zo[e];
+        m = m + gm * po[e];
         mv[e] = m;
         M[at] = m;
-        pn[e] = po * be + zo;
+        pn[e] = po[e] * be + zo[e];
       }
     }
     // Z' = Pr R' on S: R' = I - A M' on the S2 diagonals that are S's
@@ -2394,15 +2399,14 @@ class FixedImpl final : public FixedProblem {
     double* part = partf_.get();
     const double tol = prm_.tolerance;
     const int G = int(dia_ ? gridd_ : gridf_);
-    // the diagonal pass writes its partials to two buffers in turn (the FP32
-    // pass computes the step itself, from the partials of the pass before)
+    // the diagonal pass computes the step itself, from the partials of the
+    // pass before (two buffers, written in turn)
     double* const pbuf[2] = {partf_.get(), partf2_.get()};
     for (int64_t it = 0; it <= prm_.max_iterations; ++it) {
       if (it > 0 && mr)
         MrFusedStep<<<1, kStepBlk, 0, st>>>(part, G, (int)it, tol, hist_.get(), sc_.get(), flags_.get());
-      else if (it > 0 && !(dia_ && DiaSelfStep<V>()))
-        LomrFusedStep<<<1, kStepBlk, 0, st>>>(dia_ ? pbuf[(it - 1) & 1] : part, G, (int)it, tol, hist_.get(),
-                                              sc_.get(), flags_.get());
+      else if (it > 0 && !dia_)
+        LomrFusedStep<<<1, kStepBlk, 0, st>>>(part, G, (int)it, tol, hist_.get(), sc_.get(), flags_.get());
 #define SPAI_FUSED_STASH(SlotT, plan, GLV, INIT, MR, STASH)                                                   \
   FusedPass<V, SlotT, GLV, INIT, MR, STASH><<<gridf_, kBlk, smemf_, st>>>(                                     \
       (int)n_, s_pat_->ptr.get(), a0c_.get(), stq_.get(), s2d_.get(), s2_pat_->ptr.get(), av_.get(),           \
@@ -2489,15 +2493,22 @@ class FixedImpl final : public FixedProblem {
 
   template <bool kInit>
   void LaunchDiaPass(int it, const double* part_in, double* part_out) {
-#define SPAI_DIA_PASS(DIMS)                                                                                   \
-  DiaLomrPass<V, kInit, DIMS><<<gridd_, kBlk, 0, ctx_.stream>>>(                                              \
-      (int)n_, dld_, dargs_, dmask_s_.get(), dav_.get(), prv_.get(), dM_.get(), dZ_.get(), dP_.get(), sc_.get(), \
-      part_in, (int)gridd_, it, prm_.tolerance, hist_.get(), part_out, flags_.get())
-    switch (ddims_) {
-      case 1: SPAI_DIA_PASS(1); break;
-      case 2: SPAI_DIA_PASS(2); break;
-      default: SPAI_DIA_PASS(3); break;
+#define SPAI_DIA_PASS(DIMS, ONE)                                                                              \
+  DiaLomrPass<V, kInit, DIMS, ONE><<<gridd_, kBlk, 0, ctx_.stream>>>(                                         \
+      (int)n_, dld_, dargs_, dmask_s_.get(), dav_.get(), prv_.get(), dM_.get(), dZ_.get(), dP_.get(), part_in, \
+      (int)gridd_, it, prm_.tolerance, hist_.get(), part_out, flags_.get())
+#define SPAI_DIA_PASS_DIMS(ONE)               \
+  switch (ddims_) {                           \
+    case 1: SPAI_DIA_PASS(1, ONE); break;     \
+    case 2: SPAI_DIA_PASS(2, ONE); break;     \
+    default: SPAI_DIA_PASS(3, ONE); break;    \
+  }
+    if (int64_t(gridd_) * kBlk >= n_) {
+      SPAI_DIA_PASS_DIMS(true);
+    } else {
+      SPAI_DIA_PASS_DIMS(false);
     }
+#undef SPAI_DIA_PASS_DIMS
 #undef SPAI_DIA_PASS
   }
 
