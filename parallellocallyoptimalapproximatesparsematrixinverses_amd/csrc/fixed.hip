@@ -1871,6 +1871,25 @@ __global__ __launch_bounds__(kBlk) void DiaLomrPass(int n, int64_t ld, DiaPassAr
         pn[e] = po[e] * be + zo[e];
       }
     }
+    // A's values of the column's pairs (es, ea): FP32 with one column a
+    // thread loads all of them once and holds them for both products (49
+    // registers for a 3-D stencil); otherwise they are loaded where they are
+    // used (holding them would cost a wave per SIMD)
+    constexpr bool kHold = sizeof(V) == 4 && kOneCol;
+    constexpr int NH = kHold ? ND : 1;
+    V aval[NH][NH];
+    if constexpr (kHold) {
+#pragma unroll
+      for (int es = 0; es < ND; ++es)
+#pragma unroll
+        for (int ea = 0; ea < ND; ++ea) aval[es][ea] = av[size_t(ea) * ns + size_t(kk[es])];
+    }
+    auto a_of = [&](int es, int ea) -> V {
+      if constexpr (kHold)
+        return aval[es][ea];
+      else
+        return av[size_t(ea) * ns + size_t(kk[es])];
+    };
     // Z' = Pr R' on S: R' = I - A M' on the S2 diagonals that are S's
 #pragma unroll
     for (int t = 0; t < ND2; ++t) {
@@ -1880,7 +1899,7 @@ __global__ __launch_bounds__(kBlk) void DiaLomrPass(int n, int64_t ld, DiaPassAr
       for (int es = ND - 1; es >= 0; --es)
 #pragma unroll
         for (int ea = 0; ea < ND; ++ea)
-          if (sh.t[es][ea] == t) am += av[size_t(ea) * ns + size_t(kk[es])] * mv[es];
+          if (sh.t[es][ea] == t) am += a_of(es, ea) * mv[es];
       V r = -am;
       if (sh.off2[t] == 0) r += V(1);
 #pragma unroll
@@ -1895,11 +1914,7 @@ __global__ __launch_bounds__(kBlk) void DiaLomrPass(int n, int64_t ld, DiaPassAr
     }
     // R' = I - A M' on all of S2 (the same terms in the same order, so the
     // same values on S's diagonals), ||R'||^2, A [Z' P'] and the traces
-    // (LomrProjectEpi order).  A's values are read again (from cache): an
-    // opaque copy of the pointer keeps the compiler from holding the first
-    // loop's in registers.
-    const V* av2 = av;
-    asm volatile("" : "+v"(av2));
+    // (LomrProjectEpi order)
 #pragma unroll
     for (int t = 0; t < ND2; ++t) {
       V am = V(0), az = V(0), ap = V(0);
@@ -1908,7 +1923,7 @@ __global__ __launch_bounds__(kBlk) void DiaLomrPass(int n, int64_t ld, DiaPassAr
 #pragma unroll
         for (int ea = 0; ea < ND; ++ea)
           if (sh.t[es][ea] == t) {
-            const V a = av2[size_t(ea) * ns + size_t(kk[es])];
+            const V a = a_of(es, ea);
             am += a * mv[es];
             az += a * zn[es];
             ap += a * pn[es];
