@@ -676,7 +676,9 @@ __device__ __forceinline__ double DiaRest(const DiaView& dv, const MV* __restric
 
 // K1 on the diagonal layout of A (one thread per row): the same work as
 // PcgDirSpmvA, with the first row's operands loaded before the prologue.
-template <int kChunks>
+// kPipe (more than two rows a thread): each row's operands are loaded while
+// the row before is summed and stored.
+template <int kChunks, bool kPipe>
 __global__ __launch_bounds__(kBlk) void PcgDirSpmvADia(int n, int par, DiaView dv, const double* __restrict__ av,
                                                        const double* __restrict__ z, const double* __restrict__ p_in,
                                                        double* __restrict__ p_out, double* __restrict__ Ap,
@@ -700,6 +702,30 @@ __global__ __launch_bounds__(kBlk) void PcgDirSpmvADia(int n, int par, DiaView d
   auto dir = [&](double zc, double pc) { return zc + beta * pc; };
   double acc = 0.0;
   for (int row = row0; row < n; row += G * kBlk) {
+    if constexpr (kPipe) {
+      // this row's operands are in c0; the next row's go into c1 now
+      const int nxt = row + G * kBlk;
+      DiaChunk<double, 2> c1;
+      unsigned m1 = 0;
+      double z1 = 0.0, p1 = 0.0;
+      if (nxt < n) {
+        c1.Load<0>(dv, av, z, p_in, nxt, n);
+        m1 = dv.mask[nxt];
+        z1 = z[nxt];
+        p1 = p_in[nxt];
+      }
+      double s = c0.Sum<0>(m0, dv.nd, dir, 0.0);
+      if constexpr (kChunks > 1) s = DiaRest<double, 2>(dv, av, z, p_in, row, n, m0, dir, s);
+      const double pr = z0 + beta * p0;
+      p_out[row] = pr;
+      Ap[row] = s;
+      acc += pr * s;
+      c0 = c1;
+      m0 = m1;
+      z0 = z1;
+      p0 = p1;
+      continue;
+    }
     if (row != row0) {
       c0.Load<0>(dv, av, z, p_in, row, n);
       m0 = dv.mask[row];
@@ -718,7 +744,8 @@ __global__ __launch_bounds__(kBlk) void PcgDirSpmvADia(int n, int par, DiaView d
 
 // K2 on the diagonal layout of M (one thread per row): the same work as
 // PcgUpdateSpmvM, with the first row's operands loaded before the prologue.
-template <class MV, int kChunks>
+// kPipe as in PcgDirSpmvADia.
+template <class MV, int kChunks, bool kPipe>
 __global__ __launch_bounds__(kBlk) void PcgUpdateSpmvMDia(int n, int par, DiaView dv, const MV* __restrict__ mv,
                                                           double* __restrict__ x, const double* __restrict__ r_in,
                                                           double* __restrict__ r_out, const double* __restrict__ p,
@@ -745,6 +772,36 @@ __global__ __launch_bounds__(kBlk) void PcgUpdateSpmvMDia(int n, int par, DiaVie
   auto upd = [&](double rc, double ac) { return rc - alpha * ac; };
   double rr = 0.0, rz = 0.0;
   for (int row = row0; row < n; row += G * kBlk) {
+    if constexpr (kPipe) {
+      // this row's operands are in c0; the next row's go into c1 now
+      const int nxt = row + G * kBlk;
+      DiaChunk<MV, 2> c1;
+      unsigned m1 = 0;
+      double x1 = 0.0, p1 = 0.0, r1 = 0.0, a1 = 0.0;
+      if (nxt < n) {
+        c1.template Load<0>(dv, mv, r_in, Ap, nxt, n);
+        m1 = dv.mask[nxt];
+        x1 = x[nxt];
+        p1 = p[nxt];
+        r1 = r_in[nxt];
+        a1 = Ap[nxt];
+      }
+      double zs = c0.template Sum<0>(m0, dv.nd, upd, 0.0);
+      if constexpr (kChunks > 1) zs = DiaRest<MV, 2>(dv, mv, r_in, Ap, row, n, m0, upd, zs);
+      const double ri = r0 - alpha * a0;
+      x[row] = x0 + alpha * p0;
+      r_out[row] = ri;
+      z[row] = zs;
+      rr += ri * ri;
+      rz += ri * zs;
+      c0 = c1;
+      m0 = m1;
+      x0 = x1;
+      p0 = p1;
+      r0 = r1;
+      a0 = a1;
+      continue;
+    }
     if (row != row0) {
       c0.template Load<0>(dv, mv, r_in, Ap, row, n);
       m0 = dv.mask[row];
@@ -1319,10 +1376,22 @@ void PcgSolver::DropGraphs() {
     __VA_ARGS__;                     \
   }
 
+// The iteration kernels on diagonals, also by whether a thread has more
+// than two rows (kPipe: each row's loads overlap the row before).
+#define SPAI_DIA_PIPE_DISPATCH(nd, pipe, ...) \
+  if (pipe) {                                 \
+    constexpr bool kPipe = true;              \
+    SPAI_DIA_DISPATCH(nd, __VA_ARGS__);       \
+  } else {                                    \
+    constexpr bool kPipe = false;             \
+    SPAI_DIA_DISPATCH(nd, __VA_ARGS__);       \
+  }
+
 void PcgSolver::Record(int first_parity, int64_t iterations) {
   const int sga = SubgroupFor(A_.n, A_.nnz), sgm = SubgroupFor(M_.n, M_.nnz);
   hipStream_t st = ctx_.stream;
   const int n = (int)n_;
+  const bool pipe = n_ > 2 * int64_t(grid_) * kBlk;
   for (int64_t it = 0; it < iterations; ++it) {
     const int par = int((first_parity + it) & 1);
     double* p_in = p_.get() + size_t(par) * n_;
@@ -1330,7 +1399,7 @@ void PcgSolver::Record(int first_parity, int64_t iterations) {
     const double* r_in = r_.get() + size_t(par) * n_;
     double* r_out = r_.get() + size_t(par ^ 1) * n_;
     if (a_layout_ == PcgLayout::kDia) {
-      SPAI_DIA_DISPATCH(a_dia_.nd, (PcgDirSpmvADia<kChunks><<<grid_, kBlk, 0, st>>>(
+      SPAI_DIA_PIPE_DISPATCH(a_dia_.nd, pipe, (PcgDirSpmvADia<kChunks, kPipe><<<grid_, kBlk, 0, st>>>(
                                        n, par, ViewOf(a_dia_), a_dia_.val.get(), z_.get(), p_in, p_out, ap_.get(),
                                        part_.get(), scal_.get(), ist_.get(), hist_.get())));
     } else if (a_layout_ == PcgLayout::kSell) {
@@ -1343,11 +1412,11 @@ void PcgSolver::Record(int first_parity, int64_t iterations) {
                                 ap_.get(), part_.get(), scal_.get(), ist_.get(), hist_.get())));
     }
     if (m_layout_ == PcgLayout::kDia && m_fp32_) {
-      SPAI_DIA_DISPATCH(m_dia_.nd, (PcgUpdateSpmvMDia<float, kChunks><<<grid_, kBlk, 0, st>>>(
+      SPAI_DIA_PIPE_DISPATCH(m_dia_.nd, pipe, (PcgUpdateSpmvMDia<float, kChunks, kPipe><<<grid_, kBlk, 0, st>>>(
                                        n, par, ViewOf(m_dia_), m_dia_.val32.get(), x_.get(), r_in, r_out, p_out,
                                        ap_.get(), z_.get(), part_.get(), scal_.get(), ist_.get())));
     } else if (m_layout_ == PcgLayout::kDia) {
-      SPAI_DIA_DISPATCH(m_dia_.nd, (PcgUpdateSpmvMDia<double, kChunks><<<grid_, kBlk, 0, st>>>(
+      SPAI_DIA_PIPE_DISPATCH(m_dia_.nd, pipe, (PcgUpdateSpmvMDia<double, kChunks, kPipe><<<grid_, kBlk, 0, st>>>(
                                        n, par, ViewOf(m_dia_), m_dia_.val.get(), x_.get(), r_in, r_out, p_out,
                                        ap_.get(), z_.get(), part_.get(), scal_.get(), ist_.get())));
     } else if (m_layout_ == PcgLayout::kSell && m_fp32_) {
