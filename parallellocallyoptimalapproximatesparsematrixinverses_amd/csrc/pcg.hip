@@ -676,7 +676,7 @@ __device__ __forceinline__ double DiaRest(const DiaView& dv, const MV* __restric
 
 // K1 on the diagonal layout of A (one thread per row): the same work as
 // PcgDirSpmvA, with the first row's operands loaded before the prologue.
-// kPipe (more than two rows a thread): each row's operands are loaded while
+// kPipe (more than one row a thread): each row's operands are loaded while
 // the row before is summed and stored.
 template <int kChunks, bool kPipe>
 __global__ __launch_bounds__(kBlk) void PcgDirSpmvADia(int n, int par, DiaView dv, const double* __restrict__ av,
@@ -1377,7 +1377,7 @@ void PcgSolver::DropGraphs() {
   }
 
 // The iteration kernels on diagonals, also by whether a thread has more
-// than two rows (kPipe: each row's loads overlap the row before).
+// than one row (kPipe: each row's loads overlap the row before).
 #define SPAI_DIA_PIPE_DISPATCH(nd, pipe, ...) \
   if (pipe) {                                 \
     constexpr bool kPipe = true;              \
@@ -1391,7 +1391,7 @@ void PcgSolver::Record(int first_parity, int64_t iterations) {
   const int sga = SubgroupFor(A_.n, A_.nnz), sgm = SubgroupFor(M_.n, M_.nnz);
   hipStream_t st = ctx_.stream;
   const int n = (int)n_;
-  const bool pipe = n_ > 2 * int64_t(grid_) * kBlk;
+  const bool pipe = n_ > int64_t(grid_) * kBlk;
   for (int64_t it = 0; it < iterations; ++it) {
     const int par = int((first_parity + it) & 1);
     double* p_in = p_.get() + size_t(par) * n_;
