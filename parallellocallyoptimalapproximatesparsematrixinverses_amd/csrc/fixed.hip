@@ -1666,43 +1666,70 @@ unsigned Blocks1(int64_t n) {
 }
 
 // ------------------------------------------------------------ diagonal pass
-// LOMR on a fixed pattern whose matrices are a grid Laplacian's: A and S on
-// the diagonals 0, +-1, +-m, +-m^2 (up to the grid's dimension; o = col -
-// row, ascending) and S2 on all their pairwise sums.  Every matrix is held
-// by columns and diagonals: entry (j - o_d, j) sits at d n + j, 0 where the
-// column has no entry there (a bit per diagonal of S's column mask says which
-// are entries).  Entry (i, j) of A Y (Y on S) is the sum over the S entries
-// (k, j) of A(i, k) Y(k, j); with the diagonals in this order the S2 diagonal
-// that pair (S diagonal es, A diagonal ea) lands on is a constant of the
-// shape, so one thread per column forms its products with every index known
-// at compile time, every load address known before any load returns, and
-// the loads of a wavefront contiguous.  Absent entries are exact zeros:
-// a product term with one adds nothing, and no mask is needed for A or S2.
-constexpr int kDpS = 8;    // at most this many diagonals in A and in S
-constexpr int kDpS2 = 32;  // at most this many in S2
+// LOMR and MR on a fixed pattern whose matrices are a grid stencil's: A and
+// S on the diagonals of a Laplacian (0, +-1, +-m, +-m^2, up to the grid's
+// dimension) or of the 2-dimensional box stencil (every offset a + b m with
+// a, b in {-1, 0, 1}), o = col - row ascending, and S2 on all their pairwise
+// sums.  Every matrix is held by columns and diagonals: entry (j - o_d, j)
+// sits at d ld + j, 0 where the column has no entry there (a bit per
+// diagonal of S's column mask says which are entries).  Entry (i, j) of A Y
+// (Y on S) is the sum over the S entries (k, j) of A(i, k) Y(k, j); with the
+// diagonals in this order the S2 diagonal that pair (S diagonal es, A
+// diagonal ea) lands on is a constant of the shape, so one thread per column
+// forms its products with every index known at compile time, every load
+// address known before any load returns, and the loads of a wavefront
+// contiguous.  Absent entries are exact zeros: a product term with one adds
+// nothing, and no mask is needed for A or S2.
+constexpr int kDpS = 9;    // at most this many diagonals in A and in S
+constexpr int kDpS2 = 25;  // at most this many in S2
+constexpr int kDpPairs = kDpS * kDpS;
 
 struct DiaShape {
-  int nd = 0, nd2 = 0;
-  long long off[kDpS] = {};          // diagonals of A and S, with a stand-in grid side
-  long long off2[kDpS2] = {};        // S2: their pairwise sums, ascending
-  signed char t[kDpS][kDpS] = {};    // S2 diagonal of off[es] + off[ea]
-  signed char s2OfS[kDpS] = {};      // S2 diagonal of off[es]
-  signed char sOfS2[kDpS2] = {};     // S diagonal of off2[t] (-1: none)
+  int nd = 0, nd2 = 0, np = 0;
+  long long off[kDpS] = {};        // diagonals of A and S, with a stand-in grid side
+  long long off2[kDpS2] = {};      // S2: their pairwise sums, ascending
+  signed char s2OfS[kDpS] = {};    // S2 diagonal of off[es]
+  signed char sOfS2[kDpS2] = {};   // S diagonal of off2[t] (-1: none)
+  // the pairs (S diagonal es, A diagonal ea): S2 diagonal t ascending, and
+  // for one t, es descending (S rows k = j - off[es] ascending, the order of
+  // the reference's products)
+  unsigned char pt[kDpPairs] = {}, pes[kDpPairs] = {}, pea[kDpPairs] = {};
 };
 
-// Shape of a dims-dimensional Laplacian stencil (dims = 1, 2, 3).  The side
-// 2^12 stands in for the real one: the order of the sums is the same for any
-// side of at least 5, and the host checks the real offsets against it.
-constexpr DiaShape MakeDiaShape(int dims) {
+// Shape kinds: 1, 2, 3 = Laplacian in 1, 2, 3 dimensions (3, 5, 7
+// diagonals); 5 = box stencil in 2 dimensions (9).  (The 27-point box, 729
+// terms a column, held 230+ VGPRs and ran 2.5 times slower than the fused
+// pass on a 128^3 MR build; it takes the fused pass.)  The side 2^12
+// stands in for the real one: the order of the offsets and of their sums is
+// the same for any side of at least 5, and the host checks the real offsets
+// against the shape.
+constexpr DiaShape MakeDiaShape(int kind) {
   DiaShape s{};
   long long o[kDpS] = {};
   int n = 0;
-  o[n++] = 0;
-  long long unit = 1;
-  for (int d = 0; d < dims; ++d) {
-    o[n++] = unit;
-    o[n++] = -unit;
-    unit *= 4096;
+  const bool box = kind > 3;
+  const int dims = box ? kind - 3 : kind;
+  if (box) {
+    int cnt = 1;
+    for (int d = 0; d < dims; ++d) cnt *= 3;
+    for (int c = 0; c < cnt; ++c) {
+      long long v = 0, unit = 1;
+      int x = c;
+      for (int d = 0; d < dims; ++d) {
+        v += (x % 3 - 1) * unit;
+        x /= 3;
+        unit *= 4096;
+      }
+      o[n++] = v;
+    }
+  } else {
+    o[n++] = 0;
+    long long unit = 1;
+    for (int d = 0; d < dims; ++d) {
+      o[n++] = unit;
+      o[n++] = -unit;
+      unit *= 4096;
+    }
   }
   for (int i = 1; i < n; ++i)
     for (int k = i; k > 0 && o[k - 1] > o[k]; --k) {
@@ -1710,32 +1737,58 @@ constexpr DiaShape MakeDiaShape(int dims) {
       o[k] = o[k - 1];
       o[k - 1] = x;
     }
-  long long u[kDpS * kDpS] = {};
-  int m = 0;
-  for (int i = 0; i < n; ++i)
-    for (int k = 0; k < n; ++k) u[m++] = o[i] + o[k];
-  for (int i = 1; i < m; ++i)
-    for (int k = i; k > 0 && u[k - 1] > u[k]; --k) {
-      const long long x = u[k];
-      u[k] = u[k - 1];
-      u[k - 1] = x;
-    }
+  // the distinct sums, kept sorted (binary search, then insertion)
   int n2 = 0;
-  for (int i = 0; i < m; ++i)
-    if (n2 == 0 || u[i] != s.off2[n2 - 1]) s.off2[n2++] = u[i];
+  for (int i = 0; i < n; ++i)
+    for (int e = 0; e < n; ++e) {
+      const long long v = o[i] + o[e];
+      int lo = 0, hi = n2;
+      while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (s.off2[mid] < v)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      if (lo < n2 && s.off2[lo] == v) continue;
+      for (int k = n2; k > lo; --k) s.off2[k] = s.off2[k - 1];
+      s.off2[lo] = v;
+      ++n2;
+    }
   s.nd = n;
   s.nd2 = n2;
   for (int k = 0; k < n2; ++k) s.sOfS2[k] = -1;
+  // S2 diagonal of every pair, then the pairs placed by t (counting sort;
+  // es descending within one t)
+  int tof[kDpS][kDpS] = {};
+  int count[kDpS2 + 1] = {};
+  for (int i = 0; i < n; ++i)
+    for (int e = 0; e < n; ++e) {
+      const long long v = o[i] + o[e];
+      int lo = 0, hi = n2;
+      while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (s.off2[mid] < v)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      tof[i][e] = lo;
+      ++count[lo + 1];
+    }
+  for (int k = 0; k < n2; ++k) count[k + 1] += count[k];
+  for (int i = n - 1; i >= 0; --i)
+    for (int e = 0; e < n; ++e) {
+      const int q = count[tof[i][e]]++;
+      s.pt[q] = (unsigned char)tof[i][e];
+      s.pes[q] = (unsigned char)i;
+      s.pea[q] = (unsigned char)e;
+    }
+  s.np = n * n;
   for (int i = 0; i < n; ++i) {
     s.off[i] = o[i];
-    for (int k = 0; k < n2; ++k) {
-      if (s.off2[k] == o[i]) {
-        s.s2OfS[i] = (signed char)k;
-        s.sOfS2[k] = (signed char)i;
-      }
-      for (int e = 0; e < n; ++e)
-        if (s.off2[k] == o[i] + o[e]) s.t[i][e] = (signed char)k;
-    }
+    s.s2OfS[i] = (signed char)tof[i][n / 2];  // o[n / 2] = 0
+    s.sOfS2[tof[i][n / 2]] = (signed char)i;
   }
   return s;
 }
@@ -1781,29 +1834,55 @@ __device__ __forceinline__ void SumDiaPartials(const double* __restrict__ part, 
   }
 }
 
-// One LOMR iteration (kInit: R = I - A M, Z = Pr R on S, P = 0, and the
-// first traces) on the diagonal layout, the work of FusedPass: M' = M +
-// delta Z + gamma P, P' = beta P + Z on S; R' = I - A M' on S2 (||R'||^2
-// into partial 5); Z' = Pr R' on S; then A [Z' P'] and the traces of the next
-// step (LomrProjectEpi order).  R' is formed from M' rather than updated as
-// R - delta A Z - gamma A P (locally_minimal_residual.cpp:92-107): the two
-// are equal in exact arithmetic, and R, 25 diagonals for a 3-D stencil, is
-// then never stored -- the pass reads and writes only M, Z and P.  The terms
-// of each S2 entry are added with S rows k ascending (S diagonals
-// descending), the order of the reference's products.  A pass after the
-// first also does LomrFusedStep's work (history row, stop decision, the step
-// of its iteration) from the previous pass's partials, so no kernel runs
+// MR's step between passes it - 1 and it from the summed partials (thread
+// 0): ||R|| of row it - 1 (converged iff ||R|| < tol: stop 2) and alpha =
+// <Pr R, Pr A Z> / ||Pr A Z||^2 (minimal_residual.cpp:58-65; breakdown: stop
+// 1), as MrFusedStep.
+struct MrStepOut {
+  int stop;
+  double rn, alpha;
+};
+__device__ MrStepOut MrStepValues(const double (&s)[6], double tol) {
+  MrStepOut r{0, sqrt(s[5]), 0.0};
+  if (r.rn < tol) {
+    r.stop = 2;
+    return r;
+  }
+  const double num = s[0], den = s[1];
+  if (!(isfinite(den) && fabs(den) > 0.0)) {
+    r.stop = 1;
+    return r;
+  }
+  r.alpha = num / den;
+  if (!isfinite(r.alpha)) r.stop = 1;
+  return r;
+}
+
+// One LOMR iteration (kMr false; kInit: R = I - A M, Z = Pr R on S, P = 0,
+// and the first traces) on the diagonal layout, the work of FusedPass: M' =
+// M + delta Z + gamma P, P' = beta P + Z on S; R' = I - A M' on S2 (||R'||^2
+// into partial 5); Z' = Pr R' on S; then A [Z' P'] and the traces of the
+// next step (LomrProjectEpi order).  With kMr, one MR iteration: M' = M +
+// alpha Z, R' = I - A M', Z' = Pr R', A Z' and MrProjectEpi's two traces.
+// R' is formed from M' rather than updated as R - delta A Z - gamma A P
+// (locally_minimal_residual.cpp:92-107; MR recomputes it as the reference
+// does, minimal_residual.cpp:31-93): the two are equal in exact arithmetic,
+// and R (25 diagonals for a 3-D Laplacian or a 9-point box) is never
+// stored -- the pass reads and writes only M, Z (and P).  A pass after the
+// first also does the step kernel's work (history row, stop decision, the
+// step of its iteration) from the previous pass's partials, so no kernel runs
 // between passes.
-template <class V, bool kInit, int kDims, bool kOneCol>
-__global__ __launch_bounds__(kBlk) void DiaLomrPass(int n, int64_t ld, DiaPassArgs g, const unsigned* __restrict__ maskS,
-                                                    const V* __restrict__ av, const V* __restrict__ pr,
-                                                    V* __restrict__ M, V* __restrict__ Z, V* __restrict__ P,
-                                                    const double* __restrict__ part_in, int g_in, int it,
-                                                    double tol, double* __restrict__ hist,
-                                                    double* __restrict__ part_out, Flags* __restrict__ flags) {
+template <class V, bool kInit, int kShape, bool kOneCol, bool kMr>
+__global__ __launch_bounds__(kBlk) void DiaPass(int n, int64_t ld, DiaPassArgs g, const unsigned* __restrict__ maskS,
+                                                const V* __restrict__ av, const V* __restrict__ pr,
+                                                V* __restrict__ M, V* __restrict__ Z, V* __restrict__ P,
+                                                const double* __restrict__ part_in, int g_in, int it, double tol,
+                                                double* __restrict__ hist, double* __restrict__ part_out,
+                                                Flags* __restrict__ flags) {
 #pragma clang fp contract(fast)
-  constexpr DiaShape sh = MakeDiaShape(kDims);
-  constexpr int ND = sh.nd, ND2 = sh.nd2;
+  constexpr DiaShape sh = MakeDiaShape(kShape);
+  constexpr int ND = sh.nd, NP = sh.np;
+  constexpr bool kDir = !kMr;  // LOMR's direction P
   const size_t ns = size_t(ld);
   const int j0 = blockIdx.x * kBlk + threadIdx.x;
   // a column's mask and M, Z, P; with kOneCol (the grid covers n, one column
@@ -1817,37 +1896,49 @@ __global__ __launch_bounds__(kBlk) void DiaLomrPass(int n, int64_t ld, DiaPassAr
       const size_t at = size_t(e) * ns + size_t(j);
       mo[e] = M[at];
       zo[e] = kInit ? V(0) : Z[at];
-      po[e] = kInit ? V(0) : P[at];
+      po[e] = (kInit || !kDir) ? V(0) : P[at];
     }
   };
   if constexpr (kOneCol) load(min(j0, n - 1));
   if (flags->stop) return;
   V d = V(0), gm = V(0), be = V(0);
   if constexpr (!kInit) {
-    // The step of iteration it (LomrFusedStep) from the previous pass's
-    // partials, in every block: the same sums in the same order, so the same
-    // values everywhere.  Block 0 stores the history row, the iteration count
-    // and the stop flag (a block of this pass that reads the flag once set
-    // stops, as it would from its own values).
+    // The step of iteration it (LomrFusedStep, MrFusedStep) from the
+    // previous pass's partials, in every block: the same sums in the same
+    // order, so the same values everywhere.  Block 0 stores the history row,
+    // the iteration count and the stop flag (a block of this pass that reads
+    // the flag once set stops, as it would from its own values).
     __shared__ double step[3];
     __shared__ int stop;
     double s[6];
     SumDiaPartials(part_in, g_in, s);
     if (threadIdx.x == 0) {
-      const LomrStep r = LomrStepValues(s, it, tol);
-      if (blockIdx.x == 0) {
-        hist[it - 1] = r.rn;
-        flags->iterations = it - 1;
-        if (r.stop) flags->stop = r.stop;
+      int st = 0;
+      double rn = 0.0;
+      if constexpr (kMr) {
+        const MrStepOut r = MrStepValues(s, tol);
+        st = r.stop;
+        rn = r.rn;
+        step[0] = r.alpha;
+        step[1] = step[2] = 0.0;
+      } else {
+        const LomrStep r = LomrStepValues(s, it, tol);
+        st = r.stop;
+        rn = r.rn;
+        step[0] = r.delta;
+        step[1] = r.gamma;
+        step[2] = r.beta;
       }
-      step[0] = r.delta;
-      step[1] = r.gamma;
-      step[2] = r.beta;
-      stop = r.stop;
+      if (blockIdx.x == 0) {
+        hist[it - 1] = rn;
+        flags->iterations = it - 1;
+        if (st) flags->stop = st;
+      }
+      stop = st;
     }
     __syncthreads();
     if (stop) return;
-    d = V(step[0]);
+    d = V(step[0]);  // MR: alpha
     gm = V(step[1]);
     be = V(step[2]);
   }
@@ -1865,17 +1956,19 @@ __global__ __launch_bounds__(kBlk) void DiaLomrPass(int n, int64_t ld, DiaPassAr
       if constexpr (!kInit) {
         V m = mv[e];
         m = m + d * zo[e];
-        m = m + gm * po[e];
+        if constexpr (kDir) {
+          m = m + gm * po[e];
+          pn[e] = po[e] * be + zo[e];
+        }
         mv[e] = m;
         M[at] = m;
-        pn[e] = po[e] * be + zo[e];
       }
     }
     // A's values of the column's pairs (es, ea): FP32 with one column a
-    // thread loads all of them once and holds them for both products (49
-    // registers for a 3-D stencil); otherwise they are loaded where they are
-    // used (holding them would cost a wave per SIMD)
-    constexpr bool kHold = sizeof(V) == 4 && kOneCol;
+    // thread and at most 49 pairs loads all of them once and holds them for
+    // both products; otherwise they are loaded where they are used (holding
+    // them would cost waves per SIMD)
+    constexpr bool kHold = sizeof(V) == 4 && kOneCol && NP <= 49;
     constexpr int NH = kHold ? ND : 1;
     V aval[NH][NH];
     if constexpr (kHold) {
@@ -1891,53 +1984,59 @@ __global__ __launch_bounds__(kBlk) void DiaLomrPass(int n, int64_t ld, DiaPassAr
         return av[size_t(ea) * ns + size_t(kk[es])];
     };
     // Z' = Pr R' on S: R' = I - A M' on the S2 diagonals that are S's
-#pragma unroll
-    for (int t = 0; t < ND2; ++t) {
-      if (sh.sOfS2[t] < 0) continue;
+    {
       V am = V(0);
 #pragma unroll
-      for (int es = ND - 1; es >= 0; --es)
+      for (int q = 0; q < NP; ++q) {
+        const int t = sh.pt[q], es = sh.pes[q], ea = sh.pea[q];
+        if (sh.sOfS2[t] < 0) continue;
+        am += a_of(es, ea) * mv[es];
+        if (q + 1 < NP && sh.pt[q + 1] == t) continue;
+        V r = -am;
+        if (sh.off2[t] == 0) r += V(1);
 #pragma unroll
-        for (int ea = 0; ea < ND; ++ea)
-          if (sh.t[es][ea] == t) am += a_of(es, ea) * mv[es];
-      V r = -am;
-      if (sh.off2[t] == 0) r += V(1);
-#pragma unroll
-      for (int e = 0; e < ND; ++e)
-        if (sh.sOfS2[t] == e) zn[e] = ((mS >> e) & 1u) ? pr[kk[e]] * r : V(0);
+        for (int e = 0; e < ND; ++e)
+          if (sh.sOfS2[t] == e) zn[e] = ((mS >> e) & 1u) ? pr[kk[e]] * r : V(0);
+        am = V(0);
+      }
     }
 #pragma unroll
     for (int e = 0; e < ND; ++e) {
       const size_t at = size_t(e) * ns + size_t(j);
       Z[at] = zn[e];
-      P[at] = pn[e];
+      if constexpr (kDir) P[at] = pn[e];
     }
     // R' = I - A M' on all of S2 (the same terms in the same order, so the
     // same values on S's diagonals), ||R'||^2, A [Z' P'] and the traces
-    // (LomrProjectEpi order)
-#pragma unroll
-    for (int t = 0; t < ND2; ++t) {
+    // (LomrProjectEpi / MrProjectEpi order)
+    {
       V am = V(0), az = V(0), ap = V(0);
 #pragma unroll
-      for (int es = ND - 1; es >= 0; --es)
-#pragma unroll
-        for (int ea = 0; ea < ND; ++ea)
-          if (sh.t[es][ea] == t) {
-            const V a = a_of(es, ea);
-            am += a * mv[es];
-            az += a * zn[es];
-            ap += a * pn[es];
-          }
-      V r = -am;
-      if (sh.off2[t] == 0) r += V(1);
-      red[5] += double(r) * double(r);
-      const V p2 = pr[min(max(j - g.offS2[t], 0), n - 1)];
-      const double daz = az, dap = ap, dpr = p2, dprr = double(p2 * r);
-      red[0] += daz * (dpr * daz);
-      red[1] += dap * (dpr * dap);
-      red[2] += daz * (dpr * dap);
-      red[3] += dprr * daz;
-      red[4] += dprr * dap;
+      for (int q = 0; q < NP; ++q) {
+        const int t = sh.pt[q], es = sh.pes[q], ea = sh.pea[q];
+        const V a = a_of(es, ea);
+        am += a * mv[es];
+        az += a * zn[es];
+        if constexpr (kDir) ap += a * pn[es];
+        if (q + 1 < NP && sh.pt[q + 1] == t) continue;
+        V r = -am;
+        if (sh.off2[t] == 0) r += V(1);
+        red[5] += double(r) * double(r);
+        const V p2 = pr[min(max(j - g.offS2[t], 0), n - 1)];
+        if constexpr (kMr) {
+          const double tmp = double(p2 * az);
+          red[0] += double(p2 * r) * tmp;
+          red[1] += tmp * tmp;
+        } else {
+          const double daz = az, dap = ap, dpr = p2, dprr = double(p2 * r);
+          red[0] += daz * (dpr * daz);
+          red[1] += dap * (dpr * dap);
+          red[2] += daz * (dpr * dap);
+          red[3] += dprr * daz;
+          red[4] += dprr * dap;
+        }
+        am = az = ap = V(0);
+      }
     }
   }
   StorePartials<6>(red, part_out);
@@ -1998,8 +2097,8 @@ constexpr int64_t kDiaPad = 0;
 int64_t DiaStride(int64_t n) { return (n + 63) / 64 * 64 + kDiaPad; }
 
 // Column-diagonal layout of a CSC pattern: offsets (host), masks and slot
-// sources (device); nd = 0 when it has more than max_nd diagonals or more
-// than 1.5 nnz slots.
+// sources (device; only the offsets when slots is false); nd = 0 when it has
+// more than max_nd diagonals or more than 1.5 nnz slots.
 struct ColDia {
   int nd = 0;
   std::vector<int> off;
@@ -2007,7 +2106,7 @@ struct ColDia {
   DBuf<unsigned> mask;
 };
 
-ColDia MakeColDia(Ctx& c, int64_t n, const Pattern& pat, int max_nd) {
+ColDia MakeColDia(Ctx& c, int64_t n, const Pattern& pat, int max_nd, bool slots = true) {
   ColDia out;
   const int64_t len = 2 * n - 1;
   auto flag = c.Alloc<int>(len);
@@ -2020,6 +2119,8 @@ ColDia MakeColDia(Ctx& c, int64_t n, const Pattern& pat, int max_nd) {
   ColDiaOffsets<<<Blocks1(len), kBlk, 0, c.stream>>>(len, (int)n, flag.get(), pos.get(), off.get());
   out.off.resize(nd);
   c.ReadBack(out.off.data(), off.get(), sizeof(int) * nd);
+  out.nd = (int)nd;
+  if (!slots) return out;
   out.src = c.Alloc<int>(nd * n);
   out.mask = c.Alloc<unsigned>(n);
   ColDiaFill<<<Blocks1(n), kBlk, 0, c.stream>>>((int)n, (int)nd, pat.ptr.get(), pat.idx.get(), pos.get(),
@@ -2196,7 +2297,7 @@ class FixedImpl final : public FixedProblem {
       prS2_ = c.Alloc<V>(nnz_s2_ + 16);  // padding: the fused pass loads whole 16-byte chunks
       GatherRows<V><<<Blocks1(nnz_s2_), kBlk, 0, c.stream>>>(nnz_s2_, s2_pat_->idx.get(), prv_.get(), prS2_.get());
     }
-    if (method_ == SPAI_GLOBAL_LOMR) SetUpDiaPass();
+    if (method_ == SPAI_GLOBAL_LOMR || method_ == SPAI_GLOBAL_MR) SetUpDiaPass();
     M_ = c.Alloc<V>(nnz_s_);
     Z_ = c.Alloc<V>(nnz_s_);
     P_ = c.Alloc<V>(nnz_s_);
@@ -2332,11 +2433,12 @@ class FixedImpl final : public FixedProblem {
     out[10] = dia_ ? dnd2_ : 0;
   }
 
-  // The diagonal pass (DiaLomrPass) for Global LOMR when A and S lie on the
-  // diagonals of a 1-, 2- or 3-dimensional Laplacian stencil and S2 on all
-  // their sums, in the order of MakeDiaShape, each layout with at most 1.5
-  // nnz slots.  The environment variable SPAI_FIXED_LAYOUT ("csr", or "auto"
-  // the default) can keep the general fused pass.
+  // The diagonal pass (DiaPass) for Global LOMR when A and S lie on the
+  // diagonals of a 1-, 2- or 3-dimensional Laplacian stencil, and for Global
+  // MR on those or a 2-dimensional 9-point box stencil's, with S2 on all their
+  // sums, in the order of MakeDiaShape, each layout with at most 1.5 nnz
+  // slots.  The environment variable SPAI_FIXED_LAYOUT ("csr", or "auto" the
+  // default) can keep the general fused pass.
   void SetUpDiaPass() {
     const char* env = std::getenv("SPAI_FIXED_LAYOUT");
     if (env && *env && std::strcmp(env, "auto") != 0) {
@@ -2348,12 +2450,18 @@ class FixedImpl final : public FixedProblem {
     if (!a.nd) return;
     ColDia s = MakeColDia(c, n_, *s_pat_, kDpS);
     if (s.nd != a.nd || s.off != a.off) return;
-    ColDia s2 = MakeColDia(c, n_, *s2_pat_, kDpS2);
+    ColDia s2 = MakeColDia(c, n_, *s2_pat_, kDpS2, false);
     if (!s2.nd) return;
-    int dims = 0;
-    for (int k = 1; k <= 3 && !dims; ++k)
-      if (MatchesShape(MakeDiaShape(k), s.off, s2.off)) dims = k;
-    if (!dims) return;
+    const bool mr = method_ == SPAI_GLOBAL_MR;
+    int kind = 0;
+    for (int k : {1, 2, 3, 5}) {
+      if (k > 3 && !mr) break;  // box stencils: MR only
+      if (MatchesShape(MakeDiaShape(k), s.off, s2.off)) {
+        kind = k;
+        break;
+      }
+    }
+    if (!kind) return;
     for (int e = 0; e < s.nd; ++e) dargs_.offS[e] = s.off[e];
     for (int t = 0; t < s2.nd; ++t) dargs_.offS2[t] = s2.off[t];
     const int64_t la = int64_t(a.nd) * n_;
@@ -2362,11 +2470,11 @@ class FixedImpl final : public FixedProblem {
     ToDiagonals<double, V><<<Blocks1(la), kBlk, 0, c.stream>>>(la, (int)n_, dld_, a.src.get(), A_.v(), dav_.get());
     dM_ = c.Alloc<V>(int64_t(s.nd) * dld_);
     dZ_ = c.Alloc<V>(int64_t(s.nd) * dld_);
-    dP_ = c.Alloc<V>(int64_t(s.nd) * dld_);
+    if (!mr) dP_ = c.Alloc<V>(int64_t(s.nd) * dld_);
     dmask_s_ = std::move(s.mask);
     dsrc_s_ = std::move(s.src);
     partf2_ = c.Alloc<double>(6 * size_t(kMaxFusedGrid));
-    ddims_ = dims;
+    ddims_ = kind;
     dnd_ = s.nd;
     dnd2_ = s2.nd;
     gridd_ = (unsigned)std::min<int64_t>(std::max<int64_t>((n_ + kBlk - 1) / kBlk, 1), kDiaMaxGrid);
@@ -2374,20 +2482,17 @@ class FixedImpl final : public FixedProblem {
   }
 
   // Whether the offsets of S (ascending, 0 in the middle) and S2 pair up as
-  // in shape sh: off[es] + off[ea] is S2 diagonal sh.t[es][ea] for all es, ea.
+  // in shape sh: off[es] + off[ea] is S2 diagonal t for every pair (t, es,
+  // ea) of the shape.
   static bool MatchesShape(const DiaShape& sh, const std::vector<int>& off, const std::vector<int>& off2) {
     if ((int)off.size() != sh.nd || (int)off2.size() != sh.nd2 || off[sh.nd / 2] != 0) return false;
-    for (int es = 0; es < sh.nd; ++es)
-      for (int ea = 0; ea < sh.nd; ++ea) {
-        const int64_t sum = int64_t(off[es]) + off[ea];
-        const int t = sh.t[es][ea];
-        if (off2[t] != sum) return false;
-      }
+    for (int q = 0; q < sh.np; ++q)
+      if (off2[sh.pt[q]] != int64_t(off[sh.pes[q]]) + off[sh.pea[q]]) return false;
     return true;
   }
 
   // Global LOMR or MR with one fused pass per iteration (FusedPass, or
-  // DiaLomrPass on the diagonal layout).
+  // DiaPass on the diagonal layout).
   int64_t BuildFused() {
     Ctx& c = ctx_;
     hipStream_t st = c.stream;
@@ -2418,7 +2523,7 @@ class FixedImpl final : public FixedProblem {
     // pass before (two buffers, written in turn)
     double* const pbuf[2] = {partf_.get(), partf2_.get()};
     for (int64_t it = 0; it <= prm_.max_iterations; ++it) {
-      if (it > 0 && mr)
+      if (it > 0 && mr && !dia_)
         MrFusedStep<<<1, kStepBlk, 0, st>>>(part, G, (int)it, tol, hist_.get(), sc_.get(), flags_.get());
       else if (it > 0 && !dia_)
         LomrFusedStep<<<1, kStepBlk, 0, st>>>(part, G, (int)it, tol, hist_.get(), sc_.get(), flags_.get());
@@ -2508,22 +2613,32 @@ class FixedImpl final : public FixedProblem {
 
   template <bool kInit>
   void LaunchDiaPass(int it, const double* part_in, double* part_out) {
-#define SPAI_DIA_PASS(DIMS, ONE)                                                                              \
-  DiaLomrPass<V, kInit, DIMS, ONE><<<gridd_, kBlk, 0, ctx_.stream>>>(                                         \
+#define SPAI_DIA_PASS(KIND, ONE, MR)                                                                          \
+  DiaPass<V, kInit, KIND, ONE, MR><<<gridd_, kBlk, 0, ctx_.stream>>>(                                         \
       (int)n_, dld_, dargs_, dmask_s_.get(), dav_.get(), prv_.get(), dM_.get(), dZ_.get(), dP_.get(), part_in, \
       (int)gridd_, it, prm_.tolerance, hist_.get(), part_out, flags_.get())
-#define SPAI_DIA_PASS_DIMS(ONE)               \
-  switch (ddims_) {                           \
-    case 1: SPAI_DIA_PASS(1, ONE); break;     \
-    case 2: SPAI_DIA_PASS(2, ONE); break;     \
-    default: SPAI_DIA_PASS(3, ONE); break;    \
+#define SPAI_DIA_PASS_LAPLACE(ONE, MR)          \
+  switch (ddims_) {                             \
+    case 1: SPAI_DIA_PASS(1, ONE, MR); break;   \
+    case 2: SPAI_DIA_PASS(2, ONE, MR); break;   \
+    default: SPAI_DIA_PASS(3, ONE, MR); break;  \
   }
-    if (int64_t(gridd_) * kBlk >= n_) {
-      SPAI_DIA_PASS_DIMS(true);
+    const bool one = int64_t(gridd_) * kBlk >= n_;
+    if (ddims_ > 3) {
+      // the 9-point box (MR): one instance, any n
+      SPAI_DIA_PASS(5, false, true);
+    } else if (method_ == SPAI_GLOBAL_MR) {
+      if (one) {
+        SPAI_DIA_PASS_LAPLACE(true, true);
+      } else {
+        SPAI_DIA_PASS_LAPLACE(false, true);
+      }
+    } else if (one) {
+      SPAI_DIA_PASS_LAPLACE(true, false);
     } else {
-      SPAI_DIA_PASS_DIMS(false);
+      SPAI_DIA_PASS_LAPLACE(false, false);
     }
-#undef SPAI_DIA_PASS_DIMS
+#undef SPAI_DIA_PASS_LAPLACE
 #undef SPAI_DIA_PASS
   }
 
@@ -2757,7 +2872,7 @@ class FixedImpl final : public FixedProblem {
   // offsets, S's column masks and slot sources, values on diagonals
   bool dia_ = false;
   unsigned gridd_ = 1;
-  int ddims_ = 0, dnd_ = 0, dnd2_ = 0;
+  int ddims_ = 0, dnd_ = 0, dnd2_ = 0;  // ddims_: the shape kind of MakeDiaShape
   int64_t dld_ = 0;  // leading dimension of the diagonal arrays
   DiaPassArgs dargs_{};
   DBuf<unsigned> dmask_s_;

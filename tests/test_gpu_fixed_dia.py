@@ -1,10 +1,12 @@
-"""Fixed-pattern Global LOMR on the diagonal pass
-(src/global_spai/locally_minimal_residual.cpp:54-118 on the fixed pattern).
+"""Fixed-pattern Global LOMR and MR on the diagonal pass
+(src/global_spai/locally_minimal_residual.cpp:54-118,
+minimal_residual.cpp:31-93 on the fixed pattern).
 
-When A, S = pattern(A) | pattern(M0) | I and S2 = pattern(A S) | S | I lie on
-few diagonals (A and S at most 8, S2 at most 32, each with at most 1.5 nnz
-slots), LOMR runs one thread per column on the diagonal layout instead of the
-general fused pass.  Its products add the pairs of S2 entry (i, j) in the
+When A and S = pattern(A) | pattern(M0) | I lie on a grid Laplacian's
+diagonals (LOMR and MR) or the 2-D 9-point box stencil's (MR), and
+S2 = pattern(A S) | S | I on all their sums, each layout with at most 1.5
+nnz slots, the build runs one thread per column on the diagonal layout
+instead of the general fused pass.  Its products add the pairs of S2 entry (i, j) in the
 reference's order (S rows k ascending), so the results match the oracle to
 the same bars as the fused pass: FP64 M within 1e-12 relative Frobenius and
 histories within 1e-10; FP32 bulk within 1e-4 of the FP64 oracle with equal
@@ -147,19 +149,80 @@ def test_diagonal_pass_replay_and_timing(gpu_lib):
     assert_identical(runs[0], runs[2])
 
 
+def _box2d(m):
+    """9-point box stencil on an m x m grid: kron(T, T) with T = tridiag(-1,
+    4, -1) (SPD)."""
+    T = sparse.diags([-np.ones(m - 1), 4.0 * np.ones(m), -np.ones(m - 1)], [-1, 0, 1])
+    A = sparse.csc_matrix(sparse.kron(T, T))
+    A.sort_indices()
+    return A
+
+
+_MR_CASES = {
+    # (matrix, diagonals of A, of S2)
+    "poisson2d_40": (lambda: problems.poisson2d(40), 5, 13),
+    "diffusion3d_16": (lambda: problems.diffusion3d(16, seed=1), 7, 25),
+    "box2d_30": (lambda: _box2d(30), 9, 25),
+}
+
+
+def _session_method(gpu_lib, method, A, M0, iters, precision, layout="auto"):
+    with env_set("SPAI_FIXED_LAYOUT", layout):
+        return gpu_lib.Session(method, A, M0=M0, max_iterations=iters, precision=precision)
+
+
+@pytest.mark.parametrize("case", sorted(_MR_CASES))
+def test_diagonal_mr_matches_oracle(gpu_lib, oracle_lib, case):
+    """MR on the diagonal pass: FP64 against the oracle (M 1e-12, history
+    1e-10), FP32 against the FP64 oracle (1e-4), and FP64 against the fused
+    pass (1e-13)."""
+    make, na, ns2 = _MR_CASES[case]
+    A = make()
+    Pr = problems.diag_inverse(A)
+    o = oracle_lib.run("mr", A, Pr, None, 8, 0.0, 1.0, False, 1, fixed_pattern=True)
+    res = {}
+    for precision, layout in (("fp64", "auto"), ("fp32", "auto"), ("fp64", "csr")):
+        print(f"case: dia mr {case} {precision} {layout}", flush=True)
+        s = _session_method(gpu_lib, "mr", A, Pr, 8, precision, layout)
+        try:
+            info = s.info()
+            assert info["fused"] == (2 if layout == "auto" else 1), info
+            if layout == "auto":
+                assert (info["diagonals_a"], info["diagonals_s2"]) == (na, ns2), info
+            it = s.build()
+            res[(precision, layout)] = (it, s.result())
+        finally:
+            s.close()
+    it, g = res[("fp64", "auto")]
+    assert it == o["iterations"] == 8
+    assert_same_result(g, o, hist_rtol=1e-10, m_rtol=1e-12)
+    it32, g32 = res[("fp32", "auto")]
+    assert it32 == 8
+    assert rel_fro(g32["M"], o["M"]) <= 1e-4
+    np.testing.assert_allclose(g32["history"]["residual_norm"], o["history"]["residual_norm"], rtol=1e-4)
+    _, gf = res[("fp64", "csr")]
+    assert rel_fro(g["M"], gf["M"]) <= 1e-13
+
+
 def test_diagonal_pass_fallbacks(gpu_lib):
-    """Off the diagonal pass: A on 27 diagonals (> 8), MR (the pass is LOMR's
-    only), and SPAI_FIXED_LAYOUT=csr; an unknown value is refused."""
+    """Off the diagonal pass: the 27-point box (LOMR and MR), LOMR on the
+    9-point box, a pattern off every stencil shape (random Gram), and
+    SPAI_FIXED_LAYOUT=csr; an unknown value is refused."""
     A27 = problems.stencil27(8, seed=4)
-    s = _session(gpu_lib, A27, problems.diag_inverse(A27), 3, "fp64")
-    assert s.info()["fused"] == 1 and s.info()["diagonals_s2"] == 0
+    for method in ("lomr", "mr"):
+        s = _session_method(gpu_lib, method, A27, problems.diag_inverse(A27), 3, "fp64")
+        assert s.info()["fused"] == 1 and s.info()["diagonals_s2"] == 0
+        s.close()
+    B = _box2d(20)
+    s = _session(gpu_lib, B, problems.diag_inverse(B), 3, "fp64")
+    assert s.info()["fused"] == 1
+    s.close()
+    G = problems.random_gram(2000, per_row=3, seed=3, power_iterations=10)[0]
+    s = _session_method(gpu_lib, "mr", G, problems.diag_inverse(G), 3, "fp64")
+    assert s.info()["fused"] != 2
     s.close()
     A = problems.poisson2d(20)
     Pr = problems.diag_inverse(A)
-    with env_set("SPAI_FIXED_LAYOUT", "auto"):
-        s = gpu_lib.Session("mr", A, M0=Pr, max_iterations=3)
-    assert s.info()["fused"] == 1
-    s.close()
     s = _session(gpu_lib, A, Pr, 3, "fp64", layout="csr")
     assert s.info()["fused"] == 1
     s.close()
