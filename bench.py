@@ -124,27 +124,29 @@ def fused_pass_bytes(n, nnz_a, nnz_s, nnz_s2, sv, method):
     return base + nnz_s2 * sv + 4 * nnz_s * sv
 
 
-def dia_pass_bytes(n, nd_a, nd_s, sv):
-    """Algorithmic bytes of one LOMR iteration on the diagonal pass (s_v bytes
-    per value, every slot of the layout counted): A's values once (nd_a n
-    s_v), S's column masks (4 n), M, Z and P read and written (6 nd_s n s_v)
-    and Pr once (n s_v); R' is formed from M' in registers and never stored."""
-    return nd_a * n * sv + 4 * n + 6 * nd_s * n * sv + n * sv
+def dia_pass_bytes(n, nd_a, nd_s, sv, method="lomr"):
+    """Algorithmic bytes of one LOMR or MR iteration on the diagonal pass (s_v
+    bytes per value, every slot of the layout counted): A's values once (nd_a
+    n s_v), S's column masks (4 n), M, Z and (LOMR) P read and written (6 or
+    4 nd_s n s_v) and Pr once (n s_v); R' is formed from M' in registers and
+    never stored."""
+    arrays = 6 if method == "lomr" else 4
+    return nd_a * n * sv + 4 * n + arrays * nd_s * n * sv + n * sv
 
 
 def lomr_pass_bytes(n, nnz_a, info, sv, method):
     """Algorithmic bytes of the per-iteration pass the session's build runs
     (info = Session.info()): the diagonal pass or the general fused pass."""
     if info["fused"] == 2:
-        return dia_pass_bytes(n, info["diagonals_a"], info["diagonals_s"], sv)
+        return dia_pass_bytes(n, info["diagonals_a"], info["diagonals_s"], sv, method)
     return fused_pass_bytes(n, nnz_a, info["nnz_s"], info["nnz_s2"], sv, method)
 
 
 def lomr_pass_kernel(info, method, precision):
     """Name of the per-iteration pass kernel of a session's build."""
     if info["fused"] == 2:
-        dims = {3: 1, 5: 2, 7: 3}.get(info["diagonals_s"], "?")
-        return (f"DiaLomrPass<{precision}, {dims}-D stencil> (one LOMR iteration on the diagonal layout: "
+        shape = {3: "1-D Laplacian", 5: "2-D Laplacian", 7: "3-D Laplacian", 9: "2-D box"}.get(info["diagonals_s"], "?")
+        return (f"DiaPass<{precision}, {shape}> (one {method.upper()} iteration on the diagonal layout: "
                 f"{info['diagonals_s']} diagonals in A and S, R' = I - A M' formed in registers)")
     return "FusedPass<%s, %s> (one fused %s iteration)" % (precision, method.upper(), method.upper())
 
