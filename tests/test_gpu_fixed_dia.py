@@ -228,3 +228,31 @@ def test_diagonal_pass_fallbacks(gpu_lib):
     s.close()
     with pytest.raises(RuntimeError, match="SPAI_FIXED_LAYOUT"):
         _session(gpu_lib, A, Pr, 3, "fp64", layout="diagonal")
+
+
+@pytest.mark.parametrize("method, make", [
+    ("lomr", lambda: problems.diffusion3d(70, seed=1)),
+    ("mr", lambda: problems.poisson2d(600)),
+], ids=["lomr_diffusion3d_70", "mr_poisson2d_600"])
+def test_diagonal_pass_several_columns_per_thread(gpu_lib, oracle_lib, method, make):
+    """n above the pass's 1,024 x 256 threads (343,000 and 360,000 columns):
+    each thread walks several columns, loading each one's operands in the
+    loop.  FP64 against the oracle (4 iterations: M 1e-12, history 1e-10)
+    and against the fused pass (1e-13)."""
+    A = make()
+    Pr = problems.diag_inverse(A)
+    assert A.shape[0] > 1024 * 256
+    res = {}
+    for layout in ("auto", "csr"):
+        print(f"case: dia {method} several columns per thread, {layout}", flush=True)
+        s = _session_method(gpu_lib, method, A, Pr, 4, "fp64", layout)
+        try:
+            assert s.info()["fused"] == (2 if layout == "auto" else 1)
+            res[layout] = (s.build(), s.result())
+        finally:
+            s.close()
+    o = oracle_lib.run(method, A, Pr, None, 4, 0.0, 1.0, False, 16, fixed_pattern=True)
+    it, g = res["auto"]
+    assert it == o["iterations"] == 4
+    assert_same_result(g, o, hist_rtol=1e-10, m_rtol=1e-12)
+    assert rel_fro(g["M"], res["csr"][1]["M"]) <= 1e-13
