@@ -112,13 +112,25 @@ __device__ __forceinline__ void InnerProduct2(const int* __restrict__ ai, const 
   Gather<V, 2>(ai, av, ab, ae, si, y0, y1, sb, se, s0, s1);
 }
 
+// The block's sums of K quantities into part[k * gridDim.x + blockIdx.x]:
+// BlockSum's order for each (wavefront sums, then the wavefronts in order),
+// all K reduced with one barrier; thread k stores quantity k.  Called once,
+// at the end of a kernel.
 template <int K>
 __device__ __forceinline__ void StorePartials(double (&acc)[K], double* __restrict__ part) {
-  __shared__ double red[16];
+  static_assert(K <= kBlk, "one thread per quantity");
+  __shared__ double red[K][16];
+  const int w = WaveInBlock(), nw = (int(blockDim.x) + kWave - 1) / kWave;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const double t = BlockSum<double>(acc[k], red);
-    if (threadIdx.x == 0) part[k * gridDim.x + blockIdx.x] = t;
+    const double v = WaveSum(acc[k]);
+    if (LaneId() == 0) red[k][w] = v;
+  }
+  __syncthreads();
+  if (int(threadIdx.x) < K) {
+    double t = 0.0;
+    for (int i = 0; i < nw; ++i) t += red[threadIdx.x][i];
+    part[int(threadIdx.x) * gridDim.x + blockIdx.x] = t;
   }
 }
 
