@@ -129,18 +129,36 @@ __device__ __forceinline__ double RowDotDir(const int* __restrict__ ptr, const i
 
 // Block sum of v written to part[blockIdx.x].
 __device__ __forceinline__ void StorePartial(double v, double* __restrict__ part) {
+  // BlockSum's order (wavefront sums, then the wavefronts in order) with one
+  // barrier: called once, at the end of a kernel
   __shared__ double red[16];
-  v = BlockSum<double>(v, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = v;
+  v = WaveSum(v);
+  if (LaneId() == 0) red[WaveInBlock()] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (int(blockDim.x) + kWave - 1) / kWave;
+    double t = 0.0;
+    for (int i = 0; i < nw; ++i) t += red[i];
+    part[blockIdx.x] = t;
+  }
 }
 
-// Block sums of a and b written to pa[blockIdx.x] and pb[blockIdx.x].
+// Block sums of a and b written to pa[blockIdx.x] and pb[blockIdx.x]
+// (BlockSum's order for each, one barrier; at the end of a kernel).
 __device__ __forceinline__ void StorePartial2(double a, double b, double* __restrict__ pa, double* __restrict__ pb) {
-  __shared__ double red[32];
-  BlockSum2<double>(a, b, red);
-  if (threadIdx.x == 0) {
-    pa[blockIdx.x] = a;
-    pb[blockIdx.x] = b;
+  __shared__ double red[2][16];
+  a = WaveSum(a);
+  b = WaveSum(b);
+  if (LaneId() == 0) {
+    red[0][WaveInBlock()] = a;
+    red[1][WaveInBlock()] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const int nw = (int(blockDim.x) + kWave - 1) / kWave;
+    double t = 0.0;
+    for (int i = 0; i < nw; ++i) t += red[threadIdx.x][i];
+    (threadIdx.x == 0 ? pa : pb)[blockIdx.x] = t;
   }
 }
 
@@ -921,8 +939,7 @@ __global__ __launch_bounds__(kBlk) void PcgInit(int n, Dot m, const double* __re
       rz += br * zr;
     }
   }
-  StorePartial(rr, part + kPA * gridDim.x);
-  StorePartial(rz, part + kPB * gridDim.x);
+  StorePartial2(rr, rz, part + kPA * gridDim.x, part + kPB * gridDim.x);
 }
 
 // The refresh of backward_error.pyx:116-122, enqueued after every batch of
