@@ -256,3 +256,30 @@ def test_diagonal_pass_several_columns_per_thread(gpu_lib, oracle_lib, method, m
     assert it == o["iterations"] == 4
     assert_same_result(g, o, hist_rtol=1e-10, m_rtol=1e-12)
     assert rel_fro(g["M"], res["csr"][1]["M"]) <= 1e-13
+
+
+def test_diagonal_mr_stops_at_tolerance(gpu_lib, oracle_lib):
+    """MR's step in the pass prologue (minimal_residual.cpp:58-65): tolerance
+    between the residual norms of iterations k - 1 and k stops the build at
+    iteration k, as the oracle does; a given non-diagonal M0 on pattern(A)."""
+    A = problems.diffusion3d(14, seed=3)
+    Ad = sparse.csc_matrix(A)
+    M0 = Ad.copy()
+    M0.data = np.random.default_rng(5).uniform(-0.01, 0.01, M0.nnz)
+    M0 = sparse.csc_matrix(M0 + sparse.diags(1.0 / Ad.diagonal()))
+    M0.sort_indices()
+    h = oracle_lib.run("mr", A, M0, None, 10, 0.0, 1.0, False, 1, fixed_pattern=True)["history"]["residual_norm"]
+    k = next(i for i in range(2, len(h)) if h[i - 1] > h[i] * 1.001)
+    tol = float(np.sqrt(h[k - 1] * h[k]))
+    print(f"case: dia mr tolerance stop at {k}", flush=True)
+    with env_set("SPAI_FIXED_LAYOUT", "auto"):
+        s = gpu_lib.Session("mr", A, M0=M0, max_iterations=10, tolerance=tol)
+    try:
+        assert s.info()["fused"] == 2
+        it = s.build()
+        g = s.result()
+    finally:
+        s.close()
+    o = oracle_lib.run("mr", A, M0, None, 10, tol, 1.0, False, 1, fixed_pattern=True)
+    assert it == k and o["iterations"] == k and g["converged"] and o["converged"]
+    assert_same_result(g, o, hist_rtol=1e-10, m_rtol=1e-12)
