@@ -517,6 +517,9 @@ def main():
     # lomr_pass_bytes with s_v = 4
     shape = fp32["launch_shape"]
     pass_bytes = lomr_pass_bytes(n, nnz, shape, 4, "lomr")
+    # the same iteration in the general fused pass's algorithmic bytes (R
+    # stored on S2, CSC patterns), for comparison with that pass
+    fused_equiv_bytes = fused_pass_bytes(n, nnz, shape["nnz_s"], shape["nnz_s2"], 4, "lomr")
     pass_kernel = lomr_pass_kernel(shape, "lomr", "float")
     us = fp32["lomr_pass_us"]
     achieved = pass_bytes / (us * 1e-6) / 1e9
@@ -602,6 +605,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "frac_of_achievable": round(achieved / achievable, 4),
                      "algorithmic_bytes_per_launch": pass_bytes, "avg_launch_us": round(us, 3),
+                     "fused_pass_equivalent_GBs": round(fused_equiv_bytes / (us * 1e-6) / 1e9, 1),
                      "launches_timed": fp32["lomr_passes_timed"]},
         "roofline_pcg_spmv": {"kernel": spmv_kernel + ", standalone replay", "bound": "hbm",
                               "achieved": round(spmv_bytes / (spmv_us * 1e-6) / 1e9, 1), "peak": HBM_PEAK_GBS,
